@@ -1,0 +1,226 @@
+"""ctypes binding of the FP64 CPU oracle (oracle/_build/libsaras_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, where it is the checker.  The product package
+(saras-cc_amd/) never imports it.
+
+Parity status of each oracle function is pinned by tests/test_oracle_*.py:
+  or_vc_depth, or_level_metrics, or_field_metrics  -> closed forms, tables, FD
+  or_mg_* (3-D and 2-D V-cycle)                     -> dense brute force, closed
+                                                       forms, the reference's
+                                                       own r^2/6 mgTest solution
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "libsaras_oracle.so")
+
+GRID_UNIFORM, GRID_TANH_SINGLE, GRID_TANH_DOUBLE = 0, 1, 2
+BC_NEUMANN, BC_DIRICHLET, BC_DIRICHLET_TEST = 0, 1, 2
+
+
+class GridDesc(C.Structure):
+    _fields_ = [("dim", C.c_int), ("n", C.c_int * 3), ("len", C.c_double * 3),
+                ("periodic", C.c_int * 3), ("gtype", C.c_int * 3),
+                ("beta", C.c_double * 3), ("pad", C.c_int)]
+
+
+class MgParams(C.Structure):
+    _fields_ = [("vc_count", C.c_int), ("vc_tol", C.c_double), ("solve_flag", C.c_int),
+                ("mg_tol", C.c_double), ("sor", C.c_double), ("pre_smooth", C.c_int),
+                ("post_smooth", C.c_int), ("res_type", C.c_int), ("bc", C.c_int),
+                ("no_early_exit", C.c_int)]
+
+
+class MgStats(C.Structure):
+    _fields_ = [("cycles", C.c_int), ("residual", C.c_double),
+                ("coarse_iters_max", C.c_int), ("coarse_capped", C.c_int)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle library missing: run `make -C oracle` first")
+        L = C.CDLL(LIB_PATH)
+        dp = C.POINTER(C.c_double)
+        L.or_vc_depth.argtypes = [C.POINTER(GridDesc)]
+        L.or_vc_depth.restype = C.c_int
+        L.or_level_metrics.argtypes = [C.POINTER(GridDesc), C.c_int, C.c_int, dp, dp, dp, dp, dp]
+        L.or_field_metrics.argtypes = [C.POINTER(GridDesc), C.c_int, dp, dp, dp, dp]
+        L.or_mg_create.argtypes = [C.POINTER(GridDesc), C.POINTER(MgParams)]
+        L.or_mg_create.restype = C.c_void_p
+        L.or_mg_destroy.argtypes = [C.c_void_p]
+        L.or_mg_levels.argtypes = [C.c_void_p]
+        L.or_mg_max_count.argtypes = [C.c_void_p]
+        L.or_mg_level_shape.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int)]
+        L.or_mg_solve.argtypes = [C.c_void_p, dp, dp, C.c_int, dp, C.POINTER(MgStats)]
+        L.or_mg_set_level.argtypes = [C.c_void_p, C.c_int, dp, dp]
+        L.or_mg_get_level.argtypes = [C.c_void_p, C.c_int, dp, dp]
+        L.or_mg_smooth.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
+        L.or_mg_coarse_solve.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.or_mg_residual_restrict.argtypes = [C.c_void_p, C.c_int]
+        L.or_mg_prolong_correct.argtypes = [C.c_void_p, C.c_int]
+        L.or_mg_error.argtypes = [C.c_void_p, C.c_int]
+        L.or_mg_error.restype = C.c_double
+        L.or_mg_impose_bc.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.or_set_threads.argtypes = [C.c_int]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def grid_desc(n, dim=3, length=(1.0, 1.0, 1.0), periodic=(0, 0, 0), gtype=(0, 0, 0),
+              beta=(1.0, 1.0, 1.0), pad=1):
+    g = GridDesc()
+    g.dim = dim
+    n = tuple(n) if hasattr(n, "__len__") else (n, n, n)
+    for a in range(3):
+        g.n[a] = n[a]
+        g.len[a] = length[a]
+        g.periodic[a] = int(periodic[a])
+        g.gtype[a] = int(gtype[a])
+        g.beta[a] = beta[a]
+    g.pad = pad
+    return g
+
+
+def mg_params(vc_count=20, vc_tol=1e-6, solve_flag=1, mg_tol=1e-2, sor=1.0, pre=3, post=6,
+              res_type=0, bc=BC_NEUMANN, no_early_exit=0):
+    p = MgParams()
+    p.vc_count, p.vc_tol, p.solve_flag, p.mg_tol = vc_count, vc_tol, int(solve_flag), mg_tol
+    p.sor, p.pre_smooth, p.post_smooth, p.res_type = sor, pre, post, res_type
+    p.bc, p.no_early_exit = bc, int(no_early_exit)
+    return p
+
+
+def vc_depth(g):
+    return lib().or_vc_depth(C.byref(g))
+
+
+def level_metrics(g, level, axis):
+    """Return dict of arrays over [-1, n_l] for one axis."""
+    n0 = 1 if (g.dim == 2 and axis == 1) else g.n[axis]
+    nl = n0 if (g.dim == 2 and axis == 1) else n0 >> level
+    out = {k: np.zeros(nl + 2) for k in ("x", "xi", "xi_x", "xixx", "xix2")}
+    r = lib().or_level_metrics(C.byref(g), level, axis, _dp(out["x"]), _dp(out["xi"]),
+                               _dp(out["xi_x"]), _dp(out["xixx"]), _dp(out["xix2"]))
+    if r < 0:
+        raise ValueError("bad level")
+    return out
+
+
+def field_metrics(g, axis):
+    p = 1 if (g.dim == 2 and axis == 1) else g.pad
+    n = 1 if (g.dim == 2 and axis == 1) else g.n[axis]
+    out = {k: np.zeros(n + 2 * p) for k in ("x", "xi_x", "xixx", "xix2")}
+    lib().or_field_metrics(C.byref(g), axis, _dp(out["x"]), _dp(out["xi_x"]),
+                           _dp(out["xixx"]), _dp(out["xix2"]))
+    return out
+
+
+def field_shape(g, pad):
+    if g.dim == 3:
+        return (g.n[0] + 2 * pad, g.n[1] + 2 * pad, g.n[2] + 2 * pad)
+    return (g.n[0] + 2 * pad, g.n[2] + 2 * pad)
+
+
+def core(a, pad):
+    if a.ndim == 3:
+        return a[pad:-pad, pad:-pad, pad:-pad]
+    return a[pad:-pad, pad:-pad]
+
+
+class Multigrid:
+    """Oracle multigrid_d3 / multigrid_d2 (poisson.h:125-205)."""
+
+    def __init__(self, g, p):
+        self.g, self.p = g, p
+        self.h = lib().or_mg_create(C.byref(g), C.byref(p))
+        if not self.h:
+            raise ValueError("oracle: bad grid")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_mg_destroy(self.h)
+            self.h = None
+
+    @property
+    def levels(self):
+        return lib().or_mg_levels(self.h)
+
+    @property
+    def max_count(self):
+        return lib().or_mg_max_count(self.h)
+
+    def level_shape(self, level):
+        n = (C.c_int * 3)()
+        lib().or_mg_level_shape(self.h, level, n)
+        return tuple(n)
+
+    def padded_shape(self, level):
+        n = self.level_shape(level)
+        if self.g.dim == 3:
+            return (n[0] + 2, n[1] + 2, n[2] + 2)
+        return (n[0] + 2, n[2] + 2)
+
+    def solve(self, x_full, b_full, pad, history=False):
+        x_full = np.ascontiguousarray(x_full, dtype=np.float64)
+        b_full = np.ascontiguousarray(b_full, dtype=np.float64)
+        st = MgStats()
+        hist = np.zeros(max(self.p.vc_count, 1))
+        lib().or_mg_solve(self.h, _dp(x_full), _dp(b_full), pad, _dp(hist), C.byref(st))
+        out = dict(cycles=st.cycles, residual=st.residual, coarse_iters_max=st.coarse_iters_max,
+                   coarse_capped=st.coarse_capped)
+        if history:
+            out["history"] = hist[: st.cycles].copy()
+        return x_full, out
+
+    # single operations on padded level arrays
+    def set_level(self, level, lhs=None, rhs=None):
+        la = np.ascontiguousarray(lhs, dtype=np.float64) if lhs is not None else None
+        ra = np.ascontiguousarray(rhs, dtype=np.float64) if rhs is not None else None
+        if la is not None:
+            assert la.shape == self.padded_shape(level)
+        if ra is not None:
+            assert ra.shape == self.padded_shape(level)
+        lib().or_mg_set_level(self.h, level, _dp(la) if la is not None else None,
+                              _dp(ra) if ra is not None else None)
+
+    def get_level(self, level):
+        shp = self.padded_shape(level)
+        lhs, rhs = np.zeros(shp), np.zeros(shp)
+        lib().or_mg_get_level(self.h, level, _dp(lhs), _dp(rhs))
+        return lhs, rhs
+
+    def smooth(self, level, count, zero_bc):
+        lib().or_mg_smooth(self.h, level, count, int(zero_bc))
+
+    def coarse_solve(self, level, zero_bc=1):
+        return lib().or_mg_coarse_solve(self.h, level, int(zero_bc))
+
+    def residual_restrict(self, level):
+        lib().or_mg_residual_restrict(self.h, level)
+
+    def prolong_correct(self, level):
+        lib().or_mg_prolong_correct(self.h, level)
+
+    def error(self, res_type):
+        return lib().or_mg_error(self.h, res_type)
+
+    def impose_bc(self, level, zero_bc):
+        lib().or_mg_impose_bc(self.h, level, int(zero_bc))
+
+
+def set_threads(n):
+    lib().or_set_threads(int(n))

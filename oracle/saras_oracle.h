@@ -1,0 +1,121 @@
+/*
+ * saras_oracle.h -- FP64 CPU restatement of the SARAS-CC hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP
+ * product path is compared against.  Only tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg may load it; nothing in saras-cc_amd/
+ * links, includes or calls it.
+ *
+ * Every function restates the single-domain (one MPI rank) branch of the
+ * reference algorithm, following the loop order and floating-point
+ * expression order of the cited reference lines, so that the GPU FP64 path
+ * can be compared bit-for-bit where the reference itself is deterministic.
+ *
+ * Layout: a level array is (nx+2) x (ny+2) x (nz+2) doubles, row-major with
+ * z fastest, lower bound -1 on every axis (the reference's Blitz arrays with
+ * pad 1, lib/poisson/poisson.cc:386-391).  Planar (2-D) problems use ny = 1
+ * and no y ghost layer: (nx+2) x 1 x (nz+2).
+ */
+#ifndef SARAS_ORACLE_H
+#define SARAS_ORACLE_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Grid types per axis (lib/io/parser.cc:417-448). */
+enum { OR_GRID_UNIFORM = 0, OR_GRID_TANH_SINGLE = 1, OR_GRID_TANH_DOUBLE = 2 };
+
+/* Wall treatment of non-periodic axes in the Poisson solver.
+ *   NEUMANN     x(-1) =  x(0)          poisson_d3.cc:892-895 (allNeumann, production)
+ *   DIRICHLET   x(-1) = -x(0)          poisson_d3.cc:896-900
+ *   DIRICHLET_TEST x(-1) = 2 w - x(0), w = r^2/6 (3-D) or (x^2+z^2)/4 (2-D),
+ *               homogeneous (-x(0)) below level 0 / when zeroBC
+ *               (TEST_POISSON: poisson_d3.cc:881-890, 824-873; poisson_d2.cc:561-637) */
+enum { OR_BC_NEUMANN = 0, OR_BC_DIRICHLET = 1, OR_BC_DIRICHLET_TEST = 2 };
+
+typedef struct {
+    int dim;          /* 2 (planar, x-z) or 3 */
+    int n[3];         /* global cells nx, ny, nz (ny ignored when dim == 2) */
+    double len[3];    /* domain lengths Lx, Ly, Lz */
+    int periodic[3];  /* "Domain Type" P/N per axis */
+    int gtype[3];     /* OR_GRID_* ("Mesh Type" U/S/D) */
+    double beta[3];   /* tanh stretching parameters */
+    int pad;          /* field pad width: 1 (dScheme 1) or 2 (dScheme 2) */
+} or_grid_desc;
+
+typedef struct {
+    int vc_count;     /* "V-Cycle Count" */
+    double vc_tol;    /* "Residual Tolerance" */
+    int solve_flag;   /* "Solve Coarsest" */
+    double mg_tol;    /* "Solve Tolerance" (coarsest RB-GS, absolute) */
+    double sor;       /* "SOR Parameter" */
+    int pre_smooth;   /* "Pre-Smoothing Count" */
+    int post_smooth;  /* "Post-Smoothing Count" */
+    int res_type;     /* "Residual Type": 0 Linf, 1 L1, 2 L2 */
+    int bc;           /* OR_BC_* */
+    int no_early_exit;/* TEST_POISSON: run all vc_count cycles (poisson.cc:182-184) */
+} or_mg_params;
+
+typedef struct {
+    int cycles;            /* V-cycles performed */
+    double residual;       /* last computeError value */
+    int coarse_iters_max;  /* largest coarsest-level iteration count seen */
+    int coarse_capped;     /* coarsest solve hit maxCount at least once */
+} or_mg_stats;
+
+/* ---- grid (lib/grid/grid.cc) ---- */
+
+/* V-cycle depth vcdGlo (grid.cc:678-742, single domain). */
+int or_vc_depth(const or_grid_desc *g);
+
+/* Level-l metric arrays for one axis over indices [-1, n_l]: coordinate,
+ * xi, xi_x, xi_xx, xi_x^2 (globalMetrics(15 l + 5 axis + {1,0,2,3,4}),
+ * grid.cc:521-666, 411-508).  Each output holds n_l + 2 doubles. */
+int or_level_metrics(const or_grid_desc *g, int level, int axis,
+                     double *coord, double *xi, double *xi_x, double *xixx, double *xix2);
+
+/* Field-grid coordinate and metrics of one axis over [-pad, n+pad-1]
+ * (grid.cc:318-508, the arrays x, xi_x, xixx, xix2 the field stencils use). */
+int or_field_metrics(const or_grid_desc *g, int axis,
+                     double *coord, double *xi_x, double *xixx, double *xix2);
+
+/* ---- multigrid (lib/poisson) ---- */
+
+typedef struct or_mg or_mg;
+
+or_mg *or_mg_create(const or_grid_desc *g, const or_mg_params *p);
+void or_mg_destroy(or_mg *m);
+int or_mg_levels(const or_mg *m);                    /* vcdGlo + 1 */
+void or_mg_level_shape(const or_mg *m, int level, int n[3]);
+int or_mg_max_count(const or_mg *m);                 /* coarse iteration cap */
+
+/* poisson::mgSolve (poisson.cc:109-189).  x_full / b_full are caller
+ * field arrays with pad width `pad`: (nx+2p)(ny+2p)(nz+2p) (2-D: ny term = 1).
+ * The core of x_full is the initial guess; on return the core and the first
+ * ghost layer of x_full hold the solution, exactly as outLHS.F(stagFull(0)).
+ * history (optional, >= vc_count entries) receives the residual after each
+ * V-cycle. */
+int or_mg_solve(or_mg *m, double *x_full, const double *b_full, int pad,
+                double *history, or_mg_stats *st);
+
+/* Single-operation entry points on the solver's level arrays, for
+ * kernel-level parity tests.  Arrays are padded level arrays as above. */
+void or_mg_set_level(or_mg *m, int level, const double *lhs_pad, const double *rhs_pad);
+void or_mg_get_level(const or_mg *m, int level, double *lhs_pad, double *rhs_pad);
+void or_mg_smooth(or_mg *m, int level, int count, int zero_bc);    /* poisson_d3.cc:96-246 */
+int  or_mg_coarse_solve(or_mg *m, int level, int zero_bc);         /* poisson_d3.cc:249-432; returns iterations */
+void or_mg_residual_restrict(or_mg *m, int level);                 /* :71-93 then :435-513 into rhs(level+1), lhs(level+1)=0 */
+void or_mg_prolong_correct(or_mg *m, int level);                   /* :516-560 + poisson.cc:266, level+1 -> level */
+double or_mg_error(or_mg *m, int res_type);                        /* :563-663 at level 0 */
+void or_mg_impose_bc(or_mg *m, int level, int zero_bc);            /* :876-952 */
+
+/* Threads the oracle's OpenMP loops use (0 = runtime default). */
+void or_set_threads(int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

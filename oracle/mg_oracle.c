@@ -647,3 +647,8 @@ void or_mg_impose_bc(or_mg *m, int level, int zero_bc)
     m->vLevel = level; m->zeroBC = zero_bc;
     impose_bc(m, &m->lv[level]);
 }
+
+void or_mg_vcycles(or_mg *m, int count)
+{
+    for (int c = 0; c < count; c++) vcycle(m);
+}

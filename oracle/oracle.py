@@ -71,6 +71,7 @@ def lib():
         L.or_mg_error.restype = C.c_double
         L.or_mg_impose_bc.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.or_set_threads.argtypes = [C.c_int]
+        L.or_mg_vcycles.argtypes = [C.c_void_p, C.c_int]
         _lib = L
     return _lib
 
@@ -217,6 +218,9 @@ class Multigrid:
 
     def error(self, res_type):
         return lib().or_mg_error(self.h, res_type)
+
+    def vcycles(self, count):
+        lib().or_mg_vcycles(self.h, count)
 
     def impose_bc(self, level, zero_bc):
         lib().or_mg_impose_bc(self.h, level, int(zero_bc))

@@ -111,6 +111,7 @@ void or_mg_residual_restrict(or_mg *m, int level);                 /* :71-93 the
 void or_mg_prolong_correct(or_mg *m, int level);                   /* :516-560 + poisson.cc:266, level+1 -> level */
 double or_mg_error(or_mg *m, int res_type);                        /* :563-663 at level 0 */
 void or_mg_impose_bc(or_mg *m, int level, int zero_bc);            /* :876-952 */
+void or_mg_vcycles(or_mg *m, int count);                           /* poisson.cc:203-279, repeated */
 
 /* Threads the oracle's OpenMP loops use (0 = runtime default). */
 void or_set_threads(int n);

@@ -1,0 +1,832 @@
+// mg_solver.hip -- device-resident multigrid hierarchy and the C-ABI of
+// include/saras_hip.h.
+//
+// The host drives the reference's V-cycle (lib/poisson/poisson.cc:203-279)
+// level by level on one HIP stream; all level arrays stay in HBM between
+// calls, as the reference's poisson object keeps its hierarchy between time
+// steps (timestep.h:142).  Two arithmetic modes:
+//   SARAS_PREC_FP64  - the reference algorithm in FP64, bit-exact with the
+//                      FP64 CPU build for the L-inf residual type;
+//   SARAS_PREC_MIXED - FP64 residual and iterative refinement around FP32
+//                      V-cycles on the correction equation A e = r.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "grid_host.hpp"
+#include "mg_kernels.hpp"
+#include "saras_hip.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+void set_error(const std::string &s) { g_last_error = s; }
+
+#define HIP_TRY(expr)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            set_error(std::string(#expr) + ": " + hipGetErrorString(e_));               \
+            return e_ == hipErrorOutOfMemory ? SARAS_OOM : SARAS_HIP_ERROR;             \
+        }                                                                               \
+    } while (0)
+
+constexpr int kThreads = 256;
+
+inline unsigned blocks_for(long n) { return (unsigned)std::max<long>(1, (n + kThreads - 1) / kThreads); }
+
+template <typename T>
+long row_pitch(int nz) {
+    const long a = 128 / (long)sizeof(T);
+    return (nz + a - 1) / a * a;
+}
+
+template <typename T>
+struct DevLevel {
+    int n[3] = {0, 0, 0};
+    long sx = 0, sy = 0;
+    size_t elems = 0;  // allocation size in elements
+    T *x = nullptr, *b = nullptr;
+    T *c2[3] = {nullptr, nullptr, nullptr}, *c1[3] = {nullptr, nullptr, nullptr};
+    T *rlo[3] = {nullptr, nullptr, nullptr}, *rhi[3] = {nullptr, nullptr, nullptr}, *rwd[3] = {nullptr, nullptr, nullptr};
+};
+
+template <typename T>
+saras_status upload(T **dst, const std::vector<double> &src) {
+    std::vector<T> tmp(src.begin(), src.end());
+    HIP_TRY(hipMalloc((void **)dst, std::max<size_t>(1, tmp.size()) * sizeof(T)));
+    HIP_TRY(hipMemcpy(*dst, tmp.data(), tmp.size() * sizeof(T), hipMemcpyHostToDevice));
+    return SARAS_OK;
+}
+
+template <typename T>
+struct Hierarchy {
+    int dim = 3;
+    std::vector<DevLevel<T>> lv;
+    T *wall[3] = {nullptr, nullptr, nullptr};   // DIRICHLET_TEST level-0 wall values
+    T *coarse_scratch = nullptr;
+    bool coarse_lds = true;
+    size_t coarse_lds_bytes = 0;
+
+    ~Hierarchy() {
+        for (auto &L : lv) {
+            (void)hipFree(L.x); (void)hipFree(L.b);
+            for (int a = 0; a < 3; ++a) {
+                (void)hipFree(L.c2[a]); (void)hipFree(L.c1[a]);
+                (void)hipFree(L.rlo[a]); (void)hipFree(L.rhi[a]); (void)hipFree(L.rwd[a]);
+            }
+        }
+        for (int a = 0; a < 3; ++a) (void)hipFree(wall[a]);
+        (void)hipFree(coarse_scratch);
+    }
+
+    saras_status build(const saras::GridLevels &G, const std::vector<std::vector<double>> &walls) {
+        dim = G.dim;
+        const int D = G.depth;
+        lv.resize(D + 1);
+        for (int l = 0; l <= D; ++l) {
+            DevLevel<T> &L = lv[l];
+            for (int a = 0; a < 3; ++a) L.n[a] = G.axes[l][a].n;
+            if (dim == 2) L.n[1] = 1;
+            L.sy = row_pitch<T>(L.n[2]);
+            L.sx = L.sy * (long)L.n[1];
+            L.elems = (size_t)L.sx * L.n[0];
+            HIP_TRY(hipMalloc((void **)&L.x, L.elems * sizeof(T)));
+            HIP_TRY(hipMalloc((void **)&L.b, L.elems * sizeof(T)));
+            HIP_TRY(hipMemset(L.x, 0, L.elems * sizeof(T)));
+            HIP_TRY(hipMemset(L.b, 0, L.elems * sizeof(T)));
+            for (int a = 0; a < 3; ++a) {
+                if (dim == 2 && a == 1) {
+                    std::vector<double> z(1, 0.0);
+                    if (upload(&L.c2[a], z) || upload(&L.c1[a], z)) return SARAS_HIP_ERROR;
+                    continue;
+                }
+                const saras::AxisLevel &A = G.axes[l][a];
+                // poisson::setCoefficients (poisson.cc:463-482)
+                const int hInc = 1 << l;
+                const double dxi = 1.0 / double(G.axes[0][a].n);
+                const double h = hInc * dxi;
+                const double h2 = std::pow(h, 2);
+                const double i2h = 0.5 / h, ih2 = 1.0 / h2;
+                std::vector<double> c2(A.n), c1(A.n);
+                for (int i = 0; i < A.n; ++i) {
+                    c2[i] = A.xix2[i + 1] * ih2;
+                    c1[i] = A.xixx[i + 1] * i2h;
+                }
+                if (upload(&L.c2[a], c2) || upload(&L.c1[a], c1)) return SARAS_HIP_ERROR;
+                if (l > 0) {
+                    // restriction factors of coarse index I (poisson_d3.cc:494-502)
+                    const saras::AxisLevel &Fa = G.axes[l - 1][a];
+                    std::vector<double> lo(A.n), hi(A.n), wd(A.n);
+                    for (int I = 0; I < A.n; ++I) {
+                        const double X = A.coord[I + 1];
+                        const double f0 = Fa.coord[2 * I + 1], f1 = Fa.coord[2 * I + 2];
+                        lo[I] = X - f0;
+                        hi[I] = f1 - X;
+                        wd[I] = f1 - f0;
+                    }
+                    if (upload(&L.rlo[a], lo) || upload(&L.rhi[a], hi) || upload(&L.rwd[a], wd))
+                        return SARAS_HIP_ERROR;
+                }
+            }
+        }
+        for (int a = 0; a < 3; ++a)
+            if (!walls[a].empty() && upload(&wall[a], walls[a])) return SARAS_HIP_ERROR;
+        // coarsest-level workspace
+        const DevLevel<T> &C = lv[D];
+        const size_t padded = dim == 3 ? (size_t)(C.n[0] + 2) * (C.n[1] + 2) * (C.n[2] + 2)
+                                       : (size_t)(C.n[0] + 2) * (C.n[2] + 2);
+        const size_t need = (padded + (size_t)C.n[0] * C.n[1] * C.n[2]) * sizeof(T);
+        coarse_lds = need <= 48 * 1024;
+        coarse_lds_bytes = coarse_lds ? need : 0;
+        if (!coarse_lds) HIP_TRY(hipMalloc((void **)&coarse_scratch, need));
+        return SARAS_OK;
+    }
+};
+
+int bc_code(const saras_mg_params &p, const saras_grid_desc &g, int axis, int level, bool inhom_allowed) {
+    if (g.periodic[axis]) return saras::BC_PER;
+    switch (p.bc) {
+        case SARAS_BC_NEUMANN: return saras::BC_NEU;
+        case SARAS_BC_DIRICHLET: return saras::BC_DIR;
+        default: return (level == 0 && inhom_allowed) ? saras::BC_DIRV : saras::BC_DIR;
+    }
+}
+
+}  // namespace
+
+struct saras_mg {
+    saras_grid_desc g{};
+    saras_mg_params p{};
+    saras::GridLevels G;
+    int depth = 0;
+    int max_count = 0;
+    Hierarchy<double> h64;
+    std::unique_ptr<Hierarchy<float>> h32;
+    double *x64b = nullptr;            // second FP64 iterate buffer (MIXED)
+    double *partial = nullptr;         // norm partials
+    double *norm_out = nullptr;        // device [2]
+    double *norm_host = nullptr;       // pinned [2]
+    int *iters_dev = nullptr;          // coarse iteration count
+    int *iters_host = nullptr;         // pinned
+    double *stage_x = nullptr, *stage_b = nullptr;  // device staging for host pointers
+    size_t stage_elems = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int coarse_iters_max = 0;
+    int coarse_capped = 0;
+
+    ~saras_mg() {
+        (void)hipFree(x64b); (void)hipFree(partial); (void)hipFree(norm_out); (void)hipFree(iters_dev);
+        (void)hipHostFree(norm_host); (void)hipHostFree(iters_host);
+        (void)hipFree(stage_x); (void)hipFree(stage_b);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+template <typename T>
+saras::LevelArgs<T> level_args(const saras_mg *m, const Hierarchy<T> &H, int l, bool inhom) {
+    const DevLevel<T> &L = H.lv[l];
+    saras::LevelArgs<T> a{};
+    for (int q = 0; q < 3; ++q) {
+        a.n[q] = L.n[q];
+        a.c2[q] = L.c2[q];
+        a.c1[q] = L.c1[q];
+        a.bc_lo[q] = a.bc_hi[q] = bc_code(m->p, m->g, q, l, inhom);
+        a.wall[q] = H.wall[q];
+    }
+    a.sx = L.sx;
+    a.sy = L.sy;
+    return a;
+}
+
+// Level operations for one precision.  `inhom0` selects the inhomogeneous
+// DIRICHLET_TEST wall values on level 0 (false for the correction equation
+// of the mixed-precision refinement, whose walls are homogeneous).
+template <typename T>
+struct Ops {
+    saras_mg *m;
+    Hierarchy<T> &H;
+    hipStream_t s;
+    bool inhom0;
+    T w;
+
+    bool inhom(int l, bool zero_bc) const { return inhom0 && l == 0 && !zero_bc; }
+
+    void rb_sweep(int l, bool zero_bc) {
+        DevLevel<T> &L = H.lv[l];
+        auto a = level_args(m, H, l, inhom(l, zero_bc));
+        const long n = (long)L.n[0] * L.n[1] * ((L.n[2] + 1) / 2);
+        for (int colour = 0; colour < 2; ++colour) {
+            if (H.dim == 3) {
+                if (l == 0)
+                    hipLaunchKernelGGL((saras::k_rb_pass<T, 3, true>), dim3(blocks_for(n)), dim3(kThreads), 0, s, a, L.x, L.b, colour, w);
+                else
+                    hipLaunchKernelGGL((saras::k_rb_pass<T, 3, false>), dim3(blocks_for(n)), dim3(kThreads), 0, s, a, L.x, L.b, colour, w);
+            } else {
+                if (l == 0)
+                    hipLaunchKernelGGL((saras::k_rb_pass<T, 2, true>), dim3(blocks_for(n)), dim3(kThreads), 0, s, a, L.x, L.b, colour, w);
+                else
+                    hipLaunchKernelGGL((saras::k_rb_pass<T, 2, false>), dim3(blocks_for(n)), dim3(kThreads), 0, s, a, L.x, L.b, colour, w);
+            }
+        }
+    }
+
+    void coarse(int l, bool zero_bc, int mode, int sweeps) {
+        DevLevel<T> &L = H.lv[l];
+        auto a = level_args(m, H, l, inhom(l, zero_bc));
+        saras::CoarseCtl ctl{mode, sweeps, m->max_count, m->p.mg_tol, m->iters_dev};
+        const long cells = (long)L.n[0] * L.n[1] * L.n[2];
+        int threads = 64;  // power of two: the kernel's tree reduction relies on it
+        while (threads < cells && threads < 1024) threads <<= 1;
+        if (H.dim == 3)
+            hipLaunchKernelGGL((saras::k_coarse<T, 3>), dim3(1), dim3(threads), H.coarse_lds_bytes, s, a, L.x, L.b,
+                               H.coarse_scratch, H.coarse_lds ? 1 : 0, w, ctl);
+        else
+            hipLaunchKernelGGL((saras::k_coarse<T, 2>), dim3(1), dim3(threads), H.coarse_lds_bytes, s, a, L.x, L.b,
+                               H.coarse_scratch, H.coarse_lds ? 1 : 0, w, ctl);
+    }
+
+    // poisson smooth(count) at level l (coarsest level: literal kernel)
+    void smooth(int l, int count, bool zero_bc) {
+        if (count <= 0) return;
+        if (l == (int)H.lv.size() - 1) {
+            coarse(l, zero_bc, 1, count);
+            return;
+        }
+        for (int c = 0; c < count; ++c) rb_sweep(l, zero_bc);
+    }
+
+    void resid_restrict(int l) {
+        DevLevel<T> &F = H.lv[l], &C = H.lv[l + 1];
+        auto fa = level_args(m, H, l, inhom(l, false));
+        auto ca = level_args(m, H, l + 1, false);
+        saras::RestrictArgs<T> r{};
+        for (int q = 0; q < 3; ++q) {
+            r.lo[q] = C.rlo[q];
+            r.hi[q] = C.rhi[q];
+            r.wd[q] = C.rwd[q];
+        }
+        const long n = (long)C.n[0] * C.n[1] * C.n[2];
+        if (H.dim == 3)
+            hipLaunchKernelGGL((saras::k_resid_restrict<T, 3>), dim3(blocks_for(n)), dim3(kThreads), 0, s, fa, F.x, F.b,
+                               ca, C.b, C.x, r);
+        else
+            hipLaunchKernelGGL((saras::k_resid_restrict<T, 2>), dim3(blocks_for(n)), dim3(kThreads), 0, s, fa, F.x, F.b,
+                               ca, C.b, C.x, r);
+    }
+
+    void prolong_correct(int l) {
+        DevLevel<T> &F = H.lv[l], &C = H.lv[l + 1];
+        const long n = (long)F.n[0] * F.n[1] * F.n[2];
+        hipLaunchKernelGGL((saras::k_prolong_correct<T>), dim3(blocks_for(n)), dim3(kThreads), 0, s, F.n[0], F.n[1],
+                           F.n[2], F.sx, F.sy, F.x, C.x, C.sx, C.sy);
+    }
+
+    // poisson::vCycle (poisson.cc:203-279)
+    void vcycle() {
+        const int D = (int)H.lv.size() - 1;
+        smooth(0, m->p.pre_smooth, false);
+        for (int l = 0; l < D; ++l) {
+            resid_restrict(l);
+            if (l + 1 == D) {
+                if (m->p.solve_flag) coarse(D, true, 0, 0);
+                else smooth(D, m->p.pre_smooth, true);
+            } else {
+                smooth(l + 1, m->p.pre_smooth, true);
+            }
+        }
+        for (int l = D - 1; l >= 0; --l) {
+            prolong_correct(l);
+            smooth(l, m->p.post_smooth, l != 0);
+        }
+    }
+};
+
+template <int DIM>
+void launch_norm(int type, saras_mg *m, const saras::LevelArgs<double> &a, const double *x, const double *b,
+                 hipStream_t s) {
+    switch (type) {
+        case 0:
+            hipLaunchKernelGGL((saras::k_norm_partial<0, DIM>), dim3(saras::kNormBlocks), dim3(saras::kNormThreads), 0,
+                               s, a, x, b, m->partial);
+            hipLaunchKernelGGL((saras::k_norm_final<0>), dim3(1), dim3(saras::kNormThreads), 0, s, m->partial,
+                               saras::kNormBlocks, m->norm_out);
+            break;
+        case 1:
+            hipLaunchKernelGGL((saras::k_norm_partial<1, DIM>), dim3(saras::kNormBlocks), dim3(saras::kNormThreads), 0,
+                               s, a, x, b, m->partial);
+            hipLaunchKernelGGL((saras::k_norm_final<1>), dim3(1), dim3(saras::kNormThreads), 0, s, m->partial,
+                               saras::kNormBlocks, m->norm_out);
+            break;
+        default:
+            hipLaunchKernelGGL((saras::k_norm_partial<2, DIM>), dim3(saras::kNormBlocks), dim3(saras::kNormThreads), 0,
+                               s, a, x, b, m->partial);
+            hipLaunchKernelGGL((saras::k_norm_final<2>), dim3(1), dim3(saras::kNormThreads), 0, s, m->partial,
+                               saras::kNormBlocks, m->norm_out);
+            break;
+    }
+}
+
+// computeError (poisson_d3.cc:563-663): ratio formed on the host exactly as
+// the reference does.
+saras_status compute_error(saras_mg *m, int type, const double *x, hipStream_t s, double *out) {
+    auto a = level_args(m, m->h64, 0, true);
+    if (m->G.dim == 3) launch_norm<3>(type, m, a, x, m->h64.lv[0].b, s);
+    else launch_norm<2>(type, m, a, x, m->h64.lv[0].b, s);
+    HIP_TRY(hipMemcpyAsync(m->norm_host, m->norm_out, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    const double num = m->norm_host[0], den = m->norm_host[1];
+    const auto &L = m->h64.lv[0];
+    const int pointCount = L.n[0] * L.n[1] * L.n[2];
+    double r;
+    if (type == 0) r = den != 0.0 ? num / den : num;
+    else if (type == 1) r = den != 0.0 ? num / den : num / pointCount;
+    else r = den != 0.0 ? std::sqrt(num / pointCount) / std::sqrt(den / pointCount) : std::sqrt(num / pointCount);
+    *out = r;
+    return SARAS_OK;
+}
+
+saras_status fetch_coarse_iters(saras_mg *m, hipStream_t s) {
+    HIP_TRY(hipMemcpyAsync(m->iters_host, m->iters_dev, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    const int it = *m->iters_host;
+    m->coarse_iters_max = std::max(m->coarse_iters_max, it);
+    if (it > m->max_count) m->coarse_capped = 1;
+    return SARAS_OK;
+}
+
+long full_elems(const saras_grid_desc &g) {
+    const long p = g.pad;
+    if (g.dim == 3) return (g.n[0] + 2 * p) * (long)(g.n[1] + 2 * p) * (g.n[2] + 2 * p);
+    return (g.n[0] + 2 * p) * (long)(g.n[2] + 2 * p);
+}
+
+template <typename T>
+void load_core(saras_mg *m, const double *full, const DevLevel<T> &L, T *dst, hipStream_t s) {
+    const long n = (long)L.n[0] * L.n[1] * L.n[2];
+    hipLaunchKernelGGL((saras::k_load_core<T>), dim3(blocks_for(n)), dim3(kThreads), 0, s, full, m->g.pad, m->g.dim,
+                       L.n[0], L.n[1], L.n[2], L.sx, L.sy, dst);
+}
+
+// core + the [-1, n] ghost shell, phased as the reference's final imposeBC
+void store_full(saras_mg *m, double *full, const double *core, hipStream_t s) {
+    const auto &L = m->h64.lv[0];
+    const long n = (long)L.n[0] * L.n[1] * L.n[2];
+    hipLaunchKernelGGL(saras::k_store_core, dim3(blocks_for(n)), dim3(kThreads), 0, s, full, m->g.pad, m->g.dim,
+                       L.n[0], L.n[1], L.n[2], L.sx, L.sy, core);
+    const int ext[3] = {L.n[0] + 2, m->g.dim == 3 ? L.n[1] + 2 : 1, L.n[2] + 2};
+    for (int op = 0; op < 2; ++op)
+        for (int a = 0; a < 3; ++a) {
+            if (m->g.dim == 2 && a == 1) continue;
+            const bool per = m->g.periodic[a] != 0;
+            if ((op == 0) != per) continue;
+            const int o1 = a == 0 ? 1 : 0, o2 = a == 2 ? 1 : 2;
+            const long cnt = (long)ext[o1] * ext[o2];
+            const int bc = bc_code(m->p, m->g, a, 0, true);
+            hipLaunchKernelGGL(saras::k_ghost_phase, dim3(blocks_for(cnt)), dim3(kThreads), 0, s, full, m->g.pad,
+                               m->g.dim, L.n[0], L.n[1], L.n[2], a, op, bc, (const double *)m->h64.wall[a]);
+        }
+}
+
+std::vector<std::vector<double>> make_walls(const saras_grid_desc &g, const saras_mg_params &p) {
+    std::vector<std::vector<double>> w(3);
+    if (p.bc != SARAS_BC_DIRICHLET_TEST) return w;
+    // initDirichlet (poisson_d3.cc:824-873, poisson_d2.cc:514-543): r^2/6 on
+    // the walls from the field-grid cell centres; zero on the ghost shell
+    std::vector<double> fx, fy, fz, t1, t2, t3;
+    saras::field_axis(g, 0, fx, t1, t2, t3);
+    saras::field_axis(g, 2, fz, t1, t2, t3);
+    const int p0 = g.pad;
+    const int nx = g.n[0], nz = g.n[2];
+    if (g.dim == 3) {
+        saras::field_axis(g, 1, fy, t1, t2, t3);
+        const int ny = g.n[1];
+        w[0].assign((size_t)(ny + 2) * (nz + 2), 0.0);
+        w[1].assign((size_t)(nx + 2) * (nz + 2), 0.0);
+        w[2].assign((size_t)(nx + 2) * (ny + 2), 0.0);
+        double xD, yD, zD;
+        xD = g.len[0] / 2.0;
+        for (int j = 0; j < ny; ++j) {
+            yD = fy[j + p0] - 0.5;
+            for (int k = 0; k < nz; ++k) {
+                zD = fz[k + p0] - 0.5;
+                w[0][(size_t)(j + 1) * (nz + 2) + k + 1] = (xD * xD + yD * yD + zD * zD) / 6.0;
+            }
+        }
+        yD = g.len[1] / 2.0;
+        for (int i = 0; i < nx; ++i) {
+            xD = fx[i + p0] - 0.5;
+            for (int k = 0; k < nz; ++k) {
+                zD = fz[k + p0] - 0.5;
+                w[1][(size_t)(i + 1) * (nz + 2) + k + 1] = (xD * xD + yD * yD + zD * zD) / 6.0;
+            }
+        }
+        zD = g.len[2] / 2.0;
+        for (int i = 0; i < nx; ++i) {
+            xD = fx[i + p0] - 0.5;
+            for (int j = 0; j < ny; ++j) {
+                yD = fy[j + p0] - 0.5;
+                w[2][(size_t)(i + 1) * (ny + 2) + j + 1] = (xD * xD + yD * yD + zD * zD) / 6.0;
+            }
+        }
+    } else {
+        w[0].assign((size_t)(nz + 2), 0.0);
+        w[2].assign((size_t)(nx + 2), 0.0);
+        double xD = g.len[0] / 2.0, zD;
+        for (int k = 0; k < nz; ++k) {
+            zD = fz[k + p0] - 0.5;
+            w[0][k + 1] = (xD * xD + zD * zD) / 4.0;
+        }
+        zD = g.len[2] / 2.0;
+        for (int i = 0; i < nx; ++i) {
+            xD = fx[i + p0] - 0.5;
+            w[2][i + 1] = (xD * xD + zD * zD) / 4.0;
+        }
+    }
+    return w;
+}
+
+hipStream_t pick_stream(saras_mg *m, void *stream) { return stream ? (hipStream_t)stream : m->stream; }
+
+}  // namespace
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+extern "C" {
+
+const char *saras_last_error(void) { return g_last_error.c_str(); }
+
+int saras_vc_depth(const saras_grid_desc *grid) { return grid ? saras::vc_depth(*grid) : -1; }
+
+saras_status saras_level_metrics(const saras_grid_desc *grid, int level, int axis, double *coord, double *xi,
+                                 double *xi_x, double *xixx, double *xix2) {
+    if (!grid || axis < 0 || axis > 2) return SARAS_BAD_ARG;
+    saras::GridLevels G;
+    if (!saras::build_levels(*grid, G)) {
+        set_error("grid cannot be coarsened");
+        return SARAS_BAD_SHAPE;
+    }
+    if (level < 0 || level > G.depth) return SARAS_BAD_ARG;
+    const saras::AxisLevel &A = G.axes[level][axis];
+    const size_t m = A.coord.size();
+    if (coord) std::memcpy(coord, A.coord.data(), m * sizeof(double));
+    if (xi) std::memcpy(xi, A.xi.data(), m * sizeof(double));
+    if (xi_x) std::memcpy(xi_x, A.xi_x.data(), m * sizeof(double));
+    if (xixx) std::memcpy(xixx, A.xixx.data(), m * sizeof(double));
+    if (xix2) std::memcpy(xix2, A.xix2.data(), m * sizeof(double));
+    return SARAS_OK;
+}
+
+saras_status saras_mg_create(const saras_grid_desc *grid, const saras_mg_params *params, saras_mg **out) {
+    if (!grid || !params || !out) {
+        set_error("null argument");
+        return SARAS_BAD_ARG;
+    }
+    *out = nullptr;
+    const saras_grid_desc &g = *grid;
+    const saras_mg_params &p = *params;
+    if (g.dim != 2 && g.dim != 3) { set_error("dim must be 2 or 3"); return SARAS_BAD_ARG; }
+    for (int a = 0; a < 3; ++a) {
+        if (g.dim == 2 && a == 1) continue;
+        if (g.n[a] < 2) { set_error("every axis needs at least 2 cells"); return SARAS_BAD_SHAPE; }
+        if (!(g.len[a] > 0)) { set_error("domain lengths must be positive"); return SARAS_BAD_ARG; }
+    }
+    if (p.res_type < 0 || p.res_type > 2) { set_error("Residual Type must be 0, 1 or 2"); return SARAS_BAD_ARG; }
+    if (p.bc < 0 || p.bc > 2 || p.precision < 0 || p.precision > 1) { set_error("bad bc/precision"); return SARAS_BAD_ARG; }
+    if (p.vc_count < 0 || p.pre_smooth < 0 || p.post_smooth < 0) { set_error("negative counts"); return SARAS_BAD_ARG; }
+    std::unique_ptr<saras_mg> m(new saras_mg());
+    m->g = g;
+    m->p = p;
+    if (m->p.inner_cycles <= 0) m->p.inner_cycles = 3;
+    if (m->p.max_refine <= 0) m->p.max_refine = 20;
+    if (!saras::build_levels(g, m->G)) {
+        set_error("grid cannot be coarsened (setVCDepth gives a negative depth)");
+        return SARAS_BAD_SHAPE;
+    }
+    m->depth = m->G.depth;
+    {
+        const auto &C = m->G.axes[m->depth];
+        const int cx = C[0].n, cy = g.dim == 3 ? C[1].n : 1, cz = C[2].n;
+        m->max_count = g.dim == 2 ? (int)(2.5 * cx * cz) : (int)(2.5 * cx * cy * cz);
+    }
+    const auto walls = make_walls(g, p);
+    saras_status st = m->h64.build(m->G, walls);
+    if (st) return st;
+    if (p.precision == SARAS_PREC_MIXED) {
+        m->h32.reset(new Hierarchy<float>());
+        std::vector<std::vector<double>> nowall(3);
+        st = m->h32->build(m->G, nowall);
+        if (st) return st;
+        HIP_TRY(hipMalloc((void **)&m->x64b, m->h64.lv[0].elems * sizeof(double)));
+    }
+    HIP_TRY(hipMalloc((void **)&m->partial, 2 * saras::kNormBlocks * sizeof(double)));
+    HIP_TRY(hipMalloc((void **)&m->norm_out, 2 * sizeof(double)));
+    HIP_TRY(hipMalloc((void **)&m->iters_dev, sizeof(int)));
+    HIP_TRY(hipHostMalloc((void **)&m->norm_host, 2 * sizeof(double), hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void **)&m->iters_host, sizeof(int), hipHostMallocDefault));
+    HIP_TRY(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreate(&m->ev0));
+    HIP_TRY(hipEventCreate(&m->ev1));
+    *out = m.release();
+    return SARAS_OK;
+}
+
+saras_status saras_mg_destroy(saras_mg *mg) {
+    if (!mg) return SARAS_BAD_ARG;
+    (void)hipStreamSynchronize(mg->stream);
+    delete mg;
+    return SARAS_OK;
+}
+
+int saras_mg_levels(const saras_mg *mg) { return mg ? mg->depth + 1 : -1; }
+
+saras_status saras_mg_level_shape(const saras_mg *mg, int level, int n[3]) {
+    if (!mg || !n || level < 0 || level > mg->depth) return SARAS_BAD_ARG;
+    for (int a = 0; a < 3; ++a) n[a] = mg->h64.lv[level].n[a];
+    return SARAS_OK;
+}
+
+static saras_status solve_fp64(saras_mg *m, hipStream_t s, saras_mg_stats *st) {
+    Ops<double> ops{m, m->h64, s, true, m->p.sor};
+    auto &L0 = m->h64.lv[0];
+    int cycles = 0;
+    double res = 0.0;
+    bool conv = false;
+    for (int c = 0; c < m->p.vc_count; ++c) {
+        ops.vcycle();
+        if (m->p.solve_flag) {
+            saras_status e = fetch_coarse_iters(m, s);
+            if (e) return e;
+        }
+        saras_status e = compute_error(m, m->p.res_type, L0.x, s, &res);
+        if (e) return e;
+        cycles = c + 1;
+        if (res < m->p.vc_tol) {
+            conv = true;
+            break;
+        }
+    }
+    st->cycles = cycles;
+    st->refinements = 0;
+    st->residual = res;
+    st->converged = conv;
+    return SARAS_OK;
+}
+
+saras_status saras_mg_solve(saras_mg *mg, double *x_full, const double *b_full, int ptr_kind, void *stream,
+                            saras_mg_stats *stats) {
+    if (!mg || !x_full || !b_full) {
+        set_error("null argument");
+        return SARAS_BAD_ARG;
+    }
+    if (ptr_kind != SARAS_PTR_HOST && ptr_kind != SARAS_PTR_DEVICE) return SARAS_BAD_ARG;
+    saras_mg *m = mg;
+    hipStream_t s = pick_stream(m, stream);
+    const long fe = full_elems(m->g);
+    double *dx = x_full;
+    const double *db = b_full;
+    if (ptr_kind == SARAS_PTR_HOST) {
+        if (m->stage_elems < (size_t)fe) {
+            (void)hipFree(m->stage_x);
+            (void)hipFree(m->stage_b);
+            m->stage_x = m->stage_b = nullptr;
+            HIP_TRY(hipMalloc((void **)&m->stage_x, fe * sizeof(double)));
+            HIP_TRY(hipMalloc((void **)&m->stage_b, fe * sizeof(double)));
+            m->stage_elems = fe;
+        }
+        HIP_TRY(hipMemcpyAsync(m->stage_x, x_full, fe * sizeof(double), hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(m->stage_b, b_full, fe * sizeof(double), hipMemcpyHostToDevice, s));
+        dx = m->stage_x;
+        db = m->stage_b;
+    }
+    m->coarse_iters_max = 0;
+    m->coarse_capped = 0;
+    HIP_TRY(hipEventRecord(m->ev0, s));
+    auto &L0 = m->h64.lv[0];
+    load_core<double>(m, db, L0, L0.b, s);
+    load_core<double>(m, dx, L0, L0.x, s);
+    saras_mg_stats st{};
+    saras_status e = SARAS_OK;
+    if (m->p.precision == SARAS_PREC_FP64) {
+        e = solve_fp64(m, s, &st);
+    } else {
+        e = SARAS_BAD_ARG;
+        set_error("mixed precision not built yet");
+    }
+    if (e) return e;
+    store_full(m, dx, L0.x, s);
+    HIP_TRY(hipEventRecord(m->ev1, s));
+    if (ptr_kind == SARAS_PTR_HOST)
+        HIP_TRY(hipMemcpyAsync(x_full, m->stage_x, fe * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, m->ev0, m->ev1));
+    st.device_ms = ms;
+    st.coarse_iters_max = m->coarse_iters_max;
+    st.coarse_capped = m->coarse_capped;
+    if (stats) *stats = st;
+    return st.converged ? SARAS_OK : SARAS_NOT_CONVERGED;
+}
+
+// ---- single-operation entry points ----
+saras_status saras_mg_set_level(saras_mg *mg, int level, const double *lhs_pad, const double *rhs_pad) {
+    if (!mg || level < 0 || level > mg->depth) return SARAS_BAD_ARG;
+    auto &L = mg->h64.lv[level];
+    const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
+    std::vector<double> core(L.elems, 0.0);
+    const bool d3 = mg->g.dim == 3;
+    for (int which = 0; which < 2; ++which) {
+        const double *src = which == 0 ? lhs_pad : rhs_pad;
+        if (!src) continue;
+        for (int i = 0; i < nx; ++i)
+            for (int j = 0; j < ny; ++j)
+                for (int k = 0; k < nz; ++k) {
+                    const long f = d3 ? ((long)(i + 1) * (ny + 2) + (j + 1)) * (nz + 2) + (k + 1)
+                                      : (long)(i + 1) * (nz + 2) + (k + 1);
+                    core[(size_t)i * L.sx + (size_t)j * L.sy + k] = src[f];
+                }
+        HIP_TRY(hipMemcpy(which == 0 ? L.x : L.b, core.data(), L.elems * sizeof(double), hipMemcpyHostToDevice));
+    }
+    return SARAS_OK;
+}
+
+saras_status saras_mg_get_level(saras_mg *mg, int level, double *lhs_pad, double *rhs_pad) {
+    if (!mg || level < 0 || level > mg->depth) return SARAS_BAD_ARG;
+    HIP_TRY(hipStreamSynchronize(mg->stream));
+    auto &L = mg->h64.lv[level];
+    const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
+    std::vector<double> core(L.elems);
+    const bool d3 = mg->g.dim == 3;
+    const size_t padded = d3 ? (size_t)(nx + 2) * (ny + 2) * (nz + 2) : (size_t)(nx + 2) * (nz + 2);
+    for (int which = 0; which < 2; ++which) {
+        double *dst = which == 0 ? lhs_pad : rhs_pad;
+        if (!dst) continue;
+        HIP_TRY(hipMemcpy(core.data(), which == 0 ? L.x : L.b, L.elems * sizeof(double), hipMemcpyDeviceToHost));
+        std::memset(dst, 0, padded * sizeof(double));
+        for (int i = 0; i < nx; ++i)
+            for (int j = 0; j < ny; ++j)
+                for (int k = 0; k < nz; ++k) {
+                    const long f = d3 ? ((long)(i + 1) * (ny + 2) + (j + 1)) * (nz + 2) + (k + 1)
+                                      : (long)(i + 1) * (nz + 2) + (k + 1);
+                    dst[f] = core[(size_t)i * L.sx + (size_t)j * L.sy + k];
+                }
+    }
+    return SARAS_OK;
+}
+
+saras_status saras_mg_smooth(saras_mg *mg, int level, int count, int zero_bc) {
+    if (!mg || level < 0 || level > mg->depth || count < 0) return SARAS_BAD_ARG;
+    Ops<double> ops{mg, mg->h64, mg->stream, true, mg->p.sor};
+    ops.smooth(level, count, zero_bc != 0);
+    HIP_TRY(hipStreamSynchronize(mg->stream));
+    HIP_TRY(hipGetLastError());
+    return SARAS_OK;
+}
+
+saras_status saras_mg_coarse_solve(saras_mg *mg, int level, int zero_bc, int *iters) {
+    if (!mg || level < 0 || level > mg->depth) return SARAS_BAD_ARG;
+    if (level != mg->depth) {
+        set_error("the single-workgroup solve runs on the coarsest level only");
+        return SARAS_BAD_ARG;
+    }
+    Ops<double> ops{mg, mg->h64, mg->stream, true, mg->p.sor};
+    ops.coarse(level, zero_bc != 0, 0, 0);
+    HIP_TRY(hipMemcpyAsync(mg->iters_host, mg->iters_dev, sizeof(int), hipMemcpyDeviceToHost, mg->stream));
+    HIP_TRY(hipStreamSynchronize(mg->stream));
+    if (iters) *iters = *mg->iters_host;
+    return SARAS_OK;
+}
+
+saras_status saras_mg_residual_restrict(saras_mg *mg, int level) {
+    if (!mg || level < 0 || level >= mg->depth) return SARAS_BAD_ARG;
+    Ops<double> ops{mg, mg->h64, mg->stream, true, mg->p.sor};
+    ops.resid_restrict(level);
+    HIP_TRY(hipStreamSynchronize(mg->stream));
+    return SARAS_OK;
+}
+
+saras_status saras_mg_prolong_correct(saras_mg *mg, int level) {
+    if (!mg || level < 0 || level >= mg->depth) return SARAS_BAD_ARG;
+    Ops<double> ops{mg, mg->h64, mg->stream, true, mg->p.sor};
+    ops.prolong_correct(level);
+    HIP_TRY(hipStreamSynchronize(mg->stream));
+    return SARAS_OK;
+}
+
+saras_status saras_mg_error(saras_mg *mg, int res_type, double *value) {
+    if (!mg || !value || res_type < 0 || res_type > 2) return SARAS_BAD_ARG;
+    return compute_error(mg, res_type, mg->h64.lv[0].x, mg->stream, value);
+}
+
+// ---- measurement hooks ----
+saras_status saras_mg_bench_cycles(saras_mg *mg, int count, double *ms) {
+    if (!mg || count < 0 || !ms) return SARAS_BAD_ARG;
+    hipStream_t s = mg->stream;
+    HIP_TRY(hipEventRecord(mg->ev0, s));
+    if (mg->p.precision == SARAS_PREC_FP64) {
+        Ops<double> ops{mg, mg->h64, s, true, mg->p.sor};
+        for (int c = 0; c < count; ++c) ops.vcycle();
+    } else {
+        Ops<float> ops{mg, *mg->h32, s, false, (float)mg->p.sor};
+        for (int c = 0; c < count; ++c) ops.vcycle();
+    }
+    HIP_TRY(hipEventRecord(mg->ev1, s));
+    HIP_TRY(hipEventSynchronize(mg->ev1));
+    HIP_TRY(hipGetLastError());
+    float f = 0.f;
+    HIP_TRY(hipEventElapsedTime(&f, mg->ev0, mg->ev1));
+    *ms = f;
+    return SARAS_OK;
+}
+
+saras_status saras_mg_bench_sweeps(saras_mg *mg, int reps, double *ms) {
+    if (!mg || reps < 0 || !ms) return SARAS_BAD_ARG;
+    if (mg->depth == 0) {
+        set_error("level 0 is the coarsest level");
+        return SARAS_BAD_ARG;
+    }
+    hipStream_t s = mg->stream;
+    HIP_TRY(hipEventRecord(mg->ev0, s));
+    if (mg->p.precision == SARAS_PREC_FP64) {
+        Ops<double> ops{mg, mg->h64, s, true, mg->p.sor};
+        for (int c = 0; c < reps; ++c) ops.rb_sweep(0, false);
+    } else {
+        Ops<float> ops{mg, *mg->h32, s, false, (float)mg->p.sor};
+        for (int c = 0; c < reps; ++c) ops.rb_sweep(0, true);
+    }
+    HIP_TRY(hipEventRecord(mg->ev1, s));
+    HIP_TRY(hipEventSynchronize(mg->ev1));
+    HIP_TRY(hipGetLastError());
+    float f = 0.f;
+    HIP_TRY(hipEventElapsedTime(&f, mg->ev0, mg->ev1));
+    *ms = f;
+    return SARAS_OK;
+}
+
+saras_status saras_bench_stream_copy(size_t n, int reps, double *ms) {
+    if (!ms || n < 2 || reps < 1) return SARAS_BAD_ARG;
+    double *a = nullptr, *b = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const long n2 = (long)(n / 2);
+    HIP_TRY(hipMalloc((void **)&a, n2 * 2 * sizeof(double)));
+    HIP_TRY(hipMalloc((void **)&b, n2 * 2 * sizeof(double)));
+    HIP_TRY(hipMemset(a, 0, n2 * 2 * sizeof(double)));
+    HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    const unsigned blocks = 256 * 16;
+    hipLaunchKernelGGL(saras::k_stream_copy, dim3(blocks), dim3(256), 0, s, (const double2 *)a, (double2 *)b, n2);
+    HIP_TRY(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; ++r)
+        hipLaunchKernelGGL(saras::k_stream_copy, dim3(blocks), dim3(256), 0, s, (const double2 *)a, (double2 *)b, n2);
+    HIP_TRY(hipEventRecord(e1, s));
+    HIP_TRY(hipEventSynchronize(e1));
+    float f = 0.f;
+    HIP_TRY(hipEventElapsedTime(&f, e0, e1));
+    *ms = f;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(s);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return SARAS_OK;
+}
+
+// ---- device memory helpers ----
+saras_status saras_device_alloc(void **ptr, size_t bytes) {
+    if (!ptr) return SARAS_BAD_ARG;
+    HIP_TRY(hipMalloc(ptr, bytes));
+    return SARAS_OK;
+}
+saras_status saras_device_free(void *ptr) {
+    HIP_TRY(hipFree(ptr));
+    return SARAS_OK;
+}
+saras_status saras_memcpy_h2d(void *dst, const void *src, size_t bytes) {
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return SARAS_OK;
+}
+saras_status saras_memcpy_d2h(void *dst, const void *src, size_t bytes) {
+    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return SARAS_OK;
+}
+saras_status saras_device_sync(void) {
+    HIP_TRY(hipDeviceSynchronize());
+    return SARAS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,201 @@
+"""GPU parity of the FP64 multigrid path against the FP64 CPU oracle.
+
+Bar: bit-exact.  Both sides evaluate the reference's expressions in the same
+order with separately rounded operations (FMA contraction off), so every
+smoother sweep, restriction, prolongation, coarse solve, L-inf norm and whole
+solve must agree to the last bit.  Sum-based norms (L1, L2) reduce in a
+different order and are held to 1e-13 relative.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import saras_amd as S
+
+pytestmark = pytest.mark.gpu
+
+BCMAP = {O.BC_NEUMANN: S.BC_NEUMANN, O.BC_DIRICHLET: S.BC_DIRICHLET,
+         O.BC_DIRICHLET_TEST: S.BC_DIRICHLET_TEST}
+
+CASES = [
+    # n, dim, periodic, gtype, bc
+    ((8, 8, 8), 3, (0, 0, 0), (0, 0, 0), O.BC_DIRICHLET),
+    ((16, 16, 16), 3, (0, 0, 0), (2, 2, 2), O.BC_DIRICHLET),
+    ((16, 16, 32), 3, (1, 1, 0), (0, 0, 2), O.BC_NEUMANN),
+    ((16, 8, 16), 3, (1, 1, 1), (0, 0, 0), O.BC_NEUMANN),
+    ((12, 12, 12), 3, (1, 0, 1), (0, 1, 0), O.BC_NEUMANN),
+    ((16, 16, 16), 3, (0, 0, 0), (0, 0, 0), O.BC_DIRICHLET_TEST),
+    ((24, 24, 24), 3, (1, 1, 1), (0, 0, 0), O.BC_NEUMANN),
+    ((3, 3, 3), 3, (1, 1, 1), (0, 0, 0), O.BC_NEUMANN),
+    ((32, 1, 16), 2, (0, 0, 0), (2, 0, 0), O.BC_DIRICHLET),
+    ((32, 1, 32), 2, (1, 0, 0), (0, 0, 2), O.BC_NEUMANN),
+    ((32, 1, 32), 2, (0, 0, 0), (0, 0, 0), O.BC_DIRICHLET_TEST),
+    ((5, 1, 7), 2, (1, 0, 1), (0, 0, 0), O.BC_NEUMANN),
+]
+IDS = [f"{c[1]}d-{'x'.join(map(str, c[0]))}-p{''.join(map(str, c[2]))}-g{''.join(map(str, c[3]))}-bc{c[4]}"
+       for c in CASES]
+
+
+def pair(n, dim, per, gt, bc, beta=(0.7, 0.6, 0.8), **kw):
+    go = O.grid_desc(n, dim=dim, periodic=per, gtype=gt, beta=beta)
+    gs = S.grid_desc(n, dim=dim, periodic=per, gtype=gt, beta=beta)
+    po = O.mg_params(bc=bc, **kw)
+    ps = S.mg_params(bc=BCMAP[bc], **kw)
+    return O.Multigrid(go, po), S.Multigrid(gs, ps)
+
+
+def inner(a):
+    return a[(slice(1, -1),) * a.ndim]
+
+
+def rand_pad(shape, rng):
+    a = np.zeros(shape)
+    a[(slice(1, -1),) * len(shape)] = rng.standard_normal(tuple(s - 2 for s in shape))
+    return a
+
+
+@pytest.mark.parametrize("case", CASES, ids=IDS)
+@pytest.mark.parametrize("zero_bc", [0, 1])
+def test_smooth_bitexact(case, zero_bc):
+    mo, ms = pair(*case, sor=1.15)
+    rng = np.random.default_rng(11)
+    shp = mo.padded_shape(0)
+    x, b = rand_pad(shp, rng), rand_pad(shp, rng)
+    mo.set_level(0, x, b)
+    ms.set_level(0, x, b)
+    mo.smooth(0, 3, zero_bc)
+    ms.smooth(0, 3, zero_bc)
+    np.testing.assert_array_equal(inner(ms.get_level(0)[0]), inner(mo.get_level(0)[0]))
+
+
+@pytest.mark.parametrize("case", [c for c in CASES if c[0] != (3, 3, 3) and c[0] != (5, 1, 7)],
+                         ids=[i for c, i in zip(CASES, IDS) if c[0] != (3, 3, 3) and c[0] != (5, 1, 7)])
+def test_restrict_prolong_bitexact(case):
+    mo, ms = pair(*case)
+    rng = np.random.default_rng(12)
+    levels = mo.levels
+    assert ms.levels == levels
+    for lev in range(levels - 1):
+        shp = mo.padded_shape(lev)
+        x, b = rand_pad(shp, rng), rand_pad(shp, rng)
+        mo.set_level(lev, x, b)
+        ms.set_level(lev, x, b)
+        mo.impose_bc(lev, 0 if lev == 0 else 1)
+        mo.residual_restrict(lev)
+        ms.residual_restrict(lev)
+        lo, ro = mo.get_level(lev + 1)
+        ls, rs = ms.get_level(lev + 1)
+        np.testing.assert_array_equal(inner(rs), inner(ro))
+        np.testing.assert_array_equal(inner(ls), inner(lo))
+        e = rand_pad(mo.padded_shape(lev + 1), rng)
+        mo.set_level(lev + 1, e, None)
+        ms.set_level(lev + 1, e, None)
+        mo.prolong_correct(lev)
+        ms.prolong_correct(lev)
+        np.testing.assert_array_equal(inner(ms.get_level(lev)[0]), inner(mo.get_level(lev)[0]))
+
+
+@pytest.mark.parametrize("case", CASES, ids=IDS)
+def test_coarse_solve_bitexact(case):
+    mo, ms = pair(*case, mg_tol=1e-9)
+    rng = np.random.default_rng(13)
+    D = mo.levels - 1
+    shp = mo.padded_shape(D)
+    x, b = rand_pad(shp, rng), rand_pad(shp, rng)
+    mo.set_level(D, x, b)
+    ms.set_level(D, x, b)
+    zb = 1 if D > 0 else 0
+    io = mo.coarse_solve(D, zb)
+    is_ = ms.coarse_solve(D, zb)
+    assert is_ == io
+    np.testing.assert_array_equal(inner(ms.get_level(D)[0]), inner(mo.get_level(D)[0]))
+
+
+@pytest.mark.parametrize("case", CASES[:6] + CASES[8:11], ids=IDS[:6] + IDS[8:11])
+@pytest.mark.parametrize("res_type", [0, 1, 2])
+def test_error_norms(case, res_type):
+    mo, ms = pair(*case)
+    rng = np.random.default_rng(14)
+    shp = mo.padded_shape(0)
+    x, b = rand_pad(shp, rng), rand_pad(shp, rng)
+    mo.set_level(0, x, b)
+    ms.set_level(0, x, b)
+    mo.impose_bc(0, 0)
+    eo, es = mo.error(res_type), ms.error(res_type)
+    if res_type == 0:
+        assert es == eo
+    else:
+        assert es == pytest.approx(eo, rel=1e-13)
+
+
+def rhs_for(mo, rng, compatible):
+    g = mo.g
+    shp = O.field_shape(g, 1)
+    b = np.zeros(shp)
+    c = rng.standard_normal(tuple(s - 2 for s in shp))
+    if compatible:
+        c -= c.mean()
+    b[(slice(1, -1),) * len(shp)] = c
+    return b
+
+
+SOLVE_CASES = [c for c in CASES if c[0] not in ((3, 3, 3), (5, 1, 7))]
+SOLVE_IDS = [i for c, i in zip(CASES, IDS) if c[0] not in ((3, 3, 3), (5, 1, 7))]
+
+
+@pytest.mark.parametrize("case", SOLVE_CASES, ids=SOLVE_IDS)
+def test_full_solve_bitexact(case):
+    kw = dict(vc_count=12, vc_tol=1e-9, mg_tol=1e-6, pre=2, post=2)
+    mo, ms = pair(*case, **kw)
+    rng = np.random.default_rng(15)
+    b = rhs_for(mo, rng, case[4] == O.BC_NEUMANN)
+    x0 = rhs_for(mo, rng, False) * 0.1
+    xo, so = mo.solve(x0.copy(), b, 1, history=True)
+    xs, ss, status = ms.solve(x0.copy(), b)
+    assert ss["cycles"] == so["cycles"]
+    assert ss["residual"] == so["residual"]
+    assert ss["coarse_iters_max"] == so["coarse_iters_max"]
+    np.testing.assert_array_equal(xs, xo)   # core and the whole ghost shell
+
+
+def test_config1_dirichlet_sine_64():
+    """64^3 unit cube, homogeneous Dirichlet, b = -3 pi^2 sin sin sin + U(-1e-3,
+    1e-3) (seed 1), V(2,2) RB-GS, coarsest 4^3 solved to 1e-6, L-inf relative
+    residual 1e-8: identical V-cycle count, residual and solution bits."""
+    n = 64
+    mo, ms = pair((n, n, n), 3, (0, 0, 0), (0, 0, 0), O.BC_DIRICHLET,
+                  vc_count=100, vc_tol=1e-8, mg_tol=1e-6, pre=2, post=2)
+    assert mo.levels == 5 and mo.level_shape(4) == (4, 4, 4)
+    xc = (np.arange(n) + 0.5) / n
+    X, Y, Z = np.meshgrid(xc, xc, xc, indexing="ij")
+    rng = np.random.default_rng(1)
+    core = -3 * np.pi ** 2 * np.sin(np.pi * X) * np.sin(np.pi * Y) * np.sin(np.pi * Z)
+    core += rng.uniform(-1e-3, 1e-3, core.shape)
+    b = np.pad(core, 1)
+    xo, so = mo.solve(np.zeros_like(b), b, 1)
+    xs, ss, status = ms.solve(np.zeros_like(b), b)
+    assert status == S.OK and ss["converged"]
+    assert ss["cycles"] == so["cycles"] and ss["residual"] == so["residual"]
+    np.testing.assert_array_equal(xs, xo)
+    u = np.sin(np.pi * X) * np.sin(np.pi * Y) * np.sin(np.pi * Z)
+    assert np.abs(xs[1:-1, 1:-1, 1:-1] - u).max() < 2e-3
+
+
+def test_device_pointer_path_and_pad2():
+    n = 32
+    go = O.grid_desc((n, n, n), periodic=(1, 1, 0), gtype=(0, 0, 2), beta=(1, 1, 0.9), pad=2)
+    gs = S.grid_desc((n, n, n), periodic=(1, 1, 0), gtype=(0, 0, 2), beta=(1, 1, 0.9), pad=2)
+    kw = dict(vc_count=8, vc_tol=1e-12, mg_tol=1e-3, pre=3, post=6)
+    mo = O.Multigrid(go, O.mg_params(bc=O.BC_NEUMANN, **kw))
+    ms = S.Multigrid(gs, S.mg_params(bc=S.BC_NEUMANN, **kw))
+    rng = np.random.default_rng(16)
+    shp = (n + 4,) * 3
+    b = rng.standard_normal(shp)
+    x0 = rng.standard_normal(shp)   # outer pad layer must survive untouched
+    xo, so = mo.solve(x0.copy(), b, 2)
+    xd, bd = S.DeviceArray.from_host(x0), S.DeviceArray.from_host(b)
+    ss, status = ms.solve_device(xd, bd)
+    xs = xd.to_host(shp)
+    assert status == S.NOT_CONVERGED and ss["cycles"] == 8
+    np.testing.assert_array_equal(xs, xo)
+    np.testing.assert_array_equal(xs[0], x0[0])

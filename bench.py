@@ -3,16 +3,22 @@
 Workload (one "step" = one V-cycle): 256^3 cells, periodic in x and y,
 Neumann walls in z, double-sided tanh stretching beta = 1.5 in z (the grid of
 the 256^3 Rayleigh-Benard configuration), RB-SOR omega = 1, V(2,2), coarsest
-2^3 level solved to 1e-2 (the reference's "Solve Coarsest" defaults).
+2^3 level solved to 1e-2 ("Solve Coarsest" defaults).  The headline V-cycle
+is the FP32 inner cycle of the mixed-precision solver; the FP64 cycle of the
+plain path is reported beside it.
 
 value: algorithmic HBM bytes per V-cycle / device time per V-cycle, with the
 per-level byte model  B_l = N_l s (3 (nu1 + nu2) + 2 + 1/8 + 2 + 1/8)
 (3 words per smoothing sweep, residual+restriction reads x, b and writes
-b_{l+1}, prolongation reads e_{l+1} and reads/writes x).
+b_{l+1}, prolongation reads e_{l+1} and reads/writes x), s = bytes per word.
 
-Also reported: the FP64 V-cycle rate, the mixed-precision vs plain FP64 time
-to a 1e-10 relative residual on a 256^3 periodic problem, the dominant
-kernel's roofline and the FP64 CPU oracle timed on the host cores.
+solve: time to a 1e-10 relative L-inf residual on a 256^3 periodic uniform
+grid, RHS = 64 Fourier modes with amplitudes ~ |k|^-2 and zero mean, plain
+FP64 vs mixed precision, inputs already resident in HBM.
+
+roofline: the dominant kernel (finest-level fused sweep), algorithmic
+3 words per cell per launch / average launch time (HIP events).
+cpu_baseline: the FP64 CPU oracle's V-cycle on the same workload.
 """
 import argparse
 import json
@@ -33,12 +39,9 @@ N = 256
 
 def bytes_per_vcycle(shapes, s, nu1, nu2):
     tot = 0.0
-    for lev, shp in enumerate(shapes[:-1]):
-        n = float(np.prod(shp))
-        tot += n * s * (3 * (nu1 + nu2) + 2 + 1 / 8 + 2 + 1 / 8)
-    n = float(np.prod(shapes[-1]))
-    tot += n * s * 3 * (nu1 + nu2)
-    return tot
+    for shp in shapes[:-1]:
+        tot += float(np.prod(shp)) * s * (3 * (nu1 + nu2) + 2 + 1 / 8 + 2 + 1 / 8)
+    return tot + float(np.prod(shapes[-1])) * s * 3 * (nu1 + nu2)
 
 
 def rbc_grid():
@@ -54,9 +57,90 @@ def rbc_rhs(seed=2):
     return b
 
 
-def time_cycles(mg, warmup, steps):
-    mg.bench_cycles(warmup)
-    return mg.bench_cycles(steps) / steps
+def fourier_rhs(n, modes=64, kmax=16, seed=3):
+    """sum_m A_m cos(2 pi k_m . x + phi_m), A_m = U(0.5, 1) |k_m|^-2, zero mean."""
+    rng = np.random.default_rng(seed)
+    xc = (np.arange(n) + 0.5) / n
+    acc = np.zeros((n, n, n))
+    for _ in range(modes):
+        while True:
+            k = rng.integers(-kmax, kmax + 1, 3)
+            if np.abs(k).max() >= 1:
+                break
+        amp = rng.uniform(0.5, 1.0) / float(k @ k)
+        phi = rng.uniform(0, 2 * np.pi)
+        ex = np.exp(2j * np.pi * k[0] * xc)
+        ey = np.exp(2j * np.pi * k[1] * xc)
+        ez = np.exp(2j * np.pi * k[2] * xc) * np.exp(1j * phi)
+        acc += amp * np.real(ex[:, None, None] * (ey[:, None] * ez[None, :])[None, :, :])
+    acc -= acc.mean()
+    b = np.zeros((n + 2,) * 3)
+    b[1:-1, 1:-1, 1:-1] = acc
+    return b
+
+
+def cycle_rates(args, nu1, nu2):
+    g = rbc_grid()
+    b = rbc_rhs()
+    out = {}
+    for name, prec in (("fp64", S.PREC_FP64), ("mixed", S.PREC_MIXED)):
+        p = S.mg_params(vc_count=1, vc_tol=0.0, mg_tol=1e-2, pre=nu1, post=nu2, bc=S.BC_NEUMANN,
+                        precision=prec, inner_cycles=1, max_refine=1)
+        mg = S.Multigrid(g, p)
+        mg.solve(np.zeros_like(b), b)            # puts b and a first iterate on the device
+        shapes = [mg.level_shape(lv) for lv in range(mg.levels)]
+        s = 8 if prec == S.PREC_FP64 else 4
+        mg.bench_cycles(args.warmup)
+        ms = mg.bench_cycles(args.steps) / args.steps
+        reps = 20
+        mg.bench_sweeps(2)
+        launch_ms = mg.bench_sweeps(reps) / reps
+        n0 = float(np.prod(shapes[0]))
+        out[name] = dict(ms=ms, bv=bytes_per_vcycle(shapes, s, nu1, nu2), launch_ms=launch_ms,
+                         launch_bytes=3.0 * n0 * s, shapes=shapes, s=s)
+        mg.close()
+    return out
+
+
+def profile_sweeps(args, nu1, nu2):
+    """Finest-level sweeps only: one launch per sweep, `steps` launches per
+    precision, plus 10 launches of the plain copy kernel over 2^28 doubles."""
+    g = rbc_grid()
+    b = rbc_rhs()
+    for prec in (S.PREC_MIXED, S.PREC_FP64):
+        p = S.mg_params(vc_count=1, vc_tol=0.0, mg_tol=1e-2, pre=nu1, post=nu2, bc=S.BC_NEUMANN,
+                        precision=prec, inner_cycles=1, max_refine=1)
+        mg = S.Multigrid(g, p)
+        mg.solve(np.zeros_like(b), b)
+        ms = mg.bench_sweeps(args.steps) / args.steps
+        print(json.dumps({"precision": "mixed" if prec else "fp64", "sweep_ms": ms}), flush=True)
+        mg.close()
+    ms = S.bench_stream_copy(1 << 28, 10) / 10
+    print(json.dumps({"copy_ms": ms, "copy_bytes": 16 * (1 << 28)}), flush=True)
+
+
+def solve_compare(nu1, nu2, tol=1e-10):
+    n = N
+    g = S.grid_desc((n, n, n), periodic=(1, 1, 1))
+    b = fourier_rhs(n)
+    bd = S.DeviceArray.from_host(b)
+    res = {}
+    for name, prec in (("fp64", S.PREC_FP64), ("mixed", S.PREC_MIXED)):
+        p = S.mg_params(vc_count=100, vc_tol=tol, mg_tol=1e-2, pre=nu1, post=nu2, bc=S.BC_NEUMANN,
+                        precision=prec, inner_cycles=4, max_refine=40)
+        mg = S.Multigrid(g, p)
+        best = None
+        for _ in range(3):
+            xd = S.DeviceArray.from_host(np.zeros_like(b))
+            st, status = mg.solve_device(xd, bd)
+            if best is None or st["device_ms"] < best["device_ms"]:
+                best = st
+            xd.free()
+        res[name] = dict(ms=best["device_ms"], cycles=best["cycles"], refinements=best["refinements"],
+                         residual=best["residual"], converged=best["converged"])
+        mg.close()
+    bd.free()
+    return res
 
 
 def cpu_baseline(nu1, nu2, shapes, budget_s=12.0):
@@ -91,6 +175,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--profile-sweep", action="store_true",
+                    help="only finest-level sweeps (FP32 and FP64) and the copy kernel, for counter runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build measures the single-GPU path "
@@ -98,31 +185,16 @@ def main():
         sys.exit(2)
 
     nu1 = nu2 = 2
-    g = rbc_grid()
-    b = rbc_rhs()
-    res = {}
-    for name, prec in (("fp64", S.PREC_FP64),):
-        p = S.mg_params(vc_count=0, vc_tol=0.0, mg_tol=1e-2, pre=nu1, post=nu2, bc=S.BC_NEUMANN,
-                        precision=prec)
-        mg = S.Multigrid(g, p)
-        mg.solve(np.zeros_like(b), b)           # loads b, x = 0 onto the device
-        shapes = [mg.level_shape(l) for l in range(mg.levels)]
-        s = 8 if prec == S.PREC_FP64 else 4
-        ms = time_cycles(mg, args.warmup, args.steps)
-        bv = bytes_per_vcycle(shapes, s, nu1, nu2)
-        # dominant kernel: finest-level colour pass, 1.5 words per cell per launch
-        reps = 20
-        sweep_ms = mg.bench_sweeps(reps) / reps
-        launch_ms = sweep_ms / 2
-        n0 = float(np.prod(shapes[0]))
-        res[name] = dict(ms=ms, bv=bv, launch_ms=launch_ms, launch_bytes=1.5 * n0 * s, shapes=shapes)
-        mg.close()
-
-    head = res["fp64"]
+    if args.profile_sweep:
+        profile_sweeps(args, nu1, nu2)
+        return
+    rates = cycle_rates(args, nu1, nu2)
+    head, f64 = rates["mixed"], rates["fp64"]
     value = head["bv"] / (head["ms"] * 1e-3) / 1e12
     achieved = head["launch_bytes"] / (head["launch_ms"] * 1e-3) / 1e9
-    copy_n = 1 << 27
-    copy_ms = S.bench_stream_copy(copy_n, 20) / 20
+    copy_n = 1 << 28
+    S.bench_stream_copy(copy_n, 2)
+    copy_ms = S.bench_stream_copy(copy_n, 10) / 10
     copy_gbs = 16.0 * copy_n / (copy_ms * 1e-3) / 1e9
     out = {
         "metric": "V-cycle HBM TB/s at 256^3; batched solves/s + p99 latency vs plain FP64",
@@ -135,17 +207,31 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "poisson_vcycle_256^3_PPN_z-tanh1.5_neumann_V(2,2)", "cells": [N, N, N],
-                   "levels": len(head["shapes"]), "precision": "fp64"},
+                   "levels": len(head["shapes"]), "cycle_precision": "fp32 inner cycle of the mixed solver"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_rb_pass<double,3,true>", "launch_ms": round(head["launch_ms"], 5)},
+                     "kernel": "k_sweep_march<float,64,8,true>", "launch_ms": round(head["launch_ms"], 5),
+                     "bytes_per_launch": head["launch_bytes"]},
+        "fp64_vcycle": {"ms": round(f64["ms"], 4),
+                        "tbps": round(f64["bv"] / (f64["ms"] * 1e-3) / 1e12, 4),
+                        "sweep_launch_ms": round(f64["launch_ms"], 5),
+                        "sweep_gbs": round(f64["launch_bytes"] / (f64["launch_ms"] * 1e-3) / 1e9, 1)},
         "stream_copy_gbs": round(copy_gbs, 1),
     }
+    if not args.no_solve:
+        sc = solve_compare(nu1, nu2)
+        out["solve_256^3_periodic_to_1e-10"] = {
+            "fp64_ms": round(sc["fp64"]["ms"], 3), "fp64_cycles": sc["fp64"]["cycles"],
+            "fp64_residual": sc["fp64"]["residual"], "fp64_converged": sc["fp64"]["converged"],
+            "mixed_ms": round(sc["mixed"]["ms"], 3), "mixed_fp32_cycles": sc["mixed"]["cycles"],
+            "mixed_refinements": sc["mixed"]["refinements"], "mixed_residual": sc["mixed"]["residual"],
+            "mixed_converged": sc["mixed"]["converged"],
+            "speedup": round(sc["fp64"]["ms"] / sc["mixed"]["ms"], 3)}
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(nu1, nu2, head["shapes"])
+        out["cpu_baseline"] = cpu_baseline(nu1, nu2, f64["shapes"])
     print(json.dumps(out))
 
 

@@ -652,3 +652,19 @@ void or_mg_vcycles(or_mg *m, int count)
 {
     for (int c = 0; c < count; c++) vcycle(m);
 }
+
+void or_mg_residual(or_mg *m, int level, int zero_bc, double *out_pad)
+{
+    m->vLevel = level; m->zeroBC = zero_bc;
+    level_t *L = &m->lv[level];
+    impose_bc(m, L);
+    compute_residual(m);
+    memcpy(out_pad, L->tmp, level_size(L, m->dim) * sizeof(double));
+    int nx = L->n[0], ny = L->n[1], nz = L->n[2];
+    /* zero the ghost shell: only the core is a residual */
+    for (int i = -1; i <= nx; i++)
+        for (int j = (m->dim == 3 ? -1 : 0); j <= (m->dim == 3 ? ny : 0); j++)
+            for (int k = -1; k <= nz; k++)
+                if (i < 0 || i >= nx || k < 0 || k >= nz || (m->dim == 3 && (j < 0 || j >= ny)))
+                    out_pad[IX(L, i, j, k)] = 0.0;
+}

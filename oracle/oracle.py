@@ -72,6 +72,7 @@ def lib():
         L.or_mg_impose_bc.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.or_set_threads.argtypes = [C.c_int]
         L.or_mg_vcycles.argtypes = [C.c_void_p, C.c_int]
+        L.or_mg_residual.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -218,6 +219,16 @@ class Multigrid:
 
     def error(self, res_type):
         return lib().or_mg_error(self.h, res_type)
+
+    def residual(self, level, zero_bc):
+        out = np.zeros(self.padded_shape(level))
+        lib().or_mg_residual(self.h, level, int(zero_bc), _dp(out))
+        return out
+
+    def apply(self, x_pad):
+        """A x at level 0 (homogeneous walls), as a padded array."""
+        self.set_level(0, x_pad, np.zeros_like(x_pad))
+        return -self.residual(0, 1)
 
     def vcycles(self, count):
         lib().or_mg_vcycles(self.h, count)

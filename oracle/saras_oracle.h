@@ -112,6 +112,9 @@ void or_mg_prolong_correct(or_mg *m, int level);                   /* :516-560 +
 double or_mg_error(or_mg *m, int res_type);                        /* :563-663 at level 0 */
 void or_mg_impose_bc(or_mg *m, int level, int zero_bc);            /* :876-952 */
 void or_mg_vcycles(or_mg *m, int count);                           /* poisson.cc:203-279, repeated */
+/* imposeBC then the residual b - A x of computeResidual (:71-93) at a level,
+ * written into a padded array (ghosts zero). */
+void or_mg_residual(or_mg *m, int level, int zero_bc, double *out_pad);
 
 /* Threads the oracle's OpenMP loops use (0 = runtime default). */
 void or_set_threads(int n);

@@ -108,6 +108,12 @@ __device__ __forceinline__ void gather(const LevelArgs<T> &L, const T *__restric
     }
 }
 
+// Division of the SOR update: the FP64 path divides exactly as the reference
+// does; the FP32 smoothing of the mixed-precision solver, whose result is
+// corrected by the FP64 refinement anyway, uses the hardware reciprocal.
+__device__ __forceinline__ double sor_div(double a, double d) { return a / d; }
+__device__ __forceinline__ float sor_div(float a, float d) { return a * __builtin_amdgcn_rcpf(d); }
+
 // RB-SOR point update (poisson_d3.cc:109-116, poisson_d2.cc:103-108)
 template <typename T, int DIM>
 __device__ __forceinline__ T sor_update(const Stencil<T, DIM> &s, T c2x, T c1x, T c2y, T c1y, T c2z, T c1z,
@@ -125,7 +131,7 @@ __device__ __forceinline__ T sor_update(const Stencil<T, DIM> &s, T c2x, T c1x, 
         sum = sum + c1z * (s.zT - s.zB);
         den = T(2.0) * (c2x + c2z);
     }
-    return (T(1.0) - w) * s.c + ((sum - b) * w) / den;
+    return (T(1.0) - w) * s.c + sor_div((sum - b) * w, den);
 }
 
 // residual b - A x (poisson_d3.cc:81-87, poisson_d2.cc:80-84)

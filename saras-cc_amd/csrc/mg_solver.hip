@@ -14,13 +14,16 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
 #include <vector>
 
 #include "grid_host.hpp"
+#include "mg_fused.hpp"
 #include "mg_kernels.hpp"
+#include "mg_tail.hpp"
 #include "saras_hip.h"
 
 namespace {
@@ -54,6 +57,7 @@ struct DevLevel {
     long sx = 0, sy = 0;
     size_t elems = 0;  // allocation size in elements
     T *x = nullptr, *b = nullptr;
+    T *xalt = nullptr;  // second iterate buffer: fused sweeps run out of place
     T *c2[3] = {nullptr, nullptr, nullptr}, *c1[3] = {nullptr, nullptr, nullptr};
     T *rlo[3] = {nullptr, nullptr, nullptr}, *rhi[3] = {nullptr, nullptr, nullptr}, *rwd[3] = {nullptr, nullptr, nullptr};
 };
@@ -77,7 +81,7 @@ struct Hierarchy {
 
     ~Hierarchy() {
         for (auto &L : lv) {
-            (void)hipFree(L.x); (void)hipFree(L.b);
+            (void)hipFree(L.x); (void)hipFree(L.b); (void)hipFree(L.xalt);
             for (int a = 0; a < 3; ++a) {
                 (void)hipFree(L.c2[a]); (void)hipFree(L.c1[a]);
                 (void)hipFree(L.rlo[a]); (void)hipFree(L.rhi[a]); (void)hipFree(L.rwd[a]);
@@ -101,6 +105,10 @@ struct Hierarchy {
             HIP_TRY(hipMalloc((void **)&L.x, L.elems * sizeof(T)));
             HIP_TRY(hipMalloc((void **)&L.b, L.elems * sizeof(T)));
             HIP_TRY(hipMemset(L.x, 0, L.elems * sizeof(T)));
+            if (l < D && dim == 3) {
+                HIP_TRY(hipMalloc((void **)&L.xalt, L.elems * sizeof(T)));
+                HIP_TRY(hipMemset(L.xalt, 0, L.elems * sizeof(T)));
+            }
             HIP_TRY(hipMemset(L.b, 0, L.elems * sizeof(T)));
             for (int a = 0; a < 3; ++a) {
                 if (dim == 2 && a == 1) {
@@ -182,6 +190,8 @@ struct saras_mg {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int coarse_iters_max = 0;
     int coarse_capped = 0;
+    bool fused_off = false;            // SARAS_NO_FUSED_SWEEP=1 forces the two-pass smoother
+    bool tail_off = false;             // SARAS_NO_TAIL=1 runs every level with multi-block kernels
 
     ~saras_mg() {
         (void)hipFree(x64b); (void)hipFree(partial); (void)hipFree(norm_out); (void)hipFree(iters_dev);
@@ -222,11 +232,62 @@ struct Ops {
     bool inhom0;
     T w;
 
+    int pend_level = -1;  // level whose next sweep also applies the prolongation
+
     bool inhom(int l, bool zero_bc) const { return inhom0 && l == 0 && !zero_bc; }
+
+    // one RB-SOR sweep at level l: the fused out-of-place march kernel where
+    // the level tiles exactly, else the two in-place colour passes
+    template <int NP, int TJ>
+    void march(int l, const saras::LevelArgs<T> &a) {
+        DevLevel<T> &L = H.lv[l];
+        const int ntk = L.n[2] / (2 * NP), ntj = L.n[1] / TJ;
+        const int tiles2d = ntk * ntj;
+        int nchunks = (1024 + tiles2d - 1) / tiles2d;
+        nchunks = std::max(1, std::min(nchunks, L.n[0] / 8));
+        const int chunk = (L.n[0] + nchunks - 1) / nchunks;
+        nchunks = (L.n[0] + chunk - 1) / chunk;
+        const int ntiles = nchunks * tiles2d;
+        const int grid = (ntiles + saras::kXcds - 1) / saras::kXcds * saras::kXcds;
+        const T *e = nullptr;
+        long esx = 0, esy = 0;
+        if (pend_level == l) {  // prolongation folded into this first post-smoothing sweep
+            const DevLevel<T> &C = H.lv[l + 1];
+            e = C.x;
+            esx = C.sx;
+            esy = C.sy;
+            pend_level = -1;
+        }
+        if (l == 0)
+            hipLaunchKernelGGL((saras::k_sweep_march<T, NP, TJ, true>), dim3(grid), dim3(NP * TJ), 0, s, a, L.x, L.b,
+                               L.xalt, w, chunk, ntk, ntj, ntiles, e, esx, esy);
+        else
+            hipLaunchKernelGGL((saras::k_sweep_march<T, NP, TJ, false>), dim3(grid), dim3(NP * TJ), 0, s, a, L.x,
+                               L.b, L.xalt, w, chunk, ntk, ntj, ntiles, e, esx, esy);
+        std::swap(L.x, L.xalt);
+    }
+
+    bool march_ok(int l) const {
+        const DevLevel<T> &L = H.lv[l];
+        if (H.dim != 3 || !L.xalt || m->fused_off) return false;
+        for (int q = 0; q < 3; ++q)
+            if (m->g.periodic[q] && (L.n[q] & 1)) return false;
+        // the march needs many tiles and planes to pay off; smaller levels use
+        // the two colour passes
+        return L.n[1] % 16 == 0 && L.n[2] % 64 == 0 && (long)L.n[0] * L.n[1] * L.n[2] >= (8L << 20);
+    }
 
     void rb_sweep(int l, bool zero_bc) {
         DevLevel<T> &L = H.lv[l];
         auto a = level_args(m, H, l, inhom(l, zero_bc));
+        if (march_ok(l)) {
+            // FP32: 64 x 16 tiles (512 lanes); FP64: 64 x 8 tiles to keep LDS per workgroup ~50 KB
+            constexpr int TJ = sizeof(T) == 4 ? 16 : 8;
+            if (L.n[2] % 64 == 0 && L.n[1] % TJ == 0) march<32, TJ>(l, a);
+            else if (L.n[2] % 32 == 0) march<16, 8>(l, a);
+            else march<8, 8>(l, a);
+            return;
+        }
         const long n = (long)L.n[0] * L.n[1] * ((L.n[2] + 1) / 2);
         for (int colour = 0; colour < 2; ++colour) {
             if (H.dim == 3) {
@@ -269,6 +330,7 @@ struct Ops {
     }
 
     void resid_restrict(int l) {
+        if (l < 0 || l + 1 >= (int)H.lv.size()) std::abort();  // host-side guard: never launch past the hierarchy
         DevLevel<T> &F = H.lv[l], &C = H.lv[l + 1];
         auto fa = level_args(m, H, l, inhom(l, false));
         auto ca = level_args(m, H, l + 1, false);
@@ -288,18 +350,87 @@ struct Ops {
     }
 
     void prolong_correct(int l) {
+        if (l < 0 || l + 1 >= (int)H.lv.size()) std::abort();  // host-side guard: never launch past the hierarchy
         DevLevel<T> &F = H.lv[l], &C = H.lv[l + 1];
         const long n = (long)F.n[0] * F.n[1] * F.n[2];
         hipLaunchKernelGGL((saras::k_prolong_correct<T>), dim3(blocks_for(n)), dim3(kThreads), 0, s, F.n[0], F.n[1],
                            F.n[2], F.sx, F.sy, F.x, C.x, C.sx, C.sy);
     }
 
-    // poisson::vCycle (poisson.cc:203-279)
+    // first level of the single-workgroup tail: the smallest S whose levels
+    // S..D fit in 48 KB of LDS
+    int tail_start() const {
+        const int D = (int)H.lv.size() - 1;
+        size_t bytes = 0;
+        int S = D;
+        for (int l = D; l >= 0; --l) {
+            const DevLevel<T> &L = H.lv[l];
+            const size_t core = (size_t)L.n[0] * L.n[1] * L.n[2];
+            const size_t pad = H.dim == 3 ? (size_t)(L.n[0] + 2) * (L.n[1] + 2) * (L.n[2] + 2)
+                                          : (size_t)(L.n[0] + 2) * (L.n[2] + 2);
+            const size_t coef = 2 * ((size_t)L.n[0] + L.n[1] + L.n[2]);
+            bytes += ((pad + 3) / 4 * 4 + (core + 3) / 4 * 4 + coef) * sizeof(T);
+            if (bytes > 46 * 1024 || D - l + 1 > saras::kTailMaxLevels) break;
+            S = l;
+        }
+        return S;
+    }
+
+    void tail(int S) {
+        const int D = (int)H.lv.size() - 1;
+        saras::TailArgs<T> A{};
+        A.first = S;
+        A.last = D;
+        size_t off = 0;
+        for (int l = S; l <= D; ++l) {
+            const DevLevel<T> &L = H.lv[l];
+            A.L[l - S] = level_args(m, H, l, inhom(l, false));
+            for (int q = 0; q < 3; ++q) {
+                A.R[l - S].lo[q] = L.rlo[q];
+                A.R[l - S].hi[q] = L.rhi[q];
+                A.R[l - S].wd[q] = L.rwd[q];
+            }
+            const size_t core = (size_t)L.n[0] * L.n[1] * L.n[2];
+            const size_t pad = H.dim == 3 ? (size_t)(L.n[0] + 2) * (L.n[1] + 2) * (L.n[2] + 2)
+                                          : (size_t)(L.n[0] + 2) * (L.n[2] + 2);
+            A.xoff[l - S] = (int)off;
+            off += (pad + 3) / 4 * 4;
+            A.boff[l - S] = (int)off;
+            off += (core + 3) / 4 * 4;
+        }
+        A.coff = (int)off;
+        for (int l = S; l <= D; ++l) {
+            const DevLevel<T> &L = H.lv[l];
+            off += 2 * ((size_t)L.n[0] + (H.dim == 3 ? L.n[1] : 1) + L.n[2]);
+        }
+        A.x = H.lv[S].x;
+        A.b = H.lv[S].b;
+        A.pre = m->p.pre_smooth;
+        A.post = m->p.post_smooth;
+        A.solve_flag = m->p.solve_flag;
+        A.max_count = m->max_count;
+        A.w = w;
+        A.tol = m->p.mg_tol;
+        A.zero_bc_first = S != 0;
+        A.iters_out = m->iters_dev;
+        if (H.dim == 3)
+            hipLaunchKernelGGL((saras::k_tail<T, 3>), dim3(1), dim3(512), off * sizeof(T), s, A);
+        else
+            hipLaunchKernelGGL((saras::k_tail<T, 2>), dim3(1), dim3(512), off * sizeof(T), s, A);
+    }
+
+    // poisson::vCycle (poisson.cc:203-279); levels S..D run in the tail kernel
     void vcycle() {
         const int D = (int)H.lv.size() - 1;
+        const int S = m->tail_off ? D + 1 : tail_start();
+        if (S == 0) {
+            tail(0);
+            return;
+        }
         smooth(0, m->p.pre_smooth, false);
         for (int l = 0; l < D; ++l) {
             resid_restrict(l);
+            if (l + 1 == S) break;
             if (l + 1 == D) {
                 if (m->p.solve_flag) coarse(D, true, 0, 0);
                 else smooth(D, m->p.pre_smooth, true);
@@ -307,8 +438,10 @@ struct Ops {
                 smooth(l + 1, m->p.pre_smooth, true);
             }
         }
-        for (int l = D - 1; l >= 0; --l) {
-            prolong_correct(l);
+        if (S <= D) tail(S);
+        for (int l = std::min(S, D) - 1; l >= 0; --l) {
+            if (m->p.post_smooth > 0 && march_ok(l)) pend_level = l;
+            else prolong_correct(l);
             smooth(l, m->p.post_smooth, l != 0);
         }
     }
@@ -510,6 +643,12 @@ saras_status saras_mg_create(const saras_grid_desc *grid, const saras_mg_params 
     std::unique_ptr<saras_mg> m(new saras_mg());
     m->g = g;
     m->p = p;
+    {
+        const char *env = std::getenv("SARAS_NO_FUSED_SWEEP");
+        m->fused_off = env && env[0] == '1';
+        const char *env2 = std::getenv("SARAS_NO_TAIL");
+        m->tail_off = env2 && env2[0] == '1';
+    }
     if (m->p.inner_cycles <= 0) m->p.inner_cycles = 3;
     if (m->p.max_refine <= 0) m->p.max_refine = 20;
     if (!saras::build_levels(g, m->G)) {
@@ -586,6 +725,89 @@ static saras_status solve_fp64(saras_mg *m, hipStream_t s, saras_mg_stats *st) {
     return SARAS_OK;
 }
 
+}  // extern "C"
+
+template <int DIM>
+void launch_refine(int type, saras_mg *m, const saras::LevelArgs<double> &a, const float *e, long esx, long esy,
+                   float *r32, hipStream_t s) {
+    auto &L0 = m->h64.lv[0];
+    switch (type) {
+        case 0:
+            hipLaunchKernelGGL((saras::k_refine<0, DIM>), dim3(saras::kNormBlocks), dim3(saras::kNormThreads), 0, s, a,
+                               L0.x, e, esx, esy, L0.b, m->x64b, r32, esx, esy, m->partial);
+            hipLaunchKernelGGL((saras::k_norm_final<0>), dim3(1), dim3(saras::kNormThreads), 0, s, m->partial,
+                               saras::kNormBlocks, m->norm_out);
+            break;
+        case 1:
+            hipLaunchKernelGGL((saras::k_refine<1, DIM>), dim3(saras::kNormBlocks), dim3(saras::kNormThreads), 0, s, a,
+                               L0.x, e, esx, esy, L0.b, m->x64b, r32, esx, esy, m->partial);
+            hipLaunchKernelGGL((saras::k_norm_final<1>), dim3(1), dim3(saras::kNormThreads), 0, s, m->partial,
+                               saras::kNormBlocks, m->norm_out);
+            break;
+        default:
+            hipLaunchKernelGGL((saras::k_refine<2, DIM>), dim3(saras::kNormBlocks), dim3(saras::kNormThreads), 0, s, a,
+                               L0.x, e, esx, esy, L0.b, m->x64b, r32, esx, esy, m->partial);
+            hipLaunchKernelGGL((saras::k_norm_final<2>), dim3(1), dim3(saras::kNormThreads), 0, s, m->partial,
+                               saras::kNormBlocks, m->norm_out);
+            break;
+    }
+}
+
+// x64 <- x64 + e32 (if with_e), r32 <- b - A x64, residual norm of type res_type
+static saras_status refine_step(saras_mg *m, hipStream_t s, bool with_e, double *res) {
+    auto &L0 = m->h64.lv[0];
+    auto &F0 = m->h32->lv[0];
+    auto a = level_args(m, m->h64, 0, true);
+    const float *e = with_e ? F0.x : nullptr;
+    if (m->G.dim == 3) launch_refine<3>(m->p.res_type, m, a, e, F0.sx, F0.sy, F0.b, s);
+    else launch_refine<2>(m->p.res_type, m, a, e, F0.sx, F0.sy, F0.b, s);
+    std::swap(L0.x, m->x64b);
+    HIP_TRY(hipMemcpyAsync(m->norm_host, m->norm_out, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    const double num = m->norm_host[0], den = m->norm_host[1];
+    const int pointCount = L0.n[0] * L0.n[1] * L0.n[2];
+    const int type = m->p.res_type;
+    if (type == 0) *res = den != 0.0 ? num / den : num;
+    else if (type == 1) *res = den != 0.0 ? num / den : num / pointCount;
+    else *res = den != 0.0 ? std::sqrt(num / pointCount) / std::sqrt(den / pointCount) : std::sqrt(num / pointCount);
+    return SARAS_OK;
+}
+
+// Mixed precision: FP64 iterate, residual and stopping test; FP32 V-cycles
+// (same smoother, transfers and coarse solve as the reference, homogeneous
+// walls) on the correction equation A e = r.
+static saras_status solve_mixed(saras_mg *m, hipStream_t s, saras_mg_stats *st) {
+    Ops<float> ops{m, *m->h32, s, false, (float)m->p.sor};
+    auto &F0 = m->h32->lv[0];
+    double res = 0.0;
+    saras_status e = refine_step(m, s, false, &res);
+    if (e) return e;
+    int cycles = 0, refs = 0;
+    const int cap = m->p.vc_count > 0 ? m->p.vc_count : (1 << 30);
+    while (!(res < m->p.vc_tol) && refs < m->p.max_refine && cycles < cap) {
+        HIP_TRY(hipMemsetAsync(F0.x, 0, F0.elems * sizeof(float), s));
+        for (int c = 0; c < m->p.inner_cycles && cycles < cap; ++c) {
+            ops.vcycle();
+            ++cycles;
+        }
+        e = refine_step(m, s, true, &res);
+        if (e) return e;
+        ++refs;
+        if (!std::isfinite(res)) break;
+    }
+    if (m->p.solve_flag) {
+        e = fetch_coarse_iters(m, s);
+        if (e) return e;
+    }
+    st->cycles = cycles;
+    st->refinements = refs;
+    st->residual = res;
+    st->converged = res < m->p.vc_tol;
+    return SARAS_OK;
+}
+
+extern "C" {
+
 saras_status saras_mg_solve(saras_mg *mg, double *x_full, const double *b_full, int ptr_kind, void *stream,
                             saras_mg_stats *stats) {
     if (!mg || !x_full || !b_full) {
@@ -623,8 +845,7 @@ saras_status saras_mg_solve(saras_mg *mg, double *x_full, const double *b_full, 
     if (m->p.precision == SARAS_PREC_FP64) {
         e = solve_fp64(m, s, &st);
     } else {
-        e = SARAS_BAD_ARG;
-        set_error("mixed precision not built yet");
+        e = solve_mixed(m, s, &st);
     }
     if (e) return e;
     store_full(m, dx, L0.x, s);

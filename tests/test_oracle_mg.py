@@ -309,3 +309,13 @@ def test_strong_stretch_singular_diverges():
         out[bc] = st["history"]
     assert out[O.BC_NEUMANN][-1] > 10 * out[O.BC_NEUMANN][0]
     assert out[O.BC_DIRICHLET][-1] < 1e-3 * out[O.BC_DIRICHLET][0]
+
+
+def test_residual_entry_matches_dense_operator():
+    g, p = make_case((8, 8, 8), 3, (1, 1, 0), gtype=(0, 0, 2), beta=(1, 1, 0.6), bc=O.BC_NEUMANN)
+    mg = O.Multigrid(g, p)
+    A, shape, _, _ = dense_level(g, 0, O.BC_NEUMANN)
+    x = np.random.default_rng(8).standard_normal(shape)
+    ax = mg.apply(pad_core(x))
+    np.testing.assert_allclose(ax[1:-1, 1:-1, 1:-1].ravel(), A @ x.ravel(), rtol=1e-12, atol=1e-10)
+    assert np.all(ax[0] == 0) and np.all(ax[:, :, -1] == 0)

@@ -192,6 +192,8 @@ struct saras_mg {
     int coarse_capped = 0;
     bool fused_off = false;            // SARAS_NO_FUSED_SWEEP=1 forces the two-pass smoother
     bool tail_off = false;             // SARAS_NO_TAIL=1 runs every level with multi-block kernels
+    int march_wgs = 0;                 // target workgroups of a march sweep (SARAS_MARCH_WGS; 0 = per type)
+    bool march_tj8 = false;            // FP32 march with 64 x 8 tiles (SARAS_MARCH_TJ8=1)
 
     ~saras_mg() {
         (void)hipFree(x64b); (void)hipFree(partial); (void)hipFree(norm_out); (void)hipFree(iters_dev);
@@ -243,7 +245,8 @@ struct Ops {
         DevLevel<T> &L = H.lv[l];
         const int ntk = L.n[2] / (2 * NP), ntj = L.n[1] / TJ;
         const int tiles2d = ntk * ntj;
-        int nchunks = (1024 + tiles2d - 1) / tiles2d;
+        const int target = m->march_wgs > 0 ? m->march_wgs : (sizeof(T) == 4 ? 512 : 2048);
+        int nchunks = (target + tiles2d - 1) / tiles2d;
         nchunks = std::max(1, std::min(nchunks, L.n[0] / 8));
         const int chunk = (L.n[0] + nchunks - 1) / nchunks;
         nchunks = (L.n[0] + chunk - 1) / chunk;
@@ -283,7 +286,8 @@ struct Ops {
         if (march_ok(l)) {
             // FP32: 64 x 16 tiles (512 lanes); FP64: 64 x 8 tiles to keep LDS per workgroup ~50 KB
             constexpr int TJ = sizeof(T) == 4 ? 16 : 8;
-            if (L.n[2] % 64 == 0 && L.n[1] % TJ == 0) march<32, TJ>(l, a);
+            if (sizeof(T) == 4 && m->march_tj8) march<32, 8>(l, a);
+            else if (L.n[2] % 64 == 0 && L.n[1] % TJ == 0) march<32, TJ>(l, a);
             else if (L.n[2] % 32 == 0) march<16, 8>(l, a);
             else march<8, 8>(l, a);
             return;
@@ -648,6 +652,10 @@ saras_status saras_mg_create(const saras_grid_desc *grid, const saras_mg_params 
         m->fused_off = env && env[0] == '1';
         const char *env2 = std::getenv("SARAS_NO_TAIL");
         m->tail_off = env2 && env2[0] == '1';
+        const char *env3 = std::getenv("SARAS_MARCH_WGS");
+        if (env3 && std::atoi(env3) > 0) m->march_wgs = std::atoi(env3);
+        const char *env4 = std::getenv("SARAS_MARCH_TJ8");
+        m->march_tj8 = env4 && env4[0] == '1';
     }
     if (m->p.inner_cycles <= 0) m->p.inner_cycles = 3;
     if (m->p.max_refine <= 0) m->p.max_refine = 20;

@@ -1,0 +1,636 @@
+/*
+ * nse_oracle.c -- restatement of the SARAS-CC Navier-Stokes/Boussinesq time
+ * step for a single domain: lib/field (derivative.cc, vfield.cc, sfield.cc,
+ * plainvf/plainsf algebra), lib/boundary (dirichlet.cc, neumann.cc),
+ * lib/parallel/mpidata.cc face sync (periodic self-wrap), lib/timestep
+ * (timestep.cc, predictors.cc, lsRK3_d3.cc, eulerCN_d3.cc), lib/force
+ * (buoyantForce.cc) and the lib/io/tseries.cc diagnostics.
+ *
+ * TEST INFRASTRUCTURE ONLY (see saras_oracle.h).
+ *
+ * Fields are full arrays with pad p: (nx+2p)(ny+2p)(nz+2p), z fastest.
+ * Blitz++'s normalised stencils central12n/22n/14n/24n (derivative.cc:106-
+ * 306) come from an un-vendored library (Blitz++, version unpinned,
+ * CMakeLists.txt:83) that is absent here; they are restated from its
+ * published stencil operator definitions:
+ *   c12n = (F+1 - F-1) * (1/2)          c22n = (F-1 - 2 F) + F+1
+ *   c14n = (8 (F+1 - F-1) - (F+2 - F-2)) * (1/12)
+ *   c24n = ((-30 F + 16 (F+1 + F-1)) - (F+2 + F-2)) * (1/12)
+ * Their rounding order against a real Blitz++ build is unpinned; the
+ * formulas are pinned by polynomial exactness and convergence order in the
+ * tests, and the second-order edge overrides are the reference's own
+ * expressions (derivative.cc:111-112, 215-223).
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+#include "saras_oracle.h"
+
+struct or_nse {
+    or_nse_desc d;
+    int n[3], p;
+    long SX, SY;
+    size_t full;
+    double *x[3], *xi_x[3], *xixx[3], *xix2[3];   /* field metrics over [-p, n+p-1], +p offset */
+    double ih[3], ih2[3], i2d[3];                  /* derivative.cc:52-54, vfield.cc i2dx */
+    double tsi2h[3], tsih2[3];                     /* timestep::setCoefficients */
+    double hwall[6];                               /* neumann.cc:59-75 */
+    double *V[3], *P, *T, *Pp;
+    double *nse[3], *trhs, *tv[3], *ts, *pg[3], *mgr, *iter, *tmpA, *tmpB;
+    or_mg *mg;
+    int max_iter;
+    double time;
+    int jacobi_iters[4];
+    or_mg_stats mg_stats;
+};
+
+#define FX(S, i, j, k) ((long)((i) + (S)->p) * (S)->SX + (long)((j) + (S)->p) * (S)->SY + (long)((k) + (S)->p))
+
+static double *zalloc(size_t n) { return calloc(n, sizeof(double)); }
+
+or_nse *or_nse_create(const or_nse_desc *d)
+{
+    or_nse *s = calloc(1, sizeof(or_nse));
+    s->d = *d;
+    s->p = d->g.pad;
+    for (int a = 0; a < 3; a++) s->n[a] = d->g.n[a];
+    int p = s->p;
+    s->SY = s->n[2] + 2 * p;
+    s->SX = (long)(s->n[1] + 2 * p) * s->SY;
+    s->full = (size_t)(s->n[0] + 2 * p) * s->SX;
+    for (int a = 0; a < 3; a++) {
+        int m = s->n[a] + 2 * p;
+        s->x[a] = zalloc(m); s->xi_x[a] = zalloc(m); s->xixx[a] = zalloc(m); s->xix2[a] = zalloc(m);
+        or_field_metrics(&d->g, a, s->x[a], s->xi_x[a], s->xixx[a], s->xix2[a]);
+        double dxi = 1.0 / (double)s->n[a];
+        s->ih[a] = 1.0 / dxi;
+        s->ih2[a] = pow(s->ih[a], 2.0);
+        s->i2d[a] = 1.0 / (2.0 * dxi);
+        double h2 = pow(dxi, 2.0);
+        s->tsi2h[a] = 0.5 / dxi;
+        s->tsih2[a] = 1.0 / h2;
+        double L = d->g.len[a];
+        s->hwall[2 * a] = 2 * s->x[a][0 + p];
+        s->hwall[2 * a + 1] = 2 * (s->x[a][s->n[a] - 1 + p] - L);
+    }
+    for (int c = 0; c < 3; c++) {
+        s->V[c] = zalloc(s->full); s->nse[c] = zalloc(s->full); s->tv[c] = zalloc(s->full); s->pg[c] = zalloc(s->full);
+    }
+    s->P = zalloc(s->full); s->T = zalloc(s->full); s->Pp = zalloc(s->full);
+    s->trhs = zalloc(s->full); s->ts = zalloc(s->full); s->mgr = zalloc(s->full);
+    s->iter = zalloc(s->full); s->tmpA = zalloc(s->full); s->tmpB = zalloc(s->full);
+    or_mg_params mp = d->mg;
+    mp.bc = OR_BC_NEUMANN;   /* allNeumann (poisson.cc:87) */
+    s->mg = or_mg_create(&d->g, &mp);
+    s->max_iter = (int)pow(log((double)s->n[0] * s->n[1] * s->n[2]), 3);  /* timestep.cc:115 */
+    return s;
+}
+
+void or_nse_destroy(or_nse *s)
+{
+    if (!s) return;
+    for (int a = 0; a < 3; a++) { free(s->x[a]); free(s->xi_x[a]); free(s->xixx[a]); free(s->xix2[a]); }
+    for (int c = 0; c < 3; c++) { free(s->V[c]); free(s->nse[c]); free(s->tv[c]); free(s->pg[c]); }
+    free(s->P); free(s->T); free(s->Pp); free(s->trhs); free(s->ts); free(s->mgr);
+    free(s->iter); free(s->tmpA); free(s->tmpB);
+    or_mg_destroy(s->mg);
+    free(s);
+}
+
+int or_nse_max_iterations(const or_nse *s) { return s->max_iter; }
+
+/* ---- boundary conditions ---------------------------------------------- */
+
+/* mpidata::syncFaces on one rank: periodic pad-thick slabs over the full
+ * extent of the other axes, in the order x, y, z (mpidata.cc:371-382). */
+static void sync_faces(or_nse *s, double *F)
+{
+    int p = s->p, nx = s->n[0], ny = s->n[1], nz = s->n[2];
+    if (s->d.g.periodic[0])
+        for (int q = 0; q < p; q++)
+            for (int j = -p; j < ny + p; j++)
+                for (int k = -p; k < nz + p; k++) {
+                    F[FX(s, -p + q, j, k)] = F[FX(s, nx - p + q, j, k)];
+                    F[FX(s, nx + q, j, k)] = F[FX(s, q, j, k)];
+                }
+    if (s->d.g.periodic[1])
+        for (int i = -p; i < nx + p; i++)
+            for (int q = 0; q < p; q++)
+                for (int k = -p; k < nz + p; k++) {
+                    F[FX(s, i, -p + q, k)] = F[FX(s, i, ny - p + q, k)];
+                    F[FX(s, i, ny + q, k)] = F[FX(s, i, q, k)];
+                }
+    if (s->d.g.periodic[2])
+        for (int i = -p; i < nx + p; i++)
+            for (int j = -p; j < ny + p; j++)
+                for (int q = 0; q < p; q++) {
+                    F[FX(s, i, j, -p + q)] = F[FX(s, i, j, nz - p + q)];
+                    F[FX(s, i, j, nz + q)] = F[FX(s, i, j, q)];
+                }
+}
+
+/* one wall (first ghost layer over the full extent of the other axes):
+ * dirichlet.cc:72-76, neumann.cc:89-93 */
+static void wall_bc(or_nse *s, double *F, int wall, int type, double val)
+{
+    int p = s->p, n[3] = {s->n[0], s->n[1], s->n[2]};
+    int ax = wall / 2, hi = wall & 1;
+    int g = hi ? n[ax] : -1, src = hi ? n[ax] - 1 : 0;
+    double kh = val * s->hwall[wall];
+    int lo[3] = {-p, -p, -p}, up[3] = {n[0] + p - 1, n[1] + p - 1, n[2] + p - 1};
+    lo[ax] = up[ax] = 0;
+    for (int a = lo[0]; a <= up[0]; a++)
+        for (int b = lo[1]; b <= up[1]; b++)
+            for (int c = lo[2]; c <= up[2]; c++) {
+                int gi[3] = {a, b, c}, si[3] = {a, b, c};
+                gi[ax] = g; si[ax] = src;
+                double inner = F[FX(s, si[0], si[1], si[2])];
+                F[FX(s, gi[0], gi[1], gi[2])] = type == OR_FBC_DIRICHLET ? 2.0 * val - inner : inner - kh;
+            }
+}
+
+/* sfield::imposeBCs / vfield::imposeV?BC: sync, then walls x (Lft, Rgt),
+ * y (Frn, Bak), z (Top, Bot) -- vfield.cc:751-847, sfield.cc:520-537 */
+static void impose(or_nse *s, double *F, const int *type, const double *val)
+{
+    sync_faces(s, F);
+    for (int ax = 0; ax < 3; ax++) {
+        if (s->d.g.periodic[ax]) continue;
+        if (ax < 2) {
+            wall_bc(s, F, 2 * ax, type[2 * ax], val[2 * ax]);
+            wall_bc(s, F, 2 * ax + 1, type[2 * ax + 1], val[2 * ax + 1]);
+        } else {
+            wall_bc(s, F, 5, type[5], val[5]);
+            wall_bc(s, F, 4, type[4], val[4]);
+        }
+    }
+}
+
+static void impose_v(or_nse *s, int c) { impose(s, s->V[c], s->d.vbc_type[c], s->d.vbc_val[c]); }
+static void impose_t(or_nse *s) { impose(s, s->T, s->d.tbc_type, s->d.tbc_val); }
+static void impose_p(or_nse *s) { impose(s, s->P, s->d.pbc_type, s->d.pbc_val); }
+
+/* ---- derivatives (derivative.cc:104-306), core cells ------------------- */
+
+static inline double at(const or_nse *s, const double *F, int ax, int i, int j, int k, int off)
+{
+    int c[3] = {i, j, k};
+    c[ax] += off;
+    return F[FX(s, c[0], c[1], c[2])];
+}
+
+static inline int idx_ax(int ax, int i, int j, int k) { return ax == 0 ? i : (ax == 1 ? j : k); }
+
+static const double recip_12 = 1.0 / 12.0;
+
+static inline double c12n(double fm, double fp) { return (fp - fm) * 0.5; }
+static inline double c14n(double fm2, double fm, double fp, double fp2)
+{
+    return (8.0 * (fp - fm) - (fp2 - fm2)) * recip_12;
+}
+static inline double c22n(double fm, double f, double fp) { return (fm - 2.0 * f) + fp; }
+static inline double c24n(double fm2, double fm, double f, double fp, double fp2)
+{
+    return ((-30.0 * f + 16.0 * (fp + fm)) - (fp2 + fm2)) * recip_12;
+}
+
+/* first derivative along ax at core cell (i,j,k) */
+static double d1(const or_nse *s, const double *F, int ax, int i, int j, int k)
+{
+    int q = idx_ax(ax, i, j, k), n = s->n[ax];
+    double fm = at(s, F, ax, i, j, k, -1), fp = at(s, F, ax, i, j, k, 1), v;
+    if (s->d.dscheme == 1) v = c12n(fm, fp);
+    else if (q == 0 || q == n - 1) v = 0.5 * (fp - fm);
+    else v = c14n(at(s, F, ax, i, j, k, -2), fm, fp, at(s, F, ax, i, j, k, 2));
+    v *= s->ih[ax];
+    return s->xi_x[ax][q + s->p] * v;
+}
+
+/* second derivative along ax: xixx * D1 + xix2 * D2 */
+static double d2(const or_nse *s, const double *F, int ax, int i, int j, int k)
+{
+    int q = idx_ax(ax, i, j, k), n = s->n[ax];
+    double fm = at(s, F, ax, i, j, k, -1), f = F[FX(s, i, j, k)], fp = at(s, F, ax, i, j, k, 1), t, o;
+    if (s->d.dscheme == 1) {
+        t = c12n(fm, fp);
+        o = c22n(fm, f, fp);
+    } else if (q == 0 || q == n - 1) {
+        t = 0.5 * (fp - fm);
+        o = fp - 2.0 * f + fm;
+    } else {
+        double fm2 = at(s, F, ax, i, j, k, -2), fp2 = at(s, F, ax, i, j, k, 2);
+        t = c14n(fm2, fm, fp, fp2);
+        o = c24n(fm2, fm, f, fp, fp2);
+    }
+    t *= s->ih[ax];
+    o *= s->ih2[ax];
+    return s->xixx[ax][q + s->p] * t + s->xix2[ax][q + s->p] * o;
+}
+
+/* Morinishi difference of one product array: 4th order interior, 2nd order at
+ * the first/last cell; z uses the reference's core.ubound(1) edge test
+ * (vfield.cc:329,434, sfield.cc:236,295). */
+static inline int edge_cell(const or_nse *s, int ax, int q)
+{
+    int last = ax == 2 ? s->n[1] - 1 : s->n[ax] - 1;
+    return q == 0 || q == last;
+}
+
+static double mor_adv(const or_nse *s, double u, const double *F, int ax, int i, int j, int k)
+{
+    int q = idx_ax(ax, i, j, k);
+    double xm = s->xi_x[ax][q + s->p];
+    if (s->d.dscheme == 1 || edge_cell(s, ax, q))
+        return u * xm * (at(s, F, ax, i, j, k, 1) - at(s, F, ax, i, j, k, -1)) * s->i2d[ax];
+    return u * xm * (-at(s, F, ax, i, j, k, 2) + 8.0 * at(s, F, ax, i, j, k, 1) - 8.0 * at(s, F, ax, i, j, k, -1) +
+                     at(s, F, ax, i, j, k, -2)) * s->i2d[ax] / 6.0;
+}
+
+static double mor_div(const or_nse *s, const double *G, int ax, int i, int j, int k)
+{
+    int q = idx_ax(ax, i, j, k);
+    double xm = s->xi_x[ax][q + s->p];
+    if (s->d.dscheme == 1 || edge_cell(s, ax, q))
+        return xm * (at(s, G, ax, i, j, k, 1) - at(s, G, ax, i, j, k, -1)) * s->i2d[ax];
+    return xm * (-at(s, G, ax, i, j, k, 2) + 8.0 * at(s, G, ax, i, j, k, 1) - 8.0 * at(s, G, ax, i, j, k, -1) +
+                 at(s, G, ax, i, j, k, -2)) * s->i2d[ax] / 6.0;
+}
+
+#define CORE_LOOP(s)                                  \
+    _Pragma("omp parallel for schedule(static)")      \
+    for (int i = 0; i < (s)->n[0]; i++)               \
+        for (int j = 0; j < (s)->n[1]; j++)           \
+            for (int k = 0; k < (s)->n[2]; k++)
+
+/* vfield::computeDiff / sfield::computeDiff: H(core) += d2xx; += d2yy; += d2zz */
+static void compute_diff(or_nse *s, const double *F, double *H)
+{
+    CORE_LOOP(s) {
+        long c = FX(s, i, j, k);
+        H[c] += d2(s, F, 0, i, j, k);
+        H[c] += d2(s, F, 1, i, j, k);
+        H[c] += d2(s, F, 2, i, j, k);
+    }
+}
+
+/* vfield::computeNLin (nlScheme 1 central, 3 Morinishi), vfield.cc:205-462 */
+static void nlin_v(or_nse *s, double *H[3])
+{
+    double **V = s->V;
+    if (s->d.nlscheme == 1) {
+        for (int c = 0; c < 3; c++)
+            CORE_LOOP(s) {
+                long q = FX(s, i, j, k);
+                H[c][q] -= V[0][q] * d1(s, V[c], 0, i, j, k);
+                H[c][q] -= V[1][q] * d1(s, V[c], 1, i, j, k);
+                H[c][q] -= V[2][q] * d1(s, V[c], 2, i, j, k);
+            }
+        return;
+    }
+    /* products over the full arrays: tempXX = Vx Vx, tempZX = Vz Vx, tempXY = Vx Vy,
+     * tempYZ = Vy Vz, tempYY = Vy Vy, tempZZ = Vz Vz (vfield.cc:276-283) */
+    double *pr[3][3];
+    double *buf = malloc(sizeof(double) * s->full * 6);
+    double *XX = buf, *ZZ = buf + s->full, *ZX = buf + 2 * s->full, *XY = buf + 3 * s->full,
+           *YZ = buf + 4 * s->full, *YY = buf + 5 * s->full;
+    for (size_t q = 0; q < s->full; q++) {
+        XX[q] = V[0][q] * V[0][q];
+        ZZ[q] = V[2][q] * V[2][q];
+        ZX[q] = V[2][q] * V[0][q];
+        XY[q] = V[0][q] * V[1][q];
+        YZ[q] = V[1][q] * V[2][q];
+        YY[q] = V[1][q] * V[1][q];
+    }
+    /* divergence product of (advecting axis a, component c) */
+    pr[0][0] = XX; pr[0][1] = XY; pr[0][2] = ZX;
+    pr[1][0] = XY; pr[1][1] = YY; pr[1][2] = YZ;
+    pr[2][0] = ZX; pr[2][1] = YZ; pr[2][2] = ZZ;
+    CORE_LOOP(s) {
+        long q = FX(s, i, j, k);
+        for (int a = 0; a < 3; a++) {
+            double u = V[a][q];
+            for (int c = 0; c < 3; c++) {
+                double adv = mor_adv(s, u, V[c], a, i, j, k);
+                double dv = mor_div(s, pr[a][c], a, i, j, k);
+                H[c][q] -= (adv + dv) / 2;
+            }
+        }
+    }
+    free(buf);
+}
+
+/* sfield::computeNLin, sfield.cc:160-311 */
+static void nlin_s(or_nse *s, const double *F, double *H)
+{
+    double **V = s->V;
+    if (s->d.nlscheme == 1) {
+        CORE_LOOP(s) {
+            long q = FX(s, i, j, k);
+            H[q] -= V[0][q] * d1(s, F, 0, i, j, k);
+            H[q] -= V[1][q] * d1(s, F, 1, i, j, k);
+            H[q] -= V[2][q] * d1(s, F, 2, i, j, k);
+        }
+        return;
+    }
+    double *buf = malloc(sizeof(double) * s->full * 3);
+    double *TX = buf, *TY = buf + s->full, *TZ = buf + 2 * s->full;
+    for (size_t q = 0; q < s->full; q++) {
+        TX[q] = V[0][q] * F[q];
+        TY[q] = V[1][q] * F[q];
+        TZ[q] = V[2][q] * F[q];
+    }
+    double *pr[3] = {TX, TY, TZ};
+    CORE_LOOP(s) {
+        long q = FX(s, i, j, k);
+        for (int a = 0; a < 3; a++) {
+            double adv = mor_adv(s, V[a][q], F, a, i, j, k);
+            double dv = mor_div(s, pr[a], a, i, j, k);
+            H[q] -= (adv + dv) / 2;
+        }
+    }
+    free(buf);
+}
+
+/* ---- Jacobi Crank-Nicolson (predictors.cc:59-423) ---------------------- */
+static int jacobi(or_nse *s, double *F, const double *rhs, double beta, double diff,
+                  const int *type, const double *val)
+{
+    double dt = s->d.dt;
+    int it = 0;
+    for (;;) {
+        CORE_LOOP(s) {
+            double E = F[FX(s, i + 1, j, k)], W = F[FX(s, i - 1, j, k)];
+            double N = F[FX(s, i, j + 1, k)], Sv = F[FX(s, i, j - 1, k)];
+            double Tp = F[FX(s, i, j, k + 1)], B = F[FX(s, i, j, k - 1)];
+            double x2 = s->xix2[0][i + s->p], xx = s->xixx[0][i + s->p];
+            double y2 = s->xix2[1][j + s->p], yy = s->xixx[1][j + s->p];
+            double z2 = s->xix2[2][k + s->p], zz = s->xixx[2][k + s->p];
+            s->iter[FX(s, i, j, k)] =
+                ((s->tsih2[0] * x2 * (E + W) + s->tsi2h[0] * xx * (E - W) + s->tsih2[1] * y2 * (N + Sv) +
+                  s->tsi2h[1] * yy * (N - Sv) + s->tsih2[2] * z2 * (Tp + B) + s->tsi2h[2] * zz * (Tp - B)) *
+                     dt * diff * beta +
+                 rhs[FX(s, i, j, k)]) /
+                (1.0 + 2.0 * dt * diff * beta * (s->tsih2[0] * x2 + s->tsih2[1] * y2 + s->tsih2[2] * z2));
+        }
+        /* the reference copies the whole of iterTemp (predictors.cc:96), whose
+         * pads are never written; every pad cell a stencil reads is rebuilt by
+         * the BCs that follow, so copying the core is equivalent */
+        CORE_LOOP(s) F[FX(s, i, j, k)] = s->iter[FX(s, i, j, k)];
+        impose(s, F, type, val);
+        double mx = 0.0;
+        for (int i = 0; i < s->n[0]; i++)
+            for (int j = 0; j < s->n[1]; j++)
+                for (int k = 0; k < s->n[2]; k++) {
+                    double C = F[FX(s, i, j, k)];
+                    double E = F[FX(s, i + 1, j, k)], W = F[FX(s, i - 1, j, k)];
+                    double N = F[FX(s, i, j + 1, k)], Sv = F[FX(s, i, j - 1, k)];
+                    double Tp = F[FX(s, i, j, k + 1)], B = F[FX(s, i, j, k - 1)];
+                    double x2 = s->xix2[0][i + s->p], xx = s->xixx[0][i + s->p];
+                    double y2 = s->xix2[1][j + s->p], yy = s->xixx[1][j + s->p];
+                    double z2 = s->xix2[2][k + s->p], zz = s->xixx[2][k + s->p];
+                    double r = C - beta * dt * diff *
+                                       (x2 * (E - 2.0 * C + W) * s->tsih2[0] + xx * (E - W) * s->tsi2h[0] +
+                                        y2 * (N - 2.0 * C + Sv) * s->tsih2[1] + yy * (N - Sv) * s->tsi2h[1] +
+                                        z2 * (Tp - 2.0 * C + B) * s->tsih2[2] + zz * (Tp - B) * s->tsi2h[2]);
+                    double a = fabs(r - rhs[FX(s, i, j, k)]);
+                    if (a > mx) mx = a;
+                }
+        if (mx < s->d.cn_tol) break;
+        it += 1;
+        if (it > s->max_iter) return -1;
+    }
+    return it;
+}
+
+/* ---- the RK3 step (lsRK3_d3.cc:190-305, with or without T) ------------- */
+static const double alph[3] = {4.0 / 15.0, 1.0 / 15.0, 1.0 / 6.0};
+static const double gamm[3] = {8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};
+static const double zeta[3] = {0.0, -17.0 / 60.0, -5.0 / 12.0};
+
+static void fill(double *a, size_t n, double v) { for (size_t q = 0; q < n; q++) a[q] = v; }
+
+/* plainvf::multAdd (plainvf.cc:89-95): full arrays */
+static void mult_add(double *y, const double *x, double k, size_t n) { for (size_t q = 0; q < n; q++) y[q] += k * x[q]; }
+
+int or_nse_rhs(or_nse *s, int stage)
+{
+    size_t n = s->full;
+    double a = alph[stage], b = alph[stage];
+    int hasT = s->d.has_T;
+    for (int c = 0; c < 3; c++) {
+        fill(s->nse[c], n, 0.0);
+        if (stage > 0) mult_add(s->nse[c], s->tv[c], zeta[stage], n);
+    }
+    if (hasT) {
+        fill(s->trhs, n, 0.0);
+        if (stage > 0) mult_add(s->trhs, s->ts, zeta[stage], n);
+    }
+    /* diffusion */
+    for (int c = 0; c < 3; c++) { fill(s->tv[c], n, 0.0); compute_diff(s, s->V[c], s->tv[c]); }
+    if (hasT) { fill(s->ts, n, 0.0); compute_diff(s, s->T, s->ts); }
+    for (int c = 0; c < 3; c++) mult_add(s->nse[c], s->tv[c], a * s->d.nu, n);
+    if (hasT) mult_add(s->trhs, s->ts, a * s->d.kappa, n);
+    /* forcing and pressure gradient */
+    for (int c = 0; c < 3; c++) fill(s->tv[c], n, 0.0);
+    if (hasT) {
+        fill(s->ts, n, 0.0);
+        for (int c = 0; c < 3; c++)
+            for (size_t q = 0; q < n; q++) s->tv[c][q] -= s->d.buoy[c] * s->T[q];
+    }
+    for (int c = 0; c < 3; c++) fill(s->pg[c], n, 0.0);
+    CORE_LOOP(s) {
+        long q = FX(s, i, j, k);
+        s->pg[0][q] = d1(s, s->P, 0, i, j, k);
+        s->pg[1][q] = d1(s, s->P, 1, i, j, k);
+        s->pg[2][q] = d1(s, s->P, 2, i, j, k);
+    }
+    for (int c = 0; c < 3; c++)
+        for (size_t q = 0; q < n; q++) s->tv[c][q] -= s->pg[c][q];
+    for (int c = 0; c < 3; c++) mult_add(s->nse[c], s->tv[c], a + b, n);
+    if (hasT) mult_add(s->trhs, s->ts, a + b, n);
+    /* non-linear terms */
+    for (int c = 0; c < 3; c++) fill(s->tv[c], n, 0.0);
+    nlin_v(s, s->tv);
+    if (hasT) { fill(s->ts, n, 0.0); nlin_s(s, s->T, s->ts); }
+    for (int c = 0; c < 3; c++) mult_add(s->nse[c], s->tv[c], gamm[stage], n);
+    if (hasT) mult_add(s->trhs, s->ts, gamm[stage], n);
+    for (int c = 0; c < 3; c++) {
+        for (size_t q = 0; q < n; q++) s->nse[c][q] *= s->d.dt;
+        for (size_t q = 0; q < n; q++) s->nse[c][q] += s->V[c][q];
+        sync_faces(s, s->nse[c]);
+    }
+    if (hasT) {
+        for (size_t q = 0; q < n; q++) s->trhs[q] *= s->d.dt;
+        for (size_t q = 0; q < n; q++) s->trhs[q] += s->T[q];
+        sync_faces(s, s->trhs);
+    }
+    return 0;
+}
+
+int or_nse_stage(or_nse *s, int stage)
+{
+    size_t n = s->full;
+    double a = alph[stage], b = alph[stage];
+    or_nse_rhs(s, stage);
+    for (int c = 0; c < 3; c++) {
+        s->jacobi_iters[c] = jacobi(s, s->V[c], s->nse[c], b, s->d.nu, s->d.vbc_type[c], s->d.vbc_val[c]);
+        if (s->jacobi_iters[c] < 0) return 2;
+    }
+    if (s->d.has_T) {
+        s->jacobi_iters[3] = jacobi(s, s->T, s->trhs, b, s->d.kappa, s->d.tbc_type, s->d.tbc_val);
+        if (s->jacobi_iters[3] < 0) return 2;
+    }
+    /* projection */
+    fill(s->mgr, n, 0.0);
+    CORE_LOOP(s) {
+        long q = FX(s, i, j, k);
+        s->mgr[q] += d1(s, s->V[0], 0, i, j, k);
+        s->mgr[q] += d1(s, s->V[1], 1, i, j, k);
+        s->mgr[q] += d1(s, s->V[2], 2, i, j, k);
+    }
+    double sc = 1.0 / ((a + b) * s->d.dt);
+    for (size_t q = 0; q < n; q++) s->mgr[q] *= sc;
+    or_mg_solve(s->mg, s->Pp, s->mgr, s->p, NULL, &s->mg_stats);
+    sync_faces(s, s->Pp);
+    for (size_t q = 0; q < n; q++) s->P[q] += s->Pp[q];
+    for (int c = 0; c < 3; c++) fill(s->pg[c], n, 0.0);   /* holds the zeroed pads of the stage */
+    CORE_LOOP(s) {
+        long q = FX(s, i, j, k);
+        s->pg[0][q] = d1(s, s->Pp, 0, i, j, k);
+        s->pg[1][q] = d1(s, s->Pp, 1, i, j, k);
+        s->pg[2][q] = d1(s, s->Pp, 2, i, j, k);
+    }
+    double pc = (a + b) * s->d.dt;
+    for (int c = 0; c < 3; c++) {
+        for (size_t q = 0; q < n; q++) s->pg[c][q] *= pc;
+        for (size_t q = 0; q < n; q++) s->V[c][q] -= s->pg[c][q];
+    }
+    for (int c = 0; c < 3; c++) impose_v(s, c);
+    impose_p(s);
+    if (s->d.has_T) impose_t(s);
+    return 0;
+}
+
+int or_nse_step(or_nse *s)
+{
+    for (int stage = 0; stage < 3; stage++) {
+        int r = or_nse_stage(s, stage);
+        if (r) return r;
+    }
+    s->time += s->d.dt;
+    return 0;
+}
+
+void or_nse_jacobi_iters(const or_nse *s, int out[4]) { for (int c = 0; c < 4; c++) out[c] = s->jacobi_iters[c]; }
+
+void or_nse_mg_stats(const or_nse *s, or_mg_stats *st) { *st = s->mg_stats; }
+
+/* fields in/out (full arrays); set applies the initial BCs as the solver
+ * constructors do */
+void or_nse_set(or_nse *s, const double *vx, const double *vy, const double *vz, const double *p,
+                const double *t, const double *pp)
+{
+    size_t b = s->full * sizeof(double);
+    if (vx) memcpy(s->V[0], vx, b);
+    if (vy) memcpy(s->V[1], vy, b);
+    if (vz) memcpy(s->V[2], vz, b);
+    if (p) memcpy(s->P, p, b);
+    if (t) memcpy(s->T, t, b);
+    if (pp) memcpy(s->Pp, pp, b);
+    for (int c = 0; c < 3; c++) impose_v(s, c);
+    impose_p(s);
+    if (s->d.has_T) impose_t(s);
+}
+
+void or_nse_get(const or_nse *s, double *vx, double *vy, double *vz, double *p, double *t, double *pp)
+{
+    size_t b = s->full * sizeof(double);
+    if (vx) memcpy(vx, s->V[0], b);
+    if (vy) memcpy(vy, s->V[1], b);
+    if (vz) memcpy(vz, s->V[2], b);
+    if (p) memcpy(p, s->P, b);
+    if (t) memcpy(t, s->T, b);
+    if (pp) memcpy(pp, s->Pp, b);
+}
+
+void or_nse_get_rhs(const or_nse *s, double *rx, double *ry, double *rz, double *rt, double *hx, double *hy,
+                    double *hz, double *ht)
+{
+    size_t b = s->full * sizeof(double);
+    if (rx) memcpy(rx, s->nse[0], b);
+    if (ry) memcpy(ry, s->nse[1], b);
+    if (rz) memcpy(rz, s->nse[2], b);
+    if (rt) memcpy(rt, s->trhs, b);
+    if (hx) memcpy(hx, s->tv[0], b);
+    if (hy) memcpy(hy, s->tv[1], b);
+    if (hz) memcpy(hz, s->tv[2], b);
+    if (ht) memcpy(ht, s->ts, b);
+}
+
+/* single operators on caller arrays, for stencil tests */
+void or_nse_op(or_nse *s, int op, const double *F, double *out)
+{
+    size_t n = s->full;
+    memset(out, 0, n * sizeof(double));
+    switch (op) {
+        case 0: case 1: case 2:       /* first derivative along op */
+            CORE_LOOP(s) out[FX(s, i, j, k)] = d1(s, F, op, i, j, k);
+            break;
+        case 3: case 4: case 5:       /* second derivative along op-3 */
+            CORE_LOOP(s) out[FX(s, i, j, k)] = d2(s, F, op - 3, i, j, k);
+            break;
+        case 6:                       /* sfield::computeDiff */
+            compute_diff(s, F, out);
+            break;
+        case 7:                       /* sfield::computeNLin with the current V */
+            nlin_s(s, F, out);
+            break;
+    }
+}
+
+/* tseries.cc:209-290: KE, max |div|, NuVol */
+void or_nse_diag(or_nse *s, double *ke, double *divmax, double *nuvol, double *nuwall)
+{
+    int p = s->p;
+    double dxi[3];
+    for (int a = 0; a < 3; a++) dxi[a] = 1.0 / (double)s->n[a];
+    double tke = 0.0, uzt = 0.0, dm = 0.0;
+    for (int i = 0; i < s->n[0]; i++)
+        for (int j = 0; j < s->n[1]; j++)
+            for (int k = 0; k < s->n[2]; k++) {
+                long q = FX(s, i, j, k);
+                double dVol = (dxi[0] / s->xi_x[0][i + p]) * (dxi[1] / s->xi_x[1][j + p]) * (dxi[2] / s->xi_x[2][k + p]);
+                tke += (pow(s->V[0][q], 2.0) + pow(s->V[1][q], 2.0) + pow(s->V[2][q], 2.0)) * 0.5 * dVol;
+                if (s->d.has_T) uzt += s->V[2][q] * s->T[q] * dVol;
+                double dv = 0.0;
+                dv += d1(s, s->V[0], 0, i, j, k);
+                dv += d1(s, s->V[1], 1, i, j, k);
+                dv += d1(s, s->V[2], 2, i, j, k);
+                if (fabs(dv) > dm) dm = fabs(dv);
+            }
+    double vol = s->d.g.len[0] * s->d.g.len[1] * s->d.g.len[2];
+    if (ke) *ke = tke / vol;
+    if (divmax) *divmax = dm;
+    if (nuvol) *nuvol = s->d.has_T ? 1.0 + (uzt / vol) / s->d.kappa : 0.0;
+    if (nuwall && s->d.has_T) {
+        /* z-gravity, one rank holding both walls: wallDn ends as the top-wall
+         * distance and serves both sums (tseries.cc:118-128, 374-394) */
+        double dt_sum = 0.0, wall_dn = s->d.g.len[2] - s->x[2][s->n[2] - 1 + p];
+        for (int i = 0; i < s->n[0]; i++)
+            for (int j = 0; j < s->n[1]; j++) {
+                double dA = (dxi[0] / s->xi_x[0][i + p]) * (dxi[1] / s->xi_x[1][j + p]);
+                double dTdn = (1.0 - fabs(s->T[FX(s, i, j, 0)])) / wall_dn;
+                dt_sum += dTdn * dA;
+            }
+        for (int i = 0; i < s->n[0]; i++)
+            for (int j = 0; j < s->n[1]; j++) {
+                double dA = (dxi[0] / s->xi_x[0][i + p]) * (dxi[1] / s->xi_x[1][j + p]);
+                double dTdn = fabs(s->T[FX(s, i, j, s->n[2] - 1)]) / wall_dn;
+                dt_sum += dTdn * dA;
+            }
+        *nuwall = dt_sum / (s->d.g.len[0] * s->d.g.len[1] * 2);
+    }
+}

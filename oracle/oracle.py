@@ -9,6 +9,10 @@ Parity status of each oracle function is pinned by tests/test_oracle_*.py:
   or_mg_* (3-D and 2-D V-cycle)                     -> dense brute force, closed
                                                        forms, the reference's
                                                        own r^2/6 mgTest solution
+  or_nse_* (field stencils, Jacobi-CN, RK3 step)    -> polynomial exactness,
+                                                       convergence order, dense
+                                                       implicit solve, Fourier
+                                                       decay, Taylor-Green
 """
 import ctypes as C
 import os
@@ -33,6 +37,18 @@ class MgParams(C.Structure):
                 ("mg_tol", C.c_double), ("sor", C.c_double), ("pre_smooth", C.c_int),
                 ("post_smooth", C.c_int), ("res_type", C.c_int), ("bc", C.c_int),
                 ("no_early_exit", C.c_int)]
+
+
+FBC_DIRICHLET, FBC_NEUMANN = 0, 1
+
+
+class NseDesc(C.Structure):
+    _fields_ = [("g", GridDesc), ("mg", MgParams), ("dscheme", C.c_int), ("nlscheme", C.c_int),
+                ("has_T", C.c_int), ("nu", C.c_double), ("kappa", C.c_double), ("dt", C.c_double),
+                ("cn_tol", C.c_double), ("buoy", C.c_double * 3),
+                ("vbc_type", (C.c_int * 6) * 3), ("vbc_val", (C.c_double * 6) * 3),
+                ("tbc_type", C.c_int * 6), ("tbc_val", C.c_double * 6),
+                ("pbc_type", C.c_int * 6), ("pbc_val", C.c_double * 6)]
 
 
 class MgStats(C.Structure):
@@ -73,6 +89,20 @@ def lib():
         L.or_set_threads.argtypes = [C.c_int]
         L.or_mg_vcycles.argtypes = [C.c_void_p, C.c_int]
         L.or_mg_residual.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]
+        L.or_nse_create.argtypes = [C.POINTER(NseDesc)]
+        L.or_nse_create.restype = C.c_void_p
+        L.or_nse_destroy.argtypes = [C.c_void_p]
+        L.or_nse_max_iterations.argtypes = [C.c_void_p]
+        L.or_nse_set.argtypes = [C.c_void_p] + [dp] * 6
+        L.or_nse_get.argtypes = [C.c_void_p] + [dp] * 6
+        L.or_nse_rhs.argtypes = [C.c_void_p, C.c_int]
+        L.or_nse_get_rhs.argtypes = [C.c_void_p] + [dp] * 8
+        L.or_nse_stage.argtypes = [C.c_void_p, C.c_int]
+        L.or_nse_step.argtypes = [C.c_void_p]
+        L.or_nse_jacobi_iters.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+        L.or_nse_mg_stats.argtypes = [C.c_void_p, C.POINTER(MgStats)]
+        L.or_nse_op.argtypes = [C.c_void_p, C.c_int, dp, dp]
+        L.or_nse_diag.argtypes = [C.c_void_p, dp, dp, dp, dp]
         _lib = L
     return _lib
 
@@ -239,3 +269,98 @@ class Multigrid:
 
 def set_threads(n):
     lib().or_set_threads(int(n))
+
+
+def nse_desc(g, dscheme=2, nlscheme=3, has_T=1, nu=1e-2, kappa=1e-2, dt=1e-3, cn_tol=1e-6,
+             buoy=(0.0, 0.0, -1.0), vbc=None, tbc=None, pbc=None, mg=None):
+    """Descriptor of one run.  vbc: [(types[6], vals[6]) per component];
+    tbc / pbc: (types[6], vals[6]).  Defaults: the Rayleigh-Benard wiring of
+    hydro.cc:287-485 / scalar.cc:129-201 (no-slip, T = 1 bottom, 0 top,
+    Neumann P, Neumann T on side walls)."""
+    d = NseDesc()
+    d.g = g
+    d.mg = mg if mg is not None else mg_params()
+    d.dscheme, d.nlscheme, d.has_T = dscheme, nlscheme, int(has_T)
+    d.nu, d.kappa, d.dt, d.cn_tol = nu, kappa, dt, cn_tol
+    for a in range(3):
+        d.buoy[a] = buoy[a]
+    if vbc is None:
+        vbc = [([FBC_DIRICHLET] * 6, [0.0] * 6)] * 3
+    if tbc is None:
+        tbc = ([FBC_NEUMANN] * 4 + [FBC_DIRICHLET] * 2, [0.0] * 4 + [1.0, 0.0])
+    if pbc is None:
+        pbc = ([FBC_NEUMANN] * 6, [0.0] * 6)
+    for c in range(3):
+        for w in range(6):
+            d.vbc_type[c][w], d.vbc_val[c][w] = vbc[c][0][w], vbc[c][1][w]
+    for w in range(6):
+        d.tbc_type[w], d.tbc_val[w] = tbc[0][w], tbc[1][w]
+        d.pbc_type[w], d.pbc_val[w] = pbc[0][w], pbc[1][w]
+    return d
+
+
+class Nse:
+    """Oracle lsRK3_d3 time stepper with its fields (V, P, T, Pp)."""
+
+    NAMES = ("vx", "vy", "vz", "p", "t", "pp")
+
+    def __init__(self, d):
+        self.d = d
+        self.h = lib().or_nse_create(C.byref(d))
+        g = d.g
+        self.shape = (g.n[0] + 2 * g.pad, g.n[1] + 2 * g.pad, g.n[2] + 2 * g.pad)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_nse_destroy(self.h)
+            self.h = None
+
+    @property
+    def max_iterations(self):
+        return lib().or_nse_max_iterations(self.h)
+
+    def set(self, **f):
+        arrs = [np.ascontiguousarray(f[k], dtype=np.float64) if f.get(k) is not None else None
+                for k in self.NAMES]
+        for a in arrs:
+            assert a is None or a.shape == self.shape
+        lib().or_nse_set(self.h, *[_dp(a) if a is not None else None for a in arrs])
+
+    def get(self):
+        out = [np.zeros(self.shape) for _ in self.NAMES]
+        lib().or_nse_get(self.h, *[_dp(a) for a in out])
+        return dict(zip(self.NAMES, out))
+
+    def rhs(self, stage):
+        lib().or_nse_rhs(self.h, stage)
+        out = [np.zeros(self.shape) for _ in range(8)]
+        lib().or_nse_get_rhs(self.h, *[_dp(a) for a in out])
+        return dict(zip(("rx", "ry", "rz", "rt", "hx", "hy", "hz", "ht"), out))
+
+    def stage(self, stage):
+        return lib().or_nse_stage(self.h, stage)
+
+    def step(self):
+        return lib().or_nse_step(self.h)
+
+    def jacobi_iters(self):
+        out = (C.c_int * 4)()
+        lib().or_nse_jacobi_iters(self.h, out)
+        return list(out)
+
+    def mg_stats(self):
+        st = MgStats()
+        lib().or_nse_mg_stats(self.h, C.byref(st))
+        return dict(cycles=st.cycles, residual=st.residual, coarse_iters_max=st.coarse_iters_max,
+                    coarse_capped=st.coarse_capped)
+
+    def op(self, op, F):
+        F = np.ascontiguousarray(F, dtype=np.float64)
+        out = np.zeros(self.shape)
+        lib().or_nse_op(self.h, op, _dp(F), _dp(out))
+        return out
+
+    def diag(self):
+        v = [C.c_double() for _ in range(4)]
+        lib().or_nse_diag(self.h, *[C.byref(x) for x in v])
+        return dict(ke=v[0].value, divmax=v[1].value, nuvol=v[2].value, nuwall=v[3].value)

@@ -116,6 +116,60 @@ void or_mg_vcycles(or_mg *m, int count);                           /* poisson.cc
  * written into a padded array (ghosts zero). */
 void or_mg_residual(or_mg *m, int level, int zero_bc, double *out_pad);
 
+/* ---- Navier-Stokes / Boussinesq step (lib/field, lib/timestep) ---- */
+
+/* Field boundary kinds (lib/boundary/dirichlet.cc, neumann.cc).  Walls are
+ * numbered 0 x-lo (Lft), 1 x-hi (Rgt), 2 y-lo (Frn), 3 y-hi (Bak),
+ * 4 z-lo (Bot), 5 z-hi (Top); periodic axes ignore theirs. */
+enum { OR_FBC_DIRICHLET = 0, OR_FBC_NEUMANN = 1 };
+
+typedef struct {
+    or_grid_desc g;        /* field grid; g.pad = 1 (dScheme 1) or 2 (dScheme 2) */
+    or_mg_params mg;       /* pressure-correction solve (bc forced to Neumann, poisson.cc:87) */
+    int dscheme;           /* "Differentiation Scheme" 1: 2nd order, 2: 4th order */
+    int nlscheme;          /* "Non-Linear Scheme" 1: central, 3: Morinishi */
+    int has_T;             /* scalar (Boussinesq) run: lsRK3_d3::timeAdvance(V, P, T) */
+    double nu, kappa;      /* timestep.cc:70-94 */
+    double dt;             /* "Time-Step" */
+    double cn_tol;         /* "CN Tolerance" (absolute) */
+    double buoy[3];        /* gTerms = Fb g / |g| (buoyantForce.cc:46-61) */
+    int vbc_type[3][6];    /* per velocity component, per wall: OR_FBC_* */
+    double vbc_val[3][6];
+    int tbc_type[6];
+    double tbc_val[6];
+    int pbc_type[6];
+    double pbc_val[6];
+} or_nse_desc;
+
+typedef struct or_nse or_nse;
+
+or_nse *or_nse_create(const or_nse_desc *d);
+void or_nse_destroy(or_nse *s);
+int or_nse_max_iterations(const or_nse *s);          /* timestep.cc:115 */
+/* Load fields (full arrays, pad g.pad; NULL keeps the current one), then
+ * impose the BCs of V, P and T. */
+void or_nse_set(or_nse *s, const double *vx, const double *vy, const double *vz,
+                const double *p, const double *t, const double *pp);
+void or_nse_get(const or_nse *s, double *vx, double *vy, double *vz, double *p, double *t, double *pp);
+/* The explicit part of one RK3 sub-stage (lsRK3_d3.cc:199-267): fills the
+ * stage RHS (nseRHS, tmpRHS) and leaves the non-linear terms in tempVF/tempSF. */
+int or_nse_rhs(or_nse *s, int stage);
+void or_nse_get_rhs(const or_nse *s, double *rx, double *ry, double *rz, double *rt,
+                    double *hx, double *hy, double *hz, double *ht);
+/* One RK3 sub-stage (lsRK3_d3.cc:199-304) / one full step; 0, or 2 when a
+ * Jacobi solve exceeds maxIterations (predictors.cc:133-141). */
+int or_nse_stage(or_nse *s, int stage);
+int or_nse_step(or_nse *s);
+void or_nse_jacobi_iters(const or_nse *s, int out[4]);
+void or_nse_mg_stats(const or_nse *s, or_mg_stats *st);   /* last pressure-correction solve */
+/* Single operators over the core (pads of out zero): 0-2 first derivative
+ * along x/y/z, 3-5 second derivative, 6 computeDiff (Laplacian), 7 scalar
+ * computeNLin with the current V (H -= ...). */
+void or_nse_op(or_nse *s, int op, const double *F, double *out);
+/* tseries.cc:209-290 diagnostics: volume-mean KE, max |div V|, NuVol, and
+ * the wall Nusselt number of z-gravity runs. */
+void or_nse_diag(or_nse *s, double *ke, double *divmax, double *nuvol, double *nuwall);
+
 /* Threads the oracle's OpenMP loops use (0 = runtime default). */
 void or_set_threads(int n);
 

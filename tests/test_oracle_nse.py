@@ -1,0 +1,357 @@
+"""Pins of the oracle's Navier-Stokes restatement (lib/field, lib/boundary,
+lib/timestep, lib/io/tseries.cc) by what the mathematics fixes:
+
+* derivative stencils: exactness on polynomials of the computational
+  coordinate xi (4th order interior, 2nd order edge cells), chain rule with the
+  grid metrics, convergence order on a sinusoid;
+* Morinishi non-linear term: exact skew-symmetric form on low-degree
+  polynomial velocities and scalars;
+* boundary rules: a linear field's ghost value equals the field at the mirror
+  point of the wall (Dirichlet value / Neumann gradient);
+* Jacobi Crank-Nicolson: equal to the dense implicit solve (walls included);
+* RK3 stage: the exact amplification factor of a periodic Fourier mode under
+  the heat equation, and Taylor-Green decay for the full non-linear step;
+* diagnostics: conduction state Nu = 1, uniform-flow kinetic energy.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+FD, FN = O.FBC_DIRICHLET, O.FBC_NEUMANN
+
+
+def make(n, periodic=(1, 1, 1), gtype=(0, 0, 0), beta=(1.0, 1.0, 1.0), pad=2, **kw):
+    g = O.grid_desc(n, periodic=periodic, gtype=gtype, beta=beta, pad=pad)
+    kw.setdefault("mg", O.mg_params(vc_count=20, vc_tol=1e-12, solve_flag=0, pre=3, post=3))
+    d = O.nse_desc(g, **kw)
+    return g, d, O.Nse(d)
+
+
+def xi_axis(n, pad):
+    return (np.arange(-pad, n + pad) + 0.5) / n
+
+
+def along(vals, axis, shape):
+    s = [1, 1, 1]
+    s[axis] = len(vals)
+    return np.broadcast_to(np.asarray(vals).reshape(s), shape).copy()
+
+
+def core(a, p):
+    return a[p:-p, p:-p, p:-p]
+
+
+# ---- derivative stencils --------------------------------------------------
+
+POLY4 = np.polynomial.Polynomial([0.3, -1.2, 0.7, 2.1, -0.9])
+POLY2 = np.polynomial.Polynomial([0.3, -1.2, 0.7])
+
+
+@pytest.mark.parametrize("gtype", [(0, 0, 0), (2, 1, 2)])
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_derivatives_exact_on_xi_polynomials(gtype, axis):
+    n, p = 12, 2
+    g, d, s = make(n, periodic=(0, 0, 0), gtype=gtype, beta=(1.3, 0.9, 1.5))
+    xi = xi_axis(n, p)
+    m = O.field_metrics(g, axis)
+    for poly, interior_only in ((POLY2, False), (POLY4, True)):
+        F = along(poly(xi), axis, s.shape)
+        d1 = core(s.op(axis, F), p)
+        d2 = core(s.op(3 + axis, F), p)
+        e1 = along((m["xi_x"] * poly.deriv(1)(xi))[p:-p], axis, d1.shape)
+        e2 = along((m["xixx"] * poly.deriv(1)(xi) + m["xix2"] * poly.deriv(2)(xi))[p:-p], axis, d1.shape)
+        sl = [slice(None)] * 3
+        if interior_only:
+            sl[axis] = slice(1, n - 1)
+        sl = tuple(sl)
+        scale = max(np.abs(e2).max(), 1.0)
+        assert np.abs(d1[sl] - e1[sl]).max() < 1e-9 * scale
+        assert np.abs(d2[sl] - e2[sl]).max() < 1e-9 * scale
+        # derivatives across the other axes vanish
+        for other in {0, 1, 2} - {axis}:
+            assert np.abs(s.op(other, F)).max() < 1e-9 * scale
+    # degree 4 is not exact at the 2nd-order edge cells (the override is real)
+    F = along(POLY4(xi), axis, s.shape)
+    d1 = core(s.op(axis, F), p)
+    e1 = along((m["xi_x"] * POLY4.deriv(1)(xi))[p:-p], axis, d1.shape)
+    edge = [slice(None)] * 3
+    edge[axis] = 0
+    assert np.abs(d1[tuple(edge)] - e1[tuple(edge)]).max() > 1e-6
+
+
+def test_derivative_convergence_orders():
+    """sin(2 pi x) on a periodic uniform axis: 4th order in the interior,
+    2nd order at the first/last cell (the reference applies the 2nd-order
+    override on every rank-edge cell, periodic axes included)."""
+    errs_in, errs_edge = [], []
+    for n in (16, 32, 64):
+        g, d, s = make(n, periodic=(1, 1, 1))
+        p = 2
+        x = xi_axis(n, p)
+        F = along(np.sin(2 * np.pi * x), 0, s.shape)
+        d1 = core(s.op(0, F), p)[:, 0, 0]
+        ex = 2 * np.pi * np.cos(2 * np.pi * x[p:-p])
+        errs_in.append(np.abs(d1[1:-1] - ex[1:-1]).max())
+        errs_edge.append(max(abs(d1[0] - ex[0]), abs(d1[-1] - ex[-1])))
+    r_in = np.log2(np.array(errs_in[:-1]) / np.array(errs_in[1:]))
+    r_ed = np.log2(np.array(errs_edge[:-1]) / np.array(errs_edge[1:]))
+    assert np.all(r_in > 3.7) and np.all(r_ed > 1.8) and np.all(r_ed < 2.3)
+
+
+def test_laplacian_is_sum_of_second_derivatives():
+    n, p = 10, 2
+    g, d, s = make(n, periodic=(1, 0, 0), gtype=(0, 1, 2), beta=(1.0, 1.1, 1.4))
+    rng = np.random.default_rng(5)
+    F = rng.standard_normal(s.shape)
+    lap = s.op(6, F)
+    ref = ((0.0 + s.op(3, F)) + s.op(4, F)) + s.op(5, F)
+    assert np.array_equal(core(lap, p), core(ref, p))
+
+
+# ---- Morinishi non-linear term ---------------------------------------------
+
+def test_morinishi_scalar_skew_symmetric_form():
+    """H = -1/2 [u_a dF/dx_a + d(u_a F)/dx_a] with u_a linear and F quadratic
+    in xi along each axis (products of degree <= 3: exact at interior cells)."""
+    n, p = 12, 2
+    g, d, s = make(n, periodic=(0, 0, 0), gtype=(2, 0, 1), beta=(1.2, 1.0, 0.8), nlscheme=3)
+    xi = [xi_axis(n, p)] * 3
+    m = [O.field_metrics(g, a)["xi_x"] for a in range(3)]
+    sh = s.shape
+    X, Y, Z = (along(xi[a], a, sh) for a in range(3))
+    u = [0.4 + 0.3 * Y, -0.2 + 0.5 * Z, 0.1 - 0.6 * X]        # u_a independent of x_a
+    F = 0.2 + X - 0.7 * X ** 2 + 0.5 * Y ** 2 - 0.3 * Z + 0.9 * Z ** 2
+    dF = [1 - 1.4 * X, 1.0 * Y, -0.3 + 1.8 * Z]
+    s.set(vx=u[0], vy=u[1], vz=u[2], t=np.zeros(sh), p=np.zeros(sh), pp=np.zeros(sh))
+    H = s.op(7, F)
+    # BCs of the set() call overwrite the first ghost layer of V on the walls;
+    # interior cells (>= 2 away from every wall) never read it
+    exp = np.zeros(sh)
+    for a in range(3):
+        xa = along(m[a], a, sh)
+        exp -= 0.5 * (u[a] * xa * dF[a] + xa * (u[a] * dF[a]))   # d(u_a F) = u_a dF since du_a/dx_a = 0
+    sl = (slice(p + 2, -p - 2),) * 3
+    assert np.abs(H[sl] - exp[sl]).max() < 1e-9
+
+
+def test_morinishi_vector_rhs_nonlinear_term():
+    """tempVF after the RHS assembly holds the Morinishi term of V
+    (vfield.cc:271-462) on a periodic box: check against the analytic
+    -1/2 [u_j du_i/dx_j + d(u_j u_i)/dx_j] for trigonometric velocities
+    (4th-order interior, 2nd-order first/last cells -> tolerance by n)."""
+    n, p = 32, 2
+    g, d, s = make(n, periodic=(1, 1, 1), has_T=0, nu=0.0, kappa=0.0, nlscheme=3)
+    x = xi_axis(n, p)
+    sh = s.shape
+    X, Y, Z = (along(x, a, sh) for a in range(3))
+    tp = 2 * np.pi
+    u = np.sin(tp * X) * np.cos(tp * Y)
+    v = 0.5 * np.cos(tp * Z)
+    w = np.sin(tp * Y)
+    s.set(vx=u, vy=v, vz=w, p=np.zeros(sh), t=np.zeros(sh), pp=np.zeros(sh))
+    r = s.rhs(0)
+    ux, uy = tp * np.cos(tp * X) * np.cos(tp * Y), -tp * np.sin(tp * X) * np.sin(tp * Y)
+    vz = -0.5 * tp * np.sin(tp * Z)
+    wy = tp * np.cos(tp * Y)
+    div = ux  # v_y = w_z = 0
+    adv = [u * ux + v * uy, w * vz, v * wy]
+    exp = [-(adv[0] + 0.5 * div * u), -(adv[1] + 0.5 * div * v), -(adv[2] + 0.5 * div * w)]
+    for c, key in enumerate(("hx", "hy", "hz")):
+        err = np.abs(core(r[key] - exp[c], p))
+        assert err[2:-2, 2:-2, 2:-2].max() < 5e-3, (key, err.max())
+        assert err.max() < 0.15
+
+
+# ---- boundary rules ----------------------------------------------------------
+
+@pytest.mark.parametrize("kind", [FD, FN])
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_wall_ghost_is_mirror_value(kind, axis):
+    n, p = 8, 2
+    per = [1, 1, 1]
+    per[axis] = 0
+    L = 1.0
+    g, d, s = make(n, periodic=per, gtype=(2, 2, 2), beta=(1.5, 1.5, 1.5))
+    xc = O.field_metrics(g, axis)["x"]
+    a0, slope = 0.7, -1.3
+    F = along(a0 + slope * xc, axis, s.shape)
+    if kind == FD:
+        vals = [0.0] * 6
+        vals[2 * axis], vals[2 * axis + 1] = a0, a0 + slope * L
+    else:
+        vals = [0.0] * 6
+        vals[2 * axis] = vals[2 * axis + 1] = slope
+    d2 = O.nse_desc(g, has_T=1, tbc=([kind] * 6, vals), mg=d.mg)
+    s2 = O.Nse(d2)
+    s2.set(vx=np.zeros(s.shape), vy=np.zeros(s.shape), vz=np.zeros(s.shape),
+           p=np.zeros(s.shape), t=F, pp=np.zeros(s.shape))
+    T = s2.get()["t"]
+    x0, xn = xc[p], xc[-p - 1]
+    lo = [slice(p, -p)] * 3
+    hi = list(lo)
+    lo[axis], hi[axis] = p - 1, -p
+    assert np.allclose(T[tuple(lo)], a0 + slope * (-x0), atol=1e-13)
+    assert np.allclose(T[tuple(hi)], a0 + slope * (2 * L - xn), atol=1e-13)
+
+
+def test_periodic_pads_wrap_two_layers():
+    n, p = 8, 2
+    g, d, s = make(n, periodic=(1, 1, 0), gtype=(0, 0, 2), beta=(1, 1, 1.5))
+    rng = np.random.default_rng(3)
+    F = rng.standard_normal(s.shape)
+    s.set(vx=F, vy=F, vz=F, p=F, t=F, pp=np.zeros(s.shape))
+    V = s.get()["vx"]
+    c = slice(p, -p)
+    assert np.array_equal(V[0:p, c, c], V[n:n + p, c, c])
+    assert np.array_equal(V[n + p:, c, c], V[p:2 * p, c, c])
+    assert np.array_equal(V[c, 0:p, c], V[c, n:n + p, c])
+
+
+# ---- Jacobi Crank-Nicolson = dense implicit solve ---------------------------
+
+def dense_cn(g, shape_core, coef, kinds_vals):
+    """(I - c L2) T = rhs with per-wall Dirichlet/Neumann ghost rules -> A, shift"""
+    import itertools
+    N = int(np.prod(shape_core))
+    A = np.eye(N)
+    shift = np.zeros(N)
+    idx = np.arange(N).reshape(shape_core)
+    for flat, pos in enumerate(itertools.product(*[range(k) for k in shape_core])):
+        for ax in range(3):
+            n = shape_core[ax]
+            m = O.field_metrics(g, ax)
+            i = pos[ax]
+            h = 1.0 / n
+            c2, c1 = m["xix2"][i + g.pad] / h ** 2, m["xixx"][i + g.pad] * (0.5 / h)
+            A[flat, flat] -= coef * (-2.0 * c2)
+            for off, wgt in ((1, c2 + c1), (-1, c2 - c1)):
+                j = i + off
+                q = list(pos)
+                if 0 <= j < n:
+                    q[ax] = j
+                    A[flat, idx[tuple(q)]] -= coef * wgt
+                elif g.periodic[ax]:
+                    q[ax] = j % n
+                    A[flat, idx[tuple(q)]] -= coef * wgt
+                else:
+                    wall = 2 * ax + (1 if off > 0 else 0)
+                    kind, val = kinds_vals[0][wall], kinds_vals[1][wall]
+                    if kind == FD:          # ghost = 2v - C
+                        A[flat, flat] -= coef * wgt * (-1.0)
+                        shift[flat] += coef * wgt * 2 * val
+                    else:                   # ghost = C - v h_wall
+                        xc = m["x"]
+                        hw = 2 * xc[g.pad] if off < 0 else 2 * (xc[n - 1 + g.pad] - g.len[ax])
+                        A[flat, flat] -= coef * wgt
+                        shift[flat] -= coef * wgt * val * hw
+    return A, shift
+
+
+def test_jacobi_cn_equals_dense_implicit_solve():
+    n, p = 6, 2
+    kappa, dt = 0.05, 2e-3
+    tbc = ([FN, FD, FN, FN, FD, FD], [0.3, 0.5, -0.2, 0.1, 1.0, 0.0])
+    g, d, s = make(n, periodic=(0, 0, 0), gtype=(1, 2, 2), beta=(1.1, 1.3, 1.5), has_T=1,
+                   nu=0.0, kappa=kappa, dt=dt, cn_tol=1e-14, buoy=(0, 0, 0), tbc=tbc, nlscheme=1)
+    rng = np.random.default_rng(9)
+    T0 = rng.standard_normal(s.shape)
+    zero = np.zeros(s.shape)
+    s.set(vx=zero, vy=zero, vz=zero, p=zero, t=T0, pp=zero)
+    rt = s.rhs(0)["rt"]
+    s.set(t=T0)
+    assert s.stage(0) == 0
+    T1 = s.get()["t"]
+    beta0 = 4.0 / 15.0
+    A, shift = dense_cn(g, (n, n, n), beta0 * dt * kappa, tbc)
+    sol = np.linalg.solve(A, core(rt, p).ravel() + shift).reshape(n, n, n)
+    assert np.abs(core(T1, p) - sol).max() < 1e-11
+    assert s.jacobi_iters()[3] >= 1
+
+
+# ---- RK3 stage: exact discrete heat-equation amplification --------------------
+
+def test_rk3_heat_mode_amplification():
+    """V = 0, periodic box, dScheme 1: T = cos(2 pi k.x) is an eigenvector of
+    both the explicit and the implicit operator with eigenvalue
+    lam = -sum_d 4 sin^2(pi k_d h)/h^2.  Each sub-stage multiplies it by
+    (1 + alpha dt kappa lam) / (1 - beta dt kappa lam) with
+    alpha = beta = (4/15, 1/15, 1/6)."""
+    n, p = 16, 1
+    kappa, dt = 0.02, 5e-3
+    k = (1, 2, 3)
+    g, d, s = make(n, periodic=(1, 1, 1), pad=p, dscheme=1, nlscheme=1, has_T=1, nu=0.0,
+                   kappa=kappa, dt=dt, cn_tol=1e-13, buoy=(0, 0, 0))
+    x = xi_axis(n, p)
+    sh = s.shape
+    X, Y, Z = (along(x, a, sh) for a in range(3))
+    T0 = np.cos(2 * np.pi * (k[0] * X + k[1] * Y + k[2] * Z))
+    zero = np.zeros(sh)
+    s.set(vx=zero, vy=zero, vz=zero, p=zero, t=T0, pp=zero)
+    assert s.step() == 0
+    h = 1.0 / n
+    lam = -sum(4 * np.sin(np.pi * kd * h) ** 2 / h ** 2 for kd in k)
+    G = 1.0
+    for a in (4 / 15, 1 / 15, 1 / 6):
+        G *= (1 + a * dt * kappa * lam) / (1 - a * dt * kappa * lam)
+    T1 = s.get()["t"]
+    assert np.abs(core(T1, p) - G * core(T0, p)).max() < 1e-11
+    # the same run with a wrong beta (e.g. gamma instead) would give a visibly different factor
+    G_wrong = 1.0
+    for a, b in zip((4 / 15, 1 / 15, 1 / 6), (8 / 15, 5 / 12, 3 / 4)):
+        G_wrong *= (1 + a * dt * kappa * lam) / (1 - b * dt * kappa * lam)
+    assert abs(G - G_wrong) > 1e-4
+
+
+# ---- full step: Taylor-Green decay -----------------------------------------------
+
+def test_taylor_green_2d_decay():
+    """u = sin(2 pi x) cos(2 pi z), w = -cos(2 pi x) sin(2 pi z), v = 0 is an
+    exact Navier-Stokes solution with KE ~ exp(-2 (2pi)^2 2 nu t)."""
+    n, p = 24, 2
+    nu, dt, steps = 0.01, 2e-3, 5
+    g, d, s = make(n, periodic=(1, 1, 1), has_T=0, nu=nu, kappa=0.0, dt=dt, cn_tol=1e-12, nlscheme=3,
+                   mg=O.mg_params(vc_count=30, vc_tol=1e-12, solve_flag=0, pre=3, post=3))
+    x = xi_axis(n, p)
+    sh = s.shape
+    X, Z = along(x, 0, sh), along(x, 2, sh)
+    tp = 2 * np.pi
+    u = np.sin(tp * X) * np.cos(tp * Z)
+    w = -np.cos(tp * X) * np.sin(tp * Z)
+    P = 0.25 * (np.cos(2 * tp * X) + np.cos(2 * tp * Z))
+    s.set(vx=u, vy=np.zeros(sh), vz=w, p=P, t=np.zeros(sh), pp=np.zeros(sh))
+    ke0 = s.diag()["ke"]
+    assert abs(ke0 - 0.25) < 1e-12
+    for _ in range(steps):
+        assert s.step() == 0
+    ke = s.diag()["ke"]
+    exact = ke0 * np.exp(-2 * tp ** 2 * 2 * nu * steps * dt)
+    decay = ke0 - exact
+    assert abs(ke - exact) < 0.02 * decay
+    assert s.diag()["divmax"] < 0.05
+
+
+# ---- diagnostics -----------------------------------------------------------------
+
+def test_diagnostics_conduction_and_uniform_flow():
+    n, p = 16, 2
+    g, d, s = make(n, periodic=(1, 1, 0), gtype=(0, 0, 2), beta=(1, 1, 1.5), has_T=1,
+                   kappa=1e-3, nu=1e-3)
+    z = O.field_metrics(g, 2)["x"]
+    sh = s.shape
+    T = along(1.0 - z, 2, sh)
+    U = np.full(sh, 0.3)
+    zero = np.zeros(sh)
+    s.set(vx=U, vy=zero, vz=zero, p=zero, t=T, pp=zero)
+    dg = s.diag()
+    assert dg["nuvol"] == 1.0
+    assert abs(dg["nuwall"] - 1.0) < 1e-3
+    assert abs(dg["ke"] - 0.5 * 0.09) < 0.5 * 0.09 * 1e-3
+    # a constant x-velocity is divergence-free except at the first/last z cells where the
+    # no-slip ghosts (2*0 - 0.3) do not enter: d/dx of a constant is exactly zero
+    assert dg["divmax"] == 0.0
+
+
+def test_max_iterations_formula():
+    g, d, s = make(16)
+    assert s.max_iterations == int(np.log(16 ** 3) ** 3)

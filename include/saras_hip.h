@@ -164,6 +164,100 @@ saras_status saras_mg_bench_sweeps(saras_mg *mg, int reps, double *ms);
  * calibration kernel for HBM byte counters. */
 saras_status saras_bench_stream_copy(size_t n, int reps, double *ms);
 
+/* ---- Navier-Stokes / Boussinesq time step (lib/field, lib/timestep) ----
+ *
+ * One handle owns a run's fields (V, P, T and the pressure correction Pp),
+ * the RK3 work arrays (the reference's function-static nseRHS/tempVF/tmpRHS/
+ * tempSF, lsRK3_d3.cc:193-197) and the pressure-correction multigrid solver
+ * (the time stepper's mgSolver member, timestep.h:142), all resident in
+ * device memory.  Caller field arrays use the layout above with pad
+ * grid->pad (>= the differentiation order: 1 for dscheme 1, 2 for 2). */
+
+/* Field boundary kinds (lib/boundary/dirichlet.cc, neumann.cc).  Walls are
+ * numbered 0 x-lo (Left), 1 x-hi (Right), 2 y-lo (Front), 3 y-hi (Back),
+ * 4 z-lo (Bottom), 5 z-hi (Top); periodic axes ignore theirs. */
+enum { SARAS_FBC_DIRICHLET = 0, SARAS_FBC_NEUMANN = 1 };
+
+typedef struct {
+    int dscheme;             /* "Differentiation Scheme": 1 (2nd order) or 2 (4th order) */
+    int nlscheme;            /* "Non-Linear Term": 1 central, 3 Morinishi */
+    int has_T;               /* 1: scalar run, lsRK3_d3::timeAdvance(V, P, T) */
+    double nu, kappa;        /* diffusivities (timestep.cc:70-94) */
+    double dt;               /* "Time-Step" */
+    double cn_tol;           /* "Solve Tolerance" of the Jacobi CN solves (absolute) */
+    double buoy[3];          /* gTerms = Fb g/|g| of buoyantForce (buoyantForce.cc:46-61) */
+    int vbc_type[3][6];      /* SARAS_FBC_* per velocity component and wall */
+    double vbc_val[3][6];    /* Dirichlet value or Neumann gradient */
+    int tbc_type[6];
+    double tbc_val[6];
+    int pbc_type[6];
+    double pbc_val[6];
+} saras_nse_params;
+
+typedef struct {
+    int jacobi_iters[4];     /* iteration count of the last Vx, Vy, Vz, T solves */
+    saras_mg_stats mg;       /* last pressure-correction solve */
+    double device_ms;        /* device time of the call (HIP events) */
+} saras_nse_stats;
+
+typedef struct saras_nse saras_nse;
+
+/* lsRK3_d3::lsRK3_d3 + timestep::timestep (lsRK3_d3.cc:58-70,
+ * timestep.cc:57-122) with the fields of hydro/scalar (vfield, sfield).
+ * mg: the "Multigrid" keys of the pressure solve (bc is forced to Neumann,
+ * poisson.cc:87).  SARAS_BAD_SHAPE for grids the path does not cover:
+ * planar grids, pad < dscheme, and Morinishi runs with a wall in z and
+ * ny != nz (the reference's z edge test reads core.ubound(1),
+ * vfield.cc:434). */
+saras_status saras_nse_create(const saras_grid_desc *grid, const saras_mg_params *mg,
+                              const saras_nse_params *params, saras_nse **out);
+saras_status saras_nse_destroy(saras_nse *s);
+
+/* Load fields (NULL keeps the current one; only the core is read) and
+ * impose the boundary conditions (the solvers' initial imposeBCs). */
+saras_status saras_nse_set_fields(saras_nse *s, const double *vx, const double *vy, const double *vz,
+                                  const double *p, const double *t, const double *pp, int ptr_kind);
+/* Store fields: the core and every ghost cell with one index outside the
+ * core that the reference's imposeBCs/syncFaces define (periodic pads, first
+ * wall layer); edge and corner pads and the second wall layer are left
+ * untouched. */
+saras_status saras_nse_get_fields(saras_nse *s, double *vx, double *vy, double *vz, double *p, double *t,
+                                  double *pp, int ptr_kind);
+
+/* lsRK3_d3::timeAdvance(V, P[, T]) (lsRK3_d3.cc:94-177, 190-305): one full
+ * time step of three sub-stages.  SARAS_JACOBI_DIVERGED where the reference
+ * aborts (predictors.cc:133-141); a pressure solve that runs out of V-cycles
+ * is not an error (the reference is silent). */
+saras_status saras_nse_step(saras_nse *s, saras_nse_stats *stats);
+/* One sub-stage (rkLev = stage in 0..2) of the same loop. */
+saras_status saras_nse_stage(saras_nse *s, int stage, saras_nse_stats *stats);
+/* The explicit part of a sub-stage alone (lsRK3_d3.cc:199-267): fills the
+ * stage RHS and the new non-linear terms; read them with get_rhs (core). */
+saras_status saras_nse_rhs(saras_nse *s, int stage);
+saras_status saras_nse_get_rhs(saras_nse *s, double *rx, double *ry, double *rz, double *rt, double *hx,
+                               double *hy, double *hz, double *ht, int ptr_kind);
+/* timestep::solveVx/solveVy/solveVz/solveT (predictors.cc:59-145, 161-219,
+ * 235-321, 337-423): Jacobi CN solve of field 0..3 (Vx, Vy, Vz, T) against
+ * the current stage RHS; *iters gets the reference's iteration count. */
+saras_status saras_nse_solve_field(saras_nse *s, int field, double beta, int *iters);
+
+/* Single field operators over the core of caller arrays whose ghost cells
+ * are used as supplied: op 0-2 derivative::calcDerivative1_x/y/z, 3-5
+ * calcDerivative2xx/yy/zz (out = value; derivative.cc:104-306), 6
+ * sfield::computeDiff (out += lap F; sfield.cc:133-147), 7
+ * sfield::computeNLin with the handle's velocity (out += H;
+ * sfield.cc:160-311). */
+saras_status saras_field_op(saras_nse *s, int op, const double *f, double *out, int ptr_kind);
+
+/* tseries::writeTSData diagnostics (tseries.cc:209-290): out[0] volume-mean
+ * kinetic energy, out[1] max |div V|, out[2] NuVol, out[3] NuWall
+ * (z gravity). */
+saras_status saras_nse_diag(saras_nse *s, double out[4]);
+
+/* Measurement hook: `count` full steps with no host work between them
+ * beyond the reference's own convergence checks; *ms = device time. */
+saras_status saras_nse_bench_steps(saras_nse *s, int count, double *ms);
+
 /* ---- device memory helpers, so callers without a GPU framework can hand
  * the solver device pointers. ---- */
 saras_status saras_device_alloc(void **ptr, size_t bytes);

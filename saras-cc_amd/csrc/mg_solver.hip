@@ -22,6 +22,7 @@
 
 #include "grid_host.hpp"
 #include "mg_fused.hpp"
+#include "mg_internal.hpp"
 #include "mg_kernels.hpp"
 #include "mg_tail.hpp"
 #include "saras_hip.h"
@@ -813,6 +814,34 @@ static saras_status solve_mixed(saras_mg *m, hipStream_t s, saras_mg_stats *st) 
     st->converged = res < m->p.vc_tol;
     return SARAS_OK;
 }
+
+namespace saras {
+
+void mg_level0(saras_mg *m, double **x, double **b, long *sx, long *sy) {
+    auto &L0 = m->h64.lv[0];
+    *x = L0.x;
+    *b = L0.b;
+    *sx = L0.sx;
+    *sy = L0.sy;
+}
+
+hipStream_t mg_stream(saras_mg *m) { return m->stream; }
+
+void set_last_error(const std::string &msg) { set_error(msg); }
+
+saras_status mg_solve_resident(saras_mg *m, hipStream_t s, saras_mg_stats *st) {
+    m->coarse_iters_max = 0;
+    m->coarse_capped = 0;
+    saras_mg_stats r{};
+    saras_status e = m->p.precision == SARAS_PREC_FP64 ? solve_fp64(m, s, &r) : solve_mixed(m, s, &r);
+    if (e) return e;
+    r.coarse_iters_max = m->coarse_iters_max;
+    r.coarse_capped = m->coarse_capped;
+    if (st) *st = r;
+    return r.converged ? SARAS_OK : SARAS_NOT_CONVERGED;
+}
+
+}  // namespace saras
 
 extern "C" {
 

@@ -1,0 +1,473 @@
+// nse_kernels.hpp -- FP64 kernels of the Navier-Stokes/Boussinesq time step
+// (lib/field, lib/boundary, lib/timestep of the reference).
+//
+// Device layout: every field is stored core-only, nx x ny x nz doubles with
+// z fastest and a row pitch of 16 doubles (128 bytes), the same layout as the
+// multigrid level arrays, so the divergence can be written straight into the
+// pressure solver's level-0 RHS and the pressure correction read straight out
+// of its level-0 iterate.  Ghost cells are never stored: a stencil that steps
+// out of the core gets the value the reference's imposeBCs() would have put
+// there (field.cc:128-148, dirichlet.cc:72-76, neumann.cc:89-93, and the
+// pad-thick periodic syncFaces of mpidata.cc):
+//   periodic axis  F(-q) = F(n-q), F(n-1+q) = F(q-1)
+//   Dirichlet wall ghost = 2 v - F(edge)
+//   Neumann wall   ghost = F(edge) - v h_wall
+// Every stencil of the step stays within one ghost layer on a wall axis
+// (first and last cells use second-order stencils), so the second layer the
+// reference leaves stale is never needed.
+//
+// Floating point: the expressions below keep the operand order of the cited
+// reference lines and are built with -ffp-contract=off, so the FP64 results
+// match an FP64 CPU build of the same expressions bit for bit.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace saras {
+namespace nse {
+
+constexpr double kRecip12 = 1.0 / 12.0;
+
+// wall rules of one field (6 walls: x-lo, x-hi, y-lo, y-hi, z-lo, z-hi)
+struct FieldBC {
+    int type[6];     // 0 Dirichlet, 1 Neumann
+    double val[6];   // Dirichlet value
+    double kh[6];    // Neumann value * h_wall (neumann.cc:59-75)
+};
+
+struct Geom {
+    int n[3];
+    long sx, sy;         // core pitch (elements)
+    int per[3];
+    int edge_last[3];    // last index of the 2nd-order edge test per axis (Morinishi z uses ny-1)
+    int dscheme;
+    double ih[3], ih2[3];          // derivative.cc:52-54
+    double i2d[3];                 // vfield.cc:133-135
+    double tsih2[3], tsi2h[3];     // timestep.cc:156-173
+    const double *xi_x[3], *xixx[3], *xix2[3];   // core cells
+};
+
+__device__ __forceinline__ int pick(int ax, int i, int j, int k) { return ax == 0 ? i : (ax == 1 ? j : k); }
+
+// Core-only field with on-the-fly ghost cells.
+struct CoreField {
+    const double *f;
+    FieldBC bc;
+    __device__ __forceinline__ double at(const Geom &G, int i, int j, int k) const {
+        return f[(long)i * G.sx + (long)j * G.sy + k];
+    }
+    __device__ __forceinline__ double nb(const Geom &G, int ax, int i, int j, int k, int off) const {
+        int c[3] = {i, j, k};
+        int q = c[ax] + off;
+        const int n = G.n[ax];
+        if (q >= 0 && q < n) {
+            c[ax] = q;
+            return f[(long)c[0] * G.sx + (long)c[1] * G.sy + c[2]];
+        }
+        if (G.per[ax]) {
+            c[ax] = q < 0 ? q + n : q - n;
+            return f[(long)c[0] * G.sx + (long)c[1] * G.sy + c[2]];
+        }
+        const int w = 2 * ax + (q >= n ? 1 : 0);
+        c[ax] = q < 0 ? 0 : n - 1;
+        const double inner = f[(long)c[0] * G.sx + (long)c[1] * G.sy + c[2]];
+        return bc.type[w] == 0 ? 2.0 * bc.val[w] - inner : inner - bc.kh[w];
+    }
+};
+
+// Caller-layout field with stored pads (pad p, (n+2p)^3): the single-operator
+// entry points read whatever ghost values the caller supplies, as the
+// reference's derivative class reads its array.
+struct PadField {
+    const double *f;
+    long SX, SY;
+    int p;
+    __device__ __forceinline__ double at(const Geom &, int i, int j, int k) const {
+        return f[(long)(i + p) * SX + (long)(j + p) * SY + (k + p)];
+    }
+    __device__ __forceinline__ double nb(const Geom &, int ax, int i, int j, int k, int off) const {
+        int c[3] = {i, j, k};
+        c[ax] += off;
+        return f[(long)(c[0] + p) * SX + (long)(c[1] + p) * SY + (c[2] + p)];
+    }
+};
+
+// derivative::calcDerivative1_x/y/z (derivative.cc:104-192)
+template <class F>
+__device__ __forceinline__ double d1(const Geom &G, const F &A, int ax, int i, int j, int k) {
+    const int q = pick(ax, i, j, k);
+    const double fm = A.nb(G, ax, i, j, k, -1), fp = A.nb(G, ax, i, j, k, 1);
+    double v;
+    if (G.dscheme == 1) {
+        v = (fp - fm) * 0.5;
+    } else if (q == 0 || q == G.n[ax] - 1) {
+        v = 0.5 * (fp - fm);
+    } else {
+        const double fm2 = A.nb(G, ax, i, j, k, -2), fp2 = A.nb(G, ax, i, j, k, 2);
+        v = (8.0 * (fp - fm) - (fp2 - fm2)) * kRecip12;
+    }
+    v = v * G.ih[ax];
+    return G.xi_x[ax][q] * v;
+}
+
+// derivative::calcDerivative2xx/yy/zz (derivative.cc:206-306)
+template <class F>
+__device__ __forceinline__ double d2(const Geom &G, const F &A, int ax, int i, int j, int k) {
+    const int q = pick(ax, i, j, k);
+    const double fm = A.nb(G, ax, i, j, k, -1), f = A.at(G, i, j, k), fp = A.nb(G, ax, i, j, k, 1);
+    double t, o;
+    if (G.dscheme == 1) {
+        t = (fp - fm) * 0.5;
+        o = (fm - 2.0 * f) + fp;
+    } else if (q == 0 || q == G.n[ax] - 1) {
+        t = 0.5 * (fp - fm);
+        o = fp - 2.0 * f + fm;
+    } else {
+        const double fm2 = A.nb(G, ax, i, j, k, -2), fp2 = A.nb(G, ax, i, j, k, 2);
+        t = (8.0 * (fp - fm) - (fp2 - fm2)) * kRecip12;
+        o = ((-30.0 * f + 16.0 * (fp + fm)) - (fp2 + fm2)) * kRecip12;
+    }
+    t = t * G.ih[ax];
+    o = o * G.ih2[ax];
+    return G.xixx[ax][q] * t + G.xix2[ax][q] * o;
+}
+
+// computeDiff: H += d2xx; += d2yy; += d2zz (vfield.cc:150-192, sfield.cc:133-147)
+template <class F>
+__device__ __forceinline__ double laplacian(const Geom &G, const F &A, int i, int j, int k) {
+    double h = 0.0;
+    h += d2(G, A, 0, i, j, k);
+    h += d2(G, A, 1, i, j, k);
+    h += d2(G, A, 2, i, j, k);
+    return h;
+}
+
+__device__ __forceinline__ bool mor_edge(const Geom &G, int ax, int q) {
+    return G.dscheme == 1 || q == 0 || q == G.edge_last[ax];
+}
+
+// Morinishi skew-symmetric term of one (advecting axis a, advected field C)
+// pair: (adv + div) / 2 with adv = u_a xi_x dC, div = xi_x d(U_a C)
+// (vfield.cc:351-459, sfield.cc:236-307).  U is the advecting component.
+template <class FU, class FC>
+__device__ __forceinline__ double mor_pair(const Geom &G, const FU &U, const FC &Cf, int a, int i, int j, int k) {
+    const int q = pick(a, i, j, k);
+    const double xm = G.xi_x[a][q];
+    const double u = U.at(G, i, j, k);
+    const double cp = Cf.nb(G, a, i, j, k, 1), cm = Cf.nb(G, a, i, j, k, -1);
+    const double up = U.nb(G, a, i, j, k, 1), um = U.nb(G, a, i, j, k, -1);
+    const double pp = up * cp, pm = um * cm;
+    double adv, dv;
+    if (mor_edge(G, a, q)) {
+        adv = u * xm * (cp - cm) * G.i2d[a];
+        dv = xm * (pp - pm) * G.i2d[a];
+    } else {
+        const double cp2 = Cf.nb(G, a, i, j, k, 2), cm2 = Cf.nb(G, a, i, j, k, -2);
+        const double up2 = U.nb(G, a, i, j, k, 2), um2 = U.nb(G, a, i, j, k, -2);
+        const double pp2 = up2 * cp2, pm2 = um2 * cm2;
+        adv = u * xm * (-cp2 + 8.0 * cp - 8.0 * cm + cm2) * G.i2d[a] / 6.0;
+        dv = xm * (-pp2 + 8.0 * pp - 8.0 * pm + pm2) * G.i2d[a] / 6.0;
+    }
+    return (adv + dv) / 2;
+}
+
+// H_c of computeNLin for advected field C with velocity V (H starts at 0)
+template <class FV, class FC>
+__device__ __forceinline__ double nlin(const Geom &G, int nlscheme, const FV *V, const FC &Cf, int i, int j, int k) {
+    double h = 0.0;
+    if (nlscheme == 1) {
+        // central: H -= V_a dC/dx_a (vfield.cc:207-249, sfield.cc:162-178)
+        h -= V[0].at(G, i, j, k) * d1(G, Cf, 0, i, j, k);
+        h -= V[1].at(G, i, j, k) * d1(G, Cf, 1, i, j, k);
+        h -= V[2].at(G, i, j, k) * d1(G, Cf, 2, i, j, k);
+    } else {
+        h -= mor_pair(G, V[0], Cf, 0, i, j, k);
+        h -= mor_pair(G, V[1], Cf, 1, i, j, k);
+        h -= mor_pair(G, V[2], Cf, 2, i, j, k);
+    }
+    return h;
+}
+
+struct StageCoef {
+    double zeta;       // zetaRK3(rkLev)
+    double a_nu;       // alphRK3 * nu
+    double a_kappa;    // alphRK3 * kappa
+    double ab;         // alphRK3 + betaRK3
+    double gamma;      // gammRK3
+    double dt;
+    double buoy[3];    // gTerms
+    int stage;
+    int has_T;
+    int nlscheme;
+};
+
+// The explicit part of one RK3 sub-stage for every field
+// (lsRK3_d3.cc:199-267): R = (zeta H_old + alpha nu lap u + (alpha+beta)(f - grad P)
+// + gamma H) dt + u, with H the new non-linear term, written back for the
+// next sub-stage.
+__global__ void __launch_bounds__(256) k_rhs(Geom G, StageCoef c, CoreField V0, CoreField V1, CoreField V2,
+                                             CoreField T, CoreField P, double *__restrict__ H0,
+                                             double *__restrict__ H1, double *__restrict__ H2,
+                                             double *__restrict__ HT, double *__restrict__ R0,
+                                             double *__restrict__ R1, double *__restrict__ R2,
+                                             double *__restrict__ RT) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    const int i = blockIdx.z;
+    if (k >= G.n[2]) return;
+    const long idx = (long)i * G.sx + (long)j * G.sy + k;
+    const CoreField V[3] = {V0, V1, V2};
+    double *Hs[3] = {H0, H1, H2};
+    double *Rs[3] = {R0, R1, R2};
+    const double tc = c.has_T ? T.at(G, i, j, k) : 0.0;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        double nse = 0.0;
+        if (c.stage > 0) nse += c.zeta * Hs[q][idx];
+        const double diff = laplacian(G, V[q], i, j, k);
+        nse += c.a_nu * diff;
+        double f = 0.0;
+        if (c.has_T) f -= c.buoy[q] * tc;
+        f -= d1(G, P, q, i, j, k);
+        nse += c.ab * f;
+        const double h = nlin(G, c.nlscheme, V, V[q], i, j, k);
+        nse += c.gamma * h;
+        nse *= c.dt;
+        nse += V[q].at(G, i, j, k);
+        Hs[q][idx] = h;
+        Rs[q][idx] = nse;
+    }
+    if (c.has_T) {
+        double tr = 0.0;
+        if (c.stage > 0) tr += c.zeta * HT[idx];
+        const double diff = laplacian(G, T, i, j, k);
+        tr += c.a_kappa * diff;
+        tr += c.ab * 0.0;   // tempSF forcing of T is zero
+        const double h = nlin(G, c.nlscheme, V, T, i, j, k);
+        tr += c.gamma * h;
+        tr *= c.dt;
+        tr += tc;
+        HT[idx] = h;
+        RT[idx] = tr;
+    }
+}
+
+// Block max of non-negative doubles, published with one 64-bit atomic max
+// on the bit pattern (IEEE order equals unsigned order for x >= 0).
+__device__ __forceinline__ void block_max_publish(double v, unsigned long long *slot) {
+    __shared__ double red[4];
+    for (int o = 32; o > 0; o >>= 1) {
+        const double w = __shfl_xor(v, o, 64);
+        v = (w > v || w != w) ? w : v;
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) red[wv] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = red[0];
+        for (int q = 1; q < (int)(blockDim.x >> 6); ++q) m = (red[q] > m || red[q] != red[q]) ? red[q] : m;
+        atomicMax(slot, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+// One Jacobi iteration of the Crank-Nicolson solve (predictors.cc:59-145):
+// out = update(F).  With `slot`, also the residual max|F - beta dt nu lap2 F - rhs|
+// of the input F (the reference's check after its previous update).
+template <bool RES>
+__global__ void __launch_bounds__(256) k_jacobi(Geom G, CoreField F, const double *__restrict__ rhs,
+                                                double *__restrict__ out, double dt, double diff, double beta,
+                                                unsigned long long *slot) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    const int i = blockIdx.z;
+    double res = 0.0;
+    if (k < G.n[2]) {
+        const long idx = (long)i * G.sx + (long)j * G.sy + k;
+        const double C = F.at(G, i, j, k);
+        const double E = F.nb(G, 0, i, j, k, 1), W = F.nb(G, 0, i, j, k, -1);
+        const double N = F.nb(G, 1, i, j, k, 1), S = F.nb(G, 1, i, j, k, -1);
+        const double Tp = F.nb(G, 2, i, j, k, 1), B = F.nb(G, 2, i, j, k, -1);
+        const double x2 = G.xix2[0][i], xx = G.xixx[0][i];
+        const double y2 = G.xix2[1][j], yy = G.xixx[1][j];
+        const double z2 = G.xix2[2][k], zz = G.xixx[2][k];
+        const double r = rhs[idx];
+        out[idx] = ((G.tsih2[0] * x2 * (E + W) + G.tsi2h[0] * xx * (E - W) + G.tsih2[1] * y2 * (N + S) +
+                     G.tsi2h[1] * yy * (N - S) + G.tsih2[2] * z2 * (Tp + B) + G.tsi2h[2] * zz * (Tp - B)) *
+                        dt * diff * beta +
+                    r) /
+                   (1.0 + 2.0 * dt * diff * beta * (G.tsih2[0] * x2 + G.tsih2[1] * y2 + G.tsih2[2] * z2));
+        if (RES) {
+            const double t = C - beta * dt * diff *
+                                     (x2 * (E - 2.0 * C + W) * G.tsih2[0] + xx * (E - W) * G.tsi2h[0] +
+                                      y2 * (N - 2.0 * C + S) * G.tsih2[1] + yy * (N - S) * G.tsi2h[1] +
+                                      z2 * (Tp - 2.0 * C + B) * G.tsih2[2] + zz * (Tp - B) * G.tsi2h[2]);
+            res = fabs(t - r);
+        }
+    }
+    if (RES) block_max_publish(res, slot);
+}
+
+// vfield::divergence (vfield.cc:721-737) scaled into the Poisson RHS:
+// b = (((0 + dVx/dx) + dVy/dy) + dVz/dz) * sc   (lsRK3_d3.cc:278-279)
+__global__ void __launch_bounds__(256) k_div(Geom G, CoreField V0, CoreField V1, CoreField V2, double sc,
+                                             double *__restrict__ b, long bsx, long bsy) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    const int i = blockIdx.z;
+    if (k >= G.n[2]) return;
+    double d = 0.0;
+    d += d1(G, V0, 0, i, j, k);
+    d += d1(G, V1, 1, i, j, k);
+    d += d1(G, V2, 2, i, j, k);
+    b[(long)i * bsx + (long)j * bsy + k] = d * sc;
+}
+
+// P += Pp; V -= ((alpha+beta) dt) grad Pp   (lsRK3_d3.cc:285-295)
+__global__ void __launch_bounds__(256) k_correct(Geom G, CoreField Pp, double *__restrict__ P,
+                                                 double *__restrict__ V0, double *__restrict__ V1,
+                                                 double *__restrict__ V2, double pc) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    const int i = blockIdx.z;
+    if (k >= G.n[2]) return;
+    const long idx = (long)i * G.sx + (long)j * G.sy + k;
+    P[idx] += Pp.at(G, i, j, k);
+    double g0 = d1(G, Pp, 0, i, j, k), g1 = d1(G, Pp, 1, i, j, k), g2 = d1(G, Pp, 2, i, j, k);
+    g0 *= pc;
+    g1 *= pc;
+    g2 *= pc;
+    V0[idx] -= g0;
+    V1[idx] -= g1;
+    V2[idx] -= g2;
+}
+
+// tseries diagnostics, per-block partial sums (tseries.cc:209-290):
+// [0] sum KE dVol, [1] sum Vz T dVol, [2] max |div V|, [3] sum bottom-wall dTdn dA,
+// [4] sum top-wall dTdn dA.  dxi = 1/n per axis; wall_dn = the top-wall distance.
+constexpr int kDiagBlocks = 1024;
+constexpr int kDiagThreads = 256;
+
+__global__ void __launch_bounds__(kDiagThreads) k_diag(Geom G, CoreField V0, CoreField V1, CoreField V2,
+                                                       CoreField T, int has_T, double dxi0, double dxi1,
+                                                       double dxi2, double wall_dn, double *__restrict__ part) {
+    __shared__ double red[5][kDiagThreads / 64];
+    const long total = (long)G.n[0] * G.n[1] * G.n[2];
+    double ke = 0.0, uzt = 0.0, dm = 0.0, nb = 0.0, nt = 0.0;
+    for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < total; c += (long)gridDim.x * blockDim.x) {
+        const int k = (int)(c % G.n[2]);
+        const int j = (int)((c / G.n[2]) % G.n[1]);
+        const int i = (int)(c / ((long)G.n[2] * G.n[1]));
+        const double dVol = (dxi0 / G.xi_x[0][i]) * (dxi1 / G.xi_x[1][j]) * (dxi2 / G.xi_x[2][k]);
+        const double u = V0.at(G, i, j, k), v = V1.at(G, i, j, k), w = V2.at(G, i, j, k);
+        ke += (u * u + v * v + w * w) * 0.5 * dVol;
+        double d = 0.0;
+        d += d1(G, V0, 0, i, j, k);
+        d += d1(G, V1, 1, i, j, k);
+        d += d1(G, V2, 2, i, j, k);
+        d = fabs(d);
+        dm = (d > dm || d != d) ? d : dm;
+        if (has_T) {
+            const double t = T.at(G, i, j, k);
+            uzt += w * t * dVol;
+            const double dA = (dxi0 / G.xi_x[0][i]) * (dxi1 / G.xi_x[1][j]);
+            if (k == 0) nb += (1.0 - fabs(t)) / wall_dn * dA;
+            if (k == G.n[2] - 1) nt += fabs(t) / wall_dn * dA;
+        }
+    }
+    double vals[5] = {ke, uzt, dm, nb, nt};
+    for (int q = 0; q < 5; ++q) {
+        double v = vals[q];
+        for (int o = 32; o > 0; o >>= 1) {
+            const double w = __shfl_xor(v, o, 64);
+            if (q == 2) v = (w > v || w != w) ? w : v;
+            else v += w;
+        }
+        if ((threadIdx.x & 63) == 0) red[q][threadIdx.x >> 6] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 5) {
+        const int q = threadIdx.x;
+        double v = red[q][0];
+        for (int w = 1; w < kDiagThreads / 64; ++w) {
+            if (q == 2) v = (red[q][w] > v || red[q][w] != red[q][w]) ? red[q][w] : v;
+            else v += red[q][w];
+        }
+        part[(long)blockIdx.x * 5 + q] = v;
+    }
+}
+
+// caller array (pad p) -> core
+__global__ void k_field_load(const double *__restrict__ src, long SX, long SY, int p, double *__restrict__ dst,
+                             long sx, long sy, int nx, int ny, int nz) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, i = blockIdx.z;
+    if (k >= nz) return;
+    dst[(long)i * sx + (long)j * sy + k] = src[(long)(i + p) * SX + (long)(j + p) * SY + (k + p)];
+}
+
+// core + face ghost cells -> caller array (pad p): every cell with at most one
+// index outside the core gets the value the reference's field holds after
+// imposeBCs (periodic pads p deep, first wall layer); edge/corner pads and the
+// second wall layer are left as the caller had them.
+__global__ void k_field_store(Geom G, CoreField F, double *__restrict__ dst, long SX, long SY, int p) {
+    const int kk = blockIdx.x * blockDim.x + threadIdx.x;   // over nz + 2p
+    const int jj = blockIdx.y, ii = blockIdx.z;               // over ny + 2p, nx + 2p
+    if (kk >= G.n[2] + 2 * p) return;
+    const int c[3] = {ii - p, jj - p, kk - p};
+    int outside = 0, ax = -1;
+    for (int a = 0; a < 3; ++a)
+        if (c[a] < 0 || c[a] >= G.n[a]) {
+            ++outside;
+            ax = a;
+        }
+    const long o = (long)ii * SX + (long)jj * SY + kk;
+    if (outside == 0) {
+        dst[o] = F.at(G, c[0], c[1], c[2]);
+        return;
+    }
+    if (outside > 1) return;
+    const int q = c[ax];
+    if (!G.per[ax] && (q < -1 || q > G.n[ax])) return;
+    int base[3] = {c[0], c[1], c[2]};
+    base[ax] = q < 0 ? 0 : G.n[ax] - 1;
+    const int off = q < 0 ? q : q - (G.n[ax] - 1);
+    dst[o] = F.nb(G, ax, base[0], base[1], base[2], off);
+}
+
+// Single field operators on caller arrays (ghosts as supplied):
+// 0-2 first derivative along x/y/z, 3-5 second derivative (out = value),
+// 6 computeDiff (out += lap F), 7 sfield::computeNLin with V (out += H).
+template <int OP>
+__global__ void k_field_op(Geom G, PadField F, const CoreField V0, const CoreField V1, const CoreField V2,
+                           int nlscheme, double *__restrict__ out, long SX, long SY, int p) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, i = blockIdx.z;
+    if (k >= G.n[2]) return;
+    const long o = (long)(i + p) * SX + (long)(j + p) * SY + (k + p);
+    if (OP <= 2) {
+        out[o] = d1(G, F, OP, i, j, k);
+    } else if (OP <= 5) {
+        out[o] = d2(G, F, OP - 3, i, j, k);
+    } else if (OP == 6) {
+        double h = out[o];
+        h += d2(G, F, 0, i, j, k);
+        h += d2(G, F, 1, i, j, k);
+        h += d2(G, F, 2, i, j, k);
+        out[o] = h;
+    } else {
+        const CoreField V[3] = {V0, V1, V2};
+        double h = out[o];
+        if (nlscheme == 1) {
+            h -= V[0].at(G, i, j, k) * d1(G, F, 0, i, j, k);
+            h -= V[1].at(G, i, j, k) * d1(G, F, 1, i, j, k);
+            h -= V[2].at(G, i, j, k) * d1(G, F, 2, i, j, k);
+        } else {
+            h -= mor_pair(G, V[0], F, 0, i, j, k);
+            h -= mor_pair(G, V[1], F, 1, i, j, k);
+            h -= mor_pair(G, V[2], F, 2, i, j, k);
+        }
+        out[o] = h;
+    }
+}
+
+}  // namespace nse
+}  // namespace saras

@@ -1,0 +1,578 @@
+// nse_solver.hip -- device-resident RK3 / Crank-Nicolson time stepper of the
+// incompressible Navier-Stokes (+ Boussinesq scalar) equations and the
+// field-operator entry points of include/saras_hip.h.
+//
+// One sub-stage (lsRK3_d3.cc:199-304) is
+//   k_rhs        explicit terms of every field in one pass (diffusion, forcing,
+//                pressure gradient, Morinishi non-linear term, RK3 weights)
+//   k_jacobi     Jacobi CN iterations per field; each launch updates the field
+//                and returns the residual of its input iterate, so the
+//                reference's update + residual pair costs one pass
+//   k_div        divergence into the pressure solver's level-0 RHS
+//   multigrid    the FP64 (or mixed) V-cycle solver on resident arrays
+//   k_correct    P += Pp, V -= (alpha+beta) dt grad Pp
+// with every boundary condition applied on the fly by the stencils.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "grid_host.hpp"
+#include "mg_internal.hpp"
+#include "nse_kernels.hpp"
+#include "saras_hip.h"
+
+using saras::nse::CoreField;
+using saras::nse::FieldBC;
+using saras::nse::Geom;
+using saras::nse::PadField;
+
+namespace {
+
+#define NSE_TRY(expr)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            saras::set_last_error(std::string(#expr) + ": " + hipGetErrorString(e_));   \
+            return e_ == hipErrorOutOfMemory ? SARAS_OOM : SARAS_HIP_ERROR;             \
+        }                                                                               \
+    } while (0)
+
+// lsRK3_d3.cc:63-66 (beta = alpha)
+const double kAlpha[3] = {4.0 / 15.0, 1.0 / 15.0, 1.0 / 6.0};
+const double kGamma[3] = {8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0};
+const double kZeta[3] = {0.0, -17.0 / 60.0, -5.0 / 12.0};
+
+long pitch16(int nz) { return (nz + 15) / 16 * 16; }
+
+}  // namespace
+
+struct saras_nse {
+    saras_grid_desc g{};
+    saras_nse_params p{};
+    saras_mg *mg = nullptr;
+    Geom G{};
+    double *metrics = nullptr;          // 9 core metric arrays
+    FieldBC bc[4]{};                     // Vx, Vy, Vz, T
+    FieldBC pbc{};                       // P
+    FieldBC ppbc{};                      // Pp: the Poisson solver's Neumann walls (ghost = inner)
+    double *F[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};   // Vx, Vy, Vz, T, P
+    double *H[4] = {nullptr, nullptr, nullptr, nullptr};
+    double *R[4] = {nullptr, nullptr, nullptr, nullptr};
+    double *scratch = nullptr;
+    size_t elems = 0;
+    unsigned long long *slots = nullptr;
+    unsigned long long *slot_host = nullptr;
+    int max_iter = 0;
+    double *diag_part = nullptr, *diag_host = nullptr;
+    double *stage_a = nullptr, *stage_b = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int jac[4] = {0, 0, 0, 0};
+    saras_mg_stats mg_stats{};
+    double wall_dn = 0.0;
+    long full = 0;
+    int threads = 256;
+
+    ~saras_nse() {
+        for (auto *q : F) (void)hipFree(q);
+        for (auto *q : H) (void)hipFree(q);
+        for (auto *q : R) (void)hipFree(q);
+        (void)hipFree(scratch);
+        (void)hipFree(metrics);
+        (void)hipFree(slots);
+        (void)hipHostFree(slot_host);
+        (void)hipFree(diag_part);
+        (void)hipHostFree(diag_host);
+        (void)hipFree(stage_a);
+        (void)hipFree(stage_b);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (mg) saras_mg_destroy(mg);
+    }
+
+    dim3 grid3() const { return dim3((G.n[2] + threads - 1) / threads, G.n[1], G.n[0]); }
+    dim3 grid3_full(int p) const {
+        return dim3((G.n[2] + 2 * p + threads - 1) / threads, G.n[1] + 2 * p, G.n[0] + 2 * p);
+    }
+    CoreField field(int q) const { return CoreField{F[q], q == 4 ? pbc : bc[q]}; }
+    CoreField pp() const {
+        double *x, *b;
+        long sx, sy;
+        saras::mg_level0(mg, &x, &b, &sx, &sy);
+        return CoreField{x, ppbc};
+    }
+};
+
+namespace {
+
+saras_status rhs(saras_nse *s, int stage) {
+    saras::nse::StageCoef c{};
+    const double a = kAlpha[stage], b = kAlpha[stage];
+    c.zeta = kZeta[stage];
+    c.a_nu = a * s->p.nu;
+    c.a_kappa = a * s->p.kappa;
+    c.ab = a + b;
+    c.gamma = kGamma[stage];
+    c.dt = s->p.dt;
+    for (int q = 0; q < 3; ++q) c.buoy[q] = s->p.buoy[q];
+    c.stage = stage;
+    c.has_T = s->p.has_T;
+    c.nlscheme = s->p.nlscheme;
+    hipLaunchKernelGGL(saras::nse::k_rhs, s->grid3(), dim3(s->threads), 0, s->s, s->G, c, s->field(0),
+                       s->field(1), s->field(2), s->field(3), s->field(4), s->H[0], s->H[1], s->H[2], s->H[3],
+                       s->R[0], s->R[1], s->R[2], s->R[3]);
+    NSE_TRY(hipGetLastError());
+    return SARAS_OK;
+}
+
+// timestep::solveVx/Vy/Vz/solveT: Jacobi iterations until the residual of the
+// updated field is below cn_tol, at most maxIterations extra iterations.
+saras_status solve_field(saras_nse *s, int f, double beta, int *iters) {
+    const double diff = f == 3 ? s->p.kappa : s->p.nu;
+    const double dt = s->p.dt;
+    NSE_TRY(hipMemsetAsync(s->slots, 0, sizeof(unsigned long long) * (s->max_iter + 2), s->s));
+    double *cur = s->scratch, *other = s->F[f];
+    hipLaunchKernelGGL(saras::nse::k_jacobi<false>, s->grid3(), dim3(s->threads), 0, s->s, s->G,
+                       CoreField{s->F[f], s->bc[f]}, s->R[f], cur, dt, diff, beta, nullptr);
+    NSE_TRY(hipGetLastError());
+    int it = 0;
+    for (;;) {
+        hipLaunchKernelGGL(saras::nse::k_jacobi<true>, s->grid3(), dim3(s->threads), 0, s->s, s->G,
+                           CoreField{cur, s->bc[f]}, s->R[f], other, dt, diff, beta, s->slots + it);
+        NSE_TRY(hipGetLastError());
+        NSE_TRY(hipMemcpyAsync(s->slot_host, s->slots + it, sizeof(unsigned long long), hipMemcpyDeviceToHost, s->s));
+        NSE_TRY(hipStreamSynchronize(s->s));
+        double r;
+        std::memcpy(&r, s->slot_host, sizeof(double));
+        if (r < s->p.cn_tol) break;
+        it += 1;
+        if (it > s->max_iter) {
+            *iters = -1;
+            saras::set_last_error("Jacobi iterations not converging (predictors.cc:133-141)");
+            // leave the field at its latest iterate, as the reference's V holds it
+            if (cur != s->F[f]) {
+                s->scratch = s->F[f];
+                s->F[f] = cur;
+            }
+            return SARAS_JACOBI_DIVERGED;
+        }
+        std::swap(cur, other);
+    }
+    if (cur != s->F[f]) {
+        s->scratch = s->F[f];
+        s->F[f] = cur;
+    }
+    *iters = it;
+    return SARAS_OK;
+}
+
+saras_status stage_run(saras_nse *s, int stage) {
+    const double a = kAlpha[stage], b = kAlpha[stage];
+    saras_status e = rhs(s, stage);
+    if (e) return e;
+    const int nf = s->p.has_T ? 4 : 3;
+    for (int f = 0; f < nf; ++f) {
+        e = solve_field(s, f, b, &s->jac[f]);
+        if (e) return e;
+    }
+    double *x0, *b0;
+    long sx0, sy0;
+    saras::mg_level0(s->mg, &x0, &b0, &sx0, &sy0);
+    const double sc = 1.0 / ((a + b) * s->p.dt);
+    hipLaunchKernelGGL(saras::nse::k_div, s->grid3(), dim3(s->threads), 0, s->s, s->G, s->field(0), s->field(1),
+                       s->field(2), sc, b0, sx0, sy0);
+    NSE_TRY(hipGetLastError());
+    e = saras::mg_solve_resident(s->mg, s->s, &s->mg_stats);
+    if (e && e != SARAS_NOT_CONVERGED) return e;
+    const double pc = (a + b) * s->p.dt;
+    hipLaunchKernelGGL(saras::nse::k_correct, s->grid3(), dim3(s->threads), 0, s->s, s->G, s->pp(), s->F[4],
+                       s->F[0], s->F[1], s->F[2], pc);
+    NSE_TRY(hipGetLastError());
+    return SARAS_OK;
+}
+
+void fill_stats(saras_nse *s, saras_nse_stats *st, float ms) {
+    if (!st) return;
+    for (int q = 0; q < 4; ++q) st->jacobi_iters[q] = s->jac[q];
+    st->mg = s->mg_stats;
+    st->device_ms = ms;
+}
+
+saras_status ensure_stage(saras_nse *s) {
+    if (s->stage_a) return SARAS_OK;
+    NSE_TRY(hipMalloc((void **)&s->stage_a, s->full * sizeof(double)));
+    NSE_TRY(hipMalloc((void **)&s->stage_b, s->full * sizeof(double)));
+    return SARAS_OK;
+}
+
+FieldBC make_bc(const int *type, const double *val, const double *hwall) {
+    FieldBC b{};
+    for (int w = 0; w < 6; ++w) {
+        b.type[w] = type[w] == SARAS_FBC_NEUMANN ? 1 : 0;
+        b.val[w] = val[w];
+        b.kh[w] = val[w] * hwall[w];
+    }
+    return b;
+}
+
+// caller array (host or device, pad p) -> core buffer dst
+saras_status load_field(saras_nse *s, const double *src, int ptr_kind, double *dst) {
+    const double *d = src;
+    if (ptr_kind == SARAS_PTR_HOST) {
+        saras_status e = ensure_stage(s);
+        if (e) return e;
+        NSE_TRY(hipMemcpyAsync(s->stage_a, src, s->full * sizeof(double), hipMemcpyHostToDevice, s->s));
+        d = s->stage_a;
+    }
+    const int p = s->g.pad;
+    const long SY = s->g.n[2] + 2 * p, SX = SY * (s->g.n[1] + 2 * p);
+    hipLaunchKernelGGL(saras::nse::k_field_load, s->grid3(), dim3(s->threads), 0, s->s, d, SX, SY, p, dst, s->G.sx,
+                       s->G.sy, s->G.n[0], s->G.n[1], s->G.n[2]);
+    NSE_TRY(hipGetLastError());
+    if (ptr_kind == SARAS_PTR_HOST) NSE_TRY(hipStreamSynchronize(s->s));
+    return SARAS_OK;
+}
+
+__global__ void k_store_core(const double *__restrict__ src, long sx, long sy, double *__restrict__ dst, long SX,
+                             long SY, int p, int nz) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, i = blockIdx.z;
+    if (k >= nz) return;
+    dst[(long)(i + p) * SX + (long)(j + p) * SY + (k + p)] = src[(long)i * sx + (long)j * sy + k];
+}
+
+// core buffer (+ face ghosts unless core_only) -> caller array
+saras_status store_field(saras_nse *s, CoreField f, double *dst, int ptr_kind, bool core_only) {
+    double *d = dst;
+    if (ptr_kind == SARAS_PTR_HOST) {
+        saras_status e = ensure_stage(s);
+        if (e) return e;
+        NSE_TRY(hipMemcpyAsync(s->stage_a, dst, s->full * sizeof(double), hipMemcpyHostToDevice, s->s));
+        d = s->stage_a;
+    }
+    const int p = s->g.pad;
+    const long SY = s->g.n[2] + 2 * p, SX = SY * (s->g.n[1] + 2 * p);
+    if (core_only)
+        hipLaunchKernelGGL(k_store_core, s->grid3(), dim3(s->threads), 0, s->s, f.f, s->G.sx, s->G.sy, d, SX, SY, p,
+                           s->G.n[2]);
+    else
+        hipLaunchKernelGGL(saras::nse::k_field_store, s->grid3_full(p), dim3(s->threads), 0, s->s, s->G, f, d, SX,
+                           SY, p);
+    NSE_TRY(hipGetLastError());
+    if (ptr_kind == SARAS_PTR_HOST) {
+        NSE_TRY(hipMemcpyAsync(dst, s->stage_a, s->full * sizeof(double), hipMemcpyDeviceToHost, s->s));
+        NSE_TRY(hipStreamSynchronize(s->s));
+    }
+    return SARAS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+saras_status saras_nse_create(const saras_grid_desc *grid, const saras_mg_params *mgp,
+                              const saras_nse_params *params, saras_nse **out) {
+    if (!grid || !mgp || !params || !out) return SARAS_BAD_ARG;
+    *out = nullptr;
+    const saras_grid_desc &g = *grid;
+    const saras_nse_params &P = *params;
+    if (g.dim != 3) {
+        saras::set_last_error("the time-step path covers 3-D grids");
+        return SARAS_BAD_SHAPE;
+    }
+    if ((P.dscheme != 1 && P.dscheme != 2) || (P.nlscheme != 1 && P.nlscheme != 3) || g.pad < P.dscheme) {
+        saras::set_last_error("dscheme must be 1 or 2 with pad >= dscheme; nlscheme 1 or 3");
+        return SARAS_BAD_ARG;
+    }
+    for (int a = 0; a < 3; ++a)
+        if (g.n[a] < 4) return SARAS_BAD_SHAPE;
+    if (P.nlscheme == 3 && P.dscheme == 2 && !g.periodic[2] && g.n[1] != g.n[2]) {
+        saras::set_last_error("Morinishi with a z wall needs ny == nz (edge test on core.ubound(1), vfield.cc:434)");
+        return SARAS_BAD_SHAPE;
+    }
+    saras_nse *s = new saras_nse();
+    s->g = g;
+    s->p = P;
+    saras_mg_params mp = *mgp;
+    mp.bc = SARAS_BC_NEUMANN;
+    saras_status e = saras_mg_create(&g, &mp, &s->mg);
+    if (e) {
+        delete s;
+        return e;
+    }
+    s->s = saras::mg_stream(s->mg);
+    Geom &G = s->G;
+    for (int a = 0; a < 3; ++a) {
+        G.n[a] = g.n[a];
+        G.per[a] = g.periodic[a];
+    }
+    G.sy = pitch16(g.n[2]);
+    G.sx = G.sy * g.n[1];
+    {
+        double *x0, *b0;
+        long sx0, sy0;
+        saras::mg_level0(s->mg, &x0, &b0, &sx0, &sy0);
+        if (sx0 != G.sx || sy0 != G.sy) {
+            saras::set_last_error("pitch mismatch with the Poisson level-0 arrays");
+            delete s;
+            return SARAS_HIP_ERROR;
+        }
+    }
+    G.edge_last[0] = g.n[0] - 1;
+    G.edge_last[1] = g.n[1] - 1;
+    G.edge_last[2] = g.n[1] - 1;   // vfield.cc:434, sfield.cc:295
+    G.dscheme = P.dscheme;
+    double hwall[6];
+    std::vector<double> host(9 * (size_t)std::max(g.n[0], std::max(g.n[1], g.n[2])), 0.0);
+    const size_t stride = host.size() / 9;
+    for (int a = 0; a < 3; ++a) {
+        std::vector<double> coord, xi_x, xixx, xix2;
+        saras::field_axis(g, a, coord, xi_x, xixx, xix2);
+        const int p = g.pad, n = g.n[a];
+        for (int i = 0; i < n; ++i) {
+            host[(3 * a + 0) * stride + i] = xi_x[i + p];
+            host[(3 * a + 1) * stride + i] = xixx[i + p];
+            host[(3 * a + 2) * stride + i] = xix2[i + p];
+        }
+        hwall[2 * a] = 2 * coord[p];
+        hwall[2 * a + 1] = 2 * (coord[n - 1 + p] - g.len[a]);
+        // derivative.cc:52-54, vfield.cc:133-135, timestep.cc:156-173
+        const double dxi = 1.0 / double(n);
+        G.ih[a] = 1.0 / dxi;
+        G.ih2[a] = std::pow(G.ih[a], 2.0);
+        G.i2d[a] = 1.0 / (2.0 * dxi);
+        const double h2 = std::pow(dxi, 2.0);
+        G.tsi2h[a] = 0.5 / dxi;
+        G.tsih2[a] = 1.0 / h2;
+        if (a == 2) s->wall_dn = g.len[2] - coord[n - 1 + p];   // tseries.cc:124-128
+    }
+    auto fail = [&](saras_status st) {
+        delete s;
+        return st;
+    };
+    if (hipMalloc((void **)&s->metrics, host.size() * sizeof(double)) != hipSuccess) return fail(SARAS_OOM);
+    if (hipMemcpy(s->metrics, host.data(), host.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(SARAS_HIP_ERROR);
+    for (int a = 0; a < 3; ++a) {
+        G.xi_x[a] = s->metrics + (3 * a + 0) * stride;
+        G.xixx[a] = s->metrics + (3 * a + 1) * stride;
+        G.xix2[a] = s->metrics + (3 * a + 2) * stride;
+    }
+    for (int c = 0; c < 3; ++c) s->bc[c] = make_bc(P.vbc_type[c], P.vbc_val[c], hwall);
+    s->bc[3] = make_bc(P.tbc_type, P.tbc_val, hwall);
+    s->pbc = make_bc(P.pbc_type, P.pbc_val, hwall);
+    {
+        const int neu[6] = {1, 1, 1, 1, 1, 1};
+        const double zero[6] = {0, 0, 0, 0, 0, 0};
+        s->ppbc = make_bc(neu, zero, hwall);
+    }
+    s->elems = (size_t)G.sx * g.n[0];
+    const int nf = 5;
+    for (int q = 0; q < nf; ++q) {
+        if (hipMalloc((void **)&s->F[q], s->elems * sizeof(double)) != hipSuccess) return fail(SARAS_OOM);
+        (void)hipMemset(s->F[q], 0, s->elems * sizeof(double));
+    }
+    for (int q = 0; q < 4; ++q) {
+        if (hipMalloc((void **)&s->H[q], s->elems * sizeof(double)) != hipSuccess) return fail(SARAS_OOM);
+        if (hipMalloc((void **)&s->R[q], s->elems * sizeof(double)) != hipSuccess) return fail(SARAS_OOM);
+        (void)hipMemset(s->H[q], 0, s->elems * sizeof(double));
+        (void)hipMemset(s->R[q], 0, s->elems * sizeof(double));
+    }
+    if (hipMalloc((void **)&s->scratch, s->elems * sizeof(double)) != hipSuccess) return fail(SARAS_OOM);
+    // timestep.cc:115
+    s->max_iter = int(std::pow(std::log(double(g.n[0]) * g.n[1] * g.n[2]), 3));
+    if (hipMalloc((void **)&s->slots, sizeof(unsigned long long) * (s->max_iter + 2)) != hipSuccess)
+        return fail(SARAS_OOM);
+    if (hipHostMalloc((void **)&s->slot_host, sizeof(unsigned long long)) != hipSuccess) return fail(SARAS_OOM);
+    if (hipMalloc((void **)&s->diag_part, sizeof(double) * 5 * saras::nse::kDiagBlocks) != hipSuccess)
+        return fail(SARAS_OOM);
+    if (hipHostMalloc((void **)&s->diag_host, sizeof(double) * 5 * saras::nse::kDiagBlocks) != hipSuccess)
+        return fail(SARAS_OOM);
+    if (hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess) return fail(SARAS_HIP_ERROR);
+    s->full = (long)(g.n[0] + 2 * g.pad) * (g.n[1] + 2 * g.pad) * (g.n[2] + 2 * g.pad);
+    s->threads = (int)std::min<long>(256, std::max<long>(64, (g.n[2] + 63) / 64 * 64));
+    if (hipDeviceSynchronize() != hipSuccess) return fail(SARAS_HIP_ERROR);
+    *out = s;
+    return SARAS_OK;
+}
+
+saras_status saras_nse_destroy(saras_nse *s) {
+    if (!s) return SARAS_BAD_ARG;
+    (void)hipStreamSynchronize(s->s);
+    delete s;
+    return SARAS_OK;
+}
+
+saras_status saras_nse_set_fields(saras_nse *s, const double *vx, const double *vy, const double *vz,
+                                  const double *p, const double *t, const double *pp, int ptr_kind) {
+    if (!s || (ptr_kind != SARAS_PTR_HOST && ptr_kind != SARAS_PTR_DEVICE)) return SARAS_BAD_ARG;
+    const double *src[5] = {vx, vy, vz, t, p};
+    for (int q = 0; q < 5; ++q) {
+        if (!src[q]) continue;
+        saras_status e = load_field(s, src[q], ptr_kind, s->F[q]);
+        if (e) return e;
+    }
+    if (pp) {
+        double *x0, *b0;
+        long sx0, sy0;
+        saras::mg_level0(s->mg, &x0, &b0, &sx0, &sy0);
+        saras_status e = load_field(s, pp, ptr_kind, x0);
+        if (e) return e;
+    }
+    NSE_TRY(hipStreamSynchronize(s->s));
+    return SARAS_OK;
+}
+
+saras_status saras_nse_get_fields(saras_nse *s, double *vx, double *vy, double *vz, double *p, double *t, double *pp,
+                                  int ptr_kind) {
+    if (!s || (ptr_kind != SARAS_PTR_HOST && ptr_kind != SARAS_PTR_DEVICE)) return SARAS_BAD_ARG;
+    double *dst[5] = {vx, vy, vz, t, p};
+    for (int q = 0; q < 5; ++q) {
+        if (!dst[q]) continue;
+        saras_status e = store_field(s, s->field(q), dst[q], ptr_kind, false);
+        if (e) return e;
+    }
+    if (pp) {
+        saras_status e = store_field(s, s->pp(), pp, ptr_kind, false);
+        if (e) return e;
+    }
+    NSE_TRY(hipStreamSynchronize(s->s));
+    return SARAS_OK;
+}
+
+saras_status saras_nse_rhs(saras_nse *s, int stage) {
+    if (!s || stage < 0 || stage > 2) return SARAS_BAD_ARG;
+    saras_status e = rhs(s, stage);
+    if (e) return e;
+    NSE_TRY(hipStreamSynchronize(s->s));
+    return SARAS_OK;
+}
+
+saras_status saras_nse_get_rhs(saras_nse *s, double *rx, double *ry, double *rz, double *rt, double *hx, double *hy,
+                               double *hz, double *ht, int ptr_kind) {
+    if (!s) return SARAS_BAD_ARG;
+    double *dst[8] = {rx, ry, rz, rt, hx, hy, hz, ht};
+    for (int q = 0; q < 8; ++q) {
+        if (!dst[q]) continue;
+        double *src = q < 4 ? s->R[q] : s->H[q - 4];
+        saras_status e = store_field(s, CoreField{src, s->bc[0]}, dst[q], ptr_kind, true);
+        if (e) return e;
+    }
+    return SARAS_OK;
+}
+
+saras_status saras_nse_solve_field(saras_nse *s, int field, double beta, int *iters) {
+    if (!s || field < 0 || field > 3 || (field == 3 && !s->p.has_T)) return SARAS_BAD_ARG;
+    int it = 0;
+    saras_status e = solve_field(s, field, beta, &it);
+    s->jac[field] = it;
+    if (iters) *iters = it;
+    return e;
+}
+
+saras_status saras_nse_stage(saras_nse *s, int stage, saras_nse_stats *st) {
+    if (!s || stage < 0 || stage > 2) return SARAS_BAD_ARG;
+    NSE_TRY(hipEventRecord(s->ev0, s->s));
+    saras_status e = stage_run(s, stage);
+    NSE_TRY(hipEventRecord(s->ev1, s->s));
+    NSE_TRY(hipStreamSynchronize(s->s));
+    float ms = 0.f;
+    NSE_TRY(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+    fill_stats(s, st, ms);
+    return e;
+}
+
+saras_status saras_nse_step(saras_nse *s, saras_nse_stats *st) {
+    if (!s) return SARAS_BAD_ARG;
+    NSE_TRY(hipEventRecord(s->ev0, s->s));
+    saras_status e = SARAS_OK;
+    for (int k = 0; k < 3 && !e; ++k) e = stage_run(s, k);
+    NSE_TRY(hipEventRecord(s->ev1, s->s));
+    NSE_TRY(hipStreamSynchronize(s->s));
+    float ms = 0.f;
+    NSE_TRY(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+    fill_stats(s, st, ms);
+    return e;
+}
+
+saras_status saras_nse_bench_steps(saras_nse *s, int count, double *ms) {
+    if (!s || count < 0 || !ms) return SARAS_BAD_ARG;
+    NSE_TRY(hipStreamSynchronize(s->s));
+    NSE_TRY(hipEventRecord(s->ev0, s->s));
+    saras_status e = SARAS_OK;
+    for (int c = 0; c < count && !e; ++c)
+        for (int k = 0; k < 3 && !e; ++k) e = stage_run(s, k);
+    NSE_TRY(hipEventRecord(s->ev1, s->s));
+    NSE_TRY(hipStreamSynchronize(s->s));
+    float t = 0.f;
+    NSE_TRY(hipEventElapsedTime(&t, s->ev0, s->ev1));
+    *ms = t;
+    return e;
+}
+
+saras_status saras_field_op(saras_nse *s, int op, const double *f, double *out, int ptr_kind) {
+    if (!s || !f || !out || op < 0 || op > 7) return SARAS_BAD_ARG;
+    if (ptr_kind != SARAS_PTR_HOST && ptr_kind != SARAS_PTR_DEVICE) return SARAS_BAD_ARG;
+    const int p = s->g.pad;
+    const long SY = s->g.n[2] + 2 * p, SX = SY * (s->g.n[1] + 2 * p);
+    const double *df = f;
+    double *dout = out;
+    if (ptr_kind == SARAS_PTR_HOST) {
+        saras_status e = ensure_stage(s);
+        if (e) return e;
+        NSE_TRY(hipMemcpyAsync(s->stage_a, f, s->full * sizeof(double), hipMemcpyHostToDevice, s->s));
+        NSE_TRY(hipMemcpyAsync(s->stage_b, out, s->full * sizeof(double), hipMemcpyHostToDevice, s->s));
+        df = s->stage_a;
+        dout = s->stage_b;
+    }
+    PadField pf{df, SX, SY, p};
+    const CoreField v0 = s->field(0), v1 = s->field(1), v2 = s->field(2);
+    const dim3 gr = s->grid3(), bl(s->threads);
+    switch (op) {
+        case 0: hipLaunchKernelGGL(saras::nse::k_field_op<0>, gr, bl, 0, s->s, s->G, pf, v0, v1, v2, s->p.nlscheme, dout, SX, SY, p); break;
+        case 1: hipLaunchKernelGGL(saras::nse::k_field_op<1>, gr, bl, 0, s->s, s->G, pf, v0, v1, v2, s->p.nlscheme, dout, SX, SY, p); break;
+        case 2: hipLaunchKernelGGL(saras::nse::k_field_op<2>, gr, bl, 0, s->s, s->G, pf, v0, v1, v2, s->p.nlscheme, dout, SX, SY, p); break;
+        case 3: hipLaunchKernelGGL(saras::nse::k_field_op<3>, gr, bl, 0, s->s, s->G, pf, v0, v1, v2, s->p.nlscheme, dout, SX, SY, p); break;
+        case 4: hipLaunchKernelGGL(saras::nse::k_field_op<4>, gr, bl, 0, s->s, s->G, pf, v0, v1, v2, s->p.nlscheme, dout, SX, SY, p); break;
+        case 5: hipLaunchKernelGGL(saras::nse::k_field_op<5>, gr, bl, 0, s->s, s->G, pf, v0, v1, v2, s->p.nlscheme, dout, SX, SY, p); break;
+        case 6: hipLaunchKernelGGL(saras::nse::k_field_op<6>, gr, bl, 0, s->s, s->G, pf, v0, v1, v2, s->p.nlscheme, dout, SX, SY, p); break;
+        default: hipLaunchKernelGGL(saras::nse::k_field_op<7>, gr, bl, 0, s->s, s->G, pf, v0, v1, v2, s->p.nlscheme, dout, SX, SY, p); break;
+    }
+    NSE_TRY(hipGetLastError());
+    if (ptr_kind == SARAS_PTR_HOST)
+        NSE_TRY(hipMemcpyAsync(out, s->stage_b, s->full * sizeof(double), hipMemcpyDeviceToHost, s->s));
+    NSE_TRY(hipStreamSynchronize(s->s));
+    return SARAS_OK;
+}
+
+saras_status saras_nse_diag(saras_nse *s, double out[4]) {
+    if (!s || !out) return SARAS_BAD_ARG;
+    const long total = (long)s->G.n[0] * s->G.n[1] * s->G.n[2];
+    const int blocks = (int)std::min<long>(saras::nse::kDiagBlocks, (total + 255) / 256);
+    hipLaunchKernelGGL(saras::nse::k_diag, dim3(blocks), dim3(saras::nse::kDiagThreads), 0, s->s, s->G, s->field(0),
+                       s->field(1), s->field(2), s->field(3), s->p.has_T, 1.0 / double(s->G.n[0]),
+                       1.0 / double(s->G.n[1]), 1.0 / double(s->G.n[2]), s->wall_dn, s->diag_part);
+    NSE_TRY(hipGetLastError());
+    NSE_TRY(hipMemcpyAsync(s->diag_host, s->diag_part, sizeof(double) * 5 * blocks, hipMemcpyDeviceToHost, s->s));
+    NSE_TRY(hipStreamSynchronize(s->s));
+    double acc[5] = {0, 0, 0, 0, 0};
+    for (int b = 0; b < blocks; ++b) {
+        for (int q = 0; q < 5; ++q) {
+            const double v = s->diag_host[5 * b + q];
+            if (q == 2) acc[q] = (v > acc[q] || v != v) ? v : acc[q];
+            else acc[q] += v;
+        }
+    }
+    const double vol = s->g.len[0] * s->g.len[1] * s->g.len[2];
+    out[0] = acc[0] / vol;
+    out[1] = acc[2];
+    out[2] = s->p.has_T ? 1.0 + (acc[1] / vol) / s->p.kappa : 0.0;
+    out[3] = s->p.has_T ? (acc[3] + acc[4]) / (s->g.len[0] * s->g.len[1] * 2) : 0.0;
+    return SARAS_OK;
+}
+
+}  // extern "C"

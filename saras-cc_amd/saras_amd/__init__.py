@@ -46,6 +46,22 @@ class MgStats(C.Structure):
                 ("coarse_capped", C.c_int), ("device_ms", C.c_double)]
 
 
+FBC_DIRICHLET, FBC_NEUMANN = 0, 1
+
+
+class NseParams(C.Structure):
+    _fields_ = [("dscheme", C.c_int), ("nlscheme", C.c_int), ("has_T", C.c_int),
+                ("nu", C.c_double), ("kappa", C.c_double), ("dt", C.c_double),
+                ("cn_tol", C.c_double), ("buoy", C.c_double * 3),
+                ("vbc_type", (C.c_int * 6) * 3), ("vbc_val", (C.c_double * 6) * 3),
+                ("tbc_type", C.c_int * 6), ("tbc_val", C.c_double * 6),
+                ("pbc_type", C.c_int * 6), ("pbc_val", C.c_double * 6)]
+
+
+class NseStats(C.Structure):
+    _fields_ = [("jacobi_iters", C.c_int * 4), ("mg", MgStats), ("device_ms", C.c_double)]
+
+
 # every symbol include/saras_hip.h declares
 EXPORTS = [
     "saras_mg_create", "saras_mg_solve", "saras_mg_destroy", "saras_mg_levels",
@@ -54,7 +70,9 @@ EXPORTS = [
     "saras_mg_residual_restrict", "saras_mg_prolong_correct", "saras_mg_error",
     "saras_device_alloc", "saras_device_free", "saras_memcpy_h2d", "saras_memcpy_d2h",
     "saras_device_sync", "saras_last_error", "saras_mg_bench_cycles", "saras_mg_bench_sweeps",
-    "saras_bench_stream_copy",
+    "saras_bench_stream_copy", "saras_nse_create", "saras_nse_destroy", "saras_nse_set_fields",
+    "saras_nse_get_fields", "saras_nse_step", "saras_nse_stage", "saras_nse_rhs", "saras_nse_get_rhs",
+    "saras_nse_solve_field", "saras_field_op", "saras_nse_diag", "saras_nse_bench_steps",
 ]
 
 _lib = None
@@ -90,6 +108,19 @@ def lib():
         L.saras_device_free.argtypes = [vp]
         L.saras_memcpy_h2d.argtypes = [vp, vp, C.c_size_t]
         L.saras_memcpy_d2h.argtypes = [vp, vp, C.c_size_t]
+        L.saras_nse_create.argtypes = [C.POINTER(GridDesc), C.POINTER(MgParams), C.POINTER(NseParams),
+                                       C.POINTER(vp)]
+        L.saras_nse_destroy.argtypes = [vp]
+        L.saras_nse_set_fields.argtypes = [vp] + [vp] * 6 + [C.c_int]
+        L.saras_nse_get_fields.argtypes = [vp] + [vp] * 6 + [C.c_int]
+        L.saras_nse_step.argtypes = [vp, C.POINTER(NseStats)]
+        L.saras_nse_stage.argtypes = [vp, C.c_int, C.POINTER(NseStats)]
+        L.saras_nse_rhs.argtypes = [vp, C.c_int]
+        L.saras_nse_get_rhs.argtypes = [vp] + [vp] * 8 + [C.c_int]
+        L.saras_nse_solve_field.argtypes = [vp, C.c_int, C.c_double, C.POINTER(C.c_int)]
+        L.saras_field_op.argtypes = [vp, C.c_int, vp, vp, C.c_int]
+        L.saras_nse_diag.argtypes = [vp, dp]
+        L.saras_nse_bench_steps.argtypes = [vp, C.c_int, dp]
         _lib = L
     return _lib
 
@@ -276,3 +307,107 @@ def bench_stream_copy(n_doubles, reps):
     ms = C.c_double()
     check(lib().saras_bench_stream_copy(n_doubles, reps, C.byref(ms)))
     return ms.value
+
+
+def nse_params(dscheme=2, nlscheme=3, has_T=1, nu=1e-2, kappa=1e-2, dt=1e-3, cn_tol=1e-6,
+               buoy=(0.0, 0.0, -1.0), vbc=None, tbc=None, pbc=None):
+    """Time-step parameters.  vbc: [(types[6], vals[6]) per component];
+    tbc / pbc: (types[6], vals[6]).  Defaults: the Rayleigh-Benard wiring of
+    hydro.cc:287-485 / scalar.cc:129-201 (no-slip walls, T = 1 at the bottom
+    and 0 at the top, Neumann T on side walls, Neumann P)."""
+    d = NseParams()
+    d.dscheme, d.nlscheme, d.has_T = dscheme, nlscheme, int(has_T)
+    d.nu, d.kappa, d.dt, d.cn_tol = nu, kappa, dt, cn_tol
+    for a in range(3):
+        d.buoy[a] = buoy[a]
+    if vbc is None:
+        vbc = [([FBC_DIRICHLET] * 6, [0.0] * 6)] * 3
+    if tbc is None:
+        tbc = ([FBC_NEUMANN] * 4 + [FBC_DIRICHLET] * 2, [0.0] * 4 + [1.0, 0.0])
+    if pbc is None:
+        pbc = ([FBC_NEUMANN] * 6, [0.0] * 6)
+    for c in range(3):
+        for w in range(6):
+            d.vbc_type[c][w], d.vbc_val[c][w] = vbc[c][0][w], vbc[c][1][w]
+    for w in range(6):
+        d.tbc_type[w], d.tbc_val[w] = tbc[0][w], tbc[1][w]
+        d.pbc_type[w], d.pbc_val[w] = pbc[0][w], pbc[1][w]
+    return d
+
+
+def nse_stats_dict(st):
+    return dict(jacobi_iters=list(st.jacobi_iters), mg=stats_dict(st.mg), device_ms=st.device_ms)
+
+
+class Nse:
+    """lsRK3_d3 time stepper with its fields (V, P, T, Pp) on the device
+    (lsRK3_d3.cc:58-70, 190-305)."""
+
+    NAMES = ("vx", "vy", "vz", "p", "t", "pp")
+
+    def __init__(self, g, mg, p):
+        self.g, self.mgp, self.p = g, mg, p
+        self.h = C.c_void_p()
+        check(lib().saras_nse_create(C.byref(g), C.byref(mg), C.byref(p), C.byref(self.h)))
+        self.shape = (g.n[0] + 2 * g.pad, g.n[1] + 2 * g.pad, g.n[2] + 2 * g.pad)
+
+    def close(self):
+        if self.h:
+            lib().saras_nse_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set(self, **f):
+        arrs = [np.ascontiguousarray(f[k], dtype=np.float64) if f.get(k) is not None else None
+                for k in self.NAMES]
+        for a in arrs:
+            assert a is None or a.shape == self.shape
+        check(lib().saras_nse_set_fields(self.h, *[a.ctypes.data if a is not None else None for a in arrs],
+                                         PTR_HOST))
+
+    def get(self, base=None):
+        out = [np.zeros(self.shape) if base is None else np.array(base, dtype=np.float64) for _ in self.NAMES]
+        check(lib().saras_nse_get_fields(self.h, *[a.ctypes.data for a in out], PTR_HOST))
+        return dict(zip(self.NAMES, out))
+
+    def rhs(self, stage):
+        check(lib().saras_nse_rhs(self.h, stage))
+        out = [np.zeros(self.shape) for _ in range(8)]
+        check(lib().saras_nse_get_rhs(self.h, *[a.ctypes.data for a in out], PTR_HOST))
+        return dict(zip(("rx", "ry", "rz", "rt", "hx", "hy", "hz", "ht"), out))
+
+    def stage(self, stage):
+        st = NseStats()
+        s = lib().saras_nse_stage(self.h, stage, C.byref(st))
+        return s, nse_stats_dict(st)
+
+    def step(self):
+        st = NseStats()
+        s = lib().saras_nse_step(self.h, C.byref(st))
+        return s, nse_stats_dict(st)
+
+    def solve_field(self, field, beta):
+        it = C.c_int()
+        s = lib().saras_nse_solve_field(self.h, field, beta, C.byref(it))
+        return s, it.value
+
+    def op(self, op, F, out=None):
+        F = np.ascontiguousarray(F, dtype=np.float64)
+        out = np.zeros(self.shape) if out is None else np.array(out, dtype=np.float64)
+        check(lib().saras_field_op(self.h, op, F.ctypes.data, out.ctypes.data, PTR_HOST))
+        return out
+
+    def diag(self):
+        v = np.zeros(4)
+        check(lib().saras_nse_diag(self.h, _dp(v)))
+        return dict(ke=v[0], divmax=v[1], nuvol=v[2], nuwall=v[3])
+
+    def bench_steps(self, count):
+        ms = C.c_double()
+        s = lib().saras_nse_bench_steps(self.h, count, C.byref(ms))
+        return s, ms.value

@@ -67,10 +67,15 @@ struct CoreField {
             c[ax] = q < 0 ? q + n : q - n;
             return f[(long)c[0] * G.sx + (long)c[1] * G.sy + c[2]];
         }
-        const int w = 2 * ax + (q >= n ? 1 : 0);
-        c[ax] = q < 0 ? 0 : n - 1;
+        // select between the two walls of this axis without a run-time index
+        // into the rule table (keeps the table in kernel-argument memory)
+        const bool hi = q >= n;
+        const int type = hi ? bc.type[2 * ax + 1] : bc.type[2 * ax];
+        const double val = hi ? bc.val[2 * ax + 1] : bc.val[2 * ax];
+        const double kh = hi ? bc.kh[2 * ax + 1] : bc.kh[2 * ax];
+        c[ax] = hi ? n - 1 : 0;
         const double inner = f[(long)c[0] * G.sx + (long)c[1] * G.sy + c[2]];
-        return bc.type[w] == 0 ? 2.0 * bc.val[w] - inner : inner - bc.kh[w];
+        return type == 0 ? 2.0 * val - inner : inner - kh;
     }
 };
 
@@ -200,60 +205,63 @@ struct StageCoef {
     int nlscheme;
 };
 
-// The explicit part of one RK3 sub-stage for every field
-// (lsRK3_d3.cc:199-267): R = (zeta H_old + alpha nu lap u + (alpha+beta)(f - grad P)
+// The explicit part of one RK3 sub-stage (lsRK3_d3.cc:199-267) for velocity
+// component C: R = (zeta H_old + alpha nu lap u + (alpha+beta)(f - grad P)
 // + gamma H) dt + u, with H the new non-linear term, written back for the
-// next sub-stage.
-__global__ void __launch_bounds__(256) k_rhs(Geom G, StageCoef c, CoreField V0, CoreField V1, CoreField V2,
-                                             CoreField T, CoreField P, double *__restrict__ H0,
-                                             double *__restrict__ H1, double *__restrict__ H2,
-                                             double *__restrict__ HT, double *__restrict__ R0,
-                                             double *__restrict__ R1, double *__restrict__ R2,
-                                             double *__restrict__ RT) {
+// next sub-stage.  One launch per component keeps the register footprint of
+// the Morinishi stencils spill-free.
+template <int C>
+__global__ void __launch_bounds__(256) k_rhs_v(Geom G, StageCoef c, CoreField V0, CoreField V1, CoreField V2,
+                                               CoreField T, CoreField P, double *__restrict__ H,
+                                               double *__restrict__ R) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
     const int i = blockIdx.z;
     if (k >= G.n[2]) return;
     const long idx = (long)i * G.sx + (long)j * G.sy + k;
     const CoreField V[3] = {V0, V1, V2};
-    double *Hs[3] = {H0, H1, H2};
-    double *Rs[3] = {R0, R1, R2};
-    const double tc = c.has_T ? T.at(G, i, j, k) : 0.0;
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-        double nse = 0.0;
-        if (c.stage > 0) nse += c.zeta * Hs[q][idx];
-        const double diff = laplacian(G, V[q], i, j, k);
-        nse += c.a_nu * diff;
-        double f = 0.0;
-        if (c.has_T) f -= c.buoy[q] * tc;
-        f -= d1(G, P, q, i, j, k);
-        nse += c.ab * f;
-        const double h = nlin(G, c.nlscheme, V, V[q], i, j, k);
-        nse += c.gamma * h;
-        nse *= c.dt;
-        nse += V[q].at(G, i, j, k);
-        Hs[q][idx] = h;
-        Rs[q][idx] = nse;
-    }
-    if (c.has_T) {
-        double tr = 0.0;
-        if (c.stage > 0) tr += c.zeta * HT[idx];
-        const double diff = laplacian(G, T, i, j, k);
-        tr += c.a_kappa * diff;
-        tr += c.ab * 0.0;   // tempSF forcing of T is zero
-        const double h = nlin(G, c.nlscheme, V, T, i, j, k);
-        tr += c.gamma * h;
-        tr *= c.dt;
-        tr += tc;
-        HT[idx] = h;
-        RT[idx] = tr;
-    }
+    const CoreField &U = V[C];
+    double nse = 0.0;
+    if (c.stage > 0) nse += c.zeta * H[idx];
+    const double diff = laplacian(G, U, i, j, k);
+    nse += c.a_nu * diff;
+    double f = 0.0;
+    if (c.has_T) f -= c.buoy[C] * T.at(G, i, j, k);
+    f -= d1(G, P, C, i, j, k);
+    nse += c.ab * f;
+    const double h = nlin(G, c.nlscheme, V, U, i, j, k);
+    nse += c.gamma * h;
+    nse *= c.dt;
+    nse += U.at(G, i, j, k);
+    H[idx] = h;
+    R[idx] = nse;
 }
 
-// Block max of non-negative doubles, published with one 64-bit atomic max
-// on the bit pattern (IEEE order equals unsigned order for x >= 0).
-__device__ __forceinline__ void block_max_publish(double v, unsigned long long *slot) {
+// the same for the scalar (tmpRHS / tempSF); its forcing term is zero
+__global__ void __launch_bounds__(256) k_rhs_t(Geom G, StageCoef c, CoreField V0, CoreField V1, CoreField V2,
+                                               CoreField T, double *__restrict__ H, double *__restrict__ R) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    const int i = blockIdx.z;
+    if (k >= G.n[2]) return;
+    const long idx = (long)i * G.sx + (long)j * G.sy + k;
+    const CoreField V[3] = {V0, V1, V2};
+    double tr = 0.0;
+    if (c.stage > 0) tr += c.zeta * H[idx];
+    const double diff = laplacian(G, T, i, j, k);
+    tr += c.a_kappa * diff;
+    tr += c.ab * 0.0;
+    const double h = nlin(G, c.nlscheme, V, T, i, j, k);
+    tr += c.gamma * h;
+    tr *= c.dt;
+    tr += T.at(G, i, j, k);
+    H[idx] = h;
+    R[idx] = tr;
+}
+
+// Block max of non-negative doubles (NaN wins, as in a failed comparison
+// chain) written to part[block].
+__device__ __forceinline__ void block_max_store(double v, double *part, long block) {
     __shared__ double red[4];
     for (int o = 32; o > 0; o >>= 1) {
         const double w = __shfl_xor(v, o, 64);
@@ -265,7 +273,28 @@ __device__ __forceinline__ void block_max_publish(double v, unsigned long long *
     if (threadIdx.x == 0) {
         double m = red[0];
         for (int q = 1; q < (int)(blockDim.x >> 6); ++q) m = (red[q] > m || red[q] != red[q]) ? red[q] : m;
-        atomicMax(slot, (unsigned long long)__double_as_longlong(m));
+        part[block] = m;
+    }
+}
+
+// max over n partials -> out[0] (one block of 1024 threads)
+__global__ void __launch_bounds__(1024) k_max_final(const double *__restrict__ part, long n, double *__restrict__ out) {
+    __shared__ double red[16];
+    double v = 0.0;
+    for (long q = threadIdx.x; q < n; q += blockDim.x) {
+        const double w = part[q];
+        v = (w > v || w != w) ? w : v;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const double w = __shfl_xor(v, o, 64);
+        v = (w > v || w != w) ? w : v;
+    }
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = red[0];
+        for (int q = 1; q < (int)(blockDim.x >> 6); ++q) m = (red[q] > m || red[q] != red[q]) ? red[q] : m;
+        out[0] = m;
     }
 }
 
@@ -275,7 +304,7 @@ __device__ __forceinline__ void block_max_publish(double v, unsigned long long *
 template <bool RES>
 __global__ void __launch_bounds__(256) k_jacobi(Geom G, CoreField F, const double *__restrict__ rhs,
                                                 double *__restrict__ out, double dt, double diff, double beta,
-                                                unsigned long long *slot) {
+                                                double *__restrict__ part) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
     const int i = blockIdx.z;
@@ -303,7 +332,7 @@ __global__ void __launch_bounds__(256) k_jacobi(Geom G, CoreField F, const doubl
             res = fabs(t - r);
         }
     }
-    if (RES) block_max_publish(res, slot);
+    if (RES) block_max_store(res, part, ((long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x);
 }
 
 // vfield::divergence (vfield.cc:721-737) scaled into the Poisson RHS:

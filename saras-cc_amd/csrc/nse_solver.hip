@@ -64,8 +64,10 @@ struct saras_nse {
     double *R[4] = {nullptr, nullptr, nullptr, nullptr};
     double *scratch = nullptr;
     size_t elems = 0;
-    unsigned long long *slots = nullptr;
-    unsigned long long *slot_host = nullptr;
+    double *part = nullptr;             // per-block residual maxima of a Jacobi launch
+    long nblocks = 0;
+    double *res_dev = nullptr;          // device [1]
+    double *res_host = nullptr;         // pinned [1]
     int max_iter = 0;
     double *diag_part = nullptr, *diag_host = nullptr;
     double *stage_a = nullptr, *stage_b = nullptr;
@@ -83,8 +85,9 @@ struct saras_nse {
         for (auto *q : R) (void)hipFree(q);
         (void)hipFree(scratch);
         (void)hipFree(metrics);
-        (void)hipFree(slots);
-        (void)hipHostFree(slot_host);
+        (void)hipFree(part);
+        (void)hipFree(res_dev);
+        (void)hipHostFree(res_host);
         (void)hipFree(diag_part);
         (void)hipHostFree(diag_host);
         (void)hipFree(stage_a);
@@ -122,9 +125,16 @@ saras_status rhs(saras_nse *s, int stage) {
     c.stage = stage;
     c.has_T = s->p.has_T;
     c.nlscheme = s->p.nlscheme;
-    hipLaunchKernelGGL(saras::nse::k_rhs, s->grid3(), dim3(s->threads), 0, s->s, s->G, c, s->field(0),
-                       s->field(1), s->field(2), s->field(3), s->field(4), s->H[0], s->H[1], s->H[2], s->H[3],
-                       s->R[0], s->R[1], s->R[2], s->R[3]);
+    const dim3 gr = s->grid3(), bl(s->threads);
+    hipLaunchKernelGGL(saras::nse::k_rhs_v<0>, gr, bl, 0, s->s, s->G, c, s->field(0), s->field(1), s->field(2),
+                       s->field(3), s->field(4), s->H[0], s->R[0]);
+    hipLaunchKernelGGL(saras::nse::k_rhs_v<1>, gr, bl, 0, s->s, s->G, c, s->field(0), s->field(1), s->field(2),
+                       s->field(3), s->field(4), s->H[1], s->R[1]);
+    hipLaunchKernelGGL(saras::nse::k_rhs_v<2>, gr, bl, 0, s->s, s->G, c, s->field(0), s->field(1), s->field(2),
+                       s->field(3), s->field(4), s->H[2], s->R[2]);
+    if (s->p.has_T)
+        hipLaunchKernelGGL(saras::nse::k_rhs_t, gr, bl, 0, s->s, s->G, c, s->field(0), s->field(1), s->field(2),
+                           s->field(3), s->H[3], s->R[3]);
     NSE_TRY(hipGetLastError());
     return SARAS_OK;
 }
@@ -134,7 +144,6 @@ saras_status rhs(saras_nse *s, int stage) {
 saras_status solve_field(saras_nse *s, int f, double beta, int *iters) {
     const double diff = f == 3 ? s->p.kappa : s->p.nu;
     const double dt = s->p.dt;
-    NSE_TRY(hipMemsetAsync(s->slots, 0, sizeof(unsigned long long) * (s->max_iter + 2), s->s));
     double *cur = s->scratch, *other = s->F[f];
     hipLaunchKernelGGL(saras::nse::k_jacobi<false>, s->grid3(), dim3(s->threads), 0, s->s, s->G,
                        CoreField{s->F[f], s->bc[f]}, s->R[f], cur, dt, diff, beta, nullptr);
@@ -142,12 +151,12 @@ saras_status solve_field(saras_nse *s, int f, double beta, int *iters) {
     int it = 0;
     for (;;) {
         hipLaunchKernelGGL(saras::nse::k_jacobi<true>, s->grid3(), dim3(s->threads), 0, s->s, s->G,
-                           CoreField{cur, s->bc[f]}, s->R[f], other, dt, diff, beta, s->slots + it);
+                           CoreField{cur, s->bc[f]}, s->R[f], other, dt, diff, beta, s->part);
+        hipLaunchKernelGGL(saras::nse::k_max_final, dim3(1), dim3(1024), 0, s->s, s->part, s->nblocks, s->res_dev);
         NSE_TRY(hipGetLastError());
-        NSE_TRY(hipMemcpyAsync(s->slot_host, s->slots + it, sizeof(unsigned long long), hipMemcpyDeviceToHost, s->s));
+        NSE_TRY(hipMemcpyAsync(s->res_host, s->res_dev, sizeof(double), hipMemcpyDeviceToHost, s->s));
         NSE_TRY(hipStreamSynchronize(s->s));
-        double r;
-        std::memcpy(&r, s->slot_host, sizeof(double));
+        const double r = s->res_host[0];
         if (r < s->p.cn_tol) break;
         it += 1;
         if (it > s->max_iter) {
@@ -385,16 +394,20 @@ saras_status saras_nse_create(const saras_grid_desc *grid, const saras_mg_params
     if (hipMalloc((void **)&s->scratch, s->elems * sizeof(double)) != hipSuccess) return fail(SARAS_OOM);
     // timestep.cc:115
     s->max_iter = int(std::pow(std::log(double(g.n[0]) * g.n[1] * g.n[2]), 3));
-    if (hipMalloc((void **)&s->slots, sizeof(unsigned long long) * (s->max_iter + 2)) != hipSuccess)
-        return fail(SARAS_OOM);
-    if (hipHostMalloc((void **)&s->slot_host, sizeof(unsigned long long)) != hipSuccess) return fail(SARAS_OOM);
+    s->threads = (int)std::min<long>(256, std::max<long>(64, (g.n[2] + 63) / 64 * 64));
+    {
+        const dim3 gr = s->grid3();
+        s->nblocks = (long)gr.x * gr.y * gr.z;
+    }
+    if (hipMalloc((void **)&s->part, sizeof(double) * s->nblocks) != hipSuccess) return fail(SARAS_OOM);
+    if (hipMalloc((void **)&s->res_dev, sizeof(double)) != hipSuccess) return fail(SARAS_OOM);
+    if (hipHostMalloc((void **)&s->res_host, sizeof(double)) != hipSuccess) return fail(SARAS_OOM);
     if (hipMalloc((void **)&s->diag_part, sizeof(double) * 5 * saras::nse::kDiagBlocks) != hipSuccess)
         return fail(SARAS_OOM);
     if (hipHostMalloc((void **)&s->diag_host, sizeof(double) * 5 * saras::nse::kDiagBlocks) != hipSuccess)
         return fail(SARAS_OOM);
     if (hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess) return fail(SARAS_HIP_ERROR);
     s->full = (long)(g.n[0] + 2 * g.pad) * (g.n[1] + 2 * g.pad) * (g.n[2] + 2 * g.pad);
-    s->threads = (int)std::min<long>(256, std::max<long>(64, (g.n[2] + 63) / 64 * 64));
     if (hipDeviceSynchronize() != hipSuccess) return fail(SARAS_HIP_ERROR);
     *out = s;
     return SARAS_OK;

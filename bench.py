@@ -19,6 +19,12 @@ FP64 vs mixed precision, inputs already resident in HBM.
 roofline: the dominant kernel (finest-level fused sweep), algorithmic
 3 words per cell per launch / average launch time (HIP events).
 cpu_baseline: the FP64 CPU oracle's V-cycle on the same workload.
+
+rbc_256_step: the 256^3 Rayleigh-Benard time step (Ra 1e7, Pr 1, z-tanh 1.5,
+dScheme 2, Morinishi, RK3 with Jacobi CN, V(3,6) pressure solves of 20
+cycles): device time per full step, FP64 and with mixed-precision pressure
+solves.  "Solve Coarsest" is off: with it on, the reference algorithm's
+pressure solve diverges on this stretched Neumann grid in the first stage.
 """
 import argparse
 import json
@@ -35,6 +41,12 @@ import saras_amd as S  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 N = 256
+DOMINANT = "k_sweep_march<float,32,16,true>"
+# HBM bytes per launch of the dominant kernel at 256^3 from rocprofv3 --pmc
+# FETCH_SIZE / WRITE_SIZE (KiB per dispatch), calibrated against the copy
+# kernel (FETCH x 2.0, WRITE x 1.0): (2 * 72734.1 + 65536.0) KiB.
+# Source: profiles/march_hbm_traffic_256cubed.md
+DOMINANT_TRAFFIC_BYTES = (2 * 72734.1 + 65536.0) * 1024
 
 
 def bytes_per_vcycle(shapes, s, nu1, nu2):
@@ -143,6 +155,46 @@ def solve_compare(nu1, nu2, tol=1e-10):
     return res
 
 
+def rbc_fields(n, pad, seed=2):
+    """T = 1 - z + U(-1e-3, 1e-3) (seeded), u = 0, P = 1, Pp = 0 on the z-tanh grid."""
+    sh = (n + 2 * pad,) * 3
+    g = S.grid_desc(n, periodic=(1, 1, 0), gtype=(0, 0, S.GRID_TANH_DOUBLE), beta=(1, 1, 1.5), pad=pad)
+    z = S.level_metrics(g, 0, 2)["x"][1:-1]          # cell centres, grid.cc:411-508
+    rng = np.random.default_rng(seed)
+    T = np.zeros(sh)
+    c = (slice(pad, -pad),) * 3
+    T[c] = 1.0 - z[None, None, :] + rng.uniform(-1e-3, 1e-3, (n, n, n))
+    zero = np.zeros(sh)
+    return g, dict(vx=zero, vy=zero, vz=zero, p=np.ones(sh), t=T, pp=zero)
+
+
+def rbc_step(steps, warmup, n=256):
+    pad = 2
+    g, f = rbc_fields(n, pad)
+    nu = (1.0 / 1e7) ** 0.5
+    prm = S.nse_params(dscheme=2, nlscheme=3, has_T=1, nu=nu, kappa=nu, dt=1e-3, cn_tol=1e-6,
+                       buoy=(0.0, 0.0, -1.0))
+    out = {}
+    for name, prec in (("fp64", S.PREC_FP64), ("mixed", S.PREC_MIXED)):
+        mg = S.mg_params(vc_count=20, vc_tol=1e-6, solve_flag=0, mg_tol=1e-2, pre=3, post=6,
+                         bc=S.BC_NEUMANN, precision=prec, inner_cycles=2, max_refine=10)
+        s = S.Nse(g, mg, prm)
+        s.set(**f)
+        for _ in range(warmup):
+            r, st = s.step()
+            S.check(r)
+        r, ms = s.bench_steps(steps)
+        S.check(r)
+        r, st = s.step()
+        S.check(r)
+        d = s.diag()
+        out[name] = dict(ms_per_step=round(ms / steps, 3), steps=steps, jacobi_iters=st["jacobi_iters"],
+                         pressure_cycles=st["mg"]["cycles"], pressure_residual=st["mg"]["residual"],
+                         ke=d["ke"], divmax=d["divmax"], nu_vol=d["nuvol"], nu_wall=d["nuwall"])
+        s.close()
+    return out
+
+
 def cpu_baseline(nu1, nu2, shapes, budget_s=12.0):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -178,6 +230,9 @@ def main():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--profile-sweep", action="store_true",
                     help="only finest-level sweeps (FP32 and FP64) and the copy kernel, for counter runs")
+    ap.add_argument("--rbc-only", action="store_true",
+                    help="only the 256^3 Rayleigh-Benard time steps (warmup + steps, FP64 and mixed)")
+    ap.add_argument("--no-rbc", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build measures the single-GPU path "
@@ -187,6 +242,9 @@ def main():
     nu1 = nu2 = 2
     if args.profile_sweep:
         profile_sweeps(args, nu1, nu2)
+        return
+    if args.rbc_only:
+        print(json.dumps({"rbc_256_step": rbc_step(args.steps, args.warmup)}))
         return
     rates = cycle_rates(args, nu1, nu2)
     head, f64 = rates["mixed"], rates["fp64"]
@@ -212,8 +270,8 @@ def main():
         "config": {"workload": "poisson_vcycle_256^3_PPN_z-tanh1.5_neumann_V(2,2)", "cells": [N, N, N],
                    "levels": len(head["shapes"]), "cycle_precision": "fp32 inner cycle of the mixed solver"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_sweep_march<float,64,8,true>", "launch_ms": round(head["launch_ms"], 5),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": round(DOMINANT_TRAFFIC_BYTES),
+                     "kernel": DOMINANT, "launch_ms": round(head["launch_ms"], 5),
                      "bytes_per_launch": head["launch_bytes"]},
         "fp64_vcycle": {"ms": round(f64["ms"], 4),
                         "tbps": round(f64["bv"] / (f64["ms"] * 1e-3) / 1e12, 4),
@@ -230,6 +288,8 @@ def main():
             "mixed_refinements": sc["mixed"]["refinements"], "mixed_residual": sc["mixed"]["residual"],
             "mixed_converged": sc["mixed"]["converged"],
             "speedup": round(sc["fp64"]["ms"] / sc["mixed"]["ms"], 3)}
+    if not args.no_rbc:
+        out["rbc_256_step"] = rbc_step(3, 1)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(nu1, nu2, f64["shapes"])
     print(json.dumps(out))

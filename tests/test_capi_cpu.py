@@ -64,3 +64,13 @@ def test_bad_shape_is_reported_not_aborted():
     st = S.lib().saras_mg_create(C.byref(g), C.byref(p), C.byref(h))
     assert st == S.BAD_SHAPE
     assert b"cells" in S.lib().saras_last_error()
+
+
+def test_dropin_program_links_against_the_library():
+    """The C++ drop-in classes (include/saras_dropin.hpp) compile and link;
+    without arguments the program prints its usage before any device call."""
+    import subprocess
+    exe = os.path.join(ROOT, "tests", "dropin", "_build", "dropin_run")
+    assert os.path.exists(exe), "run __graft_entry__.build() first"
+    r = subprocess.run([exe], capture_output=True, text=True)
+    assert r.returncode == 2 and "usage" in r.stderr

@@ -1,0 +1,51 @@
+"""The C++ drop-in classes driven by a C++ caller (tests/dropin/dropin_run.cpp):
+one multigrid_d3::mgSolve on a caller array and two lsRK3_d3::timeAdvance
+steps of a 16^3 Rayleigh-Benard run, compared bit for bit with the oracle."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "tests", "dropin", "_build", "dropin_run")
+
+
+def test_dropin_matches_oracle(tmp_path):
+    n, pad, steps = 16, 2, 2
+    sh = (n + 2 * pad,) * 3
+    c = (slice(pad, -pad),) * 3
+    rng = np.random.default_rng(21)
+    b = np.zeros(sh)
+    b[c] = rng.standard_normal((n, n, n))
+    g = O.grid_desc(n, periodic=(1, 1, 0), gtype=(0, 0, 2), beta=(1, 1, 1.5), pad=pad)
+    z = O.field_metrics(g, 2)["x"]
+    T = np.zeros(sh)
+    T[:] = 1.0 - z[None, None, :]
+    T[c] += rng.uniform(-1e-3, 1e-3, (n, n, n))
+    inp, out = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(inp, "wb") as f:
+        f.write(b.tobytes())
+        f.write(T.tobytes())
+    r = subprocess.run([EXE, str(n), str(steps), str(inp), str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    res = np.fromfile(out, dtype=np.float64).reshape(6, *sh)
+
+    mk = dict(vc_count=20, vc_tol=1e-6, solve_flag=0, mg_tol=1e-2, pre=3, post=6)
+    mo = O.Multigrid(g, O.mg_params(bc=O.BC_NEUMANN, **mk))
+    xo, _ = mo.solve(np.zeros(sh), b, pad)
+    assert np.array_equal(res[0][c], xo[c])
+
+    nu = (1.0 / 1e7) ** 0.5
+    so = O.Nse(O.nse_desc(g, mg=O.mg_params(bc=O.BC_NEUMANN, **mk), nu=nu, kappa=nu, dt=1e-3))
+    zero = np.zeros(sh)
+    so.set(vx=zero, vy=zero, vz=zero, p=np.ones(sh), t=T, pp=zero)
+    for _ in range(steps):
+        assert so.step() == 0
+    fo = so.get()
+    for q, k in enumerate(("vx", "vy", "vz", "p", "t")):
+        assert np.array_equal(res[1 + q][c], fo[k][c]), k

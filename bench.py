@@ -20,6 +20,16 @@ roofline: the dominant kernel (finest-level fused sweep), algorithmic
 3 words per cell per launch / average launch time (HIP events).
 cpu_baseline: the FP64 CPU oracle's V-cycle on the same workload.
 
+serving: the batched request loop on the Poisson request mix (75 % 3-D
+n^3, n log-uniform in [32, 256]; 25 % 2-D n^2, n in [64, 2048]; multiples of
+16; Dirichlet / Neumann / periodic; 8-mode Fourier RHS; tolerance 1e-8):
+solves/s with every request queued at once (saturation), p50/p99 latency of
+a seeded Poisson arrival stream at 70 % of that rate, and the same requests
+solved one at a time with plain FP64 solves (one slot, no graphs), with a
+solver built per request (the plain way) and with resident hierarchies; the
+server's hierarchies are resident after one warm-up pass; RHS arrays are
+generated beforehand.
+
 rbc_256_step: the 256^3 Rayleigh-Benard time step (Ra 1e7, Pr 1, z-tanh 1.5,
 dScheme 2, Morinishi, RK3 with Jacobi CN, V(3,6) pressure solves of 20
 cycles): device time per full step, FP64 and with mixed-precision pressure
@@ -195,6 +205,46 @@ def rbc_step(steps, warmup, n=256):
     return out
 
 
+def serving(count=240, seed=4, slots=8, budget=96 << 30):
+    from saras_amd import serving as V
+
+    def note(msg):
+        print(f"[serving] {msg}", file=sys.stderr, flush=True)
+
+    reqs = V.draw_requests(count, seed=seed)
+    rhs = V.RhsCache()
+    for r in reqs:
+        rhs.get(r)
+    note(f"{count} requests, right-hand sides ready")
+    srv = S.Server(budget, slots=slots, use_graphs=1)
+    w = V.summarize(V.run_stream(srv, reqs, rhs, S.PREC_MIXED)[0])     # warm-up: hierarchies + graphs
+    note(f"warm-up pass {w['seconds']:.2f} s")
+    sat = V.summarize(V.run_stream(srv, reqs, rhs, S.PREC_MIXED)[0])
+    note(f"saturation {sat['solves_per_s']:.1f} solves/s")
+    rate = 0.7 * sat["solves_per_s"]
+    opn = V.summarize(V.run_stream(srv, reqs, rhs, S.PREC_MIXED, rate=rate)[0])
+    note(f"open loop at {rate:.1f}/s: p50 {opn['p50_ms']:.2f} ms, p99 {opn['p99_ms']:.2f} ms")
+    srv.close()
+    # the plain way: one request at a time, FP64, a solver built for each
+    # request and released after it (poisson's constructor + mgSolve)
+    cold = S.Server(budget, slots=1, use_graphs=0, keep_resident=0)
+    one_cold = V.summarize(V.run_stream(cold, reqs, rhs, S.PREC_FP64)[0])
+    cold.close()
+    note(f"one at a time, FP64, solver per request: {one_cold['solves_per_s']:.1f} solves/s")
+    # ... and with every hierarchy kept resident
+    base = S.Server(budget, slots=1, use_graphs=0)
+    V.run_stream(base, reqs, rhs, S.PREC_FP64)                          # warm-up
+    one = V.summarize(V.run_stream(base, reqs, rhs, S.PREC_FP64)[0])
+    note(f"one at a time, FP64, resident hierarchies: {one['solves_per_s']:.1f} solves/s")
+    base.close()
+    n3 = sum(1 for r in reqs if r["dim"] == 3)
+    return {"requests": count, "requests_3d": n3, "slots": slots, "mem_budget_gb": budget / 2**30,
+            "saturation": sat, "open_loop": dict(rate_per_s=round(rate, 1), **opn),
+            "one_at_a_time_fp64": one_cold, "one_at_a_time_fp64_resident": one,
+            "speedup_vs_one_at_a_time": round(sat["solves_per_s"] / one_cold["solves_per_s"], 3),
+            "speedup_vs_one_at_a_time_resident": round(sat["solves_per_s"] / one["solves_per_s"], 3)}
+
+
 def cpu_baseline(nu1, nu2, shapes, budget_s=12.0):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -233,6 +283,10 @@ def main():
     ap.add_argument("--rbc-only", action="store_true",
                     help="only the 256^3 Rayleigh-Benard time steps (warmup + steps, FP64 and mixed)")
     ap.add_argument("--no-rbc", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-only", action="store_true", help="only the batched serving measurement")
+    ap.add_argument("--slots", type=int, default=8)
+    ap.add_argument("--requests", type=int, default=240)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build measures the single-GPU path "
@@ -245,6 +299,9 @@ def main():
         return
     if args.rbc_only:
         print(json.dumps({"rbc_256_step": rbc_step(args.steps, args.warmup)}))
+        return
+    if args.serve_only:
+        print(json.dumps({"serving": serving(count=args.requests, slots=args.slots)}))
         return
     rates = cycle_rates(args, nu1, nu2)
     head, f64 = rates["mixed"], rates["fp64"]
@@ -288,6 +345,8 @@ def main():
             "mixed_refinements": sc["mixed"]["refinements"], "mixed_residual": sc["mixed"]["residual"],
             "mixed_converged": sc["mixed"]["converged"],
             "speedup": round(sc["fp64"]["ms"] / sc["mixed"]["ms"], 3)}
+    if not args.no_serve:
+        out["serving"] = serving(count=args.requests, slots=args.slots)
     if not args.no_rbc:
         out["rbc_256_step"] = rbc_step(3, 1)
     if not args.no_cpu:

@@ -258,6 +258,64 @@ saras_status saras_nse_diag(saras_nse *s, double out[4]);
  * beyond the reference's own convergence checks; *ms = device time. */
 saras_status saras_nse_bench_steps(saras_nse *s, int count, double *ms);
 
+/* ---- serving (no counterpart in the reference: these serve many solves
+ * per process, the batched request loop the reference does not have). ---- */
+
+/* Replay the handle's V-cycles from captured hipGraphs (enable = 1) or
+ * launch them one kernel at a time (0, the default).  Results are identical;
+ * graphs remove per-launch host overhead for small grids. */
+saras_status saras_mg_set_graphs(saras_mg *mg, int enable);
+
+/* Device bytes a handle of this grid and parameter set holds (hierarchies,
+ * staging, workspaces); 0 for an invalid grid. */
+size_t saras_mg_footprint(const saras_grid_desc *grid, const saras_mg_params *params);
+
+typedef struct {
+    size_t mem_budget;   /* device bytes the server may hold in solver handles */
+    int slots;           /* solves in flight at once (one worker thread and stream each) */
+    int use_graphs;      /* replay V-cycles from hipGraphs */
+    int keep_resident;   /* keep idle hierarchies for reuse (LRU, within the budget) */
+} saras_server_params;
+
+/* One Poisson solve: poisson::mgSolve on caller host arrays (layout as
+ * above, pad grid.pad).  b and x must stay valid until the completion with
+ * the same tag is returned; x is in/out like outLHS. */
+typedef struct {
+    saras_grid_desc grid;
+    saras_mg_params mg;
+    const double *b;
+    double *x;
+    long long tag;
+} saras_request;
+
+typedef struct {
+    long long tag;
+    int status;            /* saras_status of the solve */
+    saras_mg_stats stats;
+    double t_submit;       /* seconds on the server clock */
+    double t_admit;
+    double t_done;
+    char message[120];     /* error text when status is not OK / NOT_CONVERGED */
+} saras_completion;
+
+typedef struct saras_server saras_server;
+
+saras_status saras_server_create(const saras_server_params *params, saras_server **out);
+/* Queue a request (returns at once).  Requests are admitted in arrival
+ * order as slots and device memory allow; a request that waits for memory
+ * lets later, smaller ones pass. */
+saras_status saras_server_submit(saras_server *s, const saras_request *req);
+/* Move up to max finished requests into out; waits up to wait_ms for the
+ * first one.  Returns the number moved (negative on a bad argument). */
+int saras_server_poll(saras_server *s, saras_completion *out, int max, int wait_ms);
+/* Block until every submitted request has finished. */
+saras_status saras_server_drain(saras_server *s);
+/* Seconds since the server was created (its clock for t_* fields). */
+double saras_server_clock(const saras_server *s);
+/* Device bytes currently held by resident hierarchies and running solves. */
+size_t saras_server_mem_used(const saras_server *s);
+saras_status saras_server_destroy(saras_server *s);
+
 /* ---- device memory helpers, so callers without a GPU framework can hand
  * the solver device pointers. ---- */
 saras_status saras_device_alloc(void **ptr, size_t bytes);

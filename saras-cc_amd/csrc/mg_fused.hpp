@@ -312,13 +312,13 @@ __global__ void __launch_bounds__(kNormThreads) k_refine(LevelArgs<double> L, co
                                                          float *__restrict__ r32, long rsx, long rsy,
                                                          double *__restrict__ partial) {
     const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
-    const long total = (long)nx * ny * nz;
+    const long rows = (long)nx * ny;
+    const RowWalk wk(nz);
     double ar = 0.0, ab = 0.0;
-    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
-        const int k = (int)(t % nz);
-        const long rr = t / nz;
-        const int j = (int)(rr % ny);
-        const int i = (int)(rr / ny);
+    for (long row = (long)blockIdx.x * wk.rpp + wk.sub; wk.sub < wk.rpp && row < rows; row += (long)gridDim.x * wk.rpp)
+    for (int k = wk.k0; k < nz; k += wk.kspan) {
+        const int i = (int)(row / ny);
+        const int j = (int)(row - (long)i * ny);
         const long idx = (long)i * L.sx + (long)j * L.sy + k;
         auto xv = [&](int ii, int jj, int kk) -> double {
             const double v = xold[(long)ii * L.sx + (long)jj * L.sy + kk];

@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <string>
 
 #include "saras_hip.h"
@@ -19,6 +20,12 @@ void mg_level0(saras_mg *m, double **x, double **b, long *sx, long *sy);
 saras_status mg_solve_resident(saras_mg *m, hipStream_t s, saras_mg_stats *st);
 
 hipStream_t mg_stream(saras_mg *m);
+
+// Serialises device allocation, deallocation and hipGraph capture across
+// host threads: a free (which synchronises the device) or a null-stream
+// memset issued while another thread captures a stream would invalidate the
+// capture.
+std::recursive_mutex &alloc_mutex();
 
 // Text returned by saras_last_error() on the calling thread.
 void set_last_error(const std::string &msg);

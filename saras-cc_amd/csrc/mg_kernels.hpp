@@ -269,43 +269,65 @@ __device__ __forceinline__ double norm_comb(double a, double b) {
     return TYPE == 0 ? fmax(a, b) : a + b;
 }
 
+// block reduction of two values: 64-lane shuffles, then the waves in order
 template <int TYPE>
 __device__ void block_reduce2(double &a, double &b) {
-    __shared__ double sa[kNormThreads], sb[kNormThreads];
-    sa[threadIdx.x] = a;
-    sb[threadIdx.x] = b;
-    __syncthreads();
-    for (int s = kNormThreads / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) {
-            sa[threadIdx.x] = norm_comb<TYPE>(sa[threadIdx.x], sa[threadIdx.x + s]);
-            sb[threadIdx.x] = norm_comb<TYPE>(sb[threadIdx.x], sb[threadIdx.x + s]);
-        }
-        __syncthreads();
+    __shared__ double sa[kNormThreads / 64], sb[kNormThreads / 64];
+    for (int o = 32; o > 0; o >>= 1) {
+        a = norm_comb<TYPE>(a, __shfl_xor(a, o, 64));
+        b = norm_comb<TYPE>(b, __shfl_xor(b, o, 64));
     }
-    a = sa[0];
-    b = sb[0];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) {
+        sa[wv] = a;
+        sb[wv] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < (int)(blockDim.x >> 6); ++q) {
+            a = norm_comb<TYPE>(a, sa[q]);
+            b = norm_comb<TYPE>(b, sb[q]);
+        }
+    }
 }
 
+// Row-parallel traversal of a level's core: lanes run along z, each pass of
+// a block covers `rpp` rows (i, j); rows are dealt to blocks round-robin.
+struct RowWalk {
+    int kspan, rpp, sub, k0;
+    __device__ RowWalk(int nz) {
+        kspan = min((int)blockDim.x, (nz + 63) / 64 * 64);
+        rpp = blockDim.x / kspan;
+        sub = threadIdx.x / kspan;
+        k0 = threadIdx.x - sub * kspan;
+    }
+};
+
+// residual norm partials of computeError (poisson_d3.cc:563-663)
 template <int TYPE, int DIM>
 __global__ void __launch_bounds__(kNormThreads) k_norm_partial(LevelArgs<double> L, const double *__restrict__ x,
                                                                const double *__restrict__ b,
                                                                double *__restrict__ partial) {
     const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
-    const long total = (long)nx * ny * nz;
+    const long rows = (long)nx * ny;
+    const RowWalk w(nz);
     double ar = 0.0, ab = 0.0;
-    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
-        const int k = (int)(t % nz);
-        const long r = t / nz;
-        const int j = (int)(r % ny);
-        const int i = (int)(r / ny);
-        const long idx = (long)i * L.sx + (long)j * L.sy + k;
-        Stencil<double, DIM> s;
-        gather<double, DIM>(L, x, i, j, k, idx, s);
-        const double c2y = DIM == 3 ? L.c2[1][j] : 0.0, c1y = DIM == 3 ? L.c1[1][j] : 0.0;
-        const double bv = b[idx];
-        const double rv = residual<double, DIM>(s, L.c2[0][i], L.c1[0][i], c2y, c1y, L.c2[2][k], L.c1[2][k], bv);
-        ar = norm_acc<TYPE>(ar, rv);
-        ab = norm_acc<TYPE>(ab, bv);
+    if (w.sub < w.rpp) {
+        for (long r = (long)blockIdx.x * w.rpp + w.sub; r < rows; r += (long)gridDim.x * w.rpp) {
+            const int i = (int)(r / ny);
+            const int j = (int)(r - (long)i * ny);
+            const double c2x = L.c2[0][i], c1x = L.c1[0][i];
+            const double c2y = DIM == 3 ? L.c2[1][j] : 0.0, c1y = DIM == 3 ? L.c1[1][j] : 0.0;
+            for (int k = w.k0; k < nz; k += w.kspan) {
+                const long idx = (long)i * L.sx + (long)j * L.sy + k;
+                Stencil<double, DIM> s;
+                gather<double, DIM>(L, x, i, j, k, idx, s);
+                const double bv = b[idx];
+                const double rv = residual<double, DIM>(s, c2x, c1x, c2y, c1y, L.c2[2][k], L.c1[2][k], bv);
+                ar = norm_acc<TYPE>(ar, rv);
+                ab = norm_acc<TYPE>(ab, bv);
+            }
+        }
     }
     block_reduce2<TYPE>(ar, ab);
     if (threadIdx.x == 0) {

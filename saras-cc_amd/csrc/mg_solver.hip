@@ -195,8 +195,21 @@ struct saras_mg {
     bool tail_off = false;             // SARAS_NO_TAIL=1 runs every level with multi-block kernels
     int march_wgs = 0;                 // target workgroups of a march sweep (SARAS_MARCH_WGS; 0 = per type)
     bool march_tj8 = false;            // FP32 march with 64 x 8 tiles (SARAS_MARCH_TJ8=1)
+    // V-cycles replayed from captured hipGraphs (saras_mg_set_graphs).  A
+    // graph bakes in the level buffers, and out-of-place sweeps swap them, so
+    // graphs are cached per buffer state (at most two states per precision).
+    bool graphs = false;
+    struct GraphEntry {
+        std::vector<void *> key;    // (x, xalt) of every level before the cycle
+        std::vector<void *> post;   // ... and after it
+        hipGraphExec_t exec;
+        hipStream_t stream;
+    };
+    std::vector<GraphEntry> graph_cache[2];   // [0] FP64 cycles, [1] FP32 cycles
 
     ~saras_mg() {
+        for (auto &c : graph_cache)
+            for (auto &e : c) (void)hipGraphExecDestroy(e.exec);
         (void)hipFree(x64b); (void)hipFree(partial); (void)hipFree(norm_out); (void)hipFree(iters_dev);
         (void)hipHostFree(norm_host); (void)hipHostFree(iters_host);
         (void)hipFree(stage_x); (void)hipFree(stage_b);
@@ -364,10 +377,11 @@ struct Ops {
 
     // first level of the single-workgroup tail: the smallest S whose levels
     // S..D fit in 48 KB of LDS
+    // (D + 1, i.e. no tail, when even the coarsest level does not fit)
     int tail_start() const {
         const int D = (int)H.lv.size() - 1;
         size_t bytes = 0;
-        int S = D;
+        int S = D + 1;
         for (int l = D; l >= 0; --l) {
             const DevLevel<T> &L = H.lv[l];
             const size_t core = (size_t)L.n[0] * L.n[1] * L.n[2];
@@ -418,6 +432,7 @@ struct Ops {
         A.tol = m->p.mg_tol;
         A.zero_bc_first = S != 0;
         A.iters_out = m->iters_dev;
+        if (off * sizeof(T) > 48 * 1024) std::abort();  // host-side guard: tail_start() keeps levels within 46 KB
         if (H.dim == 3)
             hipLaunchKernelGGL((saras::k_tail<T, 3>), dim3(1), dim3(512), off * sizeof(T), s, A);
         else
@@ -451,6 +466,55 @@ struct Ops {
         }
     }
 };
+
+// One V-cycle of `ops`, eager or replayed from the handle's graph cache.
+template <typename T>
+saras_status run_vcycle(saras_mg *m, Ops<T> &ops) {
+    if (!m->graphs) {
+        ops.vcycle();
+        return SARAS_OK;
+    }
+    auto &cache = m->graph_cache[sizeof(T) == 4 ? 1 : 0];
+    std::vector<void *> key;
+    for (auto &L : ops.H.lv) {
+        key.push_back((void *)L.x);
+        key.push_back((void *)L.xalt);
+    }
+    for (auto &e : cache)
+        if (e.key == key && e.stream == ops.s) {
+            HIP_TRY(hipGraphLaunch(e.exec, ops.s));
+            // the replayed cycle's buffer swaps, applied on the host side
+            for (size_t l = 0; l < ops.H.lv.size(); ++l) {
+                ops.H.lv[l].x = (T *)e.post[2 * l];
+                ops.H.lv[l].xalt = (T *)e.post[2 * l + 1];
+            }
+            return SARAS_OK;
+        }
+    hipGraph_t g = nullptr;
+    std::lock_guard<std::recursive_mutex> lk(saras::alloc_mutex());
+    HIP_TRY(hipStreamBeginCapture(ops.s, hipStreamCaptureModeThreadLocal));
+    ops.vcycle();
+    hipError_t ce = hipStreamEndCapture(ops.s, &g);
+    if (ce != hipSuccess) {
+        set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+        return SARAS_HIP_ERROR;
+    }
+    hipGraphExec_t exec = nullptr;
+    hipError_t ie = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ie != hipSuccess) {
+        set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+        return SARAS_HIP_ERROR;
+    }
+    std::vector<void *> post;
+    for (auto &L : ops.H.lv) {
+        post.push_back((void *)L.x);
+        post.push_back((void *)L.xalt);
+    }
+    cache.push_back({key, post, exec, ops.s});
+    HIP_TRY(hipGraphLaunch(exec, ops.s));
+    return SARAS_OK;
+}
 
 template <int DIM>
 void launch_norm(int type, saras_mg *m, const saras::LevelArgs<double> &a, const double *x, const double *b,
@@ -645,6 +709,7 @@ saras_status saras_mg_create(const saras_grid_desc *grid, const saras_mg_params 
     if (p.res_type < 0 || p.res_type > 2) { set_error("Residual Type must be 0, 1 or 2"); return SARAS_BAD_ARG; }
     if (p.bc < 0 || p.bc > 2 || p.precision < 0 || p.precision > 1) { set_error("bad bc/precision"); return SARAS_BAD_ARG; }
     if (p.vc_count < 0 || p.pre_smooth < 0 || p.post_smooth < 0) { set_error("negative counts"); return SARAS_BAD_ARG; }
+    std::lock_guard<std::recursive_mutex> lk(saras::alloc_mutex());
     std::unique_ptr<saras_mg> m(new saras_mg());
     m->g = g;
     m->p = p;
@@ -688,6 +753,9 @@ saras_status saras_mg_create(const saras_grid_desc *grid, const saras_mg_params 
     HIP_TRY(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
     HIP_TRY(hipEventCreate(&m->ev0));
     HIP_TRY(hipEventCreate(&m->ev1));
+    // the level arrays were cleared on the null stream; the handle's stream
+    // does not synchronise with it
+    HIP_TRY(hipDeviceSynchronize());
     *out = m.release();
     return SARAS_OK;
 }
@@ -695,6 +763,7 @@ saras_status saras_mg_create(const saras_grid_desc *grid, const saras_mg_params 
 saras_status saras_mg_destroy(saras_mg *mg) {
     if (!mg) return SARAS_BAD_ARG;
     (void)hipStreamSynchronize(mg->stream);
+    std::lock_guard<std::recursive_mutex> lk(saras::alloc_mutex());
     delete mg;
     return SARAS_OK;
 }
@@ -714,7 +783,8 @@ static saras_status solve_fp64(saras_mg *m, hipStream_t s, saras_mg_stats *st) {
     double res = 0.0;
     bool conv = false;
     for (int c = 0; c < m->p.vc_count; ++c) {
-        ops.vcycle();
+        saras_status ve = run_vcycle(m, ops);
+        if (ve) return ve;
         if (m->p.solve_flag) {
             saras_status e = fetch_coarse_iters(m, s);
             if (e) return e;
@@ -796,7 +866,8 @@ static saras_status solve_mixed(saras_mg *m, hipStream_t s, saras_mg_stats *st) 
     while (!(res < m->p.vc_tol) && refs < m->p.max_refine && cycles < cap) {
         HIP_TRY(hipMemsetAsync(F0.x, 0, F0.elems * sizeof(float), s));
         for (int c = 0; c < m->p.inner_cycles && cycles < cap; ++c) {
-            ops.vcycle();
+            e = run_vcycle(m, ops);
+            if (e) return e;
             ++cycles;
         }
         e = refine_step(m, s, true, &res);
@@ -829,6 +900,11 @@ hipStream_t mg_stream(saras_mg *m) { return m->stream; }
 
 void set_last_error(const std::string &msg) { set_error(msg); }
 
+std::recursive_mutex &alloc_mutex() {
+    static std::recursive_mutex mu;
+    return mu;
+}
+
 saras_status mg_solve_resident(saras_mg *m, hipStream_t s, saras_mg_stats *st) {
     m->coarse_iters_max = 0;
     m->coarse_capped = 0;
@@ -859,6 +935,7 @@ saras_status saras_mg_solve(saras_mg *mg, double *x_full, const double *b_full, 
     const double *db = b_full;
     if (ptr_kind == SARAS_PTR_HOST) {
         if (m->stage_elems < (size_t)fe) {
+            std::lock_guard<std::recursive_mutex> lk(saras::alloc_mutex());
             (void)hipFree(m->stage_x);
             (void)hipFree(m->stage_b);
             m->stage_x = m->stage_b = nullptr;
@@ -897,6 +974,50 @@ saras_status saras_mg_solve(saras_mg *mg, double *x_full, const double *b_full, 
     st.coarse_capped = m->coarse_capped;
     if (stats) *stats = st;
     return st.converged ? SARAS_OK : SARAS_NOT_CONVERGED;
+}
+
+saras_status saras_mg_set_graphs(saras_mg *mg, int enable) {
+    if (!mg) return SARAS_BAD_ARG;
+    HIP_TRY(hipStreamSynchronize(mg->stream));
+    if (!enable)
+        for (auto &c : mg->graph_cache) {
+            for (auto &e : c) (void)hipGraphExecDestroy(e.exec);
+            c.clear();
+        }
+    mg->graphs = enable != 0;
+    return SARAS_OK;
+}
+
+size_t saras_mg_footprint(const saras_grid_desc *grid, const saras_mg_params *params) {
+    if (!grid || !params) return 0;
+    saras::GridLevels G;
+    if (!saras::build_levels(*grid, G)) return 0;
+    const int D = G.depth;
+    auto level_bytes = [&](size_t word) {
+        size_t tot = 0;
+        for (int l = 0; l <= D; ++l) {
+            int n[3];
+            for (int a = 0; a < 3; ++a) n[a] = G.axes[l][a].n;
+            if (grid->dim == 2) n[1] = 1;
+            const long a128 = 128 / (long)word;
+            const size_t sy = (size_t)((n[2] + a128 - 1) / a128 * a128);
+            const size_t elems = sy * n[1] * n[0];
+            tot += elems * word * ((l < D && grid->dim == 3) ? 3 : 2);
+            tot += 5 * 3 * (size_t)(n[0] + n[1] + n[2]) * word;
+        }
+        return tot;
+    };
+    size_t bytes = level_bytes(8);
+    const size_t full = (size_t)full_elems(*grid);
+    if (params->precision == SARAS_PREC_MIXED) {
+        bytes += level_bytes(4);
+        const auto &A = G.axes[0];
+        const size_t sy = (size_t)((A[2].n + 15) / 16 * 16);
+        bytes += sy * (grid->dim == 2 ? 1 : A[1].n) * A[0].n * 8;   // second FP64 iterate
+    }
+    bytes += 2 * full * 8;                      // host-pointer staging
+    bytes += 2 * 1024 * 1024;                   // norms, counters, coarse workspace
+    return bytes;
 }
 
 // ---- single-operation entry points ----
@@ -996,10 +1117,16 @@ saras_status saras_mg_bench_cycles(saras_mg *mg, int count, double *ms) {
     HIP_TRY(hipEventRecord(mg->ev0, s));
     if (mg->p.precision == SARAS_PREC_FP64) {
         Ops<double> ops{mg, mg->h64, s, true, mg->p.sor};
-        for (int c = 0; c < count; ++c) ops.vcycle();
+        for (int c = 0; c < count; ++c) {
+            saras_status e = run_vcycle(mg, ops);
+            if (e) return e;
+        }
     } else {
         Ops<float> ops{mg, *mg->h32, s, false, (float)mg->p.sor};
-        for (int c = 0; c < count; ++c) ops.vcycle();
+        for (int c = 0; c < count; ++c) {
+            saras_status e = run_vcycle(mg, ops);
+            if (e) return e;
+        }
     }
     HIP_TRY(hipEventRecord(mg->ev1, s));
     HIP_TRY(hipEventSynchronize(mg->ev1));

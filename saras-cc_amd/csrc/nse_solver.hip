@@ -213,6 +213,7 @@ void fill_stats(saras_nse *s, saras_nse_stats *st, float ms) {
 
 saras_status ensure_stage(saras_nse *s) {
     if (s->stage_a) return SARAS_OK;
+    std::lock_guard<std::recursive_mutex> lk(saras::alloc_mutex());
     NSE_TRY(hipMalloc((void **)&s->stage_a, s->full * sizeof(double)));
     NSE_TRY(hipMalloc((void **)&s->stage_b, s->full * sizeof(double)));
     return SARAS_OK;
@@ -313,6 +314,7 @@ saras_status saras_nse_create(const saras_grid_desc *grid, const saras_mg_params
         delete s;
         return e;
     }
+    std::lock_guard<std::recursive_mutex> lk(saras::alloc_mutex());
     s->s = saras::mg_stream(s->mg);
     Geom &G = s->G;
     for (int a = 0; a < 3; ++a) {
@@ -416,8 +418,13 @@ saras_status saras_nse_create(const saras_grid_desc *grid, const saras_mg_params
 saras_status saras_nse_destroy(saras_nse *s) {
     if (!s) return SARAS_BAD_ARG;
     (void)hipStreamSynchronize(s->s);
-    delete s;
-    return SARAS_OK;
+    saras_mg *mg = s->mg;
+    s->mg = nullptr;
+    {
+        std::lock_guard<std::recursive_mutex> lk(saras::alloc_mutex());
+        delete s;
+    }
+    return saras_mg_destroy(mg);
 }
 
 saras_status saras_nse_set_fields(saras_nse *s, const double *vx, const double *vy, const double *vz,

@@ -1,0 +1,253 @@
+// serve.cpp -- the persistent request loop of many Poisson solves on one GPU.
+//
+// Requests arrive at any time (saras_server_submit) and wait in a queue.
+// `slots` worker threads each own the solves they run; every solver handle
+// has its own HIP stream, so solves admitted together overlap on the device
+// (continuous batching: a request is admitted as soon as a slot and enough
+// device memory are free, while others are still running).  Admission is by
+// device-memory footprint: hierarchies stay resident after a solve and are
+// reused by later requests of the same grid and parameters; idle ones are
+// evicted least-recently-used when a new request needs their memory.  A
+// request whose footprint does not fit waits, letting later requests that
+// fit pass it, and is run alone if it can only ever fit on an idle device.
+//
+// The solve of every request is the plain saras_mg_solve path on its own
+// handle, so a result does not depend on what else is in flight.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <list>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "saras_hip.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+template <typename V>
+void put(std::string &k, const V &v) {
+    k.append(reinterpret_cast<const char *>(&v), sizeof(v));
+}
+
+// field-wise key of (grid, parameters): struct padding never enters it
+std::string key_of(const saras_grid_desc &g, const saras_mg_params &p) {
+    std::string k;
+    put(k, g.dim);
+    for (int a = 0; a < 3; ++a) {
+        put(k, g.n[a]);
+        put(k, g.len[a]);
+        put(k, g.periodic[a]);
+        put(k, g.gtype[a]);
+        put(k, g.beta[a]);
+    }
+    put(k, g.pad);
+    put(k, p.vc_count);
+    put(k, p.vc_tol);
+    put(k, p.solve_flag);
+    put(k, p.mg_tol);
+    put(k, p.sor);
+    put(k, p.pre_smooth);
+    put(k, p.post_smooth);
+    put(k, p.res_type);
+    put(k, p.bc);
+    put(k, p.precision);
+    put(k, p.inner_cycles);
+    put(k, p.max_refine);
+    return k;
+}
+
+// a request blocked on memory stops others from passing it after this long
+constexpr double kMaxBypassSeconds = 0.25;
+
+struct Pending {
+    saras_request r;
+    double t_submit;
+    size_t bytes;
+    std::string key;
+};
+
+struct Idle {
+    std::string key;
+    saras_mg *h;
+    size_t bytes;
+};
+
+}  // namespace
+
+struct saras_server {
+    saras_server_params p{};
+    Clock::time_point t0;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Pending> queue;
+    std::list<Idle> idle;                  // front = most recently used
+    std::vector<saras_completion> done;
+    std::vector<std::thread> workers;
+    size_t mem_used = 0;                   // idle handles + running solves
+    int running = 0;
+    long long submitted = 0, finished = 0;
+    bool stop = false;
+
+    double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
+
+    // Pick the next admissible request (caller holds mu).  Returns false when
+    // none can run now.  On success the request's memory is accounted and, if
+    // a resident handle fits it, that handle is taken from the idle list.
+    bool take(Pending &out, saras_mg *&h) {
+        for (auto it = queue.begin(); it != queue.end(); ++it) {
+            // a resident hierarchy of the same grid and parameters
+            for (auto jt = idle.begin(); jt != idle.end(); ++jt)
+                if (jt->key == it->key) {
+                    h = jt->h;
+                    it->bytes = jt->bytes;   // already accounted
+                    idle.erase(jt);
+                    out = *it;
+                    queue.erase(it);
+                    return true;
+                }
+            size_t need = it->bytes;
+            // evict idle hierarchies, least recently used first, until it fits
+            while (mem_used + need > p.mem_budget && !idle.empty()) {
+                Idle v = idle.back();
+                idle.pop_back();
+                saras_mg_destroy(v.h);
+                mem_used -= v.bytes;
+            }
+            if (mem_used + need <= p.mem_budget || (running == 0 && idle.empty())) {
+                mem_used += need;
+                h = nullptr;
+                out = *it;
+                queue.erase(it);
+                return true;
+            }
+            // this one waits for memory; later requests may pass it unless it
+            // has already waited too long
+            if (now() - it->t_submit > kMaxBypassSeconds) return false;
+        }
+        return false;
+    }
+
+    void worker() {
+        for (;;) {
+            Pending job;
+            saras_mg *h = nullptr;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                bool got = false;
+                cv_work.wait(lk, [&] {
+                    if (stop) return true;
+                    got = !queue.empty() && take(job, h);
+                    return got;
+                });
+                if (!got) return;
+                ++running;
+            }
+            saras_completion c{};
+            c.tag = job.r.tag;
+            c.t_submit = job.t_submit;
+            c.t_admit = now();
+            saras_status st = SARAS_OK;
+            if (!h) {
+                st = saras_mg_create(&job.r.grid, &job.r.mg, &h);
+                if (st == SARAS_OK && p.use_graphs) st = saras_mg_set_graphs(h, 1);
+            }
+            if (st == SARAS_OK) st = saras_mg_solve(h, job.r.x, job.r.b, SARAS_PTR_HOST, nullptr, &c.stats);
+            c.status = st;
+            c.t_done = now();
+            if (st != SARAS_OK && st != SARAS_NOT_CONVERGED)
+                std::strncpy(c.message, saras_last_error(), sizeof(c.message) - 1);
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                --running;
+                if (h && p.keep_resident && (st == SARAS_OK || st == SARAS_NOT_CONVERGED)) {
+                    idle.push_front({job.key, h, job.bytes});
+                } else {
+                    if (h) saras_mg_destroy(h);
+                    mem_used -= job.bytes;
+                }
+                done.push_back(c);
+                ++finished;
+            }
+            cv_done.notify_all();
+            cv_work.notify_all();
+        }
+    }
+};
+
+extern "C" {
+
+saras_status saras_server_create(const saras_server_params *params, saras_server **out) {
+    if (!params || !out || params->slots < 1 || params->mem_budget == 0) return SARAS_BAD_ARG;
+    saras_server *s = new saras_server();
+    s->p = *params;
+    s->t0 = Clock::now();
+    for (int w = 0; w < params->slots; ++w) s->workers.emplace_back([s] { s->worker(); });
+    *out = s;
+    return SARAS_OK;
+}
+
+saras_status saras_server_submit(saras_server *s, const saras_request *req) {
+    if (!s || !req || !req->b || !req->x) return SARAS_BAD_ARG;
+    const size_t bytes = saras_mg_footprint(&req->grid, &req->mg);
+    if (bytes == 0) return SARAS_BAD_SHAPE;
+    Pending pd;
+    pd.r = *req;
+    pd.t_submit = s->now();
+    pd.bytes = bytes;
+    pd.key = key_of(req->grid, req->mg);
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->queue.push_back(pd);
+        ++s->submitted;
+    }
+    s->cv_work.notify_one();
+    return SARAS_OK;
+}
+
+int saras_server_poll(saras_server *s, saras_completion *out, int max, int wait_ms) {
+    if (!s || (!out && max > 0) || max < 0) return -1;
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (s->done.empty() && wait_ms > 0)
+        s->cv_done.wait_for(lk, std::chrono::milliseconds(wait_ms), [&] { return !s->done.empty(); });
+    const int n = std::min<int>(max, (int)s->done.size());
+    for (int q = 0; q < n; ++q) out[q] = s->done[q];
+    s->done.erase(s->done.begin(), s->done.begin() + n);
+    return n;
+}
+
+saras_status saras_server_drain(saras_server *s) {
+    if (!s) return SARAS_BAD_ARG;
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv_done.wait(lk, [&] { return s->finished == s->submitted; });
+    return SARAS_OK;
+}
+
+double saras_server_clock(const saras_server *s) { return s ? s->now() : 0.0; }
+
+size_t saras_server_mem_used(const saras_server *s) {
+    if (!s) return 0;
+    std::lock_guard<std::mutex> lk(const_cast<saras_server *>(s)->mu);
+    return s->mem_used;
+}
+
+saras_status saras_server_destroy(saras_server *s) {
+    if (!s) return SARAS_BAD_ARG;
+    (void)saras_server_drain(s);
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    for (auto &t : s->workers) t.join();
+    for (auto &v : s->idle) saras_mg_destroy(v.h);
+    delete s;
+    return SARAS_OK;
+}
+
+}  // extern "C"

@@ -62,6 +62,21 @@ class NseStats(C.Structure):
     _fields_ = [("jacobi_iters", C.c_int * 4), ("mg", MgStats), ("device_ms", C.c_double)]
 
 
+class ServerParams(C.Structure):
+    _fields_ = [("mem_budget", C.c_size_t), ("slots", C.c_int), ("use_graphs", C.c_int),
+                ("keep_resident", C.c_int)]
+
+
+class Request(C.Structure):
+    _fields_ = [("grid", GridDesc), ("mg", MgParams), ("b", C.c_void_p), ("x", C.c_void_p),
+                ("tag", C.c_longlong)]
+
+
+class Completion(C.Structure):
+    _fields_ = [("tag", C.c_longlong), ("status", C.c_int), ("stats", MgStats), ("t_submit", C.c_double),
+                ("t_admit", C.c_double), ("t_done", C.c_double), ("message", C.c_char * 120)]
+
+
 # every symbol include/saras_hip.h declares
 EXPORTS = [
     "saras_mg_create", "saras_mg_solve", "saras_mg_destroy", "saras_mg_levels",
@@ -73,6 +88,9 @@ EXPORTS = [
     "saras_bench_stream_copy", "saras_nse_create", "saras_nse_destroy", "saras_nse_set_fields",
     "saras_nse_get_fields", "saras_nse_step", "saras_nse_stage", "saras_nse_rhs", "saras_nse_get_rhs",
     "saras_nse_solve_field", "saras_field_op", "saras_nse_diag", "saras_nse_bench_steps",
+    "saras_mg_set_graphs", "saras_mg_footprint", "saras_server_create", "saras_server_submit",
+    "saras_server_poll", "saras_server_drain", "saras_server_clock", "saras_server_mem_used",
+    "saras_server_destroy",
 ]
 
 _lib = None
@@ -121,6 +139,18 @@ def lib():
         L.saras_field_op.argtypes = [vp, C.c_int, vp, vp, C.c_int]
         L.saras_nse_diag.argtypes = [vp, dp]
         L.saras_nse_bench_steps.argtypes = [vp, C.c_int, dp]
+        L.saras_mg_set_graphs.argtypes = [vp, C.c_int]
+        L.saras_mg_footprint.argtypes = [C.POINTER(GridDesc), C.POINTER(MgParams)]
+        L.saras_mg_footprint.restype = C.c_size_t
+        L.saras_server_create.argtypes = [C.POINTER(ServerParams), C.POINTER(vp)]
+        L.saras_server_submit.argtypes = [vp, C.POINTER(Request)]
+        L.saras_server_poll.argtypes = [vp, C.POINTER(Completion), C.c_int, C.c_int]
+        L.saras_server_drain.argtypes = [vp]
+        L.saras_server_clock.argtypes = [vp]
+        L.saras_server_clock.restype = C.c_double
+        L.saras_server_mem_used.argtypes = [vp]
+        L.saras_server_mem_used.restype = C.c_size_t
+        L.saras_server_destroy.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -295,6 +325,9 @@ class Multigrid:
         check(lib().saras_mg_error(self.h, res_type, C.byref(v)))
         return v.value
 
+    def set_graphs(self, enable):
+        check(lib().saras_mg_set_graphs(self.h, int(enable)))
+
 
 def stats_dict(st):
     return dict(cycles=st.cycles, refinements=st.refinements, residual=st.residual,
@@ -411,3 +444,59 @@ class Nse:
         ms = C.c_double()
         s = lib().saras_nse_bench_steps(self.h, count, C.byref(ms))
         return s, ms.value
+
+
+def footprint(g, p):
+    return lib().saras_mg_footprint(C.byref(g), C.byref(p))
+
+
+class Server:
+    """The persistent batched request loop (saras_server_*).  Arrays handed
+    to submit() are kept referenced until their completion is polled."""
+
+    def __init__(self, mem_budget, slots, use_graphs=True, keep_resident=True):
+        sp = ServerParams()
+        sp.mem_budget, sp.slots = int(mem_budget), int(slots)
+        sp.use_graphs, sp.keep_resident = int(use_graphs), int(keep_resident)
+        self.h = C.c_void_p()
+        check(lib().saras_server_create(C.byref(sp), C.byref(self.h)))
+        self.live = {}
+
+    def submit(self, g, p, b, x, tag):
+        """b, x: C-contiguous float64 arrays in the caller layout; x is in/out."""
+        assert b.dtype == np.float64 and x.dtype == np.float64 and b.flags.c_contiguous and x.flags.c_contiguous
+        r = Request()
+        r.grid, r.mg, r.b, r.x, r.tag = g, p, b.ctypes.data, x.ctypes.data, int(tag)
+        self.live[int(tag)] = (b, x)
+        check(lib().saras_server_submit(self.h, C.byref(r)))
+
+    def poll(self, max_n=256, wait_ms=0):
+        buf = (Completion * max_n)()
+        n = lib().saras_server_poll(self.h, buf, max_n, int(wait_ms))
+        out = []
+        for q in range(n):
+            c = buf[q]
+            self.live.pop(int(c.tag), None)
+            out.append(dict(tag=c.tag, status=c.status, stats=stats_dict(c.stats), t_submit=c.t_submit,
+                            t_admit=c.t_admit, t_done=c.t_done, message=c.message.decode(errors="replace")))
+        return out
+
+    def drain(self):
+        check(lib().saras_server_drain(self.h))
+
+    def clock(self):
+        return lib().saras_server_clock(self.h)
+
+    def mem_used(self):
+        return lib().saras_server_mem_used(self.h)
+
+    def close(self):
+        if self.h:
+            lib().saras_server_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

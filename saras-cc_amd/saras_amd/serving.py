@@ -1,0 +1,151 @@
+"""Client side of the batched request loop: the Poisson request mix of the
+serving workload and an open/closed-loop driver for saras_server_*.
+
+Request mix: 75 % 3-D cubes with n drawn log-uniformly in [32, 256], 25 % 2-D
+squares with n log-uniform in [64, 2048], each rounded to the nearest
+multiple of 16; boundary conditions uniformly Dirichlet (homogeneous, every
+face), Neumann (every face) or periodic; RHS = 8 seeded Fourier modes
+(sine modes for Dirichlet, cosine modes for Neumann, zero-mean complex
+modes for periodic) with amplitudes ~ U(0.5, 1) |k|^-2; tolerance 1e-8
+(relative L-inf residual).
+"""
+import math
+import time
+
+import numpy as np
+
+from . import (BC_DIRICHLET, BC_NEUMANN, PREC_FP64, PREC_MIXED, Server, grid_desc, mg_params)
+
+KINDS = ("dirichlet", "neumann", "periodic")
+
+
+def draw_requests(count, seed=4, frac3d=0.75, n3=(32, 256), n2=(64, 2048)):
+    rng = np.random.default_rng(seed)
+    out = []
+    for t in range(count):
+        d3 = rng.random() < frac3d
+        lo, hi = n3 if d3 else n2
+        n = int(round(math.exp(rng.uniform(math.log(lo), math.log(hi))) / 16.0)) * 16
+        n = min(max(n, lo), hi)
+        kind = KINDS[int(rng.integers(0, 3))]
+        out.append(dict(tag=t, dim=3 if d3 else 2, n=n, kind=kind, rhs_seed=int(rng.integers(0, 2**31))))
+    return out
+
+
+def request_grid(req):
+    dim, n, kind = req["dim"], req["n"], req["kind"]
+    per = (1, 1, 1) if kind == "periodic" else (0, 0, 0)
+    if dim == 3:
+        return grid_desc((n, n, n), dim=3, periodic=per, pad=1)
+    return grid_desc((n, 1, n), dim=2, periodic=(per[0], 0, per[2]), pad=1)
+
+
+def request_params(req, precision, tol=1e-8):
+    """V(2,2) with the coarsest level smoothed, not solved ("Solve Coarsest"
+    off), at most 100 V-cycles.  Sizes with few factors of two keep a large
+    coarsest level under the reference's depth rule: solving it iteratively
+    costs up to 2.5 N_coarse sweeps per cycle, while smoothing it converges
+    as fast on this mix (35 of 40 sampled requests either way).  A request
+    still above tolerance after 100 cycles completes as NOT_CONVERGED, as the
+    reference's mgSolve returns silently."""
+    bc = BC_DIRICHLET if req["kind"] == "dirichlet" else BC_NEUMANN
+    return mg_params(vc_count=100, vc_tol=tol, solve_flag=0, mg_tol=1e-2, pre=2, post=2, res_type=0, bc=bc,
+                     precision=precision, inner_cycles=4, max_refine=100)
+
+
+def request_rhs(req, modes=8):
+    """Padded (pad 1) RHS of the request: separable Fourier modes on cell centres."""
+    dim, n, kind = req["dim"], req["n"], req["kind"]
+    rng = np.random.default_rng(req["rhs_seed"])
+    xc = (np.arange(n) + 0.5) / n
+    axes = 3 if dim == 3 else 2
+    acc = np.zeros((n,) * axes)
+    for _ in range(modes):
+        if kind == "dirichlet":
+            k = rng.integers(1, 9, axes)
+            f = [np.sin(np.pi * kk * xc) for kk in k]
+            amp = rng.uniform(0.5, 1.0) / float(k @ k)
+        elif kind == "neumann":
+            while True:
+                k = rng.integers(0, 9, axes)
+                if k.max() >= 1:
+                    break
+            f = [np.cos(np.pi * kk * xc) for kk in k]
+            amp = rng.uniform(0.5, 1.0) / float(k @ k)
+        else:
+            while True:
+                k = rng.integers(-4, 5, axes)
+                if np.abs(k).max() >= 1:
+                    break
+            phi = rng.uniform(0, 2 * np.pi)
+            f = [np.exp(2j * np.pi * kk * xc) for kk in k]
+            f[0] = f[0] * np.exp(1j * phi)
+            amp = rng.uniform(0.5, 1.0) / float(k @ k)
+        if axes == 3:
+            m = f[0][:, None, None] * (f[1][:, None] * f[2][None, :])[None, :, :]
+        else:
+            m = f[0][:, None] * f[1][None, :]
+        acc += amp * np.real(m)
+    if kind != "dirichlet":
+        acc -= acc.mean()
+    b = np.zeros(tuple(s + 2 for s in acc.shape))
+    b[(slice(1, -1),) * axes] = acc
+    return b
+
+
+class RhsCache:
+    """RHS arrays by (dim, n, kind, seed)."""
+
+    def __init__(self):
+        self.c = {}
+
+    def get(self, req):
+        key = (req["dim"], req["n"], req["kind"], req["rhs_seed"])
+        if key not in self.c:
+            self.c[key] = request_rhs(req)
+        return self.c[key]
+
+
+def run_stream(server, reqs, rhs, precision, rate=None, collect=None):
+    """Submit `reqs` to `server`: all at once (rate None, closed loop) or as a
+    Poisson process of `rate` requests/s (seeded).  Returns per-request
+    records; `collect` (a set of tags) keeps those solutions."""
+    rng = np.random.default_rng(12345)
+    gaps = rng.exponential(1.0 / rate, len(reqs)) if rate else np.zeros(len(reqs))
+    t_arr = np.cumsum(gaps) - gaps[0]
+    recs, sols = {}, {}
+    xs = {}
+    t0 = server.clock()
+    for q, req in enumerate(reqs):
+        if rate:
+            wait = t0 + t_arr[q] - server.clock()
+            if wait > 0:
+                time.sleep(wait)
+        g = request_grid(req)
+        p = request_params(req, precision)
+        b = rhs.get(req)
+        x = np.zeros_like(b)
+        xs[req["tag"]] = x
+        server.submit(g, p, b, x, req["tag"])
+        for c in server.poll(wait_ms=0):
+            recs[c["tag"]] = c
+    server.drain()
+    for c in server.poll(max_n=len(reqs) + 1):
+        recs[c["tag"]] = c
+    for tag, x in xs.items():
+        if collect is not None and tag in collect:
+            sols[tag] = x
+    return recs, sols
+
+
+def summarize(recs):
+    lat = np.array([r["t_done"] - r["t_submit"] for r in recs.values()])
+    t_first = min(r["t_submit"] for r in recs.values())
+    t_last = max(r["t_done"] for r in recs.values())
+    ok = sum(1 for r in recs.values() if r["status"] in (0,) and r["stats"]["converged"])
+    failed = [r for r in recs.values() if r["status"] not in (0, 1)]
+    if failed:
+        raise RuntimeError(f"{len(failed)} requests failed, first: {failed[0]}")
+    return dict(requests=len(recs), converged=ok, seconds=t_last - t_first,
+                solves_per_s=len(recs) / (t_last - t_first), p50_ms=float(np.percentile(lat, 50) * 1e3),
+                p99_ms=float(np.percentile(lat, 99) * 1e3), mean_ms=float(lat.mean() * 1e3))

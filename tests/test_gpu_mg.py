@@ -284,3 +284,23 @@ def test_mixed_precision_reaches_fp64_tolerance(g_kw, bc, name):
         c_s, c_o = c_s - c_s.mean(), c_o - c_o.mean()
     rel = np.abs(c_s - c_o).max() / np.abs(c_o).max()
     assert rel < 1e-7, rel
+
+
+@pytest.mark.parametrize("case", [((56, 56, 56), 3, (0, 0, 0), (0, 0, 0), O.BC_DIRICHLET),
+                                  ((48, 48, 48), 3, (0, 0, 0), (2, 2, 2), O.BC_DIRICHLET),
+                                  ((224, 1, 224), 2, (0, 0, 0), (0, 0, 0), O.BC_DIRICHLET)],
+                         ids=["3d-56-coarse14", "3d-48-stretched-coarse12", "2d-224-coarse14"])
+def test_full_solve_large_coarsest_level(case):
+    """Grids whose coarsest level alone exceeds the single-workgroup tail's
+    LDS budget (few factors of two): the coarsest solve runs in the
+    multi-level path with global scratch, bit-exact with the oracle."""
+    kw = dict(vc_count=4, vc_tol=1e-12, mg_tol=1e-6, pre=2, post=2)
+    mo, ms = pair(*case, **kw)
+    rng = np.random.default_rng(16)
+    b = rhs_for(mo, rng, False)
+    xo, so = mo.solve(np.zeros_like(b), b, 1, history=True)
+    xs, ss, status = ms.solve(np.zeros_like(b), b)
+    assert ss["cycles"] == so["cycles"]
+    assert ss["residual"] == so["residual"]
+    assert ss["coarse_iters_max"] == so["coarse_iters_max"]
+    np.testing.assert_array_equal(xs, xo)

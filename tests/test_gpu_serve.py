@@ -1,0 +1,100 @@
+"""GPU tests of the serving path: hipGraph-replayed V-cycles against eager
+launches, and the batched request loop (each result the same alone and in
+the mix, memory-based admission).  Bar: bit-identical, since every kernel is
+deterministic and a request's arithmetic does not depend on what else runs."""
+import numpy as np
+import pytest
+
+import saras_amd as S
+from saras_amd import serving as V
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("prec", [S.PREC_FP64, S.PREC_MIXED])
+@pytest.mark.parametrize("sweeps", [(2, 2), (3, 6)])
+@pytest.mark.parametrize("case", [dict(n=(64, 64, 64), periodic=(1, 1, 1)),
+                                  dict(n=(32, 32, 64), periodic=(1, 1, 0), gtype=(0, 0, 2), beta=(1, 1, 0.5)),
+                                  dict(n=(128, 1, 96), dim=2, periodic=(0, 0, 0))],
+                         ids=["ppp64", "ppn-stretched", "2d"])
+def test_graph_replay_matches_eager(prec, sweeps, case):
+    """Three solves in a row on a graph-replaying handle equal the same three
+    solves on an eager handle, with other eager work interleaved between the
+    replays (V(3,6) swaps buffers an odd number of times per cycle, so both
+    cached graph states are exercised)."""
+    g = S.grid_desc(case["n"], dim=case.get("dim", 3), periodic=case["periodic"], gtype=case.get("gtype", (0, 0, 0)),
+                    beta=case.get("beta", (1, 1, 1)), pad=1)
+    p = S.mg_params(vc_count=6, vc_tol=1e-14, solve_flag=1, mg_tol=1e-3, pre=sweeps[0], post=sweeps[1],
+                    bc=S.BC_NEUMANN, precision=prec, inner_cycles=2, max_refine=3)
+    eager, graph, other = S.Multigrid(g, p), S.Multigrid(g, p), S.Multigrid(g, p)
+    graph.set_graphs(1)
+    rng = np.random.default_rng(5)
+    shape = tuple(s + 2 for s in (case["n"] if case.get("dim", 3) == 3 else (case["n"][0], case["n"][2])))
+    for rep in range(3):
+        b = np.zeros(shape)
+        b[(slice(1, -1),) * len(shape)] = rng.standard_normal(tuple(s - 2 for s in shape))
+        b[(slice(1, -1),) * len(shape)] -= b[(slice(1, -1),) * len(shape)].mean()
+        xe, se, _ = eager.solve(np.zeros(shape), b)
+        other.solve(np.zeros(shape), b)           # unrelated eager work between replays
+        xg, sg, _ = graph.solve(np.zeros(shape), b)
+        assert se["cycles"] == sg["cycles"], rep
+        assert np.array_equal(xe, xg), (rep, float(np.abs(xe - xg).max()))
+
+
+def small_mix(count=24, seed=9):
+    reqs = V.draw_requests(count, seed=seed, n3=(32, 64), n2=(64, 256))
+    return reqs
+
+
+@pytest.mark.parametrize("graphs", [0, 1])
+def test_alone_equals_in_the_mix(graphs):
+    reqs = small_mix()
+    rhs = V.RhsCache()
+    mix = S.Server(mem_budget=16 << 30, slots=8, use_graphs=graphs)
+    recs, sols = V.run_stream(mix, reqs, rhs, S.PREC_MIXED, collect={r["tag"] for r in reqs})
+    mix.close()
+    assert len(recs) == len(reqs)
+    for r in reqs:
+        rec = recs[r["tag"]]
+        assert rec["status"] in (S.OK, S.NOT_CONVERGED), (r, rec)
+        assert rec["stats"]["converged"] == (rec["stats"]["residual"] < 1e-8)
+    assert sum(recs[r["tag"]]["stats"]["converged"] for r in reqs) >= len(reqs) // 2
+    # each request alone, on an idle server of one slot, then on a plain handle
+    for r in reqs[:12]:
+        alone = S.Server(mem_budget=16 << 30, slots=1, use_graphs=graphs)
+        rec1, sol1 = V.run_stream(alone, [r], rhs, S.PREC_MIXED, collect={r["tag"]})
+        alone.close()
+        assert np.array_equal(sol1[r["tag"]], sols[r["tag"]]), r
+        assert rec1[r["tag"]]["stats"]["cycles"] == recs[r["tag"]]["stats"]["cycles"]
+        h = S.Multigrid(V.request_grid(r), V.request_params(r, S.PREC_MIXED))
+        x, st, _ = h.solve(np.zeros_like(rhs.get(r)), rhs.get(r))
+        h.close()
+        assert np.array_equal(x, sols[r["tag"]]), r
+
+
+def test_memory_admission_bounds_residency():
+    """A budget of two large hierarchies: everything still completes, requests
+    that do not fit wait or evict idle hierarchies, and resident memory never
+    exceeds the budget once the queue is empty."""
+    reqs = small_mix(count=30, seed=3)
+    rhs = V.RhsCache()
+    big = max(S.footprint(V.request_grid(r), V.request_params(r, S.PREC_MIXED)) for r in reqs)
+    budget = 2 * big
+    srv = S.Server(mem_budget=budget, slots=8, use_graphs=1)
+    recs, _ = V.run_stream(srv, reqs, rhs, S.PREC_MIXED)
+    assert srv.mem_used() <= budget
+    srv.close()
+    assert len(recs) == len(reqs)
+    assert all(r["status"] in (S.OK, S.NOT_CONVERGED) for r in recs.values())
+
+
+def test_request_larger_than_budget_runs_alone():
+    r = dict(tag=0, dim=3, n=64, kind="neumann", rhs_seed=1)
+    rhs = V.RhsCache()
+    srv = S.Server(mem_budget=1 << 20, slots=4, use_graphs=0)
+    recs, sols = V.run_stream(srv, [r], rhs, S.PREC_FP64, collect={0})
+    srv.close()
+    assert recs[0]["status"] in (S.OK, S.NOT_CONVERGED)
+    h = S.Multigrid(V.request_grid(r), V.request_params(r, S.PREC_FP64))
+    x, _, _ = h.solve(np.zeros_like(rhs.get(r)), rhs.get(r))
+    assert np.array_equal(x, sols[0])

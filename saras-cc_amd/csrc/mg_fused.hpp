@@ -95,9 +95,14 @@ __global__ void __launch_bounds__(NP *TJ) k_sweep_march(LevelArgs<T> L, const T 
     constexpr int NT = NP * TJ;
     constexpr int NXP = RX * PX, NHP = RH * PX;
     static_assert(NXP <= 2 * NT && NHP <= 2 * NT, "two pair elements per thread");
-    __shared__ __align__(16) T xs[3][RX * CW];
-    __shared__ __align__(16) T hs[3][RH * CW];
-    __shared__ __align__(16) T bs[3][RH * CW];
+    // FP32: 4-plane rings and one barrier per plane (the black update lags the
+    // red one by two planes, so no ring slot is written while it is read);
+    // FP64: 3-plane rings and two barriers (LDS per workgroup stays ~50 KB).
+    constexpr bool ONEBAR = sizeof(T) == 4;
+    constexpr int NR = ONEBAR ? 4 : 3;
+    __shared__ __align__(16) T xs[NR][RX * CW];
+    __shared__ __align__(16) T hs[NR][RH * CW];
+    __shared__ __align__(16) T bs[NR][RH * CW];
 
     // XCD-aware logical tile id (grid padded to a multiple of kXcds)
     const int per = gridDim.x / kXcds;
@@ -247,8 +252,8 @@ __global__ void __launch_bounds__(NP *TJ) k_sweep_march(LevelArgs<T> L, const T 
     const int qs = (p0 == 0 && !perx) ? 0 : p0 - 1;
     const int qe = (p1 == nx && !perx) ? nx - 1 : p1;
     const int xbase = qs - 1;
-    auto sx_of = [&](int pl) { return (pl - xbase) % 3; };
-    auto sq_of = [&](int q) { return ((q - qs) % 3 + 3) % 3; };
+    auto sx_of = [&](int pl) { return (pl - xbase) % NR; };
+    auto sq_of = [&](int q) { return ((q - qs) % NR + NR) % NR; };
     struct Payload {
         V x0, x1, b0, b1;
     };
@@ -283,13 +288,25 @@ __global__ void __launch_bounds__(NP *TJ) k_sweep_march(LevelArgs<T> L, const T 
         fetch(r, q + 2);
         __syncthreads();
         make_h(sb, sx_of(q - 1), sx_of(q), sxp, q);
-        __syncthreads();
-        const int p = q - 1;
-        if (p >= p0 && p < p1) finish(sq_of(q - 2), sq_of(q - 1), sb, p, p == 0 && !perx, false, sq_of(q - 1));
+        if (ONEBAR) {
+            const int p = q - 2;
+            if (p >= p0 && p < p1)
+                finish(sq_of(q - 3), sq_of(q - 2), sq_of(q - 1), p, p == 0 && !perx, false, sq_of(q - 2));
+        } else {
+            __syncthreads();
+            const int p = q - 1;
+            if (p >= p0 && p < p1) finish(sq_of(q - 2), sq_of(q - 1), sb, p, p == 0 && !perx, false, sq_of(q - 1));
+        }
     };
     for (int q = qs; q <= qe; q += 2) {
         body(q, A);
         if (q + 1 <= qe) body(q + 1, B);
+    }
+    if (ONEBAR) {
+        // the black update of the last H plane's predecessor
+        __syncthreads();
+        const int p = qe - 1;
+        if (p >= p0 && p < p1) finish(sq_of(qe - 2), sq_of(qe - 1), sq_of(qe), p, p == 0 && !perx, false, sq_of(qe - 1));
     }
     if (qe == nx - 1 && p1 == nx && !perx) {
         // top wall: the last plane's x+ neighbour is the wall rule

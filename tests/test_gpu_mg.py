@@ -304,3 +304,47 @@ def test_full_solve_large_coarsest_level(case):
     assert ss["residual"] == so["residual"]
     assert ss["coarse_iters_max"] == so["coarse_iters_max"]
     np.testing.assert_array_equal(xs, xo)
+
+
+@pytest.mark.parametrize("case", [((128, 256, 256), 3, (1, 1, 0), (0, 0, 2), O.BC_NEUMANN),
+                                  ((256, 128, 256), 3, (0, 0, 0), (2, 0, 1), O.BC_DIRICHLET)],
+                         ids=["ppn-z-stretched", "nnn-dirichlet"])
+def test_fused_march_sweeps_bitexact_at_size(case):
+    """Levels large enough for the fused out-of-place march sweep (>= 8M
+    cells): three sweeps of the finest level agree bit for bit with the
+    reference smoother."""
+    mo, ms = pair(*case, sor=1.1, beta=(1.0, 1.2, 1.4))
+    rng = np.random.default_rng(31)
+    shp = mo.padded_shape(0)
+    x, b = rand_pad(shp, rng), rand_pad(shp, rng)
+    mo.set_level(0, x, b)
+    ms.set_level(0, x, b)
+    mo.smooth(0, 3, 0)
+    ms.smooth(0, 3, 0)
+    np.testing.assert_array_equal(inner(ms.get_level(0)[0]), inner(mo.get_level(0)[0]))
+
+
+@pytest.mark.parametrize("variant", ["two-pass", "tj8"])
+def test_fp32_march_matches_fp32_two_pass(variant, monkeypatch):
+    """FP32 cycles of the mixed solver: the fused march sweep (64 x 16 tiles,
+    one barrier per plane) and its 64 x 8 variant give the same FP32
+    arithmetic as the two in-place colour passes, so whole mixed solves are
+    bit-identical."""
+    g = S.grid_desc((128, 256, 256), periodic=(1, 1, 0), gtype=(0, 0, 2), beta=(1, 1, 1.2), pad=1)
+    p = S.mg_params(vc_count=6, vc_tol=1e-14, solve_flag=0, mg_tol=1e-2, pre=2, post=2, bc=S.BC_NEUMANN,
+                    precision=S.PREC_MIXED, inner_cycles=3, max_refine=2)
+    rng = np.random.default_rng(32)
+    b = np.zeros((130, 258, 258))
+    b[1:-1, 1:-1, 1:-1] = rng.standard_normal((128, 256, 256))
+    fused = S.Multigrid(g, p)
+    xf, sf, _ = fused.solve(np.zeros_like(b), b)
+    fused.close()
+    if variant == "two-pass":
+        monkeypatch.setenv("SARAS_NO_FUSED_SWEEP", "1")
+    else:
+        monkeypatch.setenv("SARAS_MARCH_TJ8", "1")
+    other = S.Multigrid(g, p)
+    xo, so, _ = other.solve(np.zeros_like(b), b)
+    other.close()
+    assert sf["cycles"] == so["cycles"]
+    np.testing.assert_array_equal(xf, xo)

@@ -192,6 +192,7 @@ typedef struct {
     double tbc_val[6];
     int pbc_type[6];
     double pbc_val[6];
+    int scheme;              /* "Integration Scheme": 0 lsRK3_d3 (RK3-CN), 1 eulerCN_d3 */
 } saras_nse_params;
 
 typedef struct {
@@ -225,11 +226,12 @@ saras_status saras_nse_get_fields(saras_nse *s, double *vx, double *vy, double *
                                   double *pp, int ptr_kind);
 
 /* lsRK3_d3::timeAdvance(V, P[, T]) (lsRK3_d3.cc:94-177, 190-305): one full
- * time step of three sub-stages.  SARAS_JACOBI_DIVERGED where the reference
+ * time step of three sub-stages; with scheme 1 eulerCN_d3::timeAdvance
+ * (eulerCN_d3.cc:88-165, 178-270), one stage.  SARAS_JACOBI_DIVERGED where the reference
  * aborts (predictors.cc:133-141); a pressure solve that runs out of V-cycles
  * is not an error (the reference is silent). */
 saras_status saras_nse_step(saras_nse *s, saras_nse_stats *stats);
-/* One sub-stage (rkLev = stage in 0..2) of the same loop. */
+/* One sub-stage (rkLev = stage in 0..2) of the same loop (Euler-CN: stage 0). */
 saras_status saras_nse_stage(saras_nse *s, int stage, saras_nse_stats *stats);
 /* The explicit part of a sub-stage alone (lsRK3_d3.cc:199-267): fills the
  * stage RHS and the new non-linear terms; read them with get_rhs (core). */

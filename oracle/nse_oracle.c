@@ -415,8 +415,54 @@ static void fill(double *a, size_t n, double v) { for (size_t q = 0; q < n; q++)
 /* plainvf::multAdd (plainvf.cc:89-95): full arrays */
 static void mult_add(double *y, const double *x, double k, size_t n) { for (size_t q = 0; q < n; q++) y[q] += k * x[q]; }
 
+/* eulerCN_d3::timeAdvance explicit part (eulerCN_d3.cc:92-121, 181-235):
+ * RHS = (nu/2 lap u - H + f - grad P) dt + u, accumulated in the reference's
+ * order into one array */
+static int euler_rhs(or_nse *s)
+{
+    size_t n = s->full;
+    int hasT = s->d.has_T;
+    for (int c = 0; c < 3; c++) {
+        fill(s->nse[c], n, 0.0);
+        compute_diff(s, s->V[c], s->nse[c]);
+        double k = s->d.nu * 0.5;
+        for (size_t q = 0; q < n; q++) s->nse[c][q] *= k;
+    }
+    if (hasT) {
+        fill(s->trhs, n, 0.0);
+        compute_diff(s, s->T, s->trhs);
+        double k = s->d.kappa * 0.5;
+        for (size_t q = 0; q < n; q++) s->trhs[q] *= k;
+    }
+    nlin_v(s, s->nse);
+    if (hasT) nlin_s(s, s->T, s->trhs);
+    if (hasT)
+        for (int c = 0; c < 3; c++)
+            for (size_t q = 0; q < n; q++) s->nse[c][q] -= s->d.buoy[c] * s->T[q];
+    for (int c = 0; c < 3; c++) fill(s->pg[c], n, 0.0);
+    CORE_LOOP(s) {
+        long q = FX(s, i, j, k);
+        s->pg[0][q] = d1(s, s->P, 0, i, j, k);
+        s->pg[1][q] = d1(s, s->P, 1, i, j, k);
+        s->pg[2][q] = d1(s, s->P, 2, i, j, k);
+    }
+    for (int c = 0; c < 3; c++) {
+        for (size_t q = 0; q < n; q++) s->nse[c][q] -= s->pg[c][q];
+        for (size_t q = 0; q < n; q++) s->nse[c][q] *= s->d.dt;
+        for (size_t q = 0; q < n; q++) s->nse[c][q] += s->V[c][q];
+        sync_faces(s, s->nse[c]);
+    }
+    if (hasT) {
+        for (size_t q = 0; q < n; q++) s->trhs[q] *= s->d.dt;
+        for (size_t q = 0; q < n; q++) s->trhs[q] += s->T[q];
+        sync_faces(s, s->trhs);
+    }
+    return 0;
+}
+
 int or_nse_rhs(or_nse *s, int stage)
 {
+    if (s->d.scheme == 1) return euler_rhs(s);
     size_t n = s->full;
     double a = alph[stage], b = alph[stage];
     int hasT = s->d.has_T;
@@ -473,7 +519,8 @@ int or_nse_rhs(or_nse *s, int stage)
 int or_nse_stage(or_nse *s, int stage)
 {
     size_t n = s->full;
-    double a = alph[stage], b = alph[stage];
+    int euler = s->d.scheme == 1;
+    double a = euler ? 0.0 : alph[stage], b = euler ? 0.5 : alph[stage];   /* betaCN2 = 1/2 */
     or_nse_rhs(s, stage);
     for (int c = 0; c < 3; c++) {
         s->jacobi_iters[c] = jacobi(s, s->V[c], s->nse[c], b, s->d.nu, s->d.vbc_type[c], s->d.vbc_val[c]);
@@ -491,7 +538,7 @@ int or_nse_stage(or_nse *s, int stage)
         s->mgr[q] += d1(s, s->V[1], 1, i, j, k);
         s->mgr[q] += d1(s, s->V[2], 2, i, j, k);
     }
-    double sc = 1.0 / ((a + b) * s->d.dt);
+    double sc = euler ? 1.0 / s->d.dt : 1.0 / ((a + b) * s->d.dt);
     for (size_t q = 0; q < n; q++) s->mgr[q] *= sc;
     or_mg_solve(s->mg, s->Pp, s->mgr, s->p, NULL, &s->mg_stats);
     sync_faces(s, s->Pp);
@@ -503,7 +550,7 @@ int or_nse_stage(or_nse *s, int stage)
         s->pg[1][q] = d1(s, s->Pp, 1, i, j, k);
         s->pg[2][q] = d1(s, s->Pp, 2, i, j, k);
     }
-    double pc = (a + b) * s->d.dt;
+    double pc = euler ? s->d.dt : (a + b) * s->d.dt;
     for (int c = 0; c < 3; c++) {
         for (size_t q = 0; q < n; q++) s->pg[c][q] *= pc;
         for (size_t q = 0; q < n; q++) s->V[c][q] -= s->pg[c][q];
@@ -516,7 +563,7 @@ int or_nse_stage(or_nse *s, int stage)
 
 int or_nse_step(or_nse *s)
 {
-    for (int stage = 0; stage < 3; stage++) {
+    for (int stage = 0; stage < (s->d.scheme == 1 ? 1 : 3); stage++) {
         int r = or_nse_stage(s, stage);
         if (r) return r;
     }

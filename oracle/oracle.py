@@ -48,7 +48,7 @@ class NseDesc(C.Structure):
                 ("cn_tol", C.c_double), ("buoy", C.c_double * 3),
                 ("vbc_type", (C.c_int * 6) * 3), ("vbc_val", (C.c_double * 6) * 3),
                 ("tbc_type", C.c_int * 6), ("tbc_val", C.c_double * 6),
-                ("pbc_type", C.c_int * 6), ("pbc_val", C.c_double * 6)]
+                ("pbc_type", C.c_int * 6), ("pbc_val", C.c_double * 6), ("scheme", C.c_int)]
 
 
 class MgStats(C.Structure):
@@ -272,7 +272,7 @@ def set_threads(n):
 
 
 def nse_desc(g, dscheme=2, nlscheme=3, has_T=1, nu=1e-2, kappa=1e-2, dt=1e-3, cn_tol=1e-6,
-             buoy=(0.0, 0.0, -1.0), vbc=None, tbc=None, pbc=None, mg=None):
+             buoy=(0.0, 0.0, -1.0), vbc=None, tbc=None, pbc=None, mg=None, scheme=0):
     """Descriptor of one run.  vbc: [(types[6], vals[6]) per component];
     tbc / pbc: (types[6], vals[6]).  Defaults: the Rayleigh-Benard wiring of
     hydro.cc:287-485 / scalar.cc:129-201 (no-slip, T = 1 bottom, 0 top,
@@ -282,6 +282,7 @@ def nse_desc(g, dscheme=2, nlscheme=3, has_T=1, nu=1e-2, kappa=1e-2, dt=1e-3, cn
     d.mg = mg if mg is not None else mg_params()
     d.dscheme, d.nlscheme, d.has_T = dscheme, nlscheme, int(has_T)
     d.nu, d.kappa, d.dt, d.cn_tol = nu, kappa, dt, cn_tol
+    d.scheme = scheme
     for a in range(3):
         d.buoy[a] = buoy[a]
     if vbc is None:

@@ -139,6 +139,7 @@ typedef struct {
     double tbc_val[6];
     int pbc_type[6];
     double pbc_val[6];
+    int scheme;            /* "Integration Scheme": 0 lsRK3_d3, 1 eulerCN_d3 */
 } or_nse_desc;
 
 typedef struct or_nse or_nse;
@@ -152,12 +153,16 @@ void or_nse_set(or_nse *s, const double *vx, const double *vy, const double *vz,
                 const double *p, const double *t, const double *pp);
 void or_nse_get(const or_nse *s, double *vx, double *vy, double *vz, double *p, double *t, double *pp);
 /* The explicit part of one RK3 sub-stage (lsRK3_d3.cc:199-267): fills the
- * stage RHS (nseRHS, tmpRHS) and leaves the non-linear terms in tempVF/tempSF. */
+ * stage RHS (nseRHS, tmpRHS) and leaves the non-linear terms in tempVF/tempSF.
+ * Euler-CN runs (scheme 1): the explicit part of eulerCN_d3::timeAdvance
+ * (eulerCN_d3.cc:92-121, 181-235); stage is ignored. */
 int or_nse_rhs(or_nse *s, int stage);
 void or_nse_get_rhs(const or_nse *s, double *rx, double *ry, double *rz, double *rt,
                     double *hx, double *hy, double *hz, double *ht);
-/* One RK3 sub-stage (lsRK3_d3.cc:199-304) / one full step; 0, or 2 when a
- * Jacobi solve exceeds maxIterations (predictors.cc:133-141). */
+/* One RK3 sub-stage (lsRK3_d3.cc:199-304) / one full step (three
+ * sub-stages, or the single Euler-CN stage of eulerCN_d3.cc:88-165,
+ * 178-270); 0, or 2 when a Jacobi solve exceeds maxIterations
+ * (predictors.cc:133-141). */
 int or_nse_stage(or_nse *s, int stage);
 int or_nse_step(or_nse *s);
 void or_nse_jacobi_iters(const or_nse *s, int out[4]);

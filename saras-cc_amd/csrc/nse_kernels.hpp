@@ -175,21 +175,28 @@ __device__ __forceinline__ double mor_pair(const Geom &G, const FU &U, const FC 
     return (adv + dv) / 2;
 }
 
-// H_c of computeNLin for advected field C with velocity V (H starts at 0)
+// computeNLin of advected field C with velocity V, subtracted from `acc` in
+// the reference's order (x, y, z terms)
 template <class FV, class FC>
-__device__ __forceinline__ double nlin(const Geom &G, int nlscheme, const FV *V, const FC &Cf, int i, int j, int k) {
-    double h = 0.0;
+__device__ __forceinline__ double nlin_acc(const Geom &G, int nlscheme, const FV *V, const FC &Cf, int i, int j, int k,
+                                           double acc) {
     if (nlscheme == 1) {
         // central: H -= V_a dC/dx_a (vfield.cc:207-249, sfield.cc:162-178)
-        h -= V[0].at(G, i, j, k) * d1(G, Cf, 0, i, j, k);
-        h -= V[1].at(G, i, j, k) * d1(G, Cf, 1, i, j, k);
-        h -= V[2].at(G, i, j, k) * d1(G, Cf, 2, i, j, k);
+        acc -= V[0].at(G, i, j, k) * d1(G, Cf, 0, i, j, k);
+        acc -= V[1].at(G, i, j, k) * d1(G, Cf, 1, i, j, k);
+        acc -= V[2].at(G, i, j, k) * d1(G, Cf, 2, i, j, k);
     } else {
-        h -= mor_pair(G, V[0], Cf, 0, i, j, k);
-        h -= mor_pair(G, V[1], Cf, 1, i, j, k);
-        h -= mor_pair(G, V[2], Cf, 2, i, j, k);
+        acc -= mor_pair(G, V[0], Cf, 0, i, j, k);
+        acc -= mor_pair(G, V[1], Cf, 1, i, j, k);
+        acc -= mor_pair(G, V[2], Cf, 2, i, j, k);
     }
-    return h;
+    return acc;
+}
+
+// H_c of computeNLin into a zeroed array
+template <class FV, class FC>
+__device__ __forceinline__ double nlin(const Geom &G, int nlscheme, const FV *V, const FC &Cf, int i, int j, int k) {
+    return nlin_acc(G, nlscheme, V, Cf, i, j, k, 0.0);
 }
 
 struct StageCoef {
@@ -257,6 +264,45 @@ __global__ void __launch_bounds__(256) k_rhs_t(Geom G, StageCoef c, CoreField V0
     tr += T.at(G, i, j, k);
     H[idx] = h;
     R[idx] = tr;
+}
+
+// The explicit part of the Euler-CN step for velocity component C
+// (eulerCN_d3.cc:181-226): R = ((nu/2) lap u - H - g T - dP/dx_C) dt + u,
+// accumulated in one value in the reference's order.
+template <int C>
+__global__ void __launch_bounds__(256) k_euler_v(Geom G, double k_nu, double dt, double buoy, int has_T,
+                                                 int nlscheme, CoreField V0, CoreField V1, CoreField V2,
+                                                 CoreField T, CoreField P, double *__restrict__ R) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    const int i = blockIdx.z;
+    if (k >= G.n[2]) return;
+    const CoreField V[3] = {V0, V1, V2};
+    const CoreField &U = V[C];
+    double r = laplacian(G, U, i, j, k);
+    r *= k_nu;
+    r = nlin_acc(G, nlscheme, V, U, i, j, k, r);
+    if (has_T) r -= buoy * T.at(G, i, j, k);
+    r -= d1(G, P, C, i, j, k);
+    r *= dt;
+    r += U.at(G, i, j, k);
+    R[(long)i * G.sx + (long)j * G.sy + k] = r;
+}
+
+// ... and for the scalar (eulerCN_d3.cc:188-232); its forcing is zero
+__global__ void __launch_bounds__(256) k_euler_t(Geom G, double k_kappa, double dt, int nlscheme, CoreField V0,
+                                                 CoreField V1, CoreField V2, CoreField T, double *__restrict__ R) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    const int i = blockIdx.z;
+    if (k >= G.n[2]) return;
+    const CoreField V[3] = {V0, V1, V2};
+    double r = laplacian(G, T, i, j, k);
+    r *= k_kappa;
+    r = nlin_acc(G, nlscheme, V, T, i, j, k, r);
+    r *= dt;
+    r += T.at(G, i, j, k);
+    R[(long)i * G.sx + (long)j * G.sy + k] = r;
 }
 
 // Block max of non-negative doubles (NaN wins, as in a failed comparison

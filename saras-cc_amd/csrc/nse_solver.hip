@@ -97,6 +97,7 @@ struct saras_nse {
         if (mg) saras_mg_destroy(mg);
     }
 
+    int stages() const { return p.scheme == 1 ? 1 : 3; }
     dim3 grid3() const { return dim3((G.n[2] + threads - 1) / threads, G.n[1], G.n[0]); }
     dim3 grid3_full(int p) const {
         return dim3((G.n[2] + 2 * p + threads - 1) / threads, G.n[1] + 2 * p, G.n[0] + 2 * p);
@@ -112,7 +113,25 @@ struct saras_nse {
 
 namespace {
 
+// eulerCN_d3 explicit part (eulerCN_d3.cc:92-121, 181-235)
+saras_status rhs_euler(saras_nse *s) {
+    const dim3 gr = s->grid3(), bl(s->threads);
+    const double k_nu = s->p.nu * 0.5, dt = s->p.dt;   // alphCN2 = 1/2
+    hipLaunchKernelGGL(saras::nse::k_euler_v<0>, gr, bl, 0, s->s, s->G, k_nu, dt, s->p.buoy[0], s->p.has_T,
+                       s->p.nlscheme, s->field(0), s->field(1), s->field(2), s->field(3), s->field(4), s->R[0]);
+    hipLaunchKernelGGL(saras::nse::k_euler_v<1>, gr, bl, 0, s->s, s->G, k_nu, dt, s->p.buoy[1], s->p.has_T,
+                       s->p.nlscheme, s->field(0), s->field(1), s->field(2), s->field(3), s->field(4), s->R[1]);
+    hipLaunchKernelGGL(saras::nse::k_euler_v<2>, gr, bl, 0, s->s, s->G, k_nu, dt, s->p.buoy[2], s->p.has_T,
+                       s->p.nlscheme, s->field(0), s->field(1), s->field(2), s->field(3), s->field(4), s->R[2]);
+    if (s->p.has_T)
+        hipLaunchKernelGGL(saras::nse::k_euler_t, gr, bl, 0, s->s, s->G, s->p.kappa * 0.5, dt, s->p.nlscheme,
+                           s->field(0), s->field(1), s->field(2), s->field(3), s->R[3]);
+    NSE_TRY(hipGetLastError());
+    return SARAS_OK;
+}
+
 saras_status rhs(saras_nse *s, int stage) {
+    if (s->p.scheme == 1) return rhs_euler(s);
     saras::nse::StageCoef c{};
     const double a = kAlpha[stage], b = kAlpha[stage];
     c.zeta = kZeta[stage];
@@ -180,7 +199,8 @@ saras_status solve_field(saras_nse *s, int f, double beta, int *iters) {
 }
 
 saras_status stage_run(saras_nse *s, int stage) {
-    const double a = kAlpha[stage], b = kAlpha[stage];
+    const bool euler = s->p.scheme == 1;
+    const double a = euler ? 0.0 : kAlpha[stage], b = euler ? 0.5 : kAlpha[stage];   // betaCN2 = 1/2
     saras_status e = rhs(s, stage);
     if (e) return e;
     const int nf = s->p.has_T ? 4 : 3;
@@ -191,13 +211,13 @@ saras_status stage_run(saras_nse *s, int stage) {
     double *x0, *b0;
     long sx0, sy0;
     saras::mg_level0(s->mg, &x0, &b0, &sx0, &sy0);
-    const double sc = 1.0 / ((a + b) * s->p.dt);
+    const double sc = euler ? 1.0 / s->p.dt : 1.0 / ((a + b) * s->p.dt);
     hipLaunchKernelGGL(saras::nse::k_div, s->grid3(), dim3(s->threads), 0, s->s, s->G, s->field(0), s->field(1),
                        s->field(2), sc, b0, sx0, sy0);
     NSE_TRY(hipGetLastError());
     e = saras::mg_solve_resident(s->mg, s->s, &s->mg_stats);
     if (e && e != SARAS_NOT_CONVERGED) return e;
-    const double pc = (a + b) * s->p.dt;
+    const double pc = euler ? s->p.dt : (a + b) * s->p.dt;
     hipLaunchKernelGGL(saras::nse::k_correct, s->grid3(), dim3(s->threads), 0, s->s, s->G, s->pp(), s->F[4],
                        s->F[0], s->F[1], s->F[2], pc);
     NSE_TRY(hipGetLastError());
@@ -294,8 +314,9 @@ saras_status saras_nse_create(const saras_grid_desc *grid, const saras_mg_params
         saras::set_last_error("the time-step path covers 3-D grids");
         return SARAS_BAD_SHAPE;
     }
-    if ((P.dscheme != 1 && P.dscheme != 2) || (P.nlscheme != 1 && P.nlscheme != 3) || g.pad < P.dscheme) {
-        saras::set_last_error("dscheme must be 1 or 2 with pad >= dscheme; nlscheme 1 or 3");
+    if ((P.dscheme != 1 && P.dscheme != 2) || (P.nlscheme != 1 && P.nlscheme != 3) || g.pad < P.dscheme ||
+        (P.scheme != 0 && P.scheme != 1)) {
+        saras::set_last_error("dscheme must be 1 or 2 with pad >= dscheme; nlscheme 1 or 3; scheme 0 or 1");
         return SARAS_BAD_ARG;
     }
     for (int a = 0; a < 3; ++a)
@@ -495,7 +516,7 @@ saras_status saras_nse_solve_field(saras_nse *s, int field, double beta, int *it
 }
 
 saras_status saras_nse_stage(saras_nse *s, int stage, saras_nse_stats *st) {
-    if (!s || stage < 0 || stage > 2) return SARAS_BAD_ARG;
+    if (!s || stage < 0 || stage >= s->stages()) return SARAS_BAD_ARG;
     NSE_TRY(hipEventRecord(s->ev0, s->s));
     saras_status e = stage_run(s, stage);
     NSE_TRY(hipEventRecord(s->ev1, s->s));
@@ -510,7 +531,7 @@ saras_status saras_nse_step(saras_nse *s, saras_nse_stats *st) {
     if (!s) return SARAS_BAD_ARG;
     NSE_TRY(hipEventRecord(s->ev0, s->s));
     saras_status e = SARAS_OK;
-    for (int k = 0; k < 3 && !e; ++k) e = stage_run(s, k);
+    for (int k = 0; k < s->stages() && !e; ++k) e = stage_run(s, k);
     NSE_TRY(hipEventRecord(s->ev1, s->s));
     NSE_TRY(hipStreamSynchronize(s->s));
     float ms = 0.f;
@@ -525,7 +546,7 @@ saras_status saras_nse_bench_steps(saras_nse *s, int count, double *ms) {
     NSE_TRY(hipEventRecord(s->ev0, s->s));
     saras_status e = SARAS_OK;
     for (int c = 0; c < count && !e; ++c)
-        for (int k = 0; k < 3 && !e; ++k) e = stage_run(s, k);
+        for (int k = 0; k < s->stages() && !e; ++k) e = stage_run(s, k);
     NSE_TRY(hipEventRecord(s->ev1, s->s));
     NSE_TRY(hipStreamSynchronize(s->s));
     float t = 0.f;

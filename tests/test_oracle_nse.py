@@ -355,3 +355,48 @@ def test_diagnostics_conduction_and_uniform_flow():
 def test_max_iterations_formula():
     g, d, s = make(16)
     assert s.max_iterations == int(np.log(16 ** 3) ** 3)
+
+
+def test_euler_cn_heat_mode_amplification():
+    """eulerCN_d3: explicit half of the diffusion, implicit half (beta 1/2),
+    so a Fourier mode is multiplied by (1 + dt kappa lam / 2) / (1 - dt kappa lam / 2)
+    per step."""
+    n, p = 16, 1
+    kappa, dt = 0.02, 5e-3
+    k = (2, 1, 3)
+    g, d, s = make(n, periodic=(1, 1, 1), pad=p, dscheme=1, nlscheme=1, has_T=1, nu=0.0,
+                   kappa=kappa, dt=dt, cn_tol=1e-13, buoy=(0, 0, 0), scheme=1)
+    x = xi_axis(n, p)
+    sh = s.shape
+    X, Y, Z = (along(x, a, sh) for a in range(3))
+    T0 = np.cos(2 * np.pi * (k[0] * X + k[1] * Y + k[2] * Z))
+    zero = np.zeros(sh)
+    s.set(vx=zero, vy=zero, vz=zero, p=zero, t=T0, pp=zero)
+    assert s.step() == 0
+    assert s.step() == 0
+    h = 1.0 / n
+    lam = -sum(4 * np.sin(np.pi * kd * h) ** 2 / h ** 2 for kd in k)
+    G = (1 + 0.5 * dt * kappa * lam) / (1 - 0.5 * dt * kappa * lam)
+    T2 = s.get()["t"]
+    assert np.abs(core(T2, p) - G * G * core(T0, p)).max() < 1e-11
+
+
+def test_euler_cn_taylor_green_decay():
+    n, p = 24, 2
+    nu, dt, steps = 0.01, 2e-3, 5
+    g, d, s = make(n, periodic=(1, 1, 1), has_T=0, nu=nu, kappa=0.0, dt=dt, cn_tol=1e-12, nlscheme=3, scheme=1,
+                   mg=O.mg_params(vc_count=30, vc_tol=1e-12, solve_flag=0, pre=3, post=3))
+    x = xi_axis(n, p)
+    sh = s.shape
+    X, Z = along(x, 0, sh), along(x, 2, sh)
+    tp = 2 * np.pi
+    u = np.sin(tp * X) * np.cos(tp * Z)
+    w = -np.cos(tp * X) * np.sin(tp * Z)
+    P = 0.25 * (np.cos(2 * tp * X) + np.cos(2 * tp * Z))
+    s.set(vx=u, vy=np.zeros(sh), vz=w, p=P, t=np.zeros(sh), pp=np.zeros(sh))
+    ke0 = s.diag()["ke"]
+    for _ in range(steps):
+        assert s.step() == 0
+    ke = s.diag()["ke"]
+    exact = ke0 * np.exp(-2 * tp ** 2 * 2 * nu * steps * dt)
+    assert abs(ke - exact) < 0.03 * (ke0 - exact)

@@ -180,7 +180,7 @@ enum { SARAS_FBC_DIRICHLET = 0, SARAS_FBC_NEUMANN = 1 };
 
 typedef struct {
     int dscheme;             /* "Differentiation Scheme": 1 (2nd order) or 2 (4th order) */
-    int nlscheme;            /* "Non-Linear Term": 1 central, 3 Morinishi */
+    int nlscheme;            /* "Non-Linear Term": 1 central, 2 hybrid upwind, 3 Morinishi */
     int has_T;               /* 1: scalar run, lsRK3_d3::timeAdvance(V, P, T) */
     double nu, kappa;        /* diffusivities (timestep.cc:70-94) */
     double dt;               /* "Time-Step" */
@@ -193,6 +193,8 @@ typedef struct {
     int pbc_type[6];
     double pbc_val[6];
     int scheme;              /* "Integration Scheme": 0 lsRK3_d3 (RK3-CN), 1 eulerCN_d3 */
+    double pe_limit;         /* "Peclet Limit" of the upwind scheme (2.0 in the reference deck) */
+    double central_bias;     /* "Central Bias" of the upwind scheme (0.8 in the reference deck) */
 } saras_nse_params;
 
 typedef struct {
@@ -250,6 +252,11 @@ saras_status saras_nse_solve_field(saras_nse *s, int field, double beta, int *it
  * sfield::computeNLin with the handle's velocity (out += H;
  * sfield.cc:160-311). */
 saras_status saras_field_op(saras_nse *s, int op, const double *f, double *out, int ptr_kind);
+
+/* vfield::computeTStp (vfield.cc:691-706) with the driver's cap
+ * (hydro_d3.cc:221-226): dt = min(courant / max|u|/dx + |v|/dy + |w|/dz, dt_max)
+ * over the core, dx = L/n; the handle's time step becomes *dt. */
+saras_status saras_nse_cfl(saras_nse *s, double courant, double dt_max, double *dt);
 
 /* tseries::writeTSData diagnostics (tseries.cc:209-290): out[0] volume-mean
  * kinetic energy, out[1] max |div V|, out[2] NuVol, out[3] NuWall

@@ -265,6 +265,35 @@ static double mor_div(const or_nse *s, const double *G, int ax, int i, int j, in
         for (int j = 0; j < (s)->n[1]; j++)           \
             for (int k = 0; k < (s)->n[2]; k++)
 
+/* hybrid upwind term of advecting velocity u along axis ax for field F
+ * (vfield.cc:475-689, sfield.cc:324-480): central 2nd/4th order where the
+ * cell Peclet number |u| dh / diffCoeff is below "Peclet Limit", else biased
+ * stencils weighted by "Central Bias"; dScheme 2 edge cells central 2nd. */
+static double upwind(const or_nse *s, double u, const double *F, int ax, int i, int j, int k, double diff)
+{
+    int q = idx_ax(ax, i, j, k), n = s->n[ax];
+    double xm = s->xi_x[ax][q + s->p], i2d = s->i2d[ax];
+    double om = s->d.central_bias;
+    if (s->d.dscheme == 1) {
+        double a = 2.0 - om, b = 2.0 * (1.0 - om), c = -om;
+        double dh = s->x[ax][q + 1 + s->p] - s->x[ax][q + s->p];
+        double pe = fabs(u) * dh / diff;
+        double fm = at(s, F, ax, i, j, k, -1), f = F[FX(s, i, j, k)], fp = at(s, F, ax, i, j, k, 1);
+        if (pe < s->d.pe_limit) return u * xm * (fp - fm) * i2d;
+        if (u > 0) return u * xm * (-a * fm + b * f - c * fp) * i2d;
+        return u * xm * (a * fp - b * f + c * fm) * i2d;
+    }
+    double a = 1.0 - om, b = 4.0 - 3.0 * om, c = 3.0 * (1.0 - om), d = -om;
+    double fm = at(s, F, ax, i, j, k, -1), fp = at(s, F, ax, i, j, k, 1);
+    if (q == 0 || q == n - 1) return u * xm * (fp - fm) * i2d;
+    double dh = s->x[ax][q + 1 + s->p] - s->x[ax][q + s->p];
+    double pe = fabs(u) * dh / diff;
+    double fm2 = at(s, F, ax, i, j, k, -2), f = F[FX(s, i, j, k)], fp2 = at(s, F, ax, i, j, k, 2);
+    if (pe < s->d.pe_limit) return u * xm * (-fp2 + 8.0 * fp - 8.0 * fm + fm2) * i2d / 6.0;
+    if (u > 0) return u * xm * (a * fm2 - b * fm + c * f - d * fp) * i2d;
+    return u * xm * (-a * fp2 + b * fp - c * f + d * fm) * i2d;
+}
+
 /* vfield::computeDiff / sfield::computeDiff: H(core) += d2xx; += d2yy; += d2zz */
 static void compute_diff(or_nse *s, const double *F, double *H)
 {
@@ -280,6 +309,16 @@ static void compute_diff(or_nse *s, const double *F, double *H)
 static void nlin_v(or_nse *s, double *H[3])
 {
     double **V = s->V;
+    if (s->d.nlscheme == 2) {
+        CORE_LOOP(s) {
+            long q = FX(s, i, j, k);
+            for (int a = 0; a < 3; a++) {
+                double u = V[a][q];
+                for (int c = 0; c < 3; c++) H[c][q] -= upwind(s, u, V[c], a, i, j, k, s->d.nu);
+            }
+        }
+        return;
+    }
     if (s->d.nlscheme == 1) {
         for (int c = 0; c < 3; c++)
             CORE_LOOP(s) {
@@ -326,6 +365,13 @@ static void nlin_v(or_nse *s, double *H[3])
 static void nlin_s(or_nse *s, const double *F, double *H)
 {
     double **V = s->V;
+    if (s->d.nlscheme == 2) {
+        CORE_LOOP(s) {
+            long q = FX(s, i, j, k);
+            for (int a = 0; a < 3; a++) H[q] -= upwind(s, V[a][q], F, a, i, j, k, s->d.kappa);
+        }
+        return;
+    }
     if (s->d.nlscheme == 1) {
         CORE_LOOP(s) {
             long q = FX(s, i, j, k);
@@ -680,4 +726,24 @@ void or_nse_diag(or_nse *s, double *ke, double *divmax, double *nuvol, double *n
             }
         *nuwall = dt_sum / (s->d.g.len[0] * s->d.g.len[1] * 2);
     }
+}
+
+double or_nse_cfl(or_nse *s, double courant, double dt_max)
+{
+    double avg[3];
+    for (int a = 0; a < 3; a++) avg[a] = s->d.g.len[a] / s->n[a];
+    double mx = 0.0;
+    int first = 1;
+    for (int i = 0; i < s->n[0]; i++)
+        for (int j = 0; j < s->n[1]; j++)
+            for (int k = 0; k < s->n[2]; k++) {
+                long q = FX(s, i, j, k);
+                double v = (fabs(s->V[0][q]) / avg[0] + fabs(s->V[1][q]) / avg[1]) + fabs(s->V[2][q]) / avg[2];
+                if (first || v > mx) mx = v;
+                first = 0;
+            }
+    double dt = courant / mx;
+    if (dt > dt_max) dt = dt_max;
+    s->d.dt = dt;
+    return dt;
 }

@@ -48,7 +48,8 @@ class NseDesc(C.Structure):
                 ("cn_tol", C.c_double), ("buoy", C.c_double * 3),
                 ("vbc_type", (C.c_int * 6) * 3), ("vbc_val", (C.c_double * 6) * 3),
                 ("tbc_type", C.c_int * 6), ("tbc_val", C.c_double * 6),
-                ("pbc_type", C.c_int * 6), ("pbc_val", C.c_double * 6), ("scheme", C.c_int)]
+                ("pbc_type", C.c_int * 6), ("pbc_val", C.c_double * 6), ("scheme", C.c_int),
+                ("pe_limit", C.c_double), ("central_bias", C.c_double)]
 
 
 class MgStats(C.Structure):
@@ -103,6 +104,8 @@ def lib():
         L.or_nse_mg_stats.argtypes = [C.c_void_p, C.POINTER(MgStats)]
         L.or_nse_op.argtypes = [C.c_void_p, C.c_int, dp, dp]
         L.or_nse_diag.argtypes = [C.c_void_p, dp, dp, dp, dp]
+        L.or_nse_cfl.argtypes = [C.c_void_p, C.c_double, C.c_double]
+        L.or_nse_cfl.restype = C.c_double
         _lib = L
     return _lib
 
@@ -272,7 +275,8 @@ def set_threads(n):
 
 
 def nse_desc(g, dscheme=2, nlscheme=3, has_T=1, nu=1e-2, kappa=1e-2, dt=1e-3, cn_tol=1e-6,
-             buoy=(0.0, 0.0, -1.0), vbc=None, tbc=None, pbc=None, mg=None, scheme=0):
+             buoy=(0.0, 0.0, -1.0), vbc=None, tbc=None, pbc=None, mg=None, scheme=0, pe_limit=2.0,
+             central_bias=0.8):
     """Descriptor of one run.  vbc: [(types[6], vals[6]) per component];
     tbc / pbc: (types[6], vals[6]).  Defaults: the Rayleigh-Benard wiring of
     hydro.cc:287-485 / scalar.cc:129-201 (no-slip, T = 1 bottom, 0 top,
@@ -283,6 +287,7 @@ def nse_desc(g, dscheme=2, nlscheme=3, has_T=1, nu=1e-2, kappa=1e-2, dt=1e-3, cn
     d.dscheme, d.nlscheme, d.has_T = dscheme, nlscheme, int(has_T)
     d.nu, d.kappa, d.dt, d.cn_tol = nu, kappa, dt, cn_tol
     d.scheme = scheme
+    d.pe_limit, d.central_bias = pe_limit, central_bias
     for a in range(3):
         d.buoy[a] = buoy[a]
     if vbc is None:
@@ -360,6 +365,9 @@ class Nse:
         out = np.zeros(self.shape)
         lib().or_nse_op(self.h, op, _dp(F), _dp(out))
         return out
+
+    def cfl(self, courant, dt_max):
+        return lib().or_nse_cfl(self.h, courant, dt_max)
 
     def diag(self):
         v = [C.c_double() for _ in range(4)]

@@ -127,7 +127,7 @@ typedef struct {
     or_grid_desc g;        /* field grid; g.pad = 1 (dScheme 1) or 2 (dScheme 2) */
     or_mg_params mg;       /* pressure-correction solve (bc forced to Neumann, poisson.cc:87) */
     int dscheme;           /* "Differentiation Scheme" 1: 2nd order, 2: 4th order */
-    int nlscheme;          /* "Non-Linear Scheme" 1: central, 3: Morinishi */
+    int nlscheme;          /* "Non-Linear Scheme" 1: central, 2: hybrid upwind, 3: Morinishi */
     int has_T;             /* scalar (Boussinesq) run: lsRK3_d3::timeAdvance(V, P, T) */
     double nu, kappa;      /* timestep.cc:70-94 */
     double dt;             /* "Time-Step" */
@@ -140,6 +140,8 @@ typedef struct {
     int pbc_type[6];
     double pbc_val[6];
     int scheme;            /* "Integration Scheme": 0 lsRK3_d3, 1 eulerCN_d3 */
+    double pe_limit;       /* "Peclet Limit" (upwind scheme) */
+    double central_bias;   /* "Central Bias" omega (upwind scheme) */
 } or_nse_desc;
 
 typedef struct or_nse or_nse;
@@ -174,6 +176,10 @@ void or_nse_op(or_nse *s, int op, const double *F, double *out);
 /* tseries.cc:209-290 diagnostics: volume-mean KE, max |div V|, NuVol, and
  * the wall Nusselt number of z-gravity runs. */
 void or_nse_diag(or_nse *s, double *ke, double *divmax, double *nuvol, double *nuwall);
+/* vfield::computeTStp (vfield.cc:691-706) then the driver's cap
+ * (hydro_d3.cc:221-226): dt = min(courant / max(|u|/dx + |v|/dy + |w|/dz), dt_max);
+ * the run's dt becomes that value. */
+double or_nse_cfl(or_nse *s, double courant, double dt_max);
 
 /* Threads the oracle's OpenMP loops use (0 = runtime default). */
 void or_set_threads(int n);

@@ -44,6 +44,9 @@ struct Geom {
     double i2d[3];                 // vfield.cc:133-135
     double tsih2[3], tsi2h[3];     // timestep.cc:156-173
     const double *xi_x[3], *xixx[3], *xix2[3];   // core cells
+    const double *xc[3];           // cell-centre coordinates over [0, n] (the upwind Peclet spacing)
+    double pe_limit;               // "Peclet Limit"
+    double up1[3], up2[4];         // biased-stencil weights from "Central Bias" (vfield.cc:115-130)
 };
 
 __device__ __forceinline__ int pick(int ax, int i, int j, int k) { return ax == 0 ? i : (ax == 1 ? j : k); }
@@ -175,12 +178,45 @@ __device__ __forceinline__ double mor_pair(const Geom &G, const FU &U, const FC 
     return (adv + dv) / 2;
 }
 
+// hybrid upwind term along axis `a` with advecting velocity u (vfield.cc:
+// 475-689, sfield.cc:324-480): central differences where the cell Peclet
+// number |u| dh / diff is below the limit, else biased stencils
+template <class FC>
+__device__ __forceinline__ double upwind_term(const Geom &G, double u, const FC &Cf, int a, int i, int j, int k,
+                                              double diff) {
+    const int q = pick(a, i, j, k);
+    const double xm = G.xi_x[a][q], i2d = G.i2d[a];
+    const double fm = Cf.nb(G, a, i, j, k, -1), fp = Cf.nb(G, a, i, j, k, 1);
+    if (G.dscheme == 1) {
+        const double dh = G.xc[a][q + 1] - G.xc[a][q];
+        const double pe = fabs(u) * dh / diff;
+        if (pe < G.pe_limit) return u * xm * (fp - fm) * i2d;
+        const double f = Cf.at(G, i, j, k);
+        const double A = G.up1[0], B = G.up1[1], Cc = G.up1[2];
+        if (u > 0) return u * xm * (-A * fm + B * f - Cc * fp) * i2d;
+        return u * xm * (A * fp - B * f + Cc * fm) * i2d;
+    }
+    if (q == 0 || q == G.n[a] - 1) return u * xm * (fp - fm) * i2d;
+    const double dh = G.xc[a][q + 1] - G.xc[a][q];
+    const double pe = fabs(u) * dh / diff;
+    const double fm2 = Cf.nb(G, a, i, j, k, -2), fp2 = Cf.nb(G, a, i, j, k, 2), f = Cf.at(G, i, j, k);
+    if (pe < G.pe_limit) return u * xm * (-fp2 + 8.0 * fp - 8.0 * fm + fm2) * i2d / 6.0;
+    const double A = G.up2[0], B = G.up2[1], Cc = G.up2[2], D = G.up2[3];
+    if (u > 0) return u * xm * (A * fm2 - B * fm + Cc * f - D * fp) * i2d;
+    return u * xm * (-A * fp2 + B * fp - Cc * f + D * fm) * i2d;
+}
+
 // computeNLin of advected field C with velocity V, subtracted from `acc` in
-// the reference's order (x, y, z terms)
+// the reference's order (x, y, z terms); `diff` is the field's diffusion
+// coefficient (the upwind scheme's Peclet number)
 template <class FV, class FC>
 __device__ __forceinline__ double nlin_acc(const Geom &G, int nlscheme, const FV *V, const FC &Cf, int i, int j, int k,
-                                           double acc) {
-    if (nlscheme == 1) {
+                                           double acc, double diff = 1.0) {
+    if (nlscheme == 2) {
+        acc -= upwind_term(G, V[0].at(G, i, j, k), Cf, 0, i, j, k, diff);
+        acc -= upwind_term(G, V[1].at(G, i, j, k), Cf, 1, i, j, k, diff);
+        acc -= upwind_term(G, V[2].at(G, i, j, k), Cf, 2, i, j, k, diff);
+    } else if (nlscheme == 1) {
         // central: H -= V_a dC/dx_a (vfield.cc:207-249, sfield.cc:162-178)
         acc -= V[0].at(G, i, j, k) * d1(G, Cf, 0, i, j, k);
         acc -= V[1].at(G, i, j, k) * d1(G, Cf, 1, i, j, k);
@@ -195,8 +231,9 @@ __device__ __forceinline__ double nlin_acc(const Geom &G, int nlscheme, const FV
 
 // H_c of computeNLin into a zeroed array
 template <class FV, class FC>
-__device__ __forceinline__ double nlin(const Geom &G, int nlscheme, const FV *V, const FC &Cf, int i, int j, int k) {
-    return nlin_acc(G, nlscheme, V, Cf, i, j, k, 0.0);
+__device__ __forceinline__ double nlin(const Geom &G, int nlscheme, const FV *V, const FC &Cf, int i, int j, int k,
+                                       double diff) {
+    return nlin_acc(G, nlscheme, V, Cf, i, j, k, 0.0, diff);
 }
 
 struct StageCoef {
@@ -210,6 +247,7 @@ struct StageCoef {
     int stage;
     int has_T;
     int nlscheme;
+    double nu, kappa;  // the Peclet numbers of the upwind scheme
 };
 
 // The explicit part of one RK3 sub-stage (lsRK3_d3.cc:199-267) for velocity
@@ -236,7 +274,7 @@ __global__ void __launch_bounds__(256) k_rhs_v(Geom G, StageCoef c, CoreField V0
     if (c.has_T) f -= c.buoy[C] * T.at(G, i, j, k);
     f -= d1(G, P, C, i, j, k);
     nse += c.ab * f;
-    const double h = nlin(G, c.nlscheme, V, U, i, j, k);
+    const double h = nlin(G, c.nlscheme, V, U, i, j, k, c.nu);
     nse += c.gamma * h;
     nse *= c.dt;
     nse += U.at(G, i, j, k);
@@ -258,7 +296,7 @@ __global__ void __launch_bounds__(256) k_rhs_t(Geom G, StageCoef c, CoreField V0
     const double diff = laplacian(G, T, i, j, k);
     tr += c.a_kappa * diff;
     tr += c.ab * 0.0;
-    const double h = nlin(G, c.nlscheme, V, T, i, j, k);
+    const double h = nlin(G, c.nlscheme, V, T, i, j, k, c.kappa);
     tr += c.gamma * h;
     tr *= c.dt;
     tr += T.at(G, i, j, k);
@@ -270,7 +308,7 @@ __global__ void __launch_bounds__(256) k_rhs_t(Geom G, StageCoef c, CoreField V0
 // (eulerCN_d3.cc:181-226): R = ((nu/2) lap u - H - g T - dP/dx_C) dt + u,
 // accumulated in one value in the reference's order.
 template <int C>
-__global__ void __launch_bounds__(256) k_euler_v(Geom G, double k_nu, double dt, double buoy, int has_T,
+__global__ void __launch_bounds__(256) k_euler_v(Geom G, double k_nu, double nu, double dt, double buoy, int has_T,
                                                  int nlscheme, CoreField V0, CoreField V1, CoreField V2,
                                                  CoreField T, CoreField P, double *__restrict__ R) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -281,7 +319,7 @@ __global__ void __launch_bounds__(256) k_euler_v(Geom G, double k_nu, double dt,
     const CoreField &U = V[C];
     double r = laplacian(G, U, i, j, k);
     r *= k_nu;
-    r = nlin_acc(G, nlscheme, V, U, i, j, k, r);
+    r = nlin_acc(G, nlscheme, V, U, i, j, k, r, nu);
     if (has_T) r -= buoy * T.at(G, i, j, k);
     r -= d1(G, P, C, i, j, k);
     r *= dt;
@@ -290,7 +328,7 @@ __global__ void __launch_bounds__(256) k_euler_v(Geom G, double k_nu, double dt,
 }
 
 // ... and for the scalar (eulerCN_d3.cc:188-232); its forcing is zero
-__global__ void __launch_bounds__(256) k_euler_t(Geom G, double k_kappa, double dt, int nlscheme, CoreField V0,
+__global__ void __launch_bounds__(256) k_euler_t(Geom G, double k_kappa, double kappa, double dt, int nlscheme, CoreField V0,
                                                  CoreField V1, CoreField V2, CoreField T, double *__restrict__ R) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
@@ -299,7 +337,7 @@ __global__ void __launch_bounds__(256) k_euler_t(Geom G, double k_kappa, double 
     const CoreField V[3] = {V0, V1, V2};
     double r = laplacian(G, T, i, j, k);
     r *= k_kappa;
-    r = nlin_acc(G, nlscheme, V, T, i, j, k, r);
+    r = nlin_acc(G, nlscheme, V, T, i, j, k, r, kappa);
     r *= dt;
     r += T.at(G, i, j, k);
     R[(long)i * G.sx + (long)j * G.sy + k] = r;
@@ -470,6 +508,22 @@ __global__ void __launch_bounds__(kDiagThreads) k_diag(Geom G, CoreField V0, Cor
     }
 }
 
+// vfield::computeTStp (vfield.cc:691-706): per-block maxima of
+// |u|/dx + |v|/dy + |w|/dz over the core
+__global__ void __launch_bounds__(256) k_cfl(Geom G, const double *__restrict__ u, const double *__restrict__ v,
+                                             const double *__restrict__ w, double ax, double ay, double az,
+                                             double *__restrict__ part) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    const int i = blockIdx.z;
+    double m = 0.0;
+    if (k < G.n[2]) {
+        const long idx = (long)i * G.sx + (long)j * G.sy + k;
+        m = (fabs(u[idx]) / ax + fabs(v[idx]) / ay) + fabs(w[idx]) / az;
+    }
+    block_max_store(m, part, ((long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x);
+}
+
 // caller array (pad p) -> core
 __global__ void k_field_load(const double *__restrict__ src, long SX, long SY, int p, double *__restrict__ dst,
                              long sx, long sy, int nx, int ny, int nz) {
@@ -513,7 +567,7 @@ __global__ void k_field_store(Geom G, CoreField F, double *__restrict__ dst, lon
 // 6 computeDiff (out += lap F), 7 sfield::computeNLin with V (out += H).
 template <int OP>
 __global__ void k_field_op(Geom G, PadField F, const CoreField V0, const CoreField V1, const CoreField V2,
-                           int nlscheme, double *__restrict__ out, long SX, long SY, int p) {
+                           int nlscheme, double kappa, double *__restrict__ out, long SX, long SY, int p) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y, i = blockIdx.z;
     if (k >= G.n[2]) return;
@@ -530,17 +584,7 @@ __global__ void k_field_op(Geom G, PadField F, const CoreField V0, const CoreFie
         out[o] = h;
     } else {
         const CoreField V[3] = {V0, V1, V2};
-        double h = out[o];
-        if (nlscheme == 1) {
-            h -= V[0].at(G, i, j, k) * d1(G, F, 0, i, j, k);
-            h -= V[1].at(G, i, j, k) * d1(G, F, 1, i, j, k);
-            h -= V[2].at(G, i, j, k) * d1(G, F, 2, i, j, k);
-        } else {
-            h -= mor_pair(G, V[0], F, 0, i, j, k);
-            h -= mor_pair(G, V[1], F, 1, i, j, k);
-            h -= mor_pair(G, V[2], F, 2, i, j, k);
-        }
-        out[o] = h;
+        out[o] = nlin_acc(G, nlscheme, V, F, i, j, k, out[o], kappa);
     }
 }
 

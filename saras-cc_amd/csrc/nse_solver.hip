@@ -117,14 +117,14 @@ namespace {
 saras_status rhs_euler(saras_nse *s) {
     const dim3 gr = s->grid3(), bl(s->threads);
     const double k_nu = s->p.nu * 0.5, dt = s->p.dt;   // alphCN2 = 1/2
-    hipLaunchKernelGGL(saras::nse::k_euler_v<0>, gr, bl, 0, s->s, s->G, k_nu, dt, s->p.buoy[0], s->p.has_T,
+    hipLaunchKernelGGL(saras::nse::k_euler_v<0>, gr, bl, 0, s->s, s->G, k_nu, s->p.nu, dt, s->p.buoy[0], s->p.has_T,
                        s->p.nlscheme, s->field(0), s->field(1), s->field(2), s->field(3), s->field(4), s->R[0]);
-    hipLaunchKernelGGL(saras::nse::k_euler_v<1>, gr, bl, 0, s->s, s->G, k_nu, dt, s->p.buoy[1], s->p.has_T,
+    hipLaunchKernelGGL(saras::nse::k_euler_v<1>, gr, bl, 0, s->s, s->G, k_nu, s->p.nu, dt, s->p.buoy[1], s->p.has_T,
                        s->p.nlscheme, s->field(0), s->field(1), s->field(2), s->field(3), s->field(4), s->R[1]);
-    hipLaunchKernelGGL(saras::nse::k_euler_v<2>, gr, bl, 0, s->s, s->G, k_nu, dt, s->p.buoy[2], s->p.has_T,
+    hipLaunchKernelGGL(saras::nse::k_euler_v<2>, gr, bl, 0, s->s, s->G, k_nu, s->p.nu, dt, s->p.buoy[2], s->p.has_T,
                        s->p.nlscheme, s->field(0), s->field(1), s->field(2), s->field(3), s->field(4), s->R[2]);
     if (s->p.has_T)
-        hipLaunchKernelGGL(saras::nse::k_euler_t, gr, bl, 0, s->s, s->G, s->p.kappa * 0.5, dt, s->p.nlscheme,
+        hipLaunchKernelGGL(saras::nse::k_euler_t, gr, bl, 0, s->s, s->G, s->p.kappa * 0.5, s->p.kappa, dt, s->p.nlscheme,
                            s->field(0), s->field(1), s->field(2), s->field(3), s->R[3]);
     NSE_TRY(hipGetLastError());
     return SARAS_OK;
@@ -144,6 +144,8 @@ saras_status rhs(saras_nse *s, int stage) {
     c.stage = stage;
     c.has_T = s->p.has_T;
     c.nlscheme = s->p.nlscheme;
+    c.nu = s->p.nu;
+    c.kappa = s->p.kappa;
     const dim3 gr = s->grid3(), bl(s->threads);
     hipLaunchKernelGGL(saras::nse::k_rhs_v<0>, gr, bl, 0, s->s, s->G, c, s->field(0), s->field(1), s->field(2),
                        s->field(3), s->field(4), s->H[0], s->R[0]);
@@ -314,9 +316,9 @@ saras_status saras_nse_create(const saras_grid_desc *grid, const saras_mg_params
         saras::set_last_error("the time-step path covers 3-D grids");
         return SARAS_BAD_SHAPE;
     }
-    if ((P.dscheme != 1 && P.dscheme != 2) || (P.nlscheme != 1 && P.nlscheme != 3) || g.pad < P.dscheme ||
+    if ((P.dscheme != 1 && P.dscheme != 2) || P.nlscheme < 1 || P.nlscheme > 3 || g.pad < P.dscheme ||
         (P.scheme != 0 && P.scheme != 1)) {
-        saras::set_last_error("dscheme must be 1 or 2 with pad >= dscheme; nlscheme 1 or 3; scheme 0 or 1");
+        saras::set_last_error("dscheme must be 1 or 2 with pad >= dscheme; nlscheme 1, 2 or 3; scheme 0 or 1");
         return SARAS_BAD_ARG;
     }
     for (int a = 0; a < 3; ++a)
@@ -359,8 +361,8 @@ saras_status saras_nse_create(const saras_grid_desc *grid, const saras_mg_params
     G.edge_last[2] = g.n[1] - 1;   // vfield.cc:434, sfield.cc:295
     G.dscheme = P.dscheme;
     double hwall[6];
-    std::vector<double> host(9 * (size_t)std::max(g.n[0], std::max(g.n[1], g.n[2])), 0.0);
-    const size_t stride = host.size() / 9;
+    const size_t stride = (size_t)std::max(g.n[0], std::max(g.n[1], g.n[2])) + 1;
+    std::vector<double> host(12 * stride, 0.0);
     for (int a = 0; a < 3; ++a) {
         std::vector<double> coord, xi_x, xixx, xix2;
         saras::field_axis(g, a, coord, xi_x, xixx, xix2);
@@ -370,6 +372,7 @@ saras_status saras_nse_create(const saras_grid_desc *grid, const saras_mg_params
             host[(3 * a + 1) * stride + i] = xixx[i + p];
             host[(3 * a + 2) * stride + i] = xix2[i + p];
         }
+        for (int i = 0; i <= n; ++i) host[(9 + a) * stride + i] = coord[i + p];
         hwall[2 * a] = 2 * coord[p];
         hwall[2 * a + 1] = 2 * (coord[n - 1 + p] - g.len[a]);
         // derivative.cc:52-54, vfield.cc:133-135, timestep.cc:156-173
@@ -393,6 +396,19 @@ saras_status saras_nse_create(const saras_grid_desc *grid, const saras_mg_params
         G.xi_x[a] = s->metrics + (3 * a + 0) * stride;
         G.xixx[a] = s->metrics + (3 * a + 1) * stride;
         G.xix2[a] = s->metrics + (3 * a + 2) * stride;
+        G.xc[a] = s->metrics + (9 + a) * stride;
+    }
+    // upwind weights (vfield.cc:115-130)
+    {
+        const double om = P.central_bias;
+        G.pe_limit = P.pe_limit;
+        G.up1[0] = 2.0 - om;
+        G.up1[1] = 2.0 * (1.0 - om);
+        G.up1[2] = -om;
+        G.up2[0] = 1.0 - om;
+        G.up2[1] = 4.0 - 3.0 * om;
+        G.up2[2] = 3.0 * (1.0 - om);
+        G.up2[3] = -om;
     }
     for (int c = 0; c < 3; ++c) s->bc[c] = make_bc(P.vbc_type[c], P.vbc_val[c], hwall);
     s->bc[3] = make_bc(P.tbc_type, P.tbc_val, hwall);
@@ -574,19 +590,36 @@ saras_status saras_field_op(saras_nse *s, int op, const double *f, double *out, 
     const CoreField v0 = s->field(0), v1 = s->field(1), v2 = s->field(2);
     const dim3 gr = s->grid3(), bl(s->threads);
     switch (op) {
-        case 0: hipLaunchKernelGGL(saras::nse::k_field_op<0>, gr, bl, 0, s->s, s->G, pf, v0, v1, v2, s->p.nlscheme, dout, SX, SY, p); break;
-        case 1: hipLaunchKernelGGL(saras::nse::k_field_op<1>, gr, bl, 0, s->s, s->G, pf, v0, v1, v2, s->p.nlscheme, dout, SX, SY, p); break;
-        case 2: hipLaunchKernelGGL(saras::nse::k_field_op<2>, gr, bl, 0, s->s, s->G, pf, v0, v1, v2, s->p.nlscheme, dout, SX, SY, p); break;
-        case 3: hipLaunchKernelGGL(saras::nse::k_field_op<3>, gr, bl, 0, s->s, s->G, pf, v0, v1, v2, s->p.nlscheme, dout, SX, SY, p); break;
-        case 4: hipLaunchKernelGGL(saras::nse::k_field_op<4>, gr, bl, 0, s->s, s->G, pf, v0, v1, v2, s->p.nlscheme, dout, SX, SY, p); break;
-        case 5: hipLaunchKernelGGL(saras::nse::k_field_op<5>, gr, bl, 0, s->s, s->G, pf, v0, v1, v2, s->p.nlscheme, dout, SX, SY, p); break;
-        case 6: hipLaunchKernelGGL(saras::nse::k_field_op<6>, gr, bl, 0, s->s, s->G, pf, v0, v1, v2, s->p.nlscheme, dout, SX, SY, p); break;
-        default: hipLaunchKernelGGL(saras::nse::k_field_op<7>, gr, bl, 0, s->s, s->G, pf, v0, v1, v2, s->p.nlscheme, dout, SX, SY, p); break;
+        case 0: hipLaunchKernelGGL(saras::nse::k_field_op<0>, gr, bl, 0, s->s, s->G, pf, v0, v1, v2, s->p.nlscheme, s->p.kappa, dout, SX, SY, p); break;
+        case 1: hipLaunchKernelGGL(saras::nse::k_field_op<1>, gr, bl, 0, s->s, s->G, pf, v0, v1, v2, s->p.nlscheme, s->p.kappa, dout, SX, SY, p); break;
+        case 2: hipLaunchKernelGGL(saras::nse::k_field_op<2>, gr, bl, 0, s->s, s->G, pf, v0, v1, v2, s->p.nlscheme, s->p.kappa, dout, SX, SY, p); break;
+        case 3: hipLaunchKernelGGL(saras::nse::k_field_op<3>, gr, bl, 0, s->s, s->G, pf, v0, v1, v2, s->p.nlscheme, s->p.kappa, dout, SX, SY, p); break;
+        case 4: hipLaunchKernelGGL(saras::nse::k_field_op<4>, gr, bl, 0, s->s, s->G, pf, v0, v1, v2, s->p.nlscheme, s->p.kappa, dout, SX, SY, p); break;
+        case 5: hipLaunchKernelGGL(saras::nse::k_field_op<5>, gr, bl, 0, s->s, s->G, pf, v0, v1, v2, s->p.nlscheme, s->p.kappa, dout, SX, SY, p); break;
+        case 6: hipLaunchKernelGGL(saras::nse::k_field_op<6>, gr, bl, 0, s->s, s->G, pf, v0, v1, v2, s->p.nlscheme, s->p.kappa, dout, SX, SY, p); break;
+        default: hipLaunchKernelGGL(saras::nse::k_field_op<7>, gr, bl, 0, s->s, s->G, pf, v0, v1, v2, s->p.nlscheme, s->p.kappa, dout, SX, SY, p); break;
     }
     NSE_TRY(hipGetLastError());
     if (ptr_kind == SARAS_PTR_HOST)
         NSE_TRY(hipMemcpyAsync(out, s->stage_b, s->full * sizeof(double), hipMemcpyDeviceToHost, s->s));
     NSE_TRY(hipStreamSynchronize(s->s));
+    return SARAS_OK;
+}
+
+saras_status saras_nse_cfl(saras_nse *s, double courant, double dt_max, double *dt) {
+    if (!s || !dt || !(courant > 0)) return SARAS_BAD_ARG;
+    // avgDx = xLen / globalSize(0) (vfield.cc:91-95)
+    const double ax = s->g.len[0] / s->g.n[0], ay = s->g.len[1] / s->g.n[1], az = s->g.len[2] / s->g.n[2];
+    hipLaunchKernelGGL(saras::nse::k_cfl, s->grid3(), dim3(s->threads), 0, s->s, s->G, s->F[0], s->F[1], s->F[2],
+                       ax, ay, az, s->part);
+    hipLaunchKernelGGL(saras::nse::k_max_final, dim3(1), dim3(1024), 0, s->s, s->part, s->nblocks, s->res_dev);
+    NSE_TRY(hipGetLastError());
+    NSE_TRY(hipMemcpyAsync(s->res_host, s->res_dev, sizeof(double), hipMemcpyDeviceToHost, s->s));
+    NSE_TRY(hipStreamSynchronize(s->s));
+    double d = courant / s->res_host[0];
+    if (d > dt_max) d = dt_max;
+    s->p.dt = d;
+    *dt = d;
     return SARAS_OK;
 }
 

@@ -55,7 +55,8 @@ class NseParams(C.Structure):
                 ("cn_tol", C.c_double), ("buoy", C.c_double * 3),
                 ("vbc_type", (C.c_int * 6) * 3), ("vbc_val", (C.c_double * 6) * 3),
                 ("tbc_type", C.c_int * 6), ("tbc_val", C.c_double * 6),
-                ("pbc_type", C.c_int * 6), ("pbc_val", C.c_double * 6), ("scheme", C.c_int)]
+                ("pbc_type", C.c_int * 6), ("pbc_val", C.c_double * 6), ("scheme", C.c_int),
+                ("pe_limit", C.c_double), ("central_bias", C.c_double)]
 
 
 class NseStats(C.Structure):
@@ -88,7 +89,7 @@ EXPORTS = [
     "saras_bench_stream_copy", "saras_nse_create", "saras_nse_destroy", "saras_nse_set_fields",
     "saras_nse_get_fields", "saras_nse_step", "saras_nse_stage", "saras_nse_rhs", "saras_nse_get_rhs",
     "saras_nse_solve_field", "saras_field_op", "saras_nse_diag", "saras_nse_bench_steps",
-    "saras_mg_set_graphs", "saras_mg_footprint", "saras_server_create", "saras_server_submit",
+    "saras_nse_cfl", "saras_mg_set_graphs", "saras_mg_footprint", "saras_server_create", "saras_server_submit",
     "saras_server_poll", "saras_server_drain", "saras_server_clock", "saras_server_mem_used",
     "saras_server_destroy",
 ]
@@ -139,6 +140,7 @@ def lib():
         L.saras_field_op.argtypes = [vp, C.c_int, vp, vp, C.c_int]
         L.saras_nse_diag.argtypes = [vp, dp]
         L.saras_nse_bench_steps.argtypes = [vp, C.c_int, dp]
+        L.saras_nse_cfl.argtypes = [vp, C.c_double, C.c_double, dp]
         L.saras_mg_set_graphs.argtypes = [vp, C.c_int]
         L.saras_mg_footprint.argtypes = [C.POINTER(GridDesc), C.POINTER(MgParams)]
         L.saras_mg_footprint.restype = C.c_size_t
@@ -343,7 +345,7 @@ def bench_stream_copy(n_doubles, reps):
 
 
 def nse_params(dscheme=2, nlscheme=3, has_T=1, nu=1e-2, kappa=1e-2, dt=1e-3, cn_tol=1e-6,
-               buoy=(0.0, 0.0, -1.0), vbc=None, tbc=None, pbc=None, scheme=0):
+               buoy=(0.0, 0.0, -1.0), vbc=None, tbc=None, pbc=None, scheme=0, pe_limit=2.0, central_bias=0.8):
     """Time-step parameters.  vbc: [(types[6], vals[6]) per component];
     tbc / pbc: (types[6], vals[6]).  Defaults: the Rayleigh-Benard wiring of
     hydro.cc:287-485 / scalar.cc:129-201 (no-slip walls, T = 1 at the bottom
@@ -352,6 +354,7 @@ def nse_params(dscheme=2, nlscheme=3, has_T=1, nu=1e-2, kappa=1e-2, dt=1e-3, cn_
     d.dscheme, d.nlscheme, d.has_T = dscheme, nlscheme, int(has_T)
     d.nu, d.kappa, d.dt, d.cn_tol = nu, kappa, dt, cn_tol
     d.scheme = scheme
+    d.pe_limit, d.central_bias = pe_limit, central_bias
     for a in range(3):
         d.buoy[a] = buoy[a]
     if vbc is None:
@@ -435,6 +438,11 @@ class Nse:
         out = np.zeros(self.shape) if out is None else np.array(out, dtype=np.float64)
         check(lib().saras_field_op(self.h, op, F.ctypes.data, out.ctypes.data, PTR_HOST))
         return out
+
+    def cfl(self, courant, dt_max):
+        dt = C.c_double()
+        check(lib().saras_nse_cfl(self.h, courant, dt_max, C.byref(dt)))
+        return dt.value
 
     def diag(self):
         v = np.zeros(4)

@@ -270,3 +270,49 @@ def test_euler_cn_bitexact(nlscheme):
         assert r == 0
         assert so.jacobi_iters() == st["jacobi_iters"]
         assert_fields_equal(so, ss, periodic=(1, 0, 0))
+
+
+@pytest.mark.parametrize("dscheme", [1, 2])
+def test_upwind_scheme_bitexact(dscheme):
+    """Hybrid upwind non-linear term (Non-Linear Term 2): Peclet numbers on
+    both sides of the limit, both biasing directions; the scalar operator,
+    the stage RHS and two RK3 steps agree bit for bit."""
+    pad = 2
+    kw = dict(periodic=(1, 0, 0), gtype=(0, 2, 1), beta=(1.0, 1.2, 0.9), pad=pad, nu=0.03, kappa=0.02, dt=1e-3,
+              cn_tol=1e-9, nlscheme=2, dscheme=dscheme, central_bias=0.7, pe_limit=2.0)
+    so, ss = pair((12, 12, 12), **kw)
+    rng = np.random.default_rng(19)
+    sh = so.shape
+    f = dict(vx=rng.standard_normal(sh), vy=rng.standard_normal(sh), vz=rng.standard_normal(sh),
+             p=rng.standard_normal(sh), t=rng.standard_normal(sh), pp=np.zeros(sh))
+    so.set(**f)
+    ss.set(**f)
+    c = (slice(pad, -pad),) * 3
+    F = rng.standard_normal(sh)
+    assert np.array_equal(so.op(7, F)[c], ss.op(7, F)[c])
+    ro, rs = so.rhs(0), ss.rhs(0)
+    for k in ro:
+        assert np.array_equal(ro[k][c], rs[k][c]), k
+    so.set(**f)
+    ss.set(**f)
+    for _ in range(2):
+        assert so.step() == 0
+        assert ss.step()[0] == 0
+        assert_fields_equal(so, ss, periodic=(1, 0, 0))
+
+
+def test_cfl_time_step_matches_oracle():
+    pad = 2
+    so, ss = pair((16, 16, 16), pad=pad)
+    rng = np.random.default_rng(20)
+    sh = so.shape
+    f = dict(vx=rng.standard_normal(sh), vy=rng.standard_normal(sh), vz=rng.standard_normal(sh),
+             p=np.zeros(sh), t=np.zeros(sh), pp=np.zeros(sh))
+    so.set(**f)
+    ss.set(**f)
+    assert so.cfl(0.1, 1.0) == ss.cfl(0.1, 1.0)
+    assert so.cfl(0.1, 1e-6) == ss.cfl(0.1, 1e-6) == 1e-6
+    # the next step uses the capped dt on both sides
+    assert so.step() == 0
+    assert ss.step()[0] == 0
+    assert_fields_equal(so, ss)

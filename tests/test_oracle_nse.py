@@ -400,3 +400,77 @@ def test_euler_cn_taylor_green_decay():
     ke = s.diag()["ke"]
     exact = ke0 * np.exp(-2 * tp ** 2 * 2 * nu * steps * dt)
     assert abs(ke - exact) < 0.03 * (ke0 - exact)
+
+
+def _upwind_case(dscheme, nu, bias, u0, poly):
+    n, p = 12, 2
+    g, d, s = make(n, periodic=(0, 0, 0), gtype=(2, 0, 1), beta=(1.2, 1.0, 0.8), dscheme=dscheme, nlscheme=2,
+                   nu=nu, kappa=nu, central_bias=bias, pe_limit=2.0)
+    xi = xi_axis(n, p)
+    sh = s.shape
+    X = along(xi, 0, sh)
+    zero = np.zeros(sh)
+    s.set(vx=np.full(sh, u0), vy=zero, vz=zero, p=zero, t=zero, pp=zero)
+    F = poly(X)
+    H = s.op(7, F)
+    m = O.field_metrics(g, 0)["xi_x"]
+    exp = -u0 * along(m, 0, sh) * poly.deriv(1)(X)
+    return H, exp, p, n
+
+
+@pytest.mark.parametrize("u0", [0.7, -0.7])
+def test_upwind_second_order_biased_exact_on_quadratics(u0):
+    """Pure upwinding (Central Bias 0) at high Peclet number: the one-sided
+    second-order stencils (3F - 4F- + F--)/2h and (-F++ + 4F+ - 3F)/2h
+    are exact for quadratics (interior cells)."""
+    H, exp, p, n = _upwind_case(2, 1e-9, 0.0, u0, POLY2)
+    sl = (slice(p + 2, -p - 2),) + (slice(p, -p),) * 2
+    assert np.abs(H[sl] - exp[sl]).max() < 1e-9
+
+
+def test_upwind_first_order_exact_on_linears():
+    H, exp, p, n = _upwind_case(1, 1e-9, 0.0, 0.9, np.polynomial.Polynomial([0.4, -1.3]))
+    sl = (slice(p, -p),) * 3
+    assert np.abs(H[sl] - exp[sl]).max() < 1e-9
+
+
+def test_upwind_low_peclet_is_central():
+    """Below the Peclet limit the scheme is the central one: 4th order
+    interior / 2nd order edges, the same operator as the central NL scheme."""
+    n, p = 12, 2
+    rng = np.random.default_rng(3)
+    F = rng.standard_normal((n + 2 * p,) * 3)
+    V = {k: 0.01 * rng.standard_normal((n + 2 * p,) * 3) for k in ("vx", "vy", "vz")}
+    out = []
+    for nl in (2, 1):
+        g, d, s = make(n, periodic=(1, 0, 0), gtype=(0, 2, 1), beta=(1, 1.1, 0.9), dscheme=2, nlscheme=nl,
+                       nu=10.0, kappa=10.0)
+        s.set(p=np.zeros_like(F), t=np.zeros_like(F), pp=np.zeros_like(F), **V)
+        out.append(core(s.op(7, F), p))
+    assert np.abs(out[0] - out[1]).max() < 1e-12 * max(1.0, np.abs(out[1]).max())
+
+
+def test_upwind_full_bias_is_second_order_central():
+    """Central Bias 1 turns the biased stencil into (F+ - F-)/2h."""
+    n, p = 12, 2
+    H, _, _, _ = _upwind_case(2, 1e-9, 1.0, 0.7, POLY4)
+    g = O.grid_desc(n, periodic=(0, 0, 0), gtype=(2, 0, 1), beta=(1.2, 1.0, 0.8), pad=p)
+    m = O.field_metrics(g, 0)["xi_x"]
+    xi = xi_axis(n, p)
+    F = POLY4(xi)
+    c2 = -0.7 * m[p:-p] * (F[p + 1:n + p + 1] - F[p - 1:n + p - 1]) * (n / 2.0)
+    got = H[p:-p, p + 3, p + 3]
+    assert np.abs(got - c2).max() < 1e-9
+
+
+def test_cfl_time_step():
+    n, p = 8, 2
+    g, d, s = make(n)
+    sh = s.shape
+    rng = np.random.default_rng(2)
+    V = {k: rng.standard_normal(sh) for k in ("vx", "vy", "vz")}
+    s.set(p=np.zeros(sh), t=np.zeros(sh), pp=np.zeros(sh), **V)
+    c = (slice(p, -p),) * 3
+    mx = (np.abs(V["vx"][c]) * n + np.abs(V["vy"][c]) * n + np.abs(V["vz"][c]) * n).max()
+    assert abs(s.cfl(0.1, 1.0) - 0.1 / mx) < 1e-15
+    assert s.cfl(0.1, 1e-5) == 1e-5

@@ -45,6 +45,9 @@ struct or_nse {
     double time;
     int jacobi_iters[4];
     or_mg_stats mg_stats;
+    double tstp;            /* the input time step ("Time-Step"), for the LES start */
+    double *Tsg[6];         /* sub-grid stresses Txx, Tyy, Tzz, Txy, Tyz, Tzx */
+    double sgs_diag[3];
 };
 
 #define FX(S, i, j, k) ((long)((i) + (S)->p) * (S)->SX + (long)((j) + (S)->p) * (S)->SY + (long)((k) + (S)->p))
@@ -86,6 +89,8 @@ or_nse *or_nse_create(const or_nse_desc *d)
     mp.bc = OR_BC_NEUMANN;   /* allNeumann (poisson.cc:87) */
     s->mg = or_mg_create(&d->g, &mp);
     s->max_iter = (int)pow(log((double)s->n[0] * s->n[1] * s->n[2]), 3);  /* timestep.cc:115 */
+    s->tstp = d->dt;
+    for (int q = 0; q < 6; q++) s->Tsg[q] = zalloc(s->full);
     return s;
 }
 
@@ -96,6 +101,7 @@ void or_nse_destroy(or_nse *s)
     for (int c = 0; c < 3; c++) { free(s->V[c]); free(s->nse[c]); free(s->tv[c]); free(s->pg[c]); }
     free(s->P); free(s->T); free(s->Pp); free(s->trhs); free(s->ts); free(s->mgr);
     free(s->iter); free(s->tmpA); free(s->tmpB);
+    for (int q = 0; q < 6; q++) free(s->Tsg[q]);
     or_mg_destroy(s->mg);
     free(s);
 }
@@ -461,6 +467,224 @@ static void fill(double *a, size_t n, double v) { for (size_t q = 0; q < n; q++)
 /* plainvf::multAdd (plainvf.cc:89-95): full arrays */
 static void mult_add(double *y, const double *x, double k, size_t n) { for (size_t q = 0; q < n; q++) y[q] += k * x[q]; }
 
+/* ---- stretched spiral vortex LES (lib/les/spiral.cc) -------------------- */
+#define SGS_EPS (2e-3)   /* spiral.cc:49 */
+#define SGS_PI 3.14159265358979323846   /* M_PI */
+
+typedef struct {
+    double Sxx, Syy, Szz, Sxy, Syz, Szx;
+    double e[3], K;
+} sgs_cell;
+
+/* spiral::keIntegral (spiral.cc:590-607) */
+static double ke_integral(double k)
+{
+    double k2 = k * k;
+    if (k2 < 2.42806) {
+        double pade = (3.0 + 2.5107 * k2 + 0.330357 * k2 * k2 + 0.0295481 * k2 * k2 * k2) /
+                      (1.0 + 0.336901 * k2 + 0.0416684 * k2 * k2 + 0.00187191 * k2 * k2 * k2);
+        return 0.5 * (pade - 4.06235 * pow(k2, 1.0 / 3.0));
+    }
+    double pade = (1.26429 + 0.835714 * k2 + 0.0964286 * k2 * k2) /
+                  (1.0 + 2.25 * k2 + 0.964286 * k2 * k2 + 0.0964286 * k2 * k2 * k2);
+    return 0.5 * pade * exp(-k2);
+}
+
+/* spiral::sfIntegral (spiral.cc:620-631) */
+static double sf_integral(double d)
+{
+    double d2 = d * d;
+    if (d < 0.873469) return 7.4022 * d2 - 1.82642 * d2 * d2;
+    return 12.2946 * pow(d, 2.0 / 3.0) - 6.0 - 0.573159 * pow(d, -1.5) * sin(3.14159 * d - 0.785398);
+}
+
+/* spiral::eigenvalueSymm (spiral.cc:643-~690): largest eigenvalue of S;
+ * returns 0, or 3 where the reference aborts (q >= 0) */
+static int eigval_symm(const sgs_cell *c, double *out)
+{
+    double ev[3];
+    double a = -(c->Sxx + c->Syy + c->Szz);
+    double b = c->Sxx * c->Syy - c->Sxy * c->Sxy + c->Syy * c->Szz - c->Syz * c->Syz + c->Szz * c->Sxx -
+               c->Szx * c->Szx;
+    double cc = -(c->Sxx * (c->Syy * c->Szz - c->Syz * c->Syz) + c->Sxy * (c->Syz * c->Szx - c->Sxy * c->Szz) +
+                  c->Szx * (c->Sxy * c->Syz - c->Syy * c->Szx));
+    double q = (3.0 * b - a * a) / 9.0;
+    double r = (9.0 * a * b - 27.0 * cc - 2.0 * a * a * a) / 54.0;
+    if (q >= 0.0) return 3;
+    double costheta = r / sqrt(-q * q * q);
+    double theta = costheta > 1.0 ? 0.0 : costheta < -1.0 ? SGS_PI : acos(costheta);
+    ev[0] = 2.0 * sqrt(-q) * cos((theta) / 3.0) - a / 3.0;
+    ev[1] = 2.0 * sqrt(-q) * cos((theta + 2.0 * SGS_PI) / 3.0) - a / 3.0;
+    ev[2] = 2.0 * sqrt(-q) * cos((theta + 4.0 * SGS_PI) / 3.0) - a / 3.0;
+    double t;
+    if (ev[0] > ev[1]) { t = ev[0]; ev[0] = ev[1]; ev[1] = t; }
+    if (ev[1] > ev[2]) { t = ev[1]; ev[1] = ev[2]; ev[2] = t; }
+    if (ev[0] > ev[1]) { t = ev[0]; ev[0] = ev[1]; ev[1] = t; }
+    *out = ev[2];
+    return 0;
+}
+
+/* spiral::eigenvectorSymm (spiral.cc:702-~742) */
+static int eigvec_symm(const sgs_cell *c, double ev, double v[3])
+{
+    double Sxx = c->Sxx, Syy = c->Syy, Szz = c->Szz, Sxy = c->Sxy, Syz = c->Syz, Szx = c->Szx;
+    double fNorm = sqrt(Sxx * Sxx + Syy * Syy + Szz * Szz + Sxy * Sxy + Syz * Syz + Szx * Szx);
+    if (fabs((Sxx - ev) * ((Syy - ev) * (Szz - ev) - Syz * Syz) + Sxy * (Syz * Szx - Sxy * (Szz - ev)) +
+             Szx * (Sxy * Syz - (Syy - ev) * Szx)) / fabs(fNorm) > SGS_EPS)
+        return 3;
+    double det[3] = {(Syy - ev) * (Szz - ev) - Syz * Syz, (Szz - ev) * (Sxx - ev) - Szx * Szx,
+                     (Sxx - ev) * (Syy - ev) - Sxy * Sxy};
+    double f[3] = {fabs(det[0]), fabs(det[1]), fabs(det[2])};
+    if (f[0] >= f[1] && f[0] >= f[2]) {
+        v[0] = 1.0;
+        v[1] = (-Sxy * (Szz - ev) + Szx * Syz) / det[0];
+        v[2] = (-Szx * (Syy - ev) + Sxy * Syz) / det[0];
+    } else if (f[1] >= f[2] && f[1] >= f[0]) {
+        v[0] = (-Sxy * (Szz - ev) + Syz * Szx) / det[1];
+        v[1] = 1.0;
+        v[2] = (-Syz * (Sxx - ev) + Sxy * Szx) / det[1];
+    } else if (f[2] >= f[0] && f[2] >= f[1]) {
+        v[0] = (-Szx * (Syy - ev) + Syz * Sxy) / det[2];
+        v[1] = (-Syz * (Sxx - ev) + Szx * Sxy) / det[2];
+        v[2] = 1.0;
+    } else {
+        return 3;
+    }
+    return 0;
+}
+
+/* spiral::sgsStress (spiral.cc:472-548) at core cell (i,j,k) */
+static int sgs_stress(const or_nse *s, int i, int j, int k, const double dudx[3][3], double del, sgs_cell *c,
+                      double T[6])
+{
+    c->Sxx = 0.5 * (dudx[0][0] + dudx[0][0]);
+    c->Syy = 0.5 * (dudx[1][1] + dudx[1][1]);
+    c->Szz = 0.5 * (dudx[2][2] + dudx[2][2]);
+    c->Sxy = 0.5 * (dudx[0][1] + dudx[1][0]);
+    c->Syz = 0.5 * (dudx[1][2] + dudx[2][1]);
+    c->Szx = 0.5 * (dudx[2][0] + dudx[0][2]);
+    double ev;
+    if (eigval_symm(c, &ev)) return 3;
+    double *e = c->e;
+    if (eigvec_symm(c, ev, e)) return 3;
+    double length = sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+    e[0] /= length;
+    e[1] /= length;
+    e[2] /= length;
+    double a = e[0] * e[0] * c->Sxx + e[0] * e[1] * c->Sxy + e[0] * e[2] * c->Szx + e[1] * e[0] * c->Sxy +
+               e[1] * e[1] * c->Syy + e[1] * e[2] * c->Syz + e[2] * e[0] * c->Szx + e[2] * e[1] * c->Syz +
+               e[2] * e[2] * c->Szz;
+    double lv = sqrt(2.0 * s->d.nu / (3.0 * (fabs(a) + SGS_EPS)));
+    /* structure function over the 26 neighbours */
+    double F2 = 0.0, Qd = 0.0;
+    int cnt = 0;
+    int p = s->p;
+    const double *U = s->V[0], *Vv = s->V[1], *W = s->V[2];
+    long c0 = FX(s, i, j, k);
+    for (int di = -1; di <= 1; di++)
+        for (int dj = -1; dj <= 1; dj++)
+            for (int dk = -1; dk <= 1; dk++) {
+                if (!(di || dj || dk)) continue;
+                long q = FX(s, i + di, j + dj, k + dk);
+                double du = U[q] - U[c0], dv = Vv[q] - Vv[c0], dw = W[q] - W[c0];
+                F2 += du * du + dv * dv + dw * dw;
+                double dx = s->x[0][i + di + p] - s->x[0][i + p];
+                double dy = s->x[1][j + dj + p] - s->x[1][j + p];
+                double dz = s->x[2][k + dk + p] - s->x[2][k + p];
+                double dx2 = dx * dx + dy * dy + dz * dz;
+                double dxe = dx * e[0] + dy * e[1] + dz * e[2];
+                double d = sqrt(dx2 - dxe * dxe) / del;
+                Qd += sf_integral(d);
+                cnt++;
+            }
+    F2 /= (double)cnt;
+    Qd /= (double)cnt;
+    double prefac = F2 / Qd;
+    double kc = SGS_PI / del;
+    c->K = prefac * ke_integral(kc * lv);
+    T[0] = (1.0 - e[0] * e[0]) * c->K;
+    T[1] = (1.0 - e[1] * e[1]) * c->K;
+    T[2] = (1.0 - e[2] * e[2]) * c->K;
+    T[3] = (-e[0] * e[1]) * c->K;
+    T[4] = (-e[1] * e[2]) * c->K;
+    T[5] = (-e[2] * e[0]) * c->K;
+    return 0;
+}
+
+/* spiral::computeSG(nseRHS, V) (spiral.cc:129-236); the reference leaves
+ * two of its diagnostic sums uninitialised, here every sum starts at 0 */
+static int compute_sg(or_nse *s, double *H[3])
+{
+    int p = s->p;
+    double dxi[3];
+    for (int a = 0; a < 3; a++) dxi[a] = 1.0 / (double)s->n[a];
+    double sgke = 0.0, disp = 0.0, nusg = 0.0;
+    for (int i = 0; i < s->n[0]; i++) {
+        double dx = s->x[0][i + p] - s->x[0][i - 1 + p];
+        for (int j = 0; j < s->n[1]; j++) {
+            double dy = s->x[1][j + p] - s->x[1][j - 1 + p];
+            for (int k = 0; k < s->n[2]; k++) {
+                double dz = s->x[2][k + p] - s->x[2][k - 1 + p];
+                double del = pow(dx * dy * dz, 1.0 / 3.0);
+                double dudx[3][3];
+                for (int r = 0; r < 3; r++)
+                    for (int a = 0; a < 3; a++) dudx[r][a] = d1(s, s->V[r], a, i, j, k);
+                sgs_cell c;
+                double T[6];
+                if (sgs_stress(s, i, j, k, dudx, del, &c, T)) return 3;
+                long q = FX(s, i, j, k);
+                for (int m = 0; m < 6; m++) s->Tsg[m][q] = T[m];
+                double A = dxi[0] / s->xi_x[0][i + p], B = dxi[1] / s->xi_x[1][j + p], C = dxi[2] / s->xi_x[2][k + p];
+                sgke += fabs(c.K) * A * B * C;
+                double sgDiss = -(c.Sxx * T[0] + c.Syy * T[1] + c.Szz * T[2] +
+                                  2.0 * (c.Sxy * T[3] + c.Syz * T[4] + c.Szx * T[5]));
+                disp += sgDiss * A * B * C;
+                double sNorm = sqrt(c.Sxx * c.Sxx + c.Syy * c.Syy + c.Szz * c.Szz +
+                                    2.0 * (c.Sxy * c.Sxy + c.Syz * c.Syz + c.Szx * c.Szx));
+                double tNorm = sqrt(T[0] * T[0] + T[1] * T[1] + T[2] * T[2] +
+                                    2.0 * (T[3] * T[3] + T[4] * T[4] + T[5] * T[5]));
+                double nuTurb = (tNorm / sNorm) / 2;
+                nusg += nuTurb * A * B * C;
+            }
+        }
+    }
+    s->sgs_diag[0] = sgke;
+    s->sgs_diag[1] = disp;
+    s->sgs_diag[2] = nusg;
+    for (int m = 0; m < 6; m++) sync_faces(s, s->Tsg[m]);
+    double **T = s->Tsg;   /* Txx, Tyy, Tzz, Txy, Tyz, Tzx */
+    CORE_LOOP(s) {
+        long q = FX(s, i, j, k);
+        double A11 = d1(s, T[0], 0, i, j, k), A12 = d1(s, T[3], 0, i, j, k), A13 = d1(s, T[5], 0, i, j, k);
+        double A21 = d1(s, T[3], 1, i, j, k), A22 = d1(s, T[1], 1, i, j, k), A23 = d1(s, T[4], 1, i, j, k);
+        double A31 = d1(s, T[5], 2, i, j, k), A32 = d1(s, T[4], 2, i, j, k), A33 = d1(s, T[2], 2, i, j, k);
+        H[0][q] = H[0][q] - (A11 + A21 + A31);
+        H[1][q] = H[1][q] - (A12 + A22 + A32);
+        H[2][q] = H[2][q] - (A13 + A23 + A33);
+    }
+    return 0;
+}
+
+int or_nse_sgs(or_nse *s, double *hx, double *hy, double *hz)
+{
+    double *H[3] = {hx, hy, hz};
+    return compute_sg(s, H);
+}
+
+void or_nse_sgs_diag(const or_nse *s, double out[3])
+{
+    for (int q = 0; q < 3; q++) out[q] = s->sgs_diag[q];
+}
+
+double or_nse_time(const or_nse *s) { return s->time; }
+
+void or_nse_get_sgs(const or_nse *s, double *T6)
+{
+    for (int m = 0; m < 6; m++) memcpy(T6 + (size_t)m * s->full, s->Tsg[m], s->full * sizeof(double));
+}
+
+static int les_active(const or_nse *s) { return s->d.les && s->time > 5 * s->tstp; }
+
 /* eulerCN_d3::timeAdvance explicit part (eulerCN_d3.cc:92-121, 181-235):
  * RHS = (nu/2 lap u - H + f - grad P) dt + u, accumulated in the reference's
  * order into one array */
@@ -485,6 +709,7 @@ static int euler_rhs(or_nse *s)
     if (hasT)
         for (int c = 0; c < 3; c++)
             for (size_t q = 0; q < n; q++) s->nse[c][q] -= s->d.buoy[c] * s->T[q];
+    if (les_active(s) && compute_sg(s, s->nse)) return 3;
     for (int c = 0; c < 3; c++) fill(s->pg[c], n, 0.0);
     CORE_LOOP(s) {
         long q = FX(s, i, j, k);
@@ -547,6 +772,7 @@ int or_nse_rhs(or_nse *s, int stage)
     for (int c = 0; c < 3; c++) fill(s->tv[c], n, 0.0);
     nlin_v(s, s->tv);
     if (hasT) { fill(s->ts, n, 0.0); nlin_s(s, s->T, s->ts); }
+    if (les_active(s) && compute_sg(s, s->tv)) return 3;
     for (int c = 0; c < 3; c++) mult_add(s->nse[c], s->tv[c], gamm[stage], n);
     if (hasT) mult_add(s->trhs, s->ts, gamm[stage], n);
     for (int c = 0; c < 3; c++) {
@@ -567,7 +793,8 @@ int or_nse_stage(or_nse *s, int stage)
     size_t n = s->full;
     int euler = s->d.scheme == 1;
     double a = euler ? 0.0 : alph[stage], b = euler ? 0.5 : alph[stage];   /* betaCN2 = 1/2 */
-    or_nse_rhs(s, stage);
+    int rr = or_nse_rhs(s, stage);
+    if (rr) return rr;
     for (int c = 0; c < 3; c++) {
         s->jacobi_iters[c] = jacobi(s, s->V[c], s->nse[c], b, s->d.nu, s->d.vbc_type[c], s->d.vbc_val[c]);
         if (s->jacobi_iters[c] < 0) return 2;

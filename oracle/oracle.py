@@ -49,7 +49,7 @@ class NseDesc(C.Structure):
                 ("vbc_type", (C.c_int * 6) * 3), ("vbc_val", (C.c_double * 6) * 3),
                 ("tbc_type", C.c_int * 6), ("tbc_val", C.c_double * 6),
                 ("pbc_type", C.c_int * 6), ("pbc_val", C.c_double * 6), ("scheme", C.c_int),
-                ("pe_limit", C.c_double), ("central_bias", C.c_double)]
+                ("pe_limit", C.c_double), ("central_bias", C.c_double), ("les", C.c_int)]
 
 
 class MgStats(C.Structure):
@@ -105,6 +105,11 @@ def lib():
         L.or_nse_op.argtypes = [C.c_void_p, C.c_int, dp, dp]
         L.or_nse_diag.argtypes = [C.c_void_p, dp, dp, dp, dp]
         L.or_nse_cfl.argtypes = [C.c_void_p, C.c_double, C.c_double]
+        L.or_nse_sgs.argtypes = [C.c_void_p, dp, dp, dp]
+        L.or_nse_sgs_diag.argtypes = [C.c_void_p, dp]
+        L.or_nse_get_sgs.argtypes = [C.c_void_p, dp]
+        L.or_nse_time.argtypes = [C.c_void_p]
+        L.or_nse_time.restype = C.c_double
         L.or_nse_cfl.restype = C.c_double
         _lib = L
     return _lib
@@ -276,7 +281,7 @@ def set_threads(n):
 
 def nse_desc(g, dscheme=2, nlscheme=3, has_T=1, nu=1e-2, kappa=1e-2, dt=1e-3, cn_tol=1e-6,
              buoy=(0.0, 0.0, -1.0), vbc=None, tbc=None, pbc=None, mg=None, scheme=0, pe_limit=2.0,
-             central_bias=0.8):
+             central_bias=0.8, les=0):
     """Descriptor of one run.  vbc: [(types[6], vals[6]) per component];
     tbc / pbc: (types[6], vals[6]).  Defaults: the Rayleigh-Benard wiring of
     hydro.cc:287-485 / scalar.cc:129-201 (no-slip, T = 1 bottom, 0 top,
@@ -288,6 +293,7 @@ def nse_desc(g, dscheme=2, nlscheme=3, has_T=1, nu=1e-2, kappa=1e-2, dt=1e-3, cn
     d.nu, d.kappa, d.dt, d.cn_tol = nu, kappa, dt, cn_tol
     d.scheme = scheme
     d.pe_limit, d.central_bias = pe_limit, central_bias
+    d.les = int(les)
     for a in range(3):
         d.buoy[a] = buoy[a]
     if vbc is None:
@@ -368,6 +374,23 @@ class Nse:
 
     def cfl(self, courant, dt_max):
         return lib().or_nse_cfl(self.h, courant, dt_max)
+
+    def sgs(self, H=None):
+        """spiral::computeSG on H (zeros by default): returns (status, H, diag[3])."""
+        H = [np.zeros(self.shape) for _ in range(3)] if H is None else [np.array(h, dtype=np.float64) for h in H]
+        r = lib().or_nse_sgs(self.h, *[_dp(h) for h in H])
+        dg = np.zeros(3)
+        lib().or_nse_sgs_diag(self.h, _dp(dg))
+        return r, H, dg
+
+    def stresses(self):
+        T = np.zeros((6,) + self.shape)
+        lib().or_nse_get_sgs(self.h, _dp(T))
+        return T
+
+    @property
+    def time(self):
+        return lib().or_nse_time(self.h)
 
     def diag(self):
         v = [C.c_double() for _ in range(4)]

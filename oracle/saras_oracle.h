@@ -142,6 +142,7 @@ typedef struct {
     int scheme;            /* "Integration Scheme": 0 lsRK3_d3, 1 eulerCN_d3 */
     double pe_limit;       /* "Peclet Limit" (upwind scheme) */
     double central_bias;   /* "Central Bias" omega (upwind scheme) */
+    int les;               /* "LES Model" 1: stretched spiral vortex sub-grid stress (fully periodic runs) */
 } or_nse_desc;
 
 typedef struct or_nse or_nse;
@@ -180,6 +181,17 @@ void or_nse_diag(or_nse *s, double *ke, double *divmax, double *nuvol, double *n
  * (hydro_d3.cc:221-226): dt = min(courant / max(|u|/dx + |v|/dy + |w|/dz), dt_max);
  * the run's dt becomes that value. */
 double or_nse_cfl(or_nse *s, double courant, double dt_max);
+/* spiral::computeSG(nseRHS, V) (spiral.cc:129-236) on a caller array set
+ * H[3] (full arrays): the sub-grid stresses of the current V and
+ * H -= div T; returns 0, or 3 where the reference aborts (spiral.cc:643-650,
+ * 712-719, 732-737).  Diagnostics of the last evaluation: sums of |K| dV,
+ * dissipation dV and nu_t dV (tseries divides them by the volume). */
+int or_nse_sgs(or_nse *s, double *hx, double *hy, double *hz);
+void or_nse_sgs_diag(const or_nse *s, double out[3]);
+/* the six stress fields Txx, Tyy, Tzz, Txy, Tyz, Tzx of the last evaluation
+ * (full arrays, one after the other) */
+void or_nse_get_sgs(const or_nse *s, double *T6);
+double or_nse_time(const or_nse *s);
 
 /* Threads the oracle's OpenMP loops use (0 = runtime default). */
 void or_set_threads(int n);

@@ -474,3 +474,104 @@ def test_cfl_time_step():
     mx = (np.abs(V["vx"][c]) * n + np.abs(V["vy"][c]) * n + np.abs(V["vz"][c]) * n).max()
     assert abs(s.cfl(0.1, 1.0) - 0.1 / mx) < 1e-15
     assert s.cfl(0.1, 1e-5) == 1e-5
+
+
+# ---- stretched spiral vortex LES (lib/les/spiral.cc) ---------------------------
+
+def _tg_les(n=16, seed=5, noise=0.01):
+    p = 2
+    g, d, s = make(n, periodic=(1, 1, 1), has_T=0, nu=1.0 / 1600, dt=1e-3, les=1)
+    x = xi_axis(n, p)
+    sh = s.shape
+    X, Y, Z = (along(x, a, sh) for a in range(3))
+    tp = 2 * np.pi
+    rng = np.random.default_rng(seed)
+    u = np.sin(tp * X) * np.cos(tp * Y) * np.cos(tp * Z) + noise * rng.uniform(-1, 1, sh)
+    v = -np.cos(tp * X) * np.sin(tp * Y) * np.cos(tp * Z) + noise * rng.uniform(-1, 1, sh)
+    w = noise * rng.uniform(-1, 1, sh)
+    s.set(vx=u, vy=v, vz=w, p=np.zeros(sh), t=np.zeros(sh), pp=np.zeros(sh))
+    return g, s, p
+
+
+def test_les_energy_and_structure_integrals_match_closed_forms():
+    """keIntegral ~ (1/2) k^(2/3) Gamma(-1/3, k^2) (stated max error 0.17 %)
+    and sfIntegral ~ int_0^1 4 x^(-5/3) (1 - J0(pi d x)) dx (stated 2.71 %),
+    checked through single-cell K values: K / (F2/Qd) is keIntegral(kc lv)."""
+    import scipy.special as sp
+    from scipy.integrate import quad
+
+    def ke_exact(k):
+        x = k * k
+        g23 = sp.gammaincc(2.0 / 3.0, x) * sp.gamma(2.0 / 3.0)
+        gm13 = (g23 - x ** (-1.0 / 3.0) * np.exp(-x)) / (-1.0 / 3.0)
+        return 0.5 * k ** (2.0 / 3.0) * gm13
+
+    def sf_exact(d):
+        return quad(lambda x: 4 * x ** (-5.0 / 3.0) * (1 - sp.j0(np.pi * d * x)), 0, 1, limit=200)[0]
+
+    g, s, p = _tg_les()
+    r, H, dg = s.sgs()
+    assert r == 0
+    T = s.stresses()
+    n = 16
+    c = (slice(p, -p),) * 3
+    # at every cell T is K times the projector orthogonal to the vortex axis e:
+    # trace T = 2K, T e = 0 with e the most extensive strain direction
+    K = 0.5 * (T[0] + T[1] + T[2])
+    assert (K[c] > 0).all()
+    fields = s.get()
+    for (i, j, k) in [(3, 5, 7), (0, 0, 0), (15, 8, 2), (9, 14, 11)]:
+        dudx = np.array([[s.op(a, fields[comp])[i + p, j + p, k + p] for a in range(3)]
+                         for comp in ("vx", "vy", "vz")])
+        S = 0.5 * (dudx + dudx.T)
+        w, vec = np.linalg.eigh(S)
+        e = vec[:, 2]
+        Tm = np.array([[T[0], T[3], T[5]], [T[3], T[1], T[4]], [T[5], T[4], T[2]]])[:, :, i + p, j + p, k + p]
+        Kc = K[i + p, j + p, k + p]
+        assert np.abs(Tm @ e).max() < 1e-9 * Kc
+        assert np.allclose(Tm, Kc * (np.eye(3) - np.outer(e, e)), rtol=0, atol=1e-9 * Kc)
+        # K = (F2 / Qd) keIntegral(kc lv) with the 26-neighbour structure function
+        h = 1.0 / n
+        nb = [(a, b, cc) for a in (-1, 0, 1) for b in (-1, 0, 1) for cc in (-1, 0, 1) if (a, b, cc) != (0, 0, 0)]
+        F2 = np.mean([sum((fields[f][i + p + a, j + p + b, k + p + cc] - fields[f][i + p, j + p, k + p]) ** 2
+                          for f in ("vx", "vy", "vz")) for a, b, cc in nb])
+        Qd = np.mean([sf_exact(np.sqrt(max(h * h * (a * a + b * b + cc * cc) - (h * (a * e[0] + b * e[1] + cc * e[2])) ** 2, 0.0)) / h)
+                      for a, b, cc in nb])
+        axial = e @ S @ e
+        lv = np.sqrt(2.0 * (1.0 / 1600) / (3.0 * (abs(axial) + 2e-3)))
+        K_exact = F2 / Qd * ke_exact(np.pi / h * lv)
+        assert abs(Kc - K_exact) < 0.04 * K_exact
+
+
+def test_les_divergence_of_stress_and_diagnostics():
+    g, s, p = _tg_les(seed=6)
+    r, H, dg = s.sgs()
+    assert r == 0
+    T = s.stresses()
+    c = (slice(p, -p),) * 3
+    # H -= div T with the field's own first derivative operator
+    Txx, Tyy, Tzz, Txy, Tyz, Tzx = T
+    exp = [-(s.op(0, Txx) + s.op(1, Txy) + s.op(2, Tzx)),
+           -(s.op(0, Txy) + s.op(1, Tyy) + s.op(2, Tyz)),
+           -(s.op(0, Tzx) + s.op(1, Tyz) + s.op(2, Tzz))]
+    for q in range(3):
+        assert np.allclose(H[q][c], exp[q][c], rtol=1e-12, atol=1e-14 * np.abs(exp[q][c]).max())
+    # diagnostics: sum |K| dV (dV = h^3 on a uniform unit box) and the
+    # dissipation -S:T summed over cells
+    K = 0.5 * (Txx + Tyy + Tzz)
+    assert abs(dg[0] - np.abs(K[c]).sum() / 16 ** 3) < 1e-12 * dg[0]
+    assert dg[1] != 0.0 and dg[2] > 0.0
+
+
+def test_les_starts_after_five_steps():
+    """The model is added once solTime > 5 tStp (lsRK3_d3.cc:238); before
+    that a LES run and a plain run are identical."""
+    g, s1, p = _tg_les(n=8, seed=7)
+    _, d0, s0 = make(8, periodic=(1, 1, 1), has_T=0, nu=1.0 / 1600, dt=1e-3, les=0)
+    f = s1.get()
+    s0.set(**f)
+    for step in range(7):
+        assert s1.step() == 0 and s0.step() == 0
+        same = all(np.array_equal(s1.get()[k], s0.get()[k]) for k in ("vx", "vy", "vz"))
+        # time after step q is (q+1) dt; the model enters the step that starts with time > 5 dt
+        assert same == (step <= 5), step

@@ -205,6 +205,47 @@ def rbc_step(steps, warmup, n=256):
     return out
 
 
+def les_step(steps, warmup, n=384, noise=0.01, seed=5):
+    """Decaying Taylor-Green vortex in a periodic (2 pi)^3 box, Re = 1600
+    (nu = 1/1600, unit velocity and length), seeded uniform noise of 1 % of
+    the peak velocity, spiral sub-grid model, RK3/CN, FP64 stencils and the
+    mixed-precision pressure solve.  The model enters once solTime > 5 dt, so
+    at least six warm-up steps run before the timed ones."""
+    pad = 2
+    L = 2 * np.pi
+    g = S.grid_desc(n, periodic=(1, 1, 1), length=(L, L, L), pad=pad)
+    x = (np.arange(-pad, n + pad) + 0.5) * (L / n)
+    rng = np.random.default_rng(seed)
+    sh = (n + 2 * pad,) * 3
+    sx, cx = np.sin(x), np.cos(x)
+    u = sx[:, None, None] * (cx[:, None] * cx[None, :])[None, :, :]
+    u += noise * rng.uniform(-1, 1, sh)
+    v = -(cx[:, None, None] * (sx[:, None] * cx[None, :])[None, :, :])
+    v += noise * rng.uniform(-1, 1, sh)
+    w = noise * rng.uniform(-1, 1, sh)
+    zero = np.zeros(sh)
+    prm = S.nse_params(dscheme=2, nlscheme=3, has_T=0, nu=1.0 / 1600, kappa=1.0 / 1600, dt=1e-3, cn_tol=1e-6,
+                       les=1, vbc=[([S.FBC_DIRICHLET] * 6, [0.0] * 6)] * 3)
+    mg = S.mg_params(vc_count=20, vc_tol=1e-6, solve_flag=0, mg_tol=1e-2, pre=3, post=6, bc=S.BC_NEUMANN,
+                     precision=S.PREC_MIXED, inner_cycles=2, max_refine=10)
+    s = S.Nse(g, mg, prm)
+    s.set(vx=u, vy=v, vz=w, p=zero, t=zero, pp=zero)
+    del u, v, w
+    for _ in range(max(warmup, 6)):
+        r, st = s.step()
+        S.check(r)
+    r, ms = s.bench_steps(steps)
+    S.check(r)
+    d = s.diag()
+    sg, t = s.sgs()
+    vol = L ** 3
+    s.close()
+    return dict(ms_per_step=round(ms / steps, 3), steps=steps, cells=[n, n, n], jacobi_iters=st["jacobi_iters"],
+                pressure_cycles=st["mg"]["cycles"], pressure_residual=st["mg"]["residual"], time=t, ke=d["ke"],
+                divmax=d["divmax"],
+                sgs_ke=sg["sgke"] / vol, sgs_dissipation=sg["disp"] / vol, nu_sgs=sg["nusg"] / vol)
+
+
 def serving(count=240, seed=4, slots=8, budget=96 << 30):
     from saras_amd import serving as V
 
@@ -284,6 +325,8 @@ def main():
                     help="only the 256^3 Rayleigh-Benard time steps (warmup + steps, FP64 and mixed)")
     ap.add_argument("--no-rbc", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-les", action="store_true")
+    ap.add_argument("--les-only", action="store_true", help="only the 384^3 Taylor-Green LES step")
     ap.add_argument("--serve-only", action="store_true", help="only the batched serving measurement")
     ap.add_argument("--slots", type=int, default=8)
     ap.add_argument("--requests", type=int, default=240)
@@ -299,6 +342,9 @@ def main():
         return
     if args.rbc_only:
         print(json.dumps({"rbc_256_step": rbc_step(args.steps, args.warmup)}))
+        return
+    if args.les_only:
+        print(json.dumps({"les_384_step": les_step(args.steps, args.warmup)}))
         return
     if args.serve_only:
         print(json.dumps({"serving": serving(count=args.requests, slots=args.slots)}))
@@ -349,6 +395,8 @@ def main():
         out["serving"] = serving(count=args.requests, slots=args.slots)
     if not args.no_rbc:
         out["rbc_256_step"] = rbc_step(3, 1)
+    if not args.no_les:
+        out["les_384_step"] = les_step(3, 6)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(nu1, nu2, f64["shapes"])
     print(json.dumps(out))

@@ -34,7 +34,9 @@ typedef enum {
     SARAS_BAD_SHAPE = 3,       /* grid cannot be coarsened / bad extents */
     SARAS_OOM = 4,             /* device allocation failed */
     SARAS_HIP_ERROR = 5,       /* a HIP runtime call failed */
-    SARAS_BAD_ARG = 6          /* invalid parameter or null pointer */
+    SARAS_BAD_ARG = 6,         /* invalid parameter or null pointer */
+    SARAS_SGS_FAILED = 7       /* LES eigen-decomposition failed where the reference
+                                  aborts (spiral.cc:643-650, 712-719, 732-737) */
 } saras_status;
 
 /* Grid types per axis ("Mesh Type" U/S/D, lib/io/parser.cc:417-448). */
@@ -195,6 +197,8 @@ typedef struct {
     int scheme;              /* "Integration Scheme": 0 lsRK3_d3 (RK3-CN), 1 eulerCN_d3 */
     double pe_limit;         /* "Peclet Limit" of the upwind scheme (2.0 in the reference deck) */
     double central_bias;     /* "Central Bias" of the upwind scheme (0.8 in the reference deck) */
+    int les;                 /* "LES Model" 1: stretched spiral vortex sub-grid stress, added once
+                                the run time exceeds 5 time steps (lsRK3_d3.cc:238); periodic boxes */
 } saras_nse_params;
 
 typedef struct {
@@ -252,6 +256,12 @@ saras_status saras_nse_solve_field(saras_nse *s, int field, double beta, int *it
  * sfield::computeNLin with the handle's velocity (out += H;
  * sfield.cc:160-311). */
 saras_status saras_field_op(saras_nse *s, int op, const double *f, double *out, int ptr_kind);
+
+/* Sub-grid diagnostics of the last LES evaluation (spiral.cc:186-206):
+ * out[0] sum |K| dV, out[1] sum of the SGS dissipation dV, out[2] sum nu_t dV
+ * (tseries divides each by the volume); *time = the run time of the handle
+ * (advanced by dt after each saras_nse_step). */
+saras_status saras_nse_sgs_diag(saras_nse *s, double out[3], double *time);
 
 /* vfield::computeTStp (vfield.cc:691-706) with the driver's cap
  * (hydro_d3.cc:221-226): dt = min(courant / max|u|/dx + |v|/dy + |w|/dz, dt_max)

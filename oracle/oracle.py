@@ -383,6 +383,12 @@ class Nse:
         lib().or_nse_sgs_diag(self.h, _dp(dg))
         return r, H, dg
 
+    def sgs_diag(self):
+        """Diagnostics of the last sub-grid evaluation (no new evaluation)."""
+        dg = np.zeros(3)
+        lib().or_nse_sgs_diag(self.h, _dp(dg))
+        return dg
+
     def stresses(self):
         T = np.zeros((6,) + self.shape)
         lib().or_nse_get_sgs(self.h, _dp(T))

@@ -44,7 +44,7 @@ struct Geom {
     double i2d[3];                 // vfield.cc:133-135
     double tsih2[3], tsi2h[3];     // timestep.cc:156-173
     const double *xi_x[3], *xixx[3], *xix2[3];   // core cells
-    const double *xc[3];           // cell-centre coordinates over [0, n] (the upwind Peclet spacing)
+    const double *xc[3];           // cell-centre coordinates over [-1, n] (xc[a][-1] valid)
     double pe_limit;               // "Peclet Limit"
     double up1[3], up2[4];         // biased-stencil weights from "Central Bias" (vfield.cc:115-130)
 };
@@ -79,6 +79,23 @@ struct CoreField {
         c[ax] = hi ? n - 1 : 0;
         const double inner = f[(long)c[0] * G.sx + (long)c[1] * G.sy + c[2]];
         return type == 0 ? 2.0 * val - inner : inner - kh;
+    }
+};
+
+// Core-only field of a fully periodic box (the LES stresses): every
+// neighbour wraps, no wall rule.
+struct PerField {
+    const double *f;
+    __device__ __forceinline__ double at(const Geom &G, int i, int j, int k) const {
+        return f[(long)i * G.sx + (long)j * G.sy + k];
+    }
+    __device__ __forceinline__ double nb(const Geom &G, int ax, int i, int j, int k, int off) const {
+        const int n = ax == 0 ? G.n[0] : (ax == 1 ? G.n[1] : G.n[2]);
+        int q = (ax == 0 ? i : (ax == 1 ? j : k)) + off;
+        q = q < 0 ? q + n : (q >= n ? q - n : q);
+        const long o = ax == 0 ? (long)q * G.sx + (long)j * G.sy + k
+                               : (ax == 1 ? (long)i * G.sx + (long)q * G.sy + k : (long)i * G.sx + (long)j * G.sy + q);
+        return f[o];
     }
 };
 
@@ -258,7 +275,8 @@ struct StageCoef {
 template <int C>
 __global__ void __launch_bounds__(256) k_rhs_v(Geom G, StageCoef c, CoreField V0, CoreField V1, CoreField V2,
                                                CoreField T, CoreField P, double *__restrict__ H,
-                                               double *__restrict__ R) {
+                                               double *__restrict__ R, int les, PerField Sa, PerField Sb,
+                                               PerField Sc) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
     const int i = blockIdx.z;
@@ -274,7 +292,9 @@ __global__ void __launch_bounds__(256) k_rhs_v(Geom G, StageCoef c, CoreField V0
     if (c.has_T) f -= c.buoy[C] * T.at(G, i, j, k);
     f -= d1(G, P, C, i, j, k);
     nse += c.ab * f;
-    const double h = nlin(G, c.nlscheme, V, U, i, j, k, c.nu);
+    double h = nlin(G, c.nlscheme, V, U, i, j, k, c.nu);
+    // spiral::computeSG: H = H - (dSa/dx + dSb/dy + dSc/dz) (spiral.cc:225-235)
+    if (les) h = h - (d1(G, Sa, 0, i, j, k) + d1(G, Sb, 1, i, j, k) + d1(G, Sc, 2, i, j, k));
     nse += c.gamma * h;
     nse *= c.dt;
     nse += U.at(G, i, j, k);
@@ -310,7 +330,8 @@ __global__ void __launch_bounds__(256) k_rhs_t(Geom G, StageCoef c, CoreField V0
 template <int C>
 __global__ void __launch_bounds__(256) k_euler_v(Geom G, double k_nu, double nu, double dt, double buoy, int has_T,
                                                  int nlscheme, CoreField V0, CoreField V1, CoreField V2,
-                                                 CoreField T, CoreField P, double *__restrict__ R) {
+                                                 CoreField T, CoreField P, double *__restrict__ R, int les,
+                                                 PerField Sa, PerField Sb, PerField Sc) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
     const int i = blockIdx.z;
@@ -321,6 +342,7 @@ __global__ void __launch_bounds__(256) k_euler_v(Geom G, double k_nu, double nu,
     r *= k_nu;
     r = nlin_acc(G, nlscheme, V, U, i, j, k, r, nu);
     if (has_T) r -= buoy * T.at(G, i, j, k);
+    if (les) r = r - (d1(G, Sa, 0, i, j, k) + d1(G, Sb, 1, i, j, k) + d1(G, Sc, 2, i, j, k));
     r -= d1(G, P, C, i, j, k);
     r *= dt;
     r += U.at(G, i, j, k);
@@ -341,6 +363,206 @@ __global__ void __launch_bounds__(256) k_euler_t(Geom G, double k_kappa, double 
     r *= dt;
     r += T.at(G, i, j, k);
     R[(long)i * G.sx + (long)j * G.sy + k] = r;
+}
+
+// ---- stretched spiral vortex sub-grid stress (lib/les/spiral.cc) ----------
+constexpr double kSgsEps = 2e-3;                    // spiral.cc:49
+constexpr double kPi = 3.14159265358979323846;      // M_PI
+
+__device__ __forceinline__ double ke_integral(double k) {   // spiral.cc:590-607
+    const double k2 = k * k;
+    if (k2 < 2.42806) {
+        const double pade = (3.0 + 2.5107 * k2 + 0.330357 * k2 * k2 + 0.0295481 * k2 * k2 * k2) /
+                            (1.0 + 0.336901 * k2 + 0.0416684 * k2 * k2 + 0.00187191 * k2 * k2 * k2);
+        return 0.5 * (pade - 4.06235 * pow(k2, 1.0 / 3.0));
+    }
+    const double pade = (1.26429 + 0.835714 * k2 + 0.0964286 * k2 * k2) /
+                        (1.0 + 2.25 * k2 + 0.964286 * k2 * k2 + 0.0964286 * k2 * k2 * k2);
+    return 0.5 * pade * exp(-k2);
+}
+
+__device__ __forceinline__ double sf_integral(double d) {   // spiral.cc:620-631
+    const double d2 = d * d;
+    if (d < 0.873469) return 7.4022 * d2 - 1.82642 * d2 * d2;
+    return 12.2946 * pow(d, 2.0 / 3.0) - 6.0 - 0.573159 * pow(d, -1.5) * sin(3.14159 * d - 0.785398);
+}
+
+// One core cell of spiral::computeSG (spiral.cc:157-206 with sgsStress
+// :472-548, eigenvalueSymm :643-~690, eigenvectorSymm :702-~742) on a fully
+// periodic box: writes the six stresses, returns false where the reference
+// aborts.
+__device__ bool sgs_cell(const Geom &G, const CoreField *V, double nu, int i, int j, int k, double T[6],
+                         double &K, double S[6]) {
+    double dudx[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        dudx[r][0] = d1(G, V[r], 0, i, j, k);
+        dudx[r][1] = d1(G, V[r], 1, i, j, k);
+        dudx[r][2] = d1(G, V[r], 2, i, j, k);
+    }
+    const double Sxx = 0.5 * (dudx[0][0] + dudx[0][0]), Syy = 0.5 * (dudx[1][1] + dudx[1][1]);
+    const double Szz = 0.5 * (dudx[2][2] + dudx[2][2]), Sxy = 0.5 * (dudx[0][1] + dudx[1][0]);
+    const double Syz = 0.5 * (dudx[1][2] + dudx[2][1]), Szx = 0.5 * (dudx[2][0] + dudx[0][2]);
+    S[0] = Sxx; S[1] = Syy; S[2] = Szz; S[3] = Sxy; S[4] = Syz; S[5] = Szx;
+    // eigenvalueSymm
+    const double a = -(Sxx + Syy + Szz);
+    const double b = Sxx * Syy - Sxy * Sxy + Syy * Szz - Syz * Syz + Szz * Sxx - Szx * Szx;
+    const double cc = -(Sxx * (Syy * Szz - Syz * Syz) + Sxy * (Syz * Szx - Sxy * Szz) + Szx * (Sxy * Syz - Syy * Szx));
+    const double q = (3.0 * b - a * a) / 9.0;
+    const double r = (9.0 * a * b - 27.0 * cc - 2.0 * a * a * a) / 54.0;
+    if (q >= 0.0) return false;
+    const double costheta = r / sqrt(-q * q * q);
+    const double theta = costheta > 1.0 ? 0.0 : costheta < -1.0 ? kPi : acos(costheta);
+    double ev[3];
+    ev[0] = 2.0 * sqrt(-q) * cos((theta) / 3.0) - a / 3.0;
+    ev[1] = 2.0 * sqrt(-q) * cos((theta + 2.0 * kPi) / 3.0) - a / 3.0;
+    ev[2] = 2.0 * sqrt(-q) * cos((theta + 4.0 * kPi) / 3.0) - a / 3.0;
+    double tt;
+    if (ev[0] > ev[1]) { tt = ev[0]; ev[0] = ev[1]; ev[1] = tt; }
+    if (ev[1] > ev[2]) { tt = ev[1]; ev[1] = ev[2]; ev[2] = tt; }
+    if (ev[0] > ev[1]) { tt = ev[0]; ev[0] = ev[1]; ev[1] = tt; }
+    const double lam = ev[2];
+    // eigenvectorSymm
+    const double fNorm = sqrt(Sxx * Sxx + Syy * Syy + Szz * Szz + Sxy * Sxy + Syz * Syz + Szx * Szx);
+    if (fabs((Sxx - lam) * ((Syy - lam) * (Szz - lam) - Syz * Syz) + Sxy * (Syz * Szx - Sxy * (Szz - lam)) +
+             Szx * (Sxy * Syz - (Syy - lam) * Szx)) / fabs(fNorm) > kSgsEps)
+        return false;
+    const double det0 = (Syy - lam) * (Szz - lam) - Syz * Syz, det1 = (Szz - lam) * (Sxx - lam) - Szx * Szx;
+    const double det2 = (Sxx - lam) * (Syy - lam) - Sxy * Sxy;
+    const double f0 = fabs(det0), f1 = fabs(det1), f2 = fabs(det2);
+    double e[3];
+    if (f0 >= f1 && f0 >= f2) {
+        e[0] = 1.0;
+        e[1] = (-Sxy * (Szz - lam) + Szx * Syz) / det0;
+        e[2] = (-Szx * (Syy - lam) + Sxy * Syz) / det0;
+    } else if (f1 >= f2 && f1 >= f0) {
+        e[0] = (-Sxy * (Szz - lam) + Syz * Szx) / det1;
+        e[1] = 1.0;
+        e[2] = (-Syz * (Sxx - lam) + Sxy * Szx) / det1;
+    } else if (f2 >= f0 && f2 >= f1) {
+        e[0] = (-Szx * (Syy - lam) + Syz * Sxy) / det2;
+        e[1] = (-Syz * (Sxx - lam) + Szx * Sxy) / det2;
+        e[2] = 1.0;
+    } else {
+        return false;
+    }
+    const double length = sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+    e[0] /= length;
+    e[1] /= length;
+    e[2] /= length;
+    const double ax = e[0] * e[0] * Sxx + e[0] * e[1] * Sxy + e[0] * e[2] * Szx + e[1] * e[0] * Sxy +
+                      e[1] * e[1] * Syy + e[1] * e[2] * Syz + e[2] * e[0] * Szx + e[2] * e[1] * Syz +
+                      e[2] * e[2] * Szz;
+    const double lv = sqrt(2.0 * nu / (3.0 * (fabs(ax) + kSgsEps)));
+    // structure function over the 26 neighbours (periodic wrap)
+    const double dxv = G.xc[0][i] - G.xc[0][i - 1], dyv = G.xc[1][j] - G.xc[1][j - 1];
+    const double dzv = G.xc[2][k] - G.xc[2][k - 1];
+    const double del = pow(dxv * dyv * dzv, 1.0 / 3.0);
+    const double u0 = V[0].at(G, i, j, k), v0 = V[1].at(G, i, j, k), w0 = V[2].at(G, i, j, k);
+    double F2 = 0.0, Qd = 0.0;
+    int cnt = 0;
+    for (int di = -1; di <= 1; ++di) {
+        const int ii = i + di < 0 ? i + di + G.n[0] : (i + di >= G.n[0] ? i + di - G.n[0] : i + di);
+        const double dx = G.xc[0][i + di] - G.xc[0][i];
+        for (int dj = -1; dj <= 1; ++dj) {
+            const int jj = j + dj < 0 ? j + dj + G.n[1] : (j + dj >= G.n[1] ? j + dj - G.n[1] : j + dj);
+            const double dy = G.xc[1][j + dj] - G.xc[1][j];
+            for (int dk = -1; dk <= 1; ++dk) {
+                if (!(di || dj || dk)) continue;
+                const int kk = k + dk < 0 ? k + dk + G.n[2] : (k + dk >= G.n[2] ? k + dk - G.n[2] : k + dk);
+                const double dz = G.xc[2][k + dk] - G.xc[2][k];
+                const double du = V[0].at(G, ii, jj, kk) - u0, dv = V[1].at(G, ii, jj, kk) - v0;
+                const double dw = V[2].at(G, ii, jj, kk) - w0;
+                F2 += du * du + dv * dv + dw * dw;
+                const double dx2 = dx * dx + dy * dy + dz * dz;
+                const double dxe = dx * e[0] + dy * e[1] + dz * e[2];
+                const double d = sqrt(dx2 - dxe * dxe) / del;
+                Qd += sf_integral(d);
+                ++cnt;
+            }
+        }
+    }
+    F2 /= (double)cnt;
+    Qd /= (double)cnt;
+    const double prefac = F2 / Qd;
+    const double kc = kPi / del;
+    K = prefac * ke_integral(kc * lv);
+    T[0] = (1.0 - e[0] * e[0]) * K;
+    T[1] = (1.0 - e[1] * e[1]) * K;
+    T[2] = (1.0 - e[2] * e[2]) * K;
+    T[3] = (-e[0] * e[1]) * K;
+    T[4] = (-e[1] * e[2]) * K;
+    T[5] = (-e[2] * e[0]) * K;
+    return true;
+}
+
+// sub-grid stresses of every core cell, per-block partial diagnostics
+// [sum |K| dV, sum dissipation dV, sum nu_t dV], failure flag
+__global__ void __launch_bounds__(256) k_sgs_stress(Geom G, CoreField V0, CoreField V1, CoreField V2, double nu,
+                                                    double *__restrict__ Txx, double *__restrict__ Tyy,
+                                                    double *__restrict__ Tzz, double *__restrict__ Txy,
+                                                    double *__restrict__ Tyz, double *__restrict__ Tzx,
+                                                    double *__restrict__ part, int *__restrict__ fail) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    const int i = blockIdx.z;
+    double v[3] = {0.0, 0.0, 0.0};
+    if (k < G.n[2]) {
+        const CoreField V[3] = {V0, V1, V2};
+        double T[6], K, S[6];
+        if (!sgs_cell(G, V, nu, i, j, k, T, K, S)) {
+            atomicOr(fail, 1);
+            T[0] = T[1] = T[2] = T[3] = T[4] = T[5] = 0.0;
+            K = 0.0;
+        }
+        const long idx = (long)i * G.sx + (long)j * G.sy + k;
+        Txx[idx] = T[0];
+        Tyy[idx] = T[1];
+        Tzz[idx] = T[2];
+        Txy[idx] = T[3];
+        Tyz[idx] = T[4];
+        Tzx[idx] = T[5];
+        const double A = (1.0 / G.n[0]) / G.xi_x[0][i], B = (1.0 / G.n[1]) / G.xi_x[1][j];
+        const double C = (1.0 / G.n[2]) / G.xi_x[2][k];
+        v[0] = fabs(K) * A * B * C;
+        const double sgDiss = -(S[0] * T[0] + S[1] * T[1] + S[2] * T[2] + 2.0 * (S[3] * T[3] + S[4] * T[4] + S[5] * T[5]));
+        v[1] = sgDiss * A * B * C;
+        const double sNorm = sqrt(S[0] * S[0] + S[1] * S[1] + S[2] * S[2] + 2.0 * (S[3] * S[3] + S[4] * S[4] + S[5] * S[5]));
+        const double tNorm = sqrt(T[0] * T[0] + T[1] * T[1] + T[2] * T[2] + 2.0 * (T[3] * T[3] + T[4] * T[4] + T[5] * T[5]));
+        v[2] = ((tNorm / sNorm) / 2) * A * B * C;
+    }
+    __shared__ double red[3][4];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        double x = v[q];
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+        if ((threadIdx.x & 63) == 0) red[q][threadIdx.x >> 6] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        double x = red[threadIdx.x][0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) x += red[threadIdx.x][w];
+        part[(((long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x) * 3 + threadIdx.x] = x;
+    }
+}
+
+// sums of n triples -> out[3] (one block of 1024 threads, fixed order)
+__global__ void __launch_bounds__(1024) k_sum3_final(const double *__restrict__ part, long n, double *__restrict__ out) {
+    __shared__ double red[3][16];
+    double a[3] = {0.0, 0.0, 0.0};
+    for (long q = threadIdx.x; q < n; q += blockDim.x)
+        for (int c = 0; c < 3; ++c) a[c] += part[3 * q + c];
+    for (int c = 0; c < 3; ++c) {
+        double x = a[c];
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+        if ((threadIdx.x & 63) == 0) red[c][threadIdx.x >> 6] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        double x = red[threadIdx.x][0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) x += red[threadIdx.x][w];
+        out[threadIdx.x] = x;
+    }
 }
 
 // Block max of non-negative doubles (NaN wins, as in a failed comparison

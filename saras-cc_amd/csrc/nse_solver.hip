@@ -78,6 +78,13 @@ struct saras_nse {
     double wall_dn = 0.0;
     long full = 0;
     int threads = 256;
+    // LES (spiral.cc): sub-grid stresses Txx, Tyy, Tzz, Txy, Tyz, Tzx
+    double *Tsg[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    double *sgs_part = nullptr;          // per-block [sum |K| dV, dissipation dV, nu_t dV]
+    double *sgs_dev = nullptr;           // device: 3 sums, then the failure flag (as int)
+    double *sgs_host = nullptr;          // pinned copy of sgs_dev
+    double sgs_diag[3] = {0.0, 0.0, 0.0};
+    double time = 0.0, tstp = 0.0;       // run time and the input time step (lsRK3_d3.cc:238)
 
     ~saras_nse() {
         for (auto *q : F) (void)hipFree(q);
@@ -92,6 +99,10 @@ struct saras_nse {
         (void)hipHostFree(diag_host);
         (void)hipFree(stage_a);
         (void)hipFree(stage_b);
+        for (auto *q : Tsg) (void)hipFree(q);
+        (void)hipFree(sgs_part);
+        (void)hipFree(sgs_dev);
+        (void)hipHostFree(sgs_host);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (mg) saras_mg_destroy(mg);
@@ -103,6 +114,15 @@ struct saras_nse {
         return dim3((G.n[2] + 2 * p + threads - 1) / threads, G.n[1] + 2 * p, G.n[0] + 2 * p);
     }
     CoreField field(int q) const { return CoreField{F[q], q == 4 ? pbc : bc[q]}; }
+    // stress m as a field of the periodic box
+    saras::nse::PerField tsg(int m) const { return saras::nse::PerField{Tsg[m]}; }
+    bool les_active() const { return p.les && time > 5 * tstp; }
+    // the three stresses whose x, y, z derivatives enter component c
+    // (spiral.cc:225-235): x: Txx Txy Tzx, y: Txy Tyy Tyz, z: Tzx Tyz Tzz
+    saras::nse::PerField sg(int c, int ax) const {
+        static const int m[3][3] = {{0, 3, 5}, {3, 1, 4}, {5, 4, 2}};
+        return tsg(m[c][ax]);
+    }
     CoreField pp() const {
         double *x, *b;
         long sx, sy;
@@ -113,16 +133,44 @@ struct saras_nse {
 
 namespace {
 
+// spiral::computeSG stresses and diagnostics (spiral.cc:129-206); the
+// divergence of the stresses is taken inside the RHS kernels
+saras_status compute_sg(saras_nse *s) {
+    const dim3 gr = s->grid3(), bl(s->threads);
+    NSE_TRY(hipMemsetAsync(s->sgs_dev, 0, 4 * sizeof(double), s->s));
+    hipLaunchKernelGGL(saras::nse::k_sgs_stress, gr, bl, 0, s->s, s->G, s->field(0), s->field(1), s->field(2),
+                       s->p.nu, s->Tsg[0], s->Tsg[1], s->Tsg[2], s->Tsg[3], s->Tsg[4], s->Tsg[5], s->sgs_part,
+                       reinterpret_cast<int *>(s->sgs_dev + 3));
+    hipLaunchKernelGGL(saras::nse::k_sum3_final, dim3(1), dim3(1024), 0, s->s, s->sgs_part, s->nblocks, s->sgs_dev);
+    NSE_TRY(hipGetLastError());
+    NSE_TRY(hipMemcpyAsync(s->sgs_host, s->sgs_dev, 4 * sizeof(double), hipMemcpyDeviceToHost, s->s));
+    NSE_TRY(hipStreamSynchronize(s->s));
+    if (*reinterpret_cast<const int *>(s->sgs_host + 3)) {
+        saras::set_last_error("spiral SGS: eigenvalue or eigenvector of the strain tensor failed (spiral.cc:643-737)");
+        return SARAS_SGS_FAILED;
+    }
+    for (int q = 0; q < 3; ++q) s->sgs_diag[q] = s->sgs_host[q];
+    return SARAS_OK;
+}
+
 // eulerCN_d3 explicit part (eulerCN_d3.cc:92-121, 181-235)
 saras_status rhs_euler(saras_nse *s) {
     const dim3 gr = s->grid3(), bl(s->threads);
     const double k_nu = s->p.nu * 0.5, dt = s->p.dt;   // alphCN2 = 1/2
+    const int les = s->les_active();
+    if (les) {
+        saras_status e = compute_sg(s);
+        if (e) return e;
+    }
     hipLaunchKernelGGL(saras::nse::k_euler_v<0>, gr, bl, 0, s->s, s->G, k_nu, s->p.nu, dt, s->p.buoy[0], s->p.has_T,
-                       s->p.nlscheme, s->field(0), s->field(1), s->field(2), s->field(3), s->field(4), s->R[0]);
+                       s->p.nlscheme, s->field(0), s->field(1), s->field(2), s->field(3), s->field(4), s->R[0], les,
+                       s->sg(0, 0), s->sg(0, 1), s->sg(0, 2));
     hipLaunchKernelGGL(saras::nse::k_euler_v<1>, gr, bl, 0, s->s, s->G, k_nu, s->p.nu, dt, s->p.buoy[1], s->p.has_T,
-                       s->p.nlscheme, s->field(0), s->field(1), s->field(2), s->field(3), s->field(4), s->R[1]);
+                       s->p.nlscheme, s->field(0), s->field(1), s->field(2), s->field(3), s->field(4), s->R[1], les,
+                       s->sg(1, 0), s->sg(1, 1), s->sg(1, 2));
     hipLaunchKernelGGL(saras::nse::k_euler_v<2>, gr, bl, 0, s->s, s->G, k_nu, s->p.nu, dt, s->p.buoy[2], s->p.has_T,
-                       s->p.nlscheme, s->field(0), s->field(1), s->field(2), s->field(3), s->field(4), s->R[2]);
+                       s->p.nlscheme, s->field(0), s->field(1), s->field(2), s->field(3), s->field(4), s->R[2], les,
+                       s->sg(2, 0), s->sg(2, 1), s->sg(2, 2));
     if (s->p.has_T)
         hipLaunchKernelGGL(saras::nse::k_euler_t, gr, bl, 0, s->s, s->G, s->p.kappa * 0.5, s->p.kappa, dt, s->p.nlscheme,
                            s->field(0), s->field(1), s->field(2), s->field(3), s->R[3]);
@@ -146,13 +194,21 @@ saras_status rhs(saras_nse *s, int stage) {
     c.nlscheme = s->p.nlscheme;
     c.nu = s->p.nu;
     c.kappa = s->p.kappa;
+    const int les = s->les_active();
+    if (les) {
+        saras_status e = compute_sg(s);
+        if (e) return e;
+    }
     const dim3 gr = s->grid3(), bl(s->threads);
     hipLaunchKernelGGL(saras::nse::k_rhs_v<0>, gr, bl, 0, s->s, s->G, c, s->field(0), s->field(1), s->field(2),
-                       s->field(3), s->field(4), s->H[0], s->R[0]);
+                       s->field(3), s->field(4), s->H[0], s->R[0], les, s->sg(0, 0), s->sg(0, 1),
+                       s->sg(0, 2));
     hipLaunchKernelGGL(saras::nse::k_rhs_v<1>, gr, bl, 0, s->s, s->G, c, s->field(0), s->field(1), s->field(2),
-                       s->field(3), s->field(4), s->H[1], s->R[1]);
+                       s->field(3), s->field(4), s->H[1], s->R[1], les, s->sg(1, 0), s->sg(1, 1),
+                       s->sg(1, 2));
     hipLaunchKernelGGL(saras::nse::k_rhs_v<2>, gr, bl, 0, s->s, s->G, c, s->field(0), s->field(1), s->field(2),
-                       s->field(3), s->field(4), s->H[2], s->R[2]);
+                       s->field(3), s->field(4), s->H[2], s->R[2], les, s->sg(2, 0), s->sg(2, 1),
+                       s->sg(2, 2));
     if (s->p.has_T)
         hipLaunchKernelGGL(saras::nse::k_rhs_t, gr, bl, 0, s->s, s->G, c, s->field(0), s->field(1), s->field(2),
                            s->field(3), s->H[3], s->R[3]);
@@ -327,6 +383,10 @@ saras_status saras_nse_create(const saras_grid_desc *grid, const saras_mg_params
         saras::set_last_error("Morinishi with a z wall needs ny == nz (edge test on core.ubound(1), vfield.cc:434)");
         return SARAS_BAD_SHAPE;
     }
+    if (P.les && !(g.periodic[0] && g.periodic[1] && g.periodic[2])) {
+        saras::set_last_error("the spiral LES path covers fully periodic boxes");
+        return SARAS_BAD_SHAPE;
+    }
     saras_nse *s = new saras_nse();
     s->g = g;
     s->p = P;
@@ -361,7 +421,7 @@ saras_status saras_nse_create(const saras_grid_desc *grid, const saras_mg_params
     G.edge_last[2] = g.n[1] - 1;   // vfield.cc:434, sfield.cc:295
     G.dscheme = P.dscheme;
     double hwall[6];
-    const size_t stride = (size_t)std::max(g.n[0], std::max(g.n[1], g.n[2])) + 1;
+    const size_t stride = (size_t)std::max(g.n[0], std::max(g.n[1], g.n[2])) + 2;
     std::vector<double> host(12 * stride, 0.0);
     for (int a = 0; a < 3; ++a) {
         std::vector<double> coord, xi_x, xixx, xix2;
@@ -372,7 +432,7 @@ saras_status saras_nse_create(const saras_grid_desc *grid, const saras_mg_params
             host[(3 * a + 1) * stride + i] = xixx[i + p];
             host[(3 * a + 2) * stride + i] = xix2[i + p];
         }
-        for (int i = 0; i <= n; ++i) host[(9 + a) * stride + i] = coord[i + p];
+        for (int i = -1; i <= n; ++i) host[(9 + a) * stride + 1 + i] = coord[i + p];
         hwall[2 * a] = 2 * coord[p];
         hwall[2 * a + 1] = 2 * (coord[n - 1 + p] - g.len[a]);
         // derivative.cc:52-54, vfield.cc:133-135, timestep.cc:156-173
@@ -396,7 +456,7 @@ saras_status saras_nse_create(const saras_grid_desc *grid, const saras_mg_params
         G.xi_x[a] = s->metrics + (3 * a + 0) * stride;
         G.xixx[a] = s->metrics + (3 * a + 1) * stride;
         G.xix2[a] = s->metrics + (3 * a + 2) * stride;
-        G.xc[a] = s->metrics + (9 + a) * stride;
+        G.xc[a] = s->metrics + (9 + a) * stride + 1;
     }
     // upwind weights (vfield.cc:115-130)
     {
@@ -447,6 +507,16 @@ saras_status saras_nse_create(const saras_grid_desc *grid, const saras_mg_params
         return fail(SARAS_OOM);
     if (hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess) return fail(SARAS_HIP_ERROR);
     s->full = (long)(g.n[0] + 2 * g.pad) * (g.n[1] + 2 * g.pad) * (g.n[2] + 2 * g.pad);
+    s->tstp = P.dt;
+    if (P.les) {
+        for (int m = 0; m < 6; ++m) {
+            if (hipMalloc((void **)&s->Tsg[m], s->elems * sizeof(double)) != hipSuccess) return fail(SARAS_OOM);
+            (void)hipMemset(s->Tsg[m], 0, s->elems * sizeof(double));
+        }
+        if (hipMalloc((void **)&s->sgs_part, sizeof(double) * 3 * s->nblocks) != hipSuccess) return fail(SARAS_OOM);
+        if (hipMalloc((void **)&s->sgs_dev, sizeof(double) * 4) != hipSuccess) return fail(SARAS_OOM);
+        if (hipHostMalloc((void **)&s->sgs_host, sizeof(double) * 4) != hipSuccess) return fail(SARAS_OOM);
+    }
     if (hipDeviceSynchronize() != hipSuccess) return fail(SARAS_HIP_ERROR);
     *out = s;
     return SARAS_OK;
@@ -553,6 +623,7 @@ saras_status saras_nse_step(saras_nse *s, saras_nse_stats *st) {
     float ms = 0.f;
     NSE_TRY(hipEventElapsedTime(&ms, s->ev0, s->ev1));
     fill_stats(s, st, ms);
+    if (!e) s->time += s->p.dt;
     return e;
 }
 
@@ -561,8 +632,10 @@ saras_status saras_nse_bench_steps(saras_nse *s, int count, double *ms) {
     NSE_TRY(hipStreamSynchronize(s->s));
     NSE_TRY(hipEventRecord(s->ev0, s->s));
     saras_status e = SARAS_OK;
-    for (int c = 0; c < count && !e; ++c)
+    for (int c = 0; c < count && !e; ++c) {
         for (int k = 0; k < s->stages() && !e; ++k) e = stage_run(s, k);
+        if (!e) s->time += s->p.dt;
+    }
     NSE_TRY(hipEventRecord(s->ev1, s->s));
     NSE_TRY(hipStreamSynchronize(s->s));
     float t = 0.f;
@@ -603,6 +676,14 @@ saras_status saras_field_op(saras_nse *s, int op, const double *f, double *out, 
     if (ptr_kind == SARAS_PTR_HOST)
         NSE_TRY(hipMemcpyAsync(out, s->stage_b, s->full * sizeof(double), hipMemcpyDeviceToHost, s->s));
     NSE_TRY(hipStreamSynchronize(s->s));
+    return SARAS_OK;
+}
+
+saras_status saras_nse_sgs_diag(saras_nse *s, double out[3], double *time) {
+    if (!s) return SARAS_BAD_ARG;
+    if (out)
+        for (int q = 0; q < 3; ++q) out[q] = s->sgs_diag[q];
+    if (time) *time = s->time;
     return SARAS_OK;
 }
 

@@ -14,7 +14,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libsaras_hip.so")
 
-OK, NOT_CONVERGED, JACOBI_DIVERGED, BAD_SHAPE, OOM, HIP_ERROR, BAD_ARG = range(7)
+OK, NOT_CONVERGED, JACOBI_DIVERGED, BAD_SHAPE, OOM, HIP_ERROR, BAD_ARG, SGS_FAILED = range(8)
 GRID_UNIFORM, GRID_TANH_SINGLE, GRID_TANH_DOUBLE = 0, 1, 2
 BC_NEUMANN, BC_DIRICHLET, BC_DIRICHLET_TEST = 0, 1, 2
 PREC_FP64, PREC_MIXED = 0, 1
@@ -56,7 +56,7 @@ class NseParams(C.Structure):
                 ("vbc_type", (C.c_int * 6) * 3), ("vbc_val", (C.c_double * 6) * 3),
                 ("tbc_type", C.c_int * 6), ("tbc_val", C.c_double * 6),
                 ("pbc_type", C.c_int * 6), ("pbc_val", C.c_double * 6), ("scheme", C.c_int),
-                ("pe_limit", C.c_double), ("central_bias", C.c_double)]
+                ("pe_limit", C.c_double), ("central_bias", C.c_double), ("les", C.c_int)]
 
 
 class NseStats(C.Structure):
@@ -89,7 +89,7 @@ EXPORTS = [
     "saras_bench_stream_copy", "saras_nse_create", "saras_nse_destroy", "saras_nse_set_fields",
     "saras_nse_get_fields", "saras_nse_step", "saras_nse_stage", "saras_nse_rhs", "saras_nse_get_rhs",
     "saras_nse_solve_field", "saras_field_op", "saras_nse_diag", "saras_nse_bench_steps",
-    "saras_nse_cfl", "saras_mg_set_graphs", "saras_mg_footprint", "saras_server_create", "saras_server_submit",
+    "saras_nse_cfl", "saras_nse_sgs_diag", "saras_mg_set_graphs", "saras_mg_footprint", "saras_server_create", "saras_server_submit",
     "saras_server_poll", "saras_server_drain", "saras_server_clock", "saras_server_mem_used",
     "saras_server_destroy",
 ]
@@ -141,6 +141,7 @@ def lib():
         L.saras_nse_diag.argtypes = [vp, dp]
         L.saras_nse_bench_steps.argtypes = [vp, C.c_int, dp]
         L.saras_nse_cfl.argtypes = [vp, C.c_double, C.c_double, dp]
+        L.saras_nse_sgs_diag.argtypes = [vp, dp, dp]
         L.saras_mg_set_graphs.argtypes = [vp, C.c_int]
         L.saras_mg_footprint.argtypes = [C.POINTER(GridDesc), C.POINTER(MgParams)]
         L.saras_mg_footprint.restype = C.c_size_t
@@ -345,7 +346,8 @@ def bench_stream_copy(n_doubles, reps):
 
 
 def nse_params(dscheme=2, nlscheme=3, has_T=1, nu=1e-2, kappa=1e-2, dt=1e-3, cn_tol=1e-6,
-               buoy=(0.0, 0.0, -1.0), vbc=None, tbc=None, pbc=None, scheme=0, pe_limit=2.0, central_bias=0.8):
+               buoy=(0.0, 0.0, -1.0), vbc=None, tbc=None, pbc=None, scheme=0, pe_limit=2.0, central_bias=0.8,
+               les=0):
     """Time-step parameters.  vbc: [(types[6], vals[6]) per component];
     tbc / pbc: (types[6], vals[6]).  Defaults: the Rayleigh-Benard wiring of
     hydro.cc:287-485 / scalar.cc:129-201 (no-slip walls, T = 1 at the bottom
@@ -355,6 +357,7 @@ def nse_params(dscheme=2, nlscheme=3, has_T=1, nu=1e-2, kappa=1e-2, dt=1e-3, cn_
     d.nu, d.kappa, d.dt, d.cn_tol = nu, kappa, dt, cn_tol
     d.scheme = scheme
     d.pe_limit, d.central_bias = pe_limit, central_bias
+    d.les = int(les)
     for a in range(3):
         d.buoy[a] = buoy[a]
     if vbc is None:
@@ -448,6 +451,12 @@ class Nse:
         v = np.zeros(4)
         check(lib().saras_nse_diag(self.h, _dp(v)))
         return dict(ke=v[0], divmax=v[1], nuvol=v[2], nuwall=v[3])
+
+    def sgs(self):
+        """Sub-grid diagnostics of the last LES evaluation and the run time."""
+        v, t = np.zeros(3), C.c_double()
+        check(lib().saras_nse_sgs_diag(self.h, _dp(v), C.byref(t)))
+        return dict(sgke=v[0], disp=v[1], nusg=v[2]), t.value
 
     def bench_steps(self, count):
         ms = C.c_double()

@@ -316,3 +316,67 @@ def test_cfl_time_step_matches_oracle():
     assert so.step() == 0
     assert ss.step()[0] == 0
     assert_fields_equal(so, ss)
+
+
+def tg_les_pair(n=16, seed=5, noise=0.05, scheme=0):
+    """Taylor-Green vortex with seeded noise in a periodic unit box, the
+    spiral LES model on (lsRK3_d3 or eulerCN_d3), nu = 1/1600."""
+    pad = 2
+    so, ss = pair((n, n, n), periodic=(1, 1, 1), gtype=(0, 0, 0), pad=pad, has_T=0, nu=1.0 / 1600, dt=1e-3,
+                  cn_tol=1e-10, scheme=scheme, les=1, mg=mg_kw(vc_count=10, vc_tol=1e-10, pre=3, post=3))
+    x = (np.arange(-pad, n + pad) + 0.5) / n
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    tp = 2 * np.pi
+    rng = np.random.default_rng(seed)
+    sh = so.shape
+    f = dict(vx=np.sin(tp * X) * np.cos(tp * Y) * np.cos(tp * Z) + noise * rng.uniform(-1, 1, sh),
+             vy=-np.cos(tp * X) * np.sin(tp * Y) * np.cos(tp * Z) + noise * rng.uniform(-1, 1, sh),
+             vz=noise * rng.uniform(-1, 1, sh), p=np.zeros(sh), t=np.zeros(sh), pp=np.zeros(sh))
+    so.set(**f)
+    ss.set(**f)
+    return so, ss, pad
+
+
+# The sub-grid stress evaluates pow, exp, sin, acos and cos, whose device
+# implementations may differ from the host C library in the last bit; the
+# LES terms are therefore held to a relative tolerance (1e-9 of the largest
+# value) instead of bit equality.
+LES_RTOL = 1e-9
+
+
+@pytest.mark.parametrize("scheme", [0, 1], ids=["rk3", "euler_cn"])
+def test_les_spiral_matches_oracle(scheme):
+    """Steps before solTime > 5 tStp are bit-exact (no model); then the stage
+    RHS with the spiral sub-grid stress divergence, its diagnostics and two
+    more steps agree with the oracle to LES_RTOL."""
+    so, ss, pad = tg_les_pair(scheme=scheme)
+    for step in range(6):
+        assert so.step() == 0
+        assert ss.step()[0] == 0
+    assert_fields_equal(so, ss, names=("vx", "vy", "vz", "p", "pp"), periodic=(1, 1, 1))
+    assert so.time == ss.sgs()[1]
+    c = (slice(pad, -pad),) * 3
+    ro, rs = so.rhs(0), ss.rhs(0)
+    keys = ("rx", "ry", "rz") + (("hx", "hy", "hz") if scheme == 0 else ())
+    for k in keys:
+        a, b = ro[k][c], rs[k][c]
+        assert np.abs(a - b).max() <= LES_RTOL * np.abs(a).max(), (k, np.abs(a - b).max())
+    do = so.sgs_diag()
+    dg, t = ss.sgs()
+    for q, k in enumerate(("sgke", "disp", "nusg")):
+        assert abs(dg[k] - do[q]) <= LES_RTOL * abs(do[q]), (k, dg[k], do[q])
+    f0 = so.get()
+    ss.set(**f0)   # restart both from the same state for the steps
+    for _ in range(2):
+        assert so.step() == 0
+        assert ss.step()[0] == 0
+    fo, fs = so.get(), ss.get()
+    for k in ("vx", "vy", "vz", "p"):
+        a, b = fo[k][c], fs[k][c]
+        assert np.abs(a - b).max() <= LES_RTOL * np.abs(a).max(), (k, np.abs(a - b).max())
+
+
+def test_les_needs_periodic_box():
+    with pytest.raises(S.SarasError) as e:
+        pair((16, 16, 16), periodic=(1, 1, 0), les=1)
+    assert e.value.status == S.BAD_SHAPE

@@ -22,8 +22,6 @@
 
 namespace saras {
 
-constexpr int kXcds = 8;
-
 template <typename T>
 __device__ __forceinline__ T wall_val(const LevelArgs<T> &L, int axis, int i, int j, int k) {
     const int ny = L.n[1], nz = L.n[2];
@@ -326,7 +324,7 @@ template <int TYPE, int DIM>
 __global__ void __launch_bounds__(kNormThreads) k_refine(LevelArgs<double> L, const double *__restrict__ xold,
                                                          const float *__restrict__ e, long esx, long esy,
                                                          const double *__restrict__ b, double *__restrict__ xnew,
-                                                         float *__restrict__ r32, long rsx, long rsy,
+                                                         float *__restrict__ r32, long rsx, long rsy, long ehz,
                                                          double *__restrict__ partial) {
     const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
     const long rows = (long)nx * ny;
@@ -339,7 +337,8 @@ __global__ void __launch_bounds__(kNormThreads) k_refine(LevelArgs<double> L, co
         const long idx = (long)i * L.sx + (long)j * L.sy + k;
         auto xv = [&](int ii, int jj, int kk) -> double {
             const double v = xold[(long)ii * L.sx + (long)jj * L.sy + kk];
-            return e ? v + (double)e[(long)ii * esx + (long)jj * esy + kk] : v;
+            return e ? v + (double)e[(long)ii * esx + (long)jj * esy + (ehz ? (long)(kk & 1) * ehz + (kk >> 1) : (long)kk)]
+                     : v;
         };
         Stencil<double, DIM> s;
         s.c = xv(i, j, k);
@@ -373,7 +372,7 @@ __global__ void __launch_bounds__(kNormThreads) k_refine(LevelArgs<double> L, co
         const double bv = b[idx];
         const double rv = residual<double, DIM>(s, L.c2[0][i], L.c1[0][i], c2y, c1y, L.c2[2][k], L.c1[2][k], bv);
         xnew[idx] = s.c;
-        r32[(long)i * rsx + (long)j * rsy + k] = (float)rv;
+        r32[(long)i * rsx + (long)j * rsy + (ehz ? (long)(k & 1) * ehz + (k >> 1) : (long)k)] = (float)rv;
         ar = norm_acc<TYPE>(ar, rv);
         ab = norm_acc<TYPE>(ab, bv);
     }

@@ -15,6 +15,12 @@
 //    occur on the coarsest level, which the single-workgroup kernel handles
 //    with literal ghost arrays.
 //
+// Row layouts: a level stores each row (i, j) either z-contiguous (element k
+// at offset k) or colour-split (hz > 0: even k at k/2, odd k at hz + k/2).
+// In the split layout one colour of a row is one contiguous half-row, so a
+// red or black pass streams whole vectors (lib/poisson has no such layout;
+// the arithmetic per cell is unchanged).
+//
 // Every expression keeps the reference's operand order; the file is compiled
 // with -ffp-contract=off so the FP64 path matches the FP64 CPU build bit for
 // bit.
@@ -27,18 +33,28 @@ namespace saras {
 
 enum BcCode : int { BC_PER = 0, BC_NEU = 1, BC_DIR = 2, BC_DIRV = 3 };
 
+constexpr int kXcds = 8;   // MI355X: 8 XCDs, workgroups dealt round-robin
+
 template <typename T>
 struct LevelArgs {
     int n[3];          // cells (n[1] == 1 for planar problems)
     long sx, sy;       // element strides of i and j
+    long hz;           // 0: z-contiguous rows; > 0: colour-split rows (odd k at hz + k/2)
     const T *c2[3];    // xi_x^2 / h^2 per core index (poisson.cc:453-483 x copyDerivs)
     const T *c1[3];    // xi_xx / (2 h)
+    const T *c2zs, *c1zs;  // split rows: c2[2], c1[2] reordered as [h * nz/2 + m] = c[2m + h]
     int bc_lo[3], bc_hi[3];
     // inhomogeneous wall values (BC_DIRV), padded face arrays with a zero
     // shell: x-walls [(j+1)(nz+2) + k+1], y-walls [(i+1)(nz+2) + k+1],
     // z-walls [(i+1)(ny+2) + j+1]; planar: x-walls [k+1], z-walls [i+1]
     const T *wall[3];
 };
+
+// offset of element k inside its row
+template <typename T>
+__device__ __forceinline__ long kof(const LevelArgs<T> &L, int k) {
+    return L.hz ? (long)(k & 1) * L.hz + (k >> 1) : (long)k;
+}
 
 // ---------------------------------------------------------------------------
 // neighbour values with the boundary rule applied
@@ -92,18 +108,19 @@ __device__ __forceinline__ void gather(const LevelArgs<T> &L, const T *__restric
             s.yS = bc_value(L.bc_lo[1], s.c, wrap, wv);
         }
     }
-    if (k + 1 < nz) s.zT = x[idx + 1];
+    const long row = idx - kof(L, k);
+    if (k + 1 < nz) s.zT = x[row + kof(L, k + 1)];
     else {
         T wv = T(0);
         if (L.bc_hi[2] == BC_DIRV) wv = DIM == 3 ? L.wall[2][(i + 1) * (ny + 2) + j + 1] : L.wall[2][i + 1];
-        T wrap = L.bc_hi[2] == BC_PER ? x[idx - (nz - 1)] : T(0);
+        T wrap = L.bc_hi[2] == BC_PER ? x[row + kof(L, 0)] : T(0);
         s.zT = bc_value(L.bc_hi[2], s.c, wrap, wv);
     }
-    if (k > 0) s.zB = x[idx - 1];
+    if (k > 0) s.zB = x[row + kof(L, k - 1)];
     else {
         T wv = T(0);
         if (L.bc_lo[2] == BC_DIRV) wv = DIM == 3 ? L.wall[2][(i + 1) * (ny + 2) + j + 1] : L.wall[2][i + 1];
-        T wrap = L.bc_lo[2] == BC_PER ? x[idx + (nz - 1)] : T(0);
+        T wrap = L.bc_lo[2] == BC_PER ? x[row + kof(L, nz - 1)] : T(0);
         s.zB = bc_value(L.bc_lo[2], s.c, wrap, wv);
     }
 }
@@ -168,7 +185,7 @@ __global__ void __launch_bounds__(256) k_rb_pass(LevelArgs<T> L, T *__restrict__
     const int i = (int)(r / ny);
     const int k = 2 * m + ((i + j + colour) & 1);
     if (k >= nz) return;
-    const long idx = (long)i * L.sx + (long)j * L.sy + k;
+    const long idx = (long)i * L.sx + (long)j * L.sy + kof(L, k);
     Stencil<T, DIM> s;
     gather<T, DIM>(L, x, i, j, k, idx, s);
     const T c2y = DIM == 3 ? L.c2[1][j] : T(0), c1y = DIM == 3 ? L.c1[1][j] : T(0);
@@ -206,7 +223,7 @@ __global__ void __launch_bounds__(256) k_resid_restrict(LevelArgs<T> F, const T 
         for (int dj = 0; dj < jmax; ++dj)
             for (int dk = 0; dk < 2; ++dk) {
                 const int i = 2 * I + di, j = DIM == 3 ? 2 * J + dj : 0, k = 2 * K + dk;
-                const long idx = (long)i * F.sx + (long)j * F.sy + k;
+                const long idx = (long)i * F.sx + (long)j * F.sy + kof(F, k);
                 Stencil<T, DIM> s;
                 gather<T, DIM>(F, x, i, j, k, idx, s);
                 const T c2y = DIM == 3 ? F.c2[1][j] : T(0), c1y = DIM == 3 ? F.c1[1][j] : T(0);
@@ -223,7 +240,7 @@ __global__ void __launch_bounds__(256) k_resid_restrict(LevelArgs<T> F, const T 
     } else {
         out = (res[1][0][1] + res[0][0][0] + res[1][0][0] + res[0][0][1]) / T(4);
     }
-    const long cidx = (long)I * C.sx + (long)J * C.sy + K;
+    const long cidx = (long)I * C.sx + (long)J * C.sy + kof(C, K);
     bc[cidx] = out;
     xc[cidx] = T(0);
 }
@@ -234,9 +251,9 @@ __global__ void __launch_bounds__(256) k_resid_restrict(LevelArgs<T> F, const T 
 // is left untouched while the coarse levels work, so smd is x itself.
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void __launch_bounds__(256) k_prolong_correct(int nx, int ny, int nz, long sx, long sy,
+__global__ void __launch_bounds__(256) k_prolong_correct(int nx, int ny, int nz, long sx, long sy, long hz,
                                                          T *__restrict__ x, const T *__restrict__ e, long csx,
-                                                         long csy) {
+                                                         long csy, long chz) {
     const long total = (long)nx * ny * nz;
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= total) return;
@@ -244,8 +261,9 @@ __global__ void __launch_bounds__(256) k_prolong_correct(int nx, int ny, int nz,
     const long r = t / nz;
     const int j = (int)(r % ny);
     const int i = (int)(r / ny);
-    const long idx = (long)i * sx + (long)j * sy + k;
-    const T ev = e[(long)(i >> 1) * csx + (long)(j >> 1) * csy + (k >> 1)];
+    const long idx = (long)i * sx + (long)j * sy + (hz ? (long)(k & 1) * hz + (k >> 1) : (long)k);
+    const int kc = k >> 1;
+    const T ev = e[(long)(i >> 1) * csx + (long)(j >> 1) * csy + (chz ? (long)(kc & 1) * chz + (kc >> 1) : (long)kc)];
     x[idx] = ev + x[idx];
 }
 
@@ -319,7 +337,7 @@ __global__ void __launch_bounds__(kNormThreads) k_norm_partial(LevelArgs<double>
             const double c2x = L.c2[0][i], c1x = L.c1[0][i];
             const double c2y = DIM == 3 ? L.c2[1][j] : 0.0, c1y = DIM == 3 ? L.c1[1][j] : 0.0;
             for (int k = w.k0; k < nz; k += w.kspan) {
-                const long idx = (long)i * L.sx + (long)j * L.sy + k;
+                const long idx = (long)i * L.sx + (long)j * L.sy + kof(L, k);
                 Stencil<double, DIM> s;
                 gather<double, DIM>(L, x, i, j, k, idx, s);
                 const double bv = b[idx];
@@ -493,7 +511,7 @@ __global__ void __launch_bounds__(1024) k_coarse(LevelArgs<T> L, T *__restrict__
         const long r = t / nz;
         const int j = (int)(r % ny);
         const int i = (int)(r / ny);
-        const long g = (long)i * L.sx + (long)j * L.sy + k;
+        const long g = (long)i * L.sx + (long)j * L.sy + kof(L, k);
         X[cidx<T, DIM>(i, j, k, PY, PZ)] = x[g];
         B[t] = b[g];
     }
@@ -559,7 +577,7 @@ __global__ void __launch_bounds__(1024) k_coarse(LevelArgs<T> L, T *__restrict__
         const long r = t / nz;
         const int j = (int)(r % ny);
         const int i = (int)(r / ny);
-        x[(long)i * L.sx + (long)j * L.sy + k] = X[cidx<T, DIM>(i, j, k, PY, PZ)];
+        x[(long)i * L.sx + (long)j * L.sy + kof(L, k)] = X[cidx<T, DIM>(i, j, k, PY, PZ)];
     }
     if (threadIdx.x == 0 && ctl.mode == 0 && ctl.iters_out) *ctl.iters_out = iter;
 }

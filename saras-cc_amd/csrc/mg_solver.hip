@@ -24,6 +24,7 @@
 #include "mg_fused.hpp"
 #include "mg_internal.hpp"
 #include "mg_kernels.hpp"
+#include "mg_split.hpp"
 #include "mg_tail.hpp"
 #include "saras_hip.h"
 
@@ -59,6 +60,8 @@ struct DevLevel {
     size_t elems = 0;  // allocation size in elements
     T *x = nullptr, *b = nullptr;
     T *xalt = nullptr;  // second iterate buffer: fused sweeps run out of place
+    long hz = 0;        // colour-split rows when > 0 (offset of the odd half)
+    T *c2zs = nullptr, *c1zs = nullptr;  // z coefficients in split order
     T *c2[3] = {nullptr, nullptr, nullptr}, *c1[3] = {nullptr, nullptr, nullptr};
     T *rlo[3] = {nullptr, nullptr, nullptr}, *rhi[3] = {nullptr, nullptr, nullptr}, *rwd[3] = {nullptr, nullptr, nullptr};
 };
@@ -79,10 +82,18 @@ struct Hierarchy {
     T *coarse_scratch = nullptr;
     bool coarse_lds = true;
     size_t coarse_lds_bytes = 0;
+    bool split_rows = false;   // store levels whose rows split into whole vectors colour-split
+
+    // a level takes split rows when each half-row is a whole number of 16-byte vectors
+    static bool splittable(int nz) {
+        constexpr int W = 16 / (int)sizeof(T);
+        return nz >= 2 * W && nz % (2 * W) == 0;
+    }
 
     ~Hierarchy() {
         for (auto &L : lv) {
             (void)hipFree(L.x); (void)hipFree(L.b); (void)hipFree(L.xalt);
+            (void)hipFree(L.c2zs); (void)hipFree(L.c1zs);
             for (int a = 0; a < 3; ++a) {
                 (void)hipFree(L.c2[a]); (void)hipFree(L.c1[a]);
                 (void)hipFree(L.rlo[a]); (void)hipFree(L.rhi[a]); (void)hipFree(L.rwd[a]);
@@ -103,10 +114,11 @@ struct Hierarchy {
             L.sy = row_pitch<T>(L.n[2]);
             L.sx = L.sy * (long)L.n[1];
             L.elems = (size_t)L.sx * L.n[0];
+            L.hz = split_rows && splittable(L.n[2]) ? L.sy / 2 : 0;
             HIP_TRY(hipMalloc((void **)&L.x, L.elems * sizeof(T)));
             HIP_TRY(hipMalloc((void **)&L.b, L.elems * sizeof(T)));
             HIP_TRY(hipMemset(L.x, 0, L.elems * sizeof(T)));
-            if (l < D && dim == 3) {
+            if (l < D && dim == 3 && L.hz == 0) {
                 HIP_TRY(hipMalloc((void **)&L.xalt, L.elems * sizeof(T)));
                 HIP_TRY(hipMemset(L.xalt, 0, L.elems * sizeof(T)));
             }
@@ -130,6 +142,16 @@ struct Hierarchy {
                     c1[i] = A.xixx[i + 1] * i2h;
                 }
                 if (upload(&L.c2[a], c2) || upload(&L.c1[a], c1)) return SARAS_HIP_ERROR;
+                if (a == 2 && L.hz) {
+                    const int half = A.n / 2;
+                    std::vector<double> c2s(A.n), c1s(A.n);
+                    for (int hh = 0; hh < 2; ++hh)
+                        for (int mm = 0; mm < half; ++mm) {
+                            c2s[hh * half + mm] = c2[2 * mm + hh];
+                            c1s[hh * half + mm] = c1[2 * mm + hh];
+                        }
+                    if (upload(&L.c2zs, c2s) || upload(&L.c1zs, c1s)) return SARAS_HIP_ERROR;
+                }
                 if (l > 0) {
                     // restriction factors of coarse index I (poisson_d3.cc:494-502)
                     const saras::AxisLevel &Fa = G.axes[l - 1][a];
@@ -195,6 +217,7 @@ struct saras_mg {
     bool tail_off = false;             // SARAS_NO_TAIL=1 runs every level with multi-block kernels
     int march_wgs = 0;                 // target workgroups of a march sweep (SARAS_MARCH_WGS; 0 = per type)
     bool march_tj8 = false;            // FP32 march with 64 x 8 tiles (SARAS_MARCH_TJ8=1)
+    bool split_off = false;            // SARAS_NO_SPLIT=1 keeps every FP32 level z-contiguous
     // V-cycles replayed from captured hipGraphs (saras_mg_set_graphs).  A
     // graph bakes in the level buffers, and out-of-place sweeps swap them, so
     // graphs are cached per buffer state (at most two states per precision).
@@ -234,6 +257,9 @@ saras::LevelArgs<T> level_args(const saras_mg *m, const Hierarchy<T> &H, int l, 
     }
     a.sx = L.sx;
     a.sy = L.sy;
+    a.hz = L.hz;
+    a.c2zs = L.c2zs;
+    a.c1zs = L.c1zs;
     return a;
 }
 
@@ -286,7 +312,7 @@ struct Ops {
 
     bool march_ok(int l) const {
         const DevLevel<T> &L = H.lv[l];
-        if (H.dim != 3 || !L.xalt || m->fused_off) return false;
+        if (H.dim != 3 || !L.xalt || m->fused_off || L.hz) return false;
         for (int q = 0; q < 3; ++q)
             if (m->g.periodic[q] && (L.n[q] & 1)) return false;
         // the march needs many tiles and planes to pay off; smaller levels use
@@ -297,6 +323,26 @@ struct Ops {
     void rb_sweep(int l, bool zero_bc) {
         DevLevel<T> &L = H.lv[l];
         auto a = level_args(m, H, l, inhom(l, zero_bc));
+        if (L.hz) {
+            constexpr int W = saras::Vec16<T>::W;
+            const long total = (long)L.n[0] * L.n[1] * (L.n[2] / 2 / W);
+            const int nb = (int)((total + saras::kSplitThreads - 1) / saras::kSplitThreads);
+            const int grid = (nb + saras::kXcds - 1) / saras::kXcds * saras::kXcds;
+            for (int colour = 0; colour < 2; ++colour) {
+                if (H.dim == 3) {
+                    if (l == 0)
+                        hipLaunchKernelGGL((saras::k_rb_split<T, 3, true>), dim3(grid), dim3(saras::kSplitThreads), 0, s, a, L.x, L.b, colour, w, nb);
+                    else
+                        hipLaunchKernelGGL((saras::k_rb_split<T, 3, false>), dim3(grid), dim3(saras::kSplitThreads), 0, s, a, L.x, L.b, colour, w, nb);
+                } else {
+                    if (l == 0)
+                        hipLaunchKernelGGL((saras::k_rb_split<T, 2, true>), dim3(grid), dim3(saras::kSplitThreads), 0, s, a, L.x, L.b, colour, w, nb);
+                    else
+                        hipLaunchKernelGGL((saras::k_rb_split<T, 2, false>), dim3(grid), dim3(saras::kSplitThreads), 0, s, a, L.x, L.b, colour, w, nb);
+                }
+            }
+            return;
+        }
         if (march_ok(l)) {
             // FP32: 64 x 16 tiles (512 lanes); FP64: 64 x 8 tiles to keep LDS per workgroup ~50 KB
             constexpr int TJ = sizeof(T) == 4 ? 16 : 8;
@@ -372,7 +418,7 @@ struct Ops {
         DevLevel<T> &F = H.lv[l], &C = H.lv[l + 1];
         const long n = (long)F.n[0] * F.n[1] * F.n[2];
         hipLaunchKernelGGL((saras::k_prolong_correct<T>), dim3(blocks_for(n)), dim3(kThreads), 0, s, F.n[0], F.n[1],
-                           F.n[2], F.sx, F.sy, F.x, C.x, C.sx, C.sy);
+                           F.n[2], F.sx, F.sy, F.hz, F.x, C.x, C.sx, C.sy, C.hz);
     }
 
     // first level of the single-workgroup tail: the smallest S whose levels
@@ -722,6 +768,8 @@ saras_status saras_mg_create(const saras_grid_desc *grid, const saras_mg_params 
         if (env3 && std::atoi(env3) > 0) m->march_wgs = std::atoi(env3);
         const char *env4 = std::getenv("SARAS_MARCH_TJ8");
         m->march_tj8 = env4 && env4[0] == '1';
+        const char *env5 = std::getenv("SARAS_NO_SPLIT");
+        m->split_off = env5 && env5[0] == '1';
     }
     if (m->p.inner_cycles <= 0) m->p.inner_cycles = 3;
     if (m->p.max_refine <= 0) m->p.max_refine = 20;
@@ -740,6 +788,7 @@ saras_status saras_mg_create(const saras_grid_desc *grid, const saras_mg_params 
     if (st) return st;
     if (p.precision == SARAS_PREC_MIXED) {
         m->h32.reset(new Hierarchy<float>());
+        m->h32->split_rows = !m->split_off;
         std::vector<std::vector<double>> nowall(3);
         st = m->h32->build(m->G, nowall);
         if (st) return st;
@@ -808,24 +857,24 @@ static saras_status solve_fp64(saras_mg *m, hipStream_t s, saras_mg_stats *st) {
 
 template <int DIM>
 void launch_refine(int type, saras_mg *m, const saras::LevelArgs<double> &a, const float *e, long esx, long esy,
-                   float *r32, hipStream_t s) {
+                   long ehz, float *r32, hipStream_t s) {
     auto &L0 = m->h64.lv[0];
     switch (type) {
         case 0:
             hipLaunchKernelGGL((saras::k_refine<0, DIM>), dim3(saras::kNormBlocks), dim3(saras::kNormThreads), 0, s, a,
-                               L0.x, e, esx, esy, L0.b, m->x64b, r32, esx, esy, m->partial);
+                               L0.x, e, esx, esy, L0.b, m->x64b, r32, esx, esy, ehz, m->partial);
             hipLaunchKernelGGL((saras::k_norm_final<0>), dim3(1), dim3(saras::kNormThreads), 0, s, m->partial,
                                saras::kNormBlocks, m->norm_out);
             break;
         case 1:
             hipLaunchKernelGGL((saras::k_refine<1, DIM>), dim3(saras::kNormBlocks), dim3(saras::kNormThreads), 0, s, a,
-                               L0.x, e, esx, esy, L0.b, m->x64b, r32, esx, esy, m->partial);
+                               L0.x, e, esx, esy, L0.b, m->x64b, r32, esx, esy, ehz, m->partial);
             hipLaunchKernelGGL((saras::k_norm_final<1>), dim3(1), dim3(saras::kNormThreads), 0, s, m->partial,
                                saras::kNormBlocks, m->norm_out);
             break;
         default:
             hipLaunchKernelGGL((saras::k_refine<2, DIM>), dim3(saras::kNormBlocks), dim3(saras::kNormThreads), 0, s, a,
-                               L0.x, e, esx, esy, L0.b, m->x64b, r32, esx, esy, m->partial);
+                               L0.x, e, esx, esy, L0.b, m->x64b, r32, esx, esy, ehz, m->partial);
             hipLaunchKernelGGL((saras::k_norm_final<2>), dim3(1), dim3(saras::kNormThreads), 0, s, m->partial,
                                saras::kNormBlocks, m->norm_out);
             break;
@@ -838,8 +887,8 @@ static saras_status refine_step(saras_mg *m, hipStream_t s, bool with_e, double 
     auto &F0 = m->h32->lv[0];
     auto a = level_args(m, m->h64, 0, true);
     const float *e = with_e ? F0.x : nullptr;
-    if (m->G.dim == 3) launch_refine<3>(m->p.res_type, m, a, e, F0.sx, F0.sy, F0.b, s);
-    else launch_refine<2>(m->p.res_type, m, a, e, F0.sx, F0.sy, F0.b, s);
+    if (m->G.dim == 3) launch_refine<3>(m->p.res_type, m, a, e, F0.sx, F0.sy, F0.hz, F0.b, s);
+    else launch_refine<2>(m->p.res_type, m, a, e, F0.sx, F0.sy, F0.hz, F0.b, s);
     std::swap(L0.x, m->x64b);
     HIP_TRY(hipMemcpyAsync(m->norm_host, m->norm_out, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));

@@ -164,7 +164,7 @@ __global__ void __launch_bounds__(1024) k_tail(TailArgs<T> A) {
             const long r = t / L.n[2];
             const int j = (int)(r % L.n[1]);
             const int i = (int)(r / L.n[1]);
-            const long g = (long)i * L.sx + (long)j * L.sy + k;
+            const long g = (long)i * L.sx + (long)j * L.sy + kof(L, k);
             XS[cidx<T, DIM>(i, j, k, PY, PZ)] = A.x[g];
             B(S)[t] = A.b[g];
         }
@@ -252,7 +252,7 @@ __global__ void __launch_bounds__(1024) k_tail(TailArgs<T> A) {
             const long r = t / L.n[2];
             const int j = (int)(r % L.n[1]);
             const int i = (int)(r / L.n[1]);
-            A.x[(long)i * L.sx + (long)j * L.sy + k] = X(S)[cidx<T, DIM>(i, j, k, PY, PZ)];
+            A.x[(long)i * L.sx + (long)j * L.sy + kof(L, k)] = X(S)[cidx<T, DIM>(i, j, k, PY, PZ)];
         }
     }
     if (threadIdx.x == 0 && A.iters_out && A.solve_flag) *A.iters_out = iters;

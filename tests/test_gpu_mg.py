@@ -324,27 +324,44 @@ def test_fused_march_sweeps_bitexact_at_size(case):
     np.testing.assert_array_equal(inner(ms.get_level(0)[0]), inner(mo.get_level(0)[0]))
 
 
-@pytest.mark.parametrize("variant", ["two-pass", "tj8"])
-def test_fp32_march_matches_fp32_two_pass(variant, monkeypatch):
-    """FP32 cycles of the mixed solver: the fused march sweep (64 x 16 tiles,
-    one barrier per plane) and its 64 x 8 variant give the same FP32
-    arithmetic as the two in-place colour passes, so whole mixed solves are
-    bit-identical."""
-    g = S.grid_desc((128, 256, 256), periodic=(1, 1, 0), gtype=(0, 0, 2), beta=(1, 1, 1.2), pad=1)
-    p = S.mg_params(vc_count=6, vc_tol=1e-14, solve_flag=0, mg_tol=1e-2, pre=2, post=2, bc=S.BC_NEUMANN,
+SPLIT_CASES = [
+    dict(n=(128, 256, 256), dim=3, periodic=(1, 1, 0), gtype=(0, 0, 2), beta=(1, 1, 1.2), bc=S.BC_NEUMANN),
+    dict(n=(96, 80, 72), dim=3, periodic=(0, 0, 0), gtype=(2, 0, 1), beta=(1.2, 1, 0.9), bc=S.BC_DIRICHLET),
+    dict(n=(64, 64, 64), dim=3, periodic=(1, 1, 1), gtype=(0, 0, 0), beta=(1, 1, 1), bc=S.BC_NEUMANN),
+    dict(n=(256, 1, 512), dim=2, periodic=(1, 0, 0), gtype=(0, 0, 2), beta=(1, 1, 1.3), bc=S.BC_NEUMANN),
+]
+
+
+@pytest.mark.parametrize("variant", ["z-march", "z-two-pass", "z-tj8"])
+@pytest.mark.parametrize("case", SPLIT_CASES, ids=["ppn-256", "nnn-mixed-layouts", "ppp-64", "2d-pn"])
+def test_fp32_split_rows_match_z_contiguous(case, variant, monkeypatch):
+    """FP32 cycles of the mixed solver on colour-split rows (the default for
+    every level whose half-rows are whole float4 vectors) against the
+    z-contiguous layout with the fused march sweep, its 64 x 8 variant or the
+    two in-place colour passes: the same FP32 arithmetic per cell, so whole
+    mixed solves are bit-identical (cases include hierarchies whose coarse
+    levels fall back to z-contiguous rows)."""
+    if case["dim"] == 2 and variant == "z-tj8":
+        pytest.skip("the march sweep is 3-D only")
+    g = S.grid_desc(case["n"], dim=case["dim"], periodic=case["periodic"], gtype=case["gtype"], beta=case["beta"],
+                    pad=1)
+    p = S.mg_params(vc_count=6, vc_tol=1e-14, solve_flag=0, mg_tol=1e-2, pre=2, post=2, bc=case["bc"],
                     precision=S.PREC_MIXED, inner_cycles=3, max_refine=2)
     rng = np.random.default_rng(32)
-    b = np.zeros((130, 258, 258))
-    b[1:-1, 1:-1, 1:-1] = rng.standard_normal((128, 256, 256))
-    fused = S.Multigrid(g, p)
-    xf, sf, _ = fused.solve(np.zeros_like(b), b)
-    fused.close()
-    if variant == "two-pass":
+    n = case["n"] if case["dim"] == 3 else (case["n"][0], case["n"][2])
+    b = np.zeros(tuple(m + 2 for m in n))
+    b[(slice(1, -1),) * len(n)] = rng.standard_normal(n)
+    split = S.Multigrid(g, p)
+    xs, ss, _ = split.solve(np.zeros_like(b), b)
+    split.close()
+    monkeypatch.setenv("SARAS_NO_SPLIT", "1")
+    if variant == "z-two-pass":
         monkeypatch.setenv("SARAS_NO_FUSED_SWEEP", "1")
-    else:
+    elif variant == "z-tj8":
         monkeypatch.setenv("SARAS_MARCH_TJ8", "1")
     other = S.Multigrid(g, p)
     xo, so, _ = other.solve(np.zeros_like(b), b)
     other.close()
-    assert sf["cycles"] == so["cycles"]
-    np.testing.assert_array_equal(xf, xo)
+    assert ss["cycles"] == so["cycles"]
+    assert ss["residual"] == so["residual"]
+    np.testing.assert_array_equal(xs, xo)

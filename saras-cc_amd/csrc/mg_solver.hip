@@ -405,6 +405,19 @@ struct Ops {
             r.wd[q] = C.rwd[q];
         }
         const long n = (long)C.n[0] * C.n[1] * C.n[2];
+        if (F.hz) {
+            constexpr int W = saras::Vec16<T>::W;
+            const long total = (long)C.n[0] * C.n[1] * (F.n[2] / 2 / W);
+            const int nb = (int)((total + saras::kSplitThreads - 1) / saras::kSplitThreads);
+            const int grid = (nb + saras::kXcds - 1) / saras::kXcds * saras::kXcds;
+            if (H.dim == 3)
+                hipLaunchKernelGGL((saras::k_rr_split<T, 3>), dim3(grid), dim3(saras::kSplitThreads), 0, s, fa, F.x,
+                                   F.b, ca, C.b, C.x, r, nb);
+            else
+                hipLaunchKernelGGL((saras::k_rr_split<T, 2>), dim3(grid), dim3(saras::kSplitThreads), 0, s, fa, F.x,
+                                   F.b, ca, C.b, C.x, r, nb);
+            return;
+        }
         if (H.dim == 3)
             hipLaunchKernelGGL((saras::k_resid_restrict<T, 3>), dim3(blocks_for(n)), dim3(kThreads), 0, s, fa, F.x, F.b,
                                ca, C.b, C.x, r);
@@ -417,6 +430,15 @@ struct Ops {
         if (l < 0 || l + 1 >= (int)H.lv.size()) std::abort();  // host-side guard: never launch past the hierarchy
         DevLevel<T> &F = H.lv[l], &C = H.lv[l + 1];
         const long n = (long)F.n[0] * F.n[1] * F.n[2];
+        if (F.hz) {
+            constexpr int W = saras::Vec16<T>::W;
+            const long total = (long)F.n[0] * F.n[1] * (F.n[2] / 2 / W);
+            const int nb = (int)((total + saras::kSplitThreads - 1) / saras::kSplitThreads);
+            const int grid = (nb + saras::kXcds - 1) / saras::kXcds * saras::kXcds;
+            hipLaunchKernelGGL((saras::k_prolong_split<T>), dim3(grid), dim3(saras::kSplitThreads), 0, s, F.n[0],
+                               F.n[1], F.n[2], F.sx, F.sy, F.hz, F.x, C.x, C.sx, C.sy, C.hz, nb);
+            return;
+        }
         hipLaunchKernelGGL((saras::k_prolong_correct<T>), dim3(blocks_for(n)), dim3(kThreads), 0, s, F.n[0], F.n[1],
                            F.n[2], F.sx, F.sy, F.hz, F.x, C.x, C.sx, C.sy, C.hz);
     }

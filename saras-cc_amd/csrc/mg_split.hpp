@@ -156,4 +156,234 @@ __global__ void __launch_bounds__(kSplitThreads) k_rb_split(LevelArgs<T> L, T *_
     *reinterpret_cast<V *>(x + own) = VT::make(r);
 }
 
+// W consecutive coarse cells K0..K0+W-1 of coarse row (I, J) in the coarse
+// level's own layout (split or z-contiguous), as one vector
+template <typename T>
+__device__ __forceinline__ void coarse_vec(const T *__restrict__ e, long base, long chz, int K0, T (&out)[Vec16<T>::W]) {
+    using VT = Vec16<T>;
+    constexpr int W = VT::W;
+    if (chz == 0) {
+        const typename VT::V v = *reinterpret_cast<const typename VT::V *>(e + base + K0);
+#pragma unroll
+        for (int q = 0; q < W; ++q) out[q] = VT::get(v, q);
+    } else {
+        // even K at K/2, odd K at chz + K/2 (K0 is a multiple of W)
+#pragma unroll
+        for (int q = 0; q < W; ++q) out[q] = e[base + (q & 1) * chz + ((K0 + q) >> 1)];
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void coarse_store(T *__restrict__ dst, long base, long chz, int K0,
+                                             const T (&in)[Vec16<T>::W]) {
+    using VT = Vec16<T>;
+    constexpr int W = VT::W;
+    if (chz == 0) {
+        *reinterpret_cast<typename VT::V *>(dst + base + K0) = VT::make(in);
+    } else {
+#pragma unroll
+        for (int q = 0; q < W; ++q) dst[base + (q & 1) * chz + ((K0 + q) >> 1)] = in[q];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// residual of a split level restricted to the next level (the same
+// computeResidual + coarsen expressions as k_resid_restrict,
+// poisson_d3.cc:71-93, 484-511; poisson_d2.cc:288-297), and x(l+1) = 0.  A
+// lane owns W consecutive coarse cells K0.. of coarse row (I, J): their fine
+// children are the even and odd half-rows at K0.. of the 2 x 2 child rows.
+// ---------------------------------------------------------------------------
+template <typename T, int DIM>
+__global__ void __launch_bounds__(kSplitThreads) k_rr_split(LevelArgs<T> F, const T *__restrict__ x,
+                                                            const T *__restrict__ b, LevelArgs<T> C,
+                                                            T *__restrict__ bc, T *__restrict__ xc, RestrictArgs<T> R,
+                                                            int nblocks) {
+    using VT = Vec16<T>;
+    using V = typename VT::V;
+    constexpr int W = VT::W;
+    const int per = gridDim.x / kXcds;
+    const int logical = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+    if (logical >= nblocks) return;
+    const int nx = F.n[0], ny = F.n[1], nz = F.n[2];
+    const int half = nz >> 1, q = half / W;
+    const int cy = C.n[1];
+    const long total = (long)C.n[0] * cy * q;
+    const long t = (long)logical * kSplitThreads + threadIdx.x;
+    if (t >= total) return;
+    const int crow = (int)(t / q);
+    const int v = (int)(t - (long)crow * q);
+    const int I = crow / cy, J = crow - (crow / cy) * cy;
+    const int K0 = v * W;
+    const long hz = F.hz;
+    const int lane = threadIdx.x & 63;
+    auto ldv = [&](const T *p, T (&out)[W]) {
+        const V vv = *reinterpret_cast<const V *>(p);
+#pragma unroll
+        for (int e = 0; e < W; ++e) out[e] = VT::get(vv, e);
+    };
+    // residuals of the children: [di][dj][dk][e], dk = 0 the even cell 2K, 1 the odd cell 2K+1
+    T res[2][DIM == 3 ? 2 : 1][2][W];
+    const T c2e0 = T(0);
+    (void)c2e0;
+    T cz2e[W], cz1e[W], cz2o[W], cz1o[W];
+    ldv(F.c2zs + K0, cz2e);
+    ldv(F.c1zs + K0, cz1e);
+    ldv(F.c2zs + half + K0, cz2o);
+    ldv(F.c1zs + half + K0, cz1o);
+#pragma unroll
+    for (int di = 0; di < 2; ++di) {
+#pragma unroll
+        for (int dj = 0; dj < (DIM == 3 ? 2 : 1); ++dj) {
+            const int i = 2 * I + di, j = DIM == 3 ? 2 * J + dj : 0;
+            const long rb = (long)i * F.sx + (long)j * F.sy;
+            T xe[W], xo[W], be[W], bo[W];
+            ldv(x + rb + K0, xe);
+            ldv(x + rb + hz + K0, xo);
+            ldv(b + rb + K0, be);
+            ldv(b + rb + hz + K0, bo);
+            // neighbour rows: [0] E, [1] W, [2] N, [3] S; even and odd halves
+            T ne[4][W], no[4][W];
+            auto nbr = [&](int slot, bool inside, long off, int bc, int axis) {
+                if (inside || bc == BC_PER) {
+                    ldv(x + rb + off + K0, ne[slot]);
+                    ldv(x + rb + off + hz + K0, no[slot]);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < W; ++e) {
+                        T wve = T(0), wvo = T(0);
+                        if (bc == BC_DIRV) {
+                            const int ke = 2 * (K0 + e), ko = ke + 1;
+                            if (axis == 0) {
+                                wve = DIM == 3 ? F.wall[0][(j + 1) * (nz + 2) + ke + 1] : F.wall[0][ke + 1];
+                                wvo = DIM == 3 ? F.wall[0][(j + 1) * (nz + 2) + ko + 1] : F.wall[0][ko + 1];
+                            } else {
+                                wve = F.wall[1][(i + 1) * (nz + 2) + ke + 1];
+                                wvo = F.wall[1][(i + 1) * (nz + 2) + ko + 1];
+                            }
+                        }
+                        ne[slot][e] = bc_value(bc, xe[e], T(0), wve);
+                        no[slot][e] = bc_value(bc, xo[e], T(0), wvo);
+                    }
+                }
+            };
+            nbr(0, i + 1 < nx, i + 1 < nx ? F.sx : -(long)(nx - 1) * F.sx, F.bc_hi[0], 0);
+            nbr(1, i > 0, i > 0 ? -F.sx : (long)(nx - 1) * F.sx, F.bc_lo[0], 0);
+            if (DIM == 3) {
+                nbr(2, j + 1 < ny, j + 1 < ny ? F.sy : -(long)(ny - 1) * F.sy, F.bc_hi[1], 1);
+                nbr(3, j > 0, j > 0 ? -F.sy : (long)(ny - 1) * F.sy, F.bc_lo[1], 1);
+            }
+            // z: even cell 2K has zB = odd[K-1], zT = odd[K]; odd cell has zB = even[K], zT = even[K+1]
+            const T from_prev = __shfl_up(xo[W - 1], 1, 64);
+            const T from_next = __shfl_down(xe[0], 1, 64);
+            T zlo0, zhi_last;
+            if (K0 == 0) {
+                if (F.bc_lo[2] == BC_PER) zlo0 = x[rb + hz + (half - 1)];
+                else {
+                    const T wv = F.bc_lo[2] == BC_DIRV ? (DIM == 3 ? F.wall[2][(i + 1) * (ny + 2) + j + 1] : F.wall[2][i + 1])
+                                                       : T(0);
+                    zlo0 = bc_value(F.bc_lo[2], xe[0], T(0), wv);
+                }
+            } else {
+                zlo0 = lane > 0 ? from_prev : x[rb + hz + K0 - 1];
+            }
+            if (K0 + W == half) {
+                if (F.bc_hi[2] == BC_PER) zhi_last = x[rb];
+                else {
+                    const T wv = F.bc_hi[2] == BC_DIRV ? (DIM == 3 ? F.wall[2][(i + 1) * (ny + 2) + j + 1] : F.wall[2][i + 1])
+                                                       : T(0);
+                    zhi_last = bc_value(F.bc_hi[2], xo[W - 1], T(0), wv);
+                }
+            } else {
+                zhi_last = lane < 63 ? from_next : x[rb + K0 + W];
+            }
+            const T c2x = F.c2[0][i], c1x = F.c1[0][i];
+            const T c2y = DIM == 3 ? F.c2[1][j] : T(0), c1y = DIM == 3 ? F.c1[1][j] : T(0);
+#pragma unroll
+            for (int e = 0; e < W; ++e) {
+                Stencil<T, DIM> s;
+                s.c = xe[e];
+                s.xE = ne[0][e];
+                s.xW = ne[1][e];
+                if (DIM == 3) {
+                    s.yN = ne[2][e];
+                    s.yS = ne[3][e];
+                }
+                s.zB = e > 0 ? xo[e - 1] : zlo0;
+                s.zT = xo[e];
+                res[di][dj][0][e] = residual<T, DIM>(s, c2x, c1x, c2y, c1y, cz2e[e], cz1e[e], be[e]);
+                s.c = xo[e];
+                s.xE = no[0][e];
+                s.xW = no[1][e];
+                if (DIM == 3) {
+                    s.yN = no[2][e];
+                    s.yS = no[3][e];
+                }
+                s.zB = xe[e];
+                s.zT = e < W - 1 ? xe[e + 1] : zhi_last;
+                res[di][dj][1][e] = residual<T, DIM>(s, c2x, c1x, c2y, c1y, cz2o[e], cz1o[e], bo[e]);
+            }
+        }
+    }
+    T out[W], zero[W];
+#pragma unroll
+    for (int e = 0; e < W; ++e) {
+        zero[e] = T(0);
+        if (DIM == 3) {
+            const T lx = R.lo[0][I], hx = R.hi[0][I], ly = R.lo[1][J], hy = R.hi[1][J];
+            const T lz = R.lo[2][K0 + e], hz2 = R.hi[2][K0 + e];
+            const T v000 = lx * ly * lz, v100 = hx * ly * lz, v010 = lx * hy * lz, v001 = lx * ly * hz2;
+            const T v011 = lx * hy * hz2, v101 = hx * ly * hz2, v110 = hx * hy * lz, v111 = hx * hy * hz2;
+            const T vtot = R.wd[0][I] * R.wd[1][J] * R.wd[2][K0 + e];
+            out[e] = (v111 * res[1][DIM == 3 ? 1 : 0][1][e] + v000 * res[0][0][0][e] +
+                      v110 * res[1][DIM == 3 ? 1 : 0][0][e] + v001 * res[0][0][1][e] + v101 * res[1][0][1][e] +
+                      v010 * res[0][DIM == 3 ? 1 : 0][0][e] + v011 * res[0][DIM == 3 ? 1 : 0][1][e] +
+                      v100 * res[1][0][0][e]) /
+                     vtot;
+        } else {
+            out[e] = (res[1][0][1][e] + res[0][0][0][e] + res[1][0][0][e] + res[0][0][1][e]) / T(4);
+        }
+    }
+    const long cb = (long)I * C.sx + (long)J * C.sy;
+    coarse_store(bc, cb, C.hz, K0, out);
+    coarse_store(xc, cb, C.hz, K0, zero);
+}
+
+// ---------------------------------------------------------------------------
+// prolongation by injection plus the saved iterate on a split level
+// (poisson_d3.cc:548-558, poisson.cc:266): both halves of W fine positions
+// m0.. of row (i, j) take the correction of coarse cells K = m0.. of coarse
+// row (i/2, j/2).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(kSplitThreads) k_prolong_split(int nx, int ny, int nz, long sx, long sy, long hz,
+                                                                 T *__restrict__ x, const T *__restrict__ e,
+                                                                 long csx, long csy, long chz, int nblocks) {
+    using VT = Vec16<T>;
+    using V = typename VT::V;
+    constexpr int W = VT::W;
+    const int per = gridDim.x / kXcds;
+    const int logical = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+    if (logical >= nblocks) return;
+    const int half = nz >> 1, q = half / W;
+    const long total = (long)nx * ny * q;
+    const long t = (long)logical * kSplitThreads + threadIdx.x;
+    if (t >= total) return;
+    const int row = (int)(t / q);
+    const int v = (int)(t - (long)row * q);
+    const int i = row / ny, j = row - (row / ny) * ny;
+    const int m0 = v * W;
+    T ev[W];
+    coarse_vec(e, (long)(i >> 1) * csx + (long)(j >> 1) * csy, chz, m0, ev);
+    const long rb = (long)i * sx + (long)j * sy;
+#pragma unroll
+    for (int hh = 0; hh < 2; ++hh) {
+        V *p = reinterpret_cast<V *>(x + rb + hh * hz + m0);
+        const V xv = *p;
+        T r[W];
+#pragma unroll
+        for (int q2 = 0; q2 < W; ++q2) r[q2] = ev[q2] + VT::get(xv, q2);
+        *p = VT::make(r);
+    }
+}
+
 }  // namespace saras

@@ -385,15 +385,47 @@ struct CoarseCtl {
     int *iters_out;    // iterations (mode 0)
 };
 
+// Phases of the ghost-array loops below run either on the whole workgroup
+// (separated by barriers) or, for levels of a few hundred cells, on wave 0
+// alone: LDS operations of one wavefront execute in issue order, so a
+// compiler-level fence is all that separates its phases.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+template <bool WAVE>
+__device__ __forceinline__ void lsync() {
+    if (WAVE) wave_lds_sync();
+    else __syncthreads();
+}
+template <bool WAVE>
+__device__ __forceinline__ int lthreads() {
+    return WAVE ? 64 : (int)blockDim.x;
+}
+
+// max over the workgroup (WAVE: over wave 0) of one value per thread, the
+// same in every participating thread; `red` holds one slot per wave
+template <bool WAVE>
+__device__ __forceinline__ double group_max(double v, double *red) {
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    if (WAVE) return v;
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double m = red[0];
+    for (int q = 1; q < (int)(blockDim.x >> 6); ++q) m = fmax(m, red[q]);
+    return m;
+}
+
 template <typename T, int DIM>
 __device__ __forceinline__ long cidx(int i, int j, int k, int PY, int PZ) {
     return DIM == 3 ? ((long)(i + 1) * PY + (j + 1)) * PZ + (k + 1) : (long)(i + 1) * PZ + (k + 1);
 }
 
-template <typename T, int DIM>
+template <typename T, int DIM, bool WAVE = false>
 __device__ void coarse_wrap(T *X, const int *n, const int *per, int PY, int PZ) {
     const int nx = n[0], ny = n[1], nz = n[2];
-    const int tid = threadIdx.x, nt = blockDim.x;
+    const int tid = threadIdx.x, nt = lthreads<WAVE>();
     const int jlo = DIM == 3 ? -1 : 0, jn = DIM == 3 ? ny + 2 : 1;
     if (per[0]) {
         for (int q = tid; q < jn * (nz + 2); q += nt) {
@@ -401,7 +433,7 @@ __device__ void coarse_wrap(T *X, const int *n, const int *per, int PY, int PZ) 
             X[cidx<T, DIM>(nx, j, k, PY, PZ)] = X[cidx<T, DIM>(0, j, k, PY, PZ)];
             X[cidx<T, DIM>(-1, j, k, PY, PZ)] = X[cidx<T, DIM>(nx - 1, j, k, PY, PZ)];
         }
-        __syncthreads();
+        lsync<WAVE>();
     }
     if (DIM == 3 && per[1]) {
         for (int q = tid; q < (nx + 2) * (nz + 2); q += nt) {
@@ -409,7 +441,7 @@ __device__ void coarse_wrap(T *X, const int *n, const int *per, int PY, int PZ) 
             X[cidx<T, DIM>(i, ny, k, PY, PZ)] = X[cidx<T, DIM>(i, 0, k, PY, PZ)];
             X[cidx<T, DIM>(i, -1, k, PY, PZ)] = X[cidx<T, DIM>(i, ny - 1, k, PY, PZ)];
         }
-        __syncthreads();
+        lsync<WAVE>();
     }
     if (per[2]) {
         for (int q = tid; q < (nx + 2) * jn; q += nt) {
@@ -417,7 +449,7 @@ __device__ void coarse_wrap(T *X, const int *n, const int *per, int PY, int PZ) 
             X[cidx<T, DIM>(i, j, nz, PY, PZ)] = X[cidx<T, DIM>(i, j, 0, PY, PZ)];
             X[cidx<T, DIM>(i, j, -1, PY, PZ)] = X[cidx<T, DIM>(i, j, nz - 1, PY, PZ)];
         }
-        __syncthreads();
+        lsync<WAVE>();
     }
 }
 
@@ -426,10 +458,10 @@ __device__ __forceinline__ T wall_rule(int bc, T inner, T wv) {
     return bc == BC_NEU ? inner : (bc == BC_DIR ? -inner : T(2.0) * wv - inner);
 }
 
-template <typename T, int DIM>
+template <typename T, int DIM, bool WAVE = false>
 __device__ void coarse_walls(T *X, const LevelArgs<T> &L, int PY, int PZ) {
     const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
-    const int tid = threadIdx.x, nt = blockDim.x;
+    const int tid = threadIdx.x, nt = lthreads<WAVE>();
     const int jlo = DIM == 3 ? -1 : 0, jn = DIM == 3 ? ny + 2 : 1;
     if (L.bc_lo[0] != BC_PER) {
         for (int q = tid; q < jn * (nz + 2); q += nt) {
@@ -439,7 +471,7 @@ __device__ void coarse_walls(T *X, const LevelArgs<T> &L, int PY, int PZ) {
             X[cidx<T, DIM>(-1, j, k, PY, PZ)] = wall_rule(L.bc_lo[0], X[cidx<T, DIM>(0, j, k, PY, PZ)], wv);
             X[cidx<T, DIM>(nx, j, k, PY, PZ)] = wall_rule(L.bc_hi[0], X[cidx<T, DIM>(nx - 1, j, k, PY, PZ)], wv);
         }
-        __syncthreads();
+        lsync<WAVE>();
     }
     if (DIM == 3 && L.bc_lo[1] != BC_PER) {
         for (int q = tid; q < (nx + 2) * (nz + 2); q += nt) {
@@ -448,7 +480,7 @@ __device__ void coarse_walls(T *X, const LevelArgs<T> &L, int PY, int PZ) {
             X[cidx<T, DIM>(i, -1, k, PY, PZ)] = wall_rule(L.bc_lo[1], X[cidx<T, DIM>(i, 0, k, PY, PZ)], wv);
             X[cidx<T, DIM>(i, ny, k, PY, PZ)] = wall_rule(L.bc_hi[1], X[cidx<T, DIM>(i, ny - 1, k, PY, PZ)], wv);
         }
-        __syncthreads();
+        lsync<WAVE>();
     }
     if (L.bc_lo[2] != BC_PER) {
         for (int q = tid; q < (nx + 2) * jn; q += nt) {
@@ -458,15 +490,15 @@ __device__ void coarse_walls(T *X, const LevelArgs<T> &L, int PY, int PZ) {
             X[cidx<T, DIM>(i, j, -1, PY, PZ)] = wall_rule(L.bc_lo[2], X[cidx<T, DIM>(i, j, 0, PY, PZ)], wv);
             X[cidx<T, DIM>(i, j, nz, PY, PZ)] = wall_rule(L.bc_hi[2], X[cidx<T, DIM>(i, j, nz - 1, PY, PZ)], wv);
         }
-        __syncthreads();
+        lsync<WAVE>();
     }
 }
 
-template <typename T, int DIM>
+template <typename T, int DIM, bool WAVE = false>
 __device__ void coarse_colour(T *X, const T *B, const LevelArgs<T> &L, int colour, T w, int PY, int PZ) {
     const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
     const long total = (long)nx * ny * nz;
-    for (long t = threadIdx.x; t < total; t += blockDim.x) {
+    for (long t = threadIdx.x; t < total; t += lthreads<WAVE>()) {
         const int k = (int)(t % nz);
         const long r = t / nz;
         const int j = (int)(r % ny);
@@ -486,7 +518,7 @@ __device__ void coarse_colour(T *X, const T *B, const LevelArgs<T> &L, int colou
         const T c2y = DIM == 3 ? L.c2[1][j] : T(0), c1y = DIM == 3 ? L.c1[1][j] : T(0);
         X[c] = sor_update<T, DIM>(s, L.c2[0][i], L.c1[0][i], c2y, c1y, L.c2[2][k], L.c1[2][k], B[t], w);
     }
-    __syncthreads();
+    lsync<WAVE>();
 }
 
 template <typename T, int DIM>
@@ -499,8 +531,7 @@ __global__ void __launch_bounds__(1024) k_coarse(LevelArgs<T> L, T *__restrict__
     const long total = (long)nx * ny * nz;
     T *X = use_lds ? reinterpret_cast<T *>(lds_raw) : scratch;
     T *B = use_lds ? X + padded : scratch + padded;
-    __shared__ double red[1024];
-    __shared__ int done;
+    __shared__ double red[16];
     int per[3] = {L.bc_lo[0] == BC_PER, L.bc_lo[1] == BC_PER, L.bc_lo[2] == BC_PER};
     if (DIM == 2) per[1] = 0;
 
@@ -552,24 +583,10 @@ __global__ void __launch_bounds__(1024) k_coarse(LevelArgs<T> L, T *__restrict__
             const double a = fabs((double)rv);
             if (a > lm) lm = a;
         }
-        red[threadIdx.x] = lm;
-        __syncthreads();
-        for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-            if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) {
-            int stop = 0;
-            if (red[0] < ctl.tol) stop = 1;
-            else {
-                iter += 1;
-                if (iter > ctl.max_count) stop = 1;
-            }
-            done = stop;
-        }
-        __syncthreads();
-        if (done) break;
-        __syncthreads();
+        const double mx = group_max<false>(lm, red);
+        if (mx < ctl.tol) break;
+        iter += 1;
+        if (iter > ctl.max_count) break;
     }
     // final imposeBC (:431 / :245); the ghosts themselves are not kept
     for (long t = threadIdx.x; t < total; t += blockDim.x) {

@@ -44,6 +44,11 @@ void set_error(const std::string &s) { g_last_error = s; }
     } while (0)
 
 constexpr int kThreads = 256;
+// default dynamic LDS budget of the single-workgroup tail: levels of a few
+// thousand cells run faster as multi-workgroup passes than on one CU, so the
+// tail takes only the levels that fit here (SARAS_TAIL_LDS_KB overrides it,
+// up to 60 KB: the tail's static LDS is under 2 KB)
+constexpr size_t kTailLdsDefault = 46 * 1024;
 
 inline unsigned blocks_for(long n) { return (unsigned)std::max<long>(1, (n + kThreads - 1) / kThreads); }
 
@@ -218,6 +223,7 @@ struct saras_mg {
     int march_wgs = 0;                 // target workgroups of a march sweep (SARAS_MARCH_WGS; 0 = per type)
     bool march_tj8 = false;            // FP32 march with 64 x 8 tiles (SARAS_MARCH_TJ8=1)
     bool split_off = false;            // SARAS_NO_SPLIT=1 keeps every FP32 level z-contiguous
+    size_t tail_lds = kTailLdsDefault;  // dynamic LDS budget of the tail kernel
     // V-cycles replayed from captured hipGraphs (saras_mg_set_graphs).  A
     // graph bakes in the level buffers, and out-of-place sweeps swap them, so
     // graphs are cached per buffer state (at most two states per precision).
@@ -320,6 +326,19 @@ struct Ops {
         return L.n[1] % 16 == 0 && L.n[2] % 64 == 0 && (long)L.n[0] * L.n[1] * L.n[2] >= (8L << 20);
     }
 
+    template <int DIM>
+    void split_pass(bool fine, bool pro, int grid, const saras::LevelArgs<T> &a, DevLevel<T> &L, int colour, int nb,
+                    const T *e, long esx, long esy, long ehz) {
+        const dim3 g(grid), bl(saras::kSplitThreads);
+        if (fine) {
+            if (pro) hipLaunchKernelGGL((saras::k_rb_split<T, DIM, true, true>), g, bl, 0, s, a, L.x, L.b, colour, w, nb, e, esx, esy, ehz);
+            else hipLaunchKernelGGL((saras::k_rb_split<T, DIM, true, false>), g, bl, 0, s, a, L.x, L.b, colour, w, nb, e, esx, esy, ehz);
+        } else {
+            if (pro) hipLaunchKernelGGL((saras::k_rb_split<T, DIM, false, true>), g, bl, 0, s, a, L.x, L.b, colour, w, nb, e, esx, esy, ehz);
+            else hipLaunchKernelGGL((saras::k_rb_split<T, DIM, false, false>), g, bl, 0, s, a, L.x, L.b, colour, w, nb, e, esx, esy, ehz);
+        }
+    }
+
     void rb_sweep(int l, bool zero_bc) {
         DevLevel<T> &L = H.lv[l];
         auto a = level_args(m, H, l, inhom(l, zero_bc));
@@ -328,18 +347,21 @@ struct Ops {
             const long total = (long)L.n[0] * L.n[1] * (L.n[2] / 2 / W);
             const int nb = (int)((total + saras::kSplitThreads - 1) / saras::kSplitThreads);
             const int grid = (nb + saras::kXcds - 1) / saras::kXcds * saras::kXcds;
+            // the first sweep after the prolongation applies it (see k_rb_split)
+            const bool pro = pend_level == l;
+            const T *e = nullptr;
+            long esx = 0, esy = 0, ehz = 0;
+            if (pro) {
+                const DevLevel<T> &C = H.lv[l + 1];
+                e = C.x;
+                esx = C.sx;
+                esy = C.sy;
+                ehz = C.hz;
+                pend_level = -1;
+            }
             for (int colour = 0; colour < 2; ++colour) {
-                if (H.dim == 3) {
-                    if (l == 0)
-                        hipLaunchKernelGGL((saras::k_rb_split<T, 3, true>), dim3(grid), dim3(saras::kSplitThreads), 0, s, a, L.x, L.b, colour, w, nb);
-                    else
-                        hipLaunchKernelGGL((saras::k_rb_split<T, 3, false>), dim3(grid), dim3(saras::kSplitThreads), 0, s, a, L.x, L.b, colour, w, nb);
-                } else {
-                    if (l == 0)
-                        hipLaunchKernelGGL((saras::k_rb_split<T, 2, true>), dim3(grid), dim3(saras::kSplitThreads), 0, s, a, L.x, L.b, colour, w, nb);
-                    else
-                        hipLaunchKernelGGL((saras::k_rb_split<T, 2, false>), dim3(grid), dim3(saras::kSplitThreads), 0, s, a, L.x, L.b, colour, w, nb);
-                }
+                if (H.dim == 3) split_pass<3>(l == 0, pro, grid, a, L, colour, nb, e, esx, esy, ehz);
+                else split_pass<2>(l == 0, pro, grid, a, L, colour, nb, e, esx, esy, ehz);
             }
             return;
         }
@@ -444,7 +466,7 @@ struct Ops {
     }
 
     // first level of the single-workgroup tail: the smallest S whose levels
-    // S..D fit in 48 KB of LDS
+    // S..D fit in kTailLds bytes of LDS
     // (D + 1, i.e. no tail, when even the coarsest level does not fit)
     int tail_start() const {
         const int D = (int)H.lv.size() - 1;
@@ -457,7 +479,7 @@ struct Ops {
                                           : (size_t)(L.n[0] + 2) * (L.n[2] + 2);
             const size_t coef = 2 * ((size_t)L.n[0] + L.n[1] + L.n[2]);
             bytes += ((pad + 3) / 4 * 4 + (core + 3) / 4 * 4 + coef) * sizeof(T);
-            if (bytes > 46 * 1024 || D - l + 1 > saras::kTailMaxLevels) break;
+            if (bytes > m->tail_lds || D - l + 1 > saras::kTailMaxLevels) break;
             S = l;
         }
         return S;
@@ -500,7 +522,7 @@ struct Ops {
         A.tol = m->p.mg_tol;
         A.zero_bc_first = S != 0;
         A.iters_out = m->iters_dev;
-        if (off * sizeof(T) > 48 * 1024) std::abort();  // host-side guard: tail_start() keeps levels within 46 KB
+        if (off * sizeof(T) > m->tail_lds) std::abort();  // host-side guard: tail_start() keeps levels within tail_lds
         if (H.dim == 3)
             hipLaunchKernelGGL((saras::k_tail<T, 3>), dim3(1), dim3(512), off * sizeof(T), s, A);
         else
@@ -528,7 +550,7 @@ struct Ops {
         }
         if (S <= D) tail(S);
         for (int l = std::min(S, D) - 1; l >= 0; --l) {
-            if (m->p.post_smooth > 0 && march_ok(l)) pend_level = l;
+            if (m->p.post_smooth > 0 && (march_ok(l) || H.lv[l].hz)) pend_level = l;
             else prolong_correct(l);
             smooth(l, m->p.post_smooth, l != 0);
         }
@@ -730,6 +752,7 @@ std::vector<std::vector<double>> make_walls(const saras_grid_desc &g, const sara
 
 hipStream_t pick_stream(saras_mg *m, void *stream) { return stream ? (hipStream_t)stream : m->stream; }
 
+
 }  // namespace
 
 // ===========================================================================
@@ -792,6 +815,8 @@ saras_status saras_mg_create(const saras_grid_desc *grid, const saras_mg_params 
         m->march_tj8 = env4 && env4[0] == '1';
         const char *env5 = std::getenv("SARAS_NO_SPLIT");
         m->split_off = env5 && env5[0] == '1';
+        const char *env6 = std::getenv("SARAS_TAIL_LDS_KB");
+        if (env6 && std::atoi(env6) > 0) m->tail_lds = (size_t)std::min(60, std::atoi(env6)) * 1024;
     }
     if (m->p.inner_cycles <= 0) m->p.inner_cycles = 3;
     if (m->p.max_refine <= 0) m->p.max_refine = 20;

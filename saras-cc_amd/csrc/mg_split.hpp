@@ -38,11 +38,63 @@ struct Vec16<double> {
 
 constexpr int kSplitThreads = 256;
 
-// one colour pass over a split level; the grid is a multiple of kXcds
-// blocks, `nblocks` of them carry work
-template <typename T, int DIM, bool FINE>
+// W consecutive coarse cells K0..K0+W-1 of coarse row (I, J) in the coarse
+// level's own layout (split or z-contiguous), as one vector
+template <typename T>
+__device__ __forceinline__ void coarse_vec(const T *__restrict__ e, long base, long chz, int K0, T (&out)[Vec16<T>::W]) {
+    using VT = Vec16<T>;
+    constexpr int W = VT::W;
+    if (chz == 0) {
+        const typename VT::V v = *reinterpret_cast<const typename VT::V *>(e + base + K0);
+#pragma unroll
+        for (int q = 0; q < W; ++q) out[q] = VT::get(v, q);
+    } else {
+        if constexpr (W == 4) {
+            // even K at K/2, odd K at chz + K/2 (K0 is a multiple of 4): two pairs
+            const float2 ev = *reinterpret_cast<const float2 *>(e + base + (K0 >> 1));
+            const float2 od = *reinterpret_cast<const float2 *>(e + base + chz + (K0 >> 1));
+            out[0] = ev.x;
+            out[1] = od.x;
+            out[2] = ev.y;
+            out[3] = od.y;
+        } else {
+#pragma unroll
+            for (int q = 0; q < W; ++q) out[q] = e[base + (q & 1) * chz + ((K0 + q) >> 1)];
+        }
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void coarse_store(T *__restrict__ dst, long base, long chz, int K0,
+                                             const T (&in)[Vec16<T>::W]) {
+    using VT = Vec16<T>;
+    constexpr int W = VT::W;
+    if (chz == 0) {
+        *reinterpret_cast<typename VT::V *>(dst + base + K0) = VT::make(in);
+    } else {
+#pragma unroll
+        for (int q = 0; q < W; ++q) dst[base + (q & 1) * chz + ((K0 + q) >> 1)] = in[q];
+    }
+}
+
+
+// one coarse cell (I, J, K) of a level in its own layout
+template <typename T>
+__device__ __forceinline__ T coarse_at(const T *__restrict__ e, long base, long chz, int K) {
+    return e[base + (chz ? (long)(K & 1) * chz + (K >> 1) : (long)K)];
+}
+
+// One colour pass over a split level; the grid is a multiple of kXcds
+// blocks, `nblocks` of them carry work.  PRO: the pass belongs to the first
+// sweep after the prolongation, whose input is x + P e with P the injection
+// from the coarse iterate e (poisson_d3.cc:548-558, poisson.cc:266): the red
+// pass reads every cell as e(coarse) + x, the black pass its own cells (its
+// neighbours are the red values just written), so the separate
+// prolongation pass is not needed.
+template <typename T, int DIM, bool FINE, bool PRO>
 __global__ void __launch_bounds__(kSplitThreads) k_rb_split(LevelArgs<T> L, T *__restrict__ x,
-                                                            const T *__restrict__ b, int colour, T w, int nblocks) {
+                                                            const T *__restrict__ b, int colour, T w, int nblocks,
+                                                            const T *__restrict__ e, long esx, long esy, long ehz) {
     using VT = Vec16<T>;
     using V = typename VT::V;
     constexpr int W = VT::W;
@@ -69,37 +121,61 @@ __global__ void __launch_bounds__(kSplitThreads) k_rb_split(LevelArgs<T> L, T *_
     const V c1v = *reinterpret_cast<const V *>(L.c1zs + h * half + m0);
     T c[W], o[W];
 #pragma unroll
-    for (int e = 0; e < W; ++e) {
-        c[e] = VT::get(cv, e);
-        o[e] = VT::get(ov, e);
+    for (int q = 0; q < W; ++q) {
+        c[q] = VT::get(cv, q);
+        o[q] = VT::get(ov, q);
+    }
+    // own coarse row of the prolongation: cells m0.. of both halves map to K = m0..
+    const long ebase = (long)(i >> 1) * esx + (long)(j >> 1) * esy;
+    if (PRO) {
+        T ev[W];
+        coarse_vec(e, ebase, ehz, m0, ev);
+#pragma unroll
+        for (int q = 0; q < W; ++q) {
+            c[q] = ev[q] + c[q];
+            if (colour == 0) o[q] = ev[q] + o[q];
+        }
     }
     // x and y neighbour rows: whole vectors, or the wall rule per cell
     auto kk = [&](int e) { return 2 * (m0 + e) + h; };
     T xe[W], xw[W], yn[W], ys[W];
-    auto side = [&](T(&out)[W], bool inside, long off, int bc, int axis) {
+    // nbr_i, nbr_j: the neighbour row's indices (wrapped), for its coarse row
+    auto side = [&](T(&out)[W], bool inside, long off, int bc, int axis, int nbr_i, int nbr_j) {
         if (inside || bc == BC_PER) {
             const V nv = *reinterpret_cast<const V *>(x + own + off);
 #pragma unroll
-            for (int e = 0; e < W; ++e) out[e] = VT::get(nv, e);
+            for (int q = 0; q < W; ++q) out[q] = VT::get(nv, q);
+            if (PRO && colour == 0) {
+                T ev[W];
+                coarse_vec(e, (long)(nbr_i >> 1) * esx + (long)(nbr_j >> 1) * esy, ehz, m0, ev);
+#pragma unroll
+                for (int q = 0; q < W; ++q) out[q] = ev[q] + out[q];
+            }
         } else {
 #pragma unroll
-            for (int e = 0; e < W; ++e) {
+            for (int q = 0; q < W; ++q) {
                 T wv = T(0);
                 if (bc == BC_DIRV) {
-                    const int k = kk(e);
+                    const int k = kk(q);
                     if (axis == 0) wv = DIM == 3 ? L.wall[0][(j + 1) * (nz + 2) + k + 1] : L.wall[0][k + 1];
                     else wv = L.wall[1][(i + 1) * (nz + 2) + k + 1];
                 }
-                out[e] = bc_value(bc, c[e], T(0), wv);
+                out[q] = bc_value(bc, c[q], T(0), wv);
             }
         }
     };
-    side(xe, i + 1 < nx, i + 1 < nx ? L.sx : -(long)(nx - 1) * L.sx, L.bc_hi[0], 0);
-    side(xw, i > 0, i > 0 ? -L.sx : (long)(nx - 1) * L.sx, L.bc_lo[0], 0);
+    side(xe, i + 1 < nx, i + 1 < nx ? L.sx : -(long)(nx - 1) * L.sx, L.bc_hi[0], 0, i + 1 < nx ? i + 1 : 0, j);
+    side(xw, i > 0, i > 0 ? -L.sx : (long)(nx - 1) * L.sx, L.bc_lo[0], 0, i > 0 ? i - 1 : nx - 1, j);
     if (DIM == 3) {
-        side(yn, j + 1 < ny, j + 1 < ny ? L.sy : -(long)(ny - 1) * L.sy, L.bc_hi[1], 1);
-        side(ys, j > 0, j > 0 ? -L.sy : (long)(ny - 1) * L.sy, L.bc_lo[1], 1);
+        side(yn, j + 1 < ny, j + 1 < ny ? L.sy : -(long)(ny - 1) * L.sy, L.bc_hi[1], 1, i, j + 1 < ny ? j + 1 : 0);
+        side(ys, j > 0, j > 0 ? -L.sy : (long)(ny - 1) * L.sy, L.bc_lo[1], 1, i, j > 0 ? j - 1 : ny - 1);
     }
+    // a z neighbour read straight from memory, with the correction when the
+    // red pass of a prolongation sweep reads it (coarse cell K)
+    auto zread = [&](long off, int K) -> T {
+        const T v = x[off];
+        return (PRO && colour == 0) ? coarse_at(e, ebase, ehz, K) + v : v;
+    };
     // z neighbours: k - 1 and k + 1 lie in the other half.  h == 0 (k even):
     // other[m - 1], other[m]; h == 1 (k odd): other[m], other[m + 1].  The
     // element beyond this lane's vector comes from the adjacent lane when it
@@ -110,80 +186,50 @@ __global__ void __launch_bounds__(kSplitThreads) k_rb_split(LevelArgs<T> L, T *_
     T zlo0 = T(0), zhi_last = T(0);
     if (h == 0) {
         if (m0 == 0) {
-            if (L.bc_lo[2] == BC_PER) zlo0 = x[rb + L.hz + (half - 1)];
+            if (L.bc_lo[2] == BC_PER) zlo0 = zread(rb + L.hz + (half - 1), half - 1);
             else {
                 const T wv = L.bc_lo[2] == BC_DIRV ? (DIM == 3 ? L.wall[2][(i + 1) * (ny + 2) + j + 1] : L.wall[2][i + 1])
                                                    : T(0);
                 zlo0 = bc_value(L.bc_lo[2], c[0], T(0), wv);
             }
         } else {
-            zlo0 = lane > 0 ? from_prev : x[oth - 1];
+            zlo0 = lane > 0 ? from_prev : zread(oth - 1, m0 - 1);
         }
     } else {
         if (m0 + W == half) {
-            if (L.bc_hi[2] == BC_PER) zhi_last = x[rb];
+            if (L.bc_hi[2] == BC_PER) zhi_last = zread(rb, 0);
             else {
                 const T wv = L.bc_hi[2] == BC_DIRV ? (DIM == 3 ? L.wall[2][(i + 1) * (ny + 2) + j + 1] : L.wall[2][i + 1])
                                                    : T(0);
                 zhi_last = bc_value(L.bc_hi[2], c[W - 1], T(0), wv);
             }
         } else {
-            zhi_last = lane < 63 ? from_next : x[oth + W];
+            zhi_last = lane < 63 ? from_next : zread(oth + W, m0 + W);
         }
     }
     const T c2x = L.c2[0][i], c1x = L.c1[0][i];
     const T c2y = DIM == 3 ? L.c2[1][j] : T(0), c1y = DIM == 3 ? L.c1[1][j] : T(0);
     T r[W];
 #pragma unroll
-    for (int e = 0; e < W; ++e) {
+    for (int q = 0; q < W; ++q) {
         Stencil<T, DIM> s;
-        s.c = c[e];
-        s.xE = xe[e];
-        s.xW = xw[e];
+        s.c = c[q];
+        s.xE = xe[q];
+        s.xW = xw[q];
         if (DIM == 3) {
-            s.yN = yn[e];
-            s.yS = ys[e];
+            s.yN = yn[q];
+            s.yS = ys[q];
         }
         if (h == 0) {
-            s.zB = e > 0 ? o[e - 1] : zlo0;
-            s.zT = o[e];
+            s.zB = q > 0 ? o[q - 1] : zlo0;
+            s.zT = o[q];
         } else {
-            s.zB = o[e];
-            s.zT = e < W - 1 ? o[e + 1] : zhi_last;
+            s.zB = o[q];
+            s.zT = q < W - 1 ? o[q + 1] : zhi_last;
         }
-        r[e] = sor_update<T, DIM>(s, c2x, c1x, c2y, c1y, VT::get(c2v, e), VT::get(c1v, e), VT::get(bv, e), w);
+        r[q] = sor_update<T, DIM>(s, c2x, c1x, c2y, c1y, VT::get(c2v, q), VT::get(c1v, q), VT::get(bv, q), w);
     }
     *reinterpret_cast<V *>(x + own) = VT::make(r);
-}
-
-// W consecutive coarse cells K0..K0+W-1 of coarse row (I, J) in the coarse
-// level's own layout (split or z-contiguous), as one vector
-template <typename T>
-__device__ __forceinline__ void coarse_vec(const T *__restrict__ e, long base, long chz, int K0, T (&out)[Vec16<T>::W]) {
-    using VT = Vec16<T>;
-    constexpr int W = VT::W;
-    if (chz == 0) {
-        const typename VT::V v = *reinterpret_cast<const typename VT::V *>(e + base + K0);
-#pragma unroll
-        for (int q = 0; q < W; ++q) out[q] = VT::get(v, q);
-    } else {
-        // even K at K/2, odd K at chz + K/2 (K0 is a multiple of W)
-#pragma unroll
-        for (int q = 0; q < W; ++q) out[q] = e[base + (q & 1) * chz + ((K0 + q) >> 1)];
-    }
-}
-
-template <typename T>
-__device__ __forceinline__ void coarse_store(T *__restrict__ dst, long base, long chz, int K0,
-                                             const T (&in)[Vec16<T>::W]) {
-    using VT = Vec16<T>;
-    constexpr int W = VT::W;
-    if (chz == 0) {
-        *reinterpret_cast<typename VT::V *>(dst + base + K0) = VT::make(in);
-    } else {
-#pragma unroll
-        for (int q = 0; q < W; ++q) dst[base + (q & 1) * chz + ((K0 + q) >> 1)] = in[q];
-    }
 }
 
 // ---------------------------------------------------------------------------

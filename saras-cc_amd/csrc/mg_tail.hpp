@@ -50,38 +50,40 @@ __device__ __forceinline__ T lds_residual(const T *X, const T *B, const LevelArg
     return residual<T, DIM>(s, L.c2[0][i], L.c1[0][i], c2y, c1y, L.c2[2][k], L.c1[2][k], B[t]);
 }
 
-template <typename T, int DIM>
+// levels of at most this many padded cells run their loops on wave 0 alone
+constexpr int kTailWaveCells = 512;
+
+template <typename T, int DIM, bool WAVE>
 __device__ void tail_smooth(T *X, const T *B, const LevelArgs<T> &L, int count, T w) {
     const int PZ = L.n[2] + 2, PY = DIM == 3 ? L.n[1] + 2 : 1;
     int per[3] = {L.bc_lo[0] == BC_PER, DIM == 3 && L.bc_lo[1] == BC_PER, L.bc_lo[2] == BC_PER};
     for (int n = 0; n < count; ++n) {
-        coarse_wrap<T, DIM>(X, L.n, per, PY, PZ);
-        coarse_walls<T, DIM>(X, L, PY, PZ);
-        coarse_colour<T, DIM>(X, B, L, 0, w, PY, PZ);
-        coarse_wrap<T, DIM>(X, L.n, per, PY, PZ);
-        coarse_colour<T, DIM>(X, B, L, 1, w, PY, PZ);
+        coarse_wrap<T, DIM, WAVE>(X, L.n, per, PY, PZ);
+        coarse_walls<T, DIM, WAVE>(X, L, PY, PZ);
+        coarse_colour<T, DIM, WAVE>(X, B, L, 0, w, PY, PZ);
+        coarse_wrap<T, DIM, WAVE>(X, L.n, per, PY, PZ);
+        coarse_colour<T, DIM, WAVE>(X, B, L, 1, w, PY, PZ);
     }
-    coarse_wrap<T, DIM>(X, L.n, per, PY, PZ);
-    coarse_walls<T, DIM>(X, L, PY, PZ);
+    coarse_wrap<T, DIM, WAVE>(X, L.n, per, PY, PZ);
+    coarse_walls<T, DIM, WAVE>(X, L, PY, PZ);
 }
 
 // returns the iteration count of the reference's coarsest solve
-template <typename T, int DIM>
-__device__ int tail_solve(T *X, const T *B, const LevelArgs<T> &L, T w, double tol, int max_count, double *red,
-                          int *flag) {
+template <typename T, int DIM, bool WAVE>
+__device__ int tail_solve(T *X, const T *B, const LevelArgs<T> &L, T w, double tol, int max_count, double *red) {
     const int PZ = L.n[2] + 2, PY = DIM == 3 ? L.n[1] + 2 : 1;
     const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
     const long total = (long)nx * ny * nz;
     int per[3] = {L.bc_lo[0] == BC_PER, DIM == 3 && L.bc_lo[1] == BC_PER, L.bc_lo[2] == BC_PER};
     int iter = 0;
     for (;;) {
-        coarse_wrap<T, DIM>(X, L.n, per, PY, PZ);
-        coarse_walls<T, DIM>(X, L, PY, PZ);
-        coarse_colour<T, DIM>(X, B, L, 0, w, PY, PZ);
-        coarse_wrap<T, DIM>(X, L.n, per, PY, PZ);
-        coarse_colour<T, DIM>(X, B, L, 1, w, PY, PZ);
+        coarse_wrap<T, DIM, WAVE>(X, L.n, per, PY, PZ);
+        coarse_walls<T, DIM, WAVE>(X, L, PY, PZ);
+        coarse_colour<T, DIM, WAVE>(X, B, L, 0, w, PY, PZ);
+        coarse_wrap<T, DIM, WAVE>(X, L.n, per, PY, PZ);
+        coarse_colour<T, DIM, WAVE>(X, B, L, 1, w, PY, PZ);
         double lm = -1.0e-10;
-        for (long t = threadIdx.x; t < total; t += blockDim.x) {
+        for (long t = threadIdx.x; t < total; t += lthreads<WAVE>()) {
             const int k = (int)(t % nz);
             const long r = t / nz;
             const int j = (int)(r % ny);
@@ -89,28 +91,13 @@ __device__ int tail_solve(T *X, const T *B, const LevelArgs<T> &L, T w, double t
             const double a = fabs((double)lds_residual<T, DIM>(X, B, L, i, j, k, PY, PZ));
             if (a > lm) lm = a;
         }
-        red[threadIdx.x] = lm;
-        __syncthreads();
-        for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-            if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) {
-            int stop = 0;
-            if (red[0] < tol) stop = 1;
-            else {
-                iter += 1;
-                if (iter > max_count) stop = 1;
-            }
-            *flag = stop;
-        }
-        __syncthreads();
-        const int stop = *flag;
-        __syncthreads();
-        if (stop) break;
+        const double mx = group_max<WAVE>(lm, red);
+        if (mx < tol) break;
+        iter += 1;
+        if (iter > max_count) break;
     }
-    coarse_wrap<T, DIM>(X, L.n, per, PY, PZ);
-    coarse_walls<T, DIM>(X, L, PY, PZ);
+    coarse_wrap<T, DIM, WAVE>(X, L.n, per, PY, PZ);
+    coarse_walls<T, DIM, WAVE>(X, L, PY, PZ);
     return iter;
 }
 
@@ -118,8 +105,7 @@ template <typename T, int DIM>
 __global__ void __launch_bounds__(1024) k_tail(TailArgs<T> A) {
     extern __shared__ __align__(16) unsigned char tail_raw[];
     T *lds = reinterpret_cast<T *>(tail_raw);
-    __shared__ double red[1024];
-    __shared__ int flag;
+    __shared__ double red[16];
     __shared__ LevelArgs<T> SL[kTailMaxLevels];
     const int S = A.first, D = A.last;
     // the metric coefficients of every tail level are read in every phase:
@@ -174,11 +160,30 @@ __global__ void __launch_bounds__(1024) k_tail(TailArgs<T> A) {
     // level S: the coarsest treatment when S == D > 0 (the host restricted into
     // it); otherwise pre-smoothing, which is also the whole V-cycle of a
     // single-level hierarchy (poisson.cc:228, no coarse solve)
+    // one level's smoothing or coarsest solve, on wave 0 alone when it is small
+    auto smooth_lv = [&](int l, T *XL, const T *BL, int count) {
+        if (padded(l) <= kTailWaveCells) {
+            if (threadIdx.x < 64) tail_smooth<T, DIM, true>(XL, BL, SL[l - S], count, A.w);
+            __syncthreads();
+        } else {
+            tail_smooth<T, DIM, false>(XL, BL, SL[l - S], count, A.w);
+        }
+    };
+    auto solve_lv = [&](int l, T *XL, const T *BL) {
+        int it = 0;
+        if (padded(l) <= kTailWaveCells) {
+            if (threadIdx.x < 64) it = tail_solve<T, DIM, true>(XL, BL, SL[l - S], A.w, A.tol, A.max_count, red);
+            __syncthreads();
+        } else {
+            it = tail_solve<T, DIM, false>(XL, BL, SL[l - S], A.w, A.tol, A.max_count, red);
+        }
+        return it;
+    };
     if (S == D && S > 0) {
-        if (A.solve_flag) iters = tail_solve<T, DIM>(X(S), B(S), SL[0], A.w, A.tol, A.max_count, red, &flag);
-        else tail_smooth<T, DIM>(X(S), B(S), SL[0], A.pre, A.w);
+        if (A.solve_flag) iters = solve_lv(S, X(S), B(S));
+        else smooth_lv(S, X(S), B(S), A.pre);
     } else {
-        tail_smooth<T, DIM>(X(S), B(S), SL[0], A.pre, A.w);
+        smooth_lv(S, X(S), B(S), A.pre);
     }
     // down-leg
     for (int l = S; l < D; ++l) {
@@ -216,10 +221,10 @@ __global__ void __launch_bounds__(1024) k_tail(TailArgs<T> A) {
         }
         __syncthreads();
         if (l + 1 == D) {
-            if (A.solve_flag) iters = tail_solve<T, DIM>(XC, B(D), C, A.w, A.tol, A.max_count, red, &flag);
-            else tail_smooth<T, DIM>(XC, B(D), C, A.pre, A.w);
+            if (A.solve_flag) iters = solve_lv(D, XC, B(D));
+            else smooth_lv(D, XC, B(D), A.pre);
         } else {
-            tail_smooth<T, DIM>(XC, B(l + 1), C, A.pre, A.w);
+            smooth_lv(l + 1, XC, B(l + 1), A.pre);
         }
     }
     // up-leg: injection plus the saved iterate, then post-smoothing
@@ -240,7 +245,7 @@ __global__ void __launch_bounds__(1024) k_tail(TailArgs<T> A) {
             XF[c] = XC[cidx<T, DIM>(i >> 1, j >> 1, k >> 1, cPY, cPZ)] + XF[c];
         }
         __syncthreads();
-        tail_smooth<T, DIM>(XF, B(l), F, A.post, A.w);
+        smooth_lv(l, XF, B(l), A.post);
     }
     // level S back to global memory
     {

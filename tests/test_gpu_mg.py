@@ -365,3 +365,4 @@ def test_fp32_split_rows_match_z_contiguous(case, variant, monkeypatch):
     assert ss["cycles"] == so["cycles"]
     assert ss["residual"] == so["residual"]
     np.testing.assert_array_equal(xs, xo)
+

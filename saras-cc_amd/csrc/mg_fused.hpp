@@ -334,9 +334,9 @@ __global__ void __launch_bounds__(kNormThreads) k_refine(LevelArgs<double> L, co
     for (int k = wk.k0; k < nz; k += wk.kspan) {
         const int i = (int)(row / ny);
         const int j = (int)(row - (long)i * ny);
-        const long idx = (long)i * L.sx + (long)j * L.sy + k;
+        const long idx = (long)i * L.sx + (long)j * L.sy + kof(L, k);
         auto xv = [&](int ii, int jj, int kk) -> double {
-            const double v = xold[(long)ii * L.sx + (long)jj * L.sy + kk];
+            const double v = xold[(long)ii * L.sx + (long)jj * L.sy + kof(L, kk)];
             return e ? v + (double)e[(long)ii * esx + (long)jj * esy + (ehz ? (long)(kk & 1) * ehz + (kk >> 1) : (long)kk)]
                      : v;
         };

@@ -11,9 +11,10 @@
 namespace saras {
 
 // Level-0 iterate and RHS of the FP64 hierarchy (core-only, row pitch sy,
-// plane pitch sx).  The iterate pointer changes across solves (out-of-place
-// sweeps swap buffers): fetch it again after every solve.
-void mg_level0(saras_mg *m, double **x, double **b, long *sx, long *sy);
+// plane pitch sx; rows colour-split when hz > 0: even k at k/2, odd k at
+// hz + k/2).  The iterate pointer changes across solves (out-of-place sweeps
+// swap buffers): fetch it again after every solve.
+void mg_level0(saras_mg *m, double **x, double **b, long *sx, long *sy, long *hz = nullptr);
 
 // poisson::mgSolve on the resident level-0 arrays: the iterate is the
 // initial guess, the RHS must be in place.  Runs on stream s.

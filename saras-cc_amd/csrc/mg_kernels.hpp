@@ -604,7 +604,7 @@ __global__ void __launch_bounds__(1024) k_coarse(LevelArgs<T> L, T *__restrict__
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(256) k_load_core(const double *__restrict__ full, int pad, int dim, int nx,
-                                                   int ny, int nz, long sx, long sy, T *__restrict__ core) {
+                                                   int ny, int nz, long sx, long sy, long hz, T *__restrict__ core) {
     const long total = (long)nx * ny * nz;
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= total) return;
@@ -614,11 +614,12 @@ __global__ void __launch_bounds__(256) k_load_core(const double *__restrict__ fu
     const int i = (int)(r / ny);
     const long fz = nz + 2 * pad, fy = ny + 2 * pad;
     const long f = dim == 3 ? ((long)(i + pad) * fy + (j + pad)) * fz + (k + pad) : (long)(i + pad) * fz + (k + pad);
-    core[(long)i * sx + (long)j * sy + k] = (T)full[f];
+    core[(long)i * sx + (long)j * sy + (hz ? (long)(k & 1) * hz + (k >> 1) : (long)k)] = (T)full[f];
 }
 
 __global__ void __launch_bounds__(256) k_store_core(double *__restrict__ full, int pad, int dim, int nx, int ny,
-                                                    int nz, long sx, long sy, const double *__restrict__ core) {
+                                                    int nz, long sx, long sy, long hz,
+                                                    const double *__restrict__ core) {
     const long total = (long)nx * ny * nz;
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= total) return;
@@ -628,7 +629,7 @@ __global__ void __launch_bounds__(256) k_store_core(double *__restrict__ full, i
     const int i = (int)(r / ny);
     const long fz = nz + 2 * pad, fy = ny + 2 * pad;
     const long f = dim == 3 ? ((long)(i + pad) * fy + (j + pad)) * fz + (k + pad) : (long)(i + pad) * fz + (k + pad);
-    full[f] = core[(long)i * sx + (long)j * sy + k];
+    full[f] = core[(long)i * sx + (long)j * sy + (hz ? (long)(k & 1) * hz + (k >> 1) : (long)k)];
 }
 
 // One phase of the final imposeBC on the caller's array, over the [-1, n]

@@ -222,7 +222,7 @@ struct saras_mg {
     bool tail_off = false;             // SARAS_NO_TAIL=1 runs every level with multi-block kernels
     int march_wgs = 0;                 // target workgroups of a march sweep (SARAS_MARCH_WGS; 0 = per type)
     bool march_tj8 = false;            // FP32 march with 64 x 8 tiles (SARAS_MARCH_TJ8=1)
-    bool split_off = false;            // SARAS_NO_SPLIT=1 keeps every FP32 level z-contiguous
+    bool split_off = false;            // SARAS_NO_SPLIT=1 keeps every level z-contiguous
     size_t tail_lds = kTailLdsDefault;  // dynamic LDS budget of the tail kernel
     // V-cycles replayed from captured hipGraphs (saras_mg_set_graphs).  A
     // graph bakes in the level buffers, and out-of-place sweeps swap them, so
@@ -669,7 +669,7 @@ template <typename T>
 void load_core(saras_mg *m, const double *full, const DevLevel<T> &L, T *dst, hipStream_t s) {
     const long n = (long)L.n[0] * L.n[1] * L.n[2];
     hipLaunchKernelGGL((saras::k_load_core<T>), dim3(blocks_for(n)), dim3(kThreads), 0, s, full, m->g.pad, m->g.dim,
-                       L.n[0], L.n[1], L.n[2], L.sx, L.sy, dst);
+                       L.n[0], L.n[1], L.n[2], L.sx, L.sy, L.hz, dst);
 }
 
 // core + the [-1, n] ghost shell, phased as the reference's final imposeBC
@@ -677,7 +677,7 @@ void store_full(saras_mg *m, double *full, const double *core, hipStream_t s) {
     const auto &L = m->h64.lv[0];
     const long n = (long)L.n[0] * L.n[1] * L.n[2];
     hipLaunchKernelGGL(saras::k_store_core, dim3(blocks_for(n)), dim3(kThreads), 0, s, full, m->g.pad, m->g.dim,
-                       L.n[0], L.n[1], L.n[2], L.sx, L.sy, core);
+                       L.n[0], L.n[1], L.n[2], L.sx, L.sy, L.hz, core);
     const int ext[3] = {L.n[0] + 2, m->g.dim == 3 ? L.n[1] + 2 : 1, L.n[2] + 2};
     for (int op = 0; op < 2; ++op)
         for (int a = 0; a < 3; ++a) {
@@ -831,6 +831,7 @@ saras_status saras_mg_create(const saras_grid_desc *grid, const saras_mg_params 
         m->max_count = g.dim == 2 ? (int)(2.5 * cx * cz) : (int)(2.5 * cx * cy * cz);
     }
     const auto walls = make_walls(g, p);
+    m->h64.split_rows = !m->split_off;
     saras_status st = m->h64.build(m->G, walls);
     if (st) return st;
     if (p.precision == SARAS_PREC_MIXED) {
@@ -984,12 +985,13 @@ static saras_status solve_mixed(saras_mg *m, hipStream_t s, saras_mg_stats *st) 
 
 namespace saras {
 
-void mg_level0(saras_mg *m, double **x, double **b, long *sx, long *sy) {
+void mg_level0(saras_mg *m, double **x, double **b, long *sx, long *sy, long *hz) {
     auto &L0 = m->h64.lv[0];
     *x = L0.x;
     *b = L0.b;
     *sx = L0.sx;
     *sy = L0.sy;
+    if (hz) *hz = L0.hz;
 }
 
 hipStream_t mg_stream(saras_mg *m) { return m->stream; }
@@ -1131,7 +1133,7 @@ saras_status saras_mg_set_level(saras_mg *mg, int level, const double *lhs_pad, 
                 for (int k = 0; k < nz; ++k) {
                     const long f = d3 ? ((long)(i + 1) * (ny + 2) + (j + 1)) * (nz + 2) + (k + 1)
                                       : (long)(i + 1) * (nz + 2) + (k + 1);
-                    core[(size_t)i * L.sx + (size_t)j * L.sy + k] = src[f];
+                    core[(size_t)i * L.sx + (size_t)j * L.sy + (L.hz ? (k & 1) * L.hz + (k >> 1) : k)] = src[f];
                 }
         HIP_TRY(hipMemcpy(which == 0 ? L.x : L.b, core.data(), L.elems * sizeof(double), hipMemcpyHostToDevice));
     }
@@ -1156,7 +1158,7 @@ saras_status saras_mg_get_level(saras_mg *mg, int level, double *lhs_pad, double
                 for (int k = 0; k < nz; ++k) {
                     const long f = d3 ? ((long)(i + 1) * (ny + 2) + (j + 1)) * (nz + 2) + (k + 1)
                                       : (long)(i + 1) * (nz + 2) + (k + 1);
-                    dst[f] = core[(size_t)i * L.sx + (size_t)j * L.sy + k];
+                    dst[f] = core[(size_t)i * L.sx + (size_t)j * L.sy + (L.hz ? (k & 1) * L.hz + (k >> 1) : k)];
                 }
     }
     return SARAS_OK;

@@ -99,6 +99,38 @@ struct PerField {
     }
 };
 
+// Core-only field on colour-split rows (the pressure solver's level-0
+// iterate): element k of a row at (k & 1) hz + k / 2, otherwise as CoreField.
+struct SplitField {
+    const double *f;
+    FieldBC bc;
+    long hz;
+    __device__ __forceinline__ long zo(int k) const { return (long)(k & 1) * hz + (k >> 1); }
+    __device__ __forceinline__ double at(const Geom &G, int i, int j, int k) const {
+        return f[(long)i * G.sx + (long)j * G.sy + zo(k)];
+    }
+    __device__ __forceinline__ double nb(const Geom &G, int ax, int i, int j, int k, int off) const {
+        int c[3] = {i, j, k};
+        int q = c[ax] + off;
+        const int n = G.n[ax];
+        if (q >= 0 && q < n) {
+            c[ax] = q;
+            return f[(long)c[0] * G.sx + (long)c[1] * G.sy + zo(c[2])];
+        }
+        if (G.per[ax]) {
+            c[ax] = q < 0 ? q + n : q - n;
+            return f[(long)c[0] * G.sx + (long)c[1] * G.sy + zo(c[2])];
+        }
+        const bool hi = q >= n;
+        const int type = hi ? bc.type[2 * ax + 1] : bc.type[2 * ax];
+        const double val = hi ? bc.val[2 * ax + 1] : bc.val[2 * ax];
+        const double kh = hi ? bc.kh[2 * ax + 1] : bc.kh[2 * ax];
+        c[ax] = hi ? n - 1 : 0;
+        const double inner = f[(long)c[0] * G.sx + (long)c[1] * G.sy + zo(c[2])];
+        return type == 0 ? 2.0 * val - inner : inner - kh;
+    }
+};
+
 // Caller-layout field with stored pads (pad p, (n+2p)^3): the single-operator
 // entry points read whatever ghost values the caller supplies, as the
 // reference's derivative class reads its array.
@@ -644,7 +676,7 @@ __global__ void __launch_bounds__(256) k_jacobi(Geom G, CoreField F, const doubl
 // vfield::divergence (vfield.cc:721-737) scaled into the Poisson RHS:
 // b = (((0 + dVx/dx) + dVy/dy) + dVz/dz) * sc   (lsRK3_d3.cc:278-279)
 __global__ void __launch_bounds__(256) k_div(Geom G, CoreField V0, CoreField V1, CoreField V2, double sc,
-                                             double *__restrict__ b, long bsx, long bsy) {
+                                             double *__restrict__ b, long bsx, long bsy, long bhz) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
     const int i = blockIdx.z;
@@ -653,11 +685,12 @@ __global__ void __launch_bounds__(256) k_div(Geom G, CoreField V0, CoreField V1,
     d += d1(G, V0, 0, i, j, k);
     d += d1(G, V1, 1, i, j, k);
     d += d1(G, V2, 2, i, j, k);
-    b[(long)i * bsx + (long)j * bsy + k] = d * sc;
+    b[(long)i * bsx + (long)j * bsy + (bhz ? (long)(k & 1) * bhz + (k >> 1) : (long)k)] = d * sc;
 }
 
 // P += Pp; V -= ((alpha+beta) dt) grad Pp   (lsRK3_d3.cc:285-295)
-__global__ void __launch_bounds__(256) k_correct(Geom G, CoreField Pp, double *__restrict__ P,
+template <class PF>
+__global__ void __launch_bounds__(256) k_correct(Geom G, PF Pp, double *__restrict__ P,
                                                  double *__restrict__ V0, double *__restrict__ V1,
                                                  double *__restrict__ V2, double pc) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -748,18 +781,20 @@ __global__ void __launch_bounds__(256) k_cfl(Geom G, const double *__restrict__ 
 
 // caller array (pad p) -> core
 __global__ void k_field_load(const double *__restrict__ src, long SX, long SY, int p, double *__restrict__ dst,
-                             long sx, long sy, int nx, int ny, int nz) {
+                             long sx, long sy, long hz, int nx, int ny, int nz) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y, i = blockIdx.z;
     if (k >= nz) return;
-    dst[(long)i * sx + (long)j * sy + k] = src[(long)(i + p) * SX + (long)(j + p) * SY + (k + p)];
+    dst[(long)i * sx + (long)j * sy + (hz ? (long)(k & 1) * hz + (k >> 1) : (long)k)] =
+        src[(long)(i + p) * SX + (long)(j + p) * SY + (k + p)];
 }
 
 // core + face ghost cells -> caller array (pad p): every cell with at most one
 // index outside the core gets the value the reference's field holds after
 // imposeBCs (periodic pads p deep, first wall layer); edge/corner pads and the
 // second wall layer are left as the caller had them.
-__global__ void k_field_store(Geom G, CoreField F, double *__restrict__ dst, long SX, long SY, int p) {
+template <class FF>
+__global__ void k_field_store(Geom G, FF F, double *__restrict__ dst, long SX, long SY, int p) {
     const int kk = blockIdx.x * blockDim.x + threadIdx.x;   // over nz + 2p
     const int jj = blockIdx.y, ii = blockIdx.z;               // over ny + 2p, nx + 2p
     if (kk >= G.n[2] + 2 * p) return;

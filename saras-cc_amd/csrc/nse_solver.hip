@@ -123,11 +123,12 @@ struct saras_nse {
         static const int m[3][3] = {{0, 3, 5}, {3, 1, 4}, {5, 4, 2}};
         return tsg(m[c][ax]);
     }
-    CoreField pp() const {
+    // the pressure solver's level-0 iterate (Pp), in its row layout
+    saras::nse::SplitField pp() const {
         double *x, *b;
-        long sx, sy;
-        saras::mg_level0(mg, &x, &b, &sx, &sy);
-        return CoreField{x, ppbc};
+        long sx, sy, hz;
+        saras::mg_level0(mg, &x, &b, &sx, &sy, &hz);
+        return saras::nse::SplitField{x, ppbc, hz};
     }
 };
 
@@ -267,17 +268,22 @@ saras_status stage_run(saras_nse *s, int stage) {
         if (e) return e;
     }
     double *x0, *b0;
-    long sx0, sy0;
-    saras::mg_level0(s->mg, &x0, &b0, &sx0, &sy0);
+    long sx0, sy0, hz0;
+    saras::mg_level0(s->mg, &x0, &b0, &sx0, &sy0, &hz0);
     const double sc = euler ? 1.0 / s->p.dt : 1.0 / ((a + b) * s->p.dt);
     hipLaunchKernelGGL(saras::nse::k_div, s->grid3(), dim3(s->threads), 0, s->s, s->G, s->field(0), s->field(1),
-                       s->field(2), sc, b0, sx0, sy0);
+                       s->field(2), sc, b0, sx0, sy0, hz0);
     NSE_TRY(hipGetLastError());
     e = saras::mg_solve_resident(s->mg, s->s, &s->mg_stats);
     if (e && e != SARAS_NOT_CONVERGED) return e;
     const double pc = euler ? s->p.dt : (a + b) * s->p.dt;
-    hipLaunchKernelGGL(saras::nse::k_correct, s->grid3(), dim3(s->threads), 0, s->s, s->G, s->pp(), s->F[4],
-                       s->F[0], s->F[1], s->F[2], pc);
+    const saras::nse::SplitField pp = s->pp();
+    if (pp.hz)
+        hipLaunchKernelGGL(saras::nse::k_correct<saras::nse::SplitField>, s->grid3(), dim3(s->threads), 0, s->s, s->G,
+                           pp, s->F[4], s->F[0], s->F[1], s->F[2], pc);
+    else
+        hipLaunchKernelGGL(saras::nse::k_correct<CoreField>, s->grid3(), dim3(s->threads), 0, s->s, s->G,
+                           CoreField{pp.f, pp.bc}, s->F[4], s->F[0], s->F[1], s->F[2], pc);
     NSE_TRY(hipGetLastError());
     return SARAS_OK;
 }
@@ -308,7 +314,7 @@ FieldBC make_bc(const int *type, const double *val, const double *hwall) {
 }
 
 // caller array (host or device, pad p) -> core buffer dst
-saras_status load_field(saras_nse *s, const double *src, int ptr_kind, double *dst) {
+saras_status load_field(saras_nse *s, const double *src, int ptr_kind, double *dst, long hz = 0) {
     const double *d = src;
     if (ptr_kind == SARAS_PTR_HOST) {
         saras_status e = ensure_stage(s);
@@ -319,7 +325,7 @@ saras_status load_field(saras_nse *s, const double *src, int ptr_kind, double *d
     const int p = s->g.pad;
     const long SY = s->g.n[2] + 2 * p, SX = SY * (s->g.n[1] + 2 * p);
     hipLaunchKernelGGL(saras::nse::k_field_load, s->grid3(), dim3(s->threads), 0, s->s, d, SX, SY, p, dst, s->G.sx,
-                       s->G.sy, s->G.n[0], s->G.n[1], s->G.n[2]);
+                       s->G.sy, hz, s->G.n[0], s->G.n[1], s->G.n[2]);
     NSE_TRY(hipGetLastError());
     if (ptr_kind == SARAS_PTR_HOST) NSE_TRY(hipStreamSynchronize(s->s));
     return SARAS_OK;
@@ -334,7 +340,8 @@ __global__ void k_store_core(const double *__restrict__ src, long sx, long sy, d
 }
 
 // core buffer (+ face ghosts unless core_only) -> caller array
-saras_status store_field(saras_nse *s, CoreField f, double *dst, int ptr_kind, bool core_only) {
+template <class FF>
+saras_status store_field(saras_nse *s, FF f, double *dst, int ptr_kind, bool core_only) {
     double *d = dst;
     if (ptr_kind == SARAS_PTR_HOST) {
         saras_status e = ensure_stage(s);
@@ -348,7 +355,7 @@ saras_status store_field(saras_nse *s, CoreField f, double *dst, int ptr_kind, b
         hipLaunchKernelGGL(k_store_core, s->grid3(), dim3(s->threads), 0, s->s, f.f, s->G.sx, s->G.sy, d, SX, SY, p,
                            s->G.n[2]);
     else
-        hipLaunchKernelGGL(saras::nse::k_field_store, s->grid3_full(p), dim3(s->threads), 0, s->s, s->G, f, d, SX,
+        hipLaunchKernelGGL(saras::nse::k_field_store<FF>, s->grid3_full(p), dim3(s->threads), 0, s->s, s->G, f, d, SX,
                            SY, p);
     NSE_TRY(hipGetLastError());
     if (ptr_kind == SARAS_PTR_HOST) {
@@ -545,9 +552,9 @@ saras_status saras_nse_set_fields(saras_nse *s, const double *vx, const double *
     }
     if (pp) {
         double *x0, *b0;
-        long sx0, sy0;
-        saras::mg_level0(s->mg, &x0, &b0, &sx0, &sy0);
-        saras_status e = load_field(s, pp, ptr_kind, x0);
+        long sx0, sy0, hz0;
+        saras::mg_level0(s->mg, &x0, &b0, &sx0, &sy0, &hz0);
+        saras_status e = load_field(s, pp, ptr_kind, x0, hz0);
         if (e) return e;
     }
     NSE_TRY(hipStreamSynchronize(s->s));
@@ -564,7 +571,9 @@ saras_status saras_nse_get_fields(saras_nse *s, double *vx, double *vy, double *
         if (e) return e;
     }
     if (pp) {
-        saras_status e = store_field(s, s->pp(), pp, ptr_kind, false);
+        const saras::nse::SplitField f = s->pp();
+        saras_status e = f.hz ? store_field(s, f, pp, ptr_kind, false)
+                              : store_field(s, CoreField{f.f, f.bc}, pp, ptr_kind, false);
         if (e) return e;
     }
     NSE_TRY(hipStreamSynchronize(s->s));

@@ -683,11 +683,12 @@ __global__ void __launch_bounds__(256) k_ghost_phase(double *__restrict__ full, 
     }
 }
 
-// plain streaming copy (bandwidth reference and counter calibration)
+// plain streaming copy, one 16-byte vector per lane (the achievable HBM
+// bandwidth reference and the counter calibration)
 __global__ void __launch_bounds__(256) k_stream_copy(const double2 *__restrict__ src, double2 *__restrict__ dst,
                                                      long n2) {
-    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n2; t += (long)gridDim.x * blockDim.x)
-        dst[t] = src[t];
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n2) dst[t] = src[t];
 }
 
 }  // namespace saras

@@ -609,6 +609,26 @@ saras_status run_vcycle(saras_mg *m, Ops<T> &ops) {
 template <int DIM>
 void launch_norm(int type, saras_mg *m, const saras::LevelArgs<double> &a, const double *x, const double *b,
                  hipStream_t s) {
+    if (a.hz) {
+        // split rows: the vectorised residual pass, norms only
+        const unsigned nb = saras::kNormBlocks, nt = saras::kNormThreads;
+        if (type == 0)
+            hipLaunchKernelGGL((saras::k_refine_split<0, DIM, true>), dim3(nb), dim3(nt), 0, s, a, x, nullptr, 0L, 0L,
+                               0L, b, nullptr, nullptr, m->partial);
+        else if (type == 1)
+            hipLaunchKernelGGL((saras::k_refine_split<1, DIM, true>), dim3(nb), dim3(nt), 0, s, a, x, nullptr, 0L, 0L,
+                               0L, b, nullptr, nullptr, m->partial);
+        else
+            hipLaunchKernelGGL((saras::k_refine_split<2, DIM, true>), dim3(nb), dim3(nt), 0, s, a, x, nullptr, 0L, 0L,
+                               0L, b, nullptr, nullptr, m->partial);
+        if (type == 0)
+            hipLaunchKernelGGL((saras::k_norm_final<0>), dim3(1), dim3(nt), 0, s, m->partial, saras::kNormBlocks, m->norm_out);
+        else if (type == 1)
+            hipLaunchKernelGGL((saras::k_norm_final<1>), dim3(1), dim3(nt), 0, s, m->partial, saras::kNormBlocks, m->norm_out);
+        else
+            hipLaunchKernelGGL((saras::k_norm_final<2>), dim3(1), dim3(nt), 0, s, m->partial, saras::kNormBlocks, m->norm_out);
+        return;
+    }
     switch (type) {
         case 0:
             hipLaunchKernelGGL((saras::k_norm_partial<0, DIM>), dim3(saras::kNormBlocks), dim3(saras::kNormThreads), 0,
@@ -907,6 +927,25 @@ template <int DIM>
 void launch_refine(int type, saras_mg *m, const saras::LevelArgs<double> &a, const float *e, long esx, long esy,
                    long ehz, float *r32, hipStream_t s) {
     auto &L0 = m->h64.lv[0];
+    if (L0.hz && ehz) {
+        const unsigned nb = saras::kNormBlocks, nt = saras::kNormThreads;
+        if (type == 0)
+            hipLaunchKernelGGL((saras::k_refine_split<0, DIM>), dim3(nb), dim3(nt), 0, s, a, L0.x, e, esx, esy, ehz, L0.b,
+                               m->x64b, r32, m->partial);
+        else if (type == 1)
+            hipLaunchKernelGGL((saras::k_refine_split<1, DIM>), dim3(nb), dim3(nt), 0, s, a, L0.x, e, esx, esy, ehz, L0.b,
+                               m->x64b, r32, m->partial);
+        else
+            hipLaunchKernelGGL((saras::k_refine_split<2, DIM>), dim3(nb), dim3(nt), 0, s, a, L0.x, e, esx, esy, ehz, L0.b,
+                               m->x64b, r32, m->partial);
+        if (type == 0)
+            hipLaunchKernelGGL((saras::k_norm_final<0>), dim3(1), dim3(nt), 0, s, m->partial, saras::kNormBlocks, m->norm_out);
+        else if (type == 1)
+            hipLaunchKernelGGL((saras::k_norm_final<1>), dim3(1), dim3(nt), 0, s, m->partial, saras::kNormBlocks, m->norm_out);
+        else
+            hipLaunchKernelGGL((saras::k_norm_final<2>), dim3(1), dim3(nt), 0, s, m->partial, saras::kNormBlocks, m->norm_out);
+        return;
+    }
     switch (type) {
         case 0:
             hipLaunchKernelGGL((saras::k_refine<0, DIM>), dim3(saras::kNormBlocks), dim3(saras::kNormThreads), 0, s, a,
@@ -1271,7 +1310,7 @@ saras_status saras_bench_stream_copy(size_t n, int reps, double *ms) {
     HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
-    const unsigned blocks = 256 * 16;
+    const unsigned blocks = (unsigned)((n2 + 255) / 256);
     hipLaunchKernelGGL(saras::k_stream_copy, dim3(blocks), dim3(256), 0, s, (const double2 *)a, (double2 *)b, n2);
     HIP_TRY(hipEventRecord(e0, s));
     for (int r = 0; r < reps; ++r)

@@ -432,4 +432,165 @@ __global__ void __launch_bounds__(kSplitThreads) k_prolong_split(int nx, int ny,
     }
 }
 
+// ---------------------------------------------------------------------------
+// FP64 refinement step of the mixed-precision solve on split rows (the same
+// expressions as k_refine): x_new = x_old + e, r = b - A x_new with the wall
+// rules applied to x_new, r32 = (float) r, per-block norm partials of r and
+// b.  A lane owns W = 2 positions m0.. of both halves of one row, i.e. the
+// four cells k = 2 m0 .. 2 m0 + 3; FP64 arrays use the FP64 level's split
+// rows (hz), the FP32 correction and residual the FP32 level's (ehz).
+// ---------------------------------------------------------------------------
+// NORM_ONLY: the residual norm partials of computeError alone
+// (poisson_d3.cc:563-663): no correction, nothing written but the partials.
+template <int TYPE, int DIM, bool NORM_ONLY = false>
+__global__ void __launch_bounds__(kNormThreads) k_refine_split(LevelArgs<double> L, const double *__restrict__ xold,
+                                                               const float *__restrict__ e, long esx, long esy,
+                                                               long ehz, const double *__restrict__ b,
+                                                               double *__restrict__ xnew, float *__restrict__ r32,
+                                                               double *__restrict__ partial) {
+    constexpr int W = 2;
+    const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
+    const int half = nz >> 1, q = half / W;
+    const long total = (long)nx * ny * q;
+    const long hz = L.hz;
+    const int lane = threadIdx.x & 63;
+    double ar = 0.0, ab = 0.0;
+    // rows of the x_old + e iterate: even and odd halves at m0
+    auto row_vals = [&](int ii, int jj, int m0, double (&ve)[W], double (&vo)[W]) {
+        const long rb = (long)ii * L.sx + (long)jj * L.sy;
+        const double2 xe = *reinterpret_cast<const double2 *>(xold + rb + m0);
+        const double2 xo = *reinterpret_cast<const double2 *>(xold + rb + hz + m0);
+        ve[0] = xe.x;
+        ve[1] = xe.y;
+        vo[0] = xo.x;
+        vo[1] = xo.y;
+        if (e) {
+            const long eb = (long)ii * esx + (long)jj * esy;
+            const float2 fe = *reinterpret_cast<const float2 *>(e + eb + m0);
+            const float2 fo = *reinterpret_cast<const float2 *>(e + eb + ehz + m0);
+            ve[0] = ve[0] + (double)fe.x;
+            ve[1] = ve[1] + (double)fe.y;
+            vo[0] = vo[0] + (double)fo.x;
+            vo[1] = vo[1] + (double)fo.y;
+        }
+    };
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+        const int row = (int)(t / q);
+        const int v = (int)(t - (long)row * q);
+        const int i = row / ny, j = row - (row / ny) * ny;
+        const int m0 = v * W;
+        double ce[W], co[W];
+        row_vals(i, j, m0, ce, co);
+        // x/y neighbours: [0] E, [1] W, [2] N, [3] S
+        double ne[4][W], no[4][W];
+        auto nbr = [&](int slot, bool inside, int ii, int jj, int bc, int axis) {
+            if (inside || bc == BC_PER) {
+                row_vals(ii, jj, m0, ne[slot], no[slot]);
+            } else {
+#pragma unroll
+                for (int p = 0; p < W; ++p) {
+                    double wve = 0.0, wvo = 0.0;
+                    if (bc == BC_DIRV) {
+                        const int ke = 2 * (m0 + p), ko = ke + 1;
+                        if (axis == 0) {
+                            wve = DIM == 3 ? L.wall[0][(j + 1) * (nz + 2) + ke + 1] : L.wall[0][ke + 1];
+                            wvo = DIM == 3 ? L.wall[0][(j + 1) * (nz + 2) + ko + 1] : L.wall[0][ko + 1];
+                        } else {
+                            wve = L.wall[1][(i + 1) * (nz + 2) + ke + 1];
+                            wvo = L.wall[1][(i + 1) * (nz + 2) + ko + 1];
+                        }
+                    }
+                    ne[slot][p] = bc_value<double>(bc, ce[p], 0.0, wve);
+                    no[slot][p] = bc_value<double>(bc, co[p], 0.0, wvo);
+                }
+            }
+        };
+        nbr(0, i + 1 < nx, i + 1 < nx ? i + 1 : 0, j, L.bc_hi[0], 0);
+        nbr(1, i > 0, i > 0 ? i - 1 : nx - 1, j, L.bc_lo[0], 0);
+        if (DIM == 3) {
+            nbr(2, j + 1 < ny, i, j + 1 < ny ? j + 1 : 0, L.bc_hi[1], 1);
+            nbr(3, j > 0, i, j > 0 ? j - 1 : ny - 1, L.bc_lo[1], 1);
+        }
+        // z: even cell 2m: zB = odd[m-1], zT = odd[m]; odd cell 2m+1: zB = even[m], zT = even[m+1]
+        const double from_prev = __shfl_up(co[W - 1], 1, 64);
+        const double from_next = __shfl_down(ce[0], 1, 64);
+        const long rb = (long)i * L.sx + (long)j * L.sy;
+        const long eb = (long)i * esx + (long)j * esy;
+        auto cell = [&](long off64, long off32) {
+            double x = xold[rb + off64];
+            if (e) x = x + (double)e[eb + off32];
+            return x;
+        };
+        double zlo0, zhi_last;
+        if (m0 == 0) {
+            if (L.bc_lo[2] == BC_PER) zlo0 = cell(hz + half - 1, ehz + half - 1);
+            else {
+                const double wv = L.bc_lo[2] == BC_DIRV ? (DIM == 3 ? L.wall[2][(i + 1) * (ny + 2) + j + 1] : L.wall[2][i + 1])
+                                                        : 0.0;
+                zlo0 = bc_value<double>(L.bc_lo[2], ce[0], 0.0, wv);
+            }
+        } else {
+            zlo0 = lane > 0 ? from_prev : cell(hz + m0 - 1, ehz + m0 - 1);
+        }
+        if (m0 + W == half) {
+            if (L.bc_hi[2] == BC_PER) zhi_last = cell(0, 0);
+            else {
+                const double wv = L.bc_hi[2] == BC_DIRV ? (DIM == 3 ? L.wall[2][(i + 1) * (ny + 2) + j + 1] : L.wall[2][i + 1])
+                                                        : 0.0;
+                zhi_last = bc_value<double>(L.bc_hi[2], co[W - 1], 0.0, wv);
+            }
+        } else {
+            zhi_last = lane < 63 ? from_next : cell(m0 + W, m0 + W);
+        }
+        const double c2x = L.c2[0][i], c1x = L.c1[0][i];
+        const double c2y = DIM == 3 ? L.c2[1][j] : 0.0, c1y = DIM == 3 ? L.c1[1][j] : 0.0;
+        const double2 be = *reinterpret_cast<const double2 *>(b + rb + m0);
+        const double2 bo = *reinterpret_cast<const double2 *>(b + rb + hz + m0);
+        const double bev[W] = {be.x, be.y}, bov[W] = {bo.x, bo.y};
+        double re[W], ro[W];
+#pragma unroll
+        for (int p = 0; p < W; ++p) {
+            Stencil<double, DIM> s;
+            s.c = ce[p];
+            s.xE = ne[0][p];
+            s.xW = ne[1][p];
+            if (DIM == 3) {
+                s.yN = ne[2][p];
+                s.yS = ne[3][p];
+            }
+            s.zB = p > 0 ? co[p - 1] : zlo0;
+            s.zT = co[p];
+            re[p] = residual<double, DIM>(s, c2x, c1x, c2y, c1y, L.c2zs[m0 + p], L.c1zs[m0 + p], bev[p]);
+            s.c = co[p];
+            s.xE = no[0][p];
+            s.xW = no[1][p];
+            if (DIM == 3) {
+                s.yN = no[2][p];
+                s.yS = no[3][p];
+            }
+            s.zB = ce[p];
+            s.zT = p < W - 1 ? ce[p + 1] : zhi_last;
+            ro[p] = residual<double, DIM>(s, c2x, c1x, c2y, c1y, L.c2zs[half + m0 + p], L.c1zs[half + m0 + p], bov[p]);
+        }
+        if (!NORM_ONLY) {
+            *reinterpret_cast<double2 *>(xnew + rb + m0) = make_double2(ce[0], ce[1]);
+            *reinterpret_cast<double2 *>(xnew + rb + hz + m0) = make_double2(co[0], co[1]);
+            *reinterpret_cast<float2 *>(r32 + eb + m0) = make_float2((float)re[0], (float)re[1]);
+            *reinterpret_cast<float2 *>(r32 + eb + ehz + m0) = make_float2((float)ro[0], (float)ro[1]);
+        }
+#pragma unroll
+        for (int p = 0; p < W; ++p) {
+            ar = norm_acc<TYPE>(ar, re[p]);
+            ab = norm_acc<TYPE>(ab, bev[p]);
+            ar = norm_acc<TYPE>(ar, ro[p]);
+            ab = norm_acc<TYPE>(ab, bov[p]);
+        }
+    }
+    block_reduce2<TYPE>(ar, ab);
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = ar;
+        partial[2 * blockIdx.x + 1] = ab;
+    }
+}
+
 }  // namespace saras

@@ -12,12 +12,15 @@ per-level byte model  B_l = N_l s (3 (nu1 + nu2) + 2 + 1/8 + 2 + 1/8)
 (3 words per smoothing sweep, residual+restriction reads x, b and writes
 b_{l+1}, prolongation reads e_{l+1} and reads/writes x), s = bytes per word.
 
-solve: time to a 1e-10 relative L-inf residual on a 256^3 periodic uniform
-grid, RHS = 64 Fourier modes with amplitudes ~ |k|^-2 and zero mean, plain
-FP64 vs mixed precision, inputs already resident in HBM.
+solve: time to a 1e-10 relative L-inf residual on 256^3 and 512^3 periodic
+uniform grids, RHS = 64 Fourier modes with amplitudes ~ |k|^-2 and zero mean,
+plain FP64 vs mixed precision, inputs already resident in HBM; the mixed
+solution's relative L2 / L-inf difference from the FP64 solution (which
+reproduces the FP64 CPU oracle bit for bit), modulo the mean.
 
-roofline: the dominant kernel (finest-level fused sweep), algorithmic
-3 words per cell per launch / average launch time (HIP events).
+roofline: the dominant kernel (one colour pass of the finest level on
+colour-split rows): algorithmic 1.5 words per cell per launch (half of a
+sweep's 3) / average launch time (HIP events around the finest-level sweeps).
 cpu_baseline: the FP64 CPU oracle's V-cycle on the same workload.
 
 serving: the batched request loop on the Poisson request mix (75 % 3-D
@@ -51,12 +54,12 @@ import saras_amd as S  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 N = 256
-DOMINANT = "k_sweep_march<float,32,16,true>"
-# HBM bytes per launch of the dominant kernel at 256^3 from rocprofv3 --pmc
-# FETCH_SIZE / WRITE_SIZE (KiB per dispatch), calibrated against the copy
-# kernel (FETCH x 2.0, WRITE x 1.0): (2 * 72734.1 + 65536.0) KiB.
-# Source: profiles/march_hbm_traffic_256cubed.md
-DOMINANT_TRAFFIC_BYTES = (2 * 72734.1 + 65536.0) * 1024
+DOMINANT = "k_rb_split<float,3,true,false>"
+# HBM bytes per launch of the dominant kernel (one colour pass of the finest
+# level) at 256^3 from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (KiB per
+# dispatch), calibrated against the copy kernel (FETCH x 2.0, WRITE x 1.0):
+# (2 * 50246.5 + 32768.0) KiB.  Source: profiles/sweep_hbm_traffic_256cubed.md
+DOMINANT_TRAFFIC_BYTES = (2 * 50246.5 + 32768.0) * 1024
 
 
 def bytes_per_vcycle(shapes, s, nu1, nu2):
@@ -80,11 +83,14 @@ def rbc_rhs(seed=2):
 
 
 def fourier_rhs(n, modes=64, kmax=16, seed=3):
-    """sum_m A_m cos(2 pi k_m . x + phi_m), A_m = U(0.5, 1) |k_m|^-2, zero mean."""
+    """sum_m A_m cos(2 pi k_m . x + phi_m), A_m = U(0.5, 1) |k_m|^-2, zero mean.
+    Each mode is Re(e_x(i) * [e_y(j) e_z(k) e^{i phi}]): the sum over modes is one
+    (n x 2M) @ (2M x n^2) product of real and imaginary parts."""
     rng = np.random.default_rng(seed)
     xc = (np.arange(n) + 0.5) / n
-    acc = np.zeros((n, n, n))
-    for _ in range(modes):
+    A = np.zeros((n, 2 * modes))
+    B = np.zeros((2 * modes, n * n))
+    for m in range(modes):
         while True:
             k = rng.integers(-kmax, kmax + 1, 3)
             if np.abs(k).max() >= 1:
@@ -94,7 +100,10 @@ def fourier_rhs(n, modes=64, kmax=16, seed=3):
         ex = np.exp(2j * np.pi * k[0] * xc)
         ey = np.exp(2j * np.pi * k[1] * xc)
         ez = np.exp(2j * np.pi * k[2] * xc) * np.exp(1j * phi)
-        acc += amp * np.real(ex[:, None, None] * (ey[:, None] * ez[None, :])[None, :, :])
+        yz = np.outer(ey, ez).ravel()
+        A[:, 2 * m], A[:, 2 * m + 1] = amp * ex.real, -amp * ex.imag
+        B[2 * m], B[2 * m + 1] = yz.real, yz.imag
+    acc = (A @ B).reshape(n, n, n)
     acc -= acc.mean()
     b = np.zeros((n + 2,) * 3)
     b[1:-1, 1:-1, 1:-1] = acc
@@ -116,10 +125,13 @@ def cycle_rates(args, nu1, nu2):
         ms = mg.bench_cycles(args.steps) / args.steps
         reps = 20
         mg.bench_sweeps(2)
-        launch_ms = mg.bench_sweeps(reps) / reps
+        # a sweep of the finest level is two launches of the colour-pass
+        # kernel (red, black) of equal work: per launch, half the sweep's
+        # 3 words per cell and half its time
+        launch_ms = mg.bench_sweeps(reps) / reps / 2
         n0 = float(np.prod(shapes[0]))
         out[name] = dict(ms=ms, bv=bytes_per_vcycle(shapes, s, nu1, nu2), launch_ms=launch_ms,
-                         launch_bytes=3.0 * n0 * s, shapes=shapes, s=s)
+                         launch_bytes=1.5 * n0 * s, shapes=shapes, s=s)
         mg.close()
     return out
 
@@ -141,27 +153,36 @@ def profile_sweeps(args, nu1, nu2):
     print(json.dumps({"copy_ms": ms, "copy_bytes": 16 * (1 << 28)}), flush=True)
 
 
-def solve_compare(nu1, nu2, tol=1e-10):
-    n = N
+def solve_compare(nu1, nu2, tol=1e-10, n=N, reps=3, inner=4):
+    """Plain FP64 and mixed-precision solves of the periodic Fourier problem to
+    the same relative L-inf residual, inputs resident in HBM; the mixed
+    solution's difference from the FP64 one (the FP64 path reproduces the
+    FP64 CPU oracle bit for bit), modulo the mean."""
     g = S.grid_desc((n, n, n), periodic=(1, 1, 1))
     b = fourier_rhs(n)
     bd = S.DeviceArray.from_host(b)
-    res = {}
+    res, sols = {}, {}
     for name, prec in (("fp64", S.PREC_FP64), ("mixed", S.PREC_MIXED)):
         p = S.mg_params(vc_count=100, vc_tol=tol, mg_tol=1e-2, pre=nu1, post=nu2, bc=S.BC_NEUMANN,
-                        precision=prec, inner_cycles=4, max_refine=40)
+                        precision=prec, inner_cycles=inner, max_refine=40)
         mg = S.Multigrid(g, p)
         best = None
-        for _ in range(3):
+        for r in range(reps):
             xd = S.DeviceArray.from_host(np.zeros_like(b))
             st, status = mg.solve_device(xd, bd)
             if best is None or st["device_ms"] < best["device_ms"]:
                 best = st
+            if r == reps - 1:
+                sols[name] = xd.to_host(b.shape)[1:-1, 1:-1, 1:-1]
             xd.free()
         res[name] = dict(ms=best["device_ms"], cycles=best["cycles"], refinements=best["refinements"],
                          residual=best["residual"], converged=best["converged"])
         mg.close()
     bd.free()
+    d = (sols["mixed"] - sols["mixed"].mean()) - (sols["fp64"] - sols["fp64"].mean())
+    ref = sols["fp64"] - sols["fp64"].mean()
+    res["diff_l2_rel"] = float(np.linalg.norm(d) / np.linalg.norm(ref))
+    res["diff_linf_rel"] = float(np.abs(d).max() / np.abs(ref).max())
     return res
 
 
@@ -378,19 +399,22 @@ def main():
                      "bytes_per_launch": head["launch_bytes"]},
         "fp64_vcycle": {"ms": round(f64["ms"], 4),
                         "tbps": round(f64["bv"] / (f64["ms"] * 1e-3) / 1e12, 4),
-                        "sweep_launch_ms": round(f64["launch_ms"], 5),
-                        "sweep_gbs": round(f64["launch_bytes"] / (f64["launch_ms"] * 1e-3) / 1e9, 1)},
+                        "pass_launch_ms": round(f64["launch_ms"], 5),
+                        "pass_gbs": round(f64["launch_bytes"] / (f64["launch_ms"] * 1e-3) / 1e9, 1)},
         "stream_copy_gbs": round(copy_gbs, 1),
     }
     if not args.no_solve:
-        sc = solve_compare(nu1, nu2)
-        out["solve_256^3_periodic_to_1e-10"] = {
-            "fp64_ms": round(sc["fp64"]["ms"], 3), "fp64_cycles": sc["fp64"]["cycles"],
-            "fp64_residual": sc["fp64"]["residual"], "fp64_converged": sc["fp64"]["converged"],
-            "mixed_ms": round(sc["mixed"]["ms"], 3), "mixed_fp32_cycles": sc["mixed"]["cycles"],
-            "mixed_refinements": sc["mixed"]["refinements"], "mixed_residual": sc["mixed"]["residual"],
-            "mixed_converged": sc["mixed"]["converged"],
-            "speedup": round(sc["fp64"]["ms"] / sc["mixed"]["ms"], 3)}
+        for n, reps in ((256, 3), (512, 2)):
+            sc = solve_compare(nu1, nu2, n=n, reps=reps)
+            out[f"solve_{n}^3_periodic_to_1e-10"] = {
+                "fp64_ms": round(sc["fp64"]["ms"], 3), "fp64_cycles": sc["fp64"]["cycles"],
+                "fp64_residual": sc["fp64"]["residual"], "fp64_converged": sc["fp64"]["converged"],
+                "mixed_ms": round(sc["mixed"]["ms"], 3), "mixed_fp32_cycles": sc["mixed"]["cycles"],
+                "mixed_refinements": sc["mixed"]["refinements"], "mixed_residual": sc["mixed"]["residual"],
+                "mixed_converged": sc["mixed"]["converged"],
+                "speedup": round(sc["fp64"]["ms"] / sc["mixed"]["ms"], 3),
+                "mixed_vs_fp64_l2_rel": sc["diff_l2_rel"], "mixed_vs_fp64_linf_rel": sc["diff_linf_rel"]}
+            print(f"solve {n}^3 done", file=sys.stderr, flush=True)
     if not args.no_serve:
         out["serving"] = serving(count=args.requests, slots=args.slots)
     if not args.no_rbc:

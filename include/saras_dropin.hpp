@@ -78,6 +78,18 @@ class lsRK3 {
     void timeAdvance() { check(saras_nse_step(h_, &stats_), "saras_nse_step"); }
     // tseries diagnostics: kinetic energy, max |div V|, NuVol, NuWall
     void diagnostics(double out[4]) { check(saras_nse_diag(h_, out), "saras_nse_diag"); }
+    // vfield::computeTStp with the driver's cap (vfield.cc:691-706); the
+    // returned time step is the one the following steps use
+    double computeTStp(double courant, double dt_max) {
+        double dt = 0.0;
+        check(saras_nse_cfl(h_, courant, dt_max, &dt), "saras_nse_cfl");
+        return dt;
+    }
+    // spiral LES diagnostics of the last step (sum |K| dV, SGS dissipation dV,
+    // nu_t dV; divide by the volume as tseries does) and the run time
+    void sgsDiagnostics(double out[3], double *time) {
+        check(saras_nse_sgs_diag(h_, out, time), "saras_nse_sgs_diag");
+    }
     const saras_nse_stats &stats() const { return stats_; }
     saras_nse *handle() const { return h_; }
 

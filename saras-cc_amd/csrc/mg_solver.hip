@@ -224,6 +224,8 @@ struct saras_mg {
     bool march_tj8 = false;            // FP32 march with 64 x 8 tiles (SARAS_MARCH_TJ8=1)
     bool split_off = false;            // SARAS_NO_SPLIT=1 keeps every level z-contiguous
     size_t tail_lds = kTailLdsDefault;  // dynamic LDS budget of the tail kernel
+    int tail_wave_cells = saras::kTailWaveCells;  // SARAS_TAIL_WAVE_CELLS overrides it
+    bool red_fold = true;              // SARAS_NO_RED_FOLD=1: the restriction leaves the first red pass to the sweep
     // V-cycles replayed from captured hipGraphs (saras_mg_set_graphs).  A
     // graph bakes in the level buffers, and out-of-place sweeps swap them, so
     // graphs are cached per buffer state (at most two states per precision).
@@ -281,6 +283,7 @@ struct Ops {
     T w;
 
     int pend_level = -1;  // level whose next sweep also applies the prolongation
+    int skip_red = -1;    // level whose next sweep's red pass the restriction already applied
 
     bool inhom(int l, bool zero_bc) const { return inhom0 && l == 0 && !zero_bc; }
 
@@ -359,7 +362,9 @@ struct Ops {
                 ehz = C.hz;
                 pend_level = -1;
             }
-            for (int colour = 0; colour < 2; ++colour) {
+            const int first = skip_red == l ? 1 : 0;
+            if (first) skip_red = -1;
+            for (int colour = first; colour < 2; ++colour) {
                 if (H.dim == 3) split_pass<3>(l == 0, pro, grid, a, L, colour, nb, e, esx, esy, ehz);
                 else split_pass<2>(l == 0, pro, grid, a, L, colour, nb, e, esx, esy, ehz);
             }
@@ -415,7 +420,9 @@ struct Ops {
         for (int c = 0; c < count; ++c) rb_sweep(l, zero_bc);
     }
 
-    void resid_restrict(int l) {
+    // red_next: the restriction also applies the coarse level's first
+    // pre-smoothing red pass (split levels; see k_rr_split)
+    void resid_restrict(int l, bool red_next = false) {
         if (l < 0 || l + 1 >= (int)H.lv.size()) std::abort();  // host-side guard: never launch past the hierarchy
         DevLevel<T> &F = H.lv[l], &C = H.lv[l + 1];
         auto fa = level_args(m, H, l, inhom(l, false));
@@ -432,12 +439,23 @@ struct Ops {
             const long total = (long)C.n[0] * C.n[1] * (F.n[2] / 2 / W);
             const int nb = (int)((total + saras::kSplitThreads - 1) / saras::kSplitThreads);
             const int grid = (nb + saras::kXcds - 1) / saras::kXcds * saras::kXcds;
-            if (H.dim == 3)
-                hipLaunchKernelGGL((saras::k_rr_split<T, 3>), dim3(grid), dim3(saras::kSplitThreads), 0, s, fa, F.x,
-                                   F.b, ca, C.b, C.x, r, nb);
-            else
-                hipLaunchKernelGGL((saras::k_rr_split<T, 2>), dim3(grid), dim3(saras::kSplitThreads), 0, s, fa, F.x,
-                                   F.b, ca, C.b, C.x, r, nb);
+            const bool red = red_next && C.hz;
+            if (red) skip_red = l + 1;
+            if (H.dim == 3) {
+                if (red)
+                    hipLaunchKernelGGL((saras::k_rr_split<T, 3, true>), dim3(grid), dim3(saras::kSplitThreads), 0, s,
+                                       fa, F.x, F.b, ca, C.b, C.x, r, nb, w);
+                else
+                    hipLaunchKernelGGL((saras::k_rr_split<T, 3, false>), dim3(grid), dim3(saras::kSplitThreads), 0, s,
+                                       fa, F.x, F.b, ca, C.b, C.x, r, nb, w);
+            } else {
+                if (red)
+                    hipLaunchKernelGGL((saras::k_rr_split<T, 2, true>), dim3(grid), dim3(saras::kSplitThreads), 0, s,
+                                       fa, F.x, F.b, ca, C.b, C.x, r, nb, w);
+                else
+                    hipLaunchKernelGGL((saras::k_rr_split<T, 2, false>), dim3(grid), dim3(saras::kSplitThreads), 0, s,
+                                       fa, F.x, F.b, ca, C.b, C.x, r, nb, w);
+            }
             return;
         }
         if (H.dim == 3)
@@ -521,6 +539,7 @@ struct Ops {
         A.w = w;
         A.tol = m->p.mg_tol;
         A.zero_bc_first = S != 0;
+        A.wave_cells = m->tail_wave_cells;
         A.iters_out = m->iters_dev;
         if (off * sizeof(T) > m->tail_lds) std::abort();  // host-side guard: tail_start() keeps levels within tail_lds
         if (H.dim == 3)
@@ -539,7 +558,7 @@ struct Ops {
         }
         smooth(0, m->p.pre_smooth, false);
         for (int l = 0; l < D; ++l) {
-            resid_restrict(l);
+            resid_restrict(l, m->red_fold && l + 1 < S && l + 1 < D && m->p.pre_smooth > 0);
             if (l + 1 == S) break;
             if (l + 1 == D) {
                 if (m->p.solve_flag) coarse(D, true, 0, 0);
@@ -835,6 +854,10 @@ saras_status saras_mg_create(const saras_grid_desc *grid, const saras_mg_params 
         m->march_tj8 = env4 && env4[0] == '1';
         const char *env5 = std::getenv("SARAS_NO_SPLIT");
         m->split_off = env5 && env5[0] == '1';
+        const char *env8 = std::getenv("SARAS_NO_RED_FOLD");
+        m->red_fold = !(env8 && env8[0] == '1');
+        const char *env7 = std::getenv("SARAS_TAIL_WAVE_CELLS");
+        if (env7 && std::atoi(env7) >= 0) m->tail_wave_cells = std::atoi(env7);
         const char *env6 = std::getenv("SARAS_TAIL_LDS_KB");
         if (env6 && std::atoi(env6) > 0) m->tail_lds = (size_t)std::min(60, std::atoi(env6)) * 1024;
     }

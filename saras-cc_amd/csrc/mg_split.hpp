@@ -238,12 +238,17 @@ __global__ void __launch_bounds__(kSplitThreads) k_rb_split(LevelArgs<T> L, T *_
 // poisson_d3.cc:71-93, 484-511; poisson_d2.cc:288-297), and x(l+1) = 0.  A
 // lane owns W consecutive coarse cells K0.. of coarse row (I, J): their fine
 // children are the even and odd half-rows at K0.. of the 2 x 2 child rows.
+// RED: the coarse level's first pre-smoothing red pass is applied as well.
+// Its input iterate is the zero just written, so each red cell's update reads
+// only zeros (wall rules of zero give zero) and its own new right-hand side:
+// the same sor_update expression, evaluated here instead of in a launch of
+// its own.
 // ---------------------------------------------------------------------------
-template <typename T, int DIM>
+template <typename T, int DIM, bool RED = false>
 __global__ void __launch_bounds__(kSplitThreads) k_rr_split(LevelArgs<T> F, const T *__restrict__ x,
                                                             const T *__restrict__ b, LevelArgs<T> C,
                                                             T *__restrict__ bc, T *__restrict__ xc, RestrictArgs<T> R,
-                                                            int nblocks) {
+                                                            int nblocks, T w) {
     using VT = Vec16<T>;
     using V = typename VT::V;
     constexpr int W = VT::W;
@@ -391,7 +396,33 @@ __global__ void __launch_bounds__(kSplitThreads) k_rr_split(LevelArgs<T> F, cons
     }
     const long cb = (long)I * C.sx + (long)J * C.sy;
     coarse_store(bc, cb, C.hz, K0, out);
-    coarse_store(xc, cb, C.hz, K0, zero);
+    if (RED) {
+        T xr[W];
+        const T c2x = C.c2[0][I], c1x = C.c1[0][I];
+        const T c2y = DIM == 3 ? C.c2[1][J] : T(0), c1y = DIM == 3 ? C.c1[1][J] : T(0);
+#pragma unroll
+        for (int e = 0; e < W; ++e) {
+            const int K = K0 + e;
+            if (((I + J + K) & 1) == 0) {
+                Stencil<T, DIM> s;
+                s.c = T(0);
+                s.xE = bc_value(C.bc_hi[0], T(0), T(0), T(0));
+                s.xW = bc_value(C.bc_lo[0], T(0), T(0), T(0));
+                if (DIM == 3) {
+                    s.yN = bc_value(C.bc_hi[1], T(0), T(0), T(0));
+                    s.yS = bc_value(C.bc_lo[1], T(0), T(0), T(0));
+                }
+                s.zT = bc_value(C.bc_hi[2], T(0), T(0), T(0));
+                s.zB = bc_value(C.bc_lo[2], T(0), T(0), T(0));
+                xr[e] = sor_update<T, DIM>(s, c2x, c1x, c2y, c1y, C.c2[2][K], C.c1[2][K], out[e], w);
+            } else {
+                xr[e] = T(0);
+            }
+        }
+        coarse_store(xc, cb, C.hz, K0, xr);
+    } else {
+        coarse_store(xc, cb, C.hz, K0, zero);
+    }
 }
 
 // ---------------------------------------------------------------------------

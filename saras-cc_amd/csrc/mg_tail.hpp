@@ -28,6 +28,7 @@ struct TailArgs {
     T w;
     double tol;
     int zero_bc_first;                   // 0 only when S is the finest level
+    int wave_cells;                      // levels of at most this many padded cells run on wave 0 alone
     int *iters_out;
 };
 
@@ -50,7 +51,8 @@ __device__ __forceinline__ T lds_residual(const T *X, const T *B, const LevelArg
     return residual<T, DIM>(s, L.c2[0][i], L.c1[0][i], c2y, c1y, L.c2[2][k], L.c1[2][k], B[t]);
 }
 
-// levels of at most this many padded cells run their loops on wave 0 alone
+// default of TailArgs::wave_cells: levels of at most this many padded cells
+// run their loops on wave 0 alone, without barriers
 constexpr int kTailWaveCells = 512;
 
 template <typename T, int DIM, bool WAVE>
@@ -162,7 +164,7 @@ __global__ void __launch_bounds__(1024) k_tail(TailArgs<T> A) {
     // single-level hierarchy (poisson.cc:228, no coarse solve)
     // one level's smoothing or coarsest solve, on wave 0 alone when it is small
     auto smooth_lv = [&](int l, T *XL, const T *BL, int count) {
-        if (padded(l) <= kTailWaveCells) {
+        if (padded(l) <= A.wave_cells) {
             if (threadIdx.x < 64) tail_smooth<T, DIM, true>(XL, BL, SL[l - S], count, A.w);
             __syncthreads();
         } else {
@@ -171,7 +173,7 @@ __global__ void __launch_bounds__(1024) k_tail(TailArgs<T> A) {
     };
     auto solve_lv = [&](int l, T *XL, const T *BL) {
         int it = 0;
-        if (padded(l) <= kTailWaveCells) {
+        if (padded(l) <= A.wave_cells) {
             if (threadIdx.x < 64) it = tail_solve<T, DIM, true>(XL, BL, SL[l - S], A.w, A.tol, A.max_count, red);
             __syncthreads();
         } else {

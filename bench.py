@@ -26,7 +26,8 @@ cpu_baseline: the FP64 CPU oracle's V-cycle on the same workload.
 serving: the batched request loop on the Poisson request mix (75 % 3-D
 n^3, n log-uniform in [32, 256]; 25 % 2-D n^2, n in [64, 2048]; multiples of
 16; Dirichlet / Neumann / periodic; 8-mode Fourier RHS; tolerance 1e-8):
-solves/s with every request queued at once (saturation), p50/p99 latency of
+solves/s with every request queued at once (saturation; free slots take the
+waiting request of largest footprint first), p50/p99 latency of
 a seeded Poisson arrival stream at 70 % of that rate, and the same requests
 solved one at a time with plain FP64 solves (one slot, no graphs), with a
 solver built per request (the plain way) and with resident hierarchies; the
@@ -278,7 +279,7 @@ def serving(count=240, seed=4, slots=8, budget=96 << 30):
     for r in reqs:
         rhs.get(r)
     note(f"{count} requests, right-hand sides ready")
-    srv = S.Server(budget, slots=slots, use_graphs=1)
+    srv = S.Server(budget, slots=slots, use_graphs=1, largest_first=1)
     w = V.summarize(V.run_stream(srv, reqs, rhs, S.PREC_MIXED)[0])     # warm-up: hierarchies + graphs
     note(f"warm-up pass {w['seconds']:.2f} s")
     sat = V.summarize(V.run_stream(srv, reqs, rhs, S.PREC_MIXED)[0])

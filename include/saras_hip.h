@@ -294,6 +294,8 @@ typedef struct {
     int slots;           /* solves in flight at once (one worker thread and stream each) */
     int use_graphs;      /* replay V-cycles from hipGraphs */
     int keep_resident;   /* keep idle hierarchies for reuse (LRU, within the budget) */
+    int largest_first;   /* 1: a free slot takes the waiting request of largest footprint (shortens
+                            the makespan of a burst); 0: arrival order */
 } saras_server_params;
 
 /* One Poisson solve: poisson::mgSolve on caller host arrays (layout as

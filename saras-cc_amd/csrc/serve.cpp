@@ -100,6 +100,12 @@ struct saras_server {
     // none can run now.  On success the request's memory is accounted and, if
     // a resident handle fits it, that handle is taken from the idle list.
     bool take(Pending &out, saras_mg *&h) {
+        if (p.largest_first && queue.size() > 1) {
+            // visit the waiting requests largest footprint first (stable for
+            // equal footprints, so arrival order breaks ties)
+            std::stable_sort(queue.begin(), queue.end(),
+                             [](const Pending &a, const Pending &b) { return a.bytes > b.bytes; });
+        }
         for (auto it = queue.begin(); it != queue.end(); ++it) {
             // a resident hierarchy of the same grid and parameters
             for (auto jt = idle.begin(); jt != idle.end(); ++jt)

@@ -65,7 +65,7 @@ class NseStats(C.Structure):
 
 class ServerParams(C.Structure):
     _fields_ = [("mem_budget", C.c_size_t), ("slots", C.c_int), ("use_graphs", C.c_int),
-                ("keep_resident", C.c_int)]
+                ("keep_resident", C.c_int), ("largest_first", C.c_int)]
 
 
 class Request(C.Structure):
@@ -472,10 +472,11 @@ class Server:
     """The persistent batched request loop (saras_server_*).  Arrays handed
     to submit() are kept referenced until their completion is polled."""
 
-    def __init__(self, mem_budget, slots, use_graphs=True, keep_resident=True):
+    def __init__(self, mem_budget, slots, use_graphs=True, keep_resident=True, largest_first=False):
         sp = ServerParams()
         sp.mem_budget, sp.slots = int(mem_budget), int(slots)
         sp.use_graphs, sp.keep_resident = int(use_graphs), int(keep_resident)
+        sp.largest_first = int(largest_first)
         self.h = C.c_void_p()
         check(lib().saras_server_create(C.byref(sp), C.byref(self.h)))
         self.live = {}

@@ -46,11 +46,11 @@ def small_mix(count=24, seed=9):
     return reqs
 
 
-@pytest.mark.parametrize("graphs", [0, 1])
-def test_alone_equals_in_the_mix(graphs):
+@pytest.mark.parametrize("graphs,largest_first", [(0, 0), (1, 0), (1, 1)])
+def test_alone_equals_in_the_mix(graphs, largest_first):
     reqs = small_mix()
     rhs = V.RhsCache()
-    mix = S.Server(mem_budget=16 << 30, slots=8, use_graphs=graphs)
+    mix = S.Server(mem_budget=16 << 30, slots=8, use_graphs=graphs, largest_first=largest_first)
     recs, sols = V.run_stream(mix, reqs, rhs, S.PREC_MIXED, collect={r["tag"] for r in reqs})
     mix.close()
     assert len(recs) == len(reqs)

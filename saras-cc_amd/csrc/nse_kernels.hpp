@@ -148,6 +148,29 @@ struct PadField {
     }
 };
 
+// View of a field for a row at least two rows from the x and y faces: every
+// x or y neighbour a stencil reads lies in the core and is read straight from
+// memory (the value the field's own rule would return, without its wrap and
+// wall tests); z neighbours still go through the field's rule.  The test is
+// per row, so a wavefront (one row segment) takes one path.
+template <class FB>
+struct FastXY {
+    FB base;
+    __device__ __forceinline__ double at(const Geom &G, int i, int j, int k) const { return base.at(G, i, j, k); }
+    __device__ __forceinline__ double nb(const Geom &G, int ax, int i, int j, int k, int off) const {
+        if (ax == 0) return base.at(G, i + off, j, k);
+        if (ax == 1) return base.at(G, i, j + off, k);
+        return base.nb(G, 2, i, j, k, off);
+    }
+};
+template <class FB>
+__device__ __forceinline__ FastXY<FB> fast_of(const FB &c) { return FastXY<FB>{c}; }
+// rows whose x/y stencils (reach 2) stay in the core
+__device__ __forceinline__ bool interior2(const Geom &G, int i, int j, int k) {
+    (void)k;
+    return i >= 2 && i < G.n[0] - 2 && j >= 2 && j < G.n[1] - 2;
+}
+
 // derivative::calcDerivative1_x/y/z (derivative.cc:104-192)
 template <class F>
 __device__ __forceinline__ double d1(const Geom &G, const F &A, int ax, int i, int j, int k) {
@@ -304,18 +327,14 @@ struct StageCoef {
 // + gamma H) dt + u, with H the new non-linear term, written back for the
 // next sub-stage.  One launch per component keeps the register footprint of
 // the Morinishi stencils spill-free.
-template <int C>
-__global__ void __launch_bounds__(256) k_rhs_v(Geom G, StageCoef c, CoreField V0, CoreField V1, CoreField V2,
-                                               CoreField T, CoreField P, double *__restrict__ H,
-                                               double *__restrict__ R, int les, PerField Sa, PerField Sb,
-                                               PerField Sc) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y;
-    const int i = blockIdx.z;
-    if (k >= G.n[2]) return;
+template <int C, class F>
+__device__ __forceinline__ void rhs_v_cell(const Geom &G, const StageCoef &c, const F &V0, const F &V1, const F &V2,
+                                           const F &T, const F &P, double *__restrict__ H, double *__restrict__ R,
+                                           int les, const PerField &Sa, const PerField &Sb, const PerField &Sc, int i,
+                                           int j, int k) {
     const long idx = (long)i * G.sx + (long)j * G.sy + k;
-    const CoreField V[3] = {V0, V1, V2};
-    const CoreField &U = V[C];
+    const F V[3] = {V0, V1, V2};
+    const F &U = V[C];
     double nse = 0.0;
     if (c.stage > 0) nse += c.zeta * H[idx];
     const double diff = laplacian(G, U, i, j, k);
@@ -334,15 +353,29 @@ __global__ void __launch_bounds__(256) k_rhs_v(Geom G, StageCoef c, CoreField V0
     R[idx] = nse;
 }
 
-// the same for the scalar (tmpRHS / tempSF); its forcing term is zero
-__global__ void __launch_bounds__(256) k_rhs_t(Geom G, StageCoef c, CoreField V0, CoreField V1, CoreField V2,
-                                               CoreField T, double *__restrict__ H, double *__restrict__ R) {
+template <int C>
+__global__ void __launch_bounds__(256) k_rhs_v(Geom G, StageCoef c, CoreField V0, CoreField V1, CoreField V2,
+                                               CoreField T, CoreField P, double *__restrict__ H,
+                                               double *__restrict__ R, int les, PerField Sa, PerField Sb,
+                                               PerField Sc) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
     const int i = blockIdx.z;
     if (k >= G.n[2]) return;
+    if (interior2(G, i, j, k))
+        rhs_v_cell<C>(G, c, fast_of(V0), fast_of(V1), fast_of(V2), fast_of(T), fast_of(P), H, R, les, Sa, Sb, Sc, i,
+                      j, k);
+    else
+        rhs_v_cell<C>(G, c, V0, V1, V2, T, P, H, R, les, Sa, Sb, Sc, i, j, k);
+}
+
+// the same for the scalar (tmpRHS / tempSF); its forcing term is zero
+template <class F>
+__device__ __forceinline__ void rhs_t_cell(const Geom &G, const StageCoef &c, const F &V0, const F &V1, const F &V2,
+                                           const F &T, double *__restrict__ H, double *__restrict__ R, int i, int j,
+                                           int k) {
     const long idx = (long)i * G.sx + (long)j * G.sy + k;
-    const CoreField V[3] = {V0, V1, V2};
+    const F V[3] = {V0, V1, V2};
     double tr = 0.0;
     if (c.stage > 0) tr += c.zeta * H[idx];
     const double diff = laplacian(G, T, i, j, k);
@@ -356,20 +389,26 @@ __global__ void __launch_bounds__(256) k_rhs_t(Geom G, StageCoef c, CoreField V0
     R[idx] = tr;
 }
 
-// The explicit part of the Euler-CN step for velocity component C
-// (eulerCN_d3.cc:181-226): R = ((nu/2) lap u - H - g T - dP/dx_C) dt + u,
-// accumulated in one value in the reference's order.
-template <int C>
-__global__ void __launch_bounds__(256) k_euler_v(Geom G, double k_nu, double nu, double dt, double buoy, int has_T,
-                                                 int nlscheme, CoreField V0, CoreField V1, CoreField V2,
-                                                 CoreField T, CoreField P, double *__restrict__ R, int les,
-                                                 PerField Sa, PerField Sb, PerField Sc) {
+__global__ void __launch_bounds__(256) k_rhs_t(Geom G, StageCoef c, CoreField V0, CoreField V1, CoreField V2,
+                                               CoreField T, double *__restrict__ H, double *__restrict__ R) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
     const int i = blockIdx.z;
     if (k >= G.n[2]) return;
-    const CoreField V[3] = {V0, V1, V2};
-    const CoreField &U = V[C];
+    if (interior2(G, i, j, k)) rhs_t_cell(G, c, fast_of(V0), fast_of(V1), fast_of(V2), fast_of(T), H, R, i, j, k);
+    else rhs_t_cell(G, c, V0, V1, V2, T, H, R, i, j, k);
+}
+
+// The explicit part of the Euler-CN step for velocity component C
+// (eulerCN_d3.cc:181-226): R = ((nu/2) lap u - H - g T - dP/dx_C) dt + u,
+// accumulated in one value in the reference's order.
+template <int C, class F>
+__device__ __forceinline__ void euler_v_cell(const Geom &G, double k_nu, double nu, double dt, double buoy, int has_T,
+                                             int nlscheme, const F &V0, const F &V1, const F &V2, const F &T,
+                                             const F &P, double *__restrict__ R, int les, const PerField &Sa,
+                                             const PerField &Sb, const PerField &Sc, int i, int j, int k) {
+    const F V[3] = {V0, V1, V2};
+    const F &U = V[C];
     double r = laplacian(G, U, i, j, k);
     r *= k_nu;
     r = nlin_acc(G, nlscheme, V, U, i, j, k, r, nu);
@@ -381,20 +420,46 @@ __global__ void __launch_bounds__(256) k_euler_v(Geom G, double k_nu, double nu,
     R[(long)i * G.sx + (long)j * G.sy + k] = r;
 }
 
-// ... and for the scalar (eulerCN_d3.cc:188-232); its forcing is zero
-__global__ void __launch_bounds__(256) k_euler_t(Geom G, double k_kappa, double kappa, double dt, int nlscheme, CoreField V0,
-                                                 CoreField V1, CoreField V2, CoreField T, double *__restrict__ R) {
+template <int C>
+__global__ void __launch_bounds__(256) k_euler_v(Geom G, double k_nu, double nu, double dt, double buoy, int has_T,
+                                                 int nlscheme, CoreField V0, CoreField V1, CoreField V2,
+                                                 CoreField T, CoreField P, double *__restrict__ R, int les,
+                                                 PerField Sa, PerField Sb, PerField Sc) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y;
     const int i = blockIdx.z;
     if (k >= G.n[2]) return;
-    const CoreField V[3] = {V0, V1, V2};
+    if (interior2(G, i, j, k))
+        euler_v_cell<C>(G, k_nu, nu, dt, buoy, has_T, nlscheme, fast_of(V0), fast_of(V1), fast_of(V2), fast_of(T),
+                        fast_of(P), R, les, Sa, Sb, Sc, i, j, k);
+    else
+        euler_v_cell<C>(G, k_nu, nu, dt, buoy, has_T, nlscheme, V0, V1, V2, T, P, R, les, Sa, Sb, Sc, i, j, k);
+}
+
+// ... and for the scalar (eulerCN_d3.cc:188-232); its forcing is zero
+template <class F>
+__device__ __forceinline__ void euler_t_cell(const Geom &G, double k_kappa, double kappa, double dt, int nlscheme,
+                                             const F &V0, const F &V1, const F &V2, const F &T, double *__restrict__ R,
+                                             int i, int j, int k) {
+    const F V[3] = {V0, V1, V2};
     double r = laplacian(G, T, i, j, k);
     r *= k_kappa;
     r = nlin_acc(G, nlscheme, V, T, i, j, k, r, kappa);
     r *= dt;
     r += T.at(G, i, j, k);
     R[(long)i * G.sx + (long)j * G.sy + k] = r;
+}
+
+__global__ void __launch_bounds__(256) k_euler_t(Geom G, double k_kappa, double kappa, double dt, int nlscheme, CoreField V0,
+                                                 CoreField V1, CoreField V2, CoreField T, double *__restrict__ R) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    const int i = blockIdx.z;
+    if (k >= G.n[2]) return;
+    if (interior2(G, i, j, k))
+        euler_t_cell(G, k_kappa, kappa, dt, nlscheme, fast_of(V0), fast_of(V1), fast_of(V2), fast_of(T), R, i, j, k);
+    else
+        euler_t_cell(G, k_kappa, kappa, dt, nlscheme, V0, V1, V2, T, R, i, j, k);
 }
 
 // ---- stretched spiral vortex sub-grid stress (lib/les/spiral.cc) ----------
@@ -639,15 +704,12 @@ __global__ void __launch_bounds__(1024) k_max_final(const double *__restrict__ p
 // One Jacobi iteration of the Crank-Nicolson solve (predictors.cc:59-145):
 // out = update(F).  With `slot`, also the residual max|F - beta dt nu lap2 F - rhs|
 // of the input F (the reference's check after its previous update).
-template <bool RES>
-__global__ void __launch_bounds__(256) k_jacobi(Geom G, CoreField F, const double *__restrict__ rhs,
-                                                double *__restrict__ out, double dt, double diff, double beta,
-                                                double *__restrict__ part) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y;
-    const int i = blockIdx.z;
+template <bool RES, class FF>
+__device__ __forceinline__ double jacobi_cell(const Geom &G, const FF &F, const double *__restrict__ rhs,
+                                              double *__restrict__ out, double dt, double diff, double beta, int i,
+                                              int j, int k) {
     double res = 0.0;
-    if (k < G.n[2]) {
+    {
         const long idx = (long)i * G.sx + (long)j * G.sy + k;
         const double C = F.at(G, i, j, k);
         const double E = F.nb(G, 0, i, j, k, 1), W = F.nb(G, 0, i, j, k, -1);
@@ -670,6 +732,21 @@ __global__ void __launch_bounds__(256) k_jacobi(Geom G, CoreField F, const doubl
             res = fabs(t - r);
         }
     }
+    return res;
+}
+
+template <bool RES>
+__global__ void __launch_bounds__(256) k_jacobi(Geom G, CoreField F, const double *__restrict__ rhs,
+                                                double *__restrict__ out, double dt, double diff, double beta,
+                                                double *__restrict__ part) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    const int i = blockIdx.z;
+    double res = 0.0;
+    if (k < G.n[2]) {
+        if (interior2(G, i, j, k)) res = jacobi_cell<RES>(G, fast_of(F), rhs, out, dt, diff, beta, i, j, k);
+        else res = jacobi_cell<RES>(G, F, rhs, out, dt, diff, beta, i, j, k);
+    }
     if (RES) block_max_store(res, part, ((long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x);
 }
 
@@ -682,9 +759,15 @@ __global__ void __launch_bounds__(256) k_div(Geom G, CoreField V0, CoreField V1,
     const int i = blockIdx.z;
     if (k >= G.n[2]) return;
     double d = 0.0;
-    d += d1(G, V0, 0, i, j, k);
-    d += d1(G, V1, 1, i, j, k);
-    d += d1(G, V2, 2, i, j, k);
+    if (interior2(G, i, j, k)) {
+        d += d1(G, fast_of(V0), 0, i, j, k);
+        d += d1(G, fast_of(V1), 1, i, j, k);
+        d += d1(G, fast_of(V2), 2, i, j, k);
+    } else {
+        d += d1(G, V0, 0, i, j, k);
+        d += d1(G, V1, 1, i, j, k);
+        d += d1(G, V2, 2, i, j, k);
+    }
     b[(long)i * bsx + (long)j * bsy + (bhz ? (long)(k & 1) * bhz + (k >> 1) : (long)k)] = d * sc;
 }
 
@@ -699,7 +782,17 @@ __global__ void __launch_bounds__(256) k_correct(Geom G, PF Pp, double *__restri
     if (k >= G.n[2]) return;
     const long idx = (long)i * G.sx + (long)j * G.sy + k;
     P[idx] += Pp.at(G, i, j, k);
-    double g0 = d1(G, Pp, 0, i, j, k), g1 = d1(G, Pp, 1, i, j, k), g2 = d1(G, Pp, 2, i, j, k);
+    double g0, g1, g2;
+    if (interior2(G, i, j, k)) {
+        const auto Pf = fast_of(Pp);
+        g0 = d1(G, Pf, 0, i, j, k);
+        g1 = d1(G, Pf, 1, i, j, k);
+        g2 = d1(G, Pf, 2, i, j, k);
+    } else {
+        g0 = d1(G, Pp, 0, i, j, k);
+        g1 = d1(G, Pp, 1, i, j, k);
+        g2 = d1(G, Pp, 2, i, j, k);
+    }
     g0 *= pc;
     g1 *= pc;
     g2 *= pc;

@@ -558,25 +558,50 @@ __device__ bool sgs_cell(const Geom &G, const CoreField *V, double nu, int i, in
     const double u0 = V[0].at(G, i, j, k), v0 = V[1].at(G, i, j, k), w0 = V[2].at(G, i, j, k);
     double F2 = 0.0, Qd = 0.0;
     int cnt = 0;
+#pragma unroll 1
     for (int di = -1; di <= 1; ++di) {
         const int ii = i + di < 0 ? i + di + G.n[0] : (i + di >= G.n[0] ? i + di - G.n[0] : i + di);
-        const double dx = G.xc[0][i + di] - G.xc[0][i];
         for (int dj = -1; dj <= 1; ++dj) {
             const int jj = j + dj < 0 ? j + dj + G.n[1] : (j + dj >= G.n[1] ? j + dj - G.n[1] : j + dj);
-            const double dy = G.xc[1][j + dj] - G.xc[1][j];
             for (int dk = -1; dk <= 1; ++dk) {
                 if (!(di || dj || dk)) continue;
                 const int kk = k + dk < 0 ? k + dk + G.n[2] : (k + dk >= G.n[2] ? k + dk - G.n[2] : k + dk);
-                const double dz = G.xc[2][k + dk] - G.xc[2][k];
                 const double du = V[0].at(G, ii, jj, kk) - u0, dv = V[1].at(G, ii, jj, kk) - v0;
                 const double dw = V[2].at(G, ii, jj, kk) - w0;
                 F2 += du * du + dv * dv + dw * dw;
-                const double dx2 = dx * dx + dy * dy + dz * dz;
-                const double dxe = dx * e[0] + dy * e[1] + dz * e[2];
-                const double d = sqrt(dx2 - dxe * dxe) / del;
-                Qd += sf_integral(d);
                 ++cnt;
             }
+        }
+    }
+    // sfIntegral of each neighbour offset's distance from the vortex axis.  The
+    // distance is even in the offset, so where the spacing on the two sides of
+    // the cell agrees to rounding (uniform axes) each of the 13 offsets of one
+    // half stands for its mirror image too; elsewhere all 26 are evaluated.
+    const double dxp = G.xc[0][i + 1] - G.xc[0][i], dxm = G.xc[0][i - 1] - G.xc[0][i];
+    const double dyp = G.xc[1][j + 1] - G.xc[1][j], dym = G.xc[1][j - 1] - G.xc[1][j];
+    const double dzp = G.xc[2][k + 1] - G.xc[2][k], dzm = G.xc[2][k - 1] - G.xc[2][k];
+    auto close = [](double a, double b) { return fabs(a + b) <= 1e-12 * fabs(a); };
+    const bool mirror = close(dxp, dxm) && close(dyp, dym) && close(dzp, dzm);
+    auto sf_at = [&](int di, int dj, int dk) {
+        const double dx = di < 0 ? dxm : (di > 0 ? dxp : 0.0);
+        const double dy = dj < 0 ? dym : (dj > 0 ? dyp : 0.0);
+        const double dz = dk < 0 ? dzm : (dk > 0 ? dzp : 0.0);
+        const double dx2 = dx * dx + dy * dy + dz * dz;
+        const double dxe = dx * e[0] + dy * e[1] + dz * e[2];
+        return sf_integral(sqrt(dx2 - dxe * dxe) / del);
+    };
+    if (mirror) {
+        // the 13 offsets before (0, 0, 0) in (di, dj, dk) order
+#pragma unroll 1
+        for (int t = 0; t < 13; ++t) {
+            const int di = t / 9 - 1, dj = (t / 3) % 3 - 1, dk = t % 3 - 1;
+            Qd += 2.0 * sf_at(di, dj, dk);
+        }
+    } else {
+#pragma unroll 1
+        for (int t = 0; t < 27; ++t) {
+            if (t == 13) continue;
+            Qd += sf_at(t / 9 - 1, (t / 3) % 3 - 1, t % 3 - 1);
         }
     }
     F2 /= (double)cnt;

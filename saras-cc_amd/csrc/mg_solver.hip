@@ -435,7 +435,7 @@ struct Ops {
         }
         const long n = (long)C.n[0] * C.n[1] * C.n[2];
         if (F.hz) {
-            constexpr int W = saras::Vec16<T>::W;
+            constexpr int W = saras::rr_width<T>();
             const long total = (long)C.n[0] * C.n[1] * (F.n[2] / 2 / W);
             const int nb = (int)((total + saras::kSplitThreads - 1) / saras::kSplitThreads);
             const int grid = (nb + saras::kXcds - 1) / saras::kXcds * saras::kXcds;

@@ -78,6 +78,37 @@ __device__ __forceinline__ void coarse_store(T *__restrict__ dst, long base, lon
 }
 
 
+// N consecutive values of a row at p: one vector load where N fills 16 bytes
+template <typename T, int N>
+__device__ __forceinline__ void load_n(const T *__restrict__ p, T (&out)[N]) {
+    if constexpr (N == Vec16<T>::W) {
+        const typename Vec16<T>::V v = *reinterpret_cast<const typename Vec16<T>::V *>(p);
+#pragma unroll
+        for (int q = 0; q < N; ++q) out[q] = Vec16<T>::get(v, q);
+    } else if constexpr (N == 2 && sizeof(T) == 4) {
+        const float2 v = *reinterpret_cast<const float2 *>(p);
+        out[0] = v.x;
+        out[1] = v.y;
+    } else {
+#pragma unroll
+        for (int q = 0; q < N; ++q) out[q] = p[q];
+    }
+}
+
+// N consecutive coarse cells K0.. of a coarse row in the coarse level's layout
+template <typename T, int N>
+__device__ __forceinline__ void coarse_store_n(T *__restrict__ dst, long base, long chz, int K0, const T (&in)[N]) {
+    if (chz == 0 && N == Vec16<T>::W) {
+        *reinterpret_cast<typename Vec16<T>::V *>(dst + base + K0) = Vec16<T>::make(in);
+    } else {
+#pragma unroll
+        for (int q = 0; q < N; ++q) {
+            const int K = K0 + q;
+            dst[base + (chz ? (long)(K & 1) * chz + (K >> 1) : (long)K)] = in[q];
+        }
+    }
+}
+
 // one coarse cell (I, J, K) of a level in its own layout
 template <typename T>
 __device__ __forceinline__ T coarse_at(const T *__restrict__ e, long base, long chz, int K) {
@@ -244,14 +275,19 @@ __global__ void __launch_bounds__(kSplitThreads) k_rb_split(LevelArgs<T> L, T *_
 // the same sor_update expression, evaluated here instead of in a launch of
 // its own.
 // ---------------------------------------------------------------------------
+// RW coarse cells per lane: 4 in FP32; 1 in FP64, where two per lane need
+// more than 256 registers.
+template <typename T>
+constexpr int rr_width() {
+    return sizeof(T) == 4 ? 4 : 1;
+}
+
 template <typename T, int DIM, bool RED = false>
 __global__ void __launch_bounds__(kSplitThreads) k_rr_split(LevelArgs<T> F, const T *__restrict__ x,
                                                             const T *__restrict__ b, LevelArgs<T> C,
                                                             T *__restrict__ bc, T *__restrict__ xc, RestrictArgs<T> R,
                                                             int nblocks, T w) {
-    using VT = Vec16<T>;
-    using V = typename VT::V;
-    constexpr int W = VT::W;
+    constexpr int W = rr_width<T>();
     const int per = gridDim.x / kXcds;
     const int logical = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
     if (logical >= nblocks) return;
@@ -267,11 +303,7 @@ __global__ void __launch_bounds__(kSplitThreads) k_rr_split(LevelArgs<T> F, cons
     const int K0 = v * W;
     const long hz = F.hz;
     const int lane = threadIdx.x & 63;
-    auto ldv = [&](const T *p, T (&out)[W]) {
-        const V vv = *reinterpret_cast<const V *>(p);
-#pragma unroll
-        for (int e = 0; e < W; ++e) out[e] = VT::get(vv, e);
-    };
+    auto ldv = [&](const T *p, T (&out)[W]) { load_n<T, W>(p, out); };
     // residuals of the children: [di][dj][dk][e], dk = 0 the even cell 2K, 1 the odd cell 2K+1
     T res[2][DIM == 3 ? 2 : 1][2][W];
     const T c2e0 = T(0);
@@ -395,7 +427,7 @@ __global__ void __launch_bounds__(kSplitThreads) k_rr_split(LevelArgs<T> F, cons
         }
     }
     const long cb = (long)I * C.sx + (long)J * C.sy;
-    coarse_store(bc, cb, C.hz, K0, out);
+    coarse_store_n<T, W>(bc, cb, C.hz, K0, out);
     if (RED) {
         T xr[W];
         const T c2x = C.c2[0][I], c1x = C.c1[0][I];
@@ -419,9 +451,9 @@ __global__ void __launch_bounds__(kSplitThreads) k_rr_split(LevelArgs<T> F, cons
                 xr[e] = T(0);
             }
         }
-        coarse_store(xc, cb, C.hz, K0, xr);
+        coarse_store_n<T, W>(xc, cb, C.hz, K0, xr);
     } else {
-        coarse_store(xc, cb, C.hz, K0, zero);
+        coarse_store_n<T, W>(xc, cb, C.hz, K0, zero);
     }
 }
 

@@ -279,7 +279,7 @@ __global__ void __launch_bounds__(kSplitThreads) k_rb_split(LevelArgs<T> L, T *_
 // more than 256 registers.
 template <typename T>
 constexpr int rr_width() {
-    return sizeof(T) == 4 ? 4 : 1;
+    return sizeof(T) == 4 ? 2 : 1;
 }
 
 template <typename T, int DIM, bool RED = false>

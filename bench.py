@@ -334,6 +334,39 @@ def cpu_baseline(nu1, nu2, shapes, budget_s=12.0):
                        f"({threads} OpenMP threads), {per * 1e3:.1f} ms per cycle")
 
 
+def cpu_serving_baseline(count=240, seed=4, budget_s=15.0):
+    """The serving mix solved one request at a time by the FP64 CPU oracle
+    (same grids, parameters and right-hand sides as the GPU legs), over the
+    requests that fit a bounded time budget, in stream order."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from saras_amd import serving as V
+
+    threads = min(16, os.cpu_count() or 1)
+    O.set_threads(threads)
+    reqs = V.draw_requests(count, seed=seed)
+    rhs = V.RhsCache()
+    t0 = time.perf_counter()
+    done = 0
+    for r in reqs:
+        g = V.request_grid(r)
+        p = V.request_params(r, S.PREC_FP64)
+        go = O.grid_desc(tuple(g.n), dim=g.dim, periodic=tuple(g.periodic), pad=1)
+        po = O.mg_params(vc_count=p.vc_count, vc_tol=p.vc_tol, solve_flag=p.solve_flag, mg_tol=p.mg_tol,
+                         pre=p.pre_smooth, post=p.post_smooth, res_type=p.res_type, bc=p.bc)
+        b = rhs.get(r)
+        mg = O.Multigrid(go, po)
+        mg.solve(np.zeros_like(b), b, 1)
+        del mg
+        done += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    el = time.perf_counter() - t0
+    return dict(value=round(done / el, 3), unit="solves/s", cores=threads, kind="port",
+                sample=f"the first {done} requests of the serving stream, one at a time, FP64 oracle "
+                       f"({threads} OpenMP threads), {el:.1f} s")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -424,6 +457,8 @@ def main():
         out["les_384_step"] = les_step(3, 6)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(nu1, nu2, f64["shapes"])
+        if not args.no_serve:
+            out["cpu_baseline"]["serving"] = cpu_serving_baseline(count=args.requests)
     print(json.dumps(out))
 
 

@@ -158,13 +158,13 @@ __global__ void __launch_bounds__(kSplitThreads) k_rb_split(LevelArgs<T> L, T *_
     }
     // own coarse row of the prolongation: cells m0.. of both halves map to K = m0..
     const long ebase = (long)(i >> 1) * esx + (long)(j >> 1) * esy;
+    T evo[W];
     if (PRO) {
-        T ev[W];
-        coarse_vec(e, ebase, ehz, m0, ev);
+        coarse_vec(e, ebase, ehz, m0, evo);
 #pragma unroll
         for (int q = 0; q < W; ++q) {
-            c[q] = ev[q] + c[q];
-            if (colour == 0) o[q] = ev[q] + o[q];
+            c[q] = evo[q] + c[q];
+            if (colour == 0) o[q] = evo[q] + o[q];
         }
     }
     // x and y neighbour rows: whole vectors, or the wall rule per cell
@@ -177,10 +177,16 @@ __global__ void __launch_bounds__(kSplitThreads) k_rb_split(LevelArgs<T> L, T *_
 #pragma unroll
             for (int q = 0; q < W; ++q) out[q] = VT::get(nv, q);
             if (PRO && colour == 0) {
-                T ev[W];
-                coarse_vec(e, (long)(nbr_i >> 1) * esx + (long)(nbr_j >> 1) * esy, ehz, m0, ev);
+                // a neighbour row in the same coarse row shares this lane's correction
+                if ((nbr_i >> 1) == (i >> 1) && (nbr_j >> 1) == (j >> 1)) {
 #pragma unroll
-                for (int q = 0; q < W; ++q) out[q] = ev[q] + out[q];
+                    for (int q = 0; q < W; ++q) out[q] = evo[q] + out[q];
+                } else {
+                    T ev[W];
+                    coarse_vec(e, (long)(nbr_i >> 1) * esx + (long)(nbr_j >> 1) * esy, ehz, m0, ev);
+#pragma unroll
+                    for (int q = 0; q < W; ++q) out[q] = ev[q] + out[q];
+                }
             }
         } else {
 #pragma unroll

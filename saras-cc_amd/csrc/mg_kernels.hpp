@@ -56,6 +56,27 @@ __device__ __forceinline__ long kof(const LevelArgs<T> &L, int k) {
     return L.hz ? (long)(k & 1) * L.hz + (k >> 1) : (long)k;
 }
 
+// t = q * n + rem with 0 <= rem < n: one 32-bit unsigned division whenever t
+// fits 32 bits (always, below a few billion cells), the 64-bit sequence
+// otherwise -- a 64-bit division is several times the instructions
+__device__ __forceinline__ int split_index(long t, int n, long &q) {
+    if ((unsigned long)t <= 0xffffffffUL) {
+        const unsigned ut = (unsigned)t, un = (unsigned)n, uq = ut / un;
+        q = (long)uq;
+        return (int)(ut - uq * un);
+    }
+    q = t / n;
+    return (int)(t - q * n);
+}
+
+// cell (i, j, k) of the row-major index t over an (., ny, nz) box
+__device__ __forceinline__ void cell_of(long t, int ny, int nz, int &i, int &j, int &k) {
+    long r, q;
+    k = split_index(t, nz, r);
+    j = split_index(r, ny, q);
+    i = (int)q;
+}
+
 // ---------------------------------------------------------------------------
 // neighbour values with the boundary rule applied
 // ---------------------------------------------------------------------------
@@ -179,10 +200,8 @@ __global__ void __launch_bounds__(256) k_rb_pass(LevelArgs<T> L, T *__restrict__
     const long total = (long)nx * ny * npair;
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= total) return;
-    const int m = (int)(t % npair);
-    const long r = t / npair;
-    const int j = (int)(r % ny);
-    const int i = (int)(r / ny);
+    int i, j, m;
+    cell_of(t, ny, npair, i, j, m);
     const int k = 2 * m + ((i + j + colour) & 1);
     if (k >= nz) return;
     const long idx = (long)i * L.sx + (long)j * L.sy + kof(L, k);
@@ -213,10 +232,8 @@ __global__ void __launch_bounds__(256) k_resid_restrict(LevelArgs<T> F, const T 
     const long total = (long)cx * cy * cz;
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= total) return;
-    const int K = (int)(t % cz);
-    const long r = t / cz;
-    const int J = (int)(r % cy);
-    const int I = (int)(r / cy);
+    int I, J, K;
+    cell_of(t, cy, cz, I, J, K);
     T res[2][2][2];
     const int jmax = DIM == 3 ? 2 : 1;
     for (int di = 0; di < 2; ++di)
@@ -257,10 +274,8 @@ __global__ void __launch_bounds__(256) k_prolong_correct(int nx, int ny, int nz,
     const long total = (long)nx * ny * nz;
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= total) return;
-    const int k = (int)(t % nz);
-    const long r = t / nz;
-    const int j = (int)(r % ny);
-    const int i = (int)(r / ny);
+    int i, j, k;
+    cell_of(t, ny, nz, i, j, k);
     const long idx = (long)i * sx + (long)j * sy + (hz ? (long)(k & 1) * hz + (k >> 1) : (long)k);
     const int kc = k >> 1;
     const T ev = e[(long)(i >> 1) * csx + (long)(j >> 1) * csy + (chz ? (long)(kc & 1) * chz + (kc >> 1) : (long)kc)];
@@ -422,76 +437,63 @@ __device__ __forceinline__ long cidx(int i, int j, int k, int PY, int PZ) {
     return DIM == 3 ? ((long)(i + 1) * PY + (j + 1)) * PZ + (k + 1) : (long)(i + 1) * PZ + (k + 1);
 }
 
-template <typename T, int DIM, bool WAVE = false>
-__device__ void coarse_wrap(T *X, const int *n, const int *per, int PY, int PZ) {
-    const int nx = n[0], ny = n[1], nz = n[2];
-    const int tid = threadIdx.x, nt = lthreads<WAVE>();
-    const int jlo = DIM == 3 ? -1 : 0, jn = DIM == 3 ? ny + 2 : 1;
-    if (per[0]) {
-        for (int q = tid; q < jn * (nz + 2); q += nt) {
-            const int j = jlo + q / (nz + 2), k = -1 + q % (nz + 2);
-            X[cidx<T, DIM>(nx, j, k, PY, PZ)] = X[cidx<T, DIM>(0, j, k, PY, PZ)];
-            X[cidx<T, DIM>(-1, j, k, PY, PZ)] = X[cidx<T, DIM>(nx - 1, j, k, PY, PZ)];
-        }
-        lsync<WAVE>();
-    }
-    if (DIM == 3 && per[1]) {
-        for (int q = tid; q < (nx + 2) * (nz + 2); q += nt) {
-            const int i = -1 + q / (nz + 2), k = -1 + q % (nz + 2);
-            X[cidx<T, DIM>(i, ny, k, PY, PZ)] = X[cidx<T, DIM>(i, 0, k, PY, PZ)];
-            X[cidx<T, DIM>(i, -1, k, PY, PZ)] = X[cidx<T, DIM>(i, ny - 1, k, PY, PZ)];
-        }
-        lsync<WAVE>();
-    }
-    if (per[2]) {
-        for (int q = tid; q < (nx + 2) * jn; q += nt) {
-            const int i = -1 + q / jn, j = jlo + q % jn;
-            X[cidx<T, DIM>(i, j, nz, PY, PZ)] = X[cidx<T, DIM>(i, j, 0, PY, PZ)];
-            X[cidx<T, DIM>(i, j, -1, PY, PZ)] = X[cidx<T, DIM>(i, j, nz - 1, PY, PZ)];
-        }
-        lsync<WAVE>();
-    }
-}
-
 template <typename T>
 __device__ __forceinline__ T wall_rule(int bc, T inner, T wv) {
     return bc == BC_NEU ? inner : (bc == BC_DIR ? -inner : T(2.0) * wv - inner);
 }
 
+// The face ghosts of an LDS level in one pass: periodic axes copy the
+// opposite core plane and, with `walls`, the other axes take the wall rule of
+// their own core cell (the updateFace + imposeBC pair of a sweep,
+// poisson_d3.cc:876-993).  Only ghosts beside core cells are written -- the
+// edges and corners of the shell are never read by the 7-point stencil -- so
+// every write reads core cells alone and one barrier covers all faces.
 template <typename T, int DIM, bool WAVE = false>
-__device__ void coarse_walls(T *X, const LevelArgs<T> &L, int PY, int PZ) {
-    const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
+__device__ void coarse_ghosts(T *X, const LevelArgs<T> &L, const int *per, bool walls, int PY, int PZ) {
+    const int nx = L.n[0], ny = DIM == 3 ? L.n[1] : 1, nz = L.n[2];
     const int tid = threadIdx.x, nt = lthreads<WAVE>();
-    const int jlo = DIM == 3 ? -1 : 0, jn = DIM == 3 ? ny + 2 : 1;
-    if (L.bc_lo[0] != BC_PER) {
-        for (int q = tid; q < jn * (nz + 2); q += nt) {
-            const int j = jlo + q / (nz + 2), k = -1 + q % (nz + 2);
-            T wv = T(0);
-            if (L.bc_lo[0] == BC_DIRV) wv = DIM == 3 ? L.wall[0][(j + 1) * (nz + 2) + k + 1] : L.wall[0][k + 1];
-            X[cidx<T, DIM>(-1, j, k, PY, PZ)] = wall_rule(L.bc_lo[0], X[cidx<T, DIM>(0, j, k, PY, PZ)], wv);
-            X[cidx<T, DIM>(nx, j, k, PY, PZ)] = wall_rule(L.bc_hi[0], X[cidx<T, DIM>(nx - 1, j, k, PY, PZ)], wv);
+    const int fx = (per[0] || walls) ? ny * nz : 0;
+    const int fy = DIM == 3 && (per[1] || walls) ? nx * nz : 0;
+    const int fz = (per[2] || walls) ? nx * ny : 0;
+    for (int q = tid; q < fx + fy + fz; q += nt) {
+        if (q < fx) {
+            const unsigned uq = (unsigned)q, j = uq / (unsigned)nz;
+            const int k = (int)(uq - j * (unsigned)nz), jj = (int)j;
+            if (per[0]) {
+                X[cidx<T, DIM>(nx, jj, k, PY, PZ)] = X[cidx<T, DIM>(0, jj, k, PY, PZ)];
+                X[cidx<T, DIM>(-1, jj, k, PY, PZ)] = X[cidx<T, DIM>(nx - 1, jj, k, PY, PZ)];
+            } else {
+                T wv = T(0);
+                if (L.bc_lo[0] == BC_DIRV) wv = DIM == 3 ? L.wall[0][(jj + 1) * (nz + 2) + k + 1] : L.wall[0][k + 1];
+                X[cidx<T, DIM>(-1, jj, k, PY, PZ)] = wall_rule(L.bc_lo[0], X[cidx<T, DIM>(0, jj, k, PY, PZ)], wv);
+                X[cidx<T, DIM>(nx, jj, k, PY, PZ)] = wall_rule(L.bc_hi[0], X[cidx<T, DIM>(nx - 1, jj, k, PY, PZ)], wv);
+            }
+        } else if (q < fx + fy) {
+            const unsigned uq = (unsigned)(q - fx), i = uq / (unsigned)nz;
+            const int k = (int)(uq - i * (unsigned)nz), ii = (int)i;
+            if (per[1]) {
+                X[cidx<T, DIM>(ii, ny, k, PY, PZ)] = X[cidx<T, DIM>(ii, 0, k, PY, PZ)];
+                X[cidx<T, DIM>(ii, -1, k, PY, PZ)] = X[cidx<T, DIM>(ii, ny - 1, k, PY, PZ)];
+            } else {
+                T wv = L.bc_lo[1] == BC_DIRV ? L.wall[1][(ii + 1) * (nz + 2) + k + 1] : T(0);
+                X[cidx<T, DIM>(ii, -1, k, PY, PZ)] = wall_rule(L.bc_lo[1], X[cidx<T, DIM>(ii, 0, k, PY, PZ)], wv);
+                X[cidx<T, DIM>(ii, ny, k, PY, PZ)] = wall_rule(L.bc_hi[1], X[cidx<T, DIM>(ii, ny - 1, k, PY, PZ)], wv);
+            }
+        } else {
+            const unsigned uq = (unsigned)(q - fx - fy), i = uq / (unsigned)ny;
+            const int j = (int)(uq - i * (unsigned)ny), ii = (int)i;
+            if (per[2]) {
+                X[cidx<T, DIM>(ii, j, nz, PY, PZ)] = X[cidx<T, DIM>(ii, j, 0, PY, PZ)];
+                X[cidx<T, DIM>(ii, j, -1, PY, PZ)] = X[cidx<T, DIM>(ii, j, nz - 1, PY, PZ)];
+            } else {
+                T wv = T(0);
+                if (L.bc_lo[2] == BC_DIRV) wv = DIM == 3 ? L.wall[2][(ii + 1) * (ny + 2) + j + 1] : L.wall[2][ii + 1];
+                X[cidx<T, DIM>(ii, j, -1, PY, PZ)] = wall_rule(L.bc_lo[2], X[cidx<T, DIM>(ii, j, 0, PY, PZ)], wv);
+                X[cidx<T, DIM>(ii, j, nz, PY, PZ)] = wall_rule(L.bc_hi[2], X[cidx<T, DIM>(ii, j, nz - 1, PY, PZ)], wv);
+            }
         }
-        lsync<WAVE>();
     }
-    if (DIM == 3 && L.bc_lo[1] != BC_PER) {
-        for (int q = tid; q < (nx + 2) * (nz + 2); q += nt) {
-            const int i = -1 + q / (nz + 2), k = -1 + q % (nz + 2);
-            T wv = L.bc_lo[1] == BC_DIRV ? L.wall[1][(i + 1) * (nz + 2) + k + 1] : T(0);
-            X[cidx<T, DIM>(i, -1, k, PY, PZ)] = wall_rule(L.bc_lo[1], X[cidx<T, DIM>(i, 0, k, PY, PZ)], wv);
-            X[cidx<T, DIM>(i, ny, k, PY, PZ)] = wall_rule(L.bc_hi[1], X[cidx<T, DIM>(i, ny - 1, k, PY, PZ)], wv);
-        }
-        lsync<WAVE>();
-    }
-    if (L.bc_lo[2] != BC_PER) {
-        for (int q = tid; q < (nx + 2) * jn; q += nt) {
-            const int i = -1 + q / jn, j = jlo + q % jn;
-            T wv = T(0);
-            if (L.bc_lo[2] == BC_DIRV) wv = DIM == 3 ? L.wall[2][(i + 1) * (ny + 2) + j + 1] : L.wall[2][i + 1];
-            X[cidx<T, DIM>(i, j, -1, PY, PZ)] = wall_rule(L.bc_lo[2], X[cidx<T, DIM>(i, j, 0, PY, PZ)], wv);
-            X[cidx<T, DIM>(i, j, nz, PY, PZ)] = wall_rule(L.bc_hi[2], X[cidx<T, DIM>(i, j, nz - 1, PY, PZ)], wv);
-        }
-        lsync<WAVE>();
-    }
+    lsync<WAVE>();
 }
 
 template <typename T, int DIM, bool WAVE = false>
@@ -499,10 +501,8 @@ __device__ void coarse_colour(T *X, const T *B, const LevelArgs<T> &L, int colou
     const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
     const long total = (long)nx * ny * nz;
     for (long t = threadIdx.x; t < total; t += lthreads<WAVE>()) {
-        const int k = (int)(t % nz);
-        const long r = t / nz;
-        const int j = (int)(r % ny);
-        const int i = (int)(r / ny);
+        int i, j, k;
+        cell_of(t, ny, nz, i, j, k);
         if (((i + j + k) & 1) != colour) continue;
         const long c = cidx<T, DIM>(i, j, k, PY, PZ);
         Stencil<T, DIM> s;
@@ -538,10 +538,8 @@ __global__ void __launch_bounds__(1024) k_coarse(LevelArgs<T> L, T *__restrict__
     for (long q = threadIdx.x; q < padded; q += blockDim.x) X[q] = T(0);
     __syncthreads();
     for (long t = threadIdx.x; t < total; t += blockDim.x) {
-        const int k = (int)(t % nz);
-        const long r = t / nz;
-        const int j = (int)(r % ny);
-        const int i = (int)(r / ny);
+        int i, j, k;
+        cell_of(t, ny, nz, i, j, k);
         const long g = (long)i * L.sx + (long)j * L.sy + kof(L, k);
         X[cidx<T, DIM>(i, j, k, PY, PZ)] = x[g];
         B[t] = b[g];
@@ -553,20 +551,17 @@ __global__ void __launch_bounds__(1024) k_coarse(LevelArgs<T> L, T *__restrict__
     for (;;) {
         if (ctl.mode == 1 && sweeps >= ctl.sweeps) break;
         // imposeBC: updateFace then walls
-        coarse_wrap<T, DIM>(X, L.n, per, PY, PZ);
-        coarse_walls<T, DIM>(X, L, PY, PZ);
+        coarse_ghosts<T, DIM>(X, L, per, true, PY, PZ);
         coarse_colour<T, DIM>(X, B, L, 0, w, PY, PZ);
-        coarse_wrap<T, DIM>(X, L.n, per, PY, PZ);
+        coarse_ghosts<T, DIM>(X, L, per, false, PY, PZ);
         coarse_colour<T, DIM>(X, B, L, 1, w, PY, PZ);
         ++sweeps;
         if (ctl.mode == 1) continue;
         // max |r| with the ghosts as they stand (poisson_d3.cc:398-415)
         double lm = -1.0e-10;
         for (long t = threadIdx.x; t < total; t += blockDim.x) {
-            const int k = (int)(t % nz);
-            const long r = t / nz;
-            const int j = (int)(r % ny);
-            const int i = (int)(r / ny);
+            int i, j, k;
+            cell_of(t, ny, nz, i, j, k);
             const long c = cidx<T, DIM>(i, j, k, PY, PZ);
             Stencil<T, DIM> s;
             s.c = X[c];
@@ -590,10 +585,8 @@ __global__ void __launch_bounds__(1024) k_coarse(LevelArgs<T> L, T *__restrict__
     }
     // final imposeBC (:431 / :245); the ghosts themselves are not kept
     for (long t = threadIdx.x; t < total; t += blockDim.x) {
-        const int k = (int)(t % nz);
-        const long r = t / nz;
-        const int j = (int)(r % ny);
-        const int i = (int)(r / ny);
+        int i, j, k;
+        cell_of(t, ny, nz, i, j, k);
         x[(long)i * L.sx + (long)j * L.sy + kof(L, k)] = X[cidx<T, DIM>(i, j, k, PY, PZ)];
     }
     if (threadIdx.x == 0 && ctl.mode == 0 && ctl.iters_out) *ctl.iters_out = iter;
@@ -608,10 +601,8 @@ __global__ void __launch_bounds__(256) k_load_core(const double *__restrict__ fu
     const long total = (long)nx * ny * nz;
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= total) return;
-    const int k = (int)(t % nz);
-    const long r = t / nz;
-    const int j = (int)(r % ny);
-    const int i = (int)(r / ny);
+    int i, j, k;
+    cell_of(t, ny, nz, i, j, k);
     const long fz = nz + 2 * pad, fy = ny + 2 * pad;
     const long f = dim == 3 ? ((long)(i + pad) * fy + (j + pad)) * fz + (k + pad) : (long)(i + pad) * fz + (k + pad);
     core[(long)i * sx + (long)j * sy + (hz ? (long)(k & 1) * hz + (k >> 1) : (long)k)] = (T)full[f];
@@ -623,10 +614,8 @@ __global__ void __launch_bounds__(256) k_store_core(double *__restrict__ full, i
     const long total = (long)nx * ny * nz;
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= total) return;
-    const int k = (int)(t % nz);
-    const long r = t / nz;
-    const int j = (int)(r % ny);
-    const int i = (int)(r / ny);
+    int i, j, k;
+    cell_of(t, ny, nz, i, j, k);
     const long fz = nz + 2 * pad, fy = ny + 2 * pad;
     const long f = dim == 3 ? ((long)(i + pad) * fy + (j + pad)) * fz + (k + pad) : (long)(i + pad) * fz + (k + pad);
     full[f] = core[(long)i * sx + (long)j * sy + (hz ? (long)(k & 1) * hz + (k >> 1) : (long)k)];

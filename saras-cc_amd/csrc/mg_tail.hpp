@@ -60,14 +60,12 @@ __device__ void tail_smooth(T *X, const T *B, const LevelArgs<T> &L, int count, 
     const int PZ = L.n[2] + 2, PY = DIM == 3 ? L.n[1] + 2 : 1;
     int per[3] = {L.bc_lo[0] == BC_PER, DIM == 3 && L.bc_lo[1] == BC_PER, L.bc_lo[2] == BC_PER};
     for (int n = 0; n < count; ++n) {
-        coarse_wrap<T, DIM, WAVE>(X, L.n, per, PY, PZ);
-        coarse_walls<T, DIM, WAVE>(X, L, PY, PZ);
+        coarse_ghosts<T, DIM, WAVE>(X, L, per, true, PY, PZ);
         coarse_colour<T, DIM, WAVE>(X, B, L, 0, w, PY, PZ);
-        coarse_wrap<T, DIM, WAVE>(X, L.n, per, PY, PZ);
+        coarse_ghosts<T, DIM, WAVE>(X, L, per, false, PY, PZ);
         coarse_colour<T, DIM, WAVE>(X, B, L, 1, w, PY, PZ);
     }
-    coarse_wrap<T, DIM, WAVE>(X, L.n, per, PY, PZ);
-    coarse_walls<T, DIM, WAVE>(X, L, PY, PZ);
+    coarse_ghosts<T, DIM, WAVE>(X, L, per, true, PY, PZ);
 }
 
 // returns the iteration count of the reference's coarsest solve
@@ -79,17 +77,14 @@ __device__ int tail_solve(T *X, const T *B, const LevelArgs<T> &L, T w, double t
     int per[3] = {L.bc_lo[0] == BC_PER, DIM == 3 && L.bc_lo[1] == BC_PER, L.bc_lo[2] == BC_PER};
     int iter = 0;
     for (;;) {
-        coarse_wrap<T, DIM, WAVE>(X, L.n, per, PY, PZ);
-        coarse_walls<T, DIM, WAVE>(X, L, PY, PZ);
+        coarse_ghosts<T, DIM, WAVE>(X, L, per, true, PY, PZ);
         coarse_colour<T, DIM, WAVE>(X, B, L, 0, w, PY, PZ);
-        coarse_wrap<T, DIM, WAVE>(X, L.n, per, PY, PZ);
+        coarse_ghosts<T, DIM, WAVE>(X, L, per, false, PY, PZ);
         coarse_colour<T, DIM, WAVE>(X, B, L, 1, w, PY, PZ);
         double lm = -1.0e-10;
         for (long t = threadIdx.x; t < total; t += lthreads<WAVE>()) {
-            const int k = (int)(t % nz);
-            const long r = t / nz;
-            const int j = (int)(r % ny);
-            const int i = (int)(r / ny);
+            int i, j, k;
+            cell_of(t, ny, nz, i, j, k);
             const double a = fabs((double)lds_residual<T, DIM>(X, B, L, i, j, k, PY, PZ));
             if (a > lm) lm = a;
         }
@@ -98,8 +93,7 @@ __device__ int tail_solve(T *X, const T *B, const LevelArgs<T> &L, T w, double t
         iter += 1;
         if (iter > max_count) break;
     }
-    coarse_wrap<T, DIM, WAVE>(X, L.n, per, PY, PZ);
-    coarse_walls<T, DIM, WAVE>(X, L, PY, PZ);
+    coarse_ghosts<T, DIM, WAVE>(X, L, per, true, PY, PZ);
     return iter;
 }
 
@@ -148,10 +142,8 @@ __global__ void __launch_bounds__(1024) k_tail(TailArgs<T> A) {
         __syncthreads();
         const long total = (long)L.n[0] * L.n[1] * L.n[2];
         for (long t = threadIdx.x; t < total; t += blockDim.x) {
-            const int k = (int)(t % L.n[2]);
-            const long r = t / L.n[2];
-            const int j = (int)(r % L.n[1]);
-            const int i = (int)(r / L.n[1]);
+            int i, j, k;
+            cell_of(t, L.n[1], L.n[2], i, j, k);
             const long g = (long)i * L.sx + (long)j * L.sy + kof(L, k);
             XS[cidx<T, DIM>(i, j, k, PY, PZ)] = A.x[g];
             B(S)[t] = A.b[g];
@@ -197,10 +189,8 @@ __global__ void __launch_bounds__(1024) k_tail(TailArgs<T> A) {
         for (long q = threadIdx.x; q < padded(l + 1); q += blockDim.x) XC[q] = T(0);
         const long ctotal = (long)C.n[0] * C.n[1] * C.n[2];
         for (long t = threadIdx.x; t < ctotal; t += blockDim.x) {
-            const int K = (int)(t % C.n[2]);
-            const long rr = t / C.n[2];
-            const int J = (int)(rr % C.n[1]);
-            const int I = (int)(rr / C.n[1]);
+            int I, J, K;
+            cell_of(t, C.n[1], C.n[2], I, J, K);
             T res[2][2][2];
             for (int di = 0; di < 2; ++di)
                 for (int dj = 0; dj < (DIM == 3 ? 2 : 1); ++dj)
@@ -239,10 +229,8 @@ __global__ void __launch_bounds__(1024) k_tail(TailArgs<T> A) {
         T *XF = X(l);
         const T *XC = X(l + 1);
         for (long t = threadIdx.x; t < total; t += blockDim.x) {
-            const int k = (int)(t % F.n[2]);
-            const long r = t / F.n[2];
-            const int j = (int)(r % F.n[1]);
-            const int i = (int)(r / F.n[1]);
+            int i, j, k;
+            cell_of(t, F.n[1], F.n[2], i, j, k);
             const long c = cidx<T, DIM>(i, j, k, fPY, fPZ);
             XF[c] = XC[cidx<T, DIM>(i >> 1, j >> 1, k >> 1, cPY, cPZ)] + XF[c];
         }
@@ -255,10 +243,8 @@ __global__ void __launch_bounds__(1024) k_tail(TailArgs<T> A) {
         const int PZ = L.n[2] + 2, PY = DIM == 3 ? L.n[1] + 2 : 1;
         const long total = (long)L.n[0] * L.n[1] * L.n[2];
         for (long t = threadIdx.x; t < total; t += blockDim.x) {
-            const int k = (int)(t % L.n[2]);
-            const long r = t / L.n[2];
-            const int j = (int)(r % L.n[1]);
-            const int i = (int)(r / L.n[1]);
+            int i, j, k;
+            cell_of(t, L.n[1], L.n[2], i, j, k);
             A.x[(long)i * L.sx + (long)j * L.sy + kof(L, k)] = X(S)[cidx<T, DIM>(i, j, k, PY, PZ)];
         }
     }

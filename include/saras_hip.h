@@ -85,7 +85,8 @@ typedef struct {
     int res_type;       /* "Residual Type": 0 L-inf, 1 L1, 2 L2 (relative) */
     int bc;             /* SARAS_BC_* */
     int precision;      /* SARAS_PREC_* */
-    int inner_cycles;   /* MIXED: FP32 V-cycles per FP64 refinement step */
+    int inner_cycles;   /* MIXED: most FP32 V-cycles per FP64 refinement step (fewer once the observed
+                           convergence factor shows fewer reach the tolerance) */
     int max_refine;     /* MIXED: cap on refinement steps */
 } saras_mg_params;
 

@@ -1022,17 +1022,33 @@ static saras_status solve_mixed(saras_mg *m, hipStream_t s, saras_mg_stats *st) 
     if (e) return e;
     int cycles = 0, refs = 0;
     const int cap = m->p.vc_count > 0 ? m->p.vc_count : (1 << 30);
+    // inner_cycles is the most FP32 cycles per refinement.  Once a refinement
+    // has shown the cycle's convergence factor rho, the next one runs only
+    // the cycles that factor needs to take the residual to half the
+    // tolerance; a short estimate costs one more refinement, never accuracy,
+    // since the FP64 residual alone decides convergence.
+    double rho = 1.0;
     while (!(res < m->p.vc_tol) && refs < m->p.max_refine && cycles < cap) {
+        int k = m->p.inner_cycles;
+        if (rho > 0.0 && rho < 0.9 && res > 0.0) {
+            const double need = std::ceil(std::log(0.5 * m->p.vc_tol / res) / std::log(rho));
+            if (need >= 1.0 && need < (double)k) k = (int)need;
+            else if (need < 1.0) k = 1;
+        }
         HIP_TRY(hipMemsetAsync(F0.x, 0, F0.elems * sizeof(float), s));
-        for (int c = 0; c < m->p.inner_cycles && cycles < cap; ++c) {
+        int ran = 0;
+        for (int c = 0; c < k && cycles < cap; ++c) {
             e = run_vcycle(m, ops);
             if (e) return e;
             ++cycles;
+            ++ran;
         }
+        const double before = res;
         e = refine_step(m, s, true, &res);
         if (e) return e;
         ++refs;
         if (!std::isfinite(res)) break;
+        rho = (ran > 0 && before > 0.0 && res < before) ? std::pow(res / before, 1.0 / ran) : 1.0;
     }
     if (m->p.solve_flag) {
         e = fetch_coarse_iters(m, s);

@@ -495,7 +495,8 @@ struct Ops {
             const size_t core = (size_t)L.n[0] * L.n[1] * L.n[2];
             const size_t pad = H.dim == 3 ? (size_t)(L.n[0] + 2) * (L.n[1] + 2) * (L.n[2] + 2)
                                           : (size_t)(L.n[0] + 2) * (L.n[2] + 2);
-            const size_t coef = 2 * ((size_t)L.n[0] + L.n[1] + L.n[2]);
+            // metric coefficients (2 per index) and restriction factors (3)
+            const size_t coef = 5 * ((size_t)L.n[0] + L.n[1] + L.n[2]);
             bytes += ((pad + 3) / 4 * 4 + (core + 3) / 4 * 4 + coef) * sizeof(T);
             if (bytes > m->tail_lds || D - l + 1 > saras::kTailMaxLevels) break;
             S = l;
@@ -529,6 +530,7 @@ struct Ops {
         for (int l = S; l <= D; ++l) {
             const DevLevel<T> &L = H.lv[l];
             off += 2 * ((size_t)L.n[0] + (H.dim == 3 ? L.n[1] : 1) + L.n[2]);
+            if (H.dim == 3 && l > S) off += 3 * ((size_t)L.n[0] + L.n[1] + L.n[2]);   // restriction factors
         }
         A.x = H.lv[S].x;
         A.b = H.lv[S].b;

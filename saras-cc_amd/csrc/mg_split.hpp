@@ -281,7 +281,7 @@ __global__ void __launch_bounds__(kSplitThreads) k_rb_split(LevelArgs<T> L, T *_
 // the same sor_update expression, evaluated here instead of in a launch of
 // its own.
 // ---------------------------------------------------------------------------
-// RW coarse cells per lane: 4 in FP32; 1 in FP64, where two per lane need
+// Coarse cells per lane: 2 in FP32 (4 measured slower); 1 in FP64, where two per lane need
 // more than 256 registers.
 template <typename T>
 constexpr int rr_width() {

@@ -226,6 +226,7 @@ struct saras_mg {
     size_t tail_lds = kTailLdsDefault;  // dynamic LDS budget of the tail kernel
     int tail_wave_cells = saras::kTailWaveCells;  // SARAS_TAIL_WAVE_CELLS overrides it
     bool red_fold = true;              // SARAS_NO_RED_FOLD=1: the restriction leaves the first red pass to the sweep
+    bool rr_lane = false;              // SARAS_RR_LANE=1: 3-D split restriction with all eight children in one lane
     // V-cycles replayed from captured hipGraphs (saras_mg_set_graphs).  A
     // graph bakes in the level buffers, and out-of-place sweeps swap them, so
     // graphs are cached per buffer state (at most two states per precision).
@@ -434,6 +435,20 @@ struct Ops {
             r.wd[q] = C.rwd[q];
         }
         const long n = (long)C.n[0] * C.n[1] * C.n[2];
+        if (F.hz && H.dim == 3 && !m->rr_lane) {
+            const long ntask = (long)C.n[0] * C.n[1] * (F.n[2] / 2 / saras::Vec16<T>::W);
+            const int nb = (int)((ntask + saras::kRrTasks - 1) / saras::kRrTasks);
+            const int grid = (nb + saras::kXcds - 1) / saras::kXcds * saras::kXcds;
+            const bool red = red_next && C.hz;
+            if (red) skip_red = l + 1;
+            if (red)
+                hipLaunchKernelGGL((saras::k_rr_split4<T, true>), dim3(grid), dim3(saras::kSplitThreads), 0, s, fa,
+                                   F.x, F.b, ca, C.b, C.x, r, nb, w);
+            else
+                hipLaunchKernelGGL((saras::k_rr_split4<T, false>), dim3(grid), dim3(saras::kSplitThreads), 0, s, fa,
+                                   F.x, F.b, ca, C.b, C.x, r, nb, w);
+            return;
+        }
         if (F.hz) {
             constexpr int W = saras::rr_width<T>();
             const long total = (long)C.n[0] * C.n[1] * (F.n[2] / 2 / W);
@@ -856,6 +871,8 @@ saras_status saras_mg_create(const saras_grid_desc *grid, const saras_mg_params 
         m->march_tj8 = env4 && env4[0] == '1';
         const char *env5 = std::getenv("SARAS_NO_SPLIT");
         m->split_off = env5 && env5[0] == '1';
+        const char *env9 = std::getenv("SARAS_RR_LANE");
+        m->rr_lane = env9 && env9[0] == '1';
         const char *env8 = std::getenv("SARAS_NO_RED_FOLD");
         m->red_fold = !(env8 && env8[0] == '1');
         const char *env7 = std::getenv("SARAS_TAIL_WAVE_CELLS");

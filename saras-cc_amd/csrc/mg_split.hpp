@@ -464,6 +464,201 @@ __global__ void __launch_bounds__(kSplitThreads) k_rr_split(LevelArgs<T> F, cons
 }
 
 // ---------------------------------------------------------------------------
+// Residual + restriction of a split 3-D level with the eight children of a
+// coarse cell spread over the four waves of a block.  A block owns 64
+// consecutive (coarse row, vector) tasks; wave w = (di, dj) = (w / 2, w % 2)
+// computes, for every task, the residuals of fine row (2I + di, 2J + dj) at
+// the 2W fine cells under the task's W coarse cells -- whole 16-byte vectors,
+// as in a colour pass -- and leaves the products v_{di dj dk} r in LDS.  After
+// one barrier each thread adds the eight products of one coarse cell in the
+// reference's order and divides by the volume (poisson_d3.cc:494-511).  Every
+// product and partial sum is the same rounding as in the one-lane form, so
+// the values are identical.  RED as in k_rr_split.
+// ---------------------------------------------------------------------------
+constexpr int kRrTasks = 64;
+
+// position of v_{di dj dk} r_{di dj dk} in the reference's sum
+// v111 r111 + v000 r000 + v110 r110 + v001 r001 + v101 r101 + v010 r010 + v011 r011 + v100 r100
+__device__ __forceinline__ int rr_slot(int di, int dj, int dk) {
+    switch (di * 4 + dj * 2 + dk) {   // codes 7, 0, 6, 1, 5, 2, 3, 4 in sum order
+        case 7: return 0;
+        case 0: return 1;
+        case 6: return 2;
+        case 1: return 3;
+        case 5: return 4;
+        case 2: return 5;
+        case 3: return 6;
+        default: return 7;
+    }
+}
+
+template <typename T, bool RED>
+__global__ void __launch_bounds__(kSplitThreads) k_rr_split4(LevelArgs<T> F, const T *__restrict__ x,
+                                                             const T *__restrict__ b, LevelArgs<T> C,
+                                                             T *__restrict__ bc, T *__restrict__ xc, RestrictArgs<T> R,
+                                                             int nblocks, T w) {
+    using VT = Vec16<T>;
+    constexpr int W = VT::W;
+    __shared__ __align__(16) T prod[8][kRrTasks * W];
+    const int per = gridDim.x / kXcds;
+    const int logical = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+    if (logical >= nblocks) return;
+    const int nx = F.n[0], ny = F.n[1], nz = F.n[2];
+    const int half = nz >> 1, q = half / W;
+    const int cy = C.n[1];
+    const long ntask = (long)C.n[0] * cy * q;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int di = wave >> 1, dj = wave & 1;
+    const long task = (long)logical * kRrTasks + lane;
+    const bool live = task < ntask;
+    const long hz = F.hz;
+
+    int I = 0, J = 0, K0 = 0, i = 0, j = 0;
+    long rb = 0;
+    T xe[W], xo[W];
+    for (int e = 0; e < W; ++e) xe[e] = xo[e] = T(0);
+    if (live) {
+        const long crow = task / q;
+        const int v = (int)(task - crow * q);
+        I = (int)(crow / cy);
+        J = (int)(crow - (long)I * cy);
+        K0 = v * W;
+        i = 2 * I + di;
+        j = 2 * J + dj;
+        rb = (long)i * F.sx + (long)j * F.sy;
+        load_n<T, W>(x + rb + K0, xe);
+        load_n<T, W>(x + rb + hz + K0, xo);
+    }
+    // cross-vector z neighbours from the adjacent lanes (all lanes take part;
+    // a live lane's predecessor is live, and a live lane whose successor is
+    // not ends its row, where the boundary rule applies instead)
+    const T from_prev = __shfl_up(xo[W - 1], 1, 64);
+    const T from_next = __shfl_down(xe[0], 1, 64);
+    if (live) {
+        T be[W], bo[W], cz2e[W], cz1e[W], cz2o[W], cz1o[W];
+        load_n<T, W>(b + rb + K0, be);
+        load_n<T, W>(b + rb + hz + K0, bo);
+        load_n<T, W>(F.c2zs + K0, cz2e);
+        load_n<T, W>(F.c1zs + K0, cz1e);
+        load_n<T, W>(F.c2zs + half + K0, cz2o);
+        load_n<T, W>(F.c1zs + half + K0, cz1o);
+        // neighbour rows: [0] E, [1] W, [2] N, [3] S; even and odd halves
+        T ne[4][W], no[4][W];
+        auto nbr = [&](int slot, bool inside, long off, int bcode, int axis) {
+            if (inside || bcode == BC_PER) {
+                load_n<T, W>(x + rb + off + K0, ne[slot]);
+                load_n<T, W>(x + rb + off + hz + K0, no[slot]);
+            } else {
+#pragma unroll
+                for (int e = 0; e < W; ++e) {
+                    T wve = T(0), wvo = T(0);
+                    if (bcode == BC_DIRV) {
+                        const int ke = 2 * (K0 + e), ko = ke + 1;
+                        if (axis == 0) {
+                            wve = F.wall[0][(j + 1) * (nz + 2) + ke + 1];
+                            wvo = F.wall[0][(j + 1) * (nz + 2) + ko + 1];
+                        } else {
+                            wve = F.wall[1][(i + 1) * (nz + 2) + ke + 1];
+                            wvo = F.wall[1][(i + 1) * (nz + 2) + ko + 1];
+                        }
+                    }
+                    ne[slot][e] = bc_value(bcode, xe[e], T(0), wve);
+                    no[slot][e] = bc_value(bcode, xo[e], T(0), wvo);
+                }
+            }
+        };
+        nbr(0, i + 1 < nx, i + 1 < nx ? F.sx : -(long)(nx - 1) * F.sx, F.bc_hi[0], 0);
+        nbr(1, i > 0, i > 0 ? -F.sx : (long)(nx - 1) * F.sx, F.bc_lo[0], 0);
+        nbr(2, j + 1 < ny, j + 1 < ny ? F.sy : -(long)(ny - 1) * F.sy, F.bc_hi[1], 1);
+        nbr(3, j > 0, j > 0 ? -F.sy : (long)(ny - 1) * F.sy, F.bc_lo[1], 1);
+        // z: even cell 2K has zB = odd[K-1], zT = odd[K]; odd cell has zB = even[K], zT = even[K+1]
+        T zlo0, zhi_last;
+        if (K0 == 0) {
+            if (F.bc_lo[2] == BC_PER) zlo0 = x[rb + hz + (half - 1)];
+            else {
+                const T wv = F.bc_lo[2] == BC_DIRV ? F.wall[2][(i + 1) * (ny + 2) + j + 1] : T(0);
+                zlo0 = bc_value(F.bc_lo[2], xe[0], T(0), wv);
+            }
+        } else {
+            zlo0 = lane > 0 ? from_prev : x[rb + hz + K0 - 1];
+        }
+        if (K0 + W == half) {
+            if (F.bc_hi[2] == BC_PER) zhi_last = x[rb];
+            else {
+                const T wv = F.bc_hi[2] == BC_DIRV ? F.wall[2][(i + 1) * (ny + 2) + j + 1] : T(0);
+                zhi_last = bc_value(F.bc_hi[2], xo[W - 1], T(0), wv);
+            }
+        } else {
+            zhi_last = lane < 63 ? from_next : x[rb + K0 + W];
+        }
+        const T c2x = F.c2[0][i], c1x = F.c1[0][i], c2y = F.c2[1][j], c1y = F.c1[1][j];
+        const T ax = di ? R.hi[0][I] : R.lo[0][I], ay = dj ? R.hi[1][J] : R.lo[1][J];
+        const T axy = ax * ay;
+        T lz[W], hzf[W];
+        load_n<T, W>(R.lo[2] + K0, lz);
+        load_n<T, W>(R.hi[2] + K0, hzf);
+        T p0[W], p1[W];
+#pragma unroll
+        for (int e = 0; e < W; ++e) {
+            Stencil<T, 3> s;
+            s.c = xe[e];
+            s.xE = ne[0][e];
+            s.xW = ne[1][e];
+            s.yN = ne[2][e];
+            s.yS = ne[3][e];
+            s.zB = e > 0 ? xo[e - 1] : zlo0;
+            s.zT = xo[e];
+            const T r0 = residual<T, 3>(s, c2x, c1x, c2y, c1y, cz2e[e], cz1e[e], be[e]);
+            s.c = xo[e];
+            s.xE = no[0][e];
+            s.xW = no[1][e];
+            s.yN = no[2][e];
+            s.yS = no[3][e];
+            s.zB = xe[e];
+            s.zT = e < W - 1 ? xe[e + 1] : zhi_last;
+            const T r1 = residual<T, 3>(s, c2x, c1x, c2y, c1y, cz2o[e], cz1o[e], bo[e]);
+            p0[e] = (axy * lz[e]) * r0;
+            p1[e] = (axy * hzf[e]) * r1;
+        }
+        *reinterpret_cast<typename VT::V *>(&prod[rr_slot(di, dj, 0)][lane * W]) = VT::make(p0);
+        *reinterpret_cast<typename VT::V *>(&prod[rr_slot(di, dj, 1)][lane * W]) = VT::make(p1);
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < kRrTasks * W; c += blockDim.x) {
+        const long tk = (long)logical * kRrTasks + c / W;
+        if (tk >= ntask) continue;
+        const long crow = tk / q;
+        const int v = (int)(tk - crow * q);
+        const int CI = (int)(crow / cy), CJ = (int)(crow - (long)CI * cy);
+        const int K = v * W + c % W;
+        T sum = prod[0][c] + prod[1][c];
+        sum = sum + prod[2][c];
+        sum = sum + prod[3][c];
+        sum = sum + prod[4][c];
+        sum = sum + prod[5][c];
+        sum = sum + prod[6][c];
+        sum = sum + prod[7][c];
+        const T vtot = R.wd[0][CI] * R.wd[1][CJ] * R.wd[2][K];
+        const T out = sum / vtot;
+        const long cidx = (long)CI * C.sx + (long)CJ * C.sy + kof(C, K);
+        bc[cidx] = out;
+        T xr = T(0);
+        if (RED && ((CI + CJ + K) & 1) == 0) {
+            Stencil<T, 3> s;
+            s.c = T(0);
+            s.xE = bc_value(C.bc_hi[0], T(0), T(0), T(0));
+            s.xW = bc_value(C.bc_lo[0], T(0), T(0), T(0));
+            s.yN = bc_value(C.bc_hi[1], T(0), T(0), T(0));
+            s.yS = bc_value(C.bc_lo[1], T(0), T(0), T(0));
+            s.zT = bc_value(C.bc_hi[2], T(0), T(0), T(0));
+            s.zB = bc_value(C.bc_lo[2], T(0), T(0), T(0));
+            xr = sor_update<T, 3>(s, C.c2[0][CI], C.c1[0][CI], C.c2[1][CJ], C.c1[1][CJ], C.c2[2][K], C.c1[2][K], out, w);
+        }
+        xc[cidx] = xr;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // prolongation by injection plus the saved iterate on a split level
 // (poisson_d3.cc:548-558, poisson.cc:266): both halves of W fine positions
 // m0.. of row (i, j) take the correction of coarse cells K = m0.. of coarse

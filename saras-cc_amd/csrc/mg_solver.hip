@@ -334,6 +334,20 @@ struct Ops {
     void split_pass(bool fine, bool pro, int grid, const saras::LevelArgs<T> &a, DevLevel<T> &L, int colour, int nb,
                     const T *e, long esx, long esy, long ehz) {
         const dim3 g(grid), bl(saras::kSplitThreads);
+        // rows that wrap in z, or rows split between waves, need the
+        // z-prefetching variant (see rb_split_body)
+        const int q = L.n[2] / 2 / saras::Vec16<T>::W;
+        const bool zp = a.bc_lo[2] == saras::BC_PER || !(q % 64 == 0 || 64 % q == 0);
+        if (zp) {
+            if (fine) {
+                if (pro) hipLaunchKernelGGL((saras::k_rb_split_zp<T, DIM, true, true>), g, bl, 0, s, a, L.x, L.b, colour, w, nb, e, esx, esy, ehz);
+                else hipLaunchKernelGGL((saras::k_rb_split_zp<T, DIM, true, false>), g, bl, 0, s, a, L.x, L.b, colour, w, nb, e, esx, esy, ehz);
+            } else {
+                if (pro) hipLaunchKernelGGL((saras::k_rb_split_zp<T, DIM, false, true>), g, bl, 0, s, a, L.x, L.b, colour, w, nb, e, esx, esy, ehz);
+                else hipLaunchKernelGGL((saras::k_rb_split_zp<T, DIM, false, false>), g, bl, 0, s, a, L.x, L.b, colour, w, nb, e, esx, esy, ehz);
+            }
+            return;
+        }
         if (fine) {
             if (pro) hipLaunchKernelGGL((saras::k_rb_split<T, DIM, true, true>), g, bl, 0, s, a, L.x, L.b, colour, w, nb, e, esx, esy, ehz);
             else hipLaunchKernelGGL((saras::k_rb_split<T, DIM, true, false>), g, bl, 0, s, a, L.x, L.b, colour, w, nb, e, esx, esy, ehz);

@@ -122,10 +122,13 @@ __device__ __forceinline__ T coarse_at(const T *__restrict__ e, long base, long 
 // pass reads every cell as e(coarse) + x, the black pass its own cells (its
 // neighbours are the red values just written), so the separate
 // prolongation pass is not needed.
-template <typename T, int DIM, bool FINE, bool PRO>
-__global__ void __launch_bounds__(kSplitThreads) k_rb_split(LevelArgs<T> L, T *__restrict__ x,
-                                                            const T *__restrict__ b, int colour, T w, int nblocks,
-                                                            const T *__restrict__ e, long esx, long esy, long ehz) {
+// ZPRE: the z neighbour that no adjacent lane holds (periodic wrap, or a row
+// split between waves) is loaded together with the row vectors; without it
+// that load is issued only where needed, which is cheaper when it never is.
+template <typename T, int DIM, bool PRO, bool ZPRE>
+__device__ __forceinline__ void rb_split_body(const LevelArgs<T> &L, T *__restrict__ x, const T *__restrict__ b,
+                                              int colour, T w, int nblocks, const T *__restrict__ e, long esx,
+                                              long esy, long ehz) {
     using VT = Vec16<T>;
     using V = typename VT::V;
     constexpr int W = VT::W;
@@ -150,6 +153,38 @@ __global__ void __launch_bounds__(kSplitThreads) k_rb_split(LevelArgs<T> L, T *_
     const V bv = *reinterpret_cast<const V *>(b + own);
     const V c2v = *reinterpret_cast<const V *>(L.c2zs + h * half + m0);
     const V c1v = *reinterpret_cast<const V *>(L.c1zs + h * half + m0);
+    // The z neighbour beyond this lane's vector that no adjacent lane of the
+    // wave holds (the periodic wrap at a row end, or the row's previous / next
+    // vector when it sits in another wave), loaded with the row vectors so
+    // that its latency overlaps theirs.  h == 0 (k even): element m0 - 1 of
+    // the other half; h == 1 (k odd): element m0 + W of the other half.
+    const int lane = threadIdx.x & 63;
+    long zoff = -1;
+    int zK = 0;
+    if (ZPRE) {
+    if (h == 0) {
+        if (m0 == 0) {
+            if (L.bc_lo[2] == BC_PER) {
+                zoff = rb + L.hz + (half - 1);
+                zK = half - 1;
+            }
+        } else if (lane == 0) {
+            zoff = oth - 1;
+            zK = m0 - 1;
+        }
+    } else {
+        if (m0 + W == half) {
+            if (L.bc_hi[2] == BC_PER) {
+                zoff = rb;
+                zK = 0;
+            }
+        } else if (lane == 63) {
+            zoff = oth + W;
+            zK = m0 + W;
+        }
+    }
+    }
+    const T zx = (ZPRE && zoff >= 0) ? x[zoff] : T(0);
     T c[W], o[W];
 #pragma unroll
     for (int q = 0; q < W; ++q) {
@@ -207,41 +242,44 @@ __global__ void __launch_bounds__(kSplitThreads) k_rb_split(LevelArgs<T> L, T *_
         side(yn, j + 1 < ny, j + 1 < ny ? L.sy : -(long)(ny - 1) * L.sy, L.bc_hi[1], 1, i, j + 1 < ny ? j + 1 : 0);
         side(ys, j > 0, j > 0 ? -L.sy : (long)(ny - 1) * L.sy, L.bc_lo[1], 1, i, j > 0 ? j - 1 : ny - 1);
     }
-    // a z neighbour read straight from memory, with the correction when the
-    // red pass of a prolongation sweep reads it (coarse cell K)
+    // the z value beyond the vector, with the correction when the red pass of
+    // a prolongation sweep reads it (coarse cell K): prefetched (ZPRE) or read here
     auto zread = [&](long off, int K) -> T {
         const T v = x[off];
         return (PRO && colour == 0) ? coarse_at(e, ebase, ehz, K) + v : v;
+    };
+    auto zfix = [&](long off, int K) -> T {
+        if (ZPRE) return (PRO && colour == 0) ? coarse_at(e, ebase, ehz, zK) + zx : zx;
+        return zread(off, K);
     };
     // z neighbours: k - 1 and k + 1 lie in the other half.  h == 0 (k even):
     // other[m - 1], other[m]; h == 1 (k odd): other[m], other[m + 1].  The
     // element beyond this lane's vector comes from the adjacent lane when it
     // holds the same row.
-    const int lane = threadIdx.x & 63;
     const T from_prev = __shfl_up(o[W - 1], 1, 64);
     const T from_next = __shfl_down(o[0], 1, 64);
     T zlo0 = T(0), zhi_last = T(0);
     if (h == 0) {
         if (m0 == 0) {
-            if (L.bc_lo[2] == BC_PER) zlo0 = zread(rb + L.hz + (half - 1), half - 1);
+            if (L.bc_lo[2] == BC_PER) zlo0 = zfix(rb + L.hz + (half - 1), half - 1);
             else {
                 const T wv = L.bc_lo[2] == BC_DIRV ? (DIM == 3 ? L.wall[2][(i + 1) * (ny + 2) + j + 1] : L.wall[2][i + 1])
                                                    : T(0);
                 zlo0 = bc_value(L.bc_lo[2], c[0], T(0), wv);
             }
         } else {
-            zlo0 = lane > 0 ? from_prev : zread(oth - 1, m0 - 1);
+            zlo0 = lane > 0 ? from_prev : zfix(oth - 1, m0 - 1);
         }
     } else {
         if (m0 + W == half) {
-            if (L.bc_hi[2] == BC_PER) zhi_last = zread(rb, 0);
+            if (L.bc_hi[2] == BC_PER) zhi_last = zfix(rb, 0);
             else {
                 const T wv = L.bc_hi[2] == BC_DIRV ? (DIM == 3 ? L.wall[2][(i + 1) * (ny + 2) + j + 1] : L.wall[2][i + 1])
                                                    : T(0);
                 zhi_last = bc_value(L.bc_hi[2], c[W - 1], T(0), wv);
             }
         } else {
-            zhi_last = lane < 63 ? from_next : zread(oth + W, m0 + W);
+            zhi_last = lane < 63 ? from_next : zfix(oth + W, m0 + W);
         }
     }
     const T c2x = L.c2[0][i], c1x = L.c1[0][i];
@@ -267,6 +305,20 @@ __global__ void __launch_bounds__(kSplitThreads) k_rb_split(LevelArgs<T> L, T *_
         r[q] = sor_update<T, DIM>(s, c2x, c1x, c2y, c1y, VT::get(c2v, q), VT::get(c1v, q), VT::get(bv, q), w);
     }
     *reinterpret_cast<V *>(x + own) = VT::make(r);
+}
+
+template <typename T, int DIM, bool FINE, bool PRO>
+__global__ void __launch_bounds__(kSplitThreads) k_rb_split(LevelArgs<T> L, T *__restrict__ x,
+                                                            const T *__restrict__ b, int colour, T w, int nblocks,
+                                                            const T *__restrict__ e, long esx, long esy, long ehz) {
+    rb_split_body<T, DIM, PRO, false>(L, x, b, colour, w, nblocks, e, esx, esy, ehz);
+}
+
+template <typename T, int DIM, bool FINE, bool PRO>
+__global__ void __launch_bounds__(kSplitThreads) k_rb_split_zp(LevelArgs<T> L, T *__restrict__ x,
+                                                               const T *__restrict__ b, int colour, T w, int nblocks,
+                                                               const T *__restrict__ e, long esx, long esy, long ehz) {
+    rb_split_body<T, DIM, PRO, true>(L, x, b, colour, w, nblocks, e, esx, esy, ehz);
 }
 
 // ---------------------------------------------------------------------------

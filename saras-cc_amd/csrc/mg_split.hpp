@@ -567,7 +567,7 @@ __global__ void __launch_bounds__(kSplitThreads) k_rr_split4(LevelArgs<T> F, con
 
     int I = 0, J = 0, K0 = 0, i = 0, j = 0;
     long rb = 0;
-    T xe[W], xo[W];
+    T xe[W], xo[W], zl = T(0), zh = T(0);
     for (int e = 0; e < W; ++e) xe[e] = xo[e] = T(0);
     if (live) {
         const long crow = task / q;
@@ -580,6 +580,9 @@ __global__ void __launch_bounds__(kSplitThreads) k_rr_split4(LevelArgs<T> F, con
         rb = (long)i * F.sx + (long)j * F.sy;
         load_n<T, W>(x + rb + K0, xe);
         load_n<T, W>(x + rb + hz + K0, xo);
+        // z neighbours no adjacent lane holds, loaded with the row vectors
+        if (K0 == 0 ? F.bc_lo[2] == BC_PER : lane == 0) zl = K0 == 0 ? x[rb + hz + (half - 1)] : x[rb + hz + K0 - 1];
+        if (K0 + W == half ? F.bc_hi[2] == BC_PER : lane == 63) zh = K0 + W == half ? x[rb] : x[rb + K0 + W];
     }
     // cross-vector z neighbours from the adjacent lanes (all lanes take part;
     // a live lane's predecessor is live, and a live lane whose successor is
@@ -626,22 +629,22 @@ __global__ void __launch_bounds__(kSplitThreads) k_rr_split4(LevelArgs<T> F, con
         // z: even cell 2K has zB = odd[K-1], zT = odd[K]; odd cell has zB = even[K], zT = even[K+1]
         T zlo0, zhi_last;
         if (K0 == 0) {
-            if (F.bc_lo[2] == BC_PER) zlo0 = x[rb + hz + (half - 1)];
+            if (F.bc_lo[2] == BC_PER) zlo0 = zl;
             else {
                 const T wv = F.bc_lo[2] == BC_DIRV ? F.wall[2][(i + 1) * (ny + 2) + j + 1] : T(0);
                 zlo0 = bc_value(F.bc_lo[2], xe[0], T(0), wv);
             }
         } else {
-            zlo0 = lane > 0 ? from_prev : x[rb + hz + K0 - 1];
+            zlo0 = lane > 0 ? from_prev : zl;
         }
         if (K0 + W == half) {
-            if (F.bc_hi[2] == BC_PER) zhi_last = x[rb];
+            if (F.bc_hi[2] == BC_PER) zhi_last = zh;
             else {
                 const T wv = F.bc_hi[2] == BC_DIRV ? F.wall[2][(i + 1) * (ny + 2) + j + 1] : T(0);
                 zhi_last = bc_value(F.bc_hi[2], xo[W - 1], T(0), wv);
             }
         } else {
-            zhi_last = lane < 63 ? from_next : x[rb + K0 + W];
+            zhi_last = lane < 63 ? from_next : zh;
         }
         const T c2x = F.c2[0][i], c1x = F.c1[0][i], c2y = F.c2[1][j], c1y = F.c1[1][j];
         const T ax = di ? R.hi[0][I] : R.lo[0][I], ay = dj ? R.hi[1][J] : R.lo[1][J];
@@ -795,6 +798,21 @@ __global__ void __launch_bounds__(kNormThreads) k_refine_split(LevelArgs<double>
         const int v = (int)(t - (long)row * q);
         const int i = row / ny, j = row - (row / ny) * ny;
         const int m0 = v * W;
+        const long rb = (long)i * L.sx + (long)j * L.sy;
+        const long eb = (long)i * esx + (long)j * esy;
+        auto cell = [&](long off64, long off32) {
+            double x = xold[rb + off64];
+            if (e) x = x + (double)e[eb + off32];
+            return x;
+        };
+        // the z neighbours beyond this lane's vectors that no adjacent lane
+        // holds (periodic wrap, or the row's vector in another wave), loaded
+        // first so that their latency overlaps the row loads
+        const bool need_lo = m0 == 0 ? L.bc_lo[2] == BC_PER : lane == 0;
+        const bool need_hi = m0 + W == half ? L.bc_hi[2] == BC_PER : lane == 63;
+        double zl = 0.0, zh = 0.0;
+        if (need_lo) zl = m0 == 0 ? cell(hz + half - 1, ehz + half - 1) : cell(hz + m0 - 1, ehz + m0 - 1);
+        if (need_hi) zh = m0 + W == half ? cell(0, 0) : cell(m0 + W, m0 + W);
         double ce[W], co[W];
         row_vals(i, j, m0, ce, co);
         // x/y neighbours: [0] E, [1] W, [2] N, [3] S
@@ -830,33 +848,26 @@ __global__ void __launch_bounds__(kNormThreads) k_refine_split(LevelArgs<double>
         // z: even cell 2m: zB = odd[m-1], zT = odd[m]; odd cell 2m+1: zB = even[m], zT = even[m+1]
         const double from_prev = __shfl_up(co[W - 1], 1, 64);
         const double from_next = __shfl_down(ce[0], 1, 64);
-        const long rb = (long)i * L.sx + (long)j * L.sy;
-        const long eb = (long)i * esx + (long)j * esy;
-        auto cell = [&](long off64, long off32) {
-            double x = xold[rb + off64];
-            if (e) x = x + (double)e[eb + off32];
-            return x;
-        };
         double zlo0, zhi_last;
         if (m0 == 0) {
-            if (L.bc_lo[2] == BC_PER) zlo0 = cell(hz + half - 1, ehz + half - 1);
+            if (L.bc_lo[2] == BC_PER) zlo0 = zl;
             else {
                 const double wv = L.bc_lo[2] == BC_DIRV ? (DIM == 3 ? L.wall[2][(i + 1) * (ny + 2) + j + 1] : L.wall[2][i + 1])
                                                         : 0.0;
                 zlo0 = bc_value<double>(L.bc_lo[2], ce[0], 0.0, wv);
             }
         } else {
-            zlo0 = lane > 0 ? from_prev : cell(hz + m0 - 1, ehz + m0 - 1);
+            zlo0 = lane > 0 ? from_prev : zl;
         }
         if (m0 + W == half) {
-            if (L.bc_hi[2] == BC_PER) zhi_last = cell(0, 0);
+            if (L.bc_hi[2] == BC_PER) zhi_last = zh;
             else {
                 const double wv = L.bc_hi[2] == BC_DIRV ? (DIM == 3 ? L.wall[2][(i + 1) * (ny + 2) + j + 1] : L.wall[2][i + 1])
                                                         : 0.0;
                 zhi_last = bc_value<double>(L.bc_hi[2], co[W - 1], 0.0, wv);
             }
         } else {
-            zhi_last = lane < 63 ? from_next : cell(m0 + W, m0 + W);
+            zhi_last = lane < 63 ? from_next : zh;
         }
         const double c2x = L.c2[0][i], c1x = L.c1[0][i];
         const double c2y = DIM == 3 ? L.c2[1][j] : 0.0, c1y = DIM == 3 ? L.c1[1][j] : 0.0;

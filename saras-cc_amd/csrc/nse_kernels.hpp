@@ -672,6 +672,8 @@ __global__ void __launch_bounds__(256) k_sgs_stress(Geom G, CoreField V0, CoreFi
 __global__ void __launch_bounds__(1024) k_sum3_final(const double *__restrict__ part, long n, double *__restrict__ out) {
     __shared__ double red[3][16];
     double a[3] = {0.0, 0.0, 0.0};
+    // unrolled so that several loads are in flight; the additions keep their order
+#pragma unroll 8
     for (long q = threadIdx.x; q < n; q += blockDim.x)
         for (int c = 0; c < 3; ++c) a[c] += part[3 * q + c];
     for (int c = 0; c < 3; ++c) {
@@ -709,6 +711,9 @@ __device__ __forceinline__ void block_max_store(double v, double *part, long blo
 __global__ void __launch_bounds__(1024) k_max_final(const double *__restrict__ part, long n, double *__restrict__ out) {
     __shared__ double red[16];
     double v = 0.0;
+    // unrolled so that several loads are in flight (the maximum does not
+    // depend on the order)
+#pragma unroll 8
     for (long q = threadIdx.x; q < n; q += blockDim.x) {
         const double w = part[q];
         v = (w > v || w != w) ? w : v;

@@ -500,7 +500,21 @@ saras_status saras_nse_create(const saras_grid_desc *grid, const saras_mg_params
     if (hipMalloc((void **)&s->scratch, s->elems * sizeof(double)) != hipSuccess) return fail(SARAS_OOM);
     // timestep.cc:115
     s->max_iter = int(std::pow(std::log(double(g.n[0]) * g.n[1] * g.n[2]), 3));
-    s->threads = (int)std::min<long>(256, std::max<long>(64, (g.n[2] + 63) / 64 * 64));
+    // one thread per cell of a row: the block of 64..256 threads (whole
+    // waves) that leaves the fewest idle lanes in a row's last block, the
+    // larger on a tie (384 cells: 2 blocks of 192, not 256 + half of 256)
+    {
+        int best = 64;
+        long best_idle = -1;
+        for (int t = 256; t >= 64; t -= 64) {
+            const long idle = ((long)g.n[2] + t - 1) / t * t - g.n[2];
+            if (best_idle < 0 || idle < best_idle) {
+                best = t;
+                best_idle = idle;
+            }
+        }
+        s->threads = best;
+    }
     {
         const dim3 gr = s->grid3();
         s->nblocks = (long)gr.x * gr.y * gr.z;

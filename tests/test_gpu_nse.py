@@ -142,6 +142,27 @@ def test_rbc_steps_bitexact():
         assert abs(do[k] - ds[k]) <= 1e-12 * abs(do[k]), k
 
 
+def test_rbc_rows_of_192_cells_bitexact():
+    """Rows of 192 cells run in blocks of 192 threads (no idle lanes, as at
+    384^3): two steps of a periodic 8 x 8 x 192 box with a seeded velocity
+    field (Morinishi terms, buoyancy), bit-exact."""
+    pad = 2
+    n = (8, 8, 192)
+    nu = kappa = 1e-3
+    so, ss = pair(n, periodic=(1, 1, 1), gtype=(0, 0, 0), pad=pad, nu=nu, kappa=kappa, dt=1e-3, cn_tol=1e-6)
+    z = O.field_metrics(so.d.g, 2)["x"]
+    f = rbc_init(so.shape, n, pad, z, vel=0.1)
+    so.set(**f)
+    ss.set(**f)
+    for step in range(2):
+        ro = so.step()
+        rs, st = ss.step()
+        assert ro == rs == 0
+        assert so.jacobi_iters() == st["jacobi_iters"], step
+        assert so.mg_stats()["residual"] == st["mg"]["residual"]
+        assert_fields_equal(so, ss, periodic=(1, 1, 1))
+
+
 def test_walls_everywhere_mixed_bcs_bitexact():
     """Closed box, every wall kind: Dirichlet and Neumann values on V and T,
     a velocity that is not divergence free, two steps."""

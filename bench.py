@@ -209,9 +209,10 @@ def rbc_step(steps, warmup, n=256):
     out = {}
     for name, prec in (("fp64", S.PREC_FP64), ("mixed", S.PREC_MIXED)):
         # every pressure solve of this run takes all 20 cycles (its residual
-        # plateaus near 1e-4), so the mixed solver refines every 4 FP32 cycles
+        # plateaus near 1e-4), so the mixed solver may run up to 5 FP32 cycles
+        # per refinement (3 + 5 + 5 + 5 + 2)
         mg = S.mg_params(vc_count=20, vc_tol=1e-6, solve_flag=0, mg_tol=1e-2, pre=3, post=6,
-                         bc=S.BC_NEUMANN, precision=prec, inner_cycles=4, max_refine=10)
+                         bc=S.BC_NEUMANN, precision=prec, inner_cycles=5, max_refine=10)
         s = S.Nse(g, mg, prm)
         s.set(**f)
         for _ in range(warmup):

@@ -85,8 +85,9 @@ typedef struct {
     int res_type;       /* "Residual Type": 0 L-inf, 1 L1, 2 L2 (relative) */
     int bc;             /* SARAS_BC_* */
     int precision;      /* SARAS_PREC_* */
-    int inner_cycles;   /* MIXED: most FP32 V-cycles per FP64 refinement step (fewer once the observed
-                           convergence factor shows fewer reach the tolerance) */
+    int inner_cycles;   /* MIXED: most FP32 V-cycles per FP64 refinement step; the first step runs at
+                           most 3 to measure the convergence factor, later ones the cycles that factor
+                           needs to reach the tolerance (or the FP32 accuracy floor) */
     int max_refine;     /* MIXED: cap on refinement steps */
 } saras_mg_params;
 

@@ -1055,18 +1055,27 @@ static saras_status solve_mixed(saras_mg *m, hipStream_t s, saras_mg_stats *st) 
     if (e) return e;
     int cycles = 0, refs = 0;
     const int cap = m->p.vc_count > 0 ? m->p.vc_count : (1 << 30);
-    // inner_cycles is the most FP32 cycles per refinement.  Once a refinement
-    // has shown the cycle's convergence factor rho, the next one runs only
-    // the cycles that factor needs to take the residual to half the
-    // tolerance; a short estimate costs one more refinement, never accuracy,
-    // since the FP64 residual alone decides convergence.
+    // inner_cycles is the most FP32 cycles per refinement.  The first
+    // refinement runs at most kProbeCycles of them, few enough that FP32
+    // rounding (a correction good to about kFp32Floor of the residual) does
+    // not mask the cycle's convergence factor rho, which it measures.  Each
+    // later refinement runs the cycles rho needs to take the residual to half
+    // the tolerance, or down to the FP32 floor when that comes first (cycles
+    // past it gain nothing).  A short estimate costs one more refinement,
+    // never accuracy: the FP64 residual alone decides convergence.
+    constexpr int kProbeCycles = 3;
+    constexpr double kFp32Floor = 1e-6;
     double rho = 1.0;
+    bool measured = false;
     while (!(res < m->p.vc_tol) && refs < m->p.max_refine && cycles < cap) {
         int k = m->p.inner_cycles;
-        if (rho > 0.0 && rho < 0.9 && res > 0.0) {
-            const double need = std::ceil(std::log(0.5 * m->p.vc_tol / res) / std::log(rho));
-            if (need >= 1.0 && need < (double)k) k = (int)need;
-            else if (need < 1.0) k = 1;
+        if (!measured) {
+            k = std::min(k, kProbeCycles);
+        } else if (rho > 0.0 && rho < 0.9 && res > 0.0) {
+            const double target = std::max(0.5 * m->p.vc_tol / res, kFp32Floor);
+            const double need = std::ceil(std::log(target) / std::log(rho));
+            if (need < 1.0) k = 1;
+            else if (need < (double)k) k = (int)need;
         }
         HIP_TRY(hipMemsetAsync(F0.x, 0, F0.elems * sizeof(float), s));
         int ran = 0;
@@ -1082,6 +1091,7 @@ static saras_status solve_mixed(saras_mg *m, hipStream_t s, saras_mg_stats *st) 
         ++refs;
         if (!std::isfinite(res)) break;
         rho = (ran > 0 && before > 0.0 && res < before) ? std::pow(res / before, 1.0 / ran) : 1.0;
+        measured = true;
     }
     if (m->p.solve_flag) {
         e = fetch_coarse_iters(m, s);

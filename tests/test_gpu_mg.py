@@ -286,6 +286,26 @@ def test_mixed_precision_reaches_fp64_tolerance(g_kw, bc, name):
     assert rel < 1e-7, rel
 
 
+def test_mixed_inner_cycles_follow_convergence():
+    """inner_cycles caps the FP32 cycles of a refinement; the first runs 3 to
+    measure the convergence factor, later ones the cycles it says reach the
+    tolerance (or the FP32 accuracy floor).  The FP64 oracle needs 9 cycles
+    here; a fixed 8 per refinement would run 16."""
+    tol = 1e-10
+    g_kw = dict(n=(64, 64, 64), periodic=(1, 1, 1))
+    kw = dict(vc_count=200, vc_tol=tol, mg_tol=1e-6, pre=2, post=2, solve_flag=1)
+    ms = S.Multigrid(S.grid_desc(**g_kw),
+                     S.mg_params(bc=S.BC_NEUMANN, precision=S.PREC_MIXED, inner_cycles=8, max_refine=30, **kw))
+    mo = O.Multigrid(O.grid_desc(**g_kw), O.mg_params(bc=O.BC_NEUMANN, **kw))
+    b = rhs_for(mo, np.random.default_rng(31), True)
+    xs, ss, status = ms.solve(np.zeros_like(b), b)
+    assert status == S.OK and ss["converged"], ss
+    xo, so = mo.solve(np.zeros_like(b), b, 1)
+    assert so["cycles"] > 8
+    assert ss["cycles"] <= so["cycles"] + 2, (ss, so)
+    assert oracle_residual(g_kw, O.BC_NEUMANN, xs, b) < 1.5 * tol
+
+
 @pytest.mark.parametrize("case", [((56, 56, 56), 3, (0, 0, 0), (0, 0, 0), O.BC_DIRICHLET),
                                   ((48, 48, 48), 3, (0, 0, 0), (2, 2, 2), O.BC_DIRICHLET),
                                   ((224, 1, 224), 2, (0, 0, 0), (0, 0, 0), O.BC_DIRICHLET)],

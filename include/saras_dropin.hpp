@@ -8,14 +8,23 @@
 //   saras_amd::lsRK3      stands for lsRK3_d3 (lib/timestep/lsRK3_d3.cc,
 //                         timestep.h:57-58) together with the fields it
 //                         advances; timeAdvance() is lsRK3_d3::timeAdvance
+//   saras_amd::tseries    stands for tseries (lib/io/tseries.h): the
+//                         TimeSeries.dat lines of writeTSHeader/writeTSData
+//   saras_amd::probes     stands for probes (lib/io/probes.h): the
+//                         ProbesData.dat lines of probeData
 //
 // Errors: the reference prints and calls MPI_Finalize(); exit(0) where
 // these throw saras_amd::error (status + saras_last_error() text).  A
 // V-cycle count that runs out is silent in the reference and is reported by
 // stats() here, not thrown.
 #pragma once
+#include <cmath>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "saras_hip.h"
 
@@ -96,6 +105,122 @@ class lsRK3 {
   private:
     saras_nse *h_ = nullptr;
     saras_nse_stats stats_{};
+};
+
+// tseries (tseries.cc:60-195, 209-490): the header of writeTSHeader and one
+// line per writeTSData, in the file and on the console, in the reference's
+// fixed-width format.  `scalar` selects the scalar solver's columns (probType
+// above 4: Re, Nu, KE, TE), `les` the sub-grid columns; nu is timestep's
+// mDiff (timestep.cc:120), volume the box volume.
+// writeTSData's divergence guard (max |div V| above 1e3 and rising ends the
+// run, tseries.cc:217-223) throws saras_amd::error instead of exit(0).
+class tseries {
+  public:
+    tseries(lsRK3 &solver, const std::string &path, double nu, double volume, bool scalar, bool les,
+            bool append = false, std::ostream *console = &std::cout)
+        : ts_(solver), nu_(nu), vol_(volume), scalar_(scalar), les_(les), con_(console) {
+        file_.open(path, append ? std::fstream::out | std::fstream::app : std::fstream::out);
+        if (!file_) throw error(SARAS_BAD_ARG, "tseries: cannot open " + path);
+    }
+
+    void writeTSHeader() {
+        if (!scalar_) {
+            if (con_) *con_ << std::setw(9) << "Time" << std::setw(20) << "Total KE" << std::setw(20) << "Divergence\n";
+            file_ << (les_ ? "#VARIABLES = Time, Total KE, U_rms, Divergence, Subgrid KE, SG Dissipation, Turb. Viscosity, dt\n"
+                           : "#VARIABLES = Time, Total KE, U_rms, Divergence, dt\n");
+        } else {
+            if (con_)
+                *con_ << std::setw(9) << "Time" << std::setw(20) << "Re (Urms)" << std::setw(20) << "Nu (Wall)"
+                      << std::setw(20) << "Divergence" << "\n";
+            file_ << (les_ ? "#VARIABLES = Time, Re, Nu (wall), Nu (vol), Total KE, Total TE, Divergence, Subgrid KE, "
+                             "SG Dissipation, Turb. Viscosity, dt\n"
+                           : "#VARIABLES = Time, Re, Nu (wall), Nu (vol), Total KE, Total TE, Divergence, dt\n");
+        }
+        file_.flush();
+    }
+
+    // one line for the fields as they stand at `time`, with time step dt
+    void writeTSData(double time, double dt) {
+        double d[5];
+        check(saras_nse_tsdata(ts_.handle(), d), "saras_nse_tsdata");
+        const double ke = d[0], te = d[1], div = d[2], nuvol = d[3], nuwall = d[4];
+        if (div > 1e3 && div - old_div_ > 0)
+            throw error(SARAS_BAD_ARG, "tseries: divergence exceeds permissible limits");
+        old_div_ = div;
+        double sg[3] = {0.0, 0.0, 0.0};
+        if (les_) {
+            double t = 0.0;
+            ts_.sgsDiagnostics(sg, &t);
+            for (double &v : sg) v /= vol_;
+        }
+        auto cols = [&](std::ostream &o, std::initializer_list<double> v) {
+            o << std::fixed << std::setprecision(4) << std::setw(9) << time << std::setprecision(8);
+            for (double x : v) o << std::setw(20) << x;
+        };
+        if (!scalar_) {
+            if (con_) {
+                cols(*con_, {ke, div});
+                *con_ << "\n";
+            }
+            if (les_) cols(file_, {ke, std::sqrt(2.0 * ke), div, sg[0], sg[1], sg[2], dt});
+            else cols(file_, {ke, std::sqrt(2.0 * ke), div, dt});
+        } else {
+            const double re = std::sqrt(2.0 * ke) / nu_;
+            if (con_) {
+                cols(*con_, {re, nuwall, div});
+                *con_ << "\n";
+            }
+            if (les_) cols(file_, {re, nuwall, nuvol, ke, te, div, sg[0], sg[1], sg[2], dt});
+            else cols(file_, {re, nuwall, nuvol, ke, te, div, dt});
+        }
+        file_ << "\n";
+        file_.flush();
+    }
+
+  private:
+    lsRK3 &ts_;
+    double nu_, vol_;
+    bool scalar_, les_;
+    std::ostream *con_;
+    std::ofstream file_;
+    double old_div_ = 1.0e10;
+};
+
+// probes (probes.cc:61-176): ProbesData.dat with the header of the
+// constructor and one line per probe per probeData call.  Points are the
+// probes' global core indices (0-based, as the reference's probe list);
+// fields Vx, Vy, Vz, P and, with a scalar, T.
+class probes {
+  public:
+    probes(lsRK3 &solver, const std::string &path, const std::vector<int> &ijk, bool scalar, bool restart = false)
+        : ts_(solver), ijk_(ijk), scalar_(scalar) {
+        if (ijk.size() % 3) throw error(SARAS_BAD_ARG, "probes: ijk holds (i, j, k) triples");
+        file_.open(path, std::fstream::out | std::fstream::app);
+        if (!file_) throw error(SARAS_BAD_ARG, "probes: cannot open " + path);
+        if (!restart) file_ << (scalar ? "#VARIABLES = Time, X, Y, Z, Vx, Vy, Vz, P, T" : "#VARIABLES = Time, X, Y, Z, Vx, Vy, Vz, P")
+                            << std::endl;
+    }
+
+    void probeData(double time) {
+        const int n = (int)(ijk_.size() / 3);
+        std::vector<double> v(5 * (size_t)n);
+        check(saras_nse_probe(ts_.handle(), n, ijk_.data(), v.data()), "saras_nse_probe");
+        const int nf = scalar_ ? 5 : 4;
+        for (int q = 0; q < n; ++q) {
+            file_ << std::fixed << std::setw(6) << std::setprecision(4) << time << "\t";
+            file_ << std::setw(4) << std::setprecision(0) << ijk_[3 * q] << "\t" << ijk_[3 * q + 1] << "\t"
+                  << ijk_[3 * q + 2] << "\t";
+            file_ << std::setw(16) << std::setprecision(8) << v[5 * (size_t)q];
+            for (int f = 1; f < nf; ++f) file_ << "\t" << std::setw(16) << std::setprecision(8) << v[5 * (size_t)q + f];
+            file_ << std::endl;
+        }
+    }
+
+  private:
+    lsRK3 &ts_;
+    std::vector<int> ijk_;
+    bool scalar_;
+    std::ofstream file_;
 };
 
 }  // namespace saras_amd

@@ -275,6 +275,19 @@ saras_status saras_nse_cfl(saras_nse *s, double courant, double dt_max, double *
  * (z gravity). */
 saras_status saras_nse_diag(saras_nse *s, double out[4]);
 
+/* The quantities of a TimeSeries.dat line (tseries::writeTSData,
+ * tseries.cc:209-290 hydro, 295-490 scalar, z gravity): out[0] volume-mean
+ * kinetic energy, out[1] volume-mean thermal energy 1/2 (T + z - 1)^2
+ * (0 without a scalar), out[2] max |div V|, out[3] NuVol, out[4] NuWall.
+ * Re = sqrt(2 KE) / nu and U_rms = sqrt(2 KE) are formed by the writer. */
+saras_status saras_nse_tsdata(saras_nse *s, double out[5]);
+
+/* probes::getData (probes.cc:185-205): the values of Vx, Vy, Vz, P, T at
+ * `count` core cells ijk[3q..3q+2] (0-based core indices) into
+ * out[5q..5q+4]; T is 0 without a scalar.  BAD_ARG for a point outside the
+ * core. */
+saras_status saras_nse_probe(saras_nse *s, int count, const int *ijk, double *out);
+
 /* Measurement hook: `count` full steps with no host work between them
  * beyond the reference's own convergence checks; *ms = device time. */
 saras_status saras_nse_bench_steps(saras_nse *s, int count, double *ms);

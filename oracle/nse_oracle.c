@@ -911,6 +911,26 @@ void or_nse_op(or_nse *s, int op, const double *F, double *out)
     }
 }
 
+/* tseries.cc:410-446 (z gravity): volume-mean thermal energy
+ * 1/2 (T + z - 1)^2 of the deviation from the conduction profile */
+double or_nse_thermal_energy(or_nse *s)
+{
+    int p = s->p;
+    double dxi[3];
+    for (int a = 0; a < 3; a++) dxi[a] = 1.0 / (double)s->n[a];
+    double te = 0.0;
+    if (!s->d.has_T) return 0.0;
+    for (int i = 0; i < s->n[0]; i++)
+        for (int j = 0; j < s->n[1]; j++)
+            for (int k = 0; k < s->n[2]; k++) {
+                long q = FX(s, i, j, k);
+                double dVol = (dxi[0] / s->xi_x[0][i + p]) * (dxi[1] / s->xi_x[1][j + p]) * (dxi[2] / s->xi_x[2][k + p]);
+                double theta = s->T[q] + s->x[2][k + p] - 1.0;
+                te += pow(theta, 2.0) * 0.5 * dVol;
+            }
+    return te / (s->d.g.len[0] * s->d.g.len[1] * s->d.g.len[2]);
+}
+
 /* tseries.cc:209-290: KE, max |div|, NuVol */
 void or_nse_diag(or_nse *s, double *ke, double *divmax, double *nuvol, double *nuwall)
 {

@@ -104,6 +104,8 @@ def lib():
         L.or_nse_mg_stats.argtypes = [C.c_void_p, C.POINTER(MgStats)]
         L.or_nse_op.argtypes = [C.c_void_p, C.c_int, dp, dp]
         L.or_nse_diag.argtypes = [C.c_void_p, dp, dp, dp, dp]
+        L.or_nse_thermal_energy.argtypes = [C.c_void_p]
+        L.or_nse_thermal_energy.restype = C.c_double
         L.or_nse_cfl.argtypes = [C.c_void_p, C.c_double, C.c_double]
         L.or_nse_sgs.argtypes = [C.c_void_p, dp, dp, dp]
         L.or_nse_sgs_diag.argtypes = [C.c_void_p, dp]
@@ -402,3 +404,7 @@ class Nse:
         v = [C.c_double() for _ in range(4)]
         lib().or_nse_diag(self.h, *[C.byref(x) for x in v])
         return dict(ke=v[0].value, divmax=v[1].value, nuvol=v[2].value, nuwall=v[3].value)
+
+    def thermal_energy(self):
+        """tseries.cc:410-446: volume mean of 1/2 (T + z - 1)^2."""
+        return lib().or_nse_thermal_energy(self.h)

@@ -177,6 +177,7 @@ void or_nse_op(or_nse *s, int op, const double *F, double *out);
 /* tseries.cc:209-290 diagnostics: volume-mean KE, max |div V|, NuVol, and
  * the wall Nusselt number of z-gravity runs. */
 void or_nse_diag(or_nse *s, double *ke, double *divmax, double *nuvol, double *nuwall);
+double or_nse_thermal_energy(or_nse *s);
 /* vfield::computeTStp (vfield.cc:691-706) then the driver's cap
  * (hydro_d3.cc:221-226): dt = min(courant / max(|u|/dx + |v|/dy + |w|/dz), dt_max);
  * the run's dt becomes that value. */

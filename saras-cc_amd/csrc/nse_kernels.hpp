@@ -835,14 +835,15 @@ __global__ void __launch_bounds__(256) k_correct(Geom G, PF Pp, double *__restri
 // [0] sum KE dVol, [1] sum Vz T dVol, [2] max |div V|, [3] sum bottom-wall dTdn dA,
 // [4] sum top-wall dTdn dA.  dxi = 1/n per axis; wall_dn = the top-wall distance.
 constexpr int kDiagBlocks = 1024;
+constexpr int kDiagSums = 6;   // KE, uz T, max|div|, bottom and top wall flux, thermal energy
 constexpr int kDiagThreads = 256;
 
 __global__ void __launch_bounds__(kDiagThreads) k_diag(Geom G, CoreField V0, CoreField V1, CoreField V2,
                                                        CoreField T, int has_T, double dxi0, double dxi1,
                                                        double dxi2, double wall_dn, double *__restrict__ part) {
-    __shared__ double red[5][kDiagThreads / 64];
+    __shared__ double red[kDiagSums][kDiagThreads / 64];
     const long total = (long)G.n[0] * G.n[1] * G.n[2];
-    double ke = 0.0, uzt = 0.0, dm = 0.0, nb = 0.0, nt = 0.0;
+    double ke = 0.0, uzt = 0.0, dm = 0.0, nb = 0.0, nt = 0.0, te = 0.0;
     for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < total; c += (long)gridDim.x * blockDim.x) {
         const int k = (int)(c % G.n[2]);
         const int j = (int)((c / G.n[2]) % G.n[1]);
@@ -859,13 +860,16 @@ __global__ void __launch_bounds__(kDiagThreads) k_diag(Geom G, CoreField V0, Cor
         if (has_T) {
             const double t = T.at(G, i, j, k);
             uzt += w * t * dVol;
+            // theta = T + z - 1: the deviation from the conduction profile (tseries.cc:423-429)
+            const double th = t + G.xc[2][k] - 1.0;
+            te += th * th * 0.5 * dVol;
             const double dA = (dxi0 / G.xi_x[0][i]) * (dxi1 / G.xi_x[1][j]);
             if (k == 0) nb += (1.0 - fabs(t)) / wall_dn * dA;
             if (k == G.n[2] - 1) nt += fabs(t) / wall_dn * dA;
         }
     }
-    double vals[5] = {ke, uzt, dm, nb, nt};
-    for (int q = 0; q < 5; ++q) {
+    double vals[kDiagSums] = {ke, uzt, dm, nb, nt, te};
+    for (int q = 0; q < kDiagSums; ++q) {
         double v = vals[q];
         for (int o = 32; o > 0; o >>= 1) {
             const double w = __shfl_xor(v, o, 64);
@@ -875,15 +879,31 @@ __global__ void __launch_bounds__(kDiagThreads) k_diag(Geom G, CoreField V0, Cor
         if ((threadIdx.x & 63) == 0) red[q][threadIdx.x >> 6] = v;
     }
     __syncthreads();
-    if (threadIdx.x < 5) {
+    if (threadIdx.x < kDiagSums) {
         const int q = threadIdx.x;
         double v = red[q][0];
         for (int w = 1; w < kDiagThreads / 64; ++w) {
             if (q == 2) v = (red[q][w] > v || red[q][w] != red[q][w]) ? red[q][w] : v;
             else v += red[q][w];
         }
-        part[(long)blockIdx.x * 5 + q] = v;
+        part[(long)blockIdx.x * kDiagSums + q] = v;
     }
+}
+
+// probes::getData (probes.cc:185-205): Vx, Vy, Vz, P, T at core cells
+// (i, j, k) of the list; T reads 0 without a scalar field
+__global__ void __launch_bounds__(256) k_probe(Geom G, const double *__restrict__ u, const double *__restrict__ v,
+                                               const double *__restrict__ w, const double *__restrict__ p,
+                                               const double *__restrict__ t, int n, const int *__restrict__ ijk,
+                                               double *__restrict__ out) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const long o = (long)ijk[3 * q] * G.sx + (long)ijk[3 * q + 1] * G.sy + ijk[3 * q + 2];
+    out[5 * q + 0] = u[o];
+    out[5 * q + 1] = v[o];
+    out[5 * q + 2] = w[o];
+    out[5 * q + 3] = p[o];
+    out[5 * q + 4] = t ? t[o] : 0.0;
 }
 
 // vfield::computeTStp (vfield.cc:691-706): per-block maxima of

@@ -522,9 +522,9 @@ saras_status saras_nse_create(const saras_grid_desc *grid, const saras_mg_params
     if (hipMalloc((void **)&s->part, sizeof(double) * s->nblocks) != hipSuccess) return fail(SARAS_OOM);
     if (hipMalloc((void **)&s->res_dev, sizeof(double)) != hipSuccess) return fail(SARAS_OOM);
     if (hipHostMalloc((void **)&s->res_host, sizeof(double)) != hipSuccess) return fail(SARAS_OOM);
-    if (hipMalloc((void **)&s->diag_part, sizeof(double) * 5 * saras::nse::kDiagBlocks) != hipSuccess)
+    if (hipMalloc((void **)&s->diag_part, sizeof(double) * saras::nse::kDiagSums * saras::nse::kDiagBlocks) != hipSuccess)
         return fail(SARAS_OOM);
-    if (hipHostMalloc((void **)&s->diag_host, sizeof(double) * 5 * saras::nse::kDiagBlocks) != hipSuccess)
+    if (hipHostMalloc((void **)&s->diag_host, sizeof(double) * saras::nse::kDiagSums * saras::nse::kDiagBlocks) != hipSuccess)
         return fail(SARAS_OOM);
     if (hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess) return fail(SARAS_HIP_ERROR);
     s->full = (long)(g.n[0] + 2 * g.pad) * (g.n[1] + 2 * g.pad) * (g.n[2] + 2 * g.pad);
@@ -729,28 +729,75 @@ saras_status saras_nse_cfl(saras_nse *s, double courant, double dt_max, double *
 
 saras_status saras_nse_diag(saras_nse *s, double out[4]) {
     if (!s || !out) return SARAS_BAD_ARG;
+    double ts[5];
+    const saras_status e = saras_nse_tsdata(s, ts);
+    if (e) return e;
+    out[0] = ts[0];
+    out[1] = ts[2];
+    out[2] = ts[3];
+    out[3] = ts[4];
+    return SARAS_OK;
+}
+
+saras_status saras_nse_tsdata(saras_nse *s, double out[5]) {
+    if (!s || !out) return SARAS_BAD_ARG;
+    constexpr int NS = saras::nse::kDiagSums;
     const long total = (long)s->G.n[0] * s->G.n[1] * s->G.n[2];
     const int blocks = (int)std::min<long>(saras::nse::kDiagBlocks, (total + 255) / 256);
     hipLaunchKernelGGL(saras::nse::k_diag, dim3(blocks), dim3(saras::nse::kDiagThreads), 0, s->s, s->G, s->field(0),
                        s->field(1), s->field(2), s->field(3), s->p.has_T, 1.0 / double(s->G.n[0]),
                        1.0 / double(s->G.n[1]), 1.0 / double(s->G.n[2]), s->wall_dn, s->diag_part);
     NSE_TRY(hipGetLastError());
-    NSE_TRY(hipMemcpyAsync(s->diag_host, s->diag_part, sizeof(double) * 5 * blocks, hipMemcpyDeviceToHost, s->s));
+    NSE_TRY(hipMemcpyAsync(s->diag_host, s->diag_part, sizeof(double) * NS * blocks, hipMemcpyDeviceToHost, s->s));
     NSE_TRY(hipStreamSynchronize(s->s));
-    double acc[5] = {0, 0, 0, 0, 0};
+    double acc[NS] = {0, 0, 0, 0, 0, 0};
     for (int b = 0; b < blocks; ++b) {
-        for (int q = 0; q < 5; ++q) {
-            const double v = s->diag_host[5 * b + q];
+        for (int q = 0; q < NS; ++q) {
+            const double v = s->diag_host[NS * b + q];
             if (q == 2) acc[q] = (v > acc[q] || v != v) ? v : acc[q];
             else acc[q] += v;
         }
     }
     const double vol = s->g.len[0] * s->g.len[1] * s->g.len[2];
     out[0] = acc[0] / vol;
-    out[1] = acc[2];
-    out[2] = s->p.has_T ? 1.0 + (acc[1] / vol) / s->p.kappa : 0.0;
-    out[3] = s->p.has_T ? (acc[3] + acc[4]) / (s->g.len[0] * s->g.len[1] * 2) : 0.0;
+    out[1] = s->p.has_T ? acc[5] / vol : 0.0;
+    out[2] = acc[2];
+    out[3] = s->p.has_T ? 1.0 + (acc[1] / vol) / s->p.kappa : 0.0;
+    out[4] = s->p.has_T ? (acc[3] + acc[4]) / (s->g.len[0] * s->g.len[1] * 2) : 0.0;
     return SARAS_OK;
+}
+
+saras_status saras_nse_probe(saras_nse *s, int count, const int *ijk, double *out) {
+    if (!s || count < 0 || (count > 0 && (!ijk || !out))) return SARAS_BAD_ARG;
+    if (count == 0) return SARAS_OK;
+    for (int q = 0; q < count; ++q)
+        for (int a = 0; a < 3; ++a)
+            if (ijk[3 * q + a] < 0 || ijk[3 * q + a] >= s->G.n[a]) {
+                saras::set_last_error("probe point outside the core");
+                return SARAS_BAD_ARG;
+            }
+    int *dij = nullptr;
+    double *dout = nullptr;
+    if (hipMalloc((void **)&dij, sizeof(int) * 3 * count) != hipSuccess) return SARAS_OOM;
+    if (hipMalloc((void **)&dout, sizeof(double) * 5 * count) != hipSuccess) {
+        (void)hipFree(dij);
+        return SARAS_OOM;
+    }
+    saras_status st = SARAS_OK;
+    if (hipMemcpyAsync(dij, ijk, sizeof(int) * 3 * count, hipMemcpyHostToDevice, s->s) != hipSuccess) st = SARAS_HIP_ERROR;
+    if (!st) {
+        // Vx, Vy, Vz, P, T (T only with a scalar field)
+        const double *f[5] = {s->F[0], s->F[1], s->F[2], s->F[4], s->p.has_T ? s->F[3] : nullptr};
+        hipLaunchKernelGGL(saras::nse::k_probe, dim3((count + 255) / 256), dim3(256), 0, s->s, s->G, f[0], f[1], f[2],
+                           f[3], f[4], count, dij, dout);
+        if (hipGetLastError() != hipSuccess) st = SARAS_HIP_ERROR;
+    }
+    if (!st && hipMemcpyAsync(out, dout, sizeof(double) * 5 * count, hipMemcpyDeviceToHost, s->s) != hipSuccess)
+        st = SARAS_HIP_ERROR;
+    if (!st && hipStreamSynchronize(s->s) != hipSuccess) st = SARAS_HIP_ERROR;
+    (void)hipFree(dij);
+    (void)hipFree(dout);
+    return st;
 }
 
 }  // extern "C"

@@ -89,7 +89,7 @@ EXPORTS = [
     "saras_bench_stream_copy", "saras_nse_create", "saras_nse_destroy", "saras_nse_set_fields",
     "saras_nse_get_fields", "saras_nse_step", "saras_nse_stage", "saras_nse_rhs", "saras_nse_get_rhs",
     "saras_nse_solve_field", "saras_field_op", "saras_nse_diag", "saras_nse_bench_steps",
-    "saras_nse_cfl", "saras_nse_sgs_diag", "saras_mg_set_graphs", "saras_mg_footprint", "saras_server_create", "saras_server_submit",
+    "saras_nse_cfl", "saras_nse_sgs_diag", "saras_nse_tsdata", "saras_nse_probe", "saras_mg_set_graphs", "saras_mg_footprint", "saras_server_create", "saras_server_submit",
     "saras_server_poll", "saras_server_drain", "saras_server_clock", "saras_server_mem_used",
     "saras_server_destroy",
 ]
@@ -139,6 +139,8 @@ def lib():
         L.saras_nse_solve_field.argtypes = [vp, C.c_int, C.c_double, C.POINTER(C.c_int)]
         L.saras_field_op.argtypes = [vp, C.c_int, vp, vp, C.c_int]
         L.saras_nse_diag.argtypes = [vp, dp]
+        L.saras_nse_tsdata.argtypes = [vp, dp]
+        L.saras_nse_probe.argtypes = [vp, C.c_int, C.POINTER(C.c_int), dp]
         L.saras_nse_bench_steps.argtypes = [vp, C.c_int, dp]
         L.saras_nse_cfl.argtypes = [vp, C.c_double, C.c_double, dp]
         L.saras_nse_sgs_diag.argtypes = [vp, dp, dp]
@@ -451,6 +453,19 @@ class Nse:
         v = np.zeros(4)
         check(lib().saras_nse_diag(self.h, _dp(v)))
         return dict(ke=v[0], divmax=v[1], nuvol=v[2], nuwall=v[3])
+
+    def tsdata(self):
+        """The quantities of a TimeSeries.dat line (tseries.cc:209-490)."""
+        v = np.zeros(5)
+        check(lib().saras_nse_tsdata(self.h, _dp(v)))
+        return dict(ke=v[0], te=v[1], divmax=v[2], nuvol=v[3], nuwall=v[4])
+
+    def probe(self, points):
+        """Vx, Vy, Vz, P, T at core cells `points` ((n, 3) integer indices)."""
+        ijk = np.ascontiguousarray(np.asarray(points, dtype=np.int32).reshape(-1, 3))
+        out = np.zeros((ijk.shape[0], 5))
+        check(lib().saras_nse_probe(self.h, ijk.shape[0], ijk.ctypes.data_as(C.POINTER(C.c_int)), _dp(out)))
+        return out
 
     def sgs(self):
         """Sub-grid diagnostics of the last LES evaluation and the run time."""

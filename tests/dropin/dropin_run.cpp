@@ -2,9 +2,11 @@
 // reference's callers use multigrid_d3 and lsRK3_d3: one mgSolve on a
 // caller-owned array, then time steps of a Rayleigh-Benard run.
 //
-// usage: dropin_run <n> <steps> <in.bin> <out.bin>
+// usage: dropin_run <n> <steps> <in.bin> <out.bin> [<ts.dat> <probes.dat>]
 //   in.bin : RHS b, then T, each (n+4)^3 doubles (pad 2)
 //   out.bin: mgSolve result x, then Vx, Vy, Vz, P, T after the steps
+//   ts.dat, probes.dat: TimeSeries.dat and ProbesData.dat of the steps
+//   (tseries / probes), probes at (1, 2, 3), (n-1, 0, n/2), (5, 5, 0)
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -13,8 +15,8 @@
 #include "saras_dropin.hpp"
 
 int main(int argc, char **argv) {
-    if (argc != 5) {
-        std::fprintf(stderr, "usage: %s n steps in.bin out.bin\n", argv[0]);
+    if (argc != 5 && argc != 7) {
+        std::fprintf(stderr, "usage: %s n steps in.bin out.bin [ts.dat probes.dat]\n", argv[0]);
         return 2;
     }
     const int n = std::atoi(argv[1]), steps = std::atoi(argv[2]), pad = 2;
@@ -71,7 +73,18 @@ int main(int argc, char **argv) {
         std::vector<double> zero(full, 0.0), one(full, 1.0);
         saras_amd::lsRK3 ts(g, mp, p);
         ts.setFields(zero.data(), zero.data(), zero.data(), one.data(), T.data());
-        for (int s = 0; s < steps; ++s) ts.timeAdvance();
+        if (argc == 7) {
+            saras_amd::tseries tw(ts, argv[5], p.nu, 1.0, true, false, false, nullptr);
+            saras_amd::probes pw(ts, argv[6], {1, 2, 3, n - 1, 0, n / 2, 5, 5, 0}, true);
+            tw.writeTSHeader();
+            for (int s = 0; s < steps; ++s) {
+                ts.timeAdvance();
+                tw.writeTSData((s + 1) * p.dt, p.dt);
+                pw.probeData((s + 1) * p.dt);
+            }
+        } else {
+            for (int s = 0; s < steps; ++s) ts.timeAdvance();
+        }
         std::vector<std::vector<double>> out(5, std::vector<double>(full, 0.0));
         ts.getFields(out[0].data(), out[1].data(), out[2].data(), out[3].data(), out[4].data());
         FILE *o = std::fopen(argv[4], "wb");

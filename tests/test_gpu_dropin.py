@@ -1,6 +1,8 @@
 """The C++ drop-in classes driven by a C++ caller (tests/dropin/dropin_run.cpp):
 one multigrid_d3::mgSolve on a caller array and two lsRK3_d3::timeAdvance
-steps of a 16^3 Rayleigh-Benard run, compared bit for bit with the oracle."""
+steps of a 16^3 Rayleigh-Benard run, compared bit for bit with the oracle,
+with the TimeSeries.dat and ProbesData.dat lines that tseries and probes
+write after each step compared with the same format of the oracle's values."""
 import os
 import subprocess
 
@@ -28,10 +30,12 @@ def test_dropin_matches_oracle(tmp_path):
     T[:] = 1.0 - z[None, None, :]
     T[c] += rng.uniform(-1e-3, 1e-3, (n, n, n))
     inp, out = tmp_path / "in.bin", tmp_path / "out.bin"
+    tsf, prf = tmp_path / "ts.dat", tmp_path / "probes.dat"
     with open(inp, "wb") as f:
         f.write(b.tobytes())
         f.write(T.tobytes())
-    r = subprocess.run([EXE, str(n), str(steps), str(inp), str(out)], capture_output=True, text=True, timeout=120)
+    r = subprocess.run([EXE, str(n), str(steps), str(inp), str(out), str(tsf), str(prf)], capture_output=True,
+                       text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     res = np.fromfile(out, dtype=np.float64).reshape(6, *sh)
 
@@ -44,8 +48,32 @@ def test_dropin_matches_oracle(tmp_path):
     so = O.Nse(O.nse_desc(g, mg=O.mg_params(bc=O.BC_NEUMANN, **mk), nu=nu, kappa=nu, dt=1e-3))
     zero = np.zeros(sh)
     so.set(vx=zero, vy=zero, vz=zero, p=np.ones(sh), t=T, pp=zero)
-    for _ in range(steps):
+    ts_lines, pr_lines = ["#VARIABLES = Time, Re, Nu (wall), Nu (vol), Total KE, Total TE, Divergence, dt"], \
+        ["#VARIABLES = Time, X, Y, Z, Vx, Vy, Vz, P, T"]
+    pts = [(1, 2, 3), (n - 1, 0, n // 2), (5, 5, 0)]
+    dt = 1e-3
+    for s in range(steps):
         assert so.step() == 0
+        # tseries::writeTSData, scalar columns (tseries.cc:466-482)
+        dg, te, t = so.diag(), so.thermal_energy(), (s + 1) * dt
+        vals = [np.sqrt(2.0 * dg["ke"]) / nu, dg["nuwall"], dg["nuvol"], dg["ke"], te, dg["divmax"], dt]
+        ts_lines.append(f"{t:9.4f}" + "".join(f"{v:20.8f}" for v in vals))
+        # probes::probeData (probes.cc:151-170)
+        f = so.get()
+        for (i, j, k) in pts:
+            v = [f[name][i + pad, j + pad, k + pad] for name in ("vx", "vy", "vz", "p", "t")]
+            pr_lines.append(f"{t:6.4f}\t{i:4d}\t{j}\t{k}\t" + "\t".join(f"{x:16.8f}" for x in v))
     fo = so.get()
     for q, k in enumerate(("vx", "vy", "vz", "p", "t")):
         assert np.array_equal(res[1 + q][c], fo[k][c]), k
+    # the written files: the reference's text format, values from the oracle
+    # run (sums of the diagnostics may differ in the last bits, far below the
+    # printed 8 decimals)
+    for got, want in ((tsf.read_text().splitlines(), ts_lines), (prf.read_text().splitlines(), pr_lines)):
+        assert len(got) == len(want), (got, want)
+        for g, w in zip(got, want):
+            if g != w:
+                assert len(g) == len(w) and g.split()[:1] == w.split()[:1], (g, w)
+                gv = [float(x) for x in g.replace("\t", " ").split()]
+                wv = [float(x) for x in w.replace("\t", " ").split()]
+                assert np.allclose(gv, wv, rtol=0, atol=1.5e-8), (g, w)

@@ -140,6 +140,19 @@ def test_rbc_steps_bitexact():
     assert do["divmax"] == ds["divmax"]
     for k in ("ke", "nuvol", "nuwall"):
         assert abs(do[k] - ds[k]) <= 1e-12 * abs(do[k]), k
+    # the time-series quantities (tseries.cc:295-490) and probe values (probes.cc:185-205)
+    td = ss.tsdata()
+    assert td["divmax"] == do["divmax"] and abs(td["ke"] - do["ke"]) <= 1e-12 * do["ke"]
+    te = so.thermal_energy()
+    assert te > 0 and abs(td["te"] - te) <= 1e-12 * te
+    pts = np.array([(0, 0, 0), (15, 7, 3), (4, 15, 15)])
+    pv = ss.probe(pts)
+    fo = so.get()
+    for q, (i, j, k) in enumerate(pts):
+        for f, name in enumerate(("vx", "vy", "vz", "p", "t")):
+            assert pv[q, f] == fo[name][i + pad, j + pad, k + pad], (q, name)
+    with pytest.raises(S.SarasError):
+        ss.probe([(16, 0, 0)])
 
 
 def test_rbc_rows_of_192_cells_bitexact():

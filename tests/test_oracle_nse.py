@@ -352,6 +352,24 @@ def test_diagnostics_conduction_and_uniform_flow():
     assert dg["divmax"] == 0.0
 
 
+def test_thermal_energy_conduction_and_offset():
+    """tseries.cc:410-446: the thermal energy of the conduction profile
+    T = 1 - z is zero (theta = T + z - 1 vanishes to rounding), and a uniform
+    offset c gives 1/2 c^2 (the cell volumes dXi / xi_x sum to the box volume
+    to the metric's accuracy, as for the kinetic energy above)."""
+    n, p = 16, 2
+    g, d, s = make(n, periodic=(1, 1, 0), gtype=(0, 0, 2), beta=(1, 1, 1.5), has_T=1,
+                   kappa=1e-3, nu=1e-3)
+    z = O.field_metrics(g, 2)["x"]
+    sh = s.shape
+    zero = np.zeros(sh)
+    s.set(vx=zero, vy=zero, vz=zero, p=zero, t=along(1.0 - z, 2, sh), pp=zero)
+    assert s.thermal_energy() < 1e-30
+    c = 0.2
+    s.set(vx=zero, vy=zero, vz=zero, p=zero, t=along(1.0 - z + c, 2, sh), pp=zero)
+    assert abs(s.thermal_energy() - 0.5 * c * c) < 0.5 * c * c * 1e-3
+
+
 def test_max_iterations_formula():
     g, d, s = make(16)
     assert s.max_iterations == int(np.log(16 ** 3) ** 3)

@@ -2,21 +2,27 @@
 // reference's callers use multigrid_d3 and lsRK3_d3: one mgSolve on a
 // caller-owned array, then time steps of a Rayleigh-Benard run.
 //
-// usage: dropin_run <n> <steps> <in.bin> <out.bin> [<ts.dat> <probes.dat>]
+// usage: dropin_run <n> <steps> <in.bin> <out.bin> [<ts.dat> <probes.dat> [<h5dir>]]
 //   in.bin : RHS b, then T, each (n+4)^3 doubles (pad 2)
 //   out.bin: mgSolve result x, then Vx, Vy, Vz, P, T after the steps
 //   ts.dat, probes.dat: TimeSeries.dat and ProbesData.dat of the steps
 //   (tseries / probes), probes at (1, 2, 3), (n-1, 0, n/2), (5, 5, 0)
+//   h5dir: (built with HDF5) the solution and restart files after the steps;
+//   the restart file is read into a second solver, whose fields must equal
+//   the first one's bit for bit
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
 
 #include "saras_dropin.hpp"
+#ifdef SARAS_WITH_HDF5
+#include "saras_h5.hpp"
+#endif
 
 int main(int argc, char **argv) {
-    if (argc != 5 && argc != 7) {
-        std::fprintf(stderr, "usage: %s n steps in.bin out.bin [ts.dat probes.dat]\n", argv[0]);
+    if (argc != 5 && argc != 7 && argc != 8) {
+        std::fprintf(stderr, "usage: %s n steps in.bin out.bin [ts.dat probes.dat [h5dir]]\n", argv[0]);
         return 2;
     }
     const int n = std::atoi(argv[1]), steps = std::atoi(argv[2]), pad = 2;
@@ -73,7 +79,7 @@ int main(int argc, char **argv) {
         std::vector<double> zero(full, 0.0), one(full, 1.0);
         saras_amd::lsRK3 ts(g, mp, p);
         ts.setFields(zero.data(), zero.data(), zero.data(), one.data(), T.data());
-        if (argc == 7) {
+        if (argc >= 7) {
             saras_amd::tseries tw(ts, argv[5], p.nu, 1.0, true, false, false, nullptr);
             saras_amd::probes pw(ts, argv[6], {1, 2, 3, n - 1, 0, n / 2, 5, 5, 0}, true);
             tw.writeTSHeader();
@@ -92,6 +98,28 @@ int main(int argc, char **argv) {
         std::fwrite(x.data(), sizeof(double), full, o);
         for (auto &v : out) std::fwrite(v.data(), sizeof(double), full, o);
         std::fclose(o);
+        if (argc == 8) {
+#ifdef SARAS_WITH_HDF5
+            const double time = steps * p.dt;
+            saras_amd::writer w(ts, g, true, argv[7]);
+            w.writeSolution(time);
+            w.writeRestart(time);
+            saras_amd::lsRK3 ts2(g, mp, p);
+            saras_amd::reader r(ts2, g, true, argv[7]);
+            if (r.readRestart() != time) return 6;
+            std::vector<std::vector<double>> back(5, std::vector<double>(full, 0.0));
+            ts2.getFields(back[0].data(), back[1].data(), back[2].data(), back[3].data(), back[4].data());
+            for (int q = 0; q < 5; ++q)
+                for (int i = 0; i < n; ++i)
+                    for (int j = 0; j < n; ++j)
+                        for (int k = 0; k < n; ++k) {
+                            const size_t at = ((size_t)(i + pad) * (n + 2 * pad) + (j + pad)) * (n + 2 * pad) + (k + pad);
+                            if (back[q][at] != out[q][at]) return 7;
+                        }
+#else
+            return 8;
+#endif
+        }
     } catch (const saras_amd::error &e) {
         std::fprintf(stderr, "saras error %d: %s\n", (int)e.status, e.what());
         return 5;

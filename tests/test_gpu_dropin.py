@@ -2,8 +2,13 @@
 one multigrid_d3::mgSolve on a caller array and two lsRK3_d3::timeAdvance
 steps of a 16^3 Rayleigh-Benard run, compared bit for bit with the oracle,
 with the TimeSeries.dat and ProbesData.dat lines that tseries and probes
-write after each step compared with the same format of the oracle's values."""
+write after each step compared with the same format of the oracle's values,
+and (built with HDF5) the solution and restart files of saras_h5.hpp: their
+datasets and extents, and a restart read back into a second solver bit for
+bit."""
 import os
+import re
+import shutil
 import subprocess
 
 import numpy as np
@@ -15,6 +20,35 @@ pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "tests", "dropin", "_build", "dropin_run")
+
+
+def check_h5_layout(h5dir, n, time):
+    """The files follow writer.cc:331-478: Soln_TTTT.TTTT.h5 with X, Y, Z
+    (n core cell centres each), a scalar Time and (n, n, n) fields; the
+    restart file without the coordinates.  Read with the HDF5 tools' h5dump
+    when the image has them."""
+    soln = h5dir / ("Soln_%09.4f.h5" % time)
+    assert soln.exists() and (h5dir / "restartFile.h5").exists()
+    h5dump = shutil.which("h5dump") or ("/opt/conda/bin/h5dump" if os.path.exists("/opt/conda/bin/h5dump") else None)
+    if not h5dump:
+        return
+    for path, names in ((soln, ["P", "T", "Time", "Vx", "Vy", "Vz", "X", "Y", "Z"]),
+                        (h5dir / "restartFile.h5", ["P", "T", "Time", "Vx", "Vy", "Vz"])):
+        r = subprocess.run([h5dump, "-H", str(path)], capture_output=True, text=True, timeout=60)
+        assert r.returncode == 0, r.stderr
+        found = re.findall(r'DATASET "([^"]+)" \{\s*DATATYPE\s+H5T_IEEE_F64LE\s*DATASPACE\s+(\w+)\s*(?:\{ \( ([0-9, ]+) \))?',
+                           r.stdout)
+        got = {name: (kind, dims) for name, kind, dims in found}
+        assert sorted(got) == names, r.stdout
+        for k in ("Vx", "Vy", "Vz", "P", "T"):
+            assert got[k] == ("SIMPLE", f"{n}, {n}, {n}"), got[k]
+        assert got["Time"][0] == "SCALAR"
+        for k in ("X", "Y", "Z"):
+            if k in got:
+                assert got[k] == ("SIMPLE", f"{n}"), got[k]
+    r = subprocess.run([h5dump, "-d", "Time", str(soln)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and re.search(r"\(0\): ([0-9.e-]+)", r.stdout), r.stdout
+    assert abs(float(re.search(r"\(0\): ([0-9.e-]+)", r.stdout).group(1)) - time) < 1e-12
 
 
 def test_dropin_matches_oracle(tmp_path):
@@ -34,9 +68,16 @@ def test_dropin_matches_oracle(tmp_path):
     with open(inp, "wb") as f:
         f.write(b.tobytes())
         f.write(T.tobytes())
-    r = subprocess.run([EXE, str(n), str(steps), str(inp), str(out), str(tsf), str(prf)], capture_output=True,
-                       text=True, timeout=120)
-    assert r.returncode == 0, r.stderr
+    h5dir = tmp_path / "h5"
+    h5dir.mkdir()
+    r = subprocess.run([EXE, str(n), str(steps), str(inp), str(out), str(tsf), str(prf), str(h5dir)],
+                       capture_output=True, text=True, timeout=120)
+    # 8: built without HDF5 (no HDF5 in the image); 6/7: the restart file did
+    # not read back bit for bit
+    with_h5 = os.path.isdir(os.path.join(ROOT, "tests", "dropin", "_build", "h5"))
+    assert r.returncode == (0 if with_h5 else 8), (r.returncode, r.stderr)
+    if with_h5:
+        check_h5_layout(h5dir, n, steps * 1e-3)
     res = np.fromfile(out, dtype=np.float64).reshape(6, *sh)
 
     mk = dict(vc_count=20, vc_tol=1e-6, solve_flag=0, mg_tol=1e-2, pre=3, post=6)

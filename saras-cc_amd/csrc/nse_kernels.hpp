@@ -923,13 +923,14 @@ __global__ void __launch_bounds__(256) k_cfl(Geom G, const double *__restrict__ 
 }
 
 // caller array (pad p) -> core
-__global__ void k_field_load(const double *__restrict__ src, long SX, long SY, int p, double *__restrict__ dst,
+// py: the caller array's y pad (0 for a planar run, whose arrays have no y extent)
+__global__ void k_field_load(const double *__restrict__ src, long SX, long SY, int p, int py, double *__restrict__ dst,
                              long sx, long sy, long hz, int nx, int ny, int nz) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y, i = blockIdx.z;
     if (k >= nz) return;
     dst[(long)i * sx + (long)j * sy + (hz ? (long)(k & 1) * hz + (k >> 1) : (long)k)] =
-        src[(long)(i + p) * SX + (long)(j + p) * SY + (k + p)];
+        src[(long)(i + p) * SX + (long)(j + py) * SY + (k + p)];
 }
 
 // core + face ghost cells -> caller array (pad p): every cell with at most one
@@ -937,11 +938,11 @@ __global__ void k_field_load(const double *__restrict__ src, long SX, long SY, i
 // imposeBCs (periodic pads p deep, first wall layer); edge/corner pads and the
 // second wall layer are left as the caller had them.
 template <class FF>
-__global__ void k_field_store(Geom G, FF F, double *__restrict__ dst, long SX, long SY, int p) {
+__global__ void k_field_store(Geom G, FF F, double *__restrict__ dst, long SX, long SY, int p, int py) {
     const int kk = blockIdx.x * blockDim.x + threadIdx.x;   // over nz + 2p
-    const int jj = blockIdx.y, ii = blockIdx.z;               // over ny + 2p, nx + 2p
+    const int jj = blockIdx.y, ii = blockIdx.z;               // over ny + 2py, nx + 2p
     if (kk >= G.n[2] + 2 * p) return;
-    const int c[3] = {ii - p, jj - p, kk - p};
+    const int c[3] = {ii - p, jj - py, kk - p};
     int outside = 0, ax = -1;
     for (int a = 0; a < 3; ++a)
         if (c[a] < 0 || c[a] >= G.n[a]) {

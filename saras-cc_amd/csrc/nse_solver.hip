@@ -77,6 +77,14 @@ struct saras_nse {
     saras_mg_stats mg_stats{};
     double wall_dn = 0.0;
     long full = 0;
+    // planar (2-D, x-z) run: the reference's PLANAR build, y index fixed at 0.
+    // Stored as one y row with y wrapping onto itself and every y metric and
+    // y difference factor zero, so each y term of a stencil is an exact zero.
+    bool planar = false;
+    int ypad() const { return planar ? 0 : g.pad; }
+    // tseries totalVol and the wall cross-section (tseries.cc:76-80, 96-104)
+    double volume() const { return planar ? g.len[0] * g.len[2] : g.len[0] * g.len[1] * g.len[2]; }
+    double wall_area() const { return planar ? g.len[0] : g.len[0] * g.len[1]; }
     int threads = 256;
     // LES (spiral.cc): sub-grid stresses Txx, Tyy, Tzz, Txy, Tyz, Tzx
     double *Tsg[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -111,7 +119,7 @@ struct saras_nse {
     int stages() const { return p.scheme == 1 ? 1 : 3; }
     dim3 grid3() const { return dim3((G.n[2] + threads - 1) / threads, G.n[1], G.n[0]); }
     dim3 grid3_full(int p) const {
-        return dim3((G.n[2] + 2 * p + threads - 1) / threads, G.n[1] + 2 * p, G.n[0] + 2 * p);
+        return dim3((G.n[2] + 2 * p + threads - 1) / threads, G.n[1] + 2 * ypad(), G.n[0] + 2 * p);
     }
     CoreField field(int q) const { return CoreField{F[q], q == 4 ? pbc : bc[q]}; }
     // stress m as a field of the periodic box
@@ -322,21 +330,21 @@ saras_status load_field(saras_nse *s, const double *src, int ptr_kind, double *d
         NSE_TRY(hipMemcpyAsync(s->stage_a, src, s->full * sizeof(double), hipMemcpyHostToDevice, s->s));
         d = s->stage_a;
     }
-    const int p = s->g.pad;
-    const long SY = s->g.n[2] + 2 * p, SX = SY * (s->g.n[1] + 2 * p);
-    hipLaunchKernelGGL(saras::nse::k_field_load, s->grid3(), dim3(s->threads), 0, s->s, d, SX, SY, p, dst, s->G.sx,
-                       s->G.sy, hz, s->G.n[0], s->G.n[1], s->G.n[2]);
+    const int p = s->g.pad, py = s->ypad();
+    const long SY = s->g.n[2] + 2 * p, SX = SY * (s->G.n[1] + 2 * py);
+    hipLaunchKernelGGL(saras::nse::k_field_load, s->grid3(), dim3(s->threads), 0, s->s, d, SX, SY, p, py, dst,
+                       s->G.sx, s->G.sy, hz, s->G.n[0], s->G.n[1], s->G.n[2]);
     NSE_TRY(hipGetLastError());
     if (ptr_kind == SARAS_PTR_HOST) NSE_TRY(hipStreamSynchronize(s->s));
     return SARAS_OK;
 }
 
 __global__ void k_store_core(const double *__restrict__ src, long sx, long sy, double *__restrict__ dst, long SX,
-                             long SY, int p, int nz) {
+                             long SY, int p, int py, int nz) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y, i = blockIdx.z;
     if (k >= nz) return;
-    dst[(long)(i + p) * SX + (long)(j + p) * SY + (k + p)] = src[(long)i * sx + (long)j * sy + k];
+    dst[(long)(i + p) * SX + (long)(j + py) * SY + (k + p)] = src[(long)i * sx + (long)j * sy + k];
 }
 
 // core buffer (+ face ghosts unless core_only) -> caller array
@@ -349,14 +357,14 @@ saras_status store_field(saras_nse *s, FF f, double *dst, int ptr_kind, bool cor
         NSE_TRY(hipMemcpyAsync(s->stage_a, dst, s->full * sizeof(double), hipMemcpyHostToDevice, s->s));
         d = s->stage_a;
     }
-    const int p = s->g.pad;
-    const long SY = s->g.n[2] + 2 * p, SX = SY * (s->g.n[1] + 2 * p);
+    const int p = s->g.pad, py = s->ypad();
+    const long SY = s->g.n[2] + 2 * p, SX = SY * (s->G.n[1] + 2 * py);
     if (core_only)
         hipLaunchKernelGGL(k_store_core, s->grid3(), dim3(s->threads), 0, s->s, f.f, s->G.sx, s->G.sy, d, SX, SY, p,
-                           s->G.n[2]);
+                           py, s->G.n[2]);
     else
         hipLaunchKernelGGL(saras::nse::k_field_store<FF>, s->grid3_full(p), dim3(s->threads), 0, s->s, s->G, f, d, SX,
-                           SY, p);
+                           SY, p, py);
     NSE_TRY(hipGetLastError());
     if (ptr_kind == SARAS_PTR_HOST) {
         NSE_TRY(hipMemcpyAsync(dst, s->stage_a, s->full * sizeof(double), hipMemcpyDeviceToHost, s->s));
@@ -375,9 +383,14 @@ saras_status saras_nse_create(const saras_grid_desc *grid, const saras_mg_params
     *out = nullptr;
     const saras_grid_desc &g = *grid;
     const saras_nse_params &P = *params;
-    if (g.dim != 3) {
-        saras::set_last_error("the time-step path covers 3-D grids");
+    const bool planar = g.dim == 2;
+    if (g.dim != 3 && !planar) {
+        saras::set_last_error("grid dim must be 2 (planar x-z) or 3");
         return SARAS_BAD_SHAPE;
+    }
+    if (planar && (P.dscheme != 1 || P.nlscheme == 3 || P.les || g.n[1] != 1)) {
+        saras::set_last_error("the planar path covers n_y = 1, dScheme 1 and the central or upwind non-linear term");
+        return SARAS_BAD_ARG;
     }
     if ((P.dscheme != 1 && P.dscheme != 2) || P.nlscheme < 1 || P.nlscheme > 3 || g.pad < P.dscheme ||
         (P.scheme != 0 && P.scheme != 1)) {
@@ -385,7 +398,7 @@ saras_status saras_nse_create(const saras_grid_desc *grid, const saras_mg_params
         return SARAS_BAD_ARG;
     }
     for (int a = 0; a < 3; ++a)
-        if (g.n[a] < 4) return SARAS_BAD_SHAPE;
+        if (g.n[a] < 4 && !(planar && a == 1)) return SARAS_BAD_SHAPE;
     if (P.nlscheme == 3 && P.dscheme == 2 && !g.periodic[2] && g.n[1] != g.n[2]) {
         saras::set_last_error("Morinishi with a z wall needs ny == nz (edge test on core.ubound(1), vfield.cc:434)");
         return SARAS_BAD_SHAPE;
@@ -397,6 +410,7 @@ saras_status saras_nse_create(const saras_grid_desc *grid, const saras_mg_params
     saras_nse *s = new saras_nse();
     s->g = g;
     s->p = P;
+    s->planar = planar;
     saras_mg_params mp = *mgp;
     mp.bc = SARAS_BC_NEUMANN;
     saras_status e = saras_mg_create(&g, &mp, &s->mg);
@@ -411,8 +425,12 @@ saras_status saras_nse_create(const saras_grid_desc *grid, const saras_mg_params
         G.n[a] = g.n[a];
         G.per[a] = g.periodic[a];
     }
+    if (planar) {
+        G.n[1] = 1;
+        G.per[1] = 1;
+    }
     G.sy = pitch16(g.n[2]);
-    G.sx = G.sy * g.n[1];
+    G.sx = G.sy * G.n[1];
     {
         double *x0, *b0;
         long sx0, sy0;
@@ -424,13 +442,21 @@ saras_status saras_nse_create(const saras_grid_desc *grid, const saras_mg_params
         }
     }
     G.edge_last[0] = g.n[0] - 1;
-    G.edge_last[1] = g.n[1] - 1;
-    G.edge_last[2] = g.n[1] - 1;   // vfield.cc:434, sfield.cc:295
+    G.edge_last[1] = G.n[1] - 1;
+    G.edge_last[2] = G.n[1] - 1;   // vfield.cc:434, sfield.cc:295
     G.dscheme = P.dscheme;
     double hwall[6];
     const size_t stride = (size_t)std::max(g.n[0], std::max(g.n[1], g.n[2])) + 2;
     std::vector<double> host(12 * stride, 0.0);
     for (int a = 0; a < 3; ++a) {
+        if (planar && a == 1) {
+            // no y direction: xi_y = 1 (a unit cell width in the volume
+            // element), every y metric and y difference factor zero
+            host[(3 * a + 0) * stride] = 1.0;
+            G.ih[a] = G.ih2[a] = G.i2d[a] = G.tsi2h[a] = G.tsih2[a] = 0.0;
+            hwall[2 * a] = hwall[2 * a + 1] = 0.0;
+            continue;
+        }
         std::vector<double> coord, xi_x, xixx, xix2;
         saras::field_axis(g, a, coord, xi_x, xixx, xix2);
         const int p = g.pad, n = g.n[a];
@@ -527,7 +553,7 @@ saras_status saras_nse_create(const saras_grid_desc *grid, const saras_mg_params
     if (hipHostMalloc((void **)&s->diag_host, sizeof(double) * saras::nse::kDiagSums * saras::nse::kDiagBlocks) != hipSuccess)
         return fail(SARAS_OOM);
     if (hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess) return fail(SARAS_HIP_ERROR);
-    s->full = (long)(g.n[0] + 2 * g.pad) * (g.n[1] + 2 * g.pad) * (g.n[2] + 2 * g.pad);
+    s->full = (long)(g.n[0] + 2 * g.pad) * (G.n[1] + 2 * s->ypad()) * (g.n[2] + 2 * g.pad);
     s->tstp = P.dt;
     if (P.les) {
         for (int m = 0; m < 6; ++m) {
@@ -670,6 +696,10 @@ saras_status saras_nse_bench_steps(saras_nse *s, int count, double *ms) {
 saras_status saras_field_op(saras_nse *s, int op, const double *f, double *out, int ptr_kind) {
     if (!s || !f || !out || op < 0 || op > 7) return SARAS_BAD_ARG;
     if (ptr_kind != SARAS_PTR_HOST && ptr_kind != SARAS_PTR_DEVICE) return SARAS_BAD_ARG;
+    if (s->planar) {
+        saras::set_last_error("single field operators cover 3-D handles");
+        return SARAS_BAD_SHAPE;
+    }
     const int p = s->g.pad;
     const long SY = s->g.n[2] + 2 * p, SX = SY * (s->g.n[1] + 2 * p);
     const double *df = f;
@@ -758,12 +788,12 @@ saras_status saras_nse_tsdata(saras_nse *s, double out[5]) {
             else acc[q] += v;
         }
     }
-    const double vol = s->g.len[0] * s->g.len[1] * s->g.len[2];
+    const double vol = s->volume();
     out[0] = acc[0] / vol;
     out[1] = s->p.has_T ? acc[5] / vol : 0.0;
     out[2] = acc[2];
     out[3] = s->p.has_T ? 1.0 + (acc[1] / vol) / s->p.kappa : 0.0;
-    out[4] = s->p.has_T ? (acc[3] + acc[4]) / (s->g.len[0] * s->g.len[1] * 2) : 0.0;
+    out[4] = s->p.has_T ? (acc[3] + acc[4]) / (s->wall_area() * 2) : 0.0;
     return SARAS_OK;
 }
 

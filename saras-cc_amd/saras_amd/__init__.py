@@ -391,7 +391,9 @@ class Nse:
         self.g, self.mgp, self.p = g, mg, p
         self.h = C.c_void_p()
         check(lib().saras_nse_create(C.byref(g), C.byref(mg), C.byref(p), C.byref(self.h)))
-        self.shape = (g.n[0] + 2 * g.pad, g.n[1] + 2 * g.pad, g.n[2] + 2 * g.pad)
+        # planar (dim 2) arrays have no y extent beyond the single row
+        ypad = 0 if g.dim == 2 else g.pad
+        self.shape = (g.n[0] + 2 * g.pad, g.n[1] + 2 * ypad, g.n[2] + 2 * g.pad)
 
     def close(self):
         if self.h:

@@ -1,0 +1,74 @@
+"""Planar (2-D, x-z) runs of the time-step path: the reference's PLANAR
+build (lsRK3_d2 / eulerCN_d2 with hydro_d2) restated as one y row whose y
+terms are exact zeros.  Pinned by what the mathematics fixes (survey 8(c)):
+the 2-D Taylor-Green vortex is an exact Navier-Stokes solution whose kinetic
+energy decays as exp(-4 nu t) in a (2 pi)^2 box, and by a 3-D run of the same
+flow, uniform in y, which must evolve the same x-z fields to the solvers'
+tolerances."""
+import numpy as np
+import pytest
+
+import saras_amd as S
+
+pytestmark = pytest.mark.gpu
+
+
+def tg_fields(shape, n, pad, L, dim):
+    x = (np.arange(-pad, n + pad) + 0.5) * (L / n)
+    sx, cx = np.sin(x), np.cos(x)
+    ny = shape[1]
+    u = np.broadcast_to((sx[:, None] * cx[None, :])[:, None, :], shape).copy()
+    w = np.broadcast_to((-cx[:, None] * sx[None, :])[:, None, :], shape).copy()
+    zero = np.zeros(shape)
+    return dict(vx=u, vy=zero.copy(), vz=w, p=zero.copy(), t=zero.copy(), pp=zero.copy())
+
+
+def run_tg(dim, scheme, n=64, steps=40, nu=0.05, dt=2e-3):
+    pad = 1
+    L = 2 * np.pi
+    ny = 1 if dim == 2 else 4
+    g = S.grid_desc((n, ny, n), dim=dim, periodic=(1, 1, 1), length=(L, L if dim == 3 else 1.0, L), pad=pad)
+    mg = S.mg_params(vc_count=30, vc_tol=1e-10, solve_flag=0, mg_tol=1e-2, pre=3, post=3, bc=S.BC_NEUMANN,
+                     precision=S.PREC_FP64)
+    prm = S.nse_params(dscheme=1, nlscheme=1, has_T=0, nu=nu, kappa=nu, dt=dt, cn_tol=1e-12, scheme=scheme,
+                       vbc=[([S.FBC_DIRICHLET] * 6, [0.0] * 6)] * 3)
+    s = S.Nse(g, mg, prm)
+    s.set(**tg_fields(s.shape, n, pad, L, dim))
+    ke = [s.diag()["ke"]]
+    for _ in range(steps):
+        r, st = s.step()
+        S.check(r)
+        ke.append(s.diag()["ke"])
+    f = s.get()
+    s.close()
+    return np.array(ke), f
+
+
+@pytest.mark.parametrize("scheme", [0, 1], ids=["rk3", "euler_cn"])
+def test_planar_taylor_green_decay(scheme):
+    nu, dt, steps = 0.05, 2e-3, 40
+    ke, _ = run_tg(2, scheme, nu=nu, dt=dt, steps=steps)
+    t = dt * np.arange(steps + 1)
+    exact = ke[0] * np.exp(-4 * nu * t)
+    assert abs(ke[0] - 0.25) < 1e-3          # <u^2 + w^2>/2 of the vortex
+    # second-order stencils on 64^2 and first/third-order time stepping
+    assert np.max(np.abs(ke / exact - 1)) < 2e-4, np.max(np.abs(ke / exact - 1))
+
+
+def test_planar_equals_y_uniform_3d():
+    ke2, f2 = run_tg(2, 0, n=32, steps=10)
+    ke3, f3 = run_tg(3, 0, n=32, steps=10)
+    assert np.max(np.abs(ke2 / ke3 - 1)) < 1e-9
+    c2 = (slice(1, -1), 0, slice(1, -1))
+    for k in ("vx", "vz"):
+        a, b = f2[k][c2], f3[k][1:-1, 1, 1:-1]
+        assert np.max(np.abs(a - b)) < 1e-8 * np.max(np.abs(b)), k
+    # the 3-D pressure iterate is y-uniform only to rounding (red-black order)
+    assert np.max(np.abs(f3["vy"][1:-1, 1:-1, 1:-1])) < 1e-12
+
+
+def test_planar_rejects_unsupported_schemes():
+    g = S.grid_desc((16, 1, 16), dim=2, periodic=(1, 1, 1), pad=2)
+    mg = S.mg_params(precision=S.PREC_FP64, bc=S.BC_NEUMANN)
+    with pytest.raises(S.SarasError):
+        S.Nse(g, mg, S.nse_params(dscheme=2, nlscheme=1, has_T=0, nu=0.01, kappa=0.01, dt=1e-3))

@@ -201,6 +201,9 @@ typedef struct {
     double central_bias;     /* "Central Bias" of the upwind scheme (0.8 in the reference deck) */
     int les;                 /* "LES Model" 1: stretched spiral vortex sub-grid stress, added once
                                 the run time exceeds 5 time steps (lsRK3_d3.cc:238); periodic boxes */
+    double mean_pgrad;       /* "Mean Pressure Gradient" of force type 5, constantPGrad (force.h:148-153,
+                                hydro.cc:268-271): added to the x-momentum right-hand side of the core.
+                                A hydrodynamics forcing: must be 0 when has_T is set */
 } saras_nse_params;
 
 typedef struct {
@@ -214,10 +217,11 @@ typedef struct saras_nse saras_nse;
 /* lsRK3_d3::lsRK3_d3 + timestep::timestep (lsRK3_d3.cc:58-70,
  * timestep.cc:57-122) with the fields of hydro/scalar (vfield, sfield).
  * mg: the "Multigrid" keys of the pressure solve (bc is forced to Neumann,
- * poisson.cc:87).  SARAS_BAD_SHAPE for grids the path does not cover:
- * planar grids, pad < dscheme, and Morinishi runs with a wall in z and
- * ny != nz (the reference's z edge test reads core.ubound(1),
- * vfield.cc:434). */
+ * poisson.cc:87).  grid->dim 2 is the PLANAR (x-z) build: n[1] = 1, caller
+ * arrays (nx+2p, 1, nz+2p), dscheme 1 with the central or upwind non-linear
+ * term (lsRK3_d2.cc, eulerCN_d2.cc).  SARAS_BAD_SHAPE for cases the path does
+ * not cover: pad < dscheme, and Morinishi runs with a wall in z and ny != nz
+ * (the reference's z edge test reads core.ubound(1), vfield.cc:434). */
 saras_status saras_nse_create(const saras_grid_desc *grid, const saras_mg_params *mg,
                               const saras_nse_params *params, saras_nse **out);
 saras_status saras_nse_destroy(saras_nse *s);

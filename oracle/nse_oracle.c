@@ -4,7 +4,8 @@
  * plainvf/plainsf algebra), lib/boundary (dirichlet.cc, neumann.cc),
  * lib/parallel/mpidata.cc face sync (periodic self-wrap), lib/timestep
  * (timestep.cc, predictors.cc, lsRK3_d3.cc, eulerCN_d3.cc), lib/force
- * (buoyantForce.cc) and the lib/io/tseries.cc diagnostics.
+ * (buoyantForce.cc, constantPGrad in force.h) and the lib/io/tseries.cc
+ * diagnostics.
  *
  * TEST INFRASTRUCTURE ONLY (see saras_oracle.h).
  *
@@ -709,6 +710,10 @@ static int euler_rhs(or_nse *s)
     if (hasT)
         for (int c = 0; c < 3; c++)
             for (size_t q = 0; q < n; q++) s->nse[c][q] -= s->d.buoy[c] * s->T[q];
+    /* constantPGrad::addForcing: Hv.Vx(core) += meanPGrad (force.h:152) */
+    if (s->d.mean_pgrad != 0.0) {
+        CORE_LOOP(s) s->nse[0][FX(s, i, j, k)] += s->d.mean_pgrad;
+    }
     if (les_active(s) && compute_sg(s, s->nse)) return 3;
     for (int c = 0; c < 3; c++) fill(s->pg[c], n, 0.0);
     CORE_LOOP(s) {
@@ -756,6 +761,9 @@ int or_nse_rhs(or_nse *s, int stage)
         fill(s->ts, n, 0.0);
         for (int c = 0; c < 3; c++)
             for (size_t q = 0; q < n; q++) s->tv[c][q] -= s->d.buoy[c] * s->T[q];
+    }
+    if (s->d.mean_pgrad != 0.0) {
+        CORE_LOOP(s) s->tv[0][FX(s, i, j, k)] += s->d.mean_pgrad;
     }
     for (int c = 0; c < 3; c++) fill(s->pg[c], n, 0.0);
     CORE_LOOP(s) {

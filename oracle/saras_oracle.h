@@ -143,6 +143,7 @@ typedef struct {
     double pe_limit;       /* "Peclet Limit" (upwind scheme) */
     double central_bias;   /* "Central Bias" omega (upwind scheme) */
     int les;               /* "LES Model" 1: stretched spiral vortex sub-grid stress (fully periodic runs) */
+    double mean_pgrad;     /* "Mean Pressure Gradient" of constantPGrad, force type 5 (force.h:148-153) */
 } or_nse_desc;
 
 typedef struct or_nse or_nse;

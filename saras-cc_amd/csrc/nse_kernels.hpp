@@ -316,6 +316,7 @@ struct StageCoef {
     double gamma;      // gammRK3
     double dt;
     double buoy[3];    // gTerms
+    double mpg;        // constantPGrad's mean pressure gradient (x component)
     int stage;
     int has_T;
     int nlscheme;
@@ -341,6 +342,7 @@ __device__ __forceinline__ void rhs_v_cell(const Geom &G, const StageCoef &c, co
     nse += c.a_nu * diff;
     double f = 0.0;
     if (c.has_T) f -= c.buoy[C] * T.at(G, i, j, k);
+    if (C == 0 && c.mpg != 0.0) f += c.mpg;   // constantPGrad::addForcing (force.h:152)
     f -= d1(G, P, C, i, j, k);
     nse += c.ab * f;
     double h = nlin(G, c.nlscheme, V, U, i, j, k, c.nu);
@@ -404,7 +406,7 @@ __global__ void __launch_bounds__(256) k_rhs_t(Geom G, StageCoef c, CoreField V0
 // accumulated in one value in the reference's order.
 template <int C, class F>
 __device__ __forceinline__ void euler_v_cell(const Geom &G, double k_nu, double nu, double dt, double buoy, int has_T,
-                                             int nlscheme, const F &V0, const F &V1, const F &V2, const F &T,
+                                             double mpg, int nlscheme, const F &V0, const F &V1, const F &V2, const F &T,
                                              const F &P, double *__restrict__ R, int les, const PerField &Sa,
                                              const PerField &Sb, const PerField &Sc, int i, int j, int k) {
     const F V[3] = {V0, V1, V2};
@@ -413,6 +415,7 @@ __device__ __forceinline__ void euler_v_cell(const Geom &G, double k_nu, double 
     r *= k_nu;
     r = nlin_acc(G, nlscheme, V, U, i, j, k, r, nu);
     if (has_T) r -= buoy * T.at(G, i, j, k);
+    if (mpg != 0.0) r += mpg;   // constantPGrad::addForcing (force.h:152), x component only
     if (les) r = r - (d1(G, Sa, 0, i, j, k) + d1(G, Sb, 1, i, j, k) + d1(G, Sc, 2, i, j, k));
     r -= d1(G, P, C, i, j, k);
     r *= dt;
@@ -422,7 +425,7 @@ __device__ __forceinline__ void euler_v_cell(const Geom &G, double k_nu, double 
 
 template <int C>
 __global__ void __launch_bounds__(256) k_euler_v(Geom G, double k_nu, double nu, double dt, double buoy, int has_T,
-                                                 int nlscheme, CoreField V0, CoreField V1, CoreField V2,
+                                                 double mpg, int nlscheme, CoreField V0, CoreField V1, CoreField V2,
                                                  CoreField T, CoreField P, double *__restrict__ R, int les,
                                                  PerField Sa, PerField Sb, PerField Sc) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -430,10 +433,10 @@ __global__ void __launch_bounds__(256) k_euler_v(Geom G, double k_nu, double nu,
     const int i = blockIdx.z;
     if (k >= G.n[2]) return;
     if (interior2(G, i, j, k))
-        euler_v_cell<C>(G, k_nu, nu, dt, buoy, has_T, nlscheme, fast_of(V0), fast_of(V1), fast_of(V2), fast_of(T),
+        euler_v_cell<C>(G, k_nu, nu, dt, buoy, has_T, mpg, nlscheme, fast_of(V0), fast_of(V1), fast_of(V2), fast_of(T),
                         fast_of(P), R, les, Sa, Sb, Sc, i, j, k);
     else
-        euler_v_cell<C>(G, k_nu, nu, dt, buoy, has_T, nlscheme, V0, V1, V2, T, P, R, les, Sa, Sb, Sc, i, j, k);
+        euler_v_cell<C>(G, k_nu, nu, dt, buoy, has_T, mpg, nlscheme, V0, V1, V2, T, P, R, les, Sa, Sb, Sc, i, j, k);
 }
 
 // ... and for the scalar (eulerCN_d3.cc:188-232); its forcing is zero

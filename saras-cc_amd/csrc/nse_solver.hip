@@ -172,14 +172,14 @@ saras_status rhs_euler(saras_nse *s) {
         if (e) return e;
     }
     hipLaunchKernelGGL(saras::nse::k_euler_v<0>, gr, bl, 0, s->s, s->G, k_nu, s->p.nu, dt, s->p.buoy[0], s->p.has_T,
-                       s->p.nlscheme, s->field(0), s->field(1), s->field(2), s->field(3), s->field(4), s->R[0], les,
-                       s->sg(0, 0), s->sg(0, 1), s->sg(0, 2));
+                       s->p.mean_pgrad, s->p.nlscheme, s->field(0), s->field(1), s->field(2), s->field(3), s->field(4),
+                       s->R[0], les, s->sg(0, 0), s->sg(0, 1), s->sg(0, 2));
     hipLaunchKernelGGL(saras::nse::k_euler_v<1>, gr, bl, 0, s->s, s->G, k_nu, s->p.nu, dt, s->p.buoy[1], s->p.has_T,
-                       s->p.nlscheme, s->field(0), s->field(1), s->field(2), s->field(3), s->field(4), s->R[1], les,
-                       s->sg(1, 0), s->sg(1, 1), s->sg(1, 2));
+                       0.0, s->p.nlscheme, s->field(0), s->field(1), s->field(2), s->field(3), s->field(4),
+                       s->R[1], les, s->sg(1, 0), s->sg(1, 1), s->sg(1, 2));
     hipLaunchKernelGGL(saras::nse::k_euler_v<2>, gr, bl, 0, s->s, s->G, k_nu, s->p.nu, dt, s->p.buoy[2], s->p.has_T,
-                       s->p.nlscheme, s->field(0), s->field(1), s->field(2), s->field(3), s->field(4), s->R[2], les,
-                       s->sg(2, 0), s->sg(2, 1), s->sg(2, 2));
+                       0.0, s->p.nlscheme, s->field(0), s->field(1), s->field(2), s->field(3), s->field(4),
+                       s->R[2], les, s->sg(2, 0), s->sg(2, 1), s->sg(2, 2));
     if (s->p.has_T)
         hipLaunchKernelGGL(saras::nse::k_euler_t, gr, bl, 0, s->s, s->G, s->p.kappa * 0.5, s->p.kappa, dt, s->p.nlscheme,
                            s->field(0), s->field(1), s->field(2), s->field(3), s->R[3]);
@@ -198,6 +198,7 @@ saras_status rhs(saras_nse *s, int stage) {
     c.gamma = kGamma[stage];
     c.dt = s->p.dt;
     for (int q = 0; q < 3; ++q) c.buoy[q] = s->p.buoy[q];
+    c.mpg = s->p.mean_pgrad;
     c.stage = stage;
     c.has_T = s->p.has_T;
     c.nlscheme = s->p.nlscheme;
@@ -402,6 +403,10 @@ saras_status saras_nse_create(const saras_grid_desc *grid, const saras_mg_params
     if (P.nlscheme == 3 && P.dscheme == 2 && !g.periodic[2] && g.n[1] != g.n[2]) {
         saras::set_last_error("Morinishi with a z wall needs ny == nz (edge test on core.ubound(1), vfield.cc:434)");
         return SARAS_BAD_SHAPE;
+    }
+    if (P.has_T && P.mean_pgrad != 0.0) {
+        saras::set_last_error("mean_pgrad (constantPGrad, force type 5) is a hydrodynamics forcing: has_T must be 0");
+        return SARAS_BAD_ARG;
     }
     if (P.les && !(g.periodic[0] && g.periodic[1] && g.periodic[2])) {
         saras::set_last_error("the spiral LES path covers fully periodic boxes");

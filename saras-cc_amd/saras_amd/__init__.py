@@ -56,7 +56,8 @@ class NseParams(C.Structure):
                 ("vbc_type", (C.c_int * 6) * 3), ("vbc_val", (C.c_double * 6) * 3),
                 ("tbc_type", C.c_int * 6), ("tbc_val", C.c_double * 6),
                 ("pbc_type", C.c_int * 6), ("pbc_val", C.c_double * 6), ("scheme", C.c_int),
-                ("pe_limit", C.c_double), ("central_bias", C.c_double), ("les", C.c_int)]
+                ("pe_limit", C.c_double), ("central_bias", C.c_double), ("les", C.c_int),
+                ("mean_pgrad", C.c_double)]
 
 
 class NseStats(C.Structure):
@@ -349,17 +350,20 @@ def bench_stream_copy(n_doubles, reps):
 
 def nse_params(dscheme=2, nlscheme=3, has_T=1, nu=1e-2, kappa=1e-2, dt=1e-3, cn_tol=1e-6,
                buoy=(0.0, 0.0, -1.0), vbc=None, tbc=None, pbc=None, scheme=0, pe_limit=2.0, central_bias=0.8,
-               les=0):
+               les=0, mean_pgrad=0.0):
     """Time-step parameters.  vbc: [(types[6], vals[6]) per component];
     tbc / pbc: (types[6], vals[6]).  Defaults: the Rayleigh-Benard wiring of
     hydro.cc:287-485 / scalar.cc:129-201 (no-slip walls, T = 1 at the bottom
-    and 0 at the top, Neumann T on side walls, Neumann P)."""
+    and 0 at the top, Neumann T on side walls, Neumann P).  mean_pgrad: the
+    constant x pressure gradient of force type 5 (constantPGrad, force.h:148-153;
+    hydrodynamics runs, has_T = 0)."""
     d = NseParams()
     d.dscheme, d.nlscheme, d.has_T = dscheme, nlscheme, int(has_T)
     d.nu, d.kappa, d.dt, d.cn_tol = nu, kappa, dt, cn_tol
     d.scheme = scheme
     d.pe_limit, d.central_bias = pe_limit, central_bias
     d.les = int(les)
+    d.mean_pgrad = mean_pgrad
     for a in range(3):
         d.buoy[a] = buoy[a]
     if vbc is None:

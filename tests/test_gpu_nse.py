@@ -414,3 +414,66 @@ def test_les_needs_periodic_box():
     with pytest.raises(S.SarasError) as e:
         pair((16, 16, 16), periodic=(1, 1, 0), les=1)
     assert e.value.status == S.BAD_SHAPE
+
+
+@pytest.mark.parametrize("scheme", [0, 1], ids=["rk3", "euler-cn"])
+def test_channel_constant_pgrad_bitexact(scheme):
+    """Force type 5 (constantPGrad, force.h:148-153) on the channelTest wiring:
+    x and y periodic, no-slip z walls, z tanh-stretched with beta 1, dScheme 1,
+    central non-linear term, nu = 0.1, a perturbed flow; three steps of both
+    integration schemes agree bit for bit."""
+    pad = 1
+    n = (12, 8, 16)
+    so, ss = pair(n, periodic=(1, 1, 0), gtype=(0, 0, 2), beta=(1.0, 1.0, 1.0), pad=pad, has_T=0, dscheme=1,
+                  nlscheme=1, nu=0.1, kappa=0.1, dt=2e-3, cn_tol=1e-9, scheme=scheme, mean_pgrad=1.0)
+    z = O.field_metrics(so.d.g, 2)["x"]
+    f = rbc_init(so.shape, n, pad, z, vel=0.3, seed=11)
+    f["t"] = np.zeros(so.shape)
+    so.set(**f)
+    ss.set(**f)
+    for _ in range(3):
+        assert so.step() == 0
+        r, st = ss.step()
+        assert r == 0
+        assert so.jacobi_iters() == st["jacobi_iters"]
+        assert_fields_equal(so, ss, names=("vx", "vy", "vz", "p", "pp"), periodic=(1, 1, 0))
+
+
+def run_channel(gtype, nz, dt, steps, nu=0.1, G=1.0, L=2.0):
+    n, pad = (4, 4, nz), 1
+    g = S.grid_desc(n, length=(4.0, 2.0, L), periodic=(1, 1, 0), gtype=(0, 0, gtype), beta=(1.0, 1.0, 1.0), pad=pad)
+    mg = S.mg_params(vc_count=5, vc_tol=1e-12, solve_flag=0, pre=2, post=2, bc=S.BC_NEUMANN, precision=S.PREC_FP64)
+    s = S.Nse(g, mg, S.nse_params(dscheme=1, nlscheme=1, has_T=0, nu=nu, kappa=nu, dt=dt, cn_tol=1e-13, scheme=1,
+                                  mean_pgrad=G))
+    zero = np.zeros(s.shape)
+    s.set(vx=zero, vy=zero, vz=zero, p=zero, t=zero, pp=zero)
+    for _ in range(steps):
+        r, _st = s.step()
+        assert r == 0
+    f = s.get()
+    s.close()
+    z = S.level_metrics(g, 0, 2)["x"][1:-1]
+    c = slice(pad, -pad)
+    ux = f["vx"][c, c, c]
+    assert np.abs(ux - ux[:1, :1, :]).max() == 0      # every column alike
+    assert np.abs(f["vz"][c, c, c]).max() == 0
+    return z, ux[0, 0]
+
+
+def test_channel_poiseuille_steady():
+    """The steady channel of the channelTest deck (Re 10, mean pressure
+    gradient 1, Lz = 2, Euler-CN) on the GPU.  Uniform z: the discrete steady
+    state is the parabola raised by G h^2 / (8 nu),
+    G/(2 nu) (z (L - z) + h^2/4), to 1e-9 of its peak.  The deck's tanh z grid
+    (beta 1): checkChannel.py's analytic profile
+    (tau_w delta / 2 nu)(z/delta)(2 - z/delta) within 0.3 % of its peak (the
+    oracle's steady state on that grid is 0.14 % away)."""
+    nu, G, L = 0.1, 1.0, 2.0
+    z, u = run_channel(0, 32, 0.1, 1100)
+    h = L / 32
+    ue = G / (2 * nu) * (z * (L - z) + h * h / 4)
+    assert np.abs(u - ue).max() < 1e-9 * ue.max()
+    z, u = run_channel(2, 32, 0.05, 1500)
+    delta = L / 2
+    ua = (delta * G) * delta / (2 * nu) * (z / delta) * (2 - z / delta)
+    assert np.abs(u - ua).max() < 3e-3 * ua.max()

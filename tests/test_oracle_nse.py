@@ -593,3 +593,32 @@ def test_les_starts_after_five_steps():
         same = all(np.array_equal(s1.get()[k], s0.get()[k]) for k in ("vx", "vy", "vz"))
         # time after step q is (q+1) dt; the model enters the step that starts with time > 5 dt
         assert same == (step <= 5), step
+
+
+@pytest.mark.parametrize("scheme", [0, 1], ids=["rk3", "euler-cn"])
+def test_channel_poiseuille_closed_form(scheme):
+    """Force type 5 (constantPGrad, force.h:148-153) driving the channel of the
+    reference's channelTest deck: x and y periodic, no-slip z walls, Re 10 so
+    nu = 0.1, mean pressure gradient G = 1, Lz = 2.  On a uniform z grid the
+    steady state of the discrete equations, nu D2 u = -G with the no-slip
+    ghost u(-1) = -u(0), is the Poiseuille parabola raised by G h^2 / (8 nu):
+    the second difference is exact on quadratics, and the ghost rule puts the
+    zero at the mean of the two cells beside the wall rather than on the
+    wall.  A dropped forcing, a wrong sign, a forcing on another component or
+    a missing (alpha + beta) weight lands elsewhere."""
+    n, p, L, nu, G = (4, 4, 16), 1, 2.0, 0.1, 1.0
+    g = O.grid_desc(n, length=(2.0, 2.0, L), periodic=(1, 1, 0), pad=p)
+    d = O.nse_desc(g, dscheme=1, nlscheme=1, has_T=0, nu=nu, kappa=nu, dt=0.5, cn_tol=1e-13, scheme=scheme,
+                   vbc=[([FD] * 6, [0.0] * 6)] * 3, pbc=([FN] * 6, [0.0] * 6), mean_pgrad=G,
+                   mg=O.mg_params(vc_count=5, vc_tol=1e-12, solve_flag=0, pre=2, post=2))
+    s = O.Nse(d)
+    zero = np.zeros(s.shape)
+    s.set(vx=zero, vy=zero, vz=zero, p=zero, t=zero, pp=zero)
+    for _ in range(300):
+        assert s.step() == 0
+    f = s.get()
+    h = L / n[2]
+    z = (np.arange(n[2]) + 0.5) * h
+    u = G / (2 * nu) * (z * (L - z) + h * h / 4)
+    assert np.abs(core(f["vx"], p) - u[None, None, :]).max() < 1e-9 * u.max()
+    assert np.abs(core(f["vy"], p)).max() == 0 and np.abs(core(f["vz"], p)).max() == 0

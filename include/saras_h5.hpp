@@ -12,7 +12,11 @@
 // Layout (writer.cc:60-200): a scalar dataset "Time"; in solution files the
 // 1-D datasets "X", "Y", "Z" with the core cell-centre coordinates; one
 // (n_x, n_y, n_z) dataset of the core per field, C order, named "Vx", "Vy",
-// "Vz", "P" and, with a scalar, "T".  One process holds the whole domain, so
+// "Vz", "P" and, with a scalar, "T".  Planar (dim 2) runs follow the PLANAR
+// build (writer.cc:80-160, 361-367; hydro_d2.cc:137-139, scalar_d2.cc:137-140):
+// (n_x, n_z) datasets of "Vx", "Vz", "P" and "T", coordinates "X" and "Z"
+// only; their reader expects rank 2 (reader.cc:377-437) and leaves Vy at
+// zero.  One process holds the whole domain, so
 // the files are written with serial HDF5 (the reference's MPI-IO property
 // lists select the same bytes).  The reader's consistency check (fileCheck,
 // reader.cc:377-437: rank 3, extents equal to the grid's) throws
@@ -32,14 +36,26 @@ namespace h5detail {
 
 inline const char *const kFieldNames[5] = {"Vx", "Vy", "Vz", "P", "T"};
 
-// full (pad) arrays <-> contiguous core blocks
+// full (pad) arrays <-> contiguous core blocks; planar arrays have no y pad
 struct Layout {
     int n[3], pad;
-    size_t full() const { return (size_t)(n[0] + 2 * pad) * (n[1] + 2 * pad) * (n[2] + 2 * pad); }
+    bool planar;
+    int ypad() const { return planar ? 0 : pad; }
+    size_t full() const { return (size_t)(n[0] + 2 * pad) * (n[1] + 2 * ypad()) * (n[2] + 2 * pad); }
     size_t core() const { return (size_t)n[0] * n[1] * n[2]; }
     size_t at(int i, int j, int k) const {
-        return ((size_t)(i + pad) * (n[1] + 2 * pad) + (j + pad)) * (n[2] + 2 * pad) + (k + pad);
+        return ((size_t)(i + pad) * (n[1] + 2 * ypad()) + (j + ypad())) * (n[2] + 2 * pad) + (k + pad);
     }
+    int rank() const { return planar ? 2 : 3; }
+    // dataset extents: (n_x, n_z) planar, (n_x, n_y, n_z) otherwise
+    int dims(hsize_t *d) const {
+        d[0] = (hsize_t)n[0];
+        d[1] = (hsize_t)(planar ? n[2] : n[1]);
+        d[2] = (hsize_t)n[2];
+        return rank();
+    }
+    // the fields a run writes: Vx, Vy, Vz, P, T by index; planar drops Vy
+    bool writes(int q, bool scalar) const { return !(planar && q == 1) && (q < 4 || scalar); }
     void to_core(const std::vector<double> &f, std::vector<double> &c) const {
         c.resize(core());
         size_t q = 0;
@@ -57,10 +73,12 @@ struct Layout {
 };
 
 inline Layout layout_of(const saras_grid_desc &g) {
-    if (g.dim != 3) throw error(SARAS_BAD_SHAPE, "HDF5 files: the 3-D solver only");
+    if (g.dim != 3 && !(g.dim == 2 && g.n[1] == 1))
+        throw error(SARAS_BAD_SHAPE, "HDF5 files: 3-D grids, or planar grids with n_y = 1");
     Layout L;
     for (int a = 0; a < 3; ++a) L.n[a] = g.n[a];
     L.pad = g.pad;
+    L.planar = g.dim == 2;
     return L;
 }
 
@@ -84,8 +102,8 @@ inline void write_time(hid_t file, double time) {
 }
 
 inline void write_field(hid_t file, const char *name, const Layout &L, const std::vector<double> &core) {
-    const hsize_t d[3] = {(hsize_t)L.n[0], (hsize_t)L.n[1], (hsize_t)L.n[2]};
-    const hid_t sp = H5Screate_simple(3, d, nullptr);
+    hsize_t d[3];
+    const hid_t sp = H5Screate_simple(L.dims(d), d, nullptr);
     const hid_t ds = H5Dcreate2(file, name, H5T_NATIVE_DOUBLE, sp, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
     const herr_t st = ds < 0 ? -1 : H5Dwrite(ds, H5T_NATIVE_DOUBLE, H5S_ALL, H5S_ALL, H5P_DEFAULT, core.data());
     if (ds >= 0) H5Dclose(ds);
@@ -110,6 +128,7 @@ class writer {
         try {
             static const char *const axes[3] = {"X", "Y", "Z"};
             for (int a = 0; a < 3; ++a) {
+                if (L_.planar && a == 1) continue;   // writer.cc:361-366
                 std::vector<double> c(L_.n[a] + 2), xi(L_.n[a] + 2), x1(L_.n[a] + 2), x2(L_.n[a] + 2), x3(L_.n[a] + 2);
                 check(saras_level_metrics(&g_, 0, a, c.data(), xi.data(), x1.data(), x2.data(), x3.data()),
                       "saras_level_metrics");
@@ -147,7 +166,8 @@ class writer {
         ts_.getFields(full[0].data(), full[1].data(), full[2].data(), full[3].data(),
                       scalar_ ? full[4].data() : nullptr);
         std::vector<double> core;
-        for (int q = 0; q < (scalar_ ? 5 : 4); ++q) {
+        for (int q = 0; q < 5; ++q) {
+            if (!L_.writes(q, scalar_)) continue;
             L_.to_core(full[q], core);
             h5detail::write_field(f, h5detail::kFieldNames[q], L_, core);
         }
@@ -174,7 +194,7 @@ class reader {
         H5E_END_TRY;
         if (f < 0) throw error(SARAS_BAD_ARG, "could not open restart file " + path);
         double time = 0.0;
-        std::vector<std::vector<double>> full(5);
+        std::vector<std::vector<double>> full(5, std::vector<double>(L_.full(), 0.0));
         try {
             fileCheck(f);
             const hid_t ds = H5Dopen2(f, "Time", H5P_DEFAULT);
@@ -184,7 +204,8 @@ class reader {
             }
             H5Dclose(ds);
             std::vector<double> core(L_.core());
-            for (int q = 0; q < (scalar_ ? 5 : 4); ++q) {
+            for (int q = 0; q < 5; ++q) {
+                if (!L_.writes(q, scalar_)) continue;
                 const hid_t d = H5Dopen2(f, h5detail::kFieldNames[q], H5P_DEFAULT);
                 if (d < 0 || H5Dread(d, H5T_NATIVE_DOUBLE, H5S_ALL, H5S_ALL, H5P_DEFAULT, core.data()) < 0) {
                     if (d >= 0) H5Dclose(d);
@@ -204,19 +225,22 @@ class reader {
     }
 
   private:
-    // reader::fileCheck (reader.cc:377-437): the pressure dataset is 3-D with
-    // the grid's extents
+    // reader::fileCheck (reader.cc:377-437): the pressure dataset has the
+    // run's rank (3, or 2 for planar runs) and the grid's extents
     void fileCheck(hid_t f) const {
-        const hid_t d = H5Dopen2(f, "P", H5P_DEFAULT);
+        hid_t d;
+        H5E_BEGIN_TRY { d = H5Dopen2(f, "P", H5P_DEFAULT); }
+        H5E_END_TRY;
         if (d < 0) throw error(SARAS_BAD_SHAPE, "restart file holds no P dataset");
         const hid_t sp = H5Dget_space(d);
         const int nd = H5Sget_simple_extent_ndims(sp);
-        hsize_t dims[3] = {0, 0, 0};
-        const bool ok = nd == 3 && H5Sget_simple_extent_dims(sp, dims, nullptr) == 3 && (int)dims[0] == L_.n[0] &&
-                        (int)dims[1] == L_.n[1] && (int)dims[2] == L_.n[2];
+        hsize_t want[3], dims[3] = {0, 0, 0};
+        const int r = L_.dims(want);
+        bool ok = nd == r && H5Sget_simple_extent_dims(sp, dims, nullptr) == r;
+        for (int a = 0; ok && a < r; ++a) ok = dims[a] == want[a];
         H5Sclose(sp);
         H5Dclose(d);
-        if (nd != 3) throw error(SARAS_BAD_SHAPE, "Dimensionality of restart file conflicts with solver parameters");
+        if (nd != r) throw error(SARAS_BAD_SHAPE, "Dimensionality of restart file conflicts with solver parameters");
         if (!ok) throw error(SARAS_BAD_SHAPE, "Array limits in restart file conflict with solver parameters");
     }
 

@@ -179,8 +179,14 @@ saras_status saras_bench_stream_copy(size_t n, int reps, double *ms);
 
 /* Field boundary kinds (lib/boundary/dirichlet.cc, neumann.cc).  Walls are
  * numbered 0 x-lo (Left), 1 x-hi (Right), 2 y-lo (Front), 3 y-hi (Back),
- * 4 z-lo (Bottom), 5 z-hi (Top); periodic axes ignore theirs. */
-enum { SARAS_FBC_DIRICHLET = 0, SARAS_FBC_NEUMANN = 1 };
+ * 4 z-lo (Bottom), 5 z-hi (Top); periodic axes ignore theirs.  Two kinds
+ * serve the temperature of the plume problem (probType 9, scalar.cc:174-191):
+ *   SARAS_FBC_HOT_PLATE  bottom wall only (3-D): hotPlate (hotPlate.cc:58-123),
+ *                        T = 1 on the disc of radius plate_radius about
+ *                        (Lx/2, Ly/2, 0), zero gradient elsewhere;
+ *   SARAS_FBC_PLUME_OUT  top wall only: plumeOut (plumeOut.cc:53-90), ghost T
+ *                        = T of the cell below where the ghost Vz > 0, else 0. */
+enum { SARAS_FBC_DIRICHLET = 0, SARAS_FBC_NEUMANN = 1, SARAS_FBC_HOT_PLATE = 2, SARAS_FBC_PLUME_OUT = 3 };
 
 typedef struct {
     int dscheme;             /* "Differentiation Scheme": 1 (2nd order) or 2 (4th order) */
@@ -204,6 +210,7 @@ typedef struct {
     double mean_pgrad;       /* "Mean Pressure Gradient" of force type 5, constantPGrad (force.h:148-153,
                                 hydro.cc:268-271): added to the x-momentum right-hand side of the core.
                                 A hydrodynamics forcing: must be 0 when has_T is set */
+    double plate_radius;     /* "Plate Radius" of SARAS_FBC_HOT_PLATE (ignored otherwise) */
 } saras_nse_params;
 
 typedef struct {

@@ -155,7 +155,27 @@ static void wall_bc(or_nse *s, double *F, int wall, int type, double val)
                 int gi[3] = {a, b, c}, si[3] = {a, b, c};
                 gi[ax] = g; si[ax] = src;
                 double inner = F[FX(s, si[0], si[1], si[2])];
-                F[FX(s, gi[0], gi[1], gi[2])] = type == OR_FBC_DIRICHLET ? 2.0 * val - inner : inner - kh;
+                long gq = FX(s, gi[0], gi[1], gi[2]);
+                if (type == OR_FBC_HOT_PLATE) {
+                    /* hotPlate::createPatch (hotPlate.cc:74-111): disc about
+                     * (Lx/2, Ly/2, 0), the distance taken to the ghost cell */
+                    double cx = s->d.g.len[0] / 2.0, cy = s->d.g.len[1] / 2.0, cz = 0.0;
+                    double r = sqrt(pow((s->x[0][gi[0] + p] - cx), 2) + pow((s->x[1][gi[1] + p] - cy), 2) +
+                                    pow((s->x[2][gi[2] + p] - cz), 2));
+                    double data = r <= s->d.plate_radius ? 1.0 : 0.0;
+                    double mask = r <= s->d.plate_radius ? 0.0 : 1.0;
+                    /* hotPlate::imposeBC (hotPlate.cc:114-123) */
+                    double w = inner;
+                    w = w * mask;
+                    w = w + data * (2.0 * data - inner);
+                    F[gq] = w;
+                } else if (type == OR_FBC_PLUME_OUT) {
+                    /* plumeOut::imposeBC (plumeOut.cc:71-90): the ghost Vz as it
+                     * stands when T's walls are imposed */
+                    F[gq] = s->V[2][gq] > 0 ? F[FX(s, gi[0], gi[1], gi[2] - 1)] : 0.0;
+                } else {
+                    F[gq] = type == OR_FBC_DIRICHLET ? 2.0 * val - inner : inner - kh;
+                }
             }
 }
 

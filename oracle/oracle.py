@@ -39,7 +39,7 @@ class MgParams(C.Structure):
                 ("no_early_exit", C.c_int)]
 
 
-FBC_DIRICHLET, FBC_NEUMANN = 0, 1
+FBC_DIRICHLET, FBC_NEUMANN, FBC_HOT_PLATE, FBC_PLUME_OUT = 0, 1, 2, 3
 
 
 class NseDesc(C.Structure):
@@ -50,7 +50,7 @@ class NseDesc(C.Structure):
                 ("tbc_type", C.c_int * 6), ("tbc_val", C.c_double * 6),
                 ("pbc_type", C.c_int * 6), ("pbc_val", C.c_double * 6), ("scheme", C.c_int),
                 ("pe_limit", C.c_double), ("central_bias", C.c_double), ("les", C.c_int),
-                ("mean_pgrad", C.c_double)]
+                ("mean_pgrad", C.c_double), ("plate_radius", C.c_double)]
 
 
 class MgStats(C.Structure):
@@ -284,7 +284,7 @@ def set_threads(n):
 
 def nse_desc(g, dscheme=2, nlscheme=3, has_T=1, nu=1e-2, kappa=1e-2, dt=1e-3, cn_tol=1e-6,
              buoy=(0.0, 0.0, -1.0), vbc=None, tbc=None, pbc=None, mg=None, scheme=0, pe_limit=2.0,
-             central_bias=0.8, les=0, mean_pgrad=0.0):
+             central_bias=0.8, les=0, mean_pgrad=0.0, plate_radius=0.25):
     """Descriptor of one run.  vbc: [(types[6], vals[6]) per component];
     tbc / pbc: (types[6], vals[6]).  Defaults: the Rayleigh-Benard wiring of
     hydro.cc:287-485 / scalar.cc:129-201 (no-slip, T = 1 bottom, 0 top,
@@ -298,6 +298,7 @@ def nse_desc(g, dscheme=2, nlscheme=3, has_T=1, nu=1e-2, kappa=1e-2, dt=1e-3, cn
     d.pe_limit, d.central_bias = pe_limit, central_bias
     d.les = int(les)
     d.mean_pgrad = mean_pgrad
+    d.plate_radius = plate_radius
     for a in range(3):
         d.buoy[a] = buoy[a]
     if vbc is None:

@@ -121,7 +121,9 @@ void or_mg_residual(or_mg *m, int level, int zero_bc, double *out_pad);
 /* Field boundary kinds (lib/boundary/dirichlet.cc, neumann.cc).  Walls are
  * numbered 0 x-lo (Lft), 1 x-hi (Rgt), 2 y-lo (Frn), 3 y-hi (Bak),
  * 4 z-lo (Bot), 5 z-hi (Top); periodic axes ignore theirs. */
-enum { OR_FBC_DIRICHLET = 0, OR_FBC_NEUMANN = 1 };
+/* wall kinds: dirichlet.cc, neumann.cc; for T also hotPlate (bottom wall,
+ * hotPlate.cc) and plumeOut (top wall, plumeOut.cc) */
+enum { OR_FBC_DIRICHLET = 0, OR_FBC_NEUMANN = 1, OR_FBC_HOT_PLATE = 2, OR_FBC_PLUME_OUT = 3 };
 
 typedef struct {
     or_grid_desc g;        /* field grid; g.pad = 1 (dScheme 1) or 2 (dScheme 2) */
@@ -144,6 +146,7 @@ typedef struct {
     double central_bias;   /* "Central Bias" omega (upwind scheme) */
     int les;               /* "LES Model" 1: stretched spiral vortex sub-grid stress (fully periodic runs) */
     double mean_pgrad;     /* "Mean Pressure Gradient" of constantPGrad, force type 5 (force.h:148-153) */
+    double plate_radius;   /* "Plate Radius" of the hot plate */
 } or_nse_desc;
 
 typedef struct or_nse or_nse;

@@ -29,9 +29,12 @@ constexpr double kRecip12 = 1.0 / 12.0;
 
 // wall rules of one field (6 walls: x-lo, x-hi, y-lo, y-hi, z-lo, z-hi)
 struct FieldBC {
-    int type[6];     // 0 Dirichlet, 1 Neumann
+    int type[6];     // 0 Dirichlet, 1 Neumann, 2 hot plate (z-lo), 3 plume outflow (z-hi)
     double val[6];   // Dirichlet value
     double kh[6];    // Neumann value * h_wall (neumann.cc:59-75)
+    // kinds 2 and 3: one flag per (i, j) column, nx*ny for the bottom wall
+    // then nx*ny for the top wall (see z_wall_mask)
+    const unsigned char *wmask;
 };
 
 struct Geom {
@@ -48,6 +51,19 @@ struct Geom {
     double pe_limit;               // "Peclet Limit"
     double up1[3], up2[4];         // biased-stencil weights from "Central Bias" (vfield.cc:115-130)
 };
+
+// The z-wall kinds whose ghost depends on the column:
+//   hot plate (hotPlate.cc:114-123 with the patch of :74-111): F(wall) =
+//     F(edge) * mask + data * (2 data - F(edge)), data = 1 inside the disc,
+//     i.e. 2 - F(edge) on the disc and F(edge) off it;
+//   plume outflow (plumeOut.cc:71-90): F(edge) where the flag (ghost Vz > 0
+//     at the last imposeBCs) is set, else 0.
+__device__ __forceinline__ double z_wall_mask(const Geom &G, const FieldBC &bc, int type, bool hi, int i, int j,
+                                              double inner) {
+    const bool on = bc.wmask[(hi ? (long)G.n[0] * G.n[1] : 0L) + (long)i * G.n[1] + j] != 0;
+    if (type == 2) return on ? 2.0 - inner : inner;
+    return on ? inner : 0.0;
+}
 
 __device__ __forceinline__ int pick(int ax, int i, int j, int k) { return ax == 0 ? i : (ax == 1 ? j : k); }
 
@@ -78,6 +94,7 @@ struct CoreField {
         const double kh = hi ? bc.kh[2 * ax + 1] : bc.kh[2 * ax];
         c[ax] = hi ? n - 1 : 0;
         const double inner = f[(long)c[0] * G.sx + (long)c[1] * G.sy + c[2]];
+        if (type > 1) return z_wall_mask(G, bc, type, hi, c[0], c[1], inner);
         return type == 0 ? 2.0 * val - inner : inner - kh;
     }
 };
@@ -990,6 +1007,15 @@ __global__ void k_field_op(Geom G, PadField F, const CoreField V0, const CoreFie
         const CoreField V[3] = {V0, V1, V2};
         out[o] = nlin_acc(G, nlscheme, V, F, i, j, k, out[o], kappa);
     }
+}
+
+// plumeOut's flags at T.imposeBCs: top-wall column (i, j) is set where the
+// ghost Vz (Vz's own top-wall rule) is positive (plumeOut.cc:71-90).
+__global__ void __launch_bounds__(256) k_plume_flags(Geom G, CoreField vz, unsigned char *__restrict__ flags) {
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= (long)G.n[0] * G.n[1]) return;
+    const int i = (int)(c / G.n[1]), j = (int)(c % G.n[1]);
+    flags[c] = vz.nb(G, 2, i, j, G.n[2] - 1, 1) > 0 ? 1 : 0;
 }
 
 }  // namespace nse

@@ -63,6 +63,9 @@ struct saras_nse {
     double *H[4] = {nullptr, nullptr, nullptr, nullptr};
     double *R[4] = {nullptr, nullptr, nullptr, nullptr};
     double *scratch = nullptr;
+    // per-column flags of the hot-plate (bottom) and plume-outflow (top) walls of T
+    unsigned char *wmask = nullptr;
+    bool plume = false;
     size_t elems = 0;
     double *part = nullptr;             // per-block residual maxima of a Jacobi launch
     long nblocks = 0;
@@ -99,6 +102,7 @@ struct saras_nse {
         for (auto *q : H) (void)hipFree(q);
         for (auto *q : R) (void)hipFree(q);
         (void)hipFree(scratch);
+        (void)hipFree(wmask);
         (void)hipFree(metrics);
         (void)hipFree(part);
         (void)hipFree(res_dev);
@@ -141,6 +145,8 @@ struct saras_nse {
 };
 
 namespace {
+
+saras_status plume_flags(saras_nse *s);
 
 // spiral::computeSG stresses and diagnostics (spiral.cc:129-206); the
 // divergence of the stresses is taken inside the RHS kernels
@@ -235,6 +241,12 @@ saras_status solve_field(saras_nse *s, int f, double beta, int *iters) {
     hipLaunchKernelGGL(saras::nse::k_jacobi<false>, s->grid3(), dim3(s->threads), 0, s->s, s->G,
                        CoreField{s->F[f], s->bc[f]}, s->R[f], cur, dt, diff, beta, nullptr);
     NSE_TRY(hipGetLastError());
+    // solveT imposes T's walls after each update (predictors.cc:376): from the
+    // first one on, the plume flags see the predicted Vz
+    if (f == 3) {
+        saras_status e = plume_flags(s);
+        if (e) return e;
+    }
     int it = 0;
     for (;;) {
         hipLaunchKernelGGL(saras::nse::k_jacobi<true>, s->grid3(), dim3(s->threads), 0, s->s, s->G,
@@ -294,7 +306,8 @@ saras_status stage_run(saras_nse *s, int stage) {
         hipLaunchKernelGGL(saras::nse::k_correct<CoreField>, s->grid3(), dim3(s->threads), 0, s->s, s->G,
                            CoreField{pp.f, pp.bc}, s->F[4], s->F[0], s->F[1], s->F[2], pc);
     NSE_TRY(hipGetLastError());
-    return SARAS_OK;
+    // V.imposeBCs(); P.imposeBCs(); T.imposeBCs() closing the stage
+    return plume_flags(s);
 }
 
 void fill_stats(saras_nse *s, saras_nse_stats *st, float ms) {
@@ -315,11 +328,22 @@ saras_status ensure_stage(saras_nse *s) {
 FieldBC make_bc(const int *type, const double *val, const double *hwall) {
     FieldBC b{};
     for (int w = 0; w < 6; ++w) {
-        b.type[w] = type[w] == SARAS_FBC_NEUMANN ? 1 : 0;
+        b.type[w] = type[w];   // SARAS_FBC_* values are the kernels' kinds
         b.val[w] = val[w];
         b.kh[w] = val[w] * hwall[w];
     }
     return b;
+}
+
+// plumeOut's flags follow every T.imposeBCs of the reference: they are
+// taken from the ghost Vz at that moment (plumeOut.cc:71-90)
+saras_status plume_flags(saras_nse *s) {
+    if (!s->plume) return SARAS_OK;
+    const long cols = (long)s->G.n[0] * s->G.n[1];
+    hipLaunchKernelGGL(saras::nse::k_plume_flags, dim3((unsigned)((cols + 255) / 256)), dim3(256), 0, s->s, s->G,
+                       s->field(2), s->wmask + cols);
+    NSE_TRY(hipGetLastError());
+    return SARAS_OK;
 }
 
 // caller array (host or device, pad p) -> core buffer dst
@@ -404,6 +428,21 @@ saras_status saras_nse_create(const saras_grid_desc *grid, const saras_mg_params
         saras::set_last_error("Morinishi with a z wall needs ny == nz (edge test on core.ubound(1), vfield.cc:434)");
         return SARAS_BAD_SHAPE;
     }
+    for (int w = 0; w < 6; ++w) {
+        bool ok = P.pbc_type[w] == SARAS_FBC_DIRICHLET || P.pbc_type[w] == SARAS_FBC_NEUMANN;
+        for (int c = 0; c < 3; ++c)
+            ok = ok && (P.vbc_type[c][w] == SARAS_FBC_DIRICHLET || P.vbc_type[c][w] == SARAS_FBC_NEUMANN);
+        const int t = P.tbc_type[w];
+        // hotPlate: the bottom wall of 3-D runs (scalar.cc:178-185; PLANAR builds
+        // ignore the heating plate); plumeOut: the top wall
+        const bool special = (t == SARAS_FBC_HOT_PLATE && w == 4 && !planar) || (t == SARAS_FBC_PLUME_OUT && w == 5);
+        ok = ok && (t == SARAS_FBC_DIRICHLET || t == SARAS_FBC_NEUMANN || (special && P.has_T && !g.periodic[2]));
+        if (!ok) {
+            saras::set_last_error("wall kinds: Dirichlet or Neumann; for T also the hot plate (bottom, 3-D) and the "
+                                  "plume outflow (top) on a z-wall run");
+            return SARAS_BAD_ARG;
+        }
+    }
     if (P.has_T && P.mean_pgrad != 0.0) {
         saras::set_last_error("mean_pgrad (constantPGrad, force type 5) is a hydrodynamics forcing: has_T must be 0");
         return SARAS_BAD_ARG;
@@ -451,6 +490,7 @@ saras_status saras_nse_create(const saras_grid_desc *grid, const saras_mg_params
     G.edge_last[2] = G.n[1] - 1;   // vfield.cc:434, sfield.cc:295
     G.dscheme = P.dscheme;
     double hwall[6];
+    std::vector<double> coords[3];   // field coordinates over [-pad, n + pad)
     const size_t stride = (size_t)std::max(g.n[0], std::max(g.n[1], g.n[2])) + 2;
     std::vector<double> host(12 * stride, 0.0);
     for (int a = 0; a < 3; ++a) {
@@ -462,7 +502,8 @@ saras_status saras_nse_create(const saras_grid_desc *grid, const saras_mg_params
             hwall[2 * a] = hwall[2 * a + 1] = 0.0;
             continue;
         }
-        std::vector<double> coord, xi_x, xixx, xix2;
+        std::vector<double> &coord = coords[a];
+        std::vector<double> xi_x, xixx, xix2;
         saras::field_axis(g, a, coord, xi_x, xixx, xix2);
         const int p = g.pad, n = g.n[a];
         for (int i = 0; i < n; ++i) {
@@ -510,6 +551,28 @@ saras_status saras_nse_create(const saras_grid_desc *grid, const saras_mg_params
     }
     for (int c = 0; c < 3; ++c) s->bc[c] = make_bc(P.vbc_type[c], P.vbc_val[c], hwall);
     s->bc[3] = make_bc(P.tbc_type, P.tbc_val, hwall);
+    if (P.has_T && (P.tbc_type[4] == SARAS_FBC_HOT_PLATE || P.tbc_type[5] == SARAS_FBC_PLUME_OUT)) {
+        const long cols = (long)g.n[0] * G.n[1];
+        std::vector<unsigned char> flags(2 * cols, 0);
+        if (P.tbc_type[4] == SARAS_FBC_HOT_PLATE) {
+            // hotPlate::createPatch (hotPlate.cc:74-111): the disc about
+            // (Lx/2, Ly/2, 0), distances taken to the ghost cell (z(-1))
+            const int p = g.pad;
+            const double cx = g.len[0] / 2.0, cy = g.len[1] / 2.0, cz = 0.0;
+            const double zg = coords[2][p - 1];
+            for (int i = 0; i < g.n[0]; ++i)
+                for (int j = 0; j < g.n[1]; ++j) {
+                    const double r = std::sqrt(std::pow((coords[0][i + p] - cx), 2) +
+                                               std::pow((coords[1][j + p] - cy), 2) + std::pow((zg - cz), 2));
+                    flags[(long)i * g.n[1] + j] = r <= P.plate_radius ? 1 : 0;
+                }
+        }
+        if (hipMalloc((void **)&s->wmask, flags.size()) != hipSuccess) return fail(SARAS_OOM);
+        if (hipMemcpy(s->wmask, flags.data(), flags.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return fail(SARAS_HIP_ERROR);
+        s->bc[3].wmask = s->wmask;
+        s->plume = P.tbc_type[5] == SARAS_FBC_PLUME_OUT;
+    }
     s->pbc = make_bc(P.pbc_type, P.pbc_val, hwall);
     {
         const int neu[6] = {1, 1, 1, 1, 1, 1};
@@ -602,6 +665,9 @@ saras_status saras_nse_set_fields(saras_nse *s, const double *vx, const double *
         saras_status e = load_field(s, pp, ptr_kind, x0, hz0);
         if (e) return e;
     }
+    // the drivers' initial imposeBCs of V, P and then T
+    saras_status e = plume_flags(s);
+    if (e) return e;
     NSE_TRY(hipStreamSynchronize(s->s));
     return SARAS_OK;
 }

@@ -46,7 +46,7 @@ class MgStats(C.Structure):
                 ("coarse_capped", C.c_int), ("device_ms", C.c_double)]
 
 
-FBC_DIRICHLET, FBC_NEUMANN = 0, 1
+FBC_DIRICHLET, FBC_NEUMANN, FBC_HOT_PLATE, FBC_PLUME_OUT = 0, 1, 2, 3
 
 
 class NseParams(C.Structure):
@@ -57,7 +57,7 @@ class NseParams(C.Structure):
                 ("tbc_type", C.c_int * 6), ("tbc_val", C.c_double * 6),
                 ("pbc_type", C.c_int * 6), ("pbc_val", C.c_double * 6), ("scheme", C.c_int),
                 ("pe_limit", C.c_double), ("central_bias", C.c_double), ("les", C.c_int),
-                ("mean_pgrad", C.c_double)]
+                ("mean_pgrad", C.c_double), ("plate_radius", C.c_double)]
 
 
 class NseStats(C.Structure):
@@ -350,7 +350,7 @@ def bench_stream_copy(n_doubles, reps):
 
 def nse_params(dscheme=2, nlscheme=3, has_T=1, nu=1e-2, kappa=1e-2, dt=1e-3, cn_tol=1e-6,
                buoy=(0.0, 0.0, -1.0), vbc=None, tbc=None, pbc=None, scheme=0, pe_limit=2.0, central_bias=0.8,
-               les=0, mean_pgrad=0.0):
+               les=0, mean_pgrad=0.0, plate_radius=0.25):
     """Time-step parameters.  vbc: [(types[6], vals[6]) per component];
     tbc / pbc: (types[6], vals[6]).  Defaults: the Rayleigh-Benard wiring of
     hydro.cc:287-485 / scalar.cc:129-201 (no-slip walls, T = 1 at the bottom
@@ -364,6 +364,7 @@ def nse_params(dscheme=2, nlscheme=3, has_T=1, nu=1e-2, kappa=1e-2, dt=1e-3, cn_
     d.pe_limit, d.central_bias = pe_limit, central_bias
     d.les = int(les)
     d.mean_pgrad = mean_pgrad
+    d.plate_radius = plate_radius
     for a in range(3):
         d.buoy[a] = buoy[a]
     if vbc is None:

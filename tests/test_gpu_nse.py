@@ -477,3 +477,46 @@ def test_channel_poiseuille_steady():
     delta = L / 2
     ua = (delta * G) * delta / (2 * nu) * (z / delta) * (2 - z / delta)
     assert np.abs(u - ua).max() < 3e-3 * ua.max()
+
+
+@pytest.mark.parametrize("scheme", [0, 1], ids=["rk3", "euler-cn"])
+def test_plume_hot_plate_bitexact(scheme):
+    """The plume problem's walls (probType 9): hotPlate at the bottom
+    (hotPlate.cc), plumeOut at the top (plumeOut.cc) fed by the outflow Vz
+    (zero gradient at the top, on the x walls), no-slip elsewhere; a stretched
+    12^3 box with a flow whose top Vz has both signs.  Three steps agree bit
+    for bit, ghost layers included."""
+    pad = 2
+    vz_t = [O.FBC_NEUMANN, O.FBC_NEUMANN, O.FBC_DIRICHLET, O.FBC_DIRICHLET, O.FBC_DIRICHLET, O.FBC_NEUMANN]
+    vbc = [([O.FBC_DIRICHLET] * 6, [0.0] * 6)] * 2 + [(vz_t, [0.0] * 6)]
+    tbc = ([O.FBC_NEUMANN] * 4 + [O.FBC_HOT_PLATE, O.FBC_PLUME_OUT], [0.0] * 6)
+    n = (12, 12, 12)
+    so, ss = pair(n, periodic=(0, 0, 0), gtype=(0, 0, 2), beta=(1.0, 1.0, 1.2), pad=pad, dscheme=2, nlscheme=1,
+                  nu=5e-3, kappa=5e-3, dt=2e-3, cn_tol=1e-9, scheme=scheme, vbc=vbc, tbc=tbc, plate_radius=0.3)
+    z = O.field_metrics(so.d.g, 2)["x"]
+    f = rbc_init(so.shape, n, pad, z, vel=0.3, seed=17)
+    so.set(**f)
+    ss.set(**f)
+    top = ss.get()["vz"][pad:-pad, pad:-pad, -pad]
+    assert 0 < (top > 0).sum() < n[0] * n[1]
+    assert_fields_equal(so, ss, periodic=(0, 0, 0))
+    for _ in range(3):
+        assert so.step() == 0
+        r, st = ss.step()
+        assert r == 0
+        assert so.jacobi_iters() == st["jacobi_iters"]
+        assert_fields_equal(so, ss, periodic=(0, 0, 0))
+
+
+def test_special_walls_rejected_where_the_reference_has_none():
+    g = S.grid_desc((8, 8, 8), periodic=(0, 0, 0), pad=1)
+    mg = S.mg_params(bc=S.BC_NEUMANN, precision=S.PREC_FP64)
+    bad = [dict(tbc=([S.FBC_NEUMANN] * 4 + [S.FBC_PLUME_OUT, S.FBC_NEUMANN], [0.0] * 6)),   # plume at the bottom
+           dict(tbc=([S.FBC_NEUMANN] * 5 + [S.FBC_HOT_PLATE], [0.0] * 6)),                  # plate at the top
+           dict(vbc=[([S.FBC_HOT_PLATE] * 6, [0.0] * 6)] * 3),                             # on velocity
+           dict(has_T=0, tbc=([S.FBC_NEUMANN] * 4 + [S.FBC_HOT_PLATE, S.FBC_NEUMANN], [0.0] * 6))]
+    for kw in bad:
+        args = dict(dscheme=1, nlscheme=1, has_T=1, nu=1e-2, kappa=1e-2, dt=1e-3)
+        args.update(kw)
+        with pytest.raises(S.SarasError):
+            S.Nse(g, mg, S.nse_params(**args))

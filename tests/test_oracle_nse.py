@@ -622,3 +622,74 @@ def test_channel_poiseuille_closed_form(scheme):
     u = G / (2 * nu) * (z * (L - z) + h * h / 4)
     assert np.abs(core(f["vx"], p) - u[None, None, :]).max() < 1e-9 * u.max()
     assert np.abs(core(f["vy"], p)).max() == 0 and np.abs(core(f["vz"], p)).max() == 0
+
+
+def plume_desc(n=(8, 8, 8), t_bot=None, radius=0.3, scheme=0):
+    """The plume problem's walls (probType 9, hydro.cc:287-440, scalar.cc:129-191):
+    no-slip Vx, Vy; Vz zero-gradient on the x walls and the top (outflow),
+    zero on the y walls and the bottom; T zero-gradient on the side walls,
+    the hot plate (or t_bot) at the bottom, plumeOut at the top."""
+    g = O.grid_desc(n, periodic=(0, 0, 0), pad=1)
+    vz_t = [FN, FN, FD, FD, FD, FN]
+    vbc = [([FD] * 6, [0.0] * 6), ([FD] * 6, [0.0] * 6), (vz_t, [0.0] * 6)]
+    tb = t_bot or (O.FBC_HOT_PLATE, 0.0)
+    tbc = ([FN] * 4 + [tb[0], O.FBC_PLUME_OUT], [0.0] * 4 + [tb[1], 0.0])
+    return g, O.nse_desc(g, dscheme=1, nlscheme=1, has_T=1, nu=1e-2, kappa=1e-2, dt=2e-3, cn_tol=1e-10,
+                         scheme=scheme, vbc=vbc, tbc=tbc, plate_radius=radius,
+                         mg=O.mg_params(vc_count=20, vc_tol=1e-10, solve_flag=0, pre=3, post=3))
+
+
+def plume_fields(shape, p, seed=3):
+    rng = np.random.default_rng(seed)
+    c = (slice(p, -p),) * 3
+    f = {k: np.zeros(shape) for k in ("vx", "vy", "vz", "p", "t", "pp")}
+    for k in ("vx", "vy", "vz"):
+        f[k][c] = 0.3 * rng.standard_normal(core(f[k], p).shape)
+    f["t"][c] = rng.uniform(0.0, 1.0, core(f["t"], p).shape)
+    return f
+
+
+@pytest.mark.parametrize("scheme", [0, 1], ids=["rk3", "euler-cn"])
+def test_hot_plate_is_dirichlet_or_neumann_at_extreme_radii(scheme):
+    """hotPlate (hotPlate.cc:74-123) puts T = 1 (2 - F(edge)) on the disc and
+    a zero gradient off it: a disc that covers the wall gives the run of
+    dirichlet(1), an empty disc the run of neumann(0), bit for bit."""
+    for radius, plain in ((1e9, (FD, 1.0)), (-1.0, (FN, 0.0))):
+        _, d1 = plume_desc(radius=radius, scheme=scheme)
+        _, d2 = plume_desc(t_bot=plain, scheme=scheme)
+        a, b = O.Nse(d1), O.Nse(d2)
+        f = plume_fields(a.shape, 1)
+        a.set(**f)
+        b.set(**f)
+        for _ in range(2):
+            assert a.step() == 0 and b.step() == 0
+        fa, fb = a.get(), b.get()
+        for k in ("vx", "vy", "vz", "p", "t"):
+            assert np.array_equal(core(fa[k], 1), core(fb[k], 1)), (radius, k)
+
+
+def test_hot_plate_disc_and_plume_outflow_ghosts():
+    """Ghost layers after the initial imposeBCs and after a step: bottom
+    ghosts 2 - T(0) on the disc of radius 0.3 about the wall centre (the
+    distance measured to the ghost cell, z = -h/2) and T(0) off it; top
+    ghosts T(n-1) where the ghost Vz (= Vz(n-1), zero gradient) is positive,
+    0 where it is not."""
+    n, p, R = 16, 1, 0.3
+    g, d = plume_desc(n=(n, n, n), radius=R)
+    s = O.Nse(d)
+    f = plume_fields(s.shape, p, seed=9)
+    s.set(**f)
+    x = (np.arange(n) + 0.5) / n
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    disc = np.sqrt((X - 0.5) ** 2 + (Y - 0.5) ** 2 + (0.5 / n) ** 2) <= R
+    assert 0 < disc.sum() < n * n
+    for step in range(2):
+        o = s.get()
+        T, Vz = o["t"][p:-p, p:-p], o["vz"][p:-p, p:-p]
+        bot = np.where(disc, 2.0 - T[:, :, p], T[:, :, p])
+        assert np.array_equal(T[:, :, p - 1], bot), step
+        top = np.where(Vz[:, :, -p] > 0, T[:, :, -p - 1], 0.0)
+        assert np.array_equal(Vz[:, :, -p], Vz[:, :, -p - 1])
+        assert np.array_equal(T[:, :, -p], top), step
+        assert 0 < (Vz[:, :, -p] > 0).sum() < n * n
+        assert s.step() == 0

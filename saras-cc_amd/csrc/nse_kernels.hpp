@@ -32,9 +32,6 @@ struct FieldBC {
     int type[6];     // 0 Dirichlet, 1 Neumann, 2 hot plate (z-lo), 3 plume outflow (z-hi)
     double val[6];   // Dirichlet value
     double kh[6];    // Neumann value * h_wall (neumann.cc:59-75)
-    // kinds 2 and 3: one flag per (i, j) column, nx*ny for the bottom wall
-    // then nx*ny for the top wall (see z_wall_mask)
-    const unsigned char *wmask;
 };
 
 struct Geom {
@@ -50,6 +47,9 @@ struct Geom {
     const double *xc[3];           // cell-centre coordinates over [-1, n] (xc[a][-1] valid)
     double pe_limit;               // "Peclet Limit"
     double up1[3], up2[4];         // biased-stencil weights from "Central Bias" (vfield.cc:115-130)
+    // the wall kinds 2 and 3 (temperature only): one flag per (i, j) column,
+    // nx*ny for the bottom wall then nx*ny for the top wall
+    const unsigned char *wmask;
 };
 
 // The z-wall kinds whose ghost depends on the column:
@@ -58,9 +58,8 @@ struct Geom {
 //     i.e. 2 - F(edge) on the disc and F(edge) off it;
 //   plume outflow (plumeOut.cc:71-90): F(edge) where the flag (ghost Vz > 0
 //     at the last imposeBCs) is set, else 0.
-__device__ __forceinline__ double z_wall_mask(const Geom &G, const FieldBC &bc, int type, bool hi, int i, int j,
-                                              double inner) {
-    const bool on = bc.wmask[(hi ? (long)G.n[0] * G.n[1] : 0L) + (long)i * G.n[1] + j] != 0;
+__device__ __forceinline__ double z_wall_mask(const Geom &G, int type, bool hi, int i, int j, double inner) {
+    const bool on = G.wmask[(hi ? (long)G.n[0] * G.n[1] : 0L) + (long)i * G.n[1] + j] != 0;
     if (type == 2) return on ? 2.0 - inner : inner;
     return on ? inner : 0.0;
 }
@@ -94,7 +93,7 @@ struct CoreField {
         const double kh = hi ? bc.kh[2 * ax + 1] : bc.kh[2 * ax];
         c[ax] = hi ? n - 1 : 0;
         const double inner = f[(long)c[0] * G.sx + (long)c[1] * G.sy + c[2]];
-        if (type > 1) return z_wall_mask(G, bc, type, hi, c[0], c[1], inner);
+        if (type > 1) return z_wall_mask(G, type, hi, c[0], c[1], inner);
         return type == 0 ? 2.0 * val - inner : inner - kh;
     }
 };

@@ -570,7 +570,7 @@ saras_status saras_nse_create(const saras_grid_desc *grid, const saras_mg_params
         if (hipMalloc((void **)&s->wmask, flags.size()) != hipSuccess) return fail(SARAS_OOM);
         if (hipMemcpy(s->wmask, flags.data(), flags.size(), hipMemcpyHostToDevice) != hipSuccess)
             return fail(SARAS_HIP_ERROR);
-        s->bc[3].wmask = s->wmask;
+        G.wmask = s->wmask;
         s->plume = P.tbc_type[5] == SARAS_FBC_PLUME_OUT;
     }
     s->pbc = make_bc(P.pbc_type, P.pbc_val, hwall);

@@ -985,21 +985,26 @@ void or_nse_diag(or_nse *s, double *ke, double *divmax, double *nuvol, double *n
     if (nuvol) *nuvol = s->d.has_T ? 1.0 + (uzt / vol) / s->d.kappa : 0.0;
     if (nuwall && s->d.has_T) {
         /* z-gravity, one rank holding both walls: wallDn ends as the top-wall
-         * distance and serves both sums (tseries.cc:118-128, 374-394) */
-        double dt_sum = 0.0, wall_dn = s->d.g.len[2] - s->x[2][s->n[2] - 1 + p];
+         * distance and serves both sums (tseries.cc:118-128, 374-394); an open
+         * top (plumeOut, probType 9) leaves the bottom wall alone, at its own
+         * distance, over one wall's area */
+        int open_top = s->d.tbc_type[5] == OR_FBC_PLUME_OUT;
+        double dt_sum = 0.0;
+        double wall_dn = open_top ? s->x[2][0 + p] : s->d.g.len[2] - s->x[2][s->n[2] - 1 + p];
         for (int i = 0; i < s->n[0]; i++)
             for (int j = 0; j < s->n[1]; j++) {
                 double dA = (dxi[0] / s->xi_x[0][i + p]) * (dxi[1] / s->xi_x[1][j + p]);
                 double dTdn = (1.0 - fabs(s->T[FX(s, i, j, 0)])) / wall_dn;
                 dt_sum += dTdn * dA;
             }
-        for (int i = 0; i < s->n[0]; i++)
-            for (int j = 0; j < s->n[1]; j++) {
-                double dA = (dxi[0] / s->xi_x[0][i + p]) * (dxi[1] / s->xi_x[1][j + p]);
-                double dTdn = fabs(s->T[FX(s, i, j, s->n[2] - 1)]) / wall_dn;
-                dt_sum += dTdn * dA;
-            }
-        *nuwall = dt_sum / (s->d.g.len[0] * s->d.g.len[1] * 2);
+        if (!open_top)
+            for (int i = 0; i < s->n[0]; i++)
+                for (int j = 0; j < s->n[1]; j++) {
+                    double dA = (dxi[0] / s->xi_x[0][i + p]) * (dxi[1] / s->xi_x[1][j + p]);
+                    double dTdn = fabs(s->T[FX(s, i, j, s->n[2] - 1)]) / wall_dn;
+                    dt_sum += dTdn * dA;
+                }
+        *nuwall = dt_sum / (s->d.g.len[0] * s->d.g.len[1] * (open_top ? 1 : 2));
     }
 }
 

@@ -522,7 +522,10 @@ saras_status saras_nse_create(const saras_grid_desc *grid, const saras_mg_params
         const double h2 = std::pow(dxi, 2.0);
         G.tsi2h[a] = 0.5 / dxi;
         G.tsih2[a] = 1.0 / h2;
-        if (a == 2) s->wall_dn = g.len[2] - coord[n - 1 + p];   // tseries.cc:124-128
+        // tseries.cc:118-128: the top wall's distance when both walls count;
+        // with an open top (plumeOut, probType 9) the bottom wall's
+        if (a == 2)
+            s->wall_dn = P.has_T && P.tbc_type[5] == SARAS_FBC_PLUME_OUT ? coord[p] : g.len[2] - coord[n - 1 + p];
     }
     auto fail = [&](saras_status st) {
         delete s;
@@ -864,7 +867,14 @@ saras_status saras_nse_tsdata(saras_nse *s, double out[5]) {
     out[1] = s->p.has_T ? acc[5] / vol : 0.0;
     out[2] = acc[2];
     out[3] = s->p.has_T ? 1.0 + (acc[1] / vol) / s->p.kappa : 0.0;
-    out[4] = s->p.has_T ? (acc[3] + acc[4]) / (s->wall_area() * 2) : 0.0;
+    // wall Nusselt number over both walls, or the bottom one alone under an
+    // open top (tseries.cc:114-128: wallCount 1 and no top sum for probType 9)
+    if (!s->p.has_T)
+        out[4] = 0.0;
+    else if (s->p.tbc_type[5] == SARAS_FBC_PLUME_OUT)
+        out[4] = acc[3] / (s->wall_area() * 1);
+    else
+        out[4] = (acc[3] + acc[4]) / (s->wall_area() * 2);
     return SARAS_OK;
 }
 

@@ -506,6 +506,9 @@ def test_plume_hot_plate_bitexact(scheme):
         assert r == 0
         assert so.jacobi_iters() == st["jacobi_iters"]
         assert_fields_equal(so, ss, periodic=(0, 0, 0))
+    # TimeSeries.dat's wall Nusselt number of an open top: the bottom wall alone
+    td, dg = ss.tsdata(), so.diag()
+    assert abs(td["nuwall"] - dg["nuwall"]) <= 1e-12 * abs(dg["nuwall"])
 
 
 def test_special_walls_rejected_where_the_reference_has_none():

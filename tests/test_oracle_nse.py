@@ -693,3 +693,24 @@ def test_hot_plate_disc_and_plume_outflow_ghosts():
         assert np.array_equal(T[:, :, -p], top), step
         assert 0 < (Vz[:, :, -p] > 0).sum() < n * n
         assert s.step() == 0
+
+
+def test_open_top_wall_nusselt():
+    """tseries for an open top (probType 9, tseries.cc:114-128): the bottom
+    wall alone, at its own distance z(0), over one wall's area.  With T = 0
+    the flux density is 1 / wallDn everywhere on the bottom and 0 on the top:
+    NuWall = 1 / z(0) with the open top, 1 / (2 (Lz - z(n-1))) with both
+    walls; a single-sided tanh z grid keeps the two distances apart."""
+    n, p = 8, 1
+    for top, want in ((O.FBC_PLUME_OUT, "open"), (O.FBC_DIRICHLET, "both")):
+        g = O.grid_desc((n, n, n), periodic=(1, 1, 0), gtype=(0, 0, 1), beta=(1.0, 1.0, 1.3), pad=p)
+        tbc = ([FN] * 4 + [FD, top], [0.0] * 4 + [1.0, 0.0])
+        d = O.nse_desc(g, dscheme=1, nlscheme=1, has_T=1, tbc=tbc,
+                       mg=O.mg_params(vc_count=5, vc_tol=1e-10, solve_flag=0, pre=2, post=2))
+        s = O.Nse(d)
+        zero = np.zeros(s.shape)
+        s.set(vx=zero, vy=zero, vz=zero, p=zero, t=zero, pp=zero)
+        z = O.field_metrics(g, 2)["x"][p:-p]
+        assert abs(z[0] - (1.0 - z[-1])) > 1e-3
+        expect = 1.0 / z[0] if want == "open" else 1.0 / (2 * (1.0 - z[-1]))
+        assert abs(s.diag()["nuwall"] - expect) < 1e-12 * expect, (want, s.diag()["nuwall"], expect)

@@ -225,8 +225,8 @@ typedef struct saras_nse saras_nse;
  * timestep.cc:57-122) with the fields of hydro/scalar (vfield, sfield).
  * mg: the "Multigrid" keys of the pressure solve (bc is forced to Neumann,
  * poisson.cc:87).  grid->dim 2 is the PLANAR (x-z) build: n[1] = 1, caller
- * arrays (nx+2p, 1, nz+2p), dscheme 1 with the central or upwind non-linear
- * term (lsRK3_d2.cc, eulerCN_d2.cc).  SARAS_BAD_SHAPE for cases the path does
+ * arrays (nx+2p, 1, nz+2p), dscheme 1 with any non-linear term (lsRK3_d2.cc,
+ * eulerCN_d2.cc).  SARAS_BAD_SHAPE for cases the path does
  * not cover: pad < dscheme, and Morinishi runs with a wall in z and ny != nz
  * (the reference's z edge test reads core.ubound(1), vfield.cc:434). */
 saras_status saras_nse_create(const saras_grid_desc *grid, const saras_mg_params *mg,

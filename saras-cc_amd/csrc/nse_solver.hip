@@ -413,8 +413,11 @@ saras_status saras_nse_create(const saras_grid_desc *grid, const saras_mg_params
         saras::set_last_error("grid dim must be 2 (planar x-z) or 3");
         return SARAS_BAD_SHAPE;
     }
-    if (planar && (P.dscheme != 1 || P.nlscheme == 3 || P.les || g.n[1] != 1)) {
-        saras::set_last_error("the planar path covers n_y = 1, dScheme 1 and the central or upwind non-linear term");
+    // dScheme 1 only: the PLANAR grid keeps a y pad of 1 whatever the scheme
+    // (grid.cc:92-95); with it every PLANAR loop (vfield.cc:289-349,
+    // sfield.cc:209-260) is the 3-D one without its y terms
+    if (planar && (P.dscheme != 1 || P.les || g.n[1] != 1)) {
+        saras::set_last_error("the planar path covers n_y = 1 and dScheme 1");
         return SARAS_BAD_ARG;
     }
     if ((P.dscheme != 1 && P.dscheme != 2) || P.nlscheme < 1 || P.nlscheme > 3 || g.pad < P.dscheme ||

@@ -523,3 +523,22 @@ def test_special_walls_rejected_where_the_reference_has_none():
         args.update(kw)
         with pytest.raises(S.SarasError):
             S.Nse(g, mg, S.nse_params(**args))
+
+
+def test_morinishi_second_order_steps_bitexact():
+    """Morinishi's term with dScheme 1 (firstOrder: second-order differences
+    everywhere, vfield.cc:289-462, sfield.cc:200-311) in a Boussinesq box with
+    z walls: three RK3 steps agree bit for bit."""
+    pad = 1
+    n = (12, 8, 16)
+    so, ss = pair(n, pad=pad, dscheme=1, nlscheme=3, nu=4e-3, kappa=3e-3, dt=2e-3, cn_tol=1e-9)
+    z = O.field_metrics(so.d.g, 2)["x"]
+    f = rbc_init(so.shape, n, pad, z, vel=0.1, seed=23)
+    so.set(**f)
+    ss.set(**f)
+    for _ in range(3):
+        assert so.step() == 0
+        r, st = ss.step()
+        assert r == 0
+        assert so.jacobi_iters() == st["jacobi_iters"]
+        assert_fields_equal(so, ss)

@@ -184,8 +184,11 @@ saras_status saras_bench_stream_copy(size_t n, int reps, double *ms);
  *   SARAS_FBC_HOT_PLATE  bottom wall only (3-D): hotPlate (hotPlate.cc:58-123),
  *                        T = 1 on the disc of radius plate_radius about
  *                        (Lx/2, Ly/2, 0), zero gradient elsewhere;
- *   SARAS_FBC_PLUME_OUT  top wall only: plumeOut (plumeOut.cc:53-90), ghost T
- *                        = T of the cell below where the ghost Vz > 0, else 0. */
+ *   SARAS_FBC_PLUME_OUT  top wall only (3-D): plumeOut (plumeOut.cc:53-90),
+ *                        ghost T = T of the cell below where the ghost Vz > 0,
+ *                        else 0.  Planar plume runs are refused: their 2-D
+ *                        multigrid turns the top wall Dirichlet
+ *                        (poisson_d2.cc:641-647), which is not built. */
 enum { SARAS_FBC_DIRICHLET = 0, SARAS_FBC_NEUMANN = 1, SARAS_FBC_HOT_PLATE = 2, SARAS_FBC_PLUME_OUT = 3 };
 
 typedef struct {

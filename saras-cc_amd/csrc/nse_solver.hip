@@ -437,12 +437,14 @@ saras_status saras_nse_create(const saras_grid_desc *grid, const saras_mg_params
             ok = ok && (P.vbc_type[c][w] == SARAS_FBC_DIRICHLET || P.vbc_type[c][w] == SARAS_FBC_NEUMANN);
         const int t = P.tbc_type[w];
         // hotPlate: the bottom wall of 3-D runs (scalar.cc:178-185; PLANAR builds
-        // ignore the heating plate); plumeOut: the top wall
-        const bool special = (t == SARAS_FBC_HOT_PLATE && w == 4 && !planar) || (t == SARAS_FBC_PLUME_OUT && w == 5);
+        // ignore the heating plate); plumeOut: the top wall of 3-D runs (a PLANAR
+        // plume also turns the 2-D multigrid's top wall Dirichlet,
+        // poisson_d2.cc:641-647, which is not built)
+        const bool special = !planar && ((t == SARAS_FBC_HOT_PLATE && w == 4) || (t == SARAS_FBC_PLUME_OUT && w == 5));
         ok = ok && (t == SARAS_FBC_DIRICHLET || t == SARAS_FBC_NEUMANN || (special && P.has_T && !g.periodic[2]));
         if (!ok) {
-            saras::set_last_error("wall kinds: Dirichlet or Neumann; for T also the hot plate (bottom, 3-D) and the "
-                                  "plume outflow (top) on a z-wall run");
+            saras::set_last_error("wall kinds: Dirichlet or Neumann; for T of a 3-D run with z walls also the hot "
+                                  "plate (bottom) and the plume outflow (top)");
             return SARAS_BAD_ARG;
         }
     }

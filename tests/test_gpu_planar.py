@@ -75,3 +75,7 @@ def test_planar_rejects_unsupported_schemes():
     mg = S.mg_params(precision=S.PREC_FP64, bc=S.BC_NEUMANN)
     with pytest.raises(S.SarasError):
         S.Nse(g, mg, S.nse_params(dscheme=2, nlscheme=1, has_T=0, nu=0.01, kappa=0.01, dt=1e-3))
+    gw = S.grid_desc((16, 1, 16), dim=2, periodic=(1, 1, 0), pad=1)
+    plume = ([S.FBC_NEUMANN] * 4 + [S.FBC_DIRICHLET, S.FBC_PLUME_OUT], [0.0] * 4 + [1.0, 0.0])
+    with pytest.raises(S.SarasError):
+        S.Nse(gw, mg, S.nse_params(dscheme=1, nlscheme=1, has_T=1, nu=0.01, kappa=0.01, dt=1e-3, tbc=plume))

@@ -89,6 +89,9 @@ typedef struct {
                            most 3 to measure the convergence factor, later ones the cycles that factor
                            needs to reach the tolerance (or the FP32 accuracy floor) */
     int max_refine;     /* MIXED: cap on refinement steps */
+    int top_dirichlet;  /* 1: the top (z-hi) wall of a 2-D Neumann problem is homogeneous Dirichlet, as
+                           multigrid_d2 makes it for probType 9 (poisson_d2.cc:641-647); 2-D grids with
+                           z walls and SARAS_BC_NEUMANN only */
 } saras_mg_params;
 
 typedef struct {
@@ -184,11 +187,11 @@ saras_status saras_bench_stream_copy(size_t n, int reps, double *ms);
  *   SARAS_FBC_HOT_PLATE  bottom wall only (3-D): hotPlate (hotPlate.cc:58-123),
  *                        T = 1 on the disc of radius plate_radius about
  *                        (Lx/2, Ly/2, 0), zero gradient elsewhere;
- *   SARAS_FBC_PLUME_OUT  top wall only (3-D): plumeOut (plumeOut.cc:53-90),
- *                        ghost T = T of the cell below where the ghost Vz > 0,
- *                        else 0.  Planar plume runs are refused: their 2-D
- *                        multigrid turns the top wall Dirichlet
- *                        (poisson_d2.cc:641-647), which is not built. */
+ *   SARAS_FBC_PLUME_OUT  top wall only: plumeOut (plumeOut.cc:53-90), ghost T
+ *                        = T of the cell below where the ghost Vz > 0, else 0.
+ *                        In planar runs the pressure correction's top wall then
+ *                        turns Dirichlet, as in multigrid_d2
+ *                        (poisson_d2.cc:641-647). */
 enum { SARAS_FBC_DIRICHLET = 0, SARAS_FBC_NEUMANN = 1, SARAS_FBC_HOT_PLATE = 2, SARAS_FBC_PLUME_OUT = 3 };
 
 typedef struct {

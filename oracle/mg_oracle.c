@@ -205,7 +205,8 @@ static void update_face(or_mg *m, level_t *L, double *a)
     }
 }
 
-/* imposeBC, poisson_d3.cc:876-952 / poisson_d2.cc:545-656 (not probType 9). */
+/* imposeBC, poisson_d3.cc:876-952 / poisson_d2.cc:545-656 (probType 9's
+ * Dirichlet top wall of the 2-D Neumann problem by top_dirichlet). */
 static void impose_bc(or_mg *m, level_t *L)
 {
     double *a = L->lhs;
@@ -255,7 +256,10 @@ static void impose_bc(or_mg *m, level_t *L)
                 if (inhom) wv = m->dim == 3 ? m->zWall[(i + 1) * (ny + 2) + j + 1] : m->zWall[i + 1];
                 if (bc == OR_BC_NEUMANN) {
                     a[IX(L, i, j, -1)] = a[IX(L, i, j, 0)];
-                    a[IX(L, i, j, nz)] = a[IX(L, i, j, nz - 1)];
+                    if (m->dim == 2 && m->p.top_dirichlet)
+                        a[IX(L, i, j, nz)] = -a[IX(L, i, j, nz - 1)];
+                    else
+                        a[IX(L, i, j, nz)] = a[IX(L, i, j, nz - 1)];
                 } else if (inhom) {
                     a[IX(L, i, j, -1)] = 2.0 * wv - a[IX(L, i, j, 0)];
                     a[IX(L, i, j, nz)] = 2.0 * wv - a[IX(L, i, j, nz - 1)];

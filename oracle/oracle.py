@@ -36,7 +36,7 @@ class MgParams(C.Structure):
     _fields_ = [("vc_count", C.c_int), ("vc_tol", C.c_double), ("solve_flag", C.c_int),
                 ("mg_tol", C.c_double), ("sor", C.c_double), ("pre_smooth", C.c_int),
                 ("post_smooth", C.c_int), ("res_type", C.c_int), ("bc", C.c_int),
-                ("no_early_exit", C.c_int)]
+                ("no_early_exit", C.c_int), ("top_dirichlet", C.c_int)]
 
 
 FBC_DIRICHLET, FBC_NEUMANN, FBC_HOT_PLATE, FBC_PLUME_OUT = 0, 1, 2, 3
@@ -139,11 +139,12 @@ def grid_desc(n, dim=3, length=(1.0, 1.0, 1.0), periodic=(0, 0, 0), gtype=(0, 0,
 
 
 def mg_params(vc_count=20, vc_tol=1e-6, solve_flag=1, mg_tol=1e-2, sor=1.0, pre=3, post=6,
-              res_type=0, bc=BC_NEUMANN, no_early_exit=0):
+              res_type=0, bc=BC_NEUMANN, no_early_exit=0, top_dirichlet=0):
     p = MgParams()
     p.vc_count, p.vc_tol, p.solve_flag, p.mg_tol = vc_count, vc_tol, int(solve_flag), mg_tol
     p.sor, p.pre_smooth, p.post_smooth, p.res_type = sor, pre, post, res_type
     p.bc, p.no_early_exit = bc, int(no_early_exit)
+    p.top_dirichlet = int(top_dirichlet)
     return p
 
 

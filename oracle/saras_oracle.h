@@ -57,6 +57,7 @@ typedef struct {
     int res_type;     /* "Residual Type": 0 Linf, 1 L1, 2 L2 */
     int bc;           /* OR_BC_* */
     int no_early_exit;/* TEST_POISSON: run all vc_count cycles (poisson.cc:182-184) */
+    int top_dirichlet;/* 2-D all-Neumann problem of probType 9: Dirichlet top wall (poisson_d2.cc:641-647) */
 } or_mg_params;
 
 typedef struct {

@@ -626,7 +626,8 @@ __global__ void __launch_bounds__(256) k_store_core(double *__restrict__ full, i
 // op 1: wall rule of `axis`.  Phases run in the reference's order: wraps
 // x, y, z (updateFace), then walls x, y, z (imposeBC).
 __global__ void __launch_bounds__(256) k_ghost_phase(double *__restrict__ full, int pad, int dim, int nx, int ny,
-                                                     int nz, int axis, int op, int bc, const double *__restrict__ wall) {
+                                                     int nz, int axis, int op, int bc_lo, int bc_hi,
+                                                     const double *__restrict__ wall) {
     const int ext[3] = {nx + 2, dim == 3 ? ny + 2 : 1, nz + 2};
     const int n[3] = {nx, ny, nz};
     int o1, o2;  // the two other axes
@@ -659,7 +660,7 @@ __global__ void __launch_bounds__(256) k_ghost_phase(double *__restrict__ full, 
         lsrc[axis] = 0;
         hsrc[axis] = n[axis] - 1;
         double wv = 0.0;
-        if (bc == BC_DIRV) {
+        if (bc_lo == BC_DIRV || bc_hi == BC_DIRV) {
             long wi;
             if (dim == 3) wi = (axis == 0) ? (long)(c[1] + 1) * (nz + 2) + c[2] + 1
                              : (axis == 1) ? (long)(c[0] + 1) * (nz + 2) + c[2] + 1
@@ -667,8 +668,8 @@ __global__ void __launch_bounds__(256) k_ghost_phase(double *__restrict__ full, 
             else wi = (axis == 0) ? c[2] + 1 : c[0] + 1;
             wv = wall[wi];
         }
-        full[at(lo[0], lo[1], lo[2])] = wall_rule(bc, full[at(lsrc[0], lsrc[1], lsrc[2])], wv);
-        full[at(hi[0], hi[1], hi[2])] = wall_rule(bc, full[at(hsrc[0], hsrc[1], hsrc[2])], wv);
+        full[at(lo[0], lo[1], lo[2])] = wall_rule(bc_lo, full[at(lsrc[0], lsrc[1], lsrc[2])], wv);
+        full[at(hi[0], hi[1], hi[2])] = wall_rule(bc_hi, full[at(hsrc[0], hsrc[1], hsrc[2])], wv);
     }
 }
 

@@ -187,10 +187,12 @@ struct Hierarchy {
     }
 };
 
-int bc_code(const saras_mg_params &p, const saras_grid_desc &g, int axis, int level, bool inhom_allowed) {
+int bc_code(const saras_mg_params &p, const saras_grid_desc &g, int axis, int level, bool inhom_allowed, bool hi) {
     if (g.periodic[axis]) return saras::BC_PER;
     switch (p.bc) {
-        case SARAS_BC_NEUMANN: return saras::BC_NEU;
+        // multigrid_d2::imposeBC, all-Neumann branch: probType 9 turns the
+        // top wall Dirichlet (poisson_d2.cc:641-647)
+        case SARAS_BC_NEUMANN: return (hi && axis == 2 && p.top_dirichlet) ? saras::BC_DIR : saras::BC_NEU;
         case SARAS_BC_DIRICHLET: return saras::BC_DIR;
         default: return (level == 0 && inhom_allowed) ? saras::BC_DIRV : saras::BC_DIR;
     }
@@ -261,7 +263,8 @@ saras::LevelArgs<T> level_args(const saras_mg *m, const Hierarchy<T> &H, int l, 
         a.n[q] = L.n[q];
         a.c2[q] = L.c2[q];
         a.c1[q] = L.c1[q];
-        a.bc_lo[q] = a.bc_hi[q] = bc_code(m->p, m->g, q, l, inhom);
+        a.bc_lo[q] = bc_code(m->p, m->g, q, l, inhom, false);
+        a.bc_hi[q] = bc_code(m->p, m->g, q, l, inhom, true);
         a.wall[q] = H.wall[q];
     }
     a.sx = L.sx;
@@ -756,9 +759,9 @@ void store_full(saras_mg *m, double *full, const double *core, hipStream_t s) {
             if ((op == 0) != per) continue;
             const int o1 = a == 0 ? 1 : 0, o2 = a == 2 ? 1 : 2;
             const long cnt = (long)ext[o1] * ext[o2];
-            const int bc = bc_code(m->p, m->g, a, 0, true);
+            const int bc_lo = bc_code(m->p, m->g, a, 0, true, false), bc_hi = bc_code(m->p, m->g, a, 0, true, true);
             hipLaunchKernelGGL(saras::k_ghost_phase, dim3(blocks_for(cnt)), dim3(kThreads), 0, s, full, m->g.pad,
-                               m->g.dim, L.n[0], L.n[1], L.n[2], a, op, bc, (const double *)m->h64.wall[a]);
+                               m->g.dim, L.n[0], L.n[1], L.n[2], a, op, bc_lo, bc_hi, (const double *)m->h64.wall[a]);
         }
 }
 
@@ -869,6 +872,10 @@ saras_status saras_mg_create(const saras_grid_desc *grid, const saras_mg_params 
     }
     if (p.res_type < 0 || p.res_type > 2) { set_error("Residual Type must be 0, 1 or 2"); return SARAS_BAD_ARG; }
     if (p.bc < 0 || p.bc > 2 || p.precision < 0 || p.precision > 1) { set_error("bad bc/precision"); return SARAS_BAD_ARG; }
+    if (p.top_dirichlet && !(g.dim == 2 && p.bc == SARAS_BC_NEUMANN && !g.periodic[2])) {
+        set_error("top_dirichlet: 2-D Neumann problems with z walls (poisson_d2.cc:641-647)");
+        return SARAS_BAD_ARG;
+    }
     if (p.vc_count < 0 || p.pre_smooth < 0 || p.post_smooth < 0) { set_error("negative counts"); return SARAS_BAD_ARG; }
     std::lock_guard<std::recursive_mutex> lk(saras::alloc_mutex());
     std::unique_ptr<saras_mg> m(new saras_mg());

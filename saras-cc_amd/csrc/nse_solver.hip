@@ -437,14 +437,12 @@ saras_status saras_nse_create(const saras_grid_desc *grid, const saras_mg_params
             ok = ok && (P.vbc_type[c][w] == SARAS_FBC_DIRICHLET || P.vbc_type[c][w] == SARAS_FBC_NEUMANN);
         const int t = P.tbc_type[w];
         // hotPlate: the bottom wall of 3-D runs (scalar.cc:178-185; PLANAR builds
-        // ignore the heating plate); plumeOut: the top wall of 3-D runs (a PLANAR
-        // plume also turns the 2-D multigrid's top wall Dirichlet,
-        // poisson_d2.cc:641-647, which is not built)
-        const bool special = !planar && ((t == SARAS_FBC_HOT_PLATE && w == 4) || (t == SARAS_FBC_PLUME_OUT && w == 5));
+        // ignore the heating plate); plumeOut: the top wall
+        const bool special = (t == SARAS_FBC_HOT_PLATE && w == 4 && !planar) || (t == SARAS_FBC_PLUME_OUT && w == 5);
         ok = ok && (t == SARAS_FBC_DIRICHLET || t == SARAS_FBC_NEUMANN || (special && P.has_T && !g.periodic[2]));
         if (!ok) {
-            saras::set_last_error("wall kinds: Dirichlet or Neumann; for T of a 3-D run with z walls also the hot "
-                                  "plate (bottom) and the plume outflow (top)");
+            saras::set_last_error("wall kinds: Dirichlet or Neumann; for T of a run with z walls also the hot "
+                                  "plate (bottom, 3-D) and the plume outflow (top)");
             return SARAS_BAD_ARG;
         }
     }
@@ -462,6 +460,10 @@ saras_status saras_nse_create(const saras_grid_desc *grid, const saras_mg_params
     s->planar = planar;
     saras_mg_params mp = *mgp;
     mp.bc = SARAS_BC_NEUMANN;
+    // a planar plume (probType 9) solves for Pp with a Dirichlet top wall
+    // (poisson_d2.cc:641-647); the 3-D solver keeps all walls Neumann
+    const bool open_top_2d = planar && P.has_T && P.tbc_type[5] == SARAS_FBC_PLUME_OUT;
+    mp.top_dirichlet = open_top_2d ? 1 : 0;
     saras_status e = saras_mg_create(&g, &mp, &s->mg);
     if (e) {
         delete s;
@@ -583,7 +585,9 @@ saras_status saras_nse_create(const saras_grid_desc *grid, const saras_mg_params
     }
     s->pbc = make_bc(P.pbc_type, P.pbc_val, hwall);
     {
-        const int neu[6] = {1, 1, 1, 1, 1, 1};
+        // Pp's ghosts as the solver's last imposeBC leaves them: Neumann, and
+        // -Pp(n-1) at a Dirichlet top
+        const int neu[6] = {1, 1, 1, 1, 1, open_top_2d ? 0 : 1};
         const double zero[6] = {0, 0, 0, 0, 0, 0};
         s->ppbc = make_bc(neu, zero, hwall);
     }

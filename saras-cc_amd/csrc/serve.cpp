@@ -59,6 +59,7 @@ std::string key_of(const saras_grid_desc &g, const saras_mg_params &p) {
     put(k, p.precision);
     put(k, p.inner_cycles);
     put(k, p.max_refine);
+    put(k, p.top_dirichlet);
     return k;
 }
 

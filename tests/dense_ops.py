@@ -20,8 +20,8 @@ def axis_coeffs(xix2, xixx, h):
 
 def assemble(shape, coeffs, kinds):
     """shape: core extents per active axis; coeffs: list of (c2, c1) per axis;
-    kinds: per axis 'P', 'N' or 'D'.  Returns the dense matrix A (row-major
-    flattening, last axis fastest)."""
+    kinds: per axis 'P', 'N' or 'D', or a (lo, hi) pair such as 'ND'.  Returns
+    the dense matrix A (row-major flattening, last axis fastest)."""
     shape = tuple(shape)
     N = int(np.prod(shape))
     A = np.zeros((N, N))
@@ -41,13 +41,14 @@ def assemble(shape, coeffs, kinds):
                     q[ax] = j
                     A[flat, idx[tuple(q)]] += w
                 else:
-                    if kind == "P":
+                    kk = kind if len(kind) == 1 else kind[0 if off < 0 else 1]
+                    if kk == "P":
                         q = list(pos)
                         q[ax] = j % n
                         A[flat, idx[tuple(q)]] += w
-                    elif kind == "N":
+                    elif kk == "N":
                         A[flat, flat] += w
-                    elif kind == "D":
+                    elif kk == "D":
                         A[flat, flat] -= w
                     else:
                         raise ValueError(kind)

@@ -386,3 +386,22 @@ def test_fp32_split_rows_match_z_contiguous(case, variant, monkeypatch):
     assert ss["residual"] == so["residual"]
     np.testing.assert_array_equal(xs, xo)
 
+
+
+@pytest.mark.parametrize("tail", [True, False], ids=["tail-kernel", "per-level"])
+@pytest.mark.parametrize("gt", [(0, 0, 0), (0, 0, 2)], ids=["uniform", "tanh-z"])
+def test_2d_top_dirichlet_solve_bitexact(gt, tail, monkeypatch):
+    """multigrid_d2's probType 9 walls (poisson_d2.cc:641-647): Neumann with a
+    homogeneous Dirichlet top; whole solves agree bit for bit, ghosts included."""
+    if not tail:
+        monkeypatch.setenv("SARAS_NO_TAIL", "1")
+    kw = dict(vc_count=12, vc_tol=1e-9, mg_tol=1e-6, pre=2, post=2, top_dirichlet=1)
+    mo, ms = pair((64, 1, 32), 2, (1, 0, 0), gt, O.BC_NEUMANN, **kw)
+    rng = np.random.default_rng(31)
+    b = rhs_for(mo, rng, False)
+    x0 = rhs_for(mo, rng, False) * 0.1
+    xo, so = mo.solve(x0.copy(), b, 1, history=True)
+    xs, ss, status = ms.solve(x0.copy(), b)
+    assert ss["cycles"] == so["cycles"] and ss["residual"] == so["residual"]
+    np.testing.assert_array_equal(xs, xo)
+    assert np.array_equal(xs[1:-1, -1], -xs[1:-1, -2])

@@ -76,6 +76,36 @@ def test_planar_rejects_unsupported_schemes():
     with pytest.raises(S.SarasError):
         S.Nse(g, mg, S.nse_params(dscheme=2, nlscheme=1, has_T=0, nu=0.01, kappa=0.01, dt=1e-3))
     gw = S.grid_desc((16, 1, 16), dim=2, periodic=(1, 1, 0), pad=1)
-    plume = ([S.FBC_NEUMANN] * 4 + [S.FBC_DIRICHLET, S.FBC_PLUME_OUT], [0.0] * 4 + [1.0, 0.0])
-    with pytest.raises(S.SarasError):
-        S.Nse(gw, mg, S.nse_params(dscheme=1, nlscheme=1, has_T=1, nu=0.01, kappa=0.01, dt=1e-3, tbc=plume))
+    plate = ([S.FBC_NEUMANN] * 4 + [S.FBC_HOT_PLATE, S.FBC_PLUME_OUT], [0.0] * 6)
+    with pytest.raises(S.SarasError):   # PLANAR builds have no heating plate
+        S.Nse(gw, mg, S.nse_params(dscheme=1, nlscheme=1, has_T=1, nu=0.01, kappa=0.01, dt=1e-3, tbc=plate))
+
+
+def test_planar_plume_pressure_top_wall():
+    """A planar plume (probType 9): T's top wall is plumeOut and the pressure
+    correction is solved with a Dirichlet top wall (poisson_d2.cc:641-647).
+    Fluid driven by the warm bottom leaves through the top: the steps stay
+    finite, and Pp's ghosts are -Pp(n-1) at the top and Pp(0) at the bottom."""
+    n, pad = 32, 1
+    g = S.grid_desc((n, 1, n), dim=2, periodic=(1, 1, 0), pad=pad)
+    mg = S.mg_params(vc_count=30, vc_tol=1e-10, solve_flag=0, pre=3, post=3, bc=S.BC_NEUMANN, precision=S.PREC_FP64)
+    D, N = S.FBC_DIRICHLET, S.FBC_NEUMANN
+    vbc = [([D] * 6, [0.0] * 6), ([D] * 6, [0.0] * 6), ([D] * 5 + [N], [0.0] * 6)]
+    tbc = ([N] * 4 + [D, S.FBC_PLUME_OUT], [0.0] * 4 + [1.0, 0.0])
+    s = S.Nse(g, mg, S.nse_params(dscheme=1, nlscheme=1, has_T=1, nu=1e-2, kappa=1e-2, dt=1e-3, cn_tol=1e-10,
+                                  buoy=(0.0, 0.0, -1.0), vbc=vbc, tbc=tbc))
+    rng = np.random.default_rng(2)
+    zero = np.zeros(s.shape)
+    t = zero.copy()
+    t[pad:-pad, 0, pad:-pad] = rng.uniform(0.0, 0.1, (n, n))
+    s.set(vx=zero, vy=zero, vz=zero, p=zero, t=t, pp=zero)
+    for _ in range(5):
+        r, st = s.step()
+        S.check(r)
+    f = s.get()
+    for k in ("vx", "vz", "p", "t", "pp"):
+        assert np.all(np.isfinite(f[k])), k
+    assert np.abs(f["vz"][pad:-pad, 0, -pad - 1]).max() > 0           # flow through the open top
+    pp = f["pp"][pad:-pad, 0, :]
+    np.testing.assert_array_equal(pp[:, -1], -pp[:, -2])            # Dirichlet top of the Pp solve
+    np.testing.assert_array_equal(pp[:, 0], pp[:, 1])               # Neumann bottom

@@ -319,3 +319,23 @@ def test_residual_entry_matches_dense_operator():
     ax = mg.apply(pad_core(x))
     np.testing.assert_allclose(ax[1:-1, 1:-1, 1:-1].ravel(), A @ x.ravel(), rtol=1e-12, atol=1e-10)
     assert np.all(ax[0] == 0) and np.all(ax[:, :, -1] == 0)
+
+
+@pytest.mark.parametrize("gt", [(0, 0, 0), (0, 0, 2)], ids=["uniform", "tanh-z"])
+def test_2d_neumann_top_dirichlet_matches_dense(gt):
+    """multigrid_d2 of probType 9 (poisson_d2.cc:641-647): Neumann walls with a
+    homogeneous Dirichlet top.  The problem is no longer singular, so the
+    V-cycle solution equals the dense solve itself; a Neumann top, a wrong
+    sign or the rule on the bottom wall misses it."""
+    n = (16, 1, 16)
+    g, p = make_case(n, 2, (1, 0, 0), gtype=gt, beta=(0.6, 0.5, 0.7), bc=O.BC_NEUMANN, vc_count=80,
+                     vc_tol=1e-12, mg_tol=1e-13, pre=2, post=2, top_dirichlet=1)
+    mg = O.Multigrid(g, p)
+    A, shape, coeffs, kinds = dense_level(g, 0, O.BC_NEUMANN)
+    A = assemble(shape, coeffs, [kinds[0], "ND"])
+    rng = np.random.default_rng(8)
+    xt, b = rand_in_range(A, shape, rng)
+    x, st = mg.solve(np.zeros(O.field_shape(g, 1)), np.pad(b, 1), 1)
+    xc = O.core(x, 1)
+    np.testing.assert_allclose(xc, xt, atol=1e-9 * np.abs(xt).max())
+    assert np.array_equal(x[1:-1, -1], -x[1:-1, -2])

@@ -1,10 +1,10 @@
 """Planar (2-D, x-z) runs of the time-step path: the reference's PLANAR
 build (lsRK3_d2 / eulerCN_d2 with hydro_d2) restated as one y row whose y
-terms are exact zeros.  Pinned by what the mathematics fixes (survey 8(c)):
-the 2-D Taylor-Green vortex is an exact Navier-Stokes solution whose kinetic
-energy decays as exp(-4 nu t) in a (2 pi)^2 box, and by a 3-D run of the same
-flow, uniform in y, which must evolve the same x-z fields to the solvers'
-tolerances."""
+terms are exact zeros.  Pinned by what the mathematics fixes: the 2-D
+Taylor-Green vortex is an exact Navier-Stokes solution whose kinetic energy
+decays as exp(-4 nu t) in a (2 pi)^2 box, and by a 3-D run of the same flow,
+uniform in y, which must evolve the same x-z fields to the solvers'
+tolerances.  The plume case checks the pressure correction's wall rules."""
 import numpy as np
 import pytest
 

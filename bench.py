@@ -1,32 +1,40 @@
-"""Benchmark of the multigrid V-cycle on one MI355X.
+"""Benchmark of the multigrid Poisson solve on one MI355X.
 
-Workload (one "step" = one V-cycle): 256^3 cells, periodic in x and y,
-Neumann walls in z, double-sided tanh stretching beta = 1.5 in z (the grid of
-the 256^3 Rayleigh-Benard configuration), RB-SOR omega = 1, V(2,2), coarsest
-2^3 level solved to 1e-2 ("Solve Coarsest" defaults).  The headline V-cycle
-is the FP32 inner cycle of the mixed-precision solver; the FP64 cycle of the
-plain path is reported beside it.
+Workload (one "step" = one solve): the 256^3 Poisson problem on the periodic
+unit cube, RHS = 64 Fourier modes with random phases and amplitudes
+~ U(0.5, 1) |k|^-2 (seeded, zero mean), RB-SOR omega = 1, V(2,2) cycles,
+coarsest 2^3 level solved to 1e-2, solved from a zero initial guess to a
+relative L-inf residual of 1e-10 (the FP64 accuracy the plain path reaches)
+by the mixed-precision solver: FP32 V-cycles inside an FP64 residual and
+iterative-refinement loop, inputs and outputs resident in HBM.  The plain
+FP64 solve of the same problem is timed beside it.
 
-value: algorithmic HBM bytes per V-cycle / device time per V-cycle, with the
-per-level byte model  B_l = N_l s (3 (nu1 + nu2) + 2 + 1/8 + 2 + 1/8)
+value: algorithmic HBM bytes of the mixed solve / its device time, in TB/s:
+FP32 cycles x B_V + refinement passes x 32 N + the initial residual 28 N,
+with the per-level V-cycle model  B_l = N_l s (3 (nu1 + nu2) + 2 + 1/8 + 2 + 1/8)
 (3 words per smoothing sweep, residual+restriction reads x, b and writes
-b_{l+1}, prolongation reads e_{l+1} and reads/writes x), s = bytes per word.
+b_{l+1}, prolongation reads e_{l+1} and reads/writes x), s = bytes per word;
+a refinement pass reads x64, b64, e32 and writes x64, r32.
+fp64_equivalent_tbps: the bytes the plain FP64 solve moves for the same
+result (its cycles x (B_V + 16 N for computeError)) / the mixed solve's time.
 
-solve: time to a 1e-10 relative L-inf residual on 256^3 and 512^3 periodic
-uniform grids, RHS = 64 Fourier modes with amplitudes ~ |k|^-2 and zero mean,
-plain FP64 vs mixed precision, inputs already resident in HBM; the mixed
-solution's relative L2 / L-inf difference from the FP64 solution (which
-reproduces the FP64 CPU oracle bit for bit), modulo the mean.
+fp32_vcycle / fp64_vcycle: one V-cycle of each precision on the grid of the
+256^3 Rayleigh-Benard configuration (periodic x, y; Neumann z walls; z-tanh
+beta = 1.5), B_V / time per cycle.
+solve_512^3: the same comparison at 512^3 (config 3's grid and RHS).
 
-roofline: the dominant kernel (one colour pass of the finest level on
-colour-split rows): algorithmic 1.5 words per cell per launch (half of a
-sweep's 3) / average launch time (HIP events around the finest-level sweeps).
-cpu_baseline: the FP64 CPU oracle's V-cycle on the same workload.
+roofline: the dominant kernel of the headline solve (one colour pass of the
+finest level on colour-split rows, FP32): algorithmic 1.5 words per cell per
+launch (half of a sweep's 3) / average launch time (HIP events around
+finest-level sweeps on the solve's own handle and stream).
+cpu_baseline: the FP64 CPU oracle solving the same 256^3 problem to the same
+tolerance (the reference algorithm, OpenMP threads = the host's CPU share).
 
 serving: the batched request loop on the Poisson request mix (75 % 3-D
-n^3, n log-uniform in [32, 256]; 25 % 2-D n^2, n in [64, 2048]; multiples of
-16; Dirichlet / Neumann / periodic; 8-mode Fourier RHS; tolerance 1e-8):
-solves/s with every request queued at once (saturation; free slots take the
+n^3, n log-uniform in [32, 256]; 25 % 2-D n^2, n in [64, 2048]; rounded to
+multigrid-friendly multiples of 16; Dirichlet / Neumann / periodic; 8-mode
+Fourier RHS; tolerance 1e-8; "Solve Coarsest" on): converged solves/s with
+every request queued at once (saturation; free slots take the
 waiting request of largest footprint first), p50/p99 latency of
 a seeded Poisson arrival stream at 70 % of that rate, and the same requests
 solved one at a time with plain FP64 solves (one slot, no graphs), with a
@@ -187,6 +195,83 @@ def solve_compare(nu1, nu2, tol=1e-10, n=N, reps=3, inner=4):
     return res
 
 
+def solve_bytes(shapes, nu1, nu2, prec, cycles, refinements):
+    """Algorithmic HBM bytes of a whole solve (see the module docstring)."""
+    n0 = float(np.prod(shapes[0]))
+    if prec == S.PREC_FP64:
+        return cycles * (bytes_per_vcycle(shapes, 8, nu1, nu2) + 16 * n0)
+    return cycles * bytes_per_vcycle(shapes, 4, nu1, nu2) + refinements * 32 * n0 + 28 * n0
+
+
+def headline_solve(steps, warmup, nu1, nu2, n=N, tol=1e-10, inner=4):
+    """The headline: `steps` timed mixed-precision solves of the 256^3 periodic
+    Fourier problem (each from a zero initial guess, device arrays), after
+    `warmup` untimed ones; then the plain FP64 solve of the same problem, the
+    dominant kernel's launch time on the mixed handle, and the difference of
+    the two solutions."""
+    g = S.grid_desc((n, n, n), periodic=(1, 1, 1))
+    b = fourier_rhs(n)
+    bd = S.DeviceArray.from_host(b)
+    xd = S.DeviceArray(b.nbytes)
+    out, sols = {}, {}
+    for name, prec in (("mixed", S.PREC_MIXED), ("fp64", S.PREC_FP64)):
+        p = S.mg_params(vc_count=100, vc_tol=tol, mg_tol=1e-2, pre=nu1, post=nu2, bc=S.BC_NEUMANN,
+                        precision=prec, inner_cycles=inner, max_refine=40)
+        mg = S.Multigrid(g, p)
+        shapes = [mg.level_shape(lv) for lv in range(mg.levels)]
+        k = steps if prec == S.PREC_MIXED else max(3, steps // 4)
+        for _ in range(warmup):
+            xd.zero()
+            mg.solve_device(xd, bd)
+        S.device_sync()
+        t0 = time.perf_counter()
+        dev_ms, stats = [], []
+        for _ in range(k):
+            xd.zero()
+            st, status = mg.solve_device(xd, bd)
+            S.check(status)
+            dev_ms.append(st["device_ms"])
+            stats.append(st)
+        S.device_sync()
+        wall = (time.perf_counter() - t0) * 1e3 / k
+        sols[name] = xd.to_host(b.shape)[1:-1, 1:-1, 1:-1]
+        st = stats[-1]
+        assert all(s["cycles"] == st["cycles"] and s["refinements"] == st["refinements"] for s in stats)
+        ms = float(np.mean(dev_ms))
+        bts = solve_bytes(shapes, nu1, nu2, prec, st["cycles"], st["refinements"])
+        r = dict(ms=ms, ms_min=float(np.min(dev_ms)), wall_ms=wall, solves=k, cycles=st["cycles"],
+                 refinements=st["refinements"], residual=st["residual"], converged=st["converged"],
+                 bytes=bts, tbps=bts / (ms * 1e-3) / 1e12, shapes=shapes)
+        if prec == S.PREC_MIXED:
+            # the dominant kernel: finest-level FP32 colour passes on this
+            # handle's stream (a sweep = two launches of equal work)
+            mg.bench_sweeps(2)
+            reps = 20
+            r["launch_ms"] = mg.bench_sweeps(reps) / reps / 2
+            r["launch_bytes"] = 1.5 * float(np.prod(shapes[0])) * 4
+        out[name] = r
+        mg.close()
+    bd.free()
+    xd.free()
+    d = (sols["mixed"] - sols["mixed"].mean()) - (sols["fp64"] - sols["fp64"].mean())
+    ref = sols["fp64"] - sols["fp64"].mean()
+    out["diff_l2_rel"] = float(np.linalg.norm(d) / np.linalg.norm(ref))
+    out["diff_linf_rel"] = float(np.abs(d).max() / np.abs(ref).max())
+    return out
+
+
+def field_diffs(a, b):
+    """Per field: L2 and L-inf of a - b over the core, each relative to b's
+    (absolute where b is zero)."""
+    out = {}
+    for k in b:
+        d = a[k] - b[k]
+        l2, li = float(np.sqrt(np.mean(d * d))), float(np.abs(d).max())
+        n2, ni = float(np.sqrt(np.mean(b[k] * b[k]))), float(np.abs(b[k]).max())
+        out[k] = dict(l2_rel=l2 / n2 if n2 > 0 else l2, linf_rel=li / ni if ni > 0 else li)
+    return out
+
+
 def rbc_fields(n, pad, seed=2):
     """T = 1 - z + U(-1e-3, 1e-3) (seeded), u = 0, P = 1, Pp = 0 on the z-tanh grid."""
     sh = (n + 2 * pad,) * 3
@@ -203,10 +288,10 @@ def rbc_fields(n, pad, seed=2):
 def rbc_step(steps, warmup, n=256):
     pad = 2
     g, f = rbc_fields(n, pad)
-    nu = (1.0 / 1e7) ** 0.5
-    prm = S.nse_params(dscheme=2, nlscheme=3, has_T=1, nu=nu, kappa=nu, dt=1e-3, cn_tol=1e-6,
-                       buoy=(0.0, 0.0, -1.0))
-    out = {}
+    ph = S.nse_physics(5, ra=1e7, pr=1.0, rbc_type=2, g=(0.0, 0.0, -1.0))   # Ra 1e7, Pr 1, rbcType 2
+    prm = S.nse_params(dscheme=2, nlscheme=3, has_T=1, nu=ph["nu"], kappa=ph["kappa"], dt=1e-3, cn_tol=1e-6,
+                       buoy=ph["buoy"])
+    out, fields = {}, {}
     for name, prec in (("fp64", S.PREC_FP64), ("mixed", S.PREC_MIXED)):
         # every pressure solve of this run takes all 20 cycles (its residual
         # plateaus near 1e-4), so the mixed solver may run up to 5 FP32 cycles
@@ -226,7 +311,10 @@ def rbc_step(steps, warmup, n=256):
         out[name] = dict(ms_per_step=round(ms / steps, 3), steps=steps, jacobi_iters=st["jacobi_iters"],
                          pressure_cycles=st["mg"]["cycles"], pressure_residual=st["mg"]["residual"],
                          ke=d["ke"], divmax=d["divmax"], nu_vol=d["nuvol"], nu_wall=d["nuwall"])
+        fields[name] = {k: v[(slice(pad, -pad),) * 3].copy() for k, v in s.get().items() if k != "pp"}
         s.close()
+    out["mixed_vs_fp64"] = field_diffs(fields["mixed"], fields["fp64"])
+    out["steps_compared"] = warmup + steps + 1
     return out
 
 
@@ -249,28 +337,36 @@ def les_step(steps, warmup, n=384, noise=0.01, seed=5):
     v += noise * rng.uniform(-1, 1, sh)
     w = noise * rng.uniform(-1, 1, sh)
     zero = np.zeros(sh)
-    prm = S.nse_params(dscheme=2, nlscheme=3, has_T=0, nu=1.0 / 1600, kappa=1.0 / 1600, dt=1e-3, cn_tol=1e-6,
+    nu = S.nse_physics(2, re=1600.0)["nu"]
+    prm = S.nse_params(dscheme=2, nlscheme=3, has_T=0, nu=nu, kappa=nu, dt=1e-3, cn_tol=1e-6,
                        les=1, vbc=[([S.FBC_DIRICHLET] * 6, [0.0] * 6)] * 3)
-    # these solves stop before the 20-cycle cap, so the residual is checked
-    # (one refinement) every 2 FP32 cycles
-    mg = S.mg_params(vc_count=20, vc_tol=1e-6, solve_flag=0, mg_tol=1e-2, pre=3, post=6, bc=S.BC_NEUMANN,
-                     precision=S.PREC_MIXED, inner_cycles=2, max_refine=10)
-    s = S.Nse(g, mg, prm)
-    s.set(vx=u, vy=v, vz=w, p=zero, t=zero, pp=zero)
-    del u, v, w
-    for _ in range(max(warmup, 6)):
-        r, st = s.step()
+    res, fields = {}, {}
+    for name, prec in (("mixed", S.PREC_MIXED), ("fp64", S.PREC_FP64)):
+        # these solves stop before the 20-cycle cap, so the residual is checked
+        # (one refinement) every 2 FP32 cycles
+        mg = S.mg_params(vc_count=20, vc_tol=1e-6, solve_flag=0, mg_tol=1e-2, pre=3, post=6, bc=S.BC_NEUMANN,
+                         precision=prec, inner_cycles=2, max_refine=10)
+        s = S.Nse(g, mg, prm)
+        s.set(vx=u, vy=v, vz=w, p=zero, t=zero, pp=zero)
+        for _ in range(max(warmup, 6)):
+            r, st = s.step()
+            S.check(r)
+        r, ms = s.bench_steps(steps)
         S.check(r)
-    r, ms = s.bench_steps(steps)
-    S.check(r)
-    d = s.diag()
-    sg, t = s.sgs()
+        d = s.diag()
+        sg, t = s.sgs()
+        res[name] = dict(ms=ms, st=st, d=d, sg=sg, t=t)
+        fields[name] = {k: v[(slice(pad, -pad),) * 3].copy() for k, v in s.get().items() if k in ("vx", "vy", "vz", "p")}
+        s.close()
     vol = L ** 3
-    s.close()
-    return dict(ms_per_step=round(ms / steps, 3), steps=steps, cells=[n, n, n], jacobi_iters=st["jacobi_iters"],
-                pressure_cycles=st["mg"]["cycles"], pressure_residual=st["mg"]["residual"], time=t, ke=d["ke"],
-                divmax=d["divmax"],
-                sgs_ke=sg["sgke"] / vol, sgs_dissipation=sg["disp"] / vol, nu_sgs=sg["nusg"] / vol)
+    m = res["mixed"]
+    return dict(ms_per_step=round(m["ms"] / steps, 3), fp64_pressure_ms_per_step=round(res["fp64"]["ms"] / steps, 3),
+                steps=steps, cells=[n, n, n], jacobi_iters=m["st"]["jacobi_iters"],
+                pressure_cycles=m["st"]["mg"]["cycles"], pressure_residual=m["st"]["mg"]["residual"], time=m["t"],
+                ke=m["d"]["ke"], ke_fp64_pressure=res["fp64"]["d"]["ke"], divmax=m["d"]["divmax"],
+                sgs_ke=m["sg"]["sgke"] / vol, sgs_dissipation=m["sg"]["disp"] / vol, nu_sgs=m["sg"]["nusg"] / vol,
+                mixed_vs_fp64_pressure=field_diffs(fields["mixed"], fields["fp64"]),
+                steps_compared=max(warmup, 6) + steps)
 
 
 def serving(count=240, seed=4, slots=8, budget=96 << 30):
@@ -293,6 +389,12 @@ def serving(count=240, seed=4, slots=8, budget=96 << 30):
     opn = V.summarize(V.run_stream(srv, reqs, rhs, S.PREC_MIXED, rate=rate)[0])
     note(f"open loop at {rate:.1f}/s: p50 {opn['p50_ms']:.2f} ms, p99 {opn['p99_ms']:.2f} ms")
     srv.close()
+    # the same server with plain FP64 solves: the batching gain alone
+    s64 = S.Server(budget, slots=slots, use_graphs=1, largest_first=1)
+    V.run_stream(s64, reqs, rhs, S.PREC_FP64)                           # warm-up
+    sat64 = V.summarize(V.run_stream(s64, reqs, rhs, S.PREC_FP64)[0])
+    s64.close()
+    note(f"saturation, FP64 solves {sat64['solves_per_s']:.1f} solves/s")
     # the plain way: one request at a time, FP64, a solver built for each
     # request and released after it (poisson's constructor + mgSolve)
     cold = S.Server(budget, slots=1, use_graphs=0, keep_resident=0)
@@ -307,69 +409,119 @@ def serving(count=240, seed=4, slots=8, budget=96 << 30):
     base.close()
     n3 = sum(1 for r in reqs if r["dim"] == 3)
     return {"requests": count, "requests_3d": n3, "slots": slots, "mem_budget_gb": budget / 2**30,
-            "saturation": sat, "open_loop": dict(rate_per_s=round(rate, 1), **opn),
+            "saturation": sat, "open_loop": dict(rate_per_s=round(rate, 1), **opn), "saturation_fp64": sat64,
             "one_at_a_time_fp64": one_cold, "one_at_a_time_fp64_resident": one,
             "speedup_vs_one_at_a_time": round(sat["solves_per_s"] / one_cold["solves_per_s"], 3),
-            "speedup_vs_one_at_a_time_resident": round(sat["solves_per_s"] / one["solves_per_s"], 3)}
+            "speedup_vs_one_at_a_time_resident": round(sat["solves_per_s"] / one["solves_per_s"], 3),
+            "speedup_fp64_server_vs_one_at_a_time": round(sat64["solves_per_s"] / one_cold["solves_per_s"], 3),
+            "note": "solves_per_s counts converged solves; the mixed server against one-at-a-time FP64 combines the "
+                    "batching and precision gains, the FP64 server against it shows batching alone"}
 
 
-def cpu_baseline(nu1, nu2, shapes, budget_s=12.0):
+def host_threads():
+    """Threads for the CPU legs: one per physical core of this process's CPU
+    set, at most OMP_NUM_THREADS (the host's CPU share for one GPU where the
+    environment sets it).  Returns (threads, description)."""
+    cpus = sorted(os.sched_getaffinity(0))
+    phys = set()
+    for c in cpus:
+        try:
+            base = f"/sys/devices/system/cpu/cpu{c}/topology/"
+            phys.add((open(base + "physical_package_id").read().strip(), open(base + "core_id").read().strip()))
+        except OSError:
+            phys.add(("?", str(c)))
+    model = "?"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    n = len(phys) if share <= 0 else min(len(phys), share)
+    desc = (f"{model}; {len(cpus)} logical CPUs = {len(phys)} physical cores in this process's CPU set, "
+            f"OMP_NUM_THREADS={share or 'unset'}: {n} threads, one per physical core")
+    return max(1, n), desc
+
+
+def cpu_baseline(nu1, nu2, n=N, tol=1e-10, budget_s=20.0):
+    """The FP64 CPU oracle (the reference algorithm) solving the headline
+    problem to the same tolerance, as many whole solves as fit the budget
+    (at least one)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
-    threads = min(16, os.cpu_count() or 1)
+    threads, desc = host_threads()
     O.set_threads(threads)
-    g = O.grid_desc((N, N, N), periodic=(1, 1, 0), gtype=(0, 0, 2), beta=(1.0, 1.0, 1.5))
-    mg = O.Multigrid(g, O.mg_params(vc_count=1, mg_tol=1e-2, pre=nu1, post=nu2, bc=O.BC_NEUMANN))
-    b = rbc_rhs()
-    mg.set_level(0, np.zeros_like(b), b)
-    mg.vcycles(1)
+    g = O.grid_desc((n, n, n), periodic=(1, 1, 1))
+    mg = O.Multigrid(g, O.mg_params(vc_count=100, vc_tol=tol, mg_tol=1e-2, pre=nu1, post=nu2, bc=O.BC_NEUMANN))
+    b = fourier_rhs(n)
+    shapes = [mg.level_shape(lv) for lv in range(mg.levels)]
     t0 = time.perf_counter()
-    k = 0
+    k, cycles = 0, 0
     while True:
-        mg.vcycles(1)
+        _, st = mg.solve(np.zeros_like(b), b, 1)
         k += 1
+        cycles = st["cycles"]
         el = time.perf_counter() - t0
-        if el > budget_s or k >= 40:
+        if el > budget_s * 0.5 or k >= 5:
             break
     per = el / k
-    bv = bytes_per_vcycle(shapes, 8, nu1, nu2)
-    return dict(value=round(bv / per / 1e12, 5), unit="TB/s", cores=threads, kind="port",
-                sample=f"{k} FP64 V(2,2) cycles of the 256^3 workload on the FP64 oracle "
-                       f"({threads} OpenMP threads), {per * 1e3:.1f} ms per cycle")
+    bts = solve_bytes(shapes, nu1, nu2, S.PREC_FP64, cycles, 0)
+    return dict(value=round(bts / per / 1e12, 5), unit="TB/s", cores=threads, kind="port",
+                ms_per_solve=round(per * 1e3, 1),
+                sample=f"{k} whole FP64 solves of the 256^3 headline problem to 1e-10 ({cycles} V(2,2) cycles each) "
+                       f"on the FP64 oracle, {per * 1e3:.0f} ms per solve; {desc}")
 
 
-def cpu_serving_baseline(count=240, seed=4, budget_s=15.0):
+def cpu_serving_baseline(count=240, seed=4, budget_s=20.0):
     """The serving mix solved one request at a time by the FP64 CPU oracle
-    (same grids, parameters and right-hand sides as the GPU legs), over the
-    requests that fit a bounded time budget, in stream order."""
+    (same grids, parameters and right-hand sides as the GPU legs) on a seeded
+    random sample of the stream that fits the time budget, so the sample spans
+    the size range.  value: the stream's solves/s estimated from the sample's
+    cell throughput (a solve costs about cells x cycles, and the cycle count
+    is nearly the same across the mix): count / (stream cells / sample cells
+    per second)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     from saras_amd import serving as V
 
-    threads = min(16, os.cpu_count() or 1)
+    threads, desc = host_threads()
     O.set_threads(threads)
     reqs = V.draw_requests(count, seed=seed)
+    cells = lambda r: float(r["n"]) ** r["dim"]  # noqa: E731
+    order = np.random.default_rng(seed + 1).permutation(len(reqs))
     rhs = V.RhsCache()
-    t0 = time.perf_counter()
-    done = 0
-    for r in reqs:
+    t_solve, done, c_done, dims = 0.0, 0, 0.0, []
+    t_all = time.perf_counter()
+    for q in order:
+        r = reqs[int(q)]
         g = V.request_grid(r)
         p = V.request_params(r, S.PREC_FP64)
         go = O.grid_desc(tuple(g.n), dim=g.dim, periodic=tuple(g.periodic), pad=1)
         po = O.mg_params(vc_count=p.vc_count, vc_tol=p.vc_tol, solve_flag=p.solve_flag, mg_tol=p.mg_tol,
                          pre=p.pre_smooth, post=p.post_smooth, res_type=p.res_type, bc=p.bc)
         b = rhs.get(r)
+        t0 = time.perf_counter()
         mg = O.Multigrid(go, po)
         mg.solve(np.zeros_like(b), b, 1)
         del mg
+        t_solve += time.perf_counter() - t0
         done += 1
-        if time.perf_counter() - t0 > budget_s:
+        c_done += cells(r)
+        dims.append((r["dim"], r["n"]))
+        if time.perf_counter() - t_all > budget_s:
             break
-    el = time.perf_counter() - t0
-    return dict(value=round(done / el, 3), unit="solves/s", cores=threads, kind="port",
-                sample=f"the first {done} requests of the serving stream, one at a time, FP64 oracle "
-                       f"({threads} OpenMP threads), {el:.1f} s")
+    c_all = sum(cells(r) for r in reqs)
+    est = count / (c_all / (c_done / t_solve))
+    n3 = [n for d, n in dims if d == 3]
+    n2 = [n for d, n in dims if d == 2]
+    return dict(value=round(est, 3), unit="solves/s", cores=threads, kind="port",
+                sample=f"{done} requests drawn at random from the stream ({len(n3)} 3-D, n {min(n3, default=0)}-"
+                       f"{max(n3, default=0)}; {len(n2)} 2-D, n {min(n2, default=0)}-{max(n2, default=0)}), one at a "
+                       f"time on the FP64 oracle, {t_solve:.1f} s of solves = {done / t_solve:.3f} solves/s on the "
+                       f"sample; scaled to the stream by cells; {desc}")
 
 
 def main():
@@ -378,9 +530,11 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--no-solve", action="store_true", help="skip the 512^3 solve comparison")
     ap.add_argument("--profile-sweep", action="store_true",
                     help="only finest-level sweeps (FP32 and FP64) and the copy kernel, for counter runs")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the headline solves (mixed and FP64), for profiling")
     ap.add_argument("--rbc-only", action="store_true",
                     help="only the 256^3 Rayleigh-Benard time steps (warmup + steps, FP64 and mixed)")
     ap.add_argument("--no-rbc", action="store_true")
@@ -409,10 +563,15 @@ def main():
     if args.serve_only:
         print(json.dumps({"serving": serving(count=args.requests, slots=args.slots)}))
         return
+    hs = headline_solve(args.steps, args.warmup, nu1, nu2)
+    mx, f64 = hs["mixed"], hs["fp64"]
+    if args.headline_only:
+        print(json.dumps({"mixed_ms": mx["ms"], "fp64_ms": f64["ms"]}))
+        return
+    print("headline solves done", file=sys.stderr, flush=True)
+    value = mx["tbps"]
+    achieved = mx["launch_bytes"] / (mx["launch_ms"] * 1e-3) / 1e9
     rates = cycle_rates(args, nu1, nu2)
-    head, f64 = rates["mixed"], rates["fp64"]
-    value = head["bv"] / (head["ms"] * 1e-3) / 1e12
-    achieved = head["launch_bytes"] / (head["launch_ms"] * 1e-3) / 1e9
     copy_n = 1 << 28
     S.bench_stream_copy(copy_n, 2)
     copy_ms = S.bench_stream_copy(copy_n, 10) / 10
@@ -422,46 +581,62 @@ def main():
         "value": round(value, 4),
         "unit": "TB/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": mx["solves"],
         "warmup": args.warmup,
-        "ms_per_step": round(head["ms"], 4),
+        "ms_per_step": round(mx["ms"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f32 V-cycles + f64 residual/refinement (f64-accurate result)",
         "data": "synthetic",
-        "config": {"workload": "poisson_vcycle_256^3_PPN_z-tanh1.5_neumann_V(2,2)", "cells": [N, N, N],
-                   "levels": len(head["shapes"]), "cycle_precision": "fp32 inner cycle of the mixed solver"},
+        "config": {"workload": "poisson_solve_256^3_periodic_64-mode-fourier_V(2,2)_to_1e-10_mixed",
+                   "cells": [N, N, N], "levels": len(mx["shapes"]), "tolerance": 1e-10,
+                   "step": "one whole solve from a zero initial guess, device-resident arrays"},
+        "solve": {"mixed_ms": round(mx["ms"], 4), "mixed_ms_min": round(mx["ms_min"], 4),
+                  "mixed_wall_ms": round(mx["wall_ms"], 4), "mixed_fp32_cycles": mx["cycles"],
+                  "mixed_refinements": mx["refinements"], "mixed_residual": mx["residual"],
+                  "mixed_converged": mx["converged"], "mixed_bytes": mx["bytes"],
+                  "fp64_ms": round(f64["ms"], 4), "fp64_cycles": f64["cycles"], "fp64_residual": f64["residual"],
+                  "fp64_converged": f64["converged"], "fp64_bytes": f64["bytes"],
+                  "fp64_tbps": round(f64["tbps"], 4),
+                  "fp64_equivalent_tbps": round(f64["bytes"] / (mx["ms"] * 1e-3) / 1e12, 4),
+                  "speedup_mixed_vs_fp64": round(f64["ms"] / mx["ms"], 3),
+                  "mixed_vs_fp64_l2_rel": hs["diff_l2_rel"], "mixed_vs_fp64_linf_rel": hs["diff_linf_rel"]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": round(DOMINANT_TRAFFIC_BYTES),
-                     "kernel": DOMINANT, "launch_ms": round(head["launch_ms"], 5),
-                     "bytes_per_launch": head["launch_bytes"]},
-        "fp64_vcycle": {"ms": round(f64["ms"], 4),
-                        "tbps": round(f64["bv"] / (f64["ms"] * 1e-3) / 1e12, 4),
-                        "pass_launch_ms": round(f64["launch_ms"], 5),
-                        "pass_gbs": round(f64["launch_bytes"] / (f64["launch_ms"] * 1e-3) / 1e9, 1)},
+                     "kernel": DOMINANT, "launch_ms": round(mx["launch_ms"], 5),
+                     "bytes_per_launch": mx["launch_bytes"]},
+        "fp32_vcycle": {"grid": "256^3 PPN z-tanh 1.5 (Rayleigh-Benard)", "ms": round(rates["mixed"]["ms"], 4),
+                        "tbps": round(rates["mixed"]["bv"] / (rates["mixed"]["ms"] * 1e-3) / 1e12, 4)},
+        "fp64_vcycle": {"grid": "256^3 PPN z-tanh 1.5 (Rayleigh-Benard)", "ms": round(rates["fp64"]["ms"], 4),
+                        "tbps": round(rates["fp64"]["bv"] / (rates["fp64"]["ms"] * 1e-3) / 1e12, 4),
+                        "pass_launch_ms": round(rates["fp64"]["launch_ms"], 5),
+                        "pass_gbs": round(rates["fp64"]["launch_bytes"] / (rates["fp64"]["launch_ms"] * 1e-3) / 1e9,
+                                          1)},
         "stream_copy_gbs": round(copy_gbs, 1),
     }
     if not args.no_solve:
-        for n, reps in ((256, 3), (512, 2)):
-            sc = solve_compare(nu1, nu2, n=n, reps=reps)
-            out[f"solve_{n}^3_periodic_to_1e-10"] = {
-                "fp64_ms": round(sc["fp64"]["ms"], 3), "fp64_cycles": sc["fp64"]["cycles"],
-                "fp64_residual": sc["fp64"]["residual"], "fp64_converged": sc["fp64"]["converged"],
-                "mixed_ms": round(sc["mixed"]["ms"], 3), "mixed_fp32_cycles": sc["mixed"]["cycles"],
-                "mixed_refinements": sc["mixed"]["refinements"], "mixed_residual": sc["mixed"]["residual"],
-                "mixed_converged": sc["mixed"]["converged"],
-                "speedup": round(sc["fp64"]["ms"] / sc["mixed"]["ms"], 3),
-                "mixed_vs_fp64_l2_rel": sc["diff_l2_rel"], "mixed_vs_fp64_linf_rel": sc["diff_linf_rel"]}
-            print(f"solve {n}^3 done", file=sys.stderr, flush=True)
+        sc = solve_compare(nu1, nu2, n=512, reps=2)
+        out["solve_512^3_periodic_to_1e-10"] = {
+            "fp64_ms": round(sc["fp64"]["ms"], 3), "fp64_cycles": sc["fp64"]["cycles"],
+            "fp64_residual": sc["fp64"]["residual"], "fp64_converged": sc["fp64"]["converged"],
+            "mixed_ms": round(sc["mixed"]["ms"], 3), "mixed_fp32_cycles": sc["mixed"]["cycles"],
+            "mixed_refinements": sc["mixed"]["refinements"], "mixed_residual": sc["mixed"]["residual"],
+            "mixed_converged": sc["mixed"]["converged"],
+            "speedup": round(sc["fp64"]["ms"] / sc["mixed"]["ms"], 3),
+            "mixed_vs_fp64_l2_rel": sc["diff_l2_rel"], "mixed_vs_fp64_linf_rel": sc["diff_linf_rel"]}
+        print("solve 512^3 done", file=sys.stderr, flush=True)
     if not args.no_serve:
         out["serving"] = serving(count=args.requests, slots=args.slots)
     if not args.no_rbc:
         out["rbc_256_step"] = rbc_step(3, 1)
+        print("rbc step done", file=sys.stderr, flush=True)
     if not args.no_les:
         out["les_384_step"] = les_step(3, 6)
+        print("les step done", file=sys.stderr, flush=True)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(nu1, nu2, f64["shapes"])
+        out["cpu_baseline"] = cpu_baseline(nu1, nu2)
+        print("cpu solve baseline done", file=sys.stderr, flush=True)
         if not args.no_serve:
             out["cpu_baseline"]["serving"] = cpu_serving_baseline(count=args.requests)
     print(json.dumps(out))

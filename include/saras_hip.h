@@ -227,6 +227,17 @@ typedef struct {
 
 typedef struct saras_nse saras_nse;
 
+/* The diffusivities and the buoyancy vector of a run from the deck's
+ * "Problem Type", Re, Ra, Pr, "RBC Type" and gravity axis:
+ * timestep::timestep (timestep.cc:70-98): prob_type 1..4 (hydrodynamics)
+ * nu = 1/Re (kappa = 0, buoy = 0); prob_type 5..9 (with a scalar) by
+ * rbc_type 1: (Pr, 1), 2 and 4: (sqrt(Pr/Ra), 1/sqrt(Pr Ra)), 3: (1, 1/Pr);
+ * buoyantForce::buoyantForce (buoyantForce.cc:48-60): buoy = Fb g/|g| with
+ * Fb = Ra Pr, 1, Ra, Pr for rbc_type 1..4.  SARAS_BAD_ARG where the
+ * reference aborts (an rbc_type outside 1..4) or divides by zero. */
+saras_status saras_nse_physics(int prob_type, double re, double ra, double pr, int rbc_type,
+                               const double g_axis[3], double *nu, double *kappa, double buoy[3]);
+
 /* lsRK3_d3::lsRK3_d3 + timestep::timestep (lsRK3_d3.cc:58-70,
  * timestep.cc:57-122) with the fields of hydro/scalar (vfield, sfield).
  * mg: the "Multigrid" keys of the pressure solve (bc is forced to Neumann,
@@ -376,6 +387,7 @@ saras_status saras_device_free(void *ptr);
 saras_status saras_memcpy_h2d(void *dst, const void *src, size_t bytes);
 saras_status saras_memcpy_d2h(void *dst, const void *src, size_t bytes);
 saras_status saras_device_sync(void);
+saras_status saras_device_zero(void *ptr, size_t bytes);   /* synchronous */
 
 /* Text of the last error raised on the calling thread ("" if none). */
 const char *saras_last_error(void);

@@ -1027,3 +1027,30 @@ double or_nse_cfl(or_nse *s, double courant, double dt_max)
     s->d.dt = dt;
     return dt;
 }
+
+/* timestep::timestep (timestep.cc:70-98) and buoyantForce::buoyantForce
+ * (buoyantForce.cc:48-60), as a table by rbcType: returns 0, or -1 where the
+ * reference exits (rbcType outside 1..4) or would divide by zero. */
+int or_nse_physics(int prob_type, double re, double ra, double pr, int rbc_type, const double g[3], double *nu,
+                   double *kappa, double buoy[3])
+{
+    buoy[0] = buoy[1] = buoy[2] = 0.0;
+    *kappa = 0.0;
+    if (prob_type >= 1 && prob_type <= 4) {
+        if (re <= 0.0) return -1;
+        *nu = 1.0 / re;
+        return 0;
+    }
+    if (prob_type < 5 || prob_type > 9 || rbc_type < 1 || rbc_type > 4) return -1;
+    /* rows: rbcType 1..4; columns: nu, kappa, Fb */
+    const double tab[4][3] = {{pr, 1.0, ra * pr},
+                              {sqrt(pr / ra), 1.0 / sqrt(pr * ra), 1.0},
+                              {1.0, 1.0 / pr, ra},
+                              {sqrt(pr / ra), 1.0 / sqrt(pr * ra), pr}};
+    *nu = tab[rbc_type - 1][0];
+    *kappa = tab[rbc_type - 1][1];
+    double vn = sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+    if (!(vn > 0.0)) return -1;
+    for (int a = 0; a < 3; a++) buoy[a] = tab[rbc_type - 1][2] * g[a] / vn;
+    return 0;
+}

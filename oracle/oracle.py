@@ -114,6 +114,7 @@ def lib():
         L.or_nse_time.argtypes = [C.c_void_p]
         L.or_nse_time.restype = C.c_double
         L.or_nse_cfl.restype = C.c_double
+        L.or_nse_physics.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, dp, dp, dp, dp]
         _lib = L
     return _lib
 
@@ -412,3 +413,15 @@ class Nse:
     def thermal_energy(self):
         """tseries.cc:410-446: volume mean of 1/2 (T + z - 1)^2."""
         return lib().or_nse_thermal_energy(self.h)
+
+
+def nse_physics(prob_type, re=1.0, ra=1.0, pr=1.0, rbc_type=2, g=(0.0, 0.0, -1.0)):
+    """nu, kappa and buoyancy vector by the deck (timestep.cc:70-98,
+    buoyantForce.cc:48-60); None where the reference aborts."""
+    nu, kappa = C.c_double(), C.c_double()
+    buoy = np.zeros(3)
+    gv = np.asarray(g, dtype=np.float64)
+    r = lib().or_nse_physics(prob_type, re, ra, pr, rbc_type, _dp(gv), C.byref(nu), C.byref(kappa), _dp(buoy))
+    if r != 0:
+        return None
+    return dict(nu=nu.value, kappa=kappa.value, buoy=tuple(buoy))

@@ -202,6 +202,11 @@ double or_nse_time(const or_nse *s);
 /* Threads the oracle's OpenMP loops use (0 = runtime default). */
 void or_set_threads(int n);
 
+/* nu, kappa and gTerms of a run by the deck (timestep.cc:70-98,
+ * buoyantForce.cc:48-60); -1 where the reference aborts. */
+int or_nse_physics(int prob_type, double re, double ra, double pr, int rbc_type, const double g[3], double *nu,
+                   double *kappa, double buoy[3]);
+
 #ifdef __cplusplus
 }
 #endif

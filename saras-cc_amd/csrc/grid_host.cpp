@@ -175,3 +175,54 @@ bool build_levels(const saras_grid_desc &g, GridLevels &out) {
 }
 
 }  // namespace saras
+
+extern "C" {
+
+// timestep::timestep (timestep.cc:70-98): nu = 1/Re for hydrodynamics
+// (probType <= 4); for the scalar problems (probType 5..9) the pair picked by
+// rbcType.  buoyantForce::buoyantForce (buoyantForce.cc:48-60): Fb by
+// rbcType, gTerms = Fb g / |g|.
+saras_status saras_nse_physics(int prob_type, double re, double ra, double pr, int rbc_type,
+                               const double g_axis[3], double *nu, double *kappa, double buoy[3]) {
+    if (!nu || !kappa || !buoy) return SARAS_BAD_ARG;
+    if (prob_type < 1 || prob_type > 9) return SARAS_BAD_ARG;
+    if (prob_type <= 4) {
+        if (!(re > 0.0)) return SARAS_BAD_ARG;
+        *nu = 1.0 / re;
+        *kappa = 0.0;     // no scalar field: the reference leaves kappa unset
+        buoy[0] = buoy[1] = buoy[2] = 0.0;
+        return SARAS_OK;
+    }
+    double Fb = 0.0;
+    switch (rbc_type) {
+        case 1:
+            *nu = pr;
+            *kappa = 1.0;
+            Fb = ra * pr;
+            break;
+        case 2:
+            *nu = std::sqrt(pr / ra);
+            *kappa = 1.0 / std::sqrt(pr * ra);
+            Fb = 1.0;
+            break;
+        case 3:
+            *nu = 1.0;
+            *kappa = 1.0 / pr;
+            Fb = ra;
+            break;
+        case 4:
+            *nu = std::sqrt(pr / ra);
+            *kappa = 1.0 / std::sqrt(pr * ra);
+            Fb = pr;
+            break;
+        default:
+            return SARAS_BAD_ARG;   // the reference prints an error and exits
+    }
+    if (!g_axis) return SARAS_BAD_ARG;
+    const double vnorm = std::sqrt(g_axis[0] * g_axis[0] + g_axis[1] * g_axis[1] + g_axis[2] * g_axis[2]);
+    if (!(vnorm > 0.0)) return SARAS_BAD_ARG;
+    for (int a = 0; a < 3; ++a) buoy[a] = Fb * g_axis[a] / vnorm;
+    return SARAS_OK;
+}
+
+}  // extern "C"

@@ -398,6 +398,12 @@ struct CoarseCtl {
     int max_count;     // poisson.cc:431-436
     double tol;        // mgTolerance
     int *iters_out;    // iterations (mode 0)
+    // FP32 correction cycles of the mixed solver: the solve also stops once
+    // max|r| < rel_floor * max|b| of the level, the FP32 accuracy floor of
+    // the current correction equation (an absolute mgTolerance below it would
+    // only run the solve to the iteration cap); 0 in FP64, where mgTolerance
+    // alone applies, as in the reference
+    double rel_floor;
 };
 
 // Phases of the ghost-array loops below run either on the whole workgroup
@@ -548,6 +554,12 @@ __global__ void __launch_bounds__(1024) k_coarse(LevelArgs<T> L, T *__restrict__
 
     int iter = 0;
     int sweeps = 0;
+    double tol = ctl.tol;
+    if (ctl.mode == 0 && ctl.rel_floor > 0.0) {
+        double bm = 0.0;
+        for (long t = threadIdx.x; t < total; t += blockDim.x) bm = fmax(bm, fabs((double)B[t]));
+        tol = fmax(tol, ctl.rel_floor * group_max<false>(bm, red));
+    }
     for (;;) {
         if (ctl.mode == 1 && sweeps >= ctl.sweeps) break;
         // imposeBC: updateFace then walls
@@ -579,7 +591,7 @@ __global__ void __launch_bounds__(1024) k_coarse(LevelArgs<T> L, T *__restrict__
             if (a > lm) lm = a;
         }
         const double mx = group_max<false>(lm, red);
-        if (mx < ctl.tol) break;
+        if (mx < tol) break;
         iter += 1;
         if (iter > ctl.max_count) break;
     }

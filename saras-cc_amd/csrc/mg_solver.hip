@@ -180,7 +180,9 @@ struct Hierarchy {
         const size_t padded = dim == 3 ? (size_t)(C.n[0] + 2) * (C.n[1] + 2) * (C.n[2] + 2)
                                        : (size_t)(C.n[0] + 2) * (C.n[2] + 2);
         const size_t need = (padded + (size_t)C.n[0] * C.n[1] * C.n[2]) * sizeof(T);
-        coarse_lds = need <= 48 * 1024;
+        // a workgroup may hold up to 64 KB of dynamic LDS without opting in
+        // (k_coarse's static LDS is 128 bytes): 14^3 in FP64 fits
+        coarse_lds = need <= 64 * 1024 - 256;
         coarse_lds_bytes = coarse_lds ? need : 0;
         if (!coarse_lds) HIP_TRY(hipMalloc((void **)&coarse_scratch, need));
         return SARAS_OK;
@@ -288,8 +290,23 @@ struct Ops {
 
     int pend_level = -1;  // level whose next sweep also applies the prolongation
     int skip_red = -1;    // level whose next sweep's red pass the restriction already applied
+    // a host-side guard that refused a launch: the caller returns this status
+    // (with the message in saras_last_error) instead of the process aborting
+    saras_status err = SARAS_OK;
+
+    bool refuse(const char *what) {
+        if (err == SARAS_OK) set_error(std::string("internal error: ") + what);
+        err = SARAS_HIP_ERROR;
+        return false;
+    }
 
     bool inhom(int l, bool zero_bc) const { return inhom0 && l == 0 && !zero_bc; }
+
+    // FP32 cycles solve the coarsest level to max(mgTolerance, kCoarseFp32Rel
+    // max|b| of that level): the FP32 residual of a level of up to a few
+    // thousand cells stalls near 1e-5 of its right-hand side
+    static constexpr double kCoarseFp32Rel = 1e-4;
+    double coarse_rel_floor() const { return sizeof(T) == 4 ? kCoarseFp32Rel : 0.0; }
 
     // one RB-SOR sweep at level l: the fused out-of-place march kernel where
     // the level tiles exactly, else the two in-place colour passes
@@ -416,7 +433,7 @@ struct Ops {
     void coarse(int l, bool zero_bc, int mode, int sweeps) {
         DevLevel<T> &L = H.lv[l];
         auto a = level_args(m, H, l, inhom(l, zero_bc));
-        saras::CoarseCtl ctl{mode, sweeps, m->max_count, m->p.mg_tol, m->iters_dev};
+        saras::CoarseCtl ctl{mode, sweeps, m->max_count, m->p.mg_tol, m->iters_dev, coarse_rel_floor()};
         const long cells = (long)L.n[0] * L.n[1] * L.n[2];
         int threads = 64;  // power of two: the kernel's tree reduction relies on it
         while (threads < cells && threads < 1024) threads <<= 1;
@@ -441,7 +458,11 @@ struct Ops {
     // red_next: the restriction also applies the coarse level's first
     // pre-smoothing red pass (split levels; see k_rr_split)
     void resid_restrict(int l, bool red_next = false) {
-        if (l < 0 || l + 1 >= (int)H.lv.size()) std::abort();  // host-side guard: never launch past the hierarchy
+        // host-side guard: never launch past the hierarchy
+        if (l < 0 || l + 1 >= (int)H.lv.size()) {
+            refuse("restriction from a level outside the hierarchy");
+            return;
+        }
         DevLevel<T> &F = H.lv[l], &C = H.lv[l + 1];
         auto fa = level_args(m, H, l, inhom(l, false));
         auto ca = level_args(m, H, l + 1, false);
@@ -499,7 +520,11 @@ struct Ops {
     }
 
     void prolong_correct(int l) {
-        if (l < 0 || l + 1 >= (int)H.lv.size()) std::abort();  // host-side guard: never launch past the hierarchy
+        // host-side guard: never launch past the hierarchy
+        if (l < 0 || l + 1 >= (int)H.lv.size()) {
+            refuse("prolongation onto a level outside the hierarchy");
+            return;
+        }
         DevLevel<T> &F = H.lv[l], &C = H.lv[l + 1];
         const long n = (long)F.n[0] * F.n[1] * F.n[2];
         if (F.hz) {
@@ -572,10 +597,15 @@ struct Ops {
         A.max_count = m->max_count;
         A.w = w;
         A.tol = m->p.mg_tol;
+        A.rel_floor = coarse_rel_floor();
         A.zero_bc_first = S != 0;
         A.wave_cells = m->tail_wave_cells;
         A.iters_out = m->iters_dev;
-        if (off * sizeof(T) > m->tail_lds) std::abort();  // host-side guard: tail_start() keeps levels within tail_lds
+        // host-side guard: tail_start() keeps the levels within tail_lds
+        if (off * sizeof(T) > m->tail_lds) {
+            refuse("tail levels exceed the LDS budget");
+            return;
+        }
         if (H.dim == 3)
             hipLaunchKernelGGL((saras::k_tail<T, 3>), dim3(1), dim3(512), off * sizeof(T), s, A);
         else
@@ -615,7 +645,7 @@ template <typename T>
 saras_status run_vcycle(saras_mg *m, Ops<T> &ops) {
     if (!m->graphs) {
         ops.vcycle();
-        return SARAS_OK;
+        return ops.err;
     }
     auto &cache = m->graph_cache[sizeof(T) == 4 ? 1 : 0];
     std::vector<void *> key;
@@ -638,6 +668,10 @@ saras_status run_vcycle(saras_mg *m, Ops<T> &ops) {
     HIP_TRY(hipStreamBeginCapture(ops.s, hipStreamCaptureModeThreadLocal));
     ops.vcycle();
     hipError_t ce = hipStreamEndCapture(ops.s, &g);
+    if (ops.err) {
+        if (g) (void)hipGraphDestroy(g);
+        return ops.err;
+    }
     if (ce != hipSuccess) {
         set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
         return SARAS_HIP_ERROR;
@@ -1296,6 +1330,7 @@ saras_status saras_mg_smooth(saras_mg *mg, int level, int count, int zero_bc) {
     if (!mg || level < 0 || level > mg->depth || count < 0) return SARAS_BAD_ARG;
     Ops<double> ops{mg, mg->h64, mg->stream, true, mg->p.sor};
     ops.smooth(level, count, zero_bc != 0);
+    if (ops.err) return ops.err;
     HIP_TRY(hipStreamSynchronize(mg->stream));
     HIP_TRY(hipGetLastError());
     return SARAS_OK;
@@ -1309,6 +1344,7 @@ saras_status saras_mg_coarse_solve(saras_mg *mg, int level, int zero_bc, int *it
     }
     Ops<double> ops{mg, mg->h64, mg->stream, true, mg->p.sor};
     ops.coarse(level, zero_bc != 0, 0, 0);
+    if (ops.err) return ops.err;
     HIP_TRY(hipMemcpyAsync(mg->iters_host, mg->iters_dev, sizeof(int), hipMemcpyDeviceToHost, mg->stream));
     HIP_TRY(hipStreamSynchronize(mg->stream));
     if (iters) *iters = *mg->iters_host;
@@ -1319,6 +1355,7 @@ saras_status saras_mg_residual_restrict(saras_mg *mg, int level) {
     if (!mg || level < 0 || level >= mg->depth) return SARAS_BAD_ARG;
     Ops<double> ops{mg, mg->h64, mg->stream, true, mg->p.sor};
     ops.resid_restrict(level);
+    if (ops.err) return ops.err;
     HIP_TRY(hipStreamSynchronize(mg->stream));
     return SARAS_OK;
 }
@@ -1327,6 +1364,7 @@ saras_status saras_mg_prolong_correct(saras_mg *mg, int level) {
     if (!mg || level < 0 || level >= mg->depth) return SARAS_BAD_ARG;
     Ops<double> ops{mg, mg->h64, mg->stream, true, mg->p.sor};
     ops.prolong_correct(level);
+    if (ops.err) return ops.err;
     HIP_TRY(hipStreamSynchronize(mg->stream));
     return SARAS_OK;
 }
@@ -1437,6 +1475,11 @@ saras_status saras_memcpy_d2h(void *dst, const void *src, size_t bytes) {
 }
 saras_status saras_device_sync(void) {
     HIP_TRY(hipDeviceSynchronize());
+    return SARAS_OK;
+}
+saras_status saras_device_zero(void *ptr, size_t bytes) {
+    if (!ptr) return SARAS_BAD_ARG;
+    HIP_TRY(hipMemset(ptr, 0, bytes));
     return SARAS_OK;
 }
 

@@ -34,6 +34,7 @@ struct TailArgs {
     int pre, post, solve_flag, max_count;
     T w;
     double tol;
+    double rel_floor;                    // as CoarseCtl::rel_floor
     int zero_bc_first;                   // 0 only when S is the finest level
     int wave_cells;                      // levels of at most this many padded cells run on wave 0 alone
     int *iters_out;
@@ -180,7 +181,8 @@ __device__ __forceinline__ void tghost_apply(T *X, const TGhost<T> &g, bool wall
 // sweeps, then the final imposeBC; with SOLVE, sweeps until max|r| < tol or
 // the iteration cap (poisson_d3.cc:249-432), returning the iteration count.
 template <typename T, int DIM, bool WAVE, bool SOLVE>
-__device__ int tail_level(T *lds, const TailLevel<T> &V, int count, T w, double tol, int max_count, double *red) {
+__device__ int tail_level(T *lds, const TailLevel<T> &V, int count, T w, double tol, double rel_floor, int max_count,
+                          double *red) {
     const int tid = threadIdx.x, nthr = lthreads<WAVE>();
     const int total = V.n[0] * V.n[1] * V.n[2];
     const int nface = DIM == 3 ? V.n[1] * V.n[2] + V.n[0] * V.n[2] + V.n[0] * V.n[1] : V.n[2] + V.n[0];
@@ -234,6 +236,12 @@ __device__ int tail_level(T *lds, const TailLevel<T> &V, int count, T w, double 
 
     int iter = 0;
     if (SOLVE) {
+        if (rel_floor > 0.0) {
+            // the FP32 floor of the correction equation (CoarseCtl::rel_floor)
+            double bm = 0.0;
+            for (int t = tid; t < total; t += nthr) bm = fmax(bm, fabs((double)lds[V.b + t]));
+            tol = fmax(tol, rel_floor * group_max<WAVE>(bm, red));
+        }
         for (;;) {
             sweep();
             // max |r| with the ghosts as they stand (poisson_d3.cc:398-415)
@@ -366,13 +374,13 @@ __global__ void __launch_bounds__(512) k_tail(TailArgs<T> A) {
         int it = 0;
         if (padded(V) <= A.wave_cells) {
             if (threadIdx.x < 64) {
-                if (solve) it = tail_level<T, DIM, true, true>(lds, V, 0, A.w, A.tol, A.max_count, red);
-                else tail_level<T, DIM, true, false>(lds, V, count, A.w, A.tol, A.max_count, red);
+                if (solve) it = tail_level<T, DIM, true, true>(lds, V, 0, A.w, A.tol, A.rel_floor, A.max_count, red);
+                else tail_level<T, DIM, true, false>(lds, V, count, A.w, A.tol, A.rel_floor, A.max_count, red);
             }
             __syncthreads();
         } else {
-            if (solve) it = tail_level<T, DIM, false, true>(lds, V, 0, A.w, A.tol, A.max_count, red);
-            else tail_level<T, DIM, false, false>(lds, V, count, A.w, A.tol, A.max_count, red);
+            if (solve) it = tail_level<T, DIM, false, true>(lds, V, 0, A.w, A.tol, A.rel_floor, A.max_count, red);
+            else tail_level<T, DIM, false, false>(lds, V, count, A.w, A.tol, A.rel_floor, A.max_count, red);
         }
         return it;
     };

@@ -88,6 +88,7 @@ struct saras_server {
     std::condition_variable cv_work, cv_done;
     std::deque<Pending> queue;
     std::list<Idle> idle;                  // front = most recently used
+    std::vector<saras_mg *> evicted;       // taken off `idle`, destroyed by a worker outside the lock
     std::vector<saras_completion> done;
     std::vector<std::thread> workers;
     size_t mem_used = 0;                   // idle handles + running solves
@@ -118,15 +119,20 @@ struct saras_server {
                     queue.erase(it);
                     return true;
                 }
-            size_t need = it->bytes;
-            // evict idle hierarchies, least recently used first, until it fits
-            while (mem_used + need > p.mem_budget && !idle.empty()) {
-                Idle v = idle.back();
-                idle.pop_back();
-                saras_mg_destroy(v.h);
-                mem_used -= v.bytes;
-            }
-            if (mem_used + need <= p.mem_budget || (running == 0 && idle.empty())) {
+            const size_t need = it->bytes;
+            size_t idle_bytes = 0;
+            for (const Idle &v : idle) idle_bytes += v.bytes;
+            // Admit when it fits now, when evicting idle hierarchies makes it
+            // fit, or when nothing runs (a request larger than the budget then
+            // runs alone).  Eviction happens only for an admission, so memory
+            // held by running solves never empties the cache for nothing.
+            if (mem_used + need <= p.mem_budget || mem_used - idle_bytes + need <= p.mem_budget || running == 0) {
+                // evict least recently used first, until it fits
+                while (mem_used + need > p.mem_budget && !idle.empty()) {
+                    evicted.push_back(idle.back().h);
+                    mem_used -= idle.back().bytes;
+                    idle.pop_back();
+                }
                 mem_used += need;
                 h = nullptr;
                 out = *it;
@@ -144,6 +150,7 @@ struct saras_server {
         for (;;) {
             Pending job;
             saras_mg *h = nullptr;
+            std::vector<saras_mg *> victims;
             {
                 std::unique_lock<std::mutex> lk(mu);
                 bool got = false;
@@ -152,9 +159,16 @@ struct saras_server {
                     got = !queue.empty() && take(job, h);
                     return got;
                 });
-                if (!got) return;
+                victims.swap(evicted);
+                if (!got) {
+                    lk.unlock();
+                    for (saras_mg *v : victims) saras_mg_destroy(v);
+                    return;
+                }
                 ++running;
             }
+            // hipFree synchronises the device: never under the server lock
+            for (saras_mg *v : victims) saras_mg_destroy(v);
             saras_completion c{};
             c.tag = job.r.tag;
             c.t_submit = job.t_submit;
@@ -169,18 +183,20 @@ struct saras_server {
             c.t_done = now();
             if (st != SARAS_OK && st != SARAS_NOT_CONVERGED)
                 std::strncpy(c.message, saras_last_error(), sizeof(c.message) - 1);
+            saras_mg *drop = nullptr;
             {
                 std::lock_guard<std::mutex> lk(mu);
                 --running;
                 if (h && p.keep_resident && (st == SARAS_OK || st == SARAS_NOT_CONVERGED)) {
                     idle.push_front({job.key, h, job.bytes});
                 } else {
-                    if (h) saras_mg_destroy(h);
+                    drop = h;
                     mem_used -= job.bytes;
                 }
                 done.push_back(c);
                 ++finished;
             }
+            if (drop) saras_mg_destroy(drop);
             cv_done.notify_all();
             cv_work.notify_all();
         }
@@ -253,6 +269,7 @@ saras_status saras_server_destroy(saras_server *s) {
     s->cv_work.notify_all();
     for (auto &t : s->workers) t.join();
     for (auto &v : s->idle) saras_mg_destroy(v.h);
+    for (saras_mg *v : s->evicted) saras_mg_destroy(v);
     delete s;
     return SARAS_OK;
 }

@@ -87,8 +87,8 @@ EXPORTS = [
     "saras_mg_get_level", "saras_mg_smooth", "saras_mg_coarse_solve",
     "saras_mg_residual_restrict", "saras_mg_prolong_correct", "saras_mg_error",
     "saras_device_alloc", "saras_device_free", "saras_memcpy_h2d", "saras_memcpy_d2h",
-    "saras_device_sync", "saras_last_error", "saras_mg_bench_cycles", "saras_mg_bench_sweeps",
-    "saras_bench_stream_copy", "saras_nse_create", "saras_nse_destroy", "saras_nse_set_fields",
+    "saras_device_sync", "saras_device_zero", "saras_last_error", "saras_mg_bench_cycles", "saras_mg_bench_sweeps",
+    "saras_bench_stream_copy", "saras_nse_create", "saras_nse_physics", "saras_nse_destroy", "saras_nse_set_fields",
     "saras_nse_get_fields", "saras_nse_step", "saras_nse_stage", "saras_nse_rhs", "saras_nse_get_rhs",
     "saras_nse_solve_field", "saras_field_op", "saras_nse_diag", "saras_nse_bench_steps",
     "saras_nse_cfl", "saras_nse_sgs_diag", "saras_nse_tsdata", "saras_nse_probe", "saras_mg_set_graphs", "saras_mg_footprint", "saras_server_create", "saras_server_submit",
@@ -129,6 +129,8 @@ def lib():
         L.saras_device_free.argtypes = [vp]
         L.saras_memcpy_h2d.argtypes = [vp, vp, C.c_size_t]
         L.saras_memcpy_d2h.argtypes = [vp, vp, C.c_size_t]
+        L.saras_device_zero.argtypes = [vp, C.c_size_t]
+        L.saras_nse_physics.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, dp, dp, dp, dp]
         L.saras_nse_create.argtypes = [C.POINTER(GridDesc), C.POINTER(MgParams), C.POINTER(NseParams),
                                        C.POINTER(vp)]
         L.saras_nse_destroy.argtypes = [vp]
@@ -168,9 +170,23 @@ def check(status, allow=(OK,)):
     return status
 
 
+def device_sync():
+    check(lib().saras_device_sync())
+
+
 def _dp(a):
     assert a.dtype == np.float64 and a.flags.c_contiguous
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def nse_physics(prob_type, re=1.0, ra=1.0, pr=1.0, rbc_type=2, g=(0.0, 0.0, -1.0)):
+    """nu, kappa and the buoyancy vector gTerms from the deck (timestep.cc:70-98,
+    buoyantForce.cc:48-60), for nse_params(nu=, kappa=, buoy=)."""
+    nu, kappa = C.c_double(), C.c_double()
+    buoy = np.zeros(3)
+    gv = np.ascontiguousarray(g, dtype=np.float64)
+    check(lib().saras_nse_physics(prob_type, re, ra, pr, rbc_type, _dp(gv), C.byref(nu), C.byref(kappa), _dp(buoy)))
+    return dict(nu=nu.value, kappa=kappa.value, buoy=tuple(buoy))
 
 
 def grid_desc(n, dim=3, length=(1.0, 1.0, 1.0), periodic=(0, 0, 0), gtype=(0, 0, 0),
@@ -228,6 +244,9 @@ class DeviceArray:
         assert out.nbytes == self.nbytes
         check(lib().saras_memcpy_d2h(out.ctypes.data, self.ptr, self.nbytes))
         return out
+
+    def zero(self):
+        check(lib().saras_device_zero(self.ptr, self.nbytes))
 
     def free(self):
         if self.ptr:

@@ -3,11 +3,19 @@ serving workload and an open/closed-loop driver for saras_server_*.
 
 Request mix: 75 % 3-D cubes with n drawn log-uniformly in [32, 256], 25 % 2-D
 squares with n log-uniform in [64, 2048], each rounded to the nearest
-multiple of 16; boundary conditions uniformly Dirichlet (homogeneous, every
-face), Neumann (every face) or periodic; RHS = 8 seeded Fourier modes
-(sine modes for Dirichlet, cosine modes for Neumann, zero-mean complex
-modes for periodic) with amplitudes ~ U(0.5, 1) |k|^-2; tolerance 1e-8
-(relative L-inf residual).
+multigrid-friendly extent (below); boundary conditions uniformly Dirichlet
+(homogeneous, every face), Neumann (every face) or periodic; RHS = 8 seeded
+Fourier modes (sine modes for Dirichlet, cosine modes for Neumann, zero-mean
+complex modes for periodic) with amplitudes ~ U(0.5, 1) |k|^-2; tolerance
+1e-8 (relative L-inf residual).
+
+Multigrid-friendly extents: multiples of 16 whose quotient by 16 has an odd
+part of at most 7 (32, 48, 64, 80, 96, 112, 128, 160, 192, 224, 256, 320, ...).
+The reference's depth rule (grid::setVCDepth, grid.cc:678-742) halves an axis
+while it stays even, so such an extent coarsens to at most 7 cells (14 with
+walls on every face, where the rule stops one level earlier).  A multiple of
+16 with a larger odd part (208 = 13 * 16, 2000 = 125 * 16) leaves a coarsest
+level of up to 250 cells per axis that no V-cycle budget solves.
 """
 import math
 import time
@@ -19,14 +27,30 @@ from . import (BC_DIRICHLET, BC_NEUMANN, PREC_FP64, PREC_MIXED, Server, grid_des
 KINDS = ("dirichlet", "neumann", "periodic")
 
 
+def odd_part(m):
+    while m % 2 == 0:
+        m //= 2
+    return m
+
+
+def mg_extents(lo, hi):
+    """Multigrid-friendly extents in [lo, hi]: multiples of 16 whose quotient
+    by 16 has an odd part of at most 7."""
+    return [m for m in range(16 * ((lo + 15) // 16), hi + 1, 16) if odd_part(m // 16) <= 7]
+
+
+def nearest_extent(v, lo, hi):
+    """The multigrid-friendly extent nearest to v (the smaller one on a tie)."""
+    return min(mg_extents(lo, hi), key=lambda m: (abs(m - v), m))
+
+
 def draw_requests(count, seed=4, frac3d=0.75, n3=(32, 256), n2=(64, 2048)):
     rng = np.random.default_rng(seed)
     out = []
     for t in range(count):
         d3 = rng.random() < frac3d
         lo, hi = n3 if d3 else n2
-        n = int(round(math.exp(rng.uniform(math.log(lo), math.log(hi))) / 16.0)) * 16
-        n = min(max(n, lo), hi)
+        n = nearest_extent(math.exp(rng.uniform(math.log(lo), math.log(hi))), lo, hi)
         kind = KINDS[int(rng.integers(0, 3))]
         out.append(dict(tag=t, dim=3 if d3 else 2, n=n, kind=kind, rhs_seed=int(rng.integers(0, 2**31))))
     return out
@@ -41,17 +65,17 @@ def request_grid(req):
 
 
 def request_params(req, precision, tol=1e-8):
-    """V(2,2) with the coarsest level smoothed, not solved ("Solve Coarsest"
-    off), at most 100 V-cycles.  Sizes with few factors of two keep a large
-    coarsest level under the reference's depth rule: solving it iteratively
-    costs up to 2.5 N_coarse sweeps per cycle, while smoothing it converges
-    as fast on this mix (35 of 40 sampled requests either way).  A request
-    still above tolerance after 100 cycles completes as NOT_CONVERGED, as the
-    reference's mgSolve returns silently.  Mixed precision: up to 10 FP32
-    cycles per refinement (slowly converging requests refine every 10 cycles;
-    the others stop at their tolerance by the measured convergence factor)."""
+    """V(2,2) with "Solve Coarsest" on (the reference deck's default) to an
+    absolute coarse tolerance of 1e-10, at most 100 V-cycles.  On the
+    multigrid-friendly extents every request then converges in about 8-11
+    cycles; smoothing the coarsest level instead takes up to 80 cycles where
+    it keeps 10-14 cells per axis.  A request still above tolerance after 100
+    cycles would complete as NOT_CONVERGED, as the reference's mgSolve returns
+    silently.  Mixed precision: up to 10 FP32 cycles per refinement (the
+    solver stops each refinement at its tolerance by the measured convergence
+    factor)."""
     bc = BC_DIRICHLET if req["kind"] == "dirichlet" else BC_NEUMANN
-    return mg_params(vc_count=100, vc_tol=tol, solve_flag=0, mg_tol=1e-2, pre=2, post=2, res_type=0, bc=bc,
+    return mg_params(vc_count=100, vc_tol=tol, solve_flag=1, mg_tol=1e-10, pre=2, post=2, res_type=0, bc=bc,
                      precision=precision, inner_cycles=10, max_refine=100)
 
 
@@ -141,6 +165,8 @@ def run_stream(server, reqs, rhs, precision, rate=None, collect=None):
 
 
 def summarize(recs):
+    """solves_per_s counts converged solves only (a NOT_CONVERGED completion is
+    not a solve); requests_per_s is the raw completion rate beside it."""
     lat = np.array([r["t_done"] - r["t_submit"] for r in recs.values()])
     t_first = min(r["t_submit"] for r in recs.values())
     t_last = max(r["t_done"] for r in recs.values())
@@ -148,6 +174,7 @@ def summarize(recs):
     failed = [r for r in recs.values() if r["status"] not in (0, 1)]
     if failed:
         raise RuntimeError(f"{len(failed)} requests failed, first: {failed[0]}")
-    return dict(requests=len(recs), converged=ok, seconds=t_last - t_first,
-                solves_per_s=len(recs) / (t_last - t_first), p50_ms=float(np.percentile(lat, 50) * 1e3),
+    span = t_last - t_first
+    return dict(requests=len(recs), converged=ok, seconds=span, solves_per_s=ok / span,
+                requests_per_s=len(recs) / span, p50_ms=float(np.percentile(lat, 50) * 1e3),
                 p99_ms=float(np.percentile(lat, 99) * 1e3), mean_ms=float(lat.mean() * 1e3))

@@ -74,3 +74,43 @@ def test_dropin_program_links_against_the_library():
     assert os.path.exists(exe), "run __graft_entry__.build() first"
     r = subprocess.run([exe], capture_output=True, text=True)
     assert r.returncode == 2 and "usage" in r.stderr
+
+
+def test_nse_physics_matches_the_deck_mapping():
+    """nu, kappa and the buoyancy vector by problem and rbcType
+    (timestep.cc:70-98, buoyantForce.cc:48-60): the library's host code and
+    the oracle's table agree bit for bit, and both meet the closed forms."""
+    g = (0.0, 0.0, -1.0)
+    for rbc in (1, 2, 3, 4):
+        for ra, pr in ((1e7, 1.0), (1e5, 0.7), (2.5e6, 7.0)):
+            s = S.nse_physics(5, ra=ra, pr=pr, rbc_type=rbc, g=g)
+            o = O.nse_physics(5, ra=ra, pr=pr, rbc_type=rbc, g=g)
+            assert s == o, (rbc, ra, pr, s, o)
+            fb = {1: ra * pr, 2: 1.0, 3: ra, 4: pr}[rbc]
+            assert s["buoy"] == (0.0, 0.0, -fb)
+            if rbc == 1:
+                assert (s["nu"], s["kappa"]) == (pr, 1.0)
+            elif rbc == 3:
+                assert (s["nu"], s["kappa"]) == (1.0, 1.0 / pr)
+            else:
+                # nu = sqrt(Pr/Ra), kappa = 1/sqrt(Pr Ra): nu/kappa = Pr, nu kappa = 1/Ra
+                assert abs(s["nu"] / s["kappa"] - pr) <= 1e-15 * pr
+                assert abs(s["nu"] * s["kappa"] * ra - 1.0) <= 1e-15
+    # the Rayleigh-Benard deck (Ra 1e7, Pr 1, rbcType 2): nu = kappa = 1/sqrt(1e7)
+    rb = S.nse_physics(5, ra=1e7, pr=1.0, rbc_type=2)
+    assert rb["nu"] == np.sqrt(1.0 / 1e7) and rb["kappa"] == 1.0 / np.sqrt(1e7)
+    # a tilted gravity axis is normalised (gTerms = Fb g/|g|)
+    t = S.nse_physics(6, ra=1e6, pr=2.0, rbc_type=1, g=(3.0, 0.0, -4.0))
+    assert t == O.nse_physics(6, ra=1e6, pr=2.0, rbc_type=1, g=(3.0, 0.0, -4.0))
+    assert np.allclose(t["buoy"], (2e6 * 0.6, 0.0, -2e6 * 0.8), rtol=1e-15)
+    # hydrodynamics: nu = 1/Re, no scalar, no buoyancy
+    h = S.nse_physics(2, re=1600.0)
+    assert h == dict(nu=1.0 / 1600.0, kappa=0.0, buoy=(0.0, 0.0, 0.0)) == O.nse_physics(2, re=1600.0)
+    # where the reference aborts (rbcType outside 1..4) the library refuses
+    for bad in (dict(prob_type=5, rbc_type=0), dict(prob_type=5, rbc_type=5), dict(prob_type=10),
+                dict(prob_type=5, rbc_type=2, g=(0.0, 0.0, 0.0))):
+        kw = dict(ra=1e7, pr=1.0)
+        kw.update(bad)
+        with pytest.raises(S.SarasError):
+            S.nse_physics(**kw)
+        assert O.nse_physics(**kw) is None

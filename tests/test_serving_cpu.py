@@ -21,6 +21,23 @@ def test_request_mix_matches_the_workload():
     assert V.draw_requests(50, seed=4) == V.draw_requests(50, seed=4)
 
 
+def test_mix_extents_coarsen_to_small_levels():
+    """Every drawn extent is a multigrid-friendly one: a multiple of 16 whose
+    odd part of n/16 is at most 7, so the reference's depth rule
+    (grid.cc:678-742) leaves a coarsest level of at most 7 cells per axis
+    (14 with walls on every face); the extents are the nearest such values."""
+    assert V.mg_extents(32, 256) == [32, 48, 64, 80, 96, 112, 128, 160, 192, 224, 256]
+    assert V.nearest_extent(208, 32, 256) == 192 and V.nearest_extent(209, 32, 256) == 224
+    assert V.nearest_extent(2000, 64, 2048) == 2048 and V.nearest_extent(33, 32, 256) == 32
+    for r in V.draw_requests(600, seed=4):
+        g = V.request_grid(r)
+        depth = S.vc_depth(g)
+        assert depth >= 2, r
+        coarse = r["n"] >> depth
+        assert coarse << depth == r["n"]
+        assert coarse <= (7 if r["kind"] == "periodic" else 14), (r, coarse)
+
+
 def test_request_rhs_compatibility():
     for kind in V.KINDS:
         for dim in (2, 3):

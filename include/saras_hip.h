@@ -316,6 +316,20 @@ saras_status saras_nse_tsdata(saras_nse *s, double out[5]);
  * core. */
 saras_status saras_nse_probe(saras_nse *s, int count, const int *ijk, double *out);
 
+/* The same values as saras_nse_probe for at most SARAS_PROBE_NOALLOC_MAX
+ * points, with no device allocation: no counterpart in the reference.  It is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight time loops, harnesses that forbid hipMalloc); saras_nse_probe
+ * allocates and frees its staging on every call.  The point indices and
+ * values are staged in a fixed device array of the library (about 416 KB per
+ * GPU, held until the process ends).  More points: SARAS_BAD_ARG.  Every call
+ * on one GPU shares that array; the library serialises its calls in one
+ * process with a lock, and a call returns only after its values are copied
+ * out, so the array's previous contents never matter.  Calls that overlapped
+ * on one GPU without that lock would race. */
+#define SARAS_PROBE_NOALLOC_MAX 8192
+saras_status saras_nse_probe_noalloc(saras_nse *s, int count, const int *ijk, double *out);
+
 /* Measurement hook: `count` full steps with no host work between them
  * beyond the reference's own convergence checks; *ms = device time. */
 saras_status saras_nse_bench_steps(saras_nse *s, int count, double *ms);

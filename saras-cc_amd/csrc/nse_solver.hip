@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -918,6 +919,82 @@ saras_status saras_nse_probe(saras_nse *s, int count, const int *ijk, double *ou
     (void)hipFree(dij);
     (void)hipFree(dout);
     return st;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Probe staging of saras_nse_probe_noalloc: the point indices (3 ints per
+// point) followed by the values (5 doubles per point) of at most
+// SARAS_PROBE_NOALLOC_MAX points.  Each GPU holds one copy for the life of
+// the process.  Host code never passes `scratch` itself to a kernel or a copy
+// (on the host the name is only a stand-in): it uses the device address that
+// hipGetSymbolAddress returns for the current GPU.
+constexpr size_t kProbeIdxBytes = (size_t)SARAS_PROBE_NOALLOC_MAX * 3 * sizeof(int);
+constexpr size_t kProbeScratchBytes = kProbeIdxBytes + (size_t)SARAS_PROBE_NOALLOC_MAX * 5 * sizeof(double);
+static_assert(kProbeIdxBytes % 256 == 0, "the value block starts 256-byte aligned");
+
+static __device__ __align__(256) unsigned char scratch[kProbeScratchBytes];
+
+constexpr int kMaxDevices = 64;
+
+struct ProbeScratch {
+    std::mutex mu;                     // one call at a time per GPU: calls share the GPU's array
+    unsigned char *ptr = nullptr;      // device address of `scratch` on this GPU
+    bool ready = false;
+};
+
+ProbeScratch &probe_scratch(int dev) {
+    static ProbeScratch per_dev[kMaxDevices];
+    return per_dev[dev];
+}
+
+}  // namespace
+
+extern "C" {
+
+saras_status saras_nse_probe_noalloc(saras_nse *s, int count, const int *ijk, double *out) {
+    if (!s || count < 0 || (count > 0 && (!ijk || !out))) return SARAS_BAD_ARG;
+    if (count > SARAS_PROBE_NOALLOC_MAX) {
+        saras::set_last_error("saras_nse_probe_noalloc: " + std::to_string(count) + " points exceed the limit of " +
+                              std::to_string(SARAS_PROBE_NOALLOC_MAX));
+        return SARAS_BAD_ARG;
+    }
+    if (count == 0) return SARAS_OK;
+    for (int q = 0; q < count; ++q)
+        for (int a = 0; a < 3; ++a)
+            if (ijk[3 * q + a] < 0 || ijk[3 * q + a] >= s->G.n[a]) {
+                saras::set_last_error("probe point outside the core");
+                return SARAS_BAD_ARG;
+            }
+    int dev = 0;
+    NSE_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        saras::set_last_error("saras_nse_probe_noalloc: device index out of range");
+        return SARAS_BAD_ARG;
+    }
+    ProbeScratch &ps = probe_scratch(dev);
+    std::lock_guard<std::mutex> lk(ps.mu);
+    if (!ps.ready) {
+        // once per GPU, with that GPU current
+        NSE_TRY(hipGetSymbolAddress((void **)&ps.ptr, HIP_SYMBOL(scratch)));
+        if (!ps.ptr) {
+            saras::set_last_error("saras_nse_probe_noalloc: no device address for the probe scratch");
+            return SARAS_HIP_ERROR;
+        }
+        ps.ready = true;
+    }
+    int *dij = reinterpret_cast<int *>(ps.ptr);
+    double *dout = reinterpret_cast<double *>(ps.ptr + kProbeIdxBytes);
+    NSE_TRY(hipMemcpyAsync(dij, ijk, sizeof(int) * 3 * count, hipMemcpyHostToDevice, s->s));
+    const double *f[5] = {s->F[0], s->F[1], s->F[2], s->F[4], s->p.has_T ? s->F[3] : nullptr};
+    hipLaunchKernelGGL(saras::nse::k_probe, dim3((count + 255) / 256), dim3(256), 0, s->s, s->G, f[0], f[1], f[2], f[3],
+                       f[4], count, dij, dout);
+    NSE_TRY(hipGetLastError());
+    NSE_TRY(hipMemcpyAsync(out, dout, sizeof(double) * 5 * count, hipMemcpyDeviceToHost, s->s));
+    NSE_TRY(hipStreamSynchronize(s->s));
+    return SARAS_OK;
 }
 
 }  // extern "C"

@@ -91,7 +91,7 @@ EXPORTS = [
     "saras_bench_stream_copy", "saras_nse_create", "saras_nse_physics", "saras_nse_destroy", "saras_nse_set_fields",
     "saras_nse_get_fields", "saras_nse_step", "saras_nse_stage", "saras_nse_rhs", "saras_nse_get_rhs",
     "saras_nse_solve_field", "saras_field_op", "saras_nse_diag", "saras_nse_bench_steps",
-    "saras_nse_cfl", "saras_nse_sgs_diag", "saras_nse_tsdata", "saras_nse_probe", "saras_mg_set_graphs", "saras_mg_footprint", "saras_server_create", "saras_server_submit",
+    "saras_nse_cfl", "saras_nse_sgs_diag", "saras_nse_tsdata", "saras_nse_probe", "saras_nse_probe_noalloc", "saras_mg_set_graphs", "saras_mg_footprint", "saras_server_create", "saras_server_submit",
     "saras_server_poll", "saras_server_drain", "saras_server_clock", "saras_server_mem_used",
     "saras_server_destroy",
 ]
@@ -145,6 +145,7 @@ def lib():
         L.saras_nse_diag.argtypes = [vp, dp]
         L.saras_nse_tsdata.argtypes = [vp, dp]
         L.saras_nse_probe.argtypes = [vp, C.c_int, C.POINTER(C.c_int), dp]
+        L.saras_nse_probe_noalloc.argtypes = [vp, C.c_int, C.POINTER(C.c_int), dp]
         L.saras_nse_bench_steps.argtypes = [vp, C.c_int, dp]
         L.saras_nse_cfl.argtypes = [vp, C.c_double, C.c_double, dp]
         L.saras_nse_sgs_diag.argtypes = [vp, dp, dp]
@@ -493,6 +494,14 @@ class Nse:
         ijk = np.ascontiguousarray(np.asarray(points, dtype=np.int32).reshape(-1, 3))
         out = np.zeros((ijk.shape[0], 5))
         check(lib().saras_nse_probe(self.h, ijk.shape[0], ijk.ctypes.data_as(C.POINTER(C.c_int)), _dp(out)))
+        return out
+
+    def probe_noalloc(self, points):
+        """As probe(), staged in the library's fixed device array (no device
+        allocation; at most PROBE_NOALLOC_MAX points)."""
+        ijk = np.ascontiguousarray(np.asarray(points, dtype=np.int32).reshape(-1, 3))
+        out = np.zeros((ijk.shape[0], 5))
+        check(lib().saras_nse_probe_noalloc(self.h, ijk.shape[0], ijk.ctypes.data_as(C.POINTER(C.c_int)), _dp(out)))
         return out
 
     def sgs(self):

@@ -250,6 +250,20 @@ saras_status saras_nse_create(const saras_grid_desc *grid, const saras_mg_params
                               const saras_nse_params *params, saras_nse **out);
 saras_status saras_nse_destroy(saras_nse *s);
 
+/* Device bytes a time-step handle of this grid and parameter set holds
+ * (fields, stage arrays, its pressure solver, staging); 0 for an invalid grid.
+ * No counterpart in the reference: the request loop admits by it. */
+size_t saras_nse_footprint(const saras_grid_desc *grid, const saras_mg_params *mg, const saras_nse_params *params);
+
+/* Set the run time (timestep's solTime, the driver's `time`, which
+ * reader::readRestart restores from a restart file, reader.cc:171-226);
+ * the LES model enters once it exceeds 5 time steps (lsRK3_d3.cc:238). */
+saras_status saras_nse_set_time(saras_nse *s, double time);
+
+/* saras_mg_set_graphs for the handle's pressure solver (no counterpart in
+ * the reference). */
+saras_status saras_nse_set_graphs(saras_nse *s, int enable);
+
 /* Load fields (NULL keeps the current one; only the core is read) and
  * impose the boundary conditions (the solvers' initial imposeBCs). */
 saras_status saras_nse_set_fields(saras_nse *s, const double *vx, const double *vy, const double *vz,
@@ -286,6 +300,20 @@ saras_status saras_nse_solve_field(saras_nse *s, int field, double beta, int *it
  * sfield::computeNLin with the handle's velocity (out += H;
  * sfield.cc:160-311). */
 saras_status saras_field_op(saras_nse *s, int op, const double *f, double *out, int ptr_kind);
+
+/* The vector field operators over the core of caller arrays whose ghost
+ * cells are used as supplied (layout as above, pad grid->pad):
+ *   op 0 vfield::divergence (vfield.cc:721-737): ox = dfx/dx + dfy/dy + dfz/dz
+ *   op 1 sfield::gradient (sfield.cc:496-508): (ox, oy, oz) = grad fx
+ *        (fy, fz unused)
+ *   op 2 vfield::computeDiff (vfield.cc:150-192): o_c += lap f_c
+ *   op 3 vfield::computeNLin(V, H) with V = (fx, fy, fz) advecting itself
+ *        (vfield.cc:205-689; the handle's non-linear scheme, nu for the
+ *        upwind Peclet number): o_c += H_c, i.e. o_c -= (V . grad) f_c
+ * Ops 0 and 1 assign the core, 2 and 3 accumulate into it, as the
+ * reference's methods do.  3-D handles. */
+saras_status saras_field_vec_op(saras_nse *s, int op, const double *fx, const double *fy, const double *fz,
+                                double *ox, double *oy, double *oz, int ptr_kind);
 
 /* Sub-grid diagnostics of the last LES evaluation (spiral.cc:186-206):
  * out[0] sum |K| dV, out[1] sum of the SGS dissipation dV, out[2] sum nu_t dV
@@ -378,11 +406,31 @@ typedef struct {
 
 typedef struct saras_server saras_server;
 
+/* A short Navier-Stokes run: `steps` calls of saras_nse_step on a handle of
+ * (grid, mg, nse) from the caller's fields (host arrays, layout as above,
+ * pad grid.pad; t may be NULL when nse.has_T is 0), with the run time and
+ * the pressure correction starting at zero, as a fresh lsRK3_d3/eulerCN_d3
+ * run does.  On completion vx, vy, vz, p (and t) hold the state after the
+ * last step as saras_nse_get_fields writes it.  The arrays must stay valid
+ * until the completion with the same tag is returned; its stats are those of
+ * the last pressure solve, and its status is that of the first failing step
+ * (SARAS_JACOBI_DIVERGED, ...) or SARAS_OK. */
+typedef struct {
+    saras_grid_desc grid;
+    saras_mg_params mg;
+    saras_nse_params nse;
+    int steps;
+    double *vx, *vy, *vz, *p, *t;
+    long long tag;
+} saras_nse_request;
+
 saras_status saras_server_create(const saras_server_params *params, saras_server **out);
 /* Queue a request (returns at once).  Requests are admitted in arrival
  * order as slots and device memory allow; a request that waits for memory
  * lets later, smaller ones pass. */
 saras_status saras_server_submit(saras_server *s, const saras_request *req);
+/* Queue a short Navier-Stokes run; admitted by its footprint like a solve. */
+saras_status saras_server_submit_nse(saras_server *s, const saras_nse_request *req);
 /* Move up to max finished requests into out; waits up to wait_ms for the
  * first one.  Returns the number moved (negative on a bad argument). */
 int saras_server_poll(saras_server *s, saras_completion *out, int max, int wait_ms);

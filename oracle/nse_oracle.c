@@ -33,6 +33,7 @@
 struct or_nse {
     or_nse_desc d;
     int n[3], p;
+    int planar, py;      /* PLANAR build (dim 2: ny = 1, y index 0 only, no y pad) */
     long SX, SY;
     size_t full;
     double *x[3], *xi_x[3], *xixx[3], *xix2[3];   /* field metrics over [-p, n+p-1], +p offset */
@@ -51,7 +52,7 @@ struct or_nse {
     double sgs_diag[3];
 };
 
-#define FX(S, i, j, k) ((long)((i) + (S)->p) * (S)->SX + (long)((j) + (S)->p) * (S)->SY + (long)((k) + (S)->p))
+#define FX(S, i, j, k) ((long)((i) + (S)->p) * (S)->SX + (long)((j) + (S)->py) * (S)->SY + (long)((k) + (S)->p))
 
 static double *zalloc(size_t n) { return calloc(n, sizeof(double)); }
 
@@ -62,8 +63,14 @@ or_nse *or_nse_create(const or_nse_desc *d)
     s->p = d->g.pad;
     for (int a = 0; a < 3; a++) s->n[a] = d->g.n[a];
     int p = s->p;
+    /* PLANAR (lsRK3_d2 / eulerCN_d2 with the PLANAR branches of the field
+     * classes): the y axis is one cell, never differentiated and never given
+     * boundary values, so the arrays carry no y pad */
+    s->planar = d->g.dim == 2;
+    if (s->planar) s->n[1] = 1;
+    s->py = s->planar ? 0 : p;
     s->SY = s->n[2] + 2 * p;
-    s->SX = (long)(s->n[1] + 2 * p) * s->SY;
+    s->SX = (long)(s->n[1] + 2 * s->py) * s->SY;
     s->full = (size_t)(s->n[0] + 2 * p) * s->SX;
     for (int a = 0; a < 3; a++) {
         int m = s->n[a] + 2 * p;
@@ -88,6 +95,9 @@ or_nse *or_nse_create(const or_nse_desc *d)
     s->iter = zalloc(s->full); s->tmpA = zalloc(s->full); s->tmpB = zalloc(s->full);
     or_mg_params mp = d->mg;
     mp.bc = OR_BC_NEUMANN;   /* allNeumann (poisson.cc:87) */
+    /* a PLANAR plume (probType 9: T's top wall plumeOut) solves for Pp with a
+     * Dirichlet top wall (poisson_d2.cc:641-647) */
+    mp.top_dirichlet = s->planar && d->has_T && d->tbc_type[5] == OR_FBC_PLUME_OUT;
     s->mg = or_mg_create(&d->g, &mp);
     s->max_iter = (int)pow(log((double)s->n[0] * s->n[1] * s->n[2]), 3);  /* timestep.cc:115 */
     s->tstp = d->dt;
@@ -115,15 +125,15 @@ int or_nse_max_iterations(const or_nse *s) { return s->max_iter; }
  * extent of the other axes, in the order x, y, z (mpidata.cc:371-382). */
 static void sync_faces(or_nse *s, double *F)
 {
-    int p = s->p, nx = s->n[0], ny = s->n[1], nz = s->n[2];
+    int p = s->p, py = s->py, nx = s->n[0], ny = s->n[1], nz = s->n[2];
     if (s->d.g.periodic[0])
         for (int q = 0; q < p; q++)
-            for (int j = -p; j < ny + p; j++)
+            for (int j = -py; j < ny + py; j++)
                 for (int k = -p; k < nz + p; k++) {
                     F[FX(s, -p + q, j, k)] = F[FX(s, nx - p + q, j, k)];
                     F[FX(s, nx + q, j, k)] = F[FX(s, q, j, k)];
                 }
-    if (s->d.g.periodic[1])
+    if (s->d.g.periodic[1] && !s->planar)
         for (int i = -p; i < nx + p; i++)
             for (int q = 0; q < p; q++)
                 for (int k = -p; k < nz + p; k++) {
@@ -132,7 +142,7 @@ static void sync_faces(or_nse *s, double *F)
                 }
     if (s->d.g.periodic[2])
         for (int i = -p; i < nx + p; i++)
-            for (int j = -p; j < ny + p; j++)
+            for (int j = -py; j < ny + py; j++)
                 for (int q = 0; q < p; q++) {
                     F[FX(s, i, j, -p + q)] = F[FX(s, i, j, nz - p + q)];
                     F[FX(s, i, j, nz + q)] = F[FX(s, i, j, q)];
@@ -147,7 +157,7 @@ static void wall_bc(or_nse *s, double *F, int wall, int type, double val)
     int ax = wall / 2, hi = wall & 1;
     int g = hi ? n[ax] : -1, src = hi ? n[ax] - 1 : 0;
     double kh = val * s->hwall[wall];
-    int lo[3] = {-p, -p, -p}, up[3] = {n[0] + p - 1, n[1] + p - 1, n[2] + p - 1};
+    int lo[3] = {-p, -s->py, -p}, up[3] = {n[0] + p - 1, n[1] + s->py - 1, n[2] + p - 1};
     lo[ax] = up[ax] = 0;
     for (int a = lo[0]; a <= up[0]; a++)
         for (int b = lo[1]; b <= up[1]; b++)
@@ -185,7 +195,7 @@ static void impose(or_nse *s, double *F, const int *type, const double *val)
 {
     sync_faces(s, F);
     for (int ax = 0; ax < 3; ax++) {
-        if (s->d.g.periodic[ax]) continue;
+        if (s->d.g.periodic[ax] || (ax == 1 && s->planar)) continue;   /* PLANAR: no y walls */
         if (ax < 2) {
             wall_bc(s, F, 2 * ax, type[2 * ax], val[2 * ax]);
             wall_bc(s, F, 2 * ax + 1, type[2 * ax + 1], val[2 * ax + 1]);
@@ -327,7 +337,7 @@ static void compute_diff(or_nse *s, const double *F, double *H)
     CORE_LOOP(s) {
         long c = FX(s, i, j, k);
         H[c] += d2(s, F, 0, i, j, k);
-        H[c] += d2(s, F, 1, i, j, k);
+        if (!s->planar) H[c] += d2(s, F, 1, i, j, k);
         H[c] += d2(s, F, 2, i, j, k);
     }
 }
@@ -336,24 +346,32 @@ static void compute_diff(or_nse *s, const double *F, double *H)
 static void nlin_v(or_nse *s, double *H[3])
 {
     double **V = s->V;
+    /* PLANAR (vfield.cc:211-240, 289-349): Vx and Vz only, x and z passes only;
+     * the upwind loop has no PLANAR guard, but with Vy = 0 its y pass and its
+     * Vy row (never used: solveVy is skipped) change nothing */
+    const int pl = s->planar;
     if (s->d.nlscheme == 2) {
         CORE_LOOP(s) {
             long q = FX(s, i, j, k);
             for (int a = 0; a < 3; a++) {
+                if (pl && a == 1) continue;
                 double u = V[a][q];
-                for (int c = 0; c < 3; c++) H[c][q] -= upwind(s, u, V[c], a, i, j, k, s->d.nu);
+                for (int c = 0; c < 3; c++)
+                    if (!(pl && c == 1)) H[c][q] -= upwind(s, u, V[c], a, i, j, k, s->d.nu);
             }
         }
         return;
     }
     if (s->d.nlscheme == 1) {
-        for (int c = 0; c < 3; c++)
+        for (int c = 0; c < 3; c++) {
+            if (pl && c == 1) continue;
             CORE_LOOP(s) {
                 long q = FX(s, i, j, k);
                 H[c][q] -= V[0][q] * d1(s, V[c], 0, i, j, k);
-                H[c][q] -= V[1][q] * d1(s, V[c], 1, i, j, k);
+                if (!pl) H[c][q] -= V[1][q] * d1(s, V[c], 1, i, j, k);
                 H[c][q] -= V[2][q] * d1(s, V[c], 2, i, j, k);
             }
+        }
         return;
     }
     /* products over the full arrays: tempXX = Vx Vx, tempZX = Vz Vx, tempXY = Vx Vy,
@@ -377,8 +395,10 @@ static void nlin_v(or_nse *s, double *H[3])
     CORE_LOOP(s) {
         long q = FX(s, i, j, k);
         for (int a = 0; a < 3; a++) {
+            if (pl && a == 1) continue;
             double u = V[a][q];
             for (int c = 0; c < 3; c++) {
+                if (pl && c == 1) continue;
                 double adv = mor_adv(s, u, V[c], a, i, j, k);
                 double dv = mor_div(s, pr[a][c], a, i, j, k);
                 H[c][q] -= (adv + dv) / 2;
@@ -392,10 +412,12 @@ static void nlin_v(or_nse *s, double *H[3])
 static void nlin_s(or_nse *s, const double *F, double *H)
 {
     double **V = s->V;
+    const int pl = s->planar;   /* PLANAR: x and z terms only (sfield.cc:164-178, 209-260) */
     if (s->d.nlscheme == 2) {
         CORE_LOOP(s) {
             long q = FX(s, i, j, k);
-            for (int a = 0; a < 3; a++) H[q] -= upwind(s, V[a][q], F, a, i, j, k, s->d.kappa);
+            for (int a = 0; a < 3; a++)
+                if (!(pl && a == 1)) H[q] -= upwind(s, V[a][q], F, a, i, j, k, s->d.kappa);
         }
         return;
     }
@@ -403,7 +425,7 @@ static void nlin_s(or_nse *s, const double *F, double *H)
         CORE_LOOP(s) {
             long q = FX(s, i, j, k);
             H[q] -= V[0][q] * d1(s, F, 0, i, j, k);
-            H[q] -= V[1][q] * d1(s, F, 1, i, j, k);
+            if (!pl) H[q] -= V[1][q] * d1(s, F, 1, i, j, k);
             H[q] -= V[2][q] * d1(s, F, 2, i, j, k);
         }
         return;
@@ -419,6 +441,7 @@ static void nlin_s(or_nse *s, const double *F, double *H)
     CORE_LOOP(s) {
         long q = FX(s, i, j, k);
         for (int a = 0; a < 3; a++) {
+            if (pl && a == 1) continue;
             double adv = mor_adv(s, V[a][q], F, a, i, j, k);
             double dv = mor_div(s, pr[a], a, i, j, k);
             H[q] -= (adv + dv) / 2;
@@ -434,19 +457,35 @@ static int jacobi(or_nse *s, double *F, const double *rhs, double beta, double d
     double dt = s->d.dt;
     int it = 0;
     for (;;) {
-        CORE_LOOP(s) {
-            double E = F[FX(s, i + 1, j, k)], W = F[FX(s, i - 1, j, k)];
-            double N = F[FX(s, i, j + 1, k)], Sv = F[FX(s, i, j - 1, k)];
-            double Tp = F[FX(s, i, j, k + 1)], B = F[FX(s, i, j, k - 1)];
-            double x2 = s->xix2[0][i + s->p], xx = s->xixx[0][i + s->p];
-            double y2 = s->xix2[1][j + s->p], yy = s->xixx[1][j + s->p];
-            double z2 = s->xix2[2][k + s->p], zz = s->xixx[2][k + s->p];
-            s->iter[FX(s, i, j, k)] =
-                ((s->tsih2[0] * x2 * (E + W) + s->tsi2h[0] * xx * (E - W) + s->tsih2[1] * y2 * (N + Sv) +
-                  s->tsi2h[1] * yy * (N - Sv) + s->tsih2[2] * z2 * (Tp + B) + s->tsi2h[2] * zz * (Tp - B)) *
-                     dt * diff * beta +
-                 rhs[FX(s, i, j, k)]) /
-                (1.0 + 2.0 * dt * diff * beta * (s->tsih2[0] * x2 + s->tsih2[1] * y2 + s->tsih2[2] * z2));
+        if (s->planar) {
+            /* predictors.cc:65-77 (PLANAR): no y terms */
+            CORE_LOOP(s) {
+                double E = F[FX(s, i + 1, j, k)], W = F[FX(s, i - 1, j, k)];
+                double Tp = F[FX(s, i, j, k + 1)], B = F[FX(s, i, j, k - 1)];
+                double x2 = s->xix2[0][i + s->p], xx = s->xixx[0][i + s->p];
+                double z2 = s->xix2[2][k + s->p], zz = s->xixx[2][k + s->p];
+                s->iter[FX(s, i, j, k)] =
+                    ((s->tsih2[0] * x2 * (E + W) + s->tsi2h[0] * xx * (E - W) + s->tsih2[2] * z2 * (Tp + B) +
+                      s->tsi2h[2] * zz * (Tp - B)) *
+                         dt * diff * beta +
+                     rhs[FX(s, i, j, k)]) /
+                    (1.0 + 2.0 * dt * diff * beta * (s->tsih2[0] * x2 + s->tsih2[2] * z2));
+            }
+        } else {
+            CORE_LOOP(s) {
+                double E = F[FX(s, i + 1, j, k)], W = F[FX(s, i - 1, j, k)];
+                double N = F[FX(s, i, j + 1, k)], Sv = F[FX(s, i, j - 1, k)];
+                double Tp = F[FX(s, i, j, k + 1)], B = F[FX(s, i, j, k - 1)];
+                double x2 = s->xix2[0][i + s->p], xx = s->xixx[0][i + s->p];
+                double y2 = s->xix2[1][j + s->p], yy = s->xixx[1][j + s->p];
+                double z2 = s->xix2[2][k + s->p], zz = s->xixx[2][k + s->p];
+                s->iter[FX(s, i, j, k)] =
+                    ((s->tsih2[0] * x2 * (E + W) + s->tsi2h[0] * xx * (E - W) + s->tsih2[1] * y2 * (N + Sv) +
+                      s->tsi2h[1] * yy * (N - Sv) + s->tsih2[2] * z2 * (Tp + B) + s->tsi2h[2] * zz * (Tp - B)) *
+                         dt * diff * beta +
+                     rhs[FX(s, i, j, k)]) /
+                    (1.0 + 2.0 * dt * diff * beta * (s->tsih2[0] * x2 + s->tsih2[1] * y2 + s->tsih2[2] * z2));
+            }
         }
         /* the reference copies the whole of iterTemp (predictors.cc:96), whose
          * pads are never written; every pad cell a stencil reads is rebuilt by
@@ -459,15 +498,23 @@ static int jacobi(or_nse *s, double *F, const double *rhs, double beta, double d
                 for (int k = 0; k < s->n[2]; k++) {
                     double C = F[FX(s, i, j, k)];
                     double E = F[FX(s, i + 1, j, k)], W = F[FX(s, i - 1, j, k)];
-                    double N = F[FX(s, i, j + 1, k)], Sv = F[FX(s, i, j - 1, k)];
                     double Tp = F[FX(s, i, j, k + 1)], B = F[FX(s, i, j, k - 1)];
                     double x2 = s->xix2[0][i + s->p], xx = s->xixx[0][i + s->p];
-                    double y2 = s->xix2[1][j + s->p], yy = s->xixx[1][j + s->p];
                     double z2 = s->xix2[2][k + s->p], zz = s->xixx[2][k + s->p];
-                    double r = C - beta * dt * diff *
-                                       (x2 * (E - 2.0 * C + W) * s->tsih2[0] + xx * (E - W) * s->tsi2h[0] +
-                                        y2 * (N - 2.0 * C + Sv) * s->tsih2[1] + yy * (N - Sv) * s->tsi2h[1] +
-                                        z2 * (Tp - 2.0 * C + B) * s->tsih2[2] + zz * (Tp - B) * s->tsi2h[2]);
+                    double r;
+                    if (s->planar) {
+                        /* predictors.cc:100-109 (PLANAR) */
+                        r = C - beta * dt * diff *
+                                    (x2 * (E - 2.0 * C + W) * s->tsih2[0] + xx * (E - W) * s->tsi2h[0] +
+                                     z2 * (Tp - 2.0 * C + B) * s->tsih2[2] + zz * (Tp - B) * s->tsi2h[2]);
+                    } else {
+                        double N = F[FX(s, i, j + 1, k)], Sv = F[FX(s, i, j - 1, k)];
+                        double y2 = s->xix2[1][j + s->p], yy = s->xixx[1][j + s->p];
+                        r = C - beta * dt * diff *
+                                    (x2 * (E - 2.0 * C + W) * s->tsih2[0] + xx * (E - W) * s->tsi2h[0] +
+                                     y2 * (N - 2.0 * C + Sv) * s->tsih2[1] + yy * (N - Sv) * s->tsi2h[1] +
+                                     z2 * (Tp - 2.0 * C + B) * s->tsih2[2] + zz * (Tp - B) * s->tsi2h[2]);
+                    }
                     double a = fabs(r - rhs[FX(s, i, j, k)]);
                     if (a > mx) mx = a;
                 }
@@ -739,7 +786,7 @@ static int euler_rhs(or_nse *s)
     CORE_LOOP(s) {
         long q = FX(s, i, j, k);
         s->pg[0][q] = d1(s, s->P, 0, i, j, k);
-        s->pg[1][q] = d1(s, s->P, 1, i, j, k);
+        if (!s->planar) s->pg[1][q] = d1(s, s->P, 1, i, j, k);
         s->pg[2][q] = d1(s, s->P, 2, i, j, k);
     }
     for (int c = 0; c < 3; c++) {
@@ -789,7 +836,7 @@ int or_nse_rhs(or_nse *s, int stage)
     CORE_LOOP(s) {
         long q = FX(s, i, j, k);
         s->pg[0][q] = d1(s, s->P, 0, i, j, k);
-        s->pg[1][q] = d1(s, s->P, 1, i, j, k);
+        if (!s->planar) s->pg[1][q] = d1(s, s->P, 1, i, j, k);
         s->pg[2][q] = d1(s, s->P, 2, i, j, k);
     }
     for (int c = 0; c < 3; c++)
@@ -824,6 +871,10 @@ int or_nse_stage(or_nse *s, int stage)
     int rr = or_nse_rhs(s, stage);
     if (rr) return rr;
     for (int c = 0; c < 3; c++) {
+        if (s->planar && c == 1) {   /* lsRK3_d2 / eulerCN_d2: no solveVy */
+            s->jacobi_iters[1] = 0;
+            continue;
+        }
         s->jacobi_iters[c] = jacobi(s, s->V[c], s->nse[c], b, s->d.nu, s->d.vbc_type[c], s->d.vbc_val[c]);
         if (s->jacobi_iters[c] < 0) return 2;
     }
@@ -836,7 +887,7 @@ int or_nse_stage(or_nse *s, int stage)
     CORE_LOOP(s) {
         long q = FX(s, i, j, k);
         s->mgr[q] += d1(s, s->V[0], 0, i, j, k);
-        s->mgr[q] += d1(s, s->V[1], 1, i, j, k);
+        if (!s->planar) s->mgr[q] += d1(s, s->V[1], 1, i, j, k);
         s->mgr[q] += d1(s, s->V[2], 2, i, j, k);
     }
     double sc = euler ? 1.0 / s->d.dt : 1.0 / ((a + b) * s->d.dt);
@@ -848,7 +899,7 @@ int or_nse_stage(or_nse *s, int stage)
     CORE_LOOP(s) {
         long q = FX(s, i, j, k);
         s->pg[0][q] = d1(s, s->Pp, 0, i, j, k);
-        s->pg[1][q] = d1(s, s->Pp, 1, i, j, k);
+        if (!s->planar) s->pg[1][q] = d1(s, s->Pp, 1, i, j, k);
         s->pg[2][q] = d1(s, s->Pp, 2, i, j, k);
     }
     double pc = euler ? s->d.dt : (a + b) * s->d.dt;
@@ -923,6 +974,7 @@ void or_nse_op(or_nse *s, int op, const double *F, double *out)
 {
     size_t n = s->full;
     memset(out, 0, n * sizeof(double));
+    if (s->planar) return;   /* 3-D operators (as the library's) */
     switch (op) {
         case 0: case 1: case 2:       /* first derivative along op */
             CORE_LOOP(s) out[FX(s, i, j, k)] = d1(s, F, op, i, j, k);
@@ -941,6 +993,55 @@ void or_nse_op(or_nse *s, int op, const double *F, double *out)
 
 /* tseries.cc:410-446 (z gravity): volume-mean thermal energy
  * 1/2 (T + z - 1)^2 of the deviation from the conduction profile */
+/* The vector operators of vfield/sfield on caller arrays:
+ * 0 vfield::divergence (vfield.cc:721-737), 1 sfield::gradient
+ * (sfield.cc:496-508), 2 vfield::computeDiff (vfield.cc:150-192),
+ * 3 vfield::computeNLin of (fx, fy, fz) advecting itself (vfield.cc:205-689).
+ * Ops 0 and 1 assign the core of their outputs; 2 and 3 accumulate. */
+void or_nse_vec_op(or_nse *s, int op, const double *fx, const double *fy, const double *fz, double *ox, double *oy,
+                   double *oz)
+{
+    if (s->planar) return;   /* 3-D operators (as the library's) */
+    switch (op) {
+        case 0:
+            CORE_LOOP(s) {
+                long q = FX(s, i, j, k);
+                double d = 0.0;             /* divV = 0.0; divV += ... */
+                d += d1(s, fx, 0, i, j, k);
+                d += d1(s, fy, 1, i, j, k);
+                d += d1(s, fz, 2, i, j, k);
+                ox[q] = d;
+            }
+            break;
+        case 1:
+            CORE_LOOP(s) {
+                long q = FX(s, i, j, k);
+                ox[q] = d1(s, fx, 0, i, j, k);
+                oy[q] = d1(s, fx, 1, i, j, k);
+                oz[q] = d1(s, fx, 2, i, j, k);
+            }
+            break;
+        case 2:
+            compute_diff(s, fx, ox);
+            compute_diff(s, fy, oy);
+            compute_diff(s, fz, oz);
+            break;
+        default: {
+            /* the field advects itself: run nlin_v on it in place of V */
+            double *keep[3] = {s->V[0], s->V[1], s->V[2]};
+            double *H[3] = {ox, oy, oz};
+            s->V[0] = (double *)fx;
+            s->V[1] = (double *)fy;
+            s->V[2] = (double *)fz;
+            nlin_v(s, H);
+            s->V[0] = keep[0];
+            s->V[1] = keep[1];
+            s->V[2] = keep[2];
+            break;
+        }
+    }
+}
+
 double or_nse_thermal_energy(or_nse *s)
 {
     int p = s->p;
@@ -952,11 +1053,14 @@ double or_nse_thermal_energy(or_nse *s)
         for (int j = 0; j < s->n[1]; j++)
             for (int k = 0; k < s->n[2]; k++) {
                 long q = FX(s, i, j, k);
-                double dVol = (dxi[0] / s->xi_x[0][i + p]) * (dxi[1] / s->xi_x[1][j + p]) * (dxi[2] / s->xi_x[2][k + p]);
+                double dVol = s->planar ? (dxi[0] / s->xi_x[0][i + p]) * (dxi[2] / s->xi_x[2][k + p])
+                                        : (dxi[0] / s->xi_x[0][i + p]) * (dxi[1] / s->xi_x[1][j + p]) *
+                                              (dxi[2] / s->xi_x[2][k + p]);
                 double theta = s->T[q] + s->x[2][k + p] - 1.0;
                 te += pow(theta, 2.0) * 0.5 * dVol;
             }
-    return te / (s->d.g.len[0] * s->d.g.len[1] * s->d.g.len[2]);
+    /* tseries.cc:79-83: totalVol = xLen zLen in PLANAR */
+    return te / (s->planar ? s->d.g.len[0] * s->d.g.len[2] : s->d.g.len[0] * s->d.g.len[1] * s->d.g.len[2]);
 }
 
 /* tseries.cc:209-290: KE, max |div|, NuVol */
@@ -970,16 +1074,24 @@ void or_nse_diag(or_nse *s, double *ke, double *divmax, double *nuvol, double *n
         for (int j = 0; j < s->n[1]; j++)
             for (int k = 0; k < s->n[2]; k++) {
                 long q = FX(s, i, j, k);
-                double dVol = (dxi[0] / s->xi_x[0][i + p]) * (dxi[1] / s->xi_x[1][j + p]) * (dxi[2] / s->xi_x[2][k + p]);
-                tke += (pow(s->V[0][q], 2.0) + pow(s->V[1][q], 2.0) + pow(s->V[2][q], 2.0)) * 0.5 * dVol;
+                double dVol, dv = 0.0;
+                if (s->planar) {
+                    /* tseries.cc:227-233 (PLANAR) */
+                    dVol = (dxi[0] / s->xi_x[0][i + p]) * (dxi[2] / s->xi_x[2][k + p]);
+                    tke += (pow(s->V[0][q], 2.0) + pow(s->V[2][q], 2.0)) * 0.5 * dVol;
+                    dv += d1(s, s->V[0], 0, i, j, k);
+                    dv += d1(s, s->V[2], 2, i, j, k);
+                } else {
+                    dVol = (dxi[0] / s->xi_x[0][i + p]) * (dxi[1] / s->xi_x[1][j + p]) * (dxi[2] / s->xi_x[2][k + p]);
+                    tke += (pow(s->V[0][q], 2.0) + pow(s->V[1][q], 2.0) + pow(s->V[2][q], 2.0)) * 0.5 * dVol;
+                    dv += d1(s, s->V[0], 0, i, j, k);
+                    dv += d1(s, s->V[1], 1, i, j, k);
+                    dv += d1(s, s->V[2], 2, i, j, k);
+                }
                 if (s->d.has_T) uzt += s->V[2][q] * s->T[q] * dVol;
-                double dv = 0.0;
-                dv += d1(s, s->V[0], 0, i, j, k);
-                dv += d1(s, s->V[1], 1, i, j, k);
-                dv += d1(s, s->V[2], 2, i, j, k);
                 if (fabs(dv) > dm) dm = fabs(dv);
             }
-    double vol = s->d.g.len[0] * s->d.g.len[1] * s->d.g.len[2];
+    double vol = s->planar ? s->d.g.len[0] * s->d.g.len[2] : s->d.g.len[0] * s->d.g.len[1] * s->d.g.len[2];
     if (ke) *ke = tke / vol;
     if (divmax) *divmax = dm;
     if (nuvol) *nuvol = s->d.has_T ? 1.0 + (uzt / vol) / s->d.kappa : 0.0;
@@ -993,18 +1105,22 @@ void or_nse_diag(or_nse *s, double *ke, double *divmax, double *nuvol, double *n
         double wall_dn = open_top ? s->x[2][0 + p] : s->d.g.len[2] - s->x[2][s->n[2] - 1 + p];
         for (int i = 0; i < s->n[0]; i++)
             for (int j = 0; j < s->n[1]; j++) {
-                double dA = (dxi[0] / s->xi_x[0][i + p]) * (dxi[1] / s->xi_x[1][j + p]);
+                double dA = s->planar ? dxi[0] / s->xi_x[0][i + p]
+                                      : (dxi[0] / s->xi_x[0][i + p]) * (dxi[1] / s->xi_x[1][j + p]);
                 double dTdn = (1.0 - fabs(s->T[FX(s, i, j, 0)])) / wall_dn;
                 dt_sum += dTdn * dA;
             }
         if (!open_top)
             for (int i = 0; i < s->n[0]; i++)
                 for (int j = 0; j < s->n[1]; j++) {
-                    double dA = (dxi[0] / s->xi_x[0][i + p]) * (dxi[1] / s->xi_x[1][j + p]);
+                    double dA = s->planar ? dxi[0] / s->xi_x[0][i + p]
+                                          : (dxi[0] / s->xi_x[0][i + p]) * (dxi[1] / s->xi_x[1][j + p]);
                     double dTdn = fabs(s->T[FX(s, i, j, s->n[2] - 1)]) / wall_dn;
                     dt_sum += dTdn * dA;
                 }
-        *nuwall = dt_sum / (s->d.g.len[0] * s->d.g.len[1] * (open_top ? 1 : 2));
+        /* csArea = xLen in PLANAR (tseries.cc:110-114) */
+        double cs = s->planar ? s->d.g.len[0] : s->d.g.len[0] * s->d.g.len[1];
+        *nuwall = dt_sum / (cs * (open_top ? 1 : 2));
     }
 }
 
@@ -1018,7 +1134,9 @@ double or_nse_cfl(or_nse *s, double courant, double dt_max)
         for (int j = 0; j < s->n[1]; j++)
             for (int k = 0; k < s->n[2]; k++) {
                 long q = FX(s, i, j, k);
-                double v = (fabs(s->V[0][q]) / avg[0] + fabs(s->V[1][q]) / avg[1]) + fabs(s->V[2][q]) / avg[2];
+                /* vfield.cc:694-696 (PLANAR: x and z only) */
+                double v = s->planar ? fabs(s->V[0][q]) / avg[0] + fabs(s->V[2][q]) / avg[2]
+                                     : (fabs(s->V[0][q]) / avg[0] + fabs(s->V[1][q]) / avg[1]) + fabs(s->V[2][q]) / avg[2];
                 if (first || v > mx) mx = v;
                 first = 0;
             }

@@ -104,6 +104,7 @@ def lib():
         L.or_nse_jacobi_iters.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         L.or_nse_mg_stats.argtypes = [C.c_void_p, C.POINTER(MgStats)]
         L.or_nse_op.argtypes = [C.c_void_p, C.c_int, dp, dp]
+        L.or_nse_vec_op.argtypes = [C.c_void_p, C.c_int] + [dp] * 6
         L.or_nse_diag.argtypes = [C.c_void_p, dp, dp, dp, dp]
         L.or_nse_thermal_energy.argtypes = [C.c_void_p]
         L.or_nse_thermal_energy.restype = C.c_double
@@ -327,7 +328,8 @@ class Nse:
         self.d = d
         self.h = lib().or_nse_create(C.byref(d))
         g = d.g
-        self.shape = (g.n[0] + 2 * g.pad, g.n[1] + 2 * g.pad, g.n[2] + 2 * g.pad)
+        # planar (dim 2) arrays: one y row, no y pad
+        self.shape = (g.n[0] + 2 * g.pad, 1 if g.dim == 2 else g.n[1] + 2 * g.pad, g.n[2] + 2 * g.pad)
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -378,6 +380,14 @@ class Nse:
         out = np.zeros(self.shape)
         lib().or_nse_op(self.h, op, _dp(F), _dp(out))
         return out
+
+    def vec_op(self, op, fx, fy=None, fz=None, out=None):
+        """Vector operators (divergence, gradient, computeDiff, computeNLin):
+        `out` (three arrays) are the initial outputs, copied."""
+        f = [np.ascontiguousarray(a if a is not None else fx, dtype=np.float64) for a in (fx, fy, fz)]
+        o = [np.array(a, dtype=np.float64, copy=True) for a in out] if out is not None else [np.zeros(self.shape) for _ in range(3)]
+        lib().or_nse_vec_op(self.h, op, *[_dp(a) for a in f], *[_dp(a) for a in o])
+        return o
 
     def cfl(self, courant, dt_max):
         return lib().or_nse_cfl(self.h, courant, dt_max)

@@ -179,6 +179,9 @@ void or_nse_mg_stats(const or_nse *s, or_mg_stats *st);   /* last pressure-corre
  * along x/y/z, 3-5 second derivative, 6 computeDiff (Laplacian), 7 scalar
  * computeNLin with the current V (H -= ...). */
 void or_nse_op(or_nse *s, int op, const double *F, double *out);
+/* vector operators: 0 divergence, 1 gradient, 2 computeDiff, 3 computeNLin */
+void or_nse_vec_op(or_nse *s, int op, const double *fx, const double *fy, const double *fz, double *ox, double *oy,
+                   double *oz);
 /* tseries.cc:209-290 diagnostics: volume-mean KE, max |div V|, NuVol, and
  * the wall Nusselt number of z-gravity runs. */
 void or_nse_diag(or_nse *s, double *ke, double *divmax, double *nuvol, double *nuwall);

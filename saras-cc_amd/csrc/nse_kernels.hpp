@@ -52,6 +52,14 @@ struct Geom {
     const unsigned char *wmask;
 };
 
+// The cell of this thread in the (ceil(nz/threads), ny, nx) row grids: a
+// block is a segment of row (i, j).
+__device__ __forceinline__ void row_cell(const Geom &G, int &i, int &j, int &k) {
+    (void)G;
+    j = blockIdx.y;
+    i = blockIdx.z;
+    k = blockIdx.x * blockDim.x + threadIdx.x;
+}
 // The z-wall kinds whose ghost depends on the column:
 //   hot plate (hotPlate.cc:114-123 with the patch of :74-111): F(wall) =
 //     F(edge) * mask + data * (2 data - F(edge)), data = 1 inside the disc,
@@ -376,9 +384,8 @@ __global__ void __launch_bounds__(256) k_rhs_v(Geom G, StageCoef c, CoreField V0
                                                CoreField T, CoreField P, double *__restrict__ H,
                                                double *__restrict__ R, int les, PerField Sa, PerField Sb,
                                                PerField Sc) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y;
-    const int i = blockIdx.z;
+    int i, j, k;
+    row_cell(G, i, j, k);
     if (k >= G.n[2]) return;
     if (interior2(G, i, j, k))
         rhs_v_cell<C>(G, c, fast_of(V0), fast_of(V1), fast_of(V2), fast_of(T), fast_of(P), H, R, les, Sa, Sb, Sc, i,
@@ -409,9 +416,8 @@ __device__ __forceinline__ void rhs_t_cell(const Geom &G, const StageCoef &c, co
 
 __global__ void __launch_bounds__(256) k_rhs_t(Geom G, StageCoef c, CoreField V0, CoreField V1, CoreField V2,
                                                CoreField T, double *__restrict__ H, double *__restrict__ R) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y;
-    const int i = blockIdx.z;
+    int i, j, k;
+    row_cell(G, i, j, k);
     if (k >= G.n[2]) return;
     if (interior2(G, i, j, k)) rhs_t_cell(G, c, fast_of(V0), fast_of(V1), fast_of(V2), fast_of(T), H, R, i, j, k);
     else rhs_t_cell(G, c, V0, V1, V2, T, H, R, i, j, k);
@@ -444,9 +450,8 @@ __global__ void __launch_bounds__(256) k_euler_v(Geom G, double k_nu, double nu,
                                                  double mpg, int nlscheme, CoreField V0, CoreField V1, CoreField V2,
                                                  CoreField T, CoreField P, double *__restrict__ R, int les,
                                                  PerField Sa, PerField Sb, PerField Sc) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y;
-    const int i = blockIdx.z;
+    int i, j, k;
+    row_cell(G, i, j, k);
     if (k >= G.n[2]) return;
     if (interior2(G, i, j, k))
         euler_v_cell<C>(G, k_nu, nu, dt, buoy, has_T, mpg, nlscheme, fast_of(V0), fast_of(V1), fast_of(V2), fast_of(T),
@@ -471,9 +476,8 @@ __device__ __forceinline__ void euler_t_cell(const Geom &G, double k_kappa, doub
 
 __global__ void __launch_bounds__(256) k_euler_t(Geom G, double k_kappa, double kappa, double dt, int nlscheme, CoreField V0,
                                                  CoreField V1, CoreField V2, CoreField T, double *__restrict__ R) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y;
-    const int i = blockIdx.z;
+    int i, j, k;
+    row_cell(G, i, j, k);
     if (k >= G.n[2]) return;
     if (interior2(G, i, j, k))
         euler_t_cell(G, k_kappa, kappa, dt, nlscheme, fast_of(V0), fast_of(V1), fast_of(V2), fast_of(T), R, i, j, k);
@@ -644,9 +648,8 @@ __global__ void __launch_bounds__(256) k_sgs_stress(Geom G, CoreField V0, CoreFi
                                                     double *__restrict__ Tzz, double *__restrict__ Txy,
                                                     double *__restrict__ Tyz, double *__restrict__ Tzx,
                                                     double *__restrict__ part, int *__restrict__ fail) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y;
-    const int i = blockIdx.z;
+    int i, j, k;
+    row_cell(G, i, j, k);
     double v[3] = {0.0, 0.0, 0.0};
     if (k < G.n[2]) {
         const CoreField V[3] = {V0, V1, V2};
@@ -788,9 +791,8 @@ template <bool RES>
 __global__ void __launch_bounds__(256) k_jacobi(Geom G, CoreField F, const double *__restrict__ rhs,
                                                 double *__restrict__ out, double dt, double diff, double beta,
                                                 double *__restrict__ part) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y;
-    const int i = blockIdx.z;
+    int i, j, k;
+    row_cell(G, i, j, k);
     double res = 0.0;
     if (k < G.n[2]) {
         if (interior2(G, i, j, k)) res = jacobi_cell<RES>(G, fast_of(F), rhs, out, dt, diff, beta, i, j, k);
@@ -803,9 +805,8 @@ __global__ void __launch_bounds__(256) k_jacobi(Geom G, CoreField F, const doubl
 // b = (((0 + dVx/dx) + dVy/dy) + dVz/dz) * sc   (lsRK3_d3.cc:278-279)
 __global__ void __launch_bounds__(256) k_div(Geom G, CoreField V0, CoreField V1, CoreField V2, double sc,
                                              double *__restrict__ b, long bsx, long bsy, long bhz) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y;
-    const int i = blockIdx.z;
+    int i, j, k;
+    row_cell(G, i, j, k);
     if (k >= G.n[2]) return;
     double d = 0.0;
     if (interior2(G, i, j, k)) {
@@ -825,9 +826,8 @@ template <class PF>
 __global__ void __launch_bounds__(256) k_correct(Geom G, PF Pp, double *__restrict__ P,
                                                  double *__restrict__ V0, double *__restrict__ V1,
                                                  double *__restrict__ V2, double pc) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y;
-    const int i = blockIdx.z;
+    int i, j, k;
+    row_cell(G, i, j, k);
     if (k >= G.n[2]) return;
     const long idx = (long)i * G.sx + (long)j * G.sy + k;
     P[idx] += Pp.at(G, i, j, k);
@@ -930,9 +930,8 @@ __global__ void __launch_bounds__(256) k_probe(Geom G, const double *__restrict_
 __global__ void __launch_bounds__(256) k_cfl(Geom G, const double *__restrict__ u, const double *__restrict__ v,
                                              const double *__restrict__ w, double ax, double ay, double az,
                                              double *__restrict__ part) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y;
-    const int i = blockIdx.z;
+    int i, j, k;
+    row_cell(G, i, j, k);
     double m = 0.0;
     if (k < G.n[2]) {
         const long idx = (long)i * G.sx + (long)j * G.sy + k;
@@ -1005,6 +1004,49 @@ __global__ void k_field_op(Geom G, PadField F, const CoreField V0, const CoreFie
     } else {
         const CoreField V[3] = {V0, V1, V2};
         out[o] = nlin_acc(G, nlscheme, V, F, i, j, k, out[o], kappa);
+    }
+}
+
+// The vector operators over caller arrays (ghost values as supplied), one
+// core cell per thread: 0 vfield::divergence (vfield.cc:721-737: ox = 0 +
+// dVx/dx + dVy/dy + dVz/dz), 1 sfield::gradient of A (sfield.cc:496-508:
+// (ox, oy, oz) = grad A), 2 vfield::computeDiff (vfield.cc:150-192: o_c +=
+// d2xx; += d2yy; += d2zz of component c), 3 vfield::computeNLin of the field
+// by itself (vfield.cc:205-689: o_c -= the x, y, z terms of component c for
+// the non-linear scheme; diffusivity nu for the upwind Peclet number).
+template <int OP>
+__global__ void k_field_vec_op(Geom G, PadField A, PadField B, PadField Cc, int nlscheme, double nu,
+                               double *__restrict__ ox, double *__restrict__ oy, double *__restrict__ oz, long SX,
+                               long SY, int p) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, i = blockIdx.z;
+    if (k >= G.n[2]) return;
+    const long o = (long)(i + p) * SX + (long)(j + p) * SY + (k + p);
+    if (OP == 0) {
+        double d = 0.0;
+        d += d1(G, A, 0, i, j, k);
+        d += d1(G, B, 1, i, j, k);
+        d += d1(G, Cc, 2, i, j, k);
+        ox[o] = d;
+    } else if (OP == 1) {
+        ox[o] = d1(G, A, 0, i, j, k);
+        oy[o] = d1(G, A, 1, i, j, k);
+        oz[o] = d1(G, A, 2, i, j, k);
+    } else if (OP == 2) {
+        const PadField F[3] = {A, B, Cc};
+        double *out[3] = {ox, oy, oz};
+        for (int c = 0; c < 3; ++c) {
+            double h = out[c][o];
+            h += d2(G, F[c], 0, i, j, k);
+            h += d2(G, F[c], 1, i, j, k);
+            h += d2(G, F[c], 2, i, j, k);
+            out[c][o] = h;
+        }
+    } else {
+        const PadField V[3] = {A, B, Cc};
+        ox[o] = nlin_acc(G, nlscheme, V, A, i, j, k, ox[o], nu);
+        oy[o] = nlin_acc(G, nlscheme, V, B, i, j, k, oy[o], nu);
+        oz[o] = nlin_acc(G, nlscheme, V, Cc, i, j, k, oz[o], nu);
     }
 }
 

@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -59,7 +60,7 @@ struct saras_nse {
     double *metrics = nullptr;          // 9 core metric arrays
     FieldBC bc[4]{};                     // Vx, Vy, Vz, T
     FieldBC pbc{};                       // P
-    FieldBC ppbc{};                      // Pp: the Poisson solver's Neumann walls (ghost = inner)
+    FieldBC ppbc{};                      // Pp: the Poisson solver's walls (Neumann; a planar plume's top: Dirichlet)
     double *F[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};   // Vx, Vy, Vz, T, P
     double *H[4] = {nullptr, nullptr, nullptr, nullptr};
     double *R[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -414,11 +415,16 @@ saras_status saras_nse_create(const saras_grid_desc *grid, const saras_mg_params
         saras::set_last_error("grid dim must be 2 (planar x-z) or 3");
         return SARAS_BAD_SHAPE;
     }
-    // dScheme 1 only: the PLANAR grid keeps a y pad of 1 whatever the scheme
-    // (grid.cc:92-95); with it every PLANAR loop (vfield.cc:289-349,
-    // sfield.cc:209-260) is the 3-D one without its y terms
-    if (planar && (P.dscheme != 1 || P.les || g.n[1] != 1)) {
-        saras::set_last_error("the planar path covers n_y = 1 and dScheme 1");
+    // Every PLANAR loop (vfield.cc:211-349, sfield.cc:164-260, predictors.cc)
+    // is the 3-D one without its y terms; the planar handle keeps the 3-D
+    // operand order with every y term an exact zero.  The PLANAR grid's y
+    // pad of 1 (grid.cc:92-95) is never read, so planar caller arrays carry
+    // no y pad.  With dScheme 2 the Morinishi z edge test reads
+    // core.ubound(1) = 0 (vfield.cc:329), so the last z cell takes the 4th-order
+    // stencil: that reads the second ghost layer, valid only on a periodic z
+    // axis (refused below for z walls, as for 3-D grids with ny != nz).
+    if (planar && (P.les || g.n[1] != 1)) {
+        saras::set_last_error("the planar path covers n_y = 1 without the LES model (3-D only in the reference)");
         return SARAS_BAD_ARG;
     }
     if ((P.dscheme != 1 && P.dscheme != 2) || P.nlscheme < 1 || P.nlscheme > 3 || g.pad < P.dscheme ||
@@ -650,6 +656,33 @@ saras_status saras_nse_create(const saras_grid_desc *grid, const saras_mg_params
     return SARAS_OK;
 }
 
+size_t saras_nse_footprint(const saras_grid_desc *grid, const saras_mg_params *mg, const saras_nse_params *params) {
+    if (!grid || !mg || !params) return 0;
+    saras_mg_params mp = *mg;
+    mp.bc = SARAS_BC_NEUMANN;
+    const size_t solver = saras_mg_footprint(grid, &mp);
+    if (solver == 0) return 0;
+    const bool planar = grid->dim == 2;
+    const size_t ny = planar ? 1 : (size_t)grid->n[1];
+    const size_t elems = (size_t)pitch16(grid->n[2]) * ny * grid->n[0];
+    // V, T, P, H(4), R(4), the Jacobi buffer, and the six LES stresses
+    const size_t arrays = 5 + 4 + 4 + 1 + (params->les ? 6 : 0);
+    const size_t p = grid->pad, yp = planar ? 0 : p;
+    const size_t full = (grid->n[0] + 2 * p) * (ny + 2 * yp) * (grid->n[2] + 2 * p);
+    return solver + arrays * elems * sizeof(double) + 2 * full * sizeof(double) + (1 << 20);
+}
+
+saras_status saras_nse_set_graphs(saras_nse *s, int enable) {
+    if (!s) return SARAS_BAD_ARG;
+    return saras_mg_set_graphs(s->mg, enable);
+}
+
+saras_status saras_nse_set_time(saras_nse *s, double time) {
+    if (!s || !std::isfinite(time)) return SARAS_BAD_ARG;
+    s->time = time;
+    return SARAS_OK;
+}
+
 saras_status saras_nse_destroy(saras_nse *s) {
     if (!s) return SARAS_BAD_ARG;
     (void)hipStreamSynchronize(s->s);
@@ -813,6 +846,67 @@ saras_status saras_field_op(saras_nse *s, int op, const double *f, double *out, 
     if (ptr_kind == SARAS_PTR_HOST)
         NSE_TRY(hipMemcpyAsync(out, s->stage_b, s->full * sizeof(double), hipMemcpyDeviceToHost, s->s));
     NSE_TRY(hipStreamSynchronize(s->s));
+    return SARAS_OK;
+}
+
+saras_status saras_field_vec_op(saras_nse *s, int op, const double *fx, const double *fy, const double *fz,
+                                double *ox, double *oy, double *oz, int ptr_kind) {
+    if (!s || op < 0 || op > 3) return SARAS_BAD_ARG;
+    if (ptr_kind != SARAS_PTR_HOST && ptr_kind != SARAS_PTR_DEVICE) return SARAS_BAD_ARG;
+    // inputs: V (ops 0, 2, 3) or the scalar fx (op 1); outputs: ox (op 0) or all three
+    const bool vin = op != 1, vout = op != 0;
+    if (!fx || (vin && (!fy || !fz)) || !ox || (vout && (!oy || !oz))) return SARAS_BAD_ARG;
+    if (s->planar) {
+        saras::set_last_error("single field operators cover 3-D handles");
+        return SARAS_BAD_SHAPE;
+    }
+    const int p = s->g.pad;
+    const long SY = s->g.n[2] + 2 * p, SX = SY * (s->g.n[1] + 2 * p);
+    const size_t bytes = s->full * sizeof(double);
+    const double *in[3] = {fx, vin ? fy : fx, vin ? fz : fx};
+    double *out[3] = {ox, vout ? oy : nullptr, vout ? oz : nullptr};
+    const double *din[3] = {in[0], in[1], in[2]};
+    double *dout[3] = {out[0], out[1], out[2]};
+    double *stage = nullptr;
+    if (ptr_kind == SARAS_PTR_HOST) {
+        // six full arrays for this call (a parity and tooling entry point)
+        NSE_TRY(hipMalloc((void **)&stage, 6 * bytes));
+        for (int c = 0; c < 3; ++c) {
+            double *d = stage + c * s->full;
+            if (hipMemcpyAsync(d, in[c], bytes, hipMemcpyHostToDevice, s->s) != hipSuccess) {
+                (void)hipFree(stage);
+                return SARAS_HIP_ERROR;
+            }
+            din[c] = d;
+            if (out[c]) {
+                double *e = stage + (3 + c) * s->full;
+                if (hipMemcpyAsync(e, out[c], bytes, hipMemcpyHostToDevice, s->s) != hipSuccess) {
+                    (void)hipFree(stage);
+                    return SARAS_HIP_ERROR;
+                }
+                dout[c] = e;
+            }
+        }
+    }
+    const PadField A{din[0], SX, SY, p}, B{din[1], SX, SY, p}, Cc{din[2], SX, SY, p};
+    const dim3 gr = s->grid3(), bl(s->threads);
+    const double nu = s->p.nu;
+    switch (op) {
+        case 0: hipLaunchKernelGGL(saras::nse::k_field_vec_op<0>, gr, bl, 0, s->s, s->G, A, B, Cc, s->p.nlscheme, nu, dout[0], dout[1], dout[2], SX, SY, p); break;
+        case 1: hipLaunchKernelGGL(saras::nse::k_field_vec_op<1>, gr, bl, 0, s->s, s->G, A, B, Cc, s->p.nlscheme, nu, dout[0], dout[1], dout[2], SX, SY, p); break;
+        case 2: hipLaunchKernelGGL(saras::nse::k_field_vec_op<2>, gr, bl, 0, s->s, s->G, A, B, Cc, s->p.nlscheme, nu, dout[0], dout[1], dout[2], SX, SY, p); break;
+        default: hipLaunchKernelGGL(saras::nse::k_field_vec_op<3>, gr, bl, 0, s->s, s->G, A, B, Cc, s->p.nlscheme, nu, dout[0], dout[1], dout[2], SX, SY, p); break;
+    }
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && ptr_kind == SARAS_PTR_HOST)
+        for (int c = 0; c < 3 && e == hipSuccess; ++c)
+            if (out[c]) e = hipMemcpyAsync(out[c], dout[c], bytes, hipMemcpyDeviceToHost, s->s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->s);
+    if (stage) (void)hipFree(stage);
+    if (e != hipSuccess) {
+        saras::set_last_error(std::string("saras_field_vec_op: ") + hipGetErrorString(e));
+        return SARAS_HIP_ERROR;
+    }
     return SARAS_OK;
 }
 

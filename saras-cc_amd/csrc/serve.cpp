@@ -1,4 +1,5 @@
-// serve.cpp -- the persistent request loop of many Poisson solves on one GPU.
+// serve.cpp -- the persistent request loop of many Poisson solves and short
+// Navier-Stokes runs on one GPU.
 //
 // Requests arrive at any time (saras_server_submit) and wait in a queue.
 // `slots` worker threads each own the solves they run; every solver handle
@@ -12,7 +13,10 @@
 // fit pass it, and is run alone if it can only ever fit on an idle device.
 //
 // The solve of every request is the plain saras_mg_solve path on its own
-// handle, so a result does not depend on what else is in flight.
+// handle, so a result does not depend on what else is in flight.  A short
+// Navier-Stokes request runs saras_nse_step on a handle of its own the same
+// way, starting from a zero run time and pressure correction whether the
+// handle is new or resident.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -63,19 +67,88 @@ std::string key_of(const saras_grid_desc &g, const saras_mg_params &p) {
     return k;
 }
 
+std::string key_of(const saras_grid_desc &g, const saras_mg_params &m, const saras_nse_params &q) {
+    std::string k = "nse";
+    k += key_of(g, m);
+    put(k, q.dscheme);
+    put(k, q.nlscheme);
+    put(k, q.has_T);
+    put(k, q.nu);
+    put(k, q.kappa);
+    put(k, q.dt);
+    put(k, q.cn_tol);
+    for (int a = 0; a < 3; ++a) put(k, q.buoy[a]);
+    for (int w = 0; w < 6; ++w) {
+        for (int c = 0; c < 3; ++c) {
+            put(k, q.vbc_type[c][w]);
+            put(k, q.vbc_val[c][w]);
+        }
+        put(k, q.tbc_type[w]);
+        put(k, q.tbc_val[w]);
+        put(k, q.pbc_type[w]);
+        put(k, q.pbc_val[w]);
+    }
+    put(k, q.scheme);
+    put(k, q.pe_limit);
+    put(k, q.central_bias);
+    put(k, q.les);
+    put(k, q.mean_pgrad);
+    put(k, q.plate_radius);
+    return k;
+}
+
+enum Kind { kPoisson = 0, kNse = 1 };
+
+// A handle of either kind.
+struct Handle {
+    Kind kind = kPoisson;
+    void *h = nullptr;
+    void destroy() {
+        if (!h) return;
+        if (kind == kPoisson) saras_mg_destroy(static_cast<saras_mg *>(h));
+        else saras_nse_destroy(static_cast<saras_nse *>(h));
+        h = nullptr;
+    }
+};
+
+// saras_nse_request: reset the run, `steps` time steps, fields back to the caller
+saras_status run_nse(saras_nse *h, const saras_nse_request &q, saras_completion &c) {
+    const saras_grid_desc &g = q.grid;
+    const long p = g.pad, yp = g.dim == 2 ? 0 : p, ny = g.dim == 2 ? 1 : g.n[1];
+    std::vector<double> zero((size_t)(g.n[0] + 2 * p) * (ny + 2 * yp) * (g.n[2] + 2 * p), 0.0);
+    const double *t = q.nse.has_T ? q.t : nullptr;
+    saras_status st = saras_nse_set_fields(h, q.vx, q.vy, q.vz, q.p, t, zero.data(), SARAS_PTR_HOST);
+    if (st == SARAS_OK) st = saras_nse_set_time(h, 0.0);
+    double ms = 0.0;
+    for (int k = 0; k < q.steps && st == SARAS_OK; ++k) {
+        saras_nse_stats ns{};
+        st = saras_nse_step(h, &ns);
+        c.stats = ns.mg;
+        ms += ns.device_ms;
+    }
+    if (st == SARAS_OK) st = saras_nse_get_fields(h, q.vx, q.vy, q.vz, q.p, q.nse.has_T ? q.t : nullptr, nullptr,
+                                                  SARAS_PTR_HOST);
+    c.stats.device_ms = ms;
+    c.stats.converged = st == SARAS_OK;
+    return st;
+}
+
 // a request blocked on memory stops others from passing it after this long
 constexpr double kMaxBypassSeconds = 0.25;
 
 struct Pending {
-    saras_request r;
-    double t_submit;
-    size_t bytes;
+    Kind kind = kPoisson;
+    saras_request r{};
+    saras_nse_request q{};
+    double t_submit = 0.0;
+    size_t bytes = 0;
     std::string key;
+    long long tag() const { return kind == kPoisson ? r.tag : q.tag; }
 };
 
 struct Idle {
     std::string key;
-    saras_mg *h;
+    Handle h;
     size_t bytes;
 };
 
@@ -88,7 +161,7 @@ struct saras_server {
     std::condition_variable cv_work, cv_done;
     std::deque<Pending> queue;
     std::list<Idle> idle;                  // front = most recently used
-    std::vector<saras_mg *> evicted;       // taken off `idle`, destroyed by a worker outside the lock
+    std::vector<Handle> evicted;           // taken off `idle`, destroyed by a worker outside the lock
     std::vector<saras_completion> done;
     std::vector<std::thread> workers;
     size_t mem_used = 0;                   // idle handles + running solves
@@ -101,7 +174,7 @@ struct saras_server {
     // Pick the next admissible request (caller holds mu).  Returns false when
     // none can run now.  On success the request's memory is accounted and, if
     // a resident handle fits it, that handle is taken from the idle list.
-    bool take(Pending &out, saras_mg *&h) {
+    bool take(Pending &out, Handle &h) {
         if (p.largest_first && queue.size() > 1) {
             // visit the waiting requests largest footprint first (stable for
             // equal footprints, so arrival order breaks ties)
@@ -134,7 +207,7 @@ struct saras_server {
                     idle.pop_back();
                 }
                 mem_used += need;
-                h = nullptr;
+                h = Handle{it->kind, nullptr};
                 out = *it;
                 queue.erase(it);
                 return true;
@@ -149,8 +222,8 @@ struct saras_server {
     void worker() {
         for (;;) {
             Pending job;
-            saras_mg *h = nullptr;
-            std::vector<saras_mg *> victims;
+            Handle h;
+            std::vector<Handle> victims;
             {
                 std::unique_lock<std::mutex> lk(mu);
                 bool got = false;
@@ -162,32 +235,47 @@ struct saras_server {
                 victims.swap(evicted);
                 if (!got) {
                     lk.unlock();
-                    for (saras_mg *v : victims) saras_mg_destroy(v);
+                    for (Handle &v : victims) v.destroy();
                     return;
                 }
                 ++running;
             }
             // hipFree synchronises the device: never under the server lock
-            for (saras_mg *v : victims) saras_mg_destroy(v);
+            for (Handle &v : victims) v.destroy();
             saras_completion c{};
-            c.tag = job.r.tag;
+            c.tag = job.tag();
             c.t_submit = job.t_submit;
             c.t_admit = now();
             saras_status st = SARAS_OK;
-            if (!h) {
-                st = saras_mg_create(&job.r.grid, &job.r.mg, &h);
-                if (st == SARAS_OK && p.use_graphs) st = saras_mg_set_graphs(h, 1);
+            if (job.kind == kPoisson) {
+                saras_mg *mg = static_cast<saras_mg *>(h.h);
+                if (!mg) {
+                    st = saras_mg_create(&job.r.grid, &job.r.mg, &mg);
+                    if (st == SARAS_OK) h.h = mg;
+                    if (st == SARAS_OK && p.use_graphs) st = saras_mg_set_graphs(mg, 1);
+                }
+                if (st == SARAS_OK) st = saras_mg_solve(mg, job.r.x, job.r.b, SARAS_PTR_HOST, nullptr, &c.stats);
+            } else {
+                saras_nse *ns = static_cast<saras_nse *>(h.h);
+                if (!ns) {
+                    st = saras_nse_create(&job.q.grid, &job.q.mg, &job.q.nse, &ns);
+                    if (st == SARAS_OK) h.h = ns;
+                    if (st == SARAS_OK && p.use_graphs) {
+                        // the time step's pressure solves replay their V-cycles too
+                        st = saras_nse_set_graphs(ns, 1);
+                    }
+                }
+                if (st == SARAS_OK) st = run_nse(ns, job.q, c);
             }
-            if (st == SARAS_OK) st = saras_mg_solve(h, job.r.x, job.r.b, SARAS_PTR_HOST, nullptr, &c.stats);
             c.status = st;
             c.t_done = now();
             if (st != SARAS_OK && st != SARAS_NOT_CONVERGED)
                 std::strncpy(c.message, saras_last_error(), sizeof(c.message) - 1);
-            saras_mg *drop = nullptr;
+            Handle drop;
             {
                 std::lock_guard<std::mutex> lk(mu);
                 --running;
-                if (h && p.keep_resident && (st == SARAS_OK || st == SARAS_NOT_CONVERGED)) {
+                if (h.h && p.keep_resident && (st == SARAS_OK || st == SARAS_NOT_CONVERGED)) {
                     idle.push_front({job.key, h, job.bytes});
                 } else {
                     drop = h;
@@ -196,7 +284,7 @@ struct saras_server {
                 done.push_back(c);
                 ++finished;
             }
-            if (drop) saras_mg_destroy(drop);
+            drop.destroy();
             cv_done.notify_all();
             cv_work.notify_all();
         }
@@ -220,10 +308,31 @@ saras_status saras_server_submit(saras_server *s, const saras_request *req) {
     const size_t bytes = saras_mg_footprint(&req->grid, &req->mg);
     if (bytes == 0) return SARAS_BAD_SHAPE;
     Pending pd;
+    pd.kind = kPoisson;
     pd.r = *req;
     pd.t_submit = s->now();
     pd.bytes = bytes;
     pd.key = key_of(req->grid, req->mg);
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->queue.push_back(pd);
+        ++s->submitted;
+    }
+    s->cv_work.notify_one();
+    return SARAS_OK;
+}
+
+saras_status saras_server_submit_nse(saras_server *s, const saras_nse_request *req) {
+    if (!s || !req || !req->vx || !req->vy || !req->vz || !req->p || (req->nse.has_T && !req->t) || req->steps < 0)
+        return SARAS_BAD_ARG;
+    const size_t bytes = saras_nse_footprint(&req->grid, &req->mg, &req->nse);
+    if (bytes == 0) return SARAS_BAD_SHAPE;
+    Pending pd;
+    pd.kind = kNse;
+    pd.q = *req;
+    pd.t_submit = s->now();
+    pd.bytes = bytes;
+    pd.key = key_of(req->grid, req->mg, req->nse);
     {
         std::lock_guard<std::mutex> lk(s->mu);
         s->queue.push_back(pd);
@@ -268,8 +377,8 @@ saras_status saras_server_destroy(saras_server *s) {
     }
     s->cv_work.notify_all();
     for (auto &t : s->workers) t.join();
-    for (auto &v : s->idle) saras_mg_destroy(v.h);
-    for (saras_mg *v : s->evicted) saras_mg_destroy(v);
+    for (auto &v : s->idle) v.h.destroy();
+    for (Handle &v : s->evicted) v.destroy();
     delete s;
     return SARAS_OK;
 }

@@ -75,6 +75,12 @@ class Request(C.Structure):
                 ("tag", C.c_longlong)]
 
 
+class NseRequest(C.Structure):
+    _fields_ = [("grid", GridDesc), ("mg", MgParams), ("nse", NseParams), ("steps", C.c_int),
+                ("vx", C.c_void_p), ("vy", C.c_void_p), ("vz", C.c_void_p), ("p", C.c_void_p), ("t", C.c_void_p),
+                ("tag", C.c_longlong)]
+
+
 class Completion(C.Structure):
     _fields_ = [("tag", C.c_longlong), ("status", C.c_int), ("stats", MgStats), ("t_submit", C.c_double),
                 ("t_admit", C.c_double), ("t_done", C.c_double), ("message", C.c_char * 120)]
@@ -90,8 +96,9 @@ EXPORTS = [
     "saras_device_sync", "saras_device_zero", "saras_last_error", "saras_mg_bench_cycles", "saras_mg_bench_sweeps",
     "saras_bench_stream_copy", "saras_nse_create", "saras_nse_physics", "saras_nse_destroy", "saras_nse_set_fields",
     "saras_nse_get_fields", "saras_nse_step", "saras_nse_stage", "saras_nse_rhs", "saras_nse_get_rhs",
-    "saras_nse_solve_field", "saras_field_op", "saras_nse_diag", "saras_nse_bench_steps",
-    "saras_nse_cfl", "saras_nse_sgs_diag", "saras_nse_tsdata", "saras_nse_probe", "saras_nse_probe_noalloc", "saras_mg_set_graphs", "saras_mg_footprint", "saras_server_create", "saras_server_submit",
+    "saras_nse_solve_field", "saras_field_op", "saras_field_vec_op", "saras_nse_diag", "saras_nse_bench_steps",
+    "saras_nse_cfl", "saras_nse_sgs_diag", "saras_nse_tsdata", "saras_nse_probe", "saras_nse_probe_noalloc", "saras_mg_set_graphs", "saras_mg_footprint", "saras_nse_footprint", "saras_nse_set_time", "saras_nse_set_graphs",
+    "saras_server_create", "saras_server_submit", "saras_server_submit_nse",
     "saras_server_poll", "saras_server_drain", "saras_server_clock", "saras_server_mem_used",
     "saras_server_destroy",
 ]
@@ -130,6 +137,11 @@ def lib():
         L.saras_memcpy_h2d.argtypes = [vp, vp, C.c_size_t]
         L.saras_memcpy_d2h.argtypes = [vp, vp, C.c_size_t]
         L.saras_device_zero.argtypes = [vp, C.c_size_t]
+        L.saras_nse_footprint.argtypes = [C.POINTER(GridDesc), C.POINTER(MgParams), C.POINTER(NseParams)]
+        L.saras_nse_footprint.restype = C.c_size_t
+        L.saras_nse_set_time.argtypes = [vp, C.c_double]
+        L.saras_nse_set_graphs.argtypes = [vp, C.c_int]
+        L.saras_server_submit_nse.argtypes = [vp, C.POINTER(NseRequest)]
         L.saras_nse_physics.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, dp, dp, dp, dp]
         L.saras_nse_create.argtypes = [C.POINTER(GridDesc), C.POINTER(MgParams), C.POINTER(NseParams),
                                        C.POINTER(vp)]
@@ -142,6 +154,7 @@ def lib():
         L.saras_nse_get_rhs.argtypes = [vp] + [vp] * 8 + [C.c_int]
         L.saras_nse_solve_field.argtypes = [vp, C.c_int, C.c_double, C.POINTER(C.c_int)]
         L.saras_field_op.argtypes = [vp, C.c_int, vp, vp, C.c_int]
+        L.saras_field_vec_op.argtypes = [vp, C.c_int] + [vp] * 6 + [C.c_int]
         L.saras_nse_diag.argtypes = [vp, dp]
         L.saras_nse_tsdata.argtypes = [vp, dp]
         L.saras_nse_probe.argtypes = [vp, C.c_int, C.POINTER(C.c_int), dp]
@@ -188,6 +201,10 @@ def nse_physics(prob_type, re=1.0, ra=1.0, pr=1.0, rbc_type=2, g=(0.0, 0.0, -1.0
     gv = np.ascontiguousarray(g, dtype=np.float64)
     check(lib().saras_nse_physics(prob_type, re, ra, pr, rbc_type, _dp(gv), C.byref(nu), C.byref(kappa), _dp(buoy)))
     return dict(nu=nu.value, kappa=kappa.value, buoy=tuple(buoy))
+
+
+def nse_footprint(g, mg, prm):
+    return lib().saras_nse_footprint(C.byref(g), C.byref(mg), C.byref(prm))
 
 
 def grid_desc(n, dim=3, length=(1.0, 1.0, 1.0), periodic=(0, 0, 0), gtype=(0, 0, 0),
@@ -427,6 +444,12 @@ class Nse:
             lib().saras_nse_destroy(self.h)
             self.h = C.c_void_p()
 
+    def set_time(self, t):
+        check(lib().saras_nse_set_time(self.h, float(t)))
+
+    def set_graphs(self, enable):
+        check(lib().saras_nse_set_graphs(self.h, int(enable)))
+
     def __del__(self):
         try:
             self.close()
@@ -472,6 +495,15 @@ class Nse:
         out = np.zeros(self.shape) if out is None else np.array(out, dtype=np.float64)
         check(lib().saras_field_op(self.h, op, F.ctypes.data, out.ctypes.data, PTR_HOST))
         return out
+
+    def vec_op(self, op, fx, fy=None, fz=None, out=None):
+        """saras_field_vec_op: 0 divergence, 1 gradient of fx, 2 computeDiff,
+        3 computeNLin; `out` (three arrays) are the initial outputs, copied."""
+        f = [np.ascontiguousarray(a if a is not None else fx, dtype=np.float64) for a in (fx, fy, fz)]
+        o = [np.array(a, dtype=np.float64, copy=True) for a in out] if out is not None else \
+            [np.zeros(self.shape) for _ in range(3)]
+        check(lib().saras_field_vec_op(self.h, op, *[a.ctypes.data for a in f], *[a.ctypes.data for a in o], PTR_HOST))
+        return o
 
     def cfl(self, courant, dt_max):
         dt = C.c_double()
@@ -540,6 +572,21 @@ class Server:
         r.grid, r.mg, r.b, r.x, r.tag = g, p, b.ctypes.data, x.ctypes.data, int(tag)
         self.live[int(tag)] = (b, x)
         check(lib().saras_server_submit(self.h, C.byref(r)))
+
+    def submit_nse(self, g, mg, prm, steps, fields, tag):
+        """A short Navier-Stokes run (saras_server_submit_nse): `fields` maps
+        vx, vy, vz, p (and t with a scalar) to C-contiguous float64 arrays in
+        the caller layout, updated in place to the state after `steps` steps."""
+        names = ("vx", "vy", "vz", "p") + (("t",) if prm.has_T else ())
+        for k in names:
+            a = fields[k]
+            assert a.dtype == np.float64 and a.flags.c_contiguous, k
+        r = NseRequest()
+        r.grid, r.mg, r.nse, r.steps, r.tag = g, mg, prm, int(steps), int(tag)
+        for k in names:
+            setattr(r, k, fields[k].ctypes.data)
+        self.live[int(tag)] = tuple(fields[k] for k in names)
+        check(lib().saras_server_submit_nse(self.h, C.byref(r)))
 
     def poll(self, max_n=256, wait_ms=0):
         buf = (Completion * max_n)()

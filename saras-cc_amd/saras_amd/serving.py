@@ -178,3 +178,84 @@ def summarize(recs):
     return dict(requests=len(recs), converged=ok, seconds=span, solves_per_s=ok / span,
                 requests_per_s=len(recs) / span, p50_ms=float(np.percentile(lat, 50) * 1e3),
                 p99_ms=float(np.percentile(lat, 99) * 1e3), mean_ms=float(lat.mean() * 1e3))
+
+
+# ---- short Navier-Stokes requests -------------------------------------------
+#
+# Two families of short runs of uneven sizes, each `steps` lsRK3 steps:
+#   rbc: Rayleigh-Benard boxes n^3 (periodic x, y; no-slip z walls; z-tanh
+#        beta = 1.5; Ra = 1e7, Pr = 1, rbcType 2; dScheme 2, Morinishi),
+#        T = 1 - z + U(-1e-3, 1e-3), u = 0, P = 1;
+#   tg:  Taylor-Green boxes n^3 (periodic (2 pi)^3, Re = 1600, dScheme 2,
+#        Morinishi), the taylorGreen.cc field plus 1 % seeded noise.
+# n is drawn log-uniformly in [16, 64] and rounded to a multiple of 8.
+NSE_KINDS = ("rbc", "tg")
+
+
+def draw_nse_requests(count, seed=6, n=(16, 64), steps=(2, 5), tag0=100000):
+    rng = np.random.default_rng(seed)
+    out = []
+    for t in range(count):
+        kind = NSE_KINDS[int(rng.integers(0, 2))]
+        m = int(round(math.exp(rng.uniform(math.log(n[0]), math.log(n[1]))) / 8.0)) * 8
+        m = min(max(m, n[0]), n[1])
+        out.append(dict(tag=tag0 + t, kind=kind, n=m, steps=int(rng.integers(steps[0], steps[1] + 1)),
+                        seed=int(rng.integers(0, 2**31))))
+    return out
+
+
+def nse_request_setup(req):
+    """(grid, mg params, nse params, initial fields) of a short run."""
+    from . import FBC_DIRICHLET, GRID_TANH_DOUBLE, level_metrics, nse_params, nse_physics
+
+    n, pad = req["n"], 2
+    sh = (n + 2 * pad,) * 3
+    rng = np.random.default_rng(req["seed"])
+    mg = mg_params(vc_count=20, vc_tol=1e-6, solve_flag=0, mg_tol=1e-2, pre=3, post=6, bc=BC_NEUMANN,
+                   precision=PREC_MIXED, inner_cycles=5, max_refine=10)
+    c = (slice(pad, -pad),) * 3
+    if req["kind"] == "rbc":
+        g = grid_desc(n, periodic=(1, 1, 0), gtype=(0, 0, GRID_TANH_DOUBLE), beta=(1.0, 1.0, 1.5), pad=pad)
+        ph = nse_physics(5, ra=1e7, pr=1.0, rbc_type=2, g=(0.0, 0.0, -1.0))
+        prm = nse_params(dscheme=2, nlscheme=3, has_T=1, nu=ph["nu"], kappa=ph["kappa"], dt=1e-3, cn_tol=1e-6,
+                         buoy=ph["buoy"])
+        z = level_metrics(g, 0, 2)["x"][1:-1]
+        T = np.zeros(sh)
+        T[c] = 1.0 - z[None, None, :] + rng.uniform(-1e-3, 1e-3, (n, n, n))
+        f = dict(vx=np.zeros(sh), vy=np.zeros(sh), vz=np.zeros(sh), p=np.ones(sh), t=T)
+    else:
+        L = 2 * np.pi
+        g = grid_desc(n, periodic=(1, 1, 1), length=(L, L, L), pad=pad)
+        nu = nse_physics(2, re=1600.0)["nu"]
+        prm = nse_params(dscheme=2, nlscheme=3, has_T=0, nu=nu, kappa=nu, dt=1e-3, cn_tol=1e-6,
+                         vbc=[([FBC_DIRICHLET] * 6, [0.0] * 6)] * 3)
+        x = (np.arange(-pad, n + pad) + 0.5) * (L / n)
+        sx, cx = np.sin(x), np.cos(x)
+        u = sx[:, None, None] * (cx[:, None] * cx[None, :])[None, :, :] + 0.01 * rng.uniform(-1, 1, sh)
+        v = -(cx[:, None, None] * (sx[:, None] * cx[None, :])[None, :, :]) + 0.01 * rng.uniform(-1, 1, sh)
+        w = 0.01 * rng.uniform(-1, 1, sh)
+        f = dict(vx=np.ascontiguousarray(u), vy=np.ascontiguousarray(v), vz=w, p=np.zeros(sh), t=np.zeros(sh))
+    return g, mg, prm, f
+
+
+def run_mixed_stream(server, reqs, nse_reqs, rhs, precision):
+    """Submit Poisson requests and short Navier-Stokes runs interleaved, all
+    at once; returns (records, Poisson solutions, NSE final fields) by tag."""
+    recs, sols, outs = {}, {}, {}
+    order = sorted([("p", r) for r in reqs] + [("n", r) for r in nse_reqs], key=lambda kr: kr[1]["tag"] % 7)
+    for kind, r in order:
+        if kind == "p":
+            b = rhs.get(r)
+            x = np.zeros_like(b)
+            sols[r["tag"]] = x
+            server.submit(request_grid(r), request_params(r, precision), b, x, r["tag"])
+        else:
+            g, mg, prm, f = nse_request_setup(r)
+            outs[r["tag"]] = f
+            server.submit_nse(g, mg, prm, r["steps"], f, r["tag"])
+        for c in server.poll(wait_ms=0):
+            recs[c["tag"]] = c
+    server.drain()
+    for c in server.poll(max_n=len(reqs) + len(nse_reqs) + 1):
+        recs[c["tag"]] = c
+    return recs, sols, outs

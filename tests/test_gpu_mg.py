@@ -405,3 +405,32 @@ def test_2d_top_dirichlet_solve_bitexact(gt, tail, monkeypatch):
     assert ss["cycles"] == so["cycles"] and ss["residual"] == so["residual"]
     np.testing.assert_array_equal(xs, xo)
     assert np.array_equal(xs[1:-1, -1], -xs[1:-1, -2])
+
+
+@pytest.mark.parametrize("gt", [(0, 0, 0), (0, 0, 2)], ids=["uniform", "tanh-z"])
+def test_2d_top_dirichlet_mixed_precision(gt):
+    """The probType 9 walls of multigrid_d2 (Neumann, homogeneous Dirichlet top;
+    poisson_d2.cc:641-647) on the mixed-precision path: FP32 split-row cycles
+    inside FP64 refinement reach 1e-10, confirmed by the oracle's computeError
+    with the same walls, and the solution is the FP64 oracle's to 1e-8.  The
+    stretched grid smooths its coarsest level: solving it makes the reference
+    algorithm diverge there."""
+    tol = 1e-10
+    kw = dict(vc_count=200, vc_tol=tol, mg_tol=1e-6, pre=2, post=2, top_dirichlet=1, beta=(0.7, 0.6, 0.8),
+              solve_flag=0 if gt[2] else 1)
+    mo, _ = pair((64, 1, 32), 2, (1, 0, 0), gt, O.BC_NEUMANN, **{k: v for k, v in kw.items() if k != "beta"})
+    gs = S.grid_desc((64, 1, 32), dim=2, periodic=(1, 0, 0), gtype=gt, beta=kw["beta"])
+    ms = S.Multigrid(gs, S.mg_params(bc=S.BC_NEUMANN, precision=S.PREC_MIXED, inner_cycles=3, max_refine=40,
+                                     **{k: v for k, v in kw.items() if k != "beta"}))
+    b = rhs_for(mo, np.random.default_rng(32), False)
+    xs, ss, status = ms.solve(np.zeros_like(b), b)
+    assert status == S.OK and ss["converged"] and ss["refinements"] >= 1, ss
+    mo.set_level(0, xs, b)
+    mo.impose_bc(0, 0)
+    r_cpu = mo.error(0)
+    assert r_cpu < 1.5 * tol and abs(r_cpu - ss["residual"]) <= 1e-3 * ss["residual"], (r_cpu, ss)
+    assert np.array_equal(xs[1:-1, -1], -xs[1:-1, -2])          # the Dirichlet top ghost
+    mo2, _ = pair((64, 1, 32), 2, (1, 0, 0), gt, O.BC_NEUMANN, **{k: v for k, v in kw.items() if k != "beta"})
+    xo, so = mo2.solve(np.zeros_like(b), b, 1)
+    assert so["residual"] < tol
+    assert np.abs(inner(xs) - inner(xo)).max() <= 1e-8 * np.abs(inner(xo)).max()

@@ -1,13 +1,16 @@
 """Planar (2-D, x-z) runs of the time-step path: the reference's PLANAR
 build (lsRK3_d2 / eulerCN_d2 with hydro_d2) restated as one y row whose y
-terms are exact zeros.  Pinned by what the mathematics fixes: the 2-D
+terms are exact zeros.  Bit-exact against the oracle's PLANAR restatement
+(fields with their defined ghost cells, Jacobi counts, pressure solves) for
+both integration schemes, dScheme 1 and 2, the three non-linear terms, walls
+and the plume's open top; also pinned by what the mathematics fixes: the 2-D
 Taylor-Green vortex is an exact Navier-Stokes solution whose kinetic energy
-decays as exp(-4 nu t) in a (2 pi)^2 box, and by a 3-D run of the same flow,
-uniform in y, which must evolve the same x-z fields to the solvers'
-tolerances.  The plume case checks the pressure correction's wall rules."""
+decays as exp(-4 nu t) in a (2 pi)^2 box, and a 3-D run of the same flow,
+uniform in y, evolves the same x-z fields to the solvers' tolerances."""
 import numpy as np
 import pytest
 
+import oracle as O
 import saras_amd as S
 
 pytestmark = pytest.mark.gpu
@@ -70,11 +73,114 @@ def test_planar_equals_y_uniform_3d(nlscheme):
     assert np.max(np.abs(f3["vy"][1:-1, 1:-1, 1:-1])) < 1e-12
 
 
+def planar_pair(n, periodic, gtype=(0, 0, 0), beta=(1.0, 1.0, 1.0), pad=1, mg=None, **kw):
+    mg = mg or dict(vc_count=12, vc_tol=1e-9, solve_flag=0, mg_tol=1e-2, pre=3, post=3)
+    gk = dict(dim=2, periodic=periodic, gtype=gtype, beta=beta, pad=pad)
+    so = O.Nse(O.nse_desc(O.grid_desc(n, **gk), mg=O.mg_params(bc=O.BC_NEUMANN, **mg), **kw))
+    ss = S.Nse(S.grid_desc(n, **gk), S.mg_params(bc=S.BC_NEUMANN, precision=S.PREC_FP64, **mg), S.nse_params(**kw))
+    assert so.shape == ss.shape
+    return so, ss
+
+
+def planar_mask(shape, n, pad, periodic):
+    """Core cells and the ghosts the reference's imposeBCs/syncFaces define on
+    the single y row: x and z periodic pads, the first wall layer."""
+    m = np.zeros(shape, dtype=bool)
+    I, K = np.meshgrid(np.arange(shape[0]) - pad, np.arange(shape[2]) - pad, indexing="ij")
+    ox, oz = (I < 0) | (I >= n[0]), (K < 0) | (K >= n[2])
+    okx = ox & ~oz & (periodic[0] | ((I >= -1) & (I <= n[0])))
+    okz = oz & ~ox & (periodic[2] | ((K >= -1) & (K <= n[2])))
+    m[:, 0, :] = (~ox & ~oz) | okx | okz
+    return m
+
+
+def assert_planar_equal(so, ss, n, pad, periodic, names=("vx", "vz", "p", "t", "pp")):
+    fo, fs = so.get(), ss.get()
+    m = planar_mask(so.shape, n, pad, periodic)
+    for k in names:
+        bad = np.flatnonzero(fo[k][m] != fs[k][m])
+        assert bad.size == 0, (k, bad.size, np.abs(fo[k][m] - fs[k][m]).max())
+    assert np.abs(fs["vy"]).max() == 0
+
+
+PLANAR_CASES = [
+    # (id, n, periodic, gtype, beta, pad, dscheme, nlscheme, scheme)
+    ("rbc-central-rk3", (24, 1, 16), (1, 0, 0), (0, 0, 2), (1.0, 1.0, 1.2), 1, 1, 1, 0),
+    ("rbc-upwind-rk3", (24, 1, 16), (1, 0, 0), (0, 0, 2), (1.0, 1.0, 1.2), 1, 1, 2, 0),
+    ("rbc-morinishi-rk3", (24, 1, 16), (1, 0, 0), (0, 0, 2), (1.0, 1.0, 1.2), 1, 1, 3, 0),
+    ("rbc-morinishi-euler", (24, 1, 16), (1, 0, 0), (0, 0, 2), (1.0, 1.0, 1.2), 1, 1, 3, 1),
+    ("box-d2-central-rk3", (16, 1, 16), (0, 0, 0), (2, 0, 1), (1.1, 1.0, 0.9), 2, 2, 1, 0),
+    ("box-d2-upwind-euler", (16, 1, 16), (0, 0, 0), (2, 0, 1), (1.1, 1.0, 0.9), 2, 2, 2, 1),
+    ("periodic-d2-morinishi-rk3", (16, 1, 24), (1, 0, 1), (0, 0, 0), (1.0, 1.0, 1.0), 2, 2, 3, 0),
+]
+
+
+@pytest.mark.parametrize("case", PLANAR_CASES, ids=[c[0] for c in PLANAR_CASES])
+def test_planar_steps_bitexact(case):
+    """Three planar steps against the oracle's PLANAR restatement (lsRK3_d2 /
+    eulerCN_d2, vfield/sfield PLANAR branches, predictors.cc PLANAR Jacobi):
+    a buoyant scalar with a warm bottom and seeded perturbations."""
+    _, n, periodic, gtype, beta, pad, dscheme, nlscheme, scheme = case
+    so, ss = planar_pair(n, periodic, gtype, beta, pad, dscheme=dscheme, nlscheme=nlscheme, scheme=scheme, has_T=1,
+                         nu=2e-3, kappa=3e-3, dt=2e-3, cn_tol=1e-9, buoy=(0.0, 0.0, -1.0))
+    rng = np.random.default_rng(21)
+    sh = so.shape
+    c = (slice(pad, -pad), 0, slice(pad, -pad))
+    f = {k: np.zeros(sh) for k in ("vx", "vy", "vz", "p", "t", "pp")}
+    f["t"][c] = 0.5 + 0.05 * rng.standard_normal((n[0], n[2]))
+    f["vx"][c] = 0.1 * rng.standard_normal((n[0], n[2]))
+    f["vz"][c] = 0.1 * rng.standard_normal((n[0], n[2]))
+    so.set(**f)
+    ss.set(**f)
+    assert_planar_equal(so, ss, n, pad, periodic)
+    for step in range(3):
+        assert so.step() == 0
+        r, st = ss.step()
+        assert r == S.OK
+        assert so.jacobi_iters() == st["jacobi_iters"], step
+        assert so.mg_stats()["cycles"] == st["mg"]["cycles"]
+        assert so.mg_stats()["residual"] == st["mg"]["residual"]
+        assert_planar_equal(so, ss, n, pad, periodic)
+    do, ds = so.diag(), ss.diag()
+    assert do["divmax"] == ds["divmax"]
+    for k in ("ke", "nuvol", "nuwall"):
+        assert abs(do[k] - ds[k]) <= 1e-12 * abs(do[k]), (k, do[k], ds[k])
+
+
+def test_planar_plume_bitexact():
+    """The planar plume (probType 9): plumeOut top wall of T and the Dirichlet
+    top of the pressure-correction solve (poisson_d2.cc:641-647), three steps
+    bit for bit with the oracle, ghosts of Pp included."""
+    n, pad = (24, 1, 24), 1
+    D, N = S.FBC_DIRICHLET, S.FBC_NEUMANN
+    vbc = [([D] * 6, [0.0] * 6), ([D] * 6, [0.0] * 6), ([D] * 5 + [N], [0.0] * 6)]
+    tbc = ([N] * 4 + [D, S.FBC_PLUME_OUT], [0.0] * 4 + [1.0, 0.0])
+    so, ss = planar_pair(n, (1, 0, 0), (0, 0, 1), (1.0, 1.0, 0.8), pad, dscheme=1, nlscheme=1, has_T=1, nu=1e-2,
+                         kappa=1e-2, dt=1e-3, cn_tol=1e-10, buoy=(0.0, 0.0, -1.0), vbc=vbc, tbc=tbc)
+    rng = np.random.default_rng(2)
+    sh = so.shape
+    f = {k: np.zeros(sh) for k in ("vx", "vy", "vz", "p", "t", "pp")}
+    f["t"][pad:-pad, 0, pad:-pad] = rng.uniform(0.0, 0.1, (n[0], n[2]))
+    so.set(**f)
+    ss.set(**f)
+    for step in range(3):
+        assert so.step() == 0
+        r, st = ss.step()
+        assert r == S.OK
+        assert so.jacobi_iters() == st["jacobi_iters"], step
+        assert_planar_equal(so, ss, n, pad, (1, 0, 0))
+
+
 def test_planar_rejects_unsupported_schemes():
-    g = S.grid_desc((16, 1, 16), dim=2, periodic=(1, 1, 1), pad=2)
     mg = S.mg_params(precision=S.PREC_FP64, bc=S.BC_NEUMANN)
+    # dScheme 2 Morinishi with z walls: the last z cell's 4th-order stencil
+    # reads the second ghost layer (the edge test reads core.ubound(1) = 0)
+    g2 = S.grid_desc((16, 1, 16), dim=2, periodic=(1, 1, 0), pad=2)
     with pytest.raises(S.SarasError):
-        S.Nse(g, mg, S.nse_params(dscheme=2, nlscheme=1, has_T=0, nu=0.01, kappa=0.01, dt=1e-3))
+        S.Nse(g2, mg, S.nse_params(dscheme=2, nlscheme=3, has_T=0, nu=0.01, kappa=0.01, dt=1e-3))
+    gp = S.grid_desc((16, 1, 16), dim=2, periodic=(1, 1, 1), pad=2)
+    with pytest.raises(S.SarasError):   # the LES model is 3-D only
+        S.Nse(gp, mg, S.nse_params(dscheme=2, nlscheme=3, has_T=0, nu=0.01, kappa=0.01, dt=1e-3, les=1))
     gw = S.grid_desc((16, 1, 16), dim=2, periodic=(1, 1, 0), pad=1)
     plate = ([S.FBC_NEUMANN] * 4 + [S.FBC_HOT_PLATE, S.FBC_PLUME_OUT], [0.0] * 6)
     with pytest.raises(S.SarasError):   # PLANAR builds have no heating plate

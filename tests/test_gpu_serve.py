@@ -151,3 +151,48 @@ def test_full_size_mix_alone_equals_in_the_mix():
         e = oracle_error(r, sols[r["tag"]], rhs.get(r))
         assert e < 1e-8, (r, e)
         assert abs(e - rec["stats"]["residual"]) <= 1e-12 * e, (r, e, rec["stats"]["residual"])
+
+
+def solo_nse(req):
+    """A short run on its own plain handle: set, steps, get."""
+    g, mg, prm, f = V.nse_request_setup(req)
+    h = S.Nse(g, mg, prm)
+    h.set(**f, pp=np.zeros_like(f["vx"]))
+    for _ in range(req["steps"]):
+        r, _ = h.step()
+        assert r == S.OK
+    out = h.get()
+    h.close()
+    return out
+
+
+def test_nse_requests_alone_equal_in_the_mix():
+    """Short Navier-Stokes runs (Rayleigh-Benard and Taylor-Green boxes of
+    16^3-64^3, 2-5 steps) interleaved with Poisson solves in the batched
+    loop: every run's fields are bit-identical to the same run alone on a
+    plain handle; a repeat of the first runs reuses their resident handles
+    (run time and pressure correction reset) with the same results."""
+    reqs = small_mix(count=12, seed=5)
+    nreqs = V.draw_nse_requests(10, seed=6)
+    assert {r["kind"] for r in nreqs} == {"rbc", "tg"}
+    rhs = V.RhsCache()
+    srv = S.Server(mem_budget=16 << 30, slots=8, use_graphs=1, largest_first=1)
+    recs, sols, outs = V.run_mixed_stream(srv, reqs, nreqs, rhs, S.PREC_MIXED)
+    # the same runs again: each reuses the idle handle its first run left
+    again = [dict(r, tag=r["tag"] + 500) for r in nreqs[:4]]
+    recs2, _, outs2 = V.run_mixed_stream(srv, [], again, rhs, S.PREC_MIXED)
+    srv.close()
+    for r in reqs:
+        assert recs[r["tag"]]["status"] == S.OK, (r, recs[r["tag"]])
+    core = (slice(2, -2),) * 3
+    for r in nreqs:
+        print(f"nse request {r['tag']}: {r['kind']} n={r['n']} steps={r['steps']}", flush=True)
+        assert recs[r["tag"]]["status"] == S.OK, (r, recs[r["tag"]])
+        ref = solo_nse(r)
+        names = ("vx", "vy", "vz", "p") + (("t",) if r["kind"] == "rbc" else ())
+        for k in names:
+            assert np.array_equal(outs[r["tag"]][k][core], ref[k][core]), (r, k)
+    for r, r0 in zip(again, nreqs[:4]):
+        assert recs2[r["tag"]]["status"] == S.OK
+        for k in ("vx", "vy", "vz", "p"):
+            assert np.array_equal(outs2[r["tag"]][k][core], outs[r0["tag"]][k][core]), (r, k)

@@ -2,8 +2,8 @@
 their decks' settings by the FP64 CPU oracle (oracle/, built by
 `make -C oracle`).  Run from the repository root:
 
-    python tests/golden/make_golden.py rbc256      # about 10 minutes on 8 cores
-    python tests/golden/make_golden.py tgv_les64   # about 10 minutes on 8 cores
+    python tests/golden/make_golden.py rbc256      # about 3 minutes on 8 cores
+    python tests/golden/make_golden.py tgv_les64   # about 15 minutes on 8 cores
 
 rbc256 -> tests/golden/rbc256_steps.json
     3-D Rayleigh-Benard convection, 256^3 cells, periodic x and y, no-slip

@@ -434,3 +434,4 @@ def test_2d_top_dirichlet_mixed_precision(gt):
     xo, so = mo2.solve(np.zeros_like(b), b, 1)
     assert so["residual"] < tol
     assert np.abs(inner(xs) - inner(xo)).max() <= 1e-8 * np.abs(inner(xo)).max()
+

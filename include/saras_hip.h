@@ -191,8 +191,17 @@ saras_status saras_bench_stream_copy(size_t n, int reps, double *ms);
  *                        = T of the cell below where the ghost Vz > 0, else 0.
  *                        In planar runs the pressure correction's top wall then
  *                        turns Dirichlet, as in multigrid_d2
- *                        (poisson_d2.cc:641-647). */
-enum { SARAS_FBC_DIRICHLET = 0, SARAS_FBC_NEUMANN = 1, SARAS_FBC_HOT_PLATE = 2, SARAS_FBC_PLUME_OUT = 3 };
+ *                        (poisson_d2.cc:641-647).
+ * One kind serves the pressure of the inflow problem (probType 3,
+ * hydro.cc:466-468):
+ *   SARAS_FBC_NULL       nullBC (boundary.h:205-212): imposeBC leaves the
+ *                        ghost layer as it stands, so the wall's first ghost
+ *                        layer keeps the caller's initial P ghost plus every
+ *                        pressure correction's ghost (P += Pp over the whole
+ *                        array: lsRK3_d3.cc:289, sfield.cc:565-569).  Pressure
+ *                        walls only. */
+enum { SARAS_FBC_DIRICHLET = 0, SARAS_FBC_NEUMANN = 1, SARAS_FBC_HOT_PLATE = 2, SARAS_FBC_PLUME_OUT = 3,
+       SARAS_FBC_NULL = 4 };
 
 typedef struct {
     int dscheme;             /* "Differentiation Scheme": 1 (2nd order) or 2 (4th order) */
@@ -237,6 +246,18 @@ typedef struct saras_nse saras_nse;
  * reference aborts (an rbc_type outside 1..4) or divides by zero. */
 saras_status saras_nse_physics(int prob_type, double re, double ra, double pr, int rbc_type,
                                const double g_axis[3], double *nu, double *kappa, double buoy[3]);
+
+/* The wall kinds and values of V, P and T that the drivers wire for a
+ * "Problem Type" when "Use Custom BC" is off: hydro::initVBCs
+ * (hydro.cc:349-434: lid-driven cavity 1 with Vx = 1 on the top wall,
+ * inflow 3 with Vx = 1 on x-lo and zero-gradient outflow on x-hi,
+ * free-slip 8, plume 9, no-slip otherwise), hydro::initPBCs
+ * (hydro.cc:447-484: nullBC on x-lo for 3, zero gradient elsewhere) and
+ * scalar::initTBCs (scalar.cc:163-201: by probType 5..9, the hot plate on
+ * the bottom wall when non_hg is set in 3-D, plumeOut on the top for 9).
+ * Only the type and value tables of *params are written.  SARAS_BAD_ARG
+ * for a prob_type outside 1..9. */
+saras_status saras_nse_wiring(int prob_type, int planar, int non_hg, saras_nse_params *params);
 
 /* lsRK3_d3::lsRK3_d3 + timestep::timestep (lsRK3_d3.cc:58-70,
  * timestep.cc:57-122) with the fields of hydro/scalar (vfield, sfield).

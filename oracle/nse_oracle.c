@@ -150,9 +150,11 @@ static void sync_faces(or_nse *s, double *F)
 }
 
 /* one wall (first ghost layer over the full extent of the other axes):
- * dirichlet.cc:72-76, neumann.cc:89-93 */
+ * dirichlet.cc:72-76, neumann.cc:89-93; nullBC::imposeBC (boundary.h:205-212)
+ * is empty */
 static void wall_bc(or_nse *s, double *F, int wall, int type, double val)
 {
+    if (type == OR_FBC_NULL) return;
     int p = s->p, n[3] = {s->n[0], s->n[1], s->n[2]};
     int ax = wall / 2, hi = wall & 1;
     int g = hi ? n[ax] : -1, src = hi ? n[ax] - 1 : 0;

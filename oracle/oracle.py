@@ -39,7 +39,7 @@ class MgParams(C.Structure):
                 ("no_early_exit", C.c_int), ("top_dirichlet", C.c_int)]
 
 
-FBC_DIRICHLET, FBC_NEUMANN, FBC_HOT_PLATE, FBC_PLUME_OUT = 0, 1, 2, 3
+FBC_DIRICHLET, FBC_NEUMANN, FBC_HOT_PLATE, FBC_PLUME_OUT, FBC_NULL = 0, 1, 2, 3, 4
 
 
 class NseDesc(C.Structure):

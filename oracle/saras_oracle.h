@@ -123,8 +123,9 @@ void or_mg_residual(or_mg *m, int level, int zero_bc, double *out_pad);
  * numbered 0 x-lo (Lft), 1 x-hi (Rgt), 2 y-lo (Frn), 3 y-hi (Bak),
  * 4 z-lo (Bot), 5 z-hi (Top); periodic axes ignore theirs. */
 /* wall kinds: dirichlet.cc, neumann.cc; for T also hotPlate (bottom wall,
- * hotPlate.cc) and plumeOut (top wall, plumeOut.cc) */
-enum { OR_FBC_DIRICHLET = 0, OR_FBC_NEUMANN = 1, OR_FBC_HOT_PLATE = 2, OR_FBC_PLUME_OUT = 3 };
+ * hotPlate.cc) and plumeOut (top wall, plumeOut.cc); for P also nullBC
+ * (boundary.h:205-212), whose imposeBC writes nothing */
+enum { OR_FBC_DIRICHLET = 0, OR_FBC_NEUMANN = 1, OR_FBC_HOT_PLATE = 2, OR_FBC_PLUME_OUT = 3, OR_FBC_NULL = 4 };
 
 typedef struct {
     or_grid_desc g;        /* field grid; g.pad = 1 (dScheme 1) or 2 (dScheme 2) */

@@ -225,4 +225,63 @@ saras_status saras_nse_physics(int prob_type, double re, double ra, double pr, i
     return SARAS_OK;
 }
 
+// The drivers' wall wiring without "Use Custom BC".  Walls: 0 Left (x-lo),
+// 1 Right, 2 Front (y-lo), 3 Back, 4 Bottom (z-lo), 5 Top.
+saras_status saras_nse_wiring(int prob_type, int planar, int non_hg, saras_nse_params *params) {
+    if (!params || prob_type < 1 || prob_type > 9) return SARAS_BAD_ARG;
+    const bool scalar = prob_type >= 5;
+    // PLANAR builds print a warning for the heating plate and wire no bottom
+    // wall for T at all (scalar.cc:178-185): no run to mirror
+    if (scalar && prob_type != 6 && prob_type != 7 && non_hg && planar) return SARAS_BAD_ARG;
+    saras_nse_params &p = *params;
+    auto set = [](int *type, double *val, int w, int t, double v) {
+        type[w] = t;
+        val[w] = v;
+    };
+    const int D = SARAS_FBC_DIRICHLET, N = SARAS_FBC_NEUMANN;
+    // hydro::initVBCs (hydro.cc:349-434): no-slip / no-penetration by default
+    for (int c = 0; c < 3; ++c)
+        for (int w = 0; w < 6; ++w) set(p.vbc_type[c], p.vbc_val[c], w, D, 0.0);
+    int *ut = p.vbc_type[0], *vt = p.vbc_type[1], *wt = p.vbc_type[2];
+    double *uv = p.vbc_val[0], *vv = p.vbc_val[1], *wv = p.vbc_val[2];
+    if (prob_type == 3) {   // inflow on the left, outflow on the right
+        set(ut, uv, 0, D, 1.0);
+        set(ut, uv, 1, N, 0.0);
+        set(vt, vv, 1, N, 0.0);
+        set(wt, wv, 1, N, 0.0);
+    }
+    if (prob_type == 1) set(ut, uv, 5, D, 1.0);   // lid-driven cavity
+    if (prob_type == 8)                            // free-slip bottom and top
+        for (int w = 4; w < 6; ++w) {
+            set(ut, uv, w, N, 0.0);
+            set(vt, vv, w, N, 0.0);
+        }
+    if (prob_type == 9) {   // plume: open side walls and top for Vz
+        set(wt, wv, 0, N, 0.0);
+        set(wt, wv, 1, N, 0.0);
+        set(wt, wv, 5, N, 0.0);
+    }
+    // hydro::initPBCs (hydro.cc:447-484)
+    for (int w = 0; w < 6; ++w) set(p.pbc_type, p.pbc_val, w, N, 0.0);
+    if (prob_type == 3) set(p.pbc_type, p.pbc_val, 0, SARAS_FBC_NULL, 0.0);
+    // scalar::initTBCs (scalar.cc:163-201); hydrodynamics runs carry no T and
+    // get zero-gradient walls.  probType 7 wires no bottom or top wall (its
+    // runs need a periodic z axis); those walls are zero-gradient here.
+    for (int w = 0; w < 6; ++w) set(p.tbc_type, p.tbc_val, w, N, 0.0);
+    if (prob_type == 7) {
+        set(p.tbc_type, p.tbc_val, 0, D, 1.0);
+        set(p.tbc_type, p.tbc_val, 1, D, 0.0);
+    }
+    if (prob_type == 5 || prob_type == 8 || prob_type == 9) {
+        if (non_hg) set(p.tbc_type, p.tbc_val, 4, SARAS_FBC_HOT_PLATE, 0.0);
+        else set(p.tbc_type, p.tbc_val, 4, D, 1.0);
+        if (prob_type == 9) set(p.tbc_type, p.tbc_val, 5, SARAS_FBC_PLUME_OUT, 0.0);
+        else set(p.tbc_type, p.tbc_val, 5, D, 0.0);
+    } else if (prob_type == 6) {
+        set(p.tbc_type, p.tbc_val, 4, D, 0.0);
+        set(p.tbc_type, p.tbc_val, 5, D, 1.0);
+    }
+    return SARAS_OK;
+}
+
 }  // extern "C"

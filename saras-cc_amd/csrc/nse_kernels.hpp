@@ -14,7 +14,9 @@
 //   Neumann wall   ghost = F(edge) - v h_wall
 // Every stencil of the step stays within one ghost layer on a wall axis
 // (first and last cells use second-order stencils), so the second layer the
-// reference leaves stale is never needed.
+// reference leaves stale is never needed.  The one stored ghost layer is
+// that of a nullBC pressure wall (boundary.h:205-212), whose ghosts no rule
+// gives: they keep what the whole-array updates of P leave there.
 //
 // Floating point: the expressions below keep the operand order of the cited
 // reference lines and are built with -ffp-contract=off, so the FP64 results
@@ -29,10 +31,20 @@ constexpr double kRecip12 = 1.0 / 12.0;
 
 // wall rules of one field (6 walls: x-lo, x-hi, y-lo, y-hi, z-lo, z-hi)
 struct FieldBC {
-    int type[6];     // 0 Dirichlet, 1 Neumann, 2 hot plate (z-lo), 3 plume outflow (z-hi)
+    int type[6];     // 0 Dirichlet, 1 Neumann, 2 hot plate (z-lo), 3 plume outflow (z-hi), 4 null (stored ghosts)
     double val[6];   // Dirichlet value
     double kh[6];    // Neumann value * h_wall (neumann.cc:59-75)
+    // kind 4 (nullBC, boundary.h:205-212): the wall's first ghost layer as
+    // stored values, indexed by the two other core indices (x walls: j nz + k,
+    // y walls: i nz + k, z walls: i ny + j)
+    double *ghost[6];
 };
+
+// index of the ghost of core cell (c0, c1, c2) across a wall of axis ax in
+// that wall's stored plane
+__device__ __forceinline__ long ghost_plane_index(const int *n, int ax, int c0, int c1, int c2) {
+    return ax == 0 ? (long)c1 * n[2] + c2 : (ax == 1 ? (long)c0 * n[2] + c2 : (long)c0 * n[1] + c1);
+}
 
 struct Geom {
     int n[3];
@@ -100,6 +112,10 @@ struct CoreField {
         const double val = hi ? bc.val[2 * ax + 1] : bc.val[2 * ax];
         const double kh = hi ? bc.kh[2 * ax + 1] : bc.kh[2 * ax];
         c[ax] = hi ? n - 1 : 0;
+        if (type == 4) {
+            const double *gp = hi ? bc.ghost[2 * ax + 1] : bc.ghost[2 * ax];
+            return gp[ghost_plane_index(G.n, ax, c[0], c[1], c[2])];
+        }
         const double inner = f[(long)c[0] * G.sx + (long)c[1] * G.sy + c[2]];
         if (type > 1) return z_wall_mask(G, type, hi, c[0], c[1], inner);
         return type == 0 ? 2.0 * val - inner : inner - kh;
@@ -823,7 +839,7 @@ __global__ void __launch_bounds__(256) k_div(Geom G, CoreField V0, CoreField V1,
 
 // P += Pp; V -= ((alpha+beta) dt) grad Pp   (lsRK3_d3.cc:285-295)
 template <class PF>
-__global__ void __launch_bounds__(256) k_correct(Geom G, PF Pp, double *__restrict__ P,
+__global__ void __launch_bounds__(256) k_correct(Geom G, PF Pp, double *__restrict__ P, FieldBC pb,
                                                  double *__restrict__ V0, double *__restrict__ V1,
                                                  double *__restrict__ V2, double pc) {
     int i, j, k;
@@ -831,6 +847,16 @@ __global__ void __launch_bounds__(256) k_correct(Geom G, PF Pp, double *__restri
     if (k >= G.n[2]) return;
     const long idx = (long)i * G.sx + (long)j * G.sy + k;
     P[idx] += Pp.at(G, i, j, k);
+    // P += Pp covers the ghost layers too (sfield.cc:565-569): a nullBC wall of
+    // P keeps that sum, so the ghost beside a wall cell takes Pp's ghost there
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        const int q = pick(ax, i, j, k);
+        if (q == 0 && pb.type[2 * ax] == 4)
+            pb.ghost[2 * ax][ghost_plane_index(G.n, ax, i, j, k)] += Pp.nb(G, ax, i, j, k, -1);
+        if (q == G.n[ax] - 1 && pb.type[2 * ax + 1] == 4)
+            pb.ghost[2 * ax + 1][ghost_plane_index(G.n, ax, i, j, k)] += Pp.nb(G, ax, i, j, k, 1);
+    }
     double g0, g1, g2;
     if (interior2(G, i, j, k)) {
         const auto Pf = fast_of(Pp);

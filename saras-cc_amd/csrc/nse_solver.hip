@@ -60,6 +60,7 @@ struct saras_nse {
     double *metrics = nullptr;          // 9 core metric arrays
     FieldBC bc[4]{};                     // Vx, Vy, Vz, T
     FieldBC pbc{};                       // P
+    double *pghost[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};   // nullBC walls of P
     FieldBC ppbc{};                      // Pp: the Poisson solver's walls (Neumann; a planar plume's top: Dirichlet)
     double *F[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};   // Vx, Vy, Vz, T, P
     double *H[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -114,6 +115,7 @@ struct saras_nse {
         (void)hipFree(stage_a);
         (void)hipFree(stage_b);
         for (auto *q : Tsg) (void)hipFree(q);
+        for (auto *q : pghost) (void)hipFree(q);
         (void)hipFree(sgs_part);
         (void)hipFree(sgs_dev);
         (void)hipHostFree(sgs_host);
@@ -123,6 +125,11 @@ struct saras_nse {
     }
 
     int stages() const { return p.scheme == 1 ? 1 : 3; }
+    // cells of wall w's ghost plane: the core extents of the two other axes
+    size_t plane_cells(int w) const {
+        const int ax = w / 2;
+        return ax == 0 ? (size_t)G.n[1] * G.n[2] : (ax == 1 ? (size_t)G.n[0] * G.n[2] : (size_t)G.n[0] * G.n[1]);
+    }
     dim3 grid3() const { return dim3((G.n[2] + threads - 1) / threads, G.n[1], G.n[0]); }
     dim3 grid3_full(int p) const {
         return dim3((G.n[2] + 2 * p + threads - 1) / threads, G.n[1] + 2 * ypad(), G.n[0] + 2 * p);
@@ -303,10 +310,10 @@ saras_status stage_run(saras_nse *s, int stage) {
     const saras::nse::SplitField pp = s->pp();
     if (pp.hz)
         hipLaunchKernelGGL(saras::nse::k_correct<saras::nse::SplitField>, s->grid3(), dim3(s->threads), 0, s->s, s->G,
-                           pp, s->F[4], s->F[0], s->F[1], s->F[2], pc);
+                           pp, s->F[4], s->pbc, s->F[0], s->F[1], s->F[2], pc);
     else
         hipLaunchKernelGGL(saras::nse::k_correct<CoreField>, s->grid3(), dim3(s->threads), 0, s->s, s->G,
-                           CoreField{pp.f, pp.bc}, s->F[4], s->F[0], s->F[1], s->F[2], pc);
+                           CoreField{pp.f, pp.bc}, s->F[4], s->pbc, s->F[0], s->F[1], s->F[2], pc);
     NSE_TRY(hipGetLastError());
     // V.imposeBCs(); P.imposeBCs(); T.imposeBCs() closing the stage
     return plume_flags(s);
@@ -361,6 +368,56 @@ saras_status load_field(saras_nse *s, const double *src, int ptr_kind, double *d
     const long SY = s->g.n[2] + 2 * p, SX = SY * (s->G.n[1] + 2 * py);
     hipLaunchKernelGGL(saras::nse::k_field_load, s->grid3(), dim3(s->threads), 0, s->s, d, SX, SY, p, py, dst,
                        s->G.sx, s->G.sy, hz, s->G.n[0], s->G.n[1], s->G.n[2]);
+    NSE_TRY(hipGetLastError());
+    if (ptr_kind == SARAS_PTR_HOST) NSE_TRY(hipStreamSynchronize(s->s));
+    return SARAS_OK;
+}
+
+// the first ghost layer of wall w of a caller array (pad p) -> its plane
+__global__ void k_ghost_plane_load(const double *__restrict__ src, long SX, long SY, int p, int py, int w, int nx,
+                                   int ny, int nz, double *__restrict__ dst) {
+    const int ax = w / 2, hi = w & 1;
+    const int na = ax == 0 ? ny : nx, nb = ax == 2 ? ny : nz;   // plane extents, fastest last
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)na * nb) return;
+    const int a = (int)(t / nb), b = (int)(t - (long)a * nb);
+    int c[3];
+    if (ax == 0) {
+        c[0] = hi ? nx : -1;
+        c[1] = a;
+        c[2] = b;
+    } else if (ax == 1) {
+        c[0] = a;
+        c[1] = hi ? ny : -1;
+        c[2] = b;
+    } else {
+        c[0] = a;
+        c[1] = b;
+        c[2] = hi ? nz : -1;
+    }
+    dst[t] = src[(long)(c[0] + p) * SX + (long)(c[1] + py) * SY + (c[2] + p)];
+}
+
+// P's nullBC walls take the caller's first ghost layer
+saras_status load_ghost_planes(saras_nse *s, const double *src, int ptr_kind) {
+    bool any = false;
+    for (int w = 0; w < 6; ++w) any = any || s->pghost[w];
+    if (!any) return SARAS_OK;
+    const double *d = src;
+    if (ptr_kind == SARAS_PTR_HOST) {
+        saras_status e = ensure_stage(s);
+        if (e) return e;
+        NSE_TRY(hipMemcpyAsync(s->stage_a, src, s->full * sizeof(double), hipMemcpyHostToDevice, s->s));
+        d = s->stage_a;
+    }
+    const int p = s->g.pad, py = s->ypad();
+    const long SY = s->g.n[2] + 2 * p, SX = SY * (s->G.n[1] + 2 * py);
+    for (int w = 0; w < 6; ++w) {
+        if (!s->pghost[w]) continue;
+        const long cells = (long)s->plane_cells(w);
+        hipLaunchKernelGGL(k_ghost_plane_load, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, s->s, d, SX, SY,
+                           p, py, w, s->G.n[0], s->G.n[1], s->G.n[2], s->pghost[w]);
+    }
     NSE_TRY(hipGetLastError());
     if (ptr_kind == SARAS_PTR_HOST) NSE_TRY(hipStreamSynchronize(s->s));
     return SARAS_OK;
@@ -439,7 +496,9 @@ saras_status saras_nse_create(const saras_grid_desc *grid, const saras_mg_params
         return SARAS_BAD_SHAPE;
     }
     for (int w = 0; w < 6; ++w) {
-        bool ok = P.pbc_type[w] == SARAS_FBC_DIRICHLET || P.pbc_type[w] == SARAS_FBC_NEUMANN;
+        // nullBC: a pressure wall of a wall axis (the planar y walls are none)
+        const bool null_ok = P.pbc_type[w] == SARAS_FBC_NULL && !g.periodic[w / 2] && !(planar && w / 2 == 1);
+        bool ok = P.pbc_type[w] == SARAS_FBC_DIRICHLET || P.pbc_type[w] == SARAS_FBC_NEUMANN || null_ok;
         for (int c = 0; c < 3; ++c)
             ok = ok && (P.vbc_type[c][w] == SARAS_FBC_DIRICHLET || P.vbc_type[c][w] == SARAS_FBC_NEUMANN);
         const int t = P.tbc_type[w];
@@ -449,7 +508,7 @@ saras_status saras_nse_create(const saras_grid_desc *grid, const saras_mg_params
         ok = ok && (t == SARAS_FBC_DIRICHLET || t == SARAS_FBC_NEUMANN || (special && P.has_T && !g.periodic[2]));
         if (!ok) {
             saras::set_last_error("wall kinds: Dirichlet or Neumann; for T of a run with z walls also the hot "
-                                  "plate (bottom, 3-D) and the plume outflow (top)");
+                                  "plate (bottom, 3-D) and the plume outflow (top); for P on a wall axis also null");
             return SARAS_BAD_ARG;
         }
     }
@@ -591,6 +650,13 @@ saras_status saras_nse_create(const saras_grid_desc *grid, const saras_mg_params
         s->plume = P.tbc_type[5] == SARAS_FBC_PLUME_OUT;
     }
     s->pbc = make_bc(P.pbc_type, P.pbc_val, hwall);
+    for (int w = 0; w < 6; ++w) {
+        if (P.pbc_type[w] != SARAS_FBC_NULL) continue;
+        const size_t cells = s->plane_cells(w);
+        if (hipMalloc((void **)&s->pghost[w], cells * sizeof(double)) != hipSuccess) return fail(SARAS_OOM);
+        (void)hipMemset(s->pghost[w], 0, cells * sizeof(double));
+        s->pbc.ghost[w] = s->pghost[w];
+    }
     {
         // Pp's ghosts as the solver's last imposeBC leaves them: Neumann, and
         // -Pp(n-1) at a Dirichlet top
@@ -702,6 +768,10 @@ saras_status saras_nse_set_fields(saras_nse *s, const double *vx, const double *
     for (int q = 0; q < 5; ++q) {
         if (!src[q]) continue;
         saras_status e = load_field(s, src[q], ptr_kind, s->F[q]);
+        if (e) return e;
+    }
+    if (p) {
+        saras_status e = load_ghost_planes(s, p, ptr_kind);
         if (e) return e;
     }
     if (pp) {

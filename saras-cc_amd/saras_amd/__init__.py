@@ -47,7 +47,7 @@ class MgStats(C.Structure):
                 ("coarse_capped", C.c_int), ("device_ms", C.c_double)]
 
 
-FBC_DIRICHLET, FBC_NEUMANN, FBC_HOT_PLATE, FBC_PLUME_OUT = 0, 1, 2, 3
+FBC_DIRICHLET, FBC_NEUMANN, FBC_HOT_PLATE, FBC_PLUME_OUT, FBC_NULL = 0, 1, 2, 3, 4
 
 
 class NseParams(C.Structure):
@@ -94,7 +94,7 @@ EXPORTS = [
     "saras_mg_residual_restrict", "saras_mg_prolong_correct", "saras_mg_error",
     "saras_device_alloc", "saras_device_free", "saras_memcpy_h2d", "saras_memcpy_d2h",
     "saras_device_sync", "saras_device_zero", "saras_last_error", "saras_mg_bench_cycles", "saras_mg_bench_sweeps",
-    "saras_bench_stream_copy", "saras_nse_create", "saras_nse_physics", "saras_nse_destroy", "saras_nse_set_fields",
+    "saras_bench_stream_copy", "saras_nse_create", "saras_nse_physics", "saras_nse_wiring", "saras_nse_destroy", "saras_nse_set_fields",
     "saras_nse_get_fields", "saras_nse_step", "saras_nse_stage", "saras_nse_rhs", "saras_nse_get_rhs",
     "saras_nse_solve_field", "saras_field_op", "saras_field_vec_op", "saras_nse_diag", "saras_nse_bench_steps",
     "saras_nse_cfl", "saras_nse_sgs_diag", "saras_nse_tsdata", "saras_nse_probe", "saras_nse_probe_noalloc", "saras_mg_set_graphs", "saras_mg_footprint", "saras_nse_footprint", "saras_nse_set_time", "saras_nse_set_graphs",
@@ -143,6 +143,7 @@ def lib():
         L.saras_nse_set_graphs.argtypes = [vp, C.c_int]
         L.saras_server_submit_nse.argtypes = [vp, C.POINTER(NseRequest)]
         L.saras_nse_physics.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, dp, dp, dp, dp]
+        L.saras_nse_wiring.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(NseParams)]
         L.saras_nse_create.argtypes = [C.POINTER(GridDesc), C.POINTER(MgParams), C.POINTER(NseParams),
                                        C.POINTER(vp)]
         L.saras_nse_destroy.argtypes = [vp]
@@ -201,6 +202,17 @@ def nse_physics(prob_type, re=1.0, ra=1.0, pr=1.0, rbc_type=2, g=(0.0, 0.0, -1.0
     gv = np.ascontiguousarray(g, dtype=np.float64)
     check(lib().saras_nse_physics(prob_type, re, ra, pr, rbc_type, _dp(gv), C.byref(nu), C.byref(kappa), _dp(buoy)))
     return dict(nu=nu.value, kappa=kappa.value, buoy=tuple(buoy))
+
+
+def nse_wiring(prob_type, planar=False, non_hg=False):
+    """The drivers' wall kinds and values for a "Problem Type" (hydro.cc:349-484,
+    scalar.cc:163-201), as the vbc / tbc / pbc arguments of nse_params."""
+    d = NseParams()
+    check(lib().saras_nse_wiring(prob_type, int(planar), int(non_hg), C.byref(d)))
+    vbc = [([d.vbc_type[c][w] for w in range(6)], [d.vbc_val[c][w] for w in range(6)]) for c in range(3)]
+    tbc = ([d.tbc_type[w] for w in range(6)], [d.tbc_val[w] for w in range(6)])
+    pbc = ([d.pbc_type[w] for w in range(6)], [d.pbc_val[w] for w in range(6)])
+    return dict(vbc=vbc, tbc=tbc, pbc=pbc)
 
 
 def nse_footprint(g, mg, prm):

@@ -114,3 +114,51 @@ def test_nse_physics_matches_the_deck_mapping():
         with pytest.raises(S.SarasError):
             S.nse_physics(**kw)
         assert O.nse_physics(**kw) is None
+
+
+def test_nse_wiring_follows_the_drivers_tables():
+    """Wall kinds and values by problem type without custom BCs
+    (hydro.cc:349-484, scalar.cc:163-201), wall by wall.  Walls: 0 Left,
+    1 Right, 2 Front, 3 Back, 4 Bottom, 5 Top."""
+    D, N, NUL, HOT, PLU = S.FBC_DIRICHLET, S.FBC_NEUMANN, S.FBC_NULL, S.FBC_HOT_PLATE, S.FBC_PLUME_OUT
+    noslip = ([D] * 6, [0.0] * 6)
+
+    def check(pt, vx=noslip, vy=noslip, vz=noslip, t=([N] * 6, [0.0] * 6), p=([N] * 6, [0.0] * 6), **kw):
+        w = S.nse_wiring(pt, **kw)
+        assert w["vbc"] == [vx, vy, vz], pt
+        assert w["tbc"] == t, pt
+        assert w["pbc"] == p, pt
+
+    # 1: lid-driven cavity, Vx = 1 on the top wall (uTop)
+    check(1, vx=([D] * 6, [0.0] * 5 + [1.0]))
+    # 2, 4: no-slip everywhere (periodic axes ignore their entries)
+    check(2)
+    check(4)
+    # 3: inflow Vx = 1 on the left, zero-gradient outflow on the right; P null on the left
+    out = [D, N, D, D, D, D]
+    check(3, vx=(out, [1.0] + [0.0] * 5), vy=(out, [0.0] * 6), vz=(out, [0.0] * 6),
+          p=([NUL] + [N] * 5, [0.0] * 6))
+    # 5: Rayleigh-Benard, T = 1 at the bottom and 0 at the top, adiabatic sides
+    rbc_t = ([N] * 4 + [D, D], [0.0] * 4 + [1.0, 0.0])
+    check(5, t=rbc_t)
+    check(5, t=([N] * 4 + [HOT, D], [0.0] * 6), non_hg=True)
+    # 6: stable stratification, cold bottom, hot top
+    check(6, t=([N] * 4 + [D, D], [0.0] * 4 + [0.0, 1.0]))
+    # 7: vertical convection, T = 1 on the left, 0 on the right
+    check(7, t=([D, D] + [N] * 4, [1.0, 0.0] + [0.0] * 4))
+    # 8: free-slip bottom and top for Vx and Vy, no-penetration Vz
+    fs = ([D] * 4 + [N, N], [0.0] * 6)
+    check(8, vx=fs, vy=fs, t=rbc_t)
+    # 9: plume, Vz open on the x walls and the top, plumeOut for T at the top
+    check(9, vz=([N, N, D, D, D, N], [0.0] * 6), t=([N] * 4 + [D, PLU], [0.0] * 4 + [1.0, 0.0]))
+    check(9, vz=([N, N, D, D, D, N], [0.0] * 6), t=([N] * 4 + [HOT, PLU], [0.0] * 6), non_hg=True)
+    # the defaults of nse_params are the Rayleigh-Benard wiring
+    d, w = S.nse_params(), S.nse_wiring(5)
+    for c in range(3):
+        assert [d.vbc_type[c][q] for q in range(6)] == w["vbc"][c][0]
+    assert [d.tbc_type[q] for q in range(6)] == w["tbc"][0] and [d.tbc_val[q] for q in range(6)] == w["tbc"][1]
+    # PLANAR builds wire no bottom wall for a heating plate; problem types 1..9 only
+    for bad in (dict(prob_type=5, planar=True, non_hg=True), dict(prob_type=0), dict(prob_type=10)):
+        with pytest.raises(S.SarasError):
+            S.nse_wiring(**bad)
+    assert S.nse_wiring(5, planar=True)["tbc"] == rbc_t

@@ -565,3 +565,57 @@ def test_morinishi_second_order_steps_bitexact():
         assert r == 0
         assert so.jacobi_iters() == st["jacobi_iters"]
         assert_fields_equal(so, ss)
+
+
+@pytest.mark.parametrize("scheme", [0, 1], ids=["rk3", "euler-cn"])
+def test_inflow_null_pressure_wall_bitexact(scheme):
+    """The inflow problem (probType 3) with the drivers' wiring: Vx = 1 on
+    x-lo, zero-gradient outflow on x-hi, no-slip y and z walls, P null on
+    x-lo (nullBC, boundary.h:205-212: its ghosts keep the caller's values plus
+    every correction's ghost).  The caller's P carries random ghosts; three
+    steps on a stretched 16 x 8 x 8 box agree bit for bit, the x-lo ghosts of
+    P included."""
+    pad = 2
+    w = S.nse_wiring(3)
+    assert w["pbc"][0][0] == S.FBC_NULL
+    n = (16, 8, 8)
+    so, ss = pair(n, periodic=(0, 0, 0), gtype=(0, 0, 2), beta=(1.0, 1.0, 1.2), pad=pad, dscheme=2, nlscheme=3,
+                  has_T=0, nu=2e-2, dt=2e-3, cn_tol=1e-10, scheme=scheme, mg=mg_kw(vc_count=6, vc_tol=1e-10),
+                  **w)
+    rng = np.random.default_rng(31)
+    c = (slice(pad, -pad),) * 3
+    f = {k: np.zeros(so.shape) for k in ("vx", "vy", "vz", "p", "t", "pp")}
+    f["vx"][c] = 1.0 + 0.1 * rng.standard_normal(n)
+    f["vy"][c] = 0.1 * rng.standard_normal(n)
+    f["vz"][c] = 0.1 * rng.standard_normal(n)
+    f["p"][:] = rng.uniform(-1.0, 1.0, so.shape)
+    so.set(**f)
+    ss.set(**f)
+    names = ("vx", "vy", "vz", "p", "pp")
+    assert_fields_equal(so, ss, names=names, periodic=(0, 0, 0))
+    for _ in range(3):
+        assert so.step() == 0
+        r, st = ss.step()
+        assert r in (0, S.NOT_CONVERGED)
+        assert so.jacobi_iters() == st["jacobi_iters"]
+        assert_fields_equal(so, ss, names=names, periodic=(0, 0, 0))
+    P = ss.get()["p"]
+    ghost, edge = P[pad - 1, pad:-pad, pad:-pad], P[pad, pad:-pad, pad:-pad]
+    assert not np.array_equal(ghost, edge)   # not the zero-gradient rule
+
+
+def test_null_wall_only_for_pressure_on_wall_axes():
+    g = S.grid_desc((8, 8, 8), periodic=(1, 0, 0), pad=1)
+    mg = S.mg_params(bc=S.BC_NEUMANN, precision=S.PREC_FP64)
+    nul = [S.FBC_NULL] + [S.FBC_NEUMANN] * 5
+    bad = [dict(pbc=(nul, [0.0] * 6)),                                       # x is periodic
+           dict(vbc=[([S.FBC_NULL] + [S.FBC_DIRICHLET] * 5, [0.0] * 6)] * 3),   # on velocity
+           dict(tbc=([S.FBC_NEUMANN] * 3 + [S.FBC_NULL] + [S.FBC_DIRICHLET] * 2, [0.0] * 6))]   # on T
+    for kw in bad:
+        args = dict(dscheme=1, nlscheme=1, has_T=1, nu=1e-2, kappa=1e-2, dt=1e-3)
+        args.update(kw)
+        with pytest.raises(S.SarasError):
+            S.Nse(g, mg, S.nse_params(**args))
+    # a y wall is a wall axis here
+    S.Nse(g, mg, S.nse_params(dscheme=1, nlscheme=1, has_T=0, nu=1e-2, dt=1e-3,
+                              pbc=([S.FBC_NEUMANN] * 2 + [S.FBC_NULL] + [S.FBC_NEUMANN] * 3, [0.0] * 6))).close()

@@ -848,3 +848,64 @@ def test_planar_jacobi_and_walls():
     # Dirichlet ghosts of T: 2 v - T(edge), v = 1 at the bottom and 0 at the top
     assert np.allclose(f["t"][pad:-pad, 0, 0], 2.0 - f["t"][pad:-pad, 0, 1], rtol=0, atol=1e-15)
     assert np.allclose(f["t"][pad:-pad, 0, -1], -f["t"][pad:-pad, 0, -2], rtol=0, atol=1e-15)
+
+
+def inflow_desc(n=(16, 8, 8), pad=2, scheme=0, null=True):
+    """The inflow problem's walls (probType 3, hydro.cc:349-484): Vx = 1 on
+    x-lo, zero gradient of every component on x-hi, no-slip y and z walls;
+    P null on x-lo (or zero gradient when `null` is off), zero gradient
+    elsewhere."""
+    g = O.grid_desc(n, periodic=(0, 0, 0), gtype=(0, 0, 2), beta=(1.0, 1.0, 1.2), pad=pad)
+    vbc = [([FD, FN, FD, FD, FD, FD], [1.0, 0.0, 0.0, 0.0, 0.0, 0.0])] + \
+          [([FD, FN, FD, FD, FD, FD], [0.0] * 6)] * 2
+    pbc = ([O.FBC_NULL if null else FN] + [FN] * 5, [0.0] * 6)
+    return g, O.nse_desc(g, dscheme=2, nlscheme=3, has_T=0, nu=2e-2, dt=2e-3, cn_tol=1e-10, scheme=scheme,
+                         vbc=vbc, pbc=pbc, mg=O.mg_params(vc_count=6, vc_tol=1e-10, solve_flag=0, pre=3, post=3))
+
+
+def inflow_fields(shape, p, seed=4):
+    rng = np.random.default_rng(seed)
+    c = (slice(p, -p),) * 3
+    f = {k: np.zeros(shape) for k in ("vx", "vy", "vz", "p", "t", "pp")}
+    f["vx"][c] = 1.0 + 0.1 * rng.standard_normal(core(f["vx"], p).shape)
+    for k in ("vy", "vz"):
+        f[k][c] = 0.1 * rng.standard_normal(core(f[k], p).shape)
+    f["p"][:] = rng.uniform(-1.0, 1.0, shape)   # ghosts too: a null wall keeps them
+    return f
+
+
+@pytest.mark.parametrize("scheme", [0, 1], ids=["rk3", "euler-cn"])
+def test_null_pressure_wall_keeps_the_whole_array_sum(scheme):
+    """nullBC (boundary.h:205-212) writes nothing, so the x-lo ghosts of P
+    carry the caller's values plus every correction's ghost: after each
+    stage, P(-1) = P(-1) before + Pp(-1), and Pp(-1) = Pp(0) (the solver's
+    zero-gradient wall).  A zero-gradient P wall instead gives P(-1) = P(0),
+    which differs; the x-hi wall does that in both runs."""
+    p = 2
+    _, d = inflow_desc(scheme=scheme)
+    s = O.Nse(d)
+    f = inflow_fields(s.shape, p)
+    s.set(**f)
+    c = slice(p, -p)
+    ghost = lambda a: a[p - 1, c, c]   # noqa: E731
+    before = ghost(s.get()["p"]).copy()
+    assert np.array_equal(before, ghost(f["p"]))   # the initial imposeBCs leave it
+    stages = 1 if scheme == 1 else 3
+    for step in range(2):
+        for st in range(stages):
+            assert s.stage(st) == 0
+            o = s.get()
+            pp_g = ghost(o["pp"])
+            assert np.array_equal(pp_g, o["pp"][p, c, c])
+            assert np.array_equal(ghost(o["p"]), before + pp_g), (step, st)
+            assert not np.array_equal(ghost(o["p"]), o["p"][p, c, c])
+            assert np.array_equal(o["p"][-p, c, c], o["p"][-p - 1, c, c])
+            before = ghost(o["p"]).copy()
+    # the same run with a zero-gradient x-lo wall differs in the core: the
+    # pressure gradient of the first cells reads the ghost
+    _, d2 = inflow_desc(scheme=scheme, null=False)
+    s2 = O.Nse(d2)
+    s2.set(**f)
+    for _ in range(2):
+        assert s2.step() == 0
+    assert not np.array_equal(core(s2.get()["vx"], p), core(s.get()["vx"], p))

@@ -63,13 +63,13 @@ import saras_amd as S  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 N = 256
-DOMINANT = "k_rb_split_zp<float,3,true,false>"
+DOMINANT = "k_rb_split_zp<float,3,true,false,true>"
 # HBM bytes per launch of the dominant kernel (one FP32 colour pass of the
-# finest level of the z-periodic 256^3 grid) from rocprofv3 --pmc FETCH_SIZE /
-# WRITE_SIZE (KiB per dispatch), calibrated against the copy kernel (FETCH x
-# 2.0, WRITE x 1.0): (2 * 50247.5 + 32768.0) KiB.
-# Source: profiles/headline_sweep_hbm_traffic_256cubed.md
-DOMINANT_TRAFFIC_BYTES = (2 * 50247.5 + 32768.0) * 1024
+# finest level of the z-periodic 256^3 grid, SOR parameter 1) from rocprofv3
+# --pmc FETCH_SIZE / WRITE_SIZE (KiB per dispatch), calibrated against the
+# copy kernel (FETCH x 2.0, WRITE x 1.0): (2 * 33867.3 + 32768.0) KiB.
+# Source: profiles/headline_sweep_hbm_traffic.md
+DOMINANT_TRAFFIC_BYTES = (2 * 33867.3 + 32768.0) * 1024
 
 
 def bytes_per_vcycle(shapes, s, nu1, nu2):
@@ -540,6 +540,7 @@ def main():
                     help="only finest-level sweeps (FP32 and FP64) and the copy kernel, for counter runs")
     ap.add_argument("--headline-only", action="store_true",
                     help="only the headline solves (mixed and FP64), for profiling")
+    ap.add_argument("--n", type=int, default=N, help="grid extent of the --headline-only solves")
     ap.add_argument("--rbc-only", action="store_true",
                     help="only the 256^3 Rayleigh-Benard time steps (warmup + steps, FP64 and mixed)")
     ap.add_argument("--no-rbc", action="store_true")
@@ -568,7 +569,7 @@ def main():
     if args.serve_only:
         print(json.dumps({"serving": serving(count=args.requests, slots=args.slots)}))
         return
-    hs = headline_solve(args.steps, args.warmup, nu1, nu2)
+    hs = headline_solve(args.steps, args.warmup, nu1, nu2, n=args.n if args.headline_only else N)
     mx, f64 = hs["mixed"], hs["fp64"]
     if args.headline_only:
         print(json.dumps({"mixed_ms": mx["ms"], "fp64_ms": f64["ms"]}))

@@ -152,10 +152,11 @@ __device__ __forceinline__ void gather(const LevelArgs<T> &L, const T *__restric
 __device__ __forceinline__ double sor_div(double a, double d) { return a / d; }
 __device__ __forceinline__ float sor_div(float a, float d) { return a * __builtin_amdgcn_rcpf(d); }
 
-// RB-SOR point update (poisson_d3.cc:109-116, poisson_d2.cc:103-108)
+// RB-SOR point update (poisson_d3.cc:109-116, poisson_d2.cc:103-108):
+// (1 - w) x + sor_quot, the quotient from the neighbours alone
 template <typename T, int DIM>
-__device__ __forceinline__ T sor_update(const Stencil<T, DIM> &s, T c2x, T c1x, T c2y, T c1y, T c2z, T c1z,
-                                        T b, T w) {
+__device__ __forceinline__ T sor_quot(const Stencil<T, DIM> &s, T c2x, T c1x, T c2y, T c1y, T c2z, T c1z, T b,
+                                      T w) {
     T sum = c2x * (s.xE + s.xW) + c1x * (s.xE - s.xW);
     T den;
     if (DIM == 3) {
@@ -169,7 +170,13 @@ __device__ __forceinline__ T sor_update(const Stencil<T, DIM> &s, T c2x, T c1x, 
         sum = sum + c1z * (s.zT - s.zB);
         den = T(2.0) * (c2x + c2z);
     }
-    return (T(1.0) - w) * s.c + sor_div((sum - b) * w, den);
+    return sor_div((sum - b) * w, den);
+}
+
+template <typename T, int DIM>
+__device__ __forceinline__ T sor_update(const Stencil<T, DIM> &s, T c2x, T c1x, T c2y, T c1y, T c2z, T c1z,
+                                        T b, T w) {
+    return (T(1.0) - w) * s.c + sor_quot<T, DIM>(s, c2x, c1x, c2y, c1y, c2z, c1z, b, w);
 }
 
 // residual b - A x (poisson_d3.cc:81-87, poisson_d2.cc:80-84)

@@ -350,30 +350,44 @@ struct Ops {
         return L.n[1] % 16 == 0 && L.n[2] % 64 == 0 && (long)L.n[0] * L.n[1] * L.n[2] >= (8L << 20);
     }
 
+    template <int DIM, bool ZP, bool FINE, bool PRO>
+    void split_launch(const dim3 &g, const saras::LevelArgs<T> &a, DevLevel<T> &L, int colour, int nb, const T *e,
+                      long esx, long esy, long ehz) {
+        const dim3 bl(saras::kSplitThreads);
+        // w == 1 (every reference deck): the pass that skips the cells' own values
+        if (w == T(1)) {
+            if (ZP) hipLaunchKernelGGL((saras::k_rb_split_zp<T, DIM, FINE, PRO, true>), g, bl, 0, s, a, L.x, L.b, colour, w, nb, e, esx, esy, ehz);
+            else hipLaunchKernelGGL((saras::k_rb_split<T, DIM, FINE, PRO, true>), g, bl, 0, s, a, L.x, L.b, colour, w, nb, e, esx, esy, ehz);
+        } else {
+            if (ZP) hipLaunchKernelGGL((saras::k_rb_split_zp<T, DIM, FINE, PRO, false>), g, bl, 0, s, a, L.x, L.b, colour, w, nb, e, esx, esy, ehz);
+            else hipLaunchKernelGGL((saras::k_rb_split<T, DIM, FINE, PRO, false>), g, bl, 0, s, a, L.x, L.b, colour, w, nb, e, esx, esy, ehz);
+        }
+    }
+
     template <int DIM>
     void split_pass(bool fine, bool pro, int grid, const saras::LevelArgs<T> &a, DevLevel<T> &L, int colour, int nb,
                     const T *e, long esx, long esy, long ehz) {
-        const dim3 g(grid), bl(saras::kSplitThreads);
+        const dim3 g(grid);
         // rows that wrap in z, or rows split between waves, need the
         // z-prefetching variant (see rb_split_body)
         const int q = L.n[2] / 2 / saras::Vec16<T>::W;
         const bool zp = a.bc_lo[2] == saras::BC_PER || !(q % 64 == 0 || 64 % q == 0);
         if (zp) {
             if (fine) {
-                if (pro) hipLaunchKernelGGL((saras::k_rb_split_zp<T, DIM, true, true>), g, bl, 0, s, a, L.x, L.b, colour, w, nb, e, esx, esy, ehz);
-                else hipLaunchKernelGGL((saras::k_rb_split_zp<T, DIM, true, false>), g, bl, 0, s, a, L.x, L.b, colour, w, nb, e, esx, esy, ehz);
+                if (pro) split_launch<DIM, true, true, true>(g, a, L, colour, nb, e, esx, esy, ehz);
+                else split_launch<DIM, true, true, false>(g, a, L, colour, nb, e, esx, esy, ehz);
             } else {
-                if (pro) hipLaunchKernelGGL((saras::k_rb_split_zp<T, DIM, false, true>), g, bl, 0, s, a, L.x, L.b, colour, w, nb, e, esx, esy, ehz);
-                else hipLaunchKernelGGL((saras::k_rb_split_zp<T, DIM, false, false>), g, bl, 0, s, a, L.x, L.b, colour, w, nb, e, esx, esy, ehz);
+                if (pro) split_launch<DIM, true, false, true>(g, a, L, colour, nb, e, esx, esy, ehz);
+                else split_launch<DIM, true, false, false>(g, a, L, colour, nb, e, esx, esy, ehz);
             }
             return;
         }
         if (fine) {
-            if (pro) hipLaunchKernelGGL((saras::k_rb_split<T, DIM, true, true>), g, bl, 0, s, a, L.x, L.b, colour, w, nb, e, esx, esy, ehz);
-            else hipLaunchKernelGGL((saras::k_rb_split<T, DIM, true, false>), g, bl, 0, s, a, L.x, L.b, colour, w, nb, e, esx, esy, ehz);
+            if (pro) split_launch<DIM, false, true, true>(g, a, L, colour, nb, e, esx, esy, ehz);
+            else split_launch<DIM, false, true, false>(g, a, L, colour, nb, e, esx, esy, ehz);
         } else {
-            if (pro) hipLaunchKernelGGL((saras::k_rb_split<T, DIM, false, true>), g, bl, 0, s, a, L.x, L.b, colour, w, nb, e, esx, esy, ehz);
-            else hipLaunchKernelGGL((saras::k_rb_split<T, DIM, false, false>), g, bl, 0, s, a, L.x, L.b, colour, w, nb, e, esx, esy, ehz);
+            if (pro) split_launch<DIM, false, false, true>(g, a, L, colour, nb, e, esx, esy, ehz);
+            else split_launch<DIM, false, false, false>(g, a, L, colour, nb, e, esx, esy, ehz);
         }
     }
 

@@ -125,7 +125,12 @@ __device__ __forceinline__ T coarse_at(const T *__restrict__ e, long base, long 
 // ZPRE: the z neighbour that no adjacent lane holds (periodic wrap, or a row
 // split between waves) is loaded together with the row vectors; without it
 // that load is issued only where needed, which is cheaper when it never is.
-template <typename T, int DIM, bool PRO, bool ZPRE>
+// W1: w == 1 (the "SOR Parameter" of every reference deck).  The update
+// (1 - w) x + q then equals q whenever q != 0 and x is finite (0 x is a
+// signed zero), so the cells' own values are read only where q is a zero
+// (to give the sign the reference's sum has) or where a wall rule needs them:
+// a pass reads 1.5 words per updated cell pair instead of 2.
+template <typename T, int DIM, bool PRO, bool ZPRE, bool W1>
 __device__ __forceinline__ void rb_split_body(const LevelArgs<T> &L, T *__restrict__ x, const T *__restrict__ b,
                                               int colour, T w, int nblocks, const T *__restrict__ e, long esx,
                                               long esy, long ehz) {
@@ -148,7 +153,17 @@ __device__ __forceinline__ void rb_split_body(const LevelArgs<T> &L, T *__restri
     const long rb = (long)i * L.sx + (long)j * L.sy;
     const long own = rb + h * L.hz + m0;
     const long oth = rb + (1 - h) * L.hz + m0;
-    const V cv = *reinterpret_cast<const V *>(x + own);
+    // the cell's own value: always without W1; with it only on rows whose x/y
+    // neighbour lies beyond a wall and at the z-wall ends of a row
+    bool need_c = !W1;
+    if (W1) {
+        need_c = (i + 1 >= nx && L.bc_hi[0] != BC_PER) || (i == 0 && L.bc_lo[0] != BC_PER);
+        if (DIM == 3) need_c = need_c || (j + 1 >= ny && L.bc_hi[1] != BC_PER) || (j == 0 && L.bc_lo[1] != BC_PER);
+        need_c = need_c || (h == 0 && m0 == 0 && L.bc_lo[2] != BC_PER) ||
+                 (h == 1 && m0 + W == half && L.bc_hi[2] != BC_PER);
+    }
+    V cv{};
+    if (need_c) cv = *reinterpret_cast<const V *>(x + own);
     const V ov = *reinterpret_cast<const V *>(x + oth);
     const V bv = *reinterpret_cast<const V *>(b + own);
     const V c2v = *reinterpret_cast<const V *>(L.c2zs + h * half + m0);
@@ -285,6 +300,7 @@ __device__ __forceinline__ void rb_split_body(const LevelArgs<T> &L, T *__restri
     const T c2x = L.c2[0][i], c1x = L.c1[0][i];
     const T c2y = DIM == 3 ? L.c2[1][j] : T(0), c1y = DIM == 3 ? L.c1[1][j] : T(0);
     T r[W];
+    bool zero = false;
 #pragma unroll
     for (int q = 0; q < W; ++q) {
         Stencil<T, DIM> s;
@@ -302,23 +318,33 @@ __device__ __forceinline__ void rb_split_body(const LevelArgs<T> &L, T *__restri
             s.zB = o[q];
             s.zT = q < W - 1 ? o[q + 1] : zhi_last;
         }
-        r[q] = sor_update<T, DIM>(s, c2x, c1x, c2y, c1y, VT::get(c2v, q), VT::get(c1v, q), VT::get(bv, q), w);
+        r[q] = sor_quot<T, DIM>(s, c2x, c1x, c2y, c1y, VT::get(c2v, q), VT::get(c1v, q), VT::get(bv, q), w);
+        zero = zero || r[q] == T(0);
     }
+    if (W1 && zero && !need_c) {
+        // a zero quotient takes the sign of (1 - w) x + q: read the cells now
+        const V lv = *reinterpret_cast<const V *>(x + own);
+#pragma unroll
+        for (int q = 0; q < W; ++q) c[q] = PRO ? evo[q] + VT::get(lv, q) : VT::get(lv, q);
+    }
+#pragma unroll
+    for (int q = 0; q < W; ++q)
+        if (!W1 || r[q] == T(0)) r[q] = (T(1.0) - w) * c[q] + r[q];
     *reinterpret_cast<V *>(x + own) = VT::make(r);
 }
 
-template <typename T, int DIM, bool FINE, bool PRO>
+template <typename T, int DIM, bool FINE, bool PRO, bool W1>
 __global__ void __launch_bounds__(kSplitThreads) k_rb_split(LevelArgs<T> L, T *__restrict__ x,
                                                             const T *__restrict__ b, int colour, T w, int nblocks,
                                                             const T *__restrict__ e, long esx, long esy, long ehz) {
-    rb_split_body<T, DIM, PRO, false>(L, x, b, colour, w, nblocks, e, esx, esy, ehz);
+    rb_split_body<T, DIM, PRO, false, W1>(L, x, b, colour, w, nblocks, e, esx, esy, ehz);
 }
 
-template <typename T, int DIM, bool FINE, bool PRO>
+template <typename T, int DIM, bool FINE, bool PRO, bool W1>
 __global__ void __launch_bounds__(kSplitThreads) k_rb_split_zp(LevelArgs<T> L, T *__restrict__ x,
                                                                const T *__restrict__ b, int colour, T w, int nblocks,
                                                                const T *__restrict__ e, long esx, long esy, long ehz) {
-    rb_split_body<T, DIM, PRO, true>(L, x, b, colour, w, nblocks, e, esx, esy, ehz);
+    rb_split_body<T, DIM, PRO, true, W1>(L, x, b, colour, w, nblocks, e, esx, esy, ehz);
 }
 
 // ---------------------------------------------------------------------------

@@ -48,6 +48,11 @@ struct LevelArgs {
     // shell: x-walls [(j+1)(nz+2) + k+1], y-walls [(i+1)(nz+2) + k+1],
     // z-walls [(i+1)(ny+2) + j+1]; planar: x-walls [k+1], z-walls [i+1]
     const T *wall[3];
+    // split colour passes: 0 for rows in (i, j) order; otherwise rows run in
+    // tiles of rtile j-rows (a power of two dividing n[1]), all i of a tile
+    // before the next tile, so that the x-neighbour rows a pass reads again
+    // two i-planes later are still in L2 on large levels
+    int rtile;
 };
 
 // offset of element k inside its row

@@ -18,6 +18,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <array>
 #include <vector>
 
 #include "grid_host.hpp"
@@ -88,6 +89,14 @@ struct Hierarchy {
     bool coarse_lds = true;
     size_t coarse_lds_bytes = 0;
     bool split_rows = false;   // store levels whose rows split into whole vectors colour-split
+    // host copies of the metric coefficients (c2, c1) and restriction
+    // factors (lo, hi, wd) per level and axis, for the tail's packed image
+    std::vector<std::array<std::vector<double>, 3>> hc2, hc1, hlo, hhi, hwd;
+    // the tail kernel's coefficient region of LDS for tail levels
+    // tail_first..D, packed in its order (see k_tail)
+    T *tail_coef = nullptr;
+    int tail_first = -1;
+    size_t tail_coef_n = 0;
 
     // a level takes split rows when each half-row is a whole number of 16-byte vectors
     static bool splittable(int nz) {
@@ -106,12 +115,46 @@ struct Hierarchy {
         }
         for (int a = 0; a < 3; ++a) (void)hipFree(wall[a]);
         (void)hipFree(coarse_scratch);
+        (void)hipFree(tail_coef);
+    }
+
+    // The tail's coefficient image for levels S..D: per level and axis c2
+    // then c1 (one zero each for the planar y axis), then for levels S+1..D
+    // of a 3-D hierarchy per axis the restriction factors lo, hi, wd.
+    saras_status pack_tail(int S) {
+        const int D = (int)lv.size() - 1;
+        if (S > D) return SARAS_OK;
+        std::vector<double> flat;
+        for (int l = S; l <= D; ++l)
+            for (int a = 0; a < 3; ++a) {
+                flat.insert(flat.end(), hc2[l][a].begin(), hc2[l][a].end());
+                flat.insert(flat.end(), hc1[l][a].begin(), hc1[l][a].end());
+            }
+        if (dim == 3)
+            for (int l = S + 1; l <= D; ++l)
+                for (int a = 0; a < 3; ++a) {
+                    flat.insert(flat.end(), hlo[l][a].begin(), hlo[l][a].end());
+                    flat.insert(flat.end(), hhi[l][a].begin(), hhi[l][a].end());
+                    flat.insert(flat.end(), hwd[l][a].begin(), hwd[l][a].end());
+                }
+        (void)hipFree(tail_coef);
+        tail_coef = nullptr;
+        saras_status st = upload(&tail_coef, flat);
+        if (st) return st;
+        tail_first = S;
+        tail_coef_n = flat.size();
+        return SARAS_OK;
     }
 
     saras_status build(const saras::GridLevels &G, const std::vector<std::vector<double>> &walls) {
         dim = G.dim;
         const int D = G.depth;
         lv.resize(D + 1);
+        hc2.assign(D + 1, {});
+        hc1.assign(D + 1, {});
+        hlo.assign(D + 1, {});
+        hhi.assign(D + 1, {});
+        hwd.assign(D + 1, {});
         for (int l = 0; l <= D; ++l) {
             DevLevel<T> &L = lv[l];
             for (int a = 0; a < 3; ++a) L.n[a] = G.axes[l][a].n;
@@ -132,6 +175,8 @@ struct Hierarchy {
                 if (dim == 2 && a == 1) {
                     std::vector<double> z(1, 0.0);
                     if (upload(&L.c2[a], z) || upload(&L.c1[a], z)) return SARAS_HIP_ERROR;
+                    hc2[l][a] = z;
+                    hc1[l][a] = z;
                     continue;
                 }
                 const saras::AxisLevel &A = G.axes[l][a];
@@ -147,6 +192,8 @@ struct Hierarchy {
                     c1[i] = A.xixx[i + 1] * i2h;
                 }
                 if (upload(&L.c2[a], c2) || upload(&L.c1[a], c1)) return SARAS_HIP_ERROR;
+                hc2[l][a] = c2;
+                hc1[l][a] = c1;
                 if (a == 2 && L.hz) {
                     const int half = A.n / 2;
                     std::vector<double> c2s(A.n), c1s(A.n);
@@ -170,6 +217,9 @@ struct Hierarchy {
                     }
                     if (upload(&L.rlo[a], lo) || upload(&L.rhi[a], hi) || upload(&L.rwd[a], wd))
                         return SARAS_HIP_ERROR;
+                    hlo[l][a] = lo;
+                    hhi[l][a] = hi;
+                    hwd[l][a] = wd;
                 }
             }
         }
@@ -274,7 +324,42 @@ saras::LevelArgs<T> level_args(const saras_mg *m, const Hierarchy<T> &H, int l, 
     a.hz = L.hz;
     a.c2zs = L.c2zs;
     a.c1zs = L.c1zs;
+    a.rtile = 0;
+    // A colour pass reads row (i+1, j) again two i-planes after its first
+    // read.  Where those two planes of x and b exceed half of an XCD's 4 MB
+    // L2, the rows run in j-tiles of about 256 KB of such reuse instead.
+    if (L.hz && H.dim == 3) {
+        const size_t half = (size_t)L.n[2] / 2 * sizeof(T);
+        if (4 * (size_t)L.n[1] * half > (2u << 20)) {
+            int jt = 1;
+            while (jt < 256 && (size_t)(2 * jt) * 4 * half <= (256u << 10)) jt *= 2;
+            while (jt > 1 && L.n[1] % jt) jt /= 2;
+            if (jt >= 8) a.rtile = jt;
+        }
+    }
     return a;
+}
+
+// first level of the single-workgroup tail: the smallest S whose levels S..D
+// fit in the tail's LDS budget (D + 1, i.e. no tail, when even the coarsest
+// level does not fit)
+template <typename T>
+int tail_first_level(const saras_mg *m, const Hierarchy<T> &H) {
+    const int D = (int)H.lv.size() - 1;
+    size_t bytes = 0;
+    int S = D + 1;
+    for (int l = D; l >= 0; --l) {
+        const DevLevel<T> &L = H.lv[l];
+        const size_t core = (size_t)L.n[0] * L.n[1] * L.n[2];
+        const size_t pad = H.dim == 3 ? (size_t)(L.n[0] + 2) * (L.n[1] + 2) * (L.n[2] + 2)
+                                      : (size_t)(L.n[0] + 2) * (L.n[2] + 2);
+        // metric coefficients (2 per index) and restriction factors (3)
+        const size_t coef = 5 * ((size_t)L.n[0] + L.n[1] + L.n[2]);
+        bytes += ((pad + 3) / 4 * 4 + (core + 3) / 4 * 4 + coef) * sizeof(T);
+        if (bytes > m->tail_lds || D - l + 1 > saras::kTailMaxLevels) break;
+        S = l;
+    }
+    return S;
 }
 
 // Level operations for one precision.  `inhom0` selects the inhomogeneous
@@ -554,26 +639,7 @@ struct Ops {
                            F.n[2], F.sx, F.sy, F.hz, F.x, C.x, C.sx, C.sy, C.hz);
     }
 
-    // first level of the single-workgroup tail: the smallest S whose levels
-    // S..D fit in kTailLds bytes of LDS
-    // (D + 1, i.e. no tail, when even the coarsest level does not fit)
-    int tail_start() const {
-        const int D = (int)H.lv.size() - 1;
-        size_t bytes = 0;
-        int S = D + 1;
-        for (int l = D; l >= 0; --l) {
-            const DevLevel<T> &L = H.lv[l];
-            const size_t core = (size_t)L.n[0] * L.n[1] * L.n[2];
-            const size_t pad = H.dim == 3 ? (size_t)(L.n[0] + 2) * (L.n[1] + 2) * (L.n[2] + 2)
-                                          : (size_t)(L.n[0] + 2) * (L.n[2] + 2);
-            // metric coefficients (2 per index) and restriction factors (3)
-            const size_t coef = 5 * ((size_t)L.n[0] + L.n[1] + L.n[2]);
-            bytes += ((pad + 3) / 4 * 4 + (core + 3) / 4 * 4 + coef) * sizeof(T);
-            if (bytes > m->tail_lds || D - l + 1 > saras::kTailMaxLevels) break;
-            S = l;
-        }
-        return S;
-    }
+    int tail_start() const { return tail_first_level(m, H); }
 
     void tail(int S) {
         const int D = (int)H.lv.size() - 1;
@@ -615,6 +681,17 @@ struct Ops {
         A.zero_bc_first = S != 0;
         A.wave_cells = m->tail_wave_cells;
         A.iters_out = m->iters_dev;
+        size_t ncoef = 0;
+        for (int l = S; l <= D; ++l)
+            for (int q = 0; q < 3; ++q) {
+                ncoef += 2 * (size_t)(H.dim == 2 && q == 1 ? 1 : H.lv[l].n[q]);
+                if (H.dim == 3 && l > S) ncoef += 3 * (size_t)H.lv[l].n[q];
+            }
+        if (H.tail_first != S || !H.tail_coef || H.tail_coef_n != ncoef) {
+            refuse("tail coefficients not packed for this tail");
+            return;
+        }
+        A.coef = H.tail_coef;
         // host-side guard: tail_start() keeps the levels within tail_lds
         if (off * sizeof(T) > m->tail_lds) {
             refuse("tail levels exceed the LDS budget");
@@ -972,7 +1049,11 @@ saras_status saras_mg_create(const saras_grid_desc *grid, const saras_mg_params 
         st = m->h32->build(m->G, nowall);
         if (st) return st;
         HIP_TRY(hipMalloc((void **)&m->x64b, m->h64.lv[0].elems * sizeof(double)));
+        st = m->h32->pack_tail(tail_first_level(m.get(), *m->h32));
+        if (st) return st;
     }
+    st = m->h64.pack_tail(tail_first_level(m.get(), m->h64));
+    if (st) return st;
     HIP_TRY(hipMalloc((void **)&m->partial, 2 * saras::kNormBlocks * sizeof(double)));
     HIP_TRY(hipMalloc((void **)&m->norm_out, 2 * sizeof(double)));
     HIP_TRY(hipMalloc((void **)&m->iters_dev, sizeof(int)));

@@ -147,7 +147,15 @@ __device__ __forceinline__ void rb_split_body(const LevelArgs<T> &L, T *__restri
     if (t >= total) return;
     const int row = (int)(t / q);
     const int v = (int)(t - (long)row * q);
-    const int i = row / ny, j = row - (row / ny) * ny;
+    int i, j;
+    if (L.rtile) {
+        const int jt = L.rtile, tile = row / (nx * jt), rem = row - tile * (nx * jt);
+        i = rem >> __builtin_ctz(jt);
+        j = tile * jt + (rem & (jt - 1));
+    } else {
+        i = row / ny;
+        j = row - i * ny;
+    }
     const int h = (i + j + colour) & 1;   // half of this row holding the colour
     const int m0 = v * W;
     const long rb = (long)i * L.sx + (long)j * L.sy;

@@ -29,6 +29,7 @@ struct TailArgs {
     RestrictArgs<T> R[kTailMaxLevels];   // restriction factors of level l (l > S)
     int xoff[kTailMaxLevels], boff[kTailMaxLevels];  // LDS offsets (elements)
     int coff;                            // LDS offset of the staged coefficients and restriction factors
+    const T *coef;                       // their packed image in global memory (the LDS region's order)
     T *x;                                // global core array of level S (in/out)
     const T *b;                          // global core array of level S
     int pre, post, solve_flag, max_count;
@@ -283,10 +284,10 @@ __global__ void __launch_bounds__(512) k_tail(TailArgs<T> A) {
     // Staging, with every global load issued before the first barrier: the
     // metric coefficients of each tail level and (3-D) the restriction
     // factors of levels S+1..D, which every phase reads, are copied to LDS
-    // (one flat element walk, so a thread's loads do not wait on each other)
-    // and described by LDS offsets; level S's iterate and right-hand side
-    // come in beside them.  Level S's ghost shell is not cleared: every face
-    // ghost is written before it is read.
+    // from their packed image (one flat coalesced copy) and described by LDS
+    // offsets; level S's iterate and right-hand side come in beside them.
+    // Level S's ghost shell is not cleared: every face ghost is written
+    // before it is read.
     {
         if (threadIdx.x == 0) {
             int off = A.coff;
@@ -323,36 +324,7 @@ __global__ void __launch_bounds__(512) k_tail(TailArgs<T> A) {
                 ncoef += 2 * ext(q, a);
                 if (DIM == 3 && q > 0) nres += 3 * A.L[q].n[a];
             }
-        for (int e = threadIdx.x; e < ncoef + nres; e += blockDim.x) {
-            int rem = e, off = A.coff;
-            const T *src = nullptr;
-            if (e < ncoef) {
-                for (int q = 0; q < nl && !src; ++q)
-                    for (int a = 0; a < 3 && !src; ++a) {
-                        const int n = ext(q, a);
-                        if (rem < 2 * n) src = rem < n ? A.L[q].c2[a] + rem : A.L[q].c1[a] + (rem - n);
-                        else {
-                            rem -= 2 * n;
-                            off += 2 * n;
-                        }
-                    }
-            } else {
-                rem -= ncoef;
-                off += ncoef;
-                for (int q = 1; q < nl && !src; ++q)
-                    for (int a = 0; a < 3 && !src; ++a) {
-                        const int n = A.L[q].n[a];
-                        if (rem < 3 * n)
-                            src = rem < n ? A.R[q].lo[a] + rem
-                                          : (rem < 2 * n ? A.R[q].hi[a] + (rem - n) : A.R[q].wd[a] + (rem - 2 * n));
-                        else {
-                            rem -= 3 * n;
-                            off += 3 * n;
-                        }
-                    }
-            }
-            lds[off + rem] = *src;
-        }
+        for (int e = threadIdx.x; e < ncoef + nres; e += blockDim.x) lds[A.coff + e] = A.coef[e];
         const LevelArgs<T> &L = A.L[0];
         const int PZ = L.n[2] + 2, PY = DIM == 3 ? L.n[1] + 2 : 1;
         T *XS = lds + A.xoff[0];

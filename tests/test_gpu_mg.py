@@ -472,3 +472,20 @@ def test_unit_sor_signed_zeros_bitwise(case):
             x = mo.get_level(lev)[0]
     assert neg > 0   # negative zeros occur, so their sign rule was exercised
 
+
+
+def test_large_level_tiled_rows_bitexact():
+    """On levels whose two i-planes of iterate and right-hand side exceed half
+    of an XCD's L2 (here FP64 rows of 512 cells, 512 rows per plane) the colour
+    passes visit rows in j-tiles; every cell is still updated once per pass,
+    bit for bit as the reference's loops do, for both SOR parameters."""
+    for sor in (1.0, 1.3):
+        mo, ms = pair((16, 512, 512), 3, (1, 0, 1), (0, 2, 0), O.BC_NEUMANN, sor=sor)
+        rng = np.random.default_rng(43)
+        shp = mo.padded_shape(0)
+        x, b = rand_pad(shp, rng), rand_pad(shp, rng)
+        mo.set_level(0, x, b)
+        ms.set_level(0, x, b)
+        mo.smooth(0, 2, 0)
+        ms.smooth(0, 2, 0)
+        np.testing.assert_array_equal(inner(ms.get_level(0)[0]), inner(mo.get_level(0)[0]), err_msg=f"sor {sor}")

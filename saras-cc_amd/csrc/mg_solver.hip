@@ -45,11 +45,12 @@ void set_error(const std::string &s) { g_last_error = s; }
     } while (0)
 
 constexpr int kThreads = 256;
-// default dynamic LDS budget of the single-workgroup tail: levels of a few
-// thousand cells run faster as multi-workgroup passes than on one CU, so the
-// tail takes only the levels that fit here (SARAS_TAIL_LDS_KB overrides it,
-// up to 60 KB: the tail's static LDS is under 2 KB)
-constexpr size_t kTailLdsDefault = 46 * 1024;
+// default dynamic LDS budget of the single-workgroup tail (SARAS_TAIL_LDS_KB
+// overrides it, up to 150 KB of the 160 KB a workgroup may hold: the tail's
+// static LDS is under 2 KB); the tail also takes only levels its threads
+// cover with a cell each or with colour lists (see tail_first_level)
+constexpr size_t kTailLdsDefault = 128 * 1024;
+constexpr int kTailThreads = 512;
 
 inline unsigned blocks_for(long n) { return (unsigned)std::max<long>(1, (n + kThreads - 1) / kThreads); }
 
@@ -357,6 +358,13 @@ int tail_first_level(const saras_mg *m, const Hierarchy<T> &H) {
         const size_t coef = 5 * ((size_t)L.n[0] + L.n[1] + L.n[2]);
         bytes += ((pad + 3) / 4 * 4 + (core + 3) / 4 * 4 + coef) * sizeof(T);
         if (bytes > m->tail_lds || D - l + 1 > saras::kTailMaxLevels) break;
+        // a cell per thread, or colour lists (even z extent, at most
+        // kTailColourCells cells of a colour per thread); larger levels
+        // would walk every cell in each colour pass and run faster as
+        // multi-workgroup passes
+        const bool one = core <= (size_t)kTailThreads;
+        const bool lists = L.n[2] % 2 == 0 && core / 2 <= (size_t)saras::kTailColourCells * kTailThreads;
+        if (l < D && !one && !lists) break;
         S = l;
     }
     return S;
@@ -698,9 +706,9 @@ struct Ops {
             return;
         }
         if (H.dim == 3)
-            hipLaunchKernelGGL((saras::k_tail<T, 3>), dim3(1), dim3(512), off * sizeof(T), s, A);
+            hipLaunchKernelGGL((saras::k_tail<T, 3>), dim3(1), dim3(kTailThreads), off * sizeof(T), s, A);
         else
-            hipLaunchKernelGGL((saras::k_tail<T, 2>), dim3(1), dim3(512), off * sizeof(T), s, A);
+            hipLaunchKernelGGL((saras::k_tail<T, 2>), dim3(1), dim3(kTailThreads), off * sizeof(T), s, A);
     }
 
     // poisson::vCycle (poisson.cc:203-279); levels S..D run in the tail kernel
@@ -1024,7 +1032,7 @@ saras_status saras_mg_create(const saras_grid_desc *grid, const saras_mg_params 
         const char *env7 = std::getenv("SARAS_TAIL_WAVE_CELLS");
         if (env7 && std::atoi(env7) >= 0) m->tail_wave_cells = std::atoi(env7);
         const char *env6 = std::getenv("SARAS_TAIL_LDS_KB");
-        if (env6 && std::atoi(env6) > 0) m->tail_lds = (size_t)std::min(60, std::atoi(env6)) * 1024;
+        if (env6 && std::atoi(env6) > 0) m->tail_lds = (size_t)std::min(150, std::atoi(env6)) * 1024;
     }
     if (m->p.inner_cycles <= 0) m->p.inner_cycles = 3;
     if (m->p.max_refine <= 0) m->p.max_refine = 20;

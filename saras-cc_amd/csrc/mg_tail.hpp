@@ -178,27 +178,63 @@ __device__ __forceinline__ void tghost_apply(T *X, const TGhost<T> &g, bool wall
     }
 }
 
+// cells of one colour per thread held in a colour list (see tail_level)
+constexpr int kTailColourCells = 4;
+// face-ghost items per thread held in registers
+constexpr int kTailGhostItems = 2;
+
 // The smoothing sweeps (or the coarsest solve) of one level: `count` RB-SOR
 // sweeps, then the final imposeBC; with SOLVE, sweeps until max|r| < tol or
 // the iteration cap (poisson_d3.cc:249-432), returning the iteration count.
+// A level with no more cells than threads keeps each thread's cell in
+// registers; a larger level with an even z extent and at most
+// kTailColourCells cells of a colour per thread keeps colour lists: the
+// packed (i, j, k) of the thread's cells of each colour, so a colour pass
+// visits only its own cells with every lane busy and loads all of a
+// thread's stencils before the first write (the cells of a colour never
+// read each other).  Other levels walk all cells.
 template <typename T, int DIM, bool WAVE, bool SOLVE>
 __device__ int tail_level(T *lds, const TailLevel<T> &V, int count, T w, double tol, double rel_floor, int max_count,
                           double *red) {
+    constexpr int CPT = kTailColourCells, GPT = kTailGhostItems;
     const int tid = threadIdx.x, nthr = lthreads<WAVE>();
     const int total = V.n[0] * V.n[1] * V.n[2];
     const int nface = DIM == 3 ? V.n[1] * V.n[2] + V.n[0] * V.n[2] + V.n[0] * V.n[1] : V.n[2] + V.n[0];
     const int sx = DIM == 3 ? V.PY * V.PZ : V.PZ, sy = V.PZ;
     T *X = lds + V.x;
-    const bool cached = total <= nthr, gcached = nface <= nthr;
+    const bool cached = total <= nthr, gcached = nface <= GPT * nthr;
+    const int half = total >> 1, hz2 = V.n[2] >> 1;
+    const bool lists = !cached && (V.n[2] & 1) == 0 && half <= CPT * nthr;
     TCell<T> q{};
-    TGhost<T> g{};
-    const bool act = cached && tid < total, gact = gcached && tid < nface;
+    TGhost<T> g[GPT]{};
+    const bool act = cached && tid < total;
     if (act) tcell_load<T, DIM>(lds, V, tid, q);
-    if (gact) tghost_load<T, DIM>(V, tid, g);
+    if (gcached)
+#pragma unroll
+        for (int e = 0; e < GPT; ++e)
+            if (tid + e * nthr < nface) tghost_load<T, DIM>(V, tid + e * nthr, g[e]);
+    int cell[2][CPT];   // packed i << 20 | j << 10 | k, or -1
+#pragma unroll
+    for (int col = 0; col < 2; ++col)
+#pragma unroll
+        for (int e = 0; e < CPT; ++e) {
+            cell[col][e] = -1;
+            const int u = tid + e * nthr;
+            if (lists && u < half) {
+                const unsigned row = (unsigned)u / (unsigned)hz2;
+                const int m = u - (int)(row * (unsigned)hz2);
+                const unsigned i = row / (unsigned)V.n[1];
+                const int j = (int)(row - i * (unsigned)V.n[1]);
+                const int k = 2 * m + (((int)i + j + col) & 1);
+                cell[col][e] = ((int)i << 20) | (j << 10) | k;
+            }
+        }
 
     auto ghosts = [&](bool walls) {
         if (gcached) {
-            if (gact) tghost_apply(X, g, walls);
+#pragma unroll
+            for (int e = 0; e < GPT; ++e)
+                if (tid + e * nthr < nface) tghost_apply(X, g[e], walls);
         } else {
             for (int t = tid; t < nface; t += nthr) {
                 TGhost<T> h;
@@ -216,6 +252,32 @@ __device__ int tail_level(T *lds, const TailLevel<T> &V, int count, T w, double 
     auto colour = [&](int col) {
         if (cached) {
             if (act && q.par == col) update(q);
+        } else if (lists) {
+            T r[CPT];
+            int c[CPT];
+#pragma unroll
+            for (int e = 0; e < CPT; ++e) {
+                const int p = col ? cell[1][e] : cell[0][e];
+                c[e] = -1;
+                if (p < 0) continue;
+                const int i = p >> 20, j = (p >> 10) & 1023, k = p & 1023;
+                TCell<T> t;
+                t.c = tidx<DIM>(V.PY, V.PZ, i, j, k);
+                t.c2x = lds[V.c2[0] + i];
+                t.c1x = lds[V.c1[0] + i];
+                t.c2y = DIM == 3 ? lds[V.c2[1] + j] : T(0);
+                t.c1y = DIM == 3 ? lds[V.c1[1] + j] : T(0);
+                t.c2z = lds[V.c2[2] + k];
+                t.c1z = lds[V.c1[2] + k];
+                t.b = lds[V.b + (i * V.n[1] + j) * V.n[2] + k];
+                Stencil<T, DIM> s;
+                tgather<T, DIM>(X, t.c, sx, sy, s);
+                r[e] = sor_update<T, DIM>(s, t.c2x, t.c1x, t.c2y, t.c1y, t.c2z, t.c1z, t.b, w);
+                c[e] = t.c;
+            }
+#pragma unroll
+            for (int e = 0; e < CPT; ++e)
+                if (c[e] >= 0) X[c[e]] = r[e];
         } else {
             for (int t = tid; t < total; t += nthr) {
                 int i, j, k;

@@ -279,7 +279,6 @@ struct saras_mg {
     bool march_tj8 = false;            // FP32 march with 64 x 8 tiles (SARAS_MARCH_TJ8=1)
     bool split_off = false;            // SARAS_NO_SPLIT=1 keeps every level z-contiguous
     size_t tail_lds = kTailLdsDefault;  // dynamic LDS budget of the tail kernel
-    int tail_wave_cells = saras::kTailWaveCells;  // SARAS_TAIL_WAVE_CELLS overrides it
     bool red_fold = true;              // SARAS_NO_RED_FOLD=1: the restriction leaves the first red pass to the sweep
     bool rr_lane = false;              // SARAS_RR_LANE=1: 3-D split restriction with all eight children in one lane
     // V-cycles replayed from captured hipGraphs (saras_mg_set_graphs).  A
@@ -687,7 +686,6 @@ struct Ops {
         A.tol = m->p.mg_tol;
         A.rel_floor = coarse_rel_floor();
         A.zero_bc_first = S != 0;
-        A.wave_cells = m->tail_wave_cells;
         A.iters_out = m->iters_dev;
         size_t ncoef = 0;
         for (int l = S; l <= D; ++l)
@@ -1029,8 +1027,6 @@ saras_status saras_mg_create(const saras_grid_desc *grid, const saras_mg_params 
         m->rr_lane = env9 && env9[0] == '1';
         const char *env8 = std::getenv("SARAS_NO_RED_FOLD");
         m->red_fold = !(env8 && env8[0] == '1');
-        const char *env7 = std::getenv("SARAS_TAIL_WAVE_CELLS");
-        if (env7 && std::atoi(env7) >= 0) m->tail_wave_cells = std::atoi(env7);
         const char *env6 = std::getenv("SARAS_TAIL_LDS_KB");
         if (env6 && std::atoi(env6) > 0) m->tail_lds = (size_t)std::min(150, std::atoi(env6)) * 1024;
     }

@@ -37,13 +37,9 @@ struct TailArgs {
     double tol;
     double rel_floor;                    // as CoarseCtl::rel_floor
     int zero_bc_first;                   // 0 only when S is the finest level
-    int wave_cells;                      // levels of at most this many padded cells run on wave 0 alone
     int *iters_out;
 };
 
-// default of TailArgs::wave_cells: levels of at most this many padded cells
-// run their loops on wave 0 alone, without barriers
-constexpr int kTailWaveCells = 512;
 
 // One tail level, every array an LDS element offset.
 template <typename T>
@@ -290,11 +286,13 @@ __device__ int tail_level(T *lds, const TailLevel<T> &V, int count, T w, double 
         }
         lsync<WAVE>();
     };
+    // imposeBC, red, the periodic refresh, black: one loop so that the
+    // colour pass's code appears once
     auto sweep = [&]() {
-        ghosts(true);
-        colour(0);
-        ghosts(false);
-        colour(1);
+        for (int col = 0; col < 2; ++col) {
+            ghosts(col == 0);
+            colour(col);
+        }
     };
 
     int iter = 0;
@@ -402,91 +400,109 @@ __global__ void __launch_bounds__(512) k_tail(TailArgs<T> A) {
         __syncthreads();
     }
     auto padded = [&](const TailLevel<T> &V) { return DIM == 3 ? (V.n[0] + 2) * V.PY * V.PZ : (V.n[0] + 2) * V.PZ; };
-    // one level's smoothing or coarsest solve, on wave 0 alone when it is small
+    // one level's smoothing or coarsest solve by the whole workgroup (a
+    // wave-only variant for the smallest levels measured no faster and
+    // doubled the kernel's code)
     auto level = [&](int l, bool solve, int count) {
         const TailLevel<T> V = TL[l - S];
-        int it = 0;
-        if (padded(V) <= A.wave_cells) {
-            if (threadIdx.x < 64) {
-                if (solve) it = tail_level<T, DIM, true, true>(lds, V, 0, A.w, A.tol, A.rel_floor, A.max_count, red);
-                else tail_level<T, DIM, true, false>(lds, V, count, A.w, A.tol, A.rel_floor, A.max_count, red);
+        if (solve) return tail_level<T, DIM, false, true>(lds, V, 0, A.w, A.tol, A.rel_floor, A.max_count, red);
+        tail_level<T, DIM, false, false>(lds, V, count, A.w, A.tol, A.rel_floor, A.max_count, red);
+        return 0;
+    };
+
+    // The V-cycle over levels S..D as one loop of steps, so that each phase's
+    // code appears once in the kernel (a single CU runs this kernel, and its
+    // instruction cache holds far less than one copy per call site):
+    //   step 0: level S -- the coarsest treatment when S == D > 0 (the host
+    //     restricted into it), otherwise pre-smoothing, which is also the
+    //     whole V-cycle of a single-level hierarchy (poisson.cc:228);
+    //   down-leg, per l = S..D-1: the residual of l restricted into the
+    //     right-hand side of l + 1 (whose iterate starts at zero), then
+    //     pre-smoothing of l + 1 or the coarsest solve;
+    //   up-leg, per l = D-1..S: injection plus the saved iterate, then
+    //     post-smoothing of l.
+    int iters = 0;  // meaningful in thread 0
+    const int nsteps = 1 + 4 * (D - S);
+    for (int st = 0; st < nsteps; ++st) {
+        int kind = 0, l = S, count = A.pre;   // kind 0: smooth/solve l, 1: restrict l, 2: prolong into l
+        bool solve = false;
+        if (st == 0) {
+            solve = S == D && S > 0 && A.solve_flag;
+        } else if (st <= 2 * (D - S)) {
+            const int d = st - 1;
+            l = S + d / 2;
+            if ((d & 1) == 0) {
+                kind = 1;
+            } else {
+                l += 1;
+                solve = l == D && A.solve_flag;
+            }
+        } else {
+            const int u = st - 1 - 2 * (D - S);
+            l = D - 1 - u / 2;
+            kind = (u & 1) == 0 ? 2 : 0;
+            count = A.post;
+        }
+        if (kind == 1) {
+            const TailLevel<T> F = TL[l - S];
+            const TailLevel<T> C = TL[l + 1 - S];
+            const int fsx = DIM == 3 ? F.PY * F.PZ : F.PZ, fsy = F.PZ;
+            const T *XF = lds + F.x;
+            T *XC = lds + C.x;
+            for (int q = threadIdx.x; q < padded(C); q += blockDim.x) XC[q] = T(0);
+            const int ctotal = C.n[0] * C.n[1] * C.n[2];
+            for (int t = threadIdx.x; t < ctotal; t += blockDim.x) {
+                int I, J, K;
+                tcoords(t, C.n[1], C.n[2], I, J, K);
+                T res[2][2][2];
+                for (int di = 0; di < 2; ++di)
+                    for (int dj = 0; dj < (DIM == 3 ? 2 : 1); ++dj)
+                        for (int dk = 0; dk < 2; ++dk) {
+                            const int i = 2 * I + di, j = DIM == 3 ? 2 * J + dj : 0, k = 2 * K + dk;
+                            TCell<T> r;
+                            r.c = tidx<DIM>(F.PY, F.PZ, i, j, k);
+                            r.c2x = lds[F.c2[0] + i];
+                            r.c1x = lds[F.c1[0] + i];
+                            r.c2y = DIM == 3 ? lds[F.c2[1] + j] : T(0);
+                            r.c1y = DIM == 3 ? lds[F.c1[1] + j] : T(0);
+                            r.c2z = lds[F.c2[2] + k];
+                            r.c1z = lds[F.c1[2] + k];
+                            r.b = lds[F.b + (i * F.n[1] + j) * F.n[2] + k];
+                            res[di][dj][dk] = tresidual<T, DIM>(XF, r, fsx, fsy);
+                        }
+                T out;
+                if (DIM == 3) {
+                    const T lx = lds[C.lo[0] + I], hx = lds[C.hi[0] + I], ly = lds[C.lo[1] + J],
+                            hy = lds[C.hi[1] + J], lz = lds[C.lo[2] + K], hz = lds[C.hi[2] + K];
+                    const T v000 = lx * ly * lz, v100 = hx * ly * lz, v010 = lx * hy * lz, v001 = lx * ly * hz;
+                    const T v011 = lx * hy * hz, v101 = hx * ly * hz, v110 = hx * hy * lz, v111 = hx * hy * hz;
+                    const T vtot = lds[C.wd[0] + I] * lds[C.wd[1] + J] * lds[C.wd[2] + K];
+                    out = (v111 * res[1][1][1] + v000 * res[0][0][0] + v110 * res[1][1][0] + v001 * res[0][0][1] +
+                           v101 * res[1][0][1] + v010 * res[0][1][0] + v011 * res[0][1][1] + v100 * res[1][0][0]) /
+                          vtot;
+                } else {
+                    out = (res[1][0][1] + res[0][0][0] + res[1][0][0] + res[0][0][1]) / T(4);
+                }
+                lds[C.b + t] = out;
+            }
+            __syncthreads();
+        } else if (kind == 2) {
+            const TailLevel<T> F = TL[l - S];
+            const TailLevel<T> C = TL[l + 1 - S];
+            const int total = F.n[0] * F.n[1] * F.n[2];
+            T *XF = lds + F.x;
+            const T *XC = lds + C.x;
+            for (int t = threadIdx.x; t < total; t += blockDim.x) {
+                int i, j, k;
+                tcoords(t, F.n[1], F.n[2], i, j, k);
+                const int c = tidx<DIM>(F.PY, F.PZ, i, j, k);
+                XF[c] = XC[tidx<DIM>(C.PY, C.PZ, i >> 1, j >> 1, k >> 1)] + XF[c];
             }
             __syncthreads();
         } else {
-            if (solve) it = tail_level<T, DIM, false, true>(lds, V, 0, A.w, A.tol, A.rel_floor, A.max_count, red);
-            else tail_level<T, DIM, false, false>(lds, V, count, A.w, A.tol, A.rel_floor, A.max_count, red);
+            const int it = level(l, solve, count);
+            if (solve) iters = it;
         }
-        return it;
-    };
-
-    int iters = 0;  // meaningful in thread 0
-    // level S: the coarsest treatment when S == D > 0 (the host restricted into
-    // it); otherwise pre-smoothing, which is also the whole V-cycle of a
-    // single-level hierarchy (poisson.cc:228, no coarse solve)
-    if (S == D && S > 0 && A.solve_flag) iters = level(S, true, 0);
-    else level(S, false, A.pre);
-    // down-leg: residual of level l restricted into the right-hand side of
-    // l + 1, whose iterate starts at zero
-    for (int l = S; l < D; ++l) {
-        const TailLevel<T> F = TL[l - S];
-        const TailLevel<T> C = TL[l + 1 - S];
-        const int fsx = DIM == 3 ? F.PY * F.PZ : F.PZ, fsy = F.PZ;
-        const T *XF = lds + F.x;
-        T *XC = lds + C.x;
-        for (int q = threadIdx.x; q < padded(C); q += blockDim.x) XC[q] = T(0);
-        const int ctotal = C.n[0] * C.n[1] * C.n[2];
-        for (int t = threadIdx.x; t < ctotal; t += blockDim.x) {
-            int I, J, K;
-            tcoords(t, C.n[1], C.n[2], I, J, K);
-            T res[2][2][2];
-            for (int di = 0; di < 2; ++di)
-                for (int dj = 0; dj < (DIM == 3 ? 2 : 1); ++dj)
-                    for (int dk = 0; dk < 2; ++dk) {
-                        const int i = 2 * I + di, j = DIM == 3 ? 2 * J + dj : 0, k = 2 * K + dk;
-                        TCell<T> r;
-                        r.c = tidx<DIM>(F.PY, F.PZ, i, j, k);
-                        r.c2x = lds[F.c2[0] + i];
-                        r.c1x = lds[F.c1[0] + i];
-                        r.c2y = DIM == 3 ? lds[F.c2[1] + j] : T(0);
-                        r.c1y = DIM == 3 ? lds[F.c1[1] + j] : T(0);
-                        r.c2z = lds[F.c2[2] + k];
-                        r.c1z = lds[F.c1[2] + k];
-                        r.b = lds[F.b + (i * F.n[1] + j) * F.n[2] + k];
-                        res[di][dj][dk] = tresidual<T, DIM>(XF, r, fsx, fsy);
-                    }
-            T out;
-            if (DIM == 3) {
-                const T lx = lds[C.lo[0] + I], hx = lds[C.hi[0] + I], ly = lds[C.lo[1] + J], hy = lds[C.hi[1] + J],
-                        lz = lds[C.lo[2] + K], hz = lds[C.hi[2] + K];
-                const T v000 = lx * ly * lz, v100 = hx * ly * lz, v010 = lx * hy * lz, v001 = lx * ly * hz;
-                const T v011 = lx * hy * hz, v101 = hx * ly * hz, v110 = hx * hy * lz, v111 = hx * hy * hz;
-                const T vtot = lds[C.wd[0] + I] * lds[C.wd[1] + J] * lds[C.wd[2] + K];
-                out = (v111 * res[1][1][1] + v000 * res[0][0][0] + v110 * res[1][1][0] + v001 * res[0][0][1] +
-                       v101 * res[1][0][1] + v010 * res[0][1][0] + v011 * res[0][1][1] + v100 * res[1][0][0]) / vtot;
-            } else {
-                out = (res[1][0][1] + res[0][0][0] + res[1][0][0] + res[0][0][1]) / T(4);
-            }
-            lds[C.b + t] = out;
-        }
-        __syncthreads();
-        if (l + 1 == D && A.solve_flag) iters = level(D, true, 0);
-        else level(l + 1, false, A.pre);
-    }
-    // up-leg: injection plus the saved iterate, then post-smoothing
-    for (int l = D - 1; l >= S; --l) {
-        const TailLevel<T> F = TL[l - S];
-        const TailLevel<T> C = TL[l + 1 - S];
-        const int total = F.n[0] * F.n[1] * F.n[2];
-        T *XF = lds + F.x;
-        const T *XC = lds + C.x;
-        for (int t = threadIdx.x; t < total; t += blockDim.x) {
-            int i, j, k;
-            tcoords(t, F.n[1], F.n[2], i, j, k);
-            const int c = tidx<DIM>(F.PY, F.PZ, i, j, k);
-            XF[c] = XC[tidx<DIM>(C.PY, C.PZ, i >> 1, j >> 1, k >> 1)] + XF[c];
-        }
-        __syncthreads();
-        level(l, false, A.post);
     }
     // level S back to global memory
     {

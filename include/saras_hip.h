@@ -263,10 +263,12 @@ saras_status saras_nse_wiring(int prob_type, int planar, int non_hg, saras_nse_p
  * timestep.cc:57-122) with the fields of hydro/scalar (vfield, sfield).
  * mg: the "Multigrid" keys of the pressure solve (bc is forced to Neumann,
  * poisson.cc:87).  grid->dim 2 is the PLANAR (x-z) build: n[1] = 1, caller
- * arrays (nx+2p, 1, nz+2p), dscheme 1 with any non-linear term (lsRK3_d2.cc,
- * eulerCN_d2.cc).  SARAS_BAD_SHAPE for cases the path does
- * not cover: pad < dscheme, and Morinishi runs with a wall in z and ny != nz
- * (the reference's z edge test reads core.ubound(1), vfield.cc:434). */
+ * arrays (nx+2p, 1, nz+2p), either dscheme with any non-linear term
+ * (lsRK3_d2.cc, eulerCN_d2.cc).  Morinishi runs with dscheme 2 and a wall in z
+ * whose z edge test (on core.ubound(1), vfield.cc:329, 434) misses the last z
+ * cell -- n_y != n_z, or planar -- read the second ghost layer above the top
+ * wall, which keeps the values the caller's arrays hold at set_fields.
+ * SARAS_BAD_SHAPE for extents below 4; SARAS_BAD_ARG for pad < dscheme. */
 saras_status saras_nse_create(const saras_grid_desc *grid, const saras_mg_params *mg,
                               const saras_nse_params *params, saras_nse **out);
 saras_status saras_nse_destroy(saras_nse *s);

@@ -14,9 +14,11 @@
 //   Neumann wall   ghost = F(edge) - v h_wall
 // Every stencil of the step stays within one ghost layer on a wall axis
 // (first and last cells use second-order stencils), so the second layer the
-// reference leaves stale is never needed.  The one stored ghost layer is
-// that of a nullBC pressure wall (boundary.h:205-212), whose ghosts no rule
-// gives: they keep what the whole-array updates of P leave there.
+// reference leaves stale is not needed -- except above a z-hi wall for
+// Morinishi's term when n_y != n_z (its edge test reads the y extent), where
+// that layer is stored.  The other stored ghost layer is that of a nullBC
+// pressure wall (boundary.h:205-212), whose ghosts no rule gives: they keep
+// what the whole-array updates of P leave there.
 //
 // Floating point: the expressions below keep the operand order of the cited
 // reference lines and are built with -ffp-contract=off, so the FP64 results
@@ -38,6 +40,12 @@ struct FieldBC {
     // stored values, indexed by the two other core indices (x walls: j nz + k,
     // y walls: i nz + k, z walls: i ny + j)
     double *ghost[6];
+    // the second ghost layer above a z-hi wall (i ny + j), stored where a
+    // stencil reads it: Morinishi's dScheme 2 edge test on z uses the y extent
+    // (vfield.cc:329, 434; sfield.cc:236, 295), so when n_y != n_z the last
+    // z cell takes the 4th-order difference.  No imposeBCs and no step
+    // writes that layer, so it keeps the caller's values.
+    const double *top2;
 };
 
 // index of the ghost of core cell (c0, c1, c2) across a wall of axis ax in
@@ -108,6 +116,7 @@ struct CoreField {
         // select between the two walls of this axis without a run-time index
         // into the rule table (keeps the table in kernel-argument memory)
         const bool hi = q >= n;
+        if (ax == 2 && q == n + 1 && bc.top2) return bc.top2[(long)c[0] * G.n[1] + c[1]];
         const int type = hi ? bc.type[2 * ax + 1] : bc.type[2 * ax];
         const double val = hi ? bc.val[2 * ax + 1] : bc.val[2 * ax];
         const double kh = hi ? bc.kh[2 * ax + 1] : bc.kh[2 * ax];

@@ -61,6 +61,8 @@ struct saras_nse {
     FieldBC bc[4]{};                     // Vx, Vy, Vz, T
     FieldBC pbc{};                       // P
     double *pghost[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};   // nullBC walls of P
+    // second ghost layer above the z-hi wall of Vx, Vy, Vz, T (FieldBC::top2)
+    double *top2[4] = {nullptr, nullptr, nullptr, nullptr};
     FieldBC ppbc{};                      // Pp: the Poisson solver's walls (Neumann; a planar plume's top: Dirichlet)
     double *F[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};   // Vx, Vy, Vz, T, P
     double *H[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -116,6 +118,7 @@ struct saras_nse {
         (void)hipFree(stage_b);
         for (auto *q : Tsg) (void)hipFree(q);
         for (auto *q : pghost) (void)hipFree(q);
+        for (auto *q : top2) (void)hipFree(q);
         (void)hipFree(sgs_part);
         (void)hipFree(sgs_dev);
         (void)hipHostFree(sgs_host);
@@ -423,6 +426,34 @@ saras_status load_ghost_planes(saras_nse *s, const double *src, int ptr_kind) {
     return SARAS_OK;
 }
 
+// the second ghost layer above the z-hi wall of a caller array (pad p >= 2)
+// -> plane (i ny + j)
+__global__ void k_top2_load(const double *__restrict__ src, long SX, long SY, int p, int py, int nx, int ny, int nz,
+                            double *__restrict__ dst) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nx * ny) return;
+    const int i = (int)(t / ny), j = (int)(t - (long)i * ny);
+    dst[t] = src[(long)(i + p) * SX + (long)(j + py) * SY + (nz + 1 + p)];
+}
+
+saras_status load_top2(saras_nse *s, const double *src, int ptr_kind, double *dst) {
+    const double *d = src;
+    if (ptr_kind == SARAS_PTR_HOST) {
+        saras_status e = ensure_stage(s);
+        if (e) return e;
+        NSE_TRY(hipMemcpyAsync(s->stage_a, src, s->full * sizeof(double), hipMemcpyHostToDevice, s->s));
+        d = s->stage_a;
+    }
+    const int p = s->g.pad, py = s->ypad();
+    const long SY = s->g.n[2] + 2 * p, SX = SY * (s->G.n[1] + 2 * py);
+    const long cells = (long)s->G.n[0] * s->G.n[1];
+    hipLaunchKernelGGL(k_top2_load, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, s->s, d, SX, SY, p, py,
+                       s->G.n[0], s->G.n[1], s->G.n[2], dst);
+    NSE_TRY(hipGetLastError());
+    if (ptr_kind == SARAS_PTR_HOST) NSE_TRY(hipStreamSynchronize(s->s));
+    return SARAS_OK;
+}
+
 __global__ void k_store_core(const double *__restrict__ src, long sx, long sy, double *__restrict__ dst, long SX,
                              long SY, int p, int py, int nz) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -478,8 +509,8 @@ saras_status saras_nse_create(const saras_grid_desc *grid, const saras_mg_params
     // pad of 1 (grid.cc:92-95) is never read, so planar caller arrays carry
     // no y pad.  With dScheme 2 the Morinishi z edge test reads
     // core.ubound(1) = 0 (vfield.cc:329), so the last z cell takes the 4th-order
-    // stencil: that reads the second ghost layer, valid only on a periodic z
-    // axis (refused below for z walls, as for 3-D grids with ny != nz).
+    // stencil: on a z wall that reads the second ghost layer above it, stored
+    // as for 3-D grids with ny != nz.
     if (planar && (P.les || g.n[1] != 1)) {
         saras::set_last_error("the planar path covers n_y = 1 without the LES model (3-D only in the reference)");
         return SARAS_BAD_ARG;
@@ -491,10 +522,6 @@ saras_status saras_nse_create(const saras_grid_desc *grid, const saras_mg_params
     }
     for (int a = 0; a < 3; ++a)
         if (g.n[a] < 4 && !(planar && a == 1)) return SARAS_BAD_SHAPE;
-    if (P.nlscheme == 3 && P.dscheme == 2 && !g.periodic[2] && g.n[1] != g.n[2]) {
-        saras::set_last_error("Morinishi with a z wall needs ny == nz (edge test on core.ubound(1), vfield.cc:434)");
-        return SARAS_BAD_SHAPE;
-    }
     for (int w = 0; w < 6; ++w) {
         // nullBC: a pressure wall of a wall axis (the planar y walls are none)
         const bool null_ok = P.pbc_type[w] == SARAS_FBC_NULL && !g.periodic[w / 2] && !(planar && w / 2 == 1);
@@ -657,6 +684,17 @@ saras_status saras_nse_create(const saras_grid_desc *grid, const saras_mg_params
         (void)hipMemset(s->pghost[w], 0, cells * sizeof(double));
         s->pbc.ghost[w] = s->pghost[w];
     }
+    // Morinishi (dScheme 2) on a z-walled grid whose z edge test misses the
+    // last z cell: that cell reads the second layer above the top wall
+    if (P.nlscheme == 3 && P.dscheme == 2 && !g.periodic[2] && G.edge_last[2] != g.n[2] - 1) {
+        const size_t cells = (size_t)g.n[0] * G.n[1];
+        for (int q = 0; q < 4; ++q) {
+            if (q == 3 && !P.has_T) continue;
+            if (hipMalloc((void **)&s->top2[q], cells * sizeof(double)) != hipSuccess) return fail(SARAS_OOM);
+            (void)hipMemset(s->top2[q], 0, cells * sizeof(double));
+            s->bc[q].top2 = s->top2[q];
+        }
+    }
     {
         // Pp's ghosts as the solver's last imposeBC leaves them: Neumann, and
         // -Pp(n-1) at a Dirichlet top
@@ -772,6 +810,11 @@ saras_status saras_nse_set_fields(saras_nse *s, const double *vx, const double *
     }
     if (p) {
         saras_status e = load_ghost_planes(s, p, ptr_kind);
+        if (e) return e;
+    }
+    for (int q = 0; q < 4; ++q) {
+        if (!src[q] || !s->top2[q]) continue;
+        saras_status e = load_top2(s, src[q], ptr_kind, s->top2[q]);
         if (e) return e;
     }
     if (pp) {

@@ -112,6 +112,9 @@ PLANAR_CASES = [
     ("box-d2-central-rk3", (16, 1, 16), (0, 0, 0), (2, 0, 1), (1.1, 1.0, 0.9), 2, 2, 1, 0),
     ("box-d2-upwind-euler", (16, 1, 16), (0, 0, 0), (2, 0, 1), (1.1, 1.0, 0.9), 2, 2, 2, 1),
     ("periodic-d2-morinishi-rk3", (16, 1, 24), (1, 0, 1), (0, 0, 0), (1.0, 1.0, 1.0), 2, 2, 3, 0),
+    # z walls with dScheme 2: the z edge test reads core.ubound(1) = 0, so the
+    # last z cell reads the second ghost layer above the top wall
+    ("rbc-d2-morinishi-rk3", (24, 1, 16), (1, 0, 0), (0, 0, 2), (1.0, 1.0, 1.2), 2, 2, 3, 0),
 ]
 
 
@@ -130,6 +133,9 @@ def test_planar_steps_bitexact(case):
     f["t"][c] = 0.5 + 0.05 * rng.standard_normal((n[0], n[2]))
     f["vx"][c] = 0.1 * rng.standard_normal((n[0], n[2]))
     f["vz"][c] = 0.1 * rng.standard_normal((n[0], n[2]))
+    if pad == 2 and not periodic[2]:
+        for k in ("vx", "vz", "t"):   # the second layer above the top wall, kept as given
+            f[k][:, 0, -1] = rng.uniform(-0.5, 0.5, sh[0])
     so.set(**f)
     ss.set(**f)
     assert_planar_equal(so, ss, n, pad, periodic)
@@ -173,11 +179,6 @@ def test_planar_plume_bitexact():
 
 def test_planar_rejects_unsupported_schemes():
     mg = S.mg_params(precision=S.PREC_FP64, bc=S.BC_NEUMANN)
-    # dScheme 2 Morinishi with z walls: the last z cell's 4th-order stencil
-    # reads the second ghost layer (the edge test reads core.ubound(1) = 0)
-    g2 = S.grid_desc((16, 1, 16), dim=2, periodic=(1, 1, 0), pad=2)
-    with pytest.raises(S.SarasError):
-        S.Nse(g2, mg, S.nse_params(dscheme=2, nlscheme=3, has_T=0, nu=0.01, kappa=0.01, dt=1e-3))
     gp = S.grid_desc((16, 1, 16), dim=2, periodic=(1, 1, 1), pad=2)
     with pytest.raises(S.SarasError):   # the LES model is 3-D only
         S.Nse(gp, mg, S.nse_params(dscheme=2, nlscheme=3, has_T=0, nu=0.01, kappa=0.01, dt=1e-3, les=1))

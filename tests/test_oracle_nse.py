@@ -909,3 +909,34 @@ def test_null_pressure_wall_keeps_the_whole_array_sum(scheme):
     for _ in range(2):
         assert s2.step() == 0
     assert not np.array_equal(core(s2.get()["vx"], p), core(s.get()["vx"], p))
+
+
+@pytest.mark.parametrize("n,reads", [((8, 8, 8), False), ((8, 6, 8), True)], ids=["ny=nz", "ny<nz"])
+def test_morinishi_z_edge_test_reads_second_top_layer_only_when_ny_ne_nz(n, reads):
+    """Morinishi's z edge test compares k with core.ubound(1) (vfield.cc:329,
+    434; sfield.cc:236, 295): with n_y == n_z the last z cell is an edge cell
+    and the second ghost layer above the top wall is never read; with
+    n_y != n_z that cell takes the 4th-order stencil and reads it.  The
+    layer's values change the RHS of the last z cells alone, or nothing."""
+    p = 2
+    g, d, s = make(n, periodic=(1, 1, 0), pad=p, dscheme=2, nlscheme=3, has_T=1, nu=3e-3, kappa=2e-3)
+    rng = np.random.default_rng(8)
+    base = {k: rng.uniform(-0.3, 0.3, s.shape) for k in ("vx", "vy", "vz", "t")}
+    base.update(p=np.zeros(s.shape), pp=np.zeros(s.shape))
+    out = []
+    for bump in (0.0, 0.5):
+        f = {k: v.copy() for k, v in base.items()}
+        for k in ("vx", "vy", "vz", "t"):
+            f[k][:, :, -1] += bump
+        s2 = O.Nse(d)
+        s2.set(**f)
+        out.append(s2.rhs(0))
+    c = (slice(p, -p),) * 3
+    for k in out[0]:
+        a, b = out[0][k][c], out[1][k][c]
+        diff = np.argwhere(a != b)
+        if not reads:
+            assert diff.size == 0, k
+        elif k in ("hx", "hy", "hz", "ht", "rx", "ry", "rz", "rt"):
+            assert diff.size > 0, k
+            assert (diff[:, 2] == n[2] - 1).all(), k

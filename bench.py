@@ -361,6 +361,8 @@ def les_step(steps, warmup, n=384, noise=0.01, seed=5):
         d = s.diag()
         sg, t = s.sgs()
         res[name] = dict(ms=ms, st=st, d=d, sg=sg, t=t)
+        if prec == S.PREC_MIXED:
+            res[name]["sgs_ms"] = s.bench_sgs(3)
         fields[name] = {k: v[(slice(pad, -pad),) * 3].copy() for k, v in s.get().items() if k in ("vx", "vy", "vz", "p")}
         s.close()
     vol = L ** 3
@@ -371,7 +373,23 @@ def les_step(steps, warmup, n=384, noise=0.01, seed=5):
                 ke=m["d"]["ke"], ke_fp64_pressure=res["fp64"]["d"]["ke"], divmax=m["d"]["divmax"],
                 sgs_ke=m["sg"]["sgke"] / vol, sgs_dissipation=m["sg"]["disp"] / vol, nu_sgs=m["sg"]["nusg"] / vol,
                 mixed_vs_fp64_pressure=field_diffs(fields["mixed"], fields["fp64"]),
-                steps_compared=max(warmup, 6) + steps)
+                steps_compared=max(warmup, 6) + steps, sgs_stress=sgs_rate(m["sgs_ms"], n))
+
+
+# FP64 operations per cell of the sub-grid stress kernel at 384^3, from
+# rocprofv3 --pmc SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 (wave instructions x
+# 64 lanes, an FMA counted twice): 317.46 GFLOP per launch.
+# Source: profiles/les_step_384cubed.md
+SGS_FLOP_PER_CELL = 5606.5
+FP64_VECTOR_PEAK_TFLOPS = 78.6
+
+
+def sgs_rate(ms, n):
+    """The compute-bound sub-grid stress kernel, outside the TB/s metric:
+    its launch time on the handle's stream and the FP64 rate it reaches."""
+    tf = SGS_FLOP_PER_CELL * float(n) ** 3 / (ms * 1e-3) / 1e12
+    return dict(ms_per_launch=round(ms, 4), flop_per_cell=SGS_FLOP_PER_CELL, tflops_fp64=round(tf, 2),
+                peak_fp64_vector_tflops=FP64_VECTOR_PEAK_TFLOPS, frac=round(tf / FP64_VECTOR_PEAK_TFLOPS, 4))
 
 
 def serving(count=240, seed=4, slots=8, budget=96 << 30):

@@ -385,6 +385,13 @@ saras_status saras_nse_probe_noalloc(saras_nse *s, int count, const int *ijk, do
  * beyond the reference's own convergence checks; *ms = device time. */
 saras_status saras_nse_bench_steps(saras_nse *s, int count, double *ms);
 
+/* Measurement hook of the sub-grid stress kernel alone (the per-cell part
+ * of spiral::computeSG, spiral.cc:157-206 with sgsStress :472-548): `count`
+ * launches on the current velocity, *ms = mean device time per launch.
+ * Writes the handle's stress arrays as a step would.  SARAS_BAD_ARG for a
+ * handle without the model. */
+saras_status saras_nse_bench_sgs(saras_nse *s, int count, double *ms);
+
 /* ---- serving (no counterpart in the reference: these serve many solves
  * per process, the batched request loop the reference does not have). ---- */
 

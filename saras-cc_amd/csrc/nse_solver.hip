@@ -923,6 +923,27 @@ saras_status saras_nse_bench_steps(saras_nse *s, int count, double *ms) {
     return e;
 }
 
+saras_status saras_nse_bench_sgs(saras_nse *s, int count, double *ms) {
+    if (!s || count < 1 || !ms) return SARAS_BAD_ARG;
+    if (!s->p.les) {
+        saras::set_last_error("the handle has no sub-grid model (les = 0)");
+        return SARAS_BAD_ARG;
+    }
+    NSE_TRY(hipStreamSynchronize(s->s));
+    NSE_TRY(hipEventRecord(s->ev0, s->s));
+    for (int c = 0; c < count; ++c)
+        hipLaunchKernelGGL(saras::nse::k_sgs_stress, s->grid3(), dim3(s->threads), 0, s->s, s->G, s->field(0),
+                           s->field(1), s->field(2), s->p.nu, s->Tsg[0], s->Tsg[1], s->Tsg[2], s->Tsg[3], s->Tsg[4],
+                           s->Tsg[5], s->sgs_part, reinterpret_cast<int *>(s->sgs_dev + 3));
+    NSE_TRY(hipGetLastError());
+    NSE_TRY(hipEventRecord(s->ev1, s->s));
+    NSE_TRY(hipStreamSynchronize(s->s));
+    float t = 0.f;
+    NSE_TRY(hipEventElapsedTime(&t, s->ev0, s->ev1));
+    *ms = t / count;
+    return SARAS_OK;
+}
+
 saras_status saras_field_op(saras_nse *s, int op, const double *f, double *out, int ptr_kind) {
     if (!s || !f || !out || op < 0 || op > 7) return SARAS_BAD_ARG;
     if (ptr_kind != SARAS_PTR_HOST && ptr_kind != SARAS_PTR_DEVICE) return SARAS_BAD_ARG;

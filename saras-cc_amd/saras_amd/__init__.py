@@ -96,7 +96,7 @@ EXPORTS = [
     "saras_device_sync", "saras_device_zero", "saras_last_error", "saras_mg_bench_cycles", "saras_mg_bench_sweeps",
     "saras_bench_stream_copy", "saras_nse_create", "saras_nse_physics", "saras_nse_wiring", "saras_nse_destroy", "saras_nse_set_fields",
     "saras_nse_get_fields", "saras_nse_step", "saras_nse_stage", "saras_nse_rhs", "saras_nse_get_rhs",
-    "saras_nse_solve_field", "saras_field_op", "saras_field_vec_op", "saras_nse_diag", "saras_nse_bench_steps",
+    "saras_nse_solve_field", "saras_field_op", "saras_field_vec_op", "saras_nse_diag", "saras_nse_bench_steps", "saras_nse_bench_sgs",
     "saras_nse_cfl", "saras_nse_sgs_diag", "saras_nse_tsdata", "saras_nse_probe", "saras_nse_probe_noalloc", "saras_mg_set_graphs", "saras_mg_footprint", "saras_nse_footprint", "saras_nse_set_time", "saras_nse_set_graphs",
     "saras_server_create", "saras_server_submit", "saras_server_submit_nse",
     "saras_server_poll", "saras_server_drain", "saras_server_clock", "saras_server_mem_used",
@@ -161,6 +161,7 @@ def lib():
         L.saras_nse_probe.argtypes = [vp, C.c_int, C.POINTER(C.c_int), dp]
         L.saras_nse_probe_noalloc.argtypes = [vp, C.c_int, C.POINTER(C.c_int), dp]
         L.saras_nse_bench_steps.argtypes = [vp, C.c_int, dp]
+        L.saras_nse_bench_sgs.argtypes = [vp, C.c_int, dp]
         L.saras_nse_cfl.argtypes = [vp, C.c_double, C.c_double, dp]
         L.saras_nse_sgs_diag.argtypes = [vp, dp, dp]
         L.saras_mg_set_graphs.argtypes = [vp, C.c_int]
@@ -558,6 +559,11 @@ class Nse:
         ms = C.c_double()
         s = lib().saras_nse_bench_steps(self.h, count, C.byref(ms))
         return s, ms.value
+
+    def bench_sgs(self, count):
+        ms = C.c_double()
+        check(lib().saras_nse_bench_sgs(self.h, count, C.byref(ms)))
+        return ms.value
 
 
 def footprint(g, p):

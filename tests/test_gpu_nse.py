@@ -653,3 +653,16 @@ def test_morinishi_z_walls_ny_ne_nz_bitexact(n):
     ss.step()
     assert not np.array_equal(so2.get()["vx"], ss.get()["vx"])
 
+
+
+def test_sgs_bench_hook():
+    """saras_nse_bench_sgs times the sub-grid stress kernel alone; a handle
+    without the model refuses."""
+    so, ss, _ = tg_les_pair(n=16)
+    ms = ss.bench_sgs(2)
+    assert ms > 0.0
+    g = S.grid_desc((8, 8, 8), periodic=(1, 1, 1), pad=2)
+    plain = S.Nse(g, S.mg_params(bc=S.BC_NEUMANN, precision=S.PREC_FP64),
+                  S.nse_params(dscheme=2, nlscheme=3, has_T=0, nu=1e-2, dt=1e-3))
+    with pytest.raises(S.SarasError):
+        plain.bench_sgs(1)

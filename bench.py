@@ -323,12 +323,11 @@ def rbc_step(steps, warmup, n=256):
     return out
 
 
-def les_step(steps, warmup, n=384, noise=0.01, seed=5):
+def tgv_les_case(n, noise=0.01, seed=5):
     """Decaying Taylor-Green vortex in a periodic (2 pi)^3 box, Re = 1600
     (nu = 1/1600, unit velocity and length), seeded uniform noise of 1 % of
-    the peak velocity, spiral sub-grid model, RK3/CN, FP64 stencils and the
-    mixed-precision pressure solve.  The model enters once solTime > 5 dt, so
-    at least six warm-up steps run before the timed ones."""
+    the peak velocity, spiral sub-grid model, RK3/CN with FP64 stencils.
+    Returns the grid, the parameters and the initial fields."""
     pad = 2
     L = 2 * np.pi
     g = S.grid_desc(n, periodic=(1, 1, 1), length=(L, L, L), pad=pad)
@@ -345,14 +344,27 @@ def les_step(steps, warmup, n=384, noise=0.01, seed=5):
     nu = S.nse_physics(2, re=1600.0)["nu"]
     prm = S.nse_params(dscheme=2, nlscheme=3, has_T=0, nu=nu, kappa=nu, dt=1e-3, cn_tol=1e-6,
                        les=1, vbc=[([S.FBC_DIRICHLET] * 6, [0.0] * 6)] * 3)
+    return g, prm, dict(vx=u, vy=v, vz=w, p=zero, t=zero, pp=zero)
+
+
+def tgv_les_mg(prec):
+    # these solves stop before the 20-cycle cap, so the residual is checked
+    # (one refinement) every 2 FP32 cycles
+    return S.mg_params(vc_count=20, vc_tol=1e-6, solve_flag=0, mg_tol=1e-2, pre=3, post=6, bc=S.BC_NEUMANN,
+                       precision=prec, inner_cycles=2, max_refine=10)
+
+
+def les_step(steps, warmup, n=384):
+    """The Taylor-Green LES with the mixed-precision pressure solve, timed;
+    the same run with FP64 pressure solves beside it.  The model enters once
+    solTime > 5 dt, so at least six warm-up steps run before the timed ones."""
+    pad = 2
+    L = 2 * np.pi
+    g, prm, f0 = tgv_les_case(n)
     res, fields = {}, {}
     for name, prec in (("mixed", S.PREC_MIXED), ("fp64", S.PREC_FP64)):
-        # these solves stop before the 20-cycle cap, so the residual is checked
-        # (one refinement) every 2 FP32 cycles
-        mg = S.mg_params(vc_count=20, vc_tol=1e-6, solve_flag=0, mg_tol=1e-2, pre=3, post=6, bc=S.BC_NEUMANN,
-                         precision=prec, inner_cycles=2, max_refine=10)
-        s = S.Nse(g, mg, prm)
-        s.set(vx=u, vy=v, vz=w, p=zero, t=zero, pp=zero)
+        s = S.Nse(g, tgv_les_mg(prec), prm)
+        s.set(**f0)
         for _ in range(max(warmup, 6)):
             r, st = s.step()
             S.check(r)
@@ -374,6 +386,59 @@ def les_step(steps, warmup, n=384, noise=0.01, seed=5):
                 sgs_ke=m["sg"]["sgke"] / vol, sgs_dissipation=m["sg"]["disp"] / vol, nu_sgs=m["sg"]["nusg"] / vol,
                 mixed_vs_fp64_pressure=field_diffs(fields["mixed"], fields["fp64"]),
                 steps_compared=max(warmup, 6) + steps, sgs_stress=sgs_rate(m["sgs_ms"], n))
+
+
+def les_history(steps, n=384):
+    """The Taylor-Green LES over `steps` steps, once with the mixed-precision
+    pressure solve and once with FP64 pressure solves (the FP64 path follows
+    the oracle's history to 1e-8 at 64^3 over 500 steps): per step the run
+    time, KE, max|div| and the sub-grid diagnostics, and the dissipation
+    eps = -dKE/dt by centred differences.  Reports the largest differences of
+    the two histories, a few samples of them, and the final fields'
+    differences."""
+    pad = 2
+    L = 2 * np.pi
+    vol = L ** 3
+    g, prm, f0 = tgv_les_case(n)
+    hist, fields, wall = {}, {}, {}
+    for name, prec in (("mixed", S.PREC_MIXED), ("fp64", S.PREC_FP64)):
+        s = S.Nse(g, tgv_les_mg(prec), prm)
+        s.set(**f0)
+        h = []
+        t0 = time.time()
+        for step in range(steps):
+            r, st = s.step()
+            S.check(r)
+            d = s.diag()
+            sg, t = s.sgs()
+            h.append(dict(time=t, ke=d["ke"], divmax=d["divmax"], sgs_ke=sg["sgke"] / vol,
+                          sgs_dissipation=sg["disp"] / vol, nu_sgs=sg["nusg"] / vol))
+            if (step + 1) % 50 == 0:
+                print(f"les history {name} step {step + 1}: ke {d['ke']:.12e}", file=sys.stderr, flush=True)
+        wall[name] = time.time() - t0
+        hist[name] = h
+        fields[name] = {k: v[(slice(pad, -pad),) * 3].copy() for k, v in s.get().items() if k in ("vx", "vy", "vz", "p")}
+        s.close()
+    a, b = hist["mixed"], hist["fp64"]
+    worst = {}
+    for k in ("ke", "sgs_ke", "sgs_dissipation", "nu_sgs"):
+        worst[k] = max(abs(x[k] - y[k]) / max(abs(y[k]), 1e-300) for x, y in zip(a, b))
+    worst["divmax"] = max(abs(x["divmax"] - y["divmax"]) for x, y in zip(a, b)) / max(y["divmax"] for y in b)
+    dt = 1e-3
+    ke_a = np.array([x["ke"] for x in a])
+    ke_b = np.array([y["ke"] for y in b])
+    eps_a = -(ke_a[2:] - ke_a[:-2]) / (2 * dt)
+    eps_b = -(ke_b[2:] - ke_b[:-2]) / (2 * dt)
+    worst["eps"] = float(np.abs(eps_a - eps_b).max() / np.abs(eps_b).max())
+    marks = sorted(set([0, steps // 4, steps // 2, 3 * steps // 4, steps - 1]))
+    samples = [dict(step=i + 1, time=b[i]["time"], ke_fp64=b[i]["ke"], ke_mixed=a[i]["ke"],
+                    eps_fp64=float(eps_b[i - 1]) if 1 <= i <= steps - 2 else None,
+                    eps_mixed=float(eps_a[i - 1]) if 1 <= i <= steps - 2 else None,
+                    sgs_dissipation_fp64=b[i]["sgs_dissipation"], sgs_dissipation_mixed=a[i]["sgs_dissipation"])
+               for i in marks]
+    return dict(cells=[n, n, n], steps=steps, wall_s={k: round(v, 1) for k, v in wall.items()},
+                mixed_vs_fp64_pressure_history_max_rel=worst, samples=samples,
+                mixed_vs_fp64_pressure_final_fields=field_diffs(fields["mixed"], fields["fp64"]))
 
 
 # FP64 operations per cell of the sub-grid stress kernel at 384^3, from
@@ -565,6 +630,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-les", action="store_true")
     ap.add_argument("--les-only", action="store_true", help="only the 384^3 Taylor-Green LES step")
+    ap.add_argument("--les-history", type=int, default=0, metavar="STEPS",
+                    help="only the 384^3 Taylor-Green LES histories (mixed against FP64 pressure) over STEPS steps")
     ap.add_argument("--serve-only", action="store_true", help="only the batched serving measurement")
     ap.add_argument("--slots", type=int, default=8)
     ap.add_argument("--requests", type=int, default=240)
@@ -580,6 +647,9 @@ def main():
         return
     if args.rbc_only:
         print(json.dumps({"rbc_256_step": rbc_step(args.steps, args.warmup)}))
+        return
+    if args.les_history:
+        print(json.dumps({"les_384_history": les_history(args.les_history)}))
         return
     if args.les_only:
         print(json.dumps({"les_384_step": les_step(args.steps, args.warmup)}))

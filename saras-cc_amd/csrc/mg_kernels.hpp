@@ -301,6 +301,7 @@ __global__ void __launch_bounds__(256) k_prolong_correct(int nx, int ny, int nz,
 // ---------------------------------------------------------------------------
 constexpr int kNormBlocks = 1024;
 constexpr int kNormThreads = 256;
+static_assert(kNormBlocks % kXcds == 0, "norm grids are dealt to XCDs in equal shares");
 
 template <int TYPE>
 __device__ __forceinline__ double norm_acc(double a, double v) {

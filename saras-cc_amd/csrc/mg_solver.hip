@@ -798,13 +798,13 @@ void launch_norm(int type, saras_mg *m, const saras::LevelArgs<double> &a, const
         const unsigned nb = saras::kNormBlocks, nt = saras::kNormThreads;
         if (type == 0)
             hipLaunchKernelGGL((saras::k_refine_split<0, DIM, true>), dim3(nb), dim3(nt), 0, s, a, x, nullptr, 0L, 0L,
-                               0L, b, nullptr, nullptr, m->partial);
+                               0L, b, nullptr, nullptr, m->partial, saras::FullRows{});
         else if (type == 1)
             hipLaunchKernelGGL((saras::k_refine_split<1, DIM, true>), dim3(nb), dim3(nt), 0, s, a, x, nullptr, 0L, 0L,
-                               0L, b, nullptr, nullptr, m->partial);
+                               0L, b, nullptr, nullptr, m->partial, saras::FullRows{});
         else
             hipLaunchKernelGGL((saras::k_refine_split<2, DIM, true>), dim3(nb), dim3(nt), 0, s, a, x, nullptr, 0L, 0L,
-                               0L, b, nullptr, nullptr, m->partial);
+                               0L, b, nullptr, nullptr, m->partial, saras::FullRows{});
         if (type == 0)
             hipLaunchKernelGGL((saras::k_norm_final<0>), dim3(1), dim3(nt), 0, s, m->partial, saras::kNormBlocks, m->norm_out);
         else if (type == 1)
@@ -1119,27 +1119,35 @@ static saras_status solve_fp64(saras_mg *m, hipStream_t s, saras_mg_stats *st) {
 
 }  // extern "C"
 
+template <int TYPE, int DIM, bool CX, bool CB>
+void launch_refine_split(saras_mg *m, const saras::LevelArgs<double> &a, const float *e, long esx, long esy, long ehz,
+                         float *r32, const saras::FullRows &fr, hipStream_t s) {
+    auto &L0 = m->h64.lv[0];
+    const unsigned nb = saras::kNormBlocks, nt = saras::kNormThreads;
+    hipLaunchKernelGGL((saras::k_refine_split<TYPE, DIM, false, CX, CB>), dim3(nb), dim3(nt), 0, s, a, L0.x, e, esx,
+                       esy, ehz, L0.b, m->x64b, r32, m->partial, fr);
+    hipLaunchKernelGGL((saras::k_norm_final<TYPE>), dim3(1), dim3(nt), 0, s, m->partial, saras::kNormBlocks,
+                       m->norm_out);
+}
+
+template <int TYPE, int DIM>
+void launch_refine_split(saras_mg *m, const saras::LevelArgs<double> &a, const float *e, long esx, long esy, long ehz,
+                         float *r32, const saras::FullRows &fr, hipStream_t s) {
+    if (fr.x) launch_refine_split<TYPE, DIM, true, true>(m, a, e, esx, esy, ehz, r32, fr, s);
+    else if (fr.b) launch_refine_split<TYPE, DIM, false, true>(m, a, e, esx, esy, ehz, r32, fr, s);
+    else launch_refine_split<TYPE, DIM, false, false>(m, a, e, esx, esy, ehz, r32, fr, s);
+}
+
+// fr: the caller's arrays when x_old (only with no correction e) and/or b are
+// read from them instead of level 0; empty otherwise
 template <int DIM>
 void launch_refine(int type, saras_mg *m, const saras::LevelArgs<double> &a, const float *e, long esx, long esy,
-                   long ehz, float *r32, hipStream_t s) {
+                   long ehz, float *r32, const saras::FullRows &fr, hipStream_t s) {
     auto &L0 = m->h64.lv[0];
     if (L0.hz && ehz) {
-        const unsigned nb = saras::kNormBlocks, nt = saras::kNormThreads;
-        if (type == 0)
-            hipLaunchKernelGGL((saras::k_refine_split<0, DIM>), dim3(nb), dim3(nt), 0, s, a, L0.x, e, esx, esy, ehz, L0.b,
-                               m->x64b, r32, m->partial);
-        else if (type == 1)
-            hipLaunchKernelGGL((saras::k_refine_split<1, DIM>), dim3(nb), dim3(nt), 0, s, a, L0.x, e, esx, esy, ehz, L0.b,
-                               m->x64b, r32, m->partial);
-        else
-            hipLaunchKernelGGL((saras::k_refine_split<2, DIM>), dim3(nb), dim3(nt), 0, s, a, L0.x, e, esx, esy, ehz, L0.b,
-                               m->x64b, r32, m->partial);
-        if (type == 0)
-            hipLaunchKernelGGL((saras::k_norm_final<0>), dim3(1), dim3(nt), 0, s, m->partial, saras::kNormBlocks, m->norm_out);
-        else if (type == 1)
-            hipLaunchKernelGGL((saras::k_norm_final<1>), dim3(1), dim3(nt), 0, s, m->partial, saras::kNormBlocks, m->norm_out);
-        else
-            hipLaunchKernelGGL((saras::k_norm_final<2>), dim3(1), dim3(nt), 0, s, m->partial, saras::kNormBlocks, m->norm_out);
+        if (type == 0) launch_refine_split<0, DIM>(m, a, e, esx, esy, ehz, r32, fr, s);
+        else if (type == 1) launch_refine_split<1, DIM>(m, a, e, esx, esy, ehz, r32, fr, s);
+        else launch_refine_split<2, DIM>(m, a, e, esx, esy, ehz, r32, fr, s);
         return;
     }
     switch (type) {
@@ -1165,13 +1173,23 @@ void launch_refine(int type, saras_mg *m, const saras::LevelArgs<double> &a, con
 }
 
 // x64 <- x64 + e32 (if with_e), r32 <- b - A x64, residual norm of type res_type
-static saras_status refine_step(saras_mg *m, hipStream_t s, bool with_e, double *res) {
+static saras_status refine_step(saras_mg *m, hipStream_t s, bool with_e, const saras::FullRows &fr, double *res) {
     auto &L0 = m->h64.lv[0];
     auto &F0 = m->h32->lv[0];
     auto a = level_args(m, m->h64, 0, true);
     const float *e = with_e ? F0.x : nullptr;
-    if (m->G.dim == 3) launch_refine<3>(m->p.res_type, m, a, e, F0.sx, F0.sy, F0.hz, F0.b, s);
-    else launch_refine<2>(m->p.res_type, m, a, e, F0.sx, F0.sy, F0.hz, F0.b, s);
+    // host-side guards: the caller-layout reads exist on split rows only, and
+    // x_old is read from the caller's array only for the initial residual
+    if ((fr.x || fr.b) && !(L0.hz && F0.hz)) {
+        set_error("internal error: caller-layout refinement on a level without split rows");
+        return SARAS_HIP_ERROR;
+    }
+    if (fr.x && with_e) {
+        set_error("internal error: caller-layout iterate with a correction");
+        return SARAS_HIP_ERROR;
+    }
+    if (m->G.dim == 3) launch_refine<3>(m->p.res_type, m, a, e, F0.sx, F0.sy, F0.hz, F0.b, fr, s);
+    else launch_refine<2>(m->p.res_type, m, a, e, F0.sx, F0.sy, F0.hz, F0.b, fr, s);
     std::swap(L0.x, m->x64b);
     HIP_TRY(hipMemcpyAsync(m->norm_host, m->norm_out, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
@@ -1187,11 +1205,16 @@ static saras_status refine_step(saras_mg *m, hipStream_t s, bool with_e, double 
 // Mixed precision: FP64 iterate, residual and stopping test; FP32 V-cycles
 // (same smoother, transfers and coarse solve as the reference, homogeneous
 // walls) on the correction equation A e = r.
-static saras_status solve_mixed(saras_mg *m, hipStream_t s, saras_mg_stats *st) {
+// full: the caller's x (initial guess) and b, read in place by the FP64
+// passes when both levels 0 have split rows (empty: level 0 holds them).
+static saras_status solve_mixed(saras_mg *m, hipStream_t s, saras_mg_stats *st,
+                                const saras::FullRows &full = saras::FullRows{}) {
     Ops<float> ops{m, *m->h32, s, false, (float)m->p.sor};
     auto &F0 = m->h32->lv[0];
     double res = 0.0;
-    saras_status e = refine_step(m, s, false, &res);
+    saras::FullRows later = full;
+    later.x = nullptr;
+    saras_status e = refine_step(m, s, false, full, &res);
     if (e) return e;
     int cycles = 0, refs = 0;
     const int cap = m->p.vc_count > 0 ? m->p.vc_count : (1 << 30);
@@ -1226,7 +1249,7 @@ static saras_status solve_mixed(saras_mg *m, hipStream_t s, saras_mg_stats *st) 
             ++ran;
         }
         const double before = res;
-        e = refine_step(m, s, true, &res);
+        e = refine_step(m, s, true, later, &res);
         if (e) return e;
         ++refs;
         if (!std::isfinite(res)) break;
@@ -1311,13 +1334,26 @@ saras_status saras_mg_solve(saras_mg *mg, double *x_full, const double *b_full, 
     m->coarse_capped = 0;
     HIP_TRY(hipEventRecord(m->ev0, s));
     auto &L0 = m->h64.lv[0];
-    load_core<double>(m, db, L0, L0.b, s);
-    load_core<double>(m, dx, L0, L0.x, s);
     saras_mg_stats st{};
     saras_status e = SARAS_OK;
     if (m->p.precision == SARAS_PREC_FP64) {
+        load_core<double>(m, db, L0, L0.b, s);
+        load_core<double>(m, dx, L0, L0.x, s);
         e = solve_fp64(m, s, &st);
+    } else if (L0.hz && m->h32->lv[0].hz) {
+        // the FP64 passes of the mixed solve read b and the initial guess
+        // from the caller's arrays; level 0 receives x from the first pass
+        const long p = m->g.pad, fz = L0.n[2] + 2 * p, fy = m->g.dim == 3 ? L0.n[1] + 2 * p : 1;
+        saras::FullRows fr;
+        fr.x = dx;
+        fr.b = db;
+        fr.si = m->g.dim == 3 ? fy * fz : fz;
+        fr.sj = m->g.dim == 3 ? fz : 0;
+        fr.off = m->g.dim == 3 ? p * fy * fz + p * fz + p : p * fz + p;
+        e = solve_mixed(m, s, &st, fr);
     } else {
+        load_core<double>(m, db, L0, L0.b, s);
+        load_core<double>(m, dx, L0, L0.x, s);
         e = solve_mixed(m, s, &st);
     }
     if (e) return e;
@@ -1579,6 +1615,9 @@ saras_status saras_device_sync(void) {
 saras_status saras_device_zero(void *ptr, size_t bytes) {
     if (!ptr) return SARAS_BAD_ARG;
     HIP_TRY(hipMemset(ptr, 0, bytes));
+    // hipMemset on device memory may return before the fill has run; the
+    // solver handles' streams do not wait for the null stream
+    HIP_TRY(hipDeviceSynchronize());
     return SARAS_OK;
 }
 

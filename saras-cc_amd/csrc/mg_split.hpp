@@ -795,12 +795,21 @@ __global__ void __launch_bounds__(kSplitThreads) k_prolong_split(int nx, int ny,
 // ---------------------------------------------------------------------------
 // NORM_ONLY: the residual norm partials of computeError alone
 // (poisson_d3.cc:563-663): no correction, nothing written but the partials.
-template <int TYPE, int DIM, bool NORM_ONLY = false>
+// CX / CB: x_old / b are read from the caller's padded array (FullRows: the
+// core cell (i, j, k) at off + i si + j sj + k) instead of the level's split
+// rows, so the solve needs no separate layout conversion of its inputs; the
+// values, and so the results, are the same.
+struct FullRows {
+    const double *x = nullptr, *b = nullptr;
+    long si = 0, sj = 0, off = 0;
+};
+
+template <int TYPE, int DIM, bool NORM_ONLY = false, bool CX = false, bool CB = false>
 __global__ void __launch_bounds__(kNormThreads) k_refine_split(LevelArgs<double> L, const double *__restrict__ xold,
                                                                const float *__restrict__ e, long esx, long esy,
                                                                long ehz, const double *__restrict__ b,
                                                                double *__restrict__ xnew, float *__restrict__ r32,
-                                                               double *__restrict__ partial) {
+                                                               double *__restrict__ partial, FullRows F) {
     constexpr int W = 2;
     const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
     const int half = nz >> 1, q = half / W;
@@ -810,6 +819,14 @@ __global__ void __launch_bounds__(kNormThreads) k_refine_split(LevelArgs<double>
     double ar = 0.0, ab = 0.0;
     // rows of the x_old + e iterate: even and odd halves at m0
     auto row_vals = [&](int ii, int jj, int m0, double (&ve)[W], double (&vo)[W]) {
+        if (CX) {
+            const double *__restrict__ xr = F.x + F.off + (long)ii * F.si + (long)jj * F.sj + 2 * m0;
+            ve[0] = xr[0];
+            vo[0] = xr[1];
+            ve[1] = xr[2];
+            vo[1] = xr[3];
+            return;
+        }
         const long rb = (long)ii * L.sx + (long)jj * L.sy;
         const double2 xe = *reinterpret_cast<const double2 *>(xold + rb + m0);
         const double2 xo = *reinterpret_cast<const double2 *>(xold + rb + hz + m0);
@@ -827,16 +844,40 @@ __global__ void __launch_bounds__(kNormThreads) k_refine_split(LevelArgs<double>
             vo[1] = vo[1] + (double)fo.y;
         }
     };
-    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+    // Rows are walked in one of two orders.  Where a level's neighbour
+    // planes exceed half an XCD's L2 (L.rtile, set for the colour passes too),
+    // each XCD walks one contiguous range of j-tiled rows, its blocks striding
+    // through it together, so a row's x and y neighbours were fetched into
+    // the same L2 a few rows before; otherwise the grid strides over all rows
+    // in order (the level's x and b are then served from the Infinity Cache
+    // and this order measured faster).  The grid is a multiple of kXcds.
+    const bool xwalk = L.rtile != 0;
+    const int per = xwalk ? gridDim.x / kXcds : gridDim.x;
+    const int xcd = xwalk ? blockIdx.x % kXcds : 0, kb = xwalk ? blockIdx.x / kXcds : blockIdx.x;
+    const long rows = (long)nx * ny;
+    const long xrows = xwalk ? (rows + kXcds - 1) / kXcds : rows;
+    const long lo = (long)xcd * xrows * q, hi = min(total, lo + xrows * q);
+    for (long t = lo + (long)kb * blockDim.x + threadIdx.x; t < hi; t += (long)per * blockDim.x) {
         const int row = (int)(t / q);
         const int v = (int)(t - (long)row * q);
-        const int i = row / ny, j = row - (row / ny) * ny;
+        int i, j;
+        if (xwalk) {
+            const int jt = L.rtile, tile = row / (nx * jt), rem = row - tile * (nx * jt);
+            i = rem >> __builtin_ctz(jt);
+            j = tile * jt + (rem & (jt - 1));
+        } else {
+            i = row / ny;
+            j = row - i * ny;
+        }
         const int m0 = v * W;
         const long rb = (long)i * L.sx + (long)j * L.sy;
         const long eb = (long)i * esx + (long)j * esy;
-        auto cell = [&](long off64, long off32) {
-            double x = xold[rb + off64];
-            if (e) x = x + (double)e[eb + off32];
+        // the iterate at cell k of this row
+        auto cell = [&](int k) {
+            if (CX) return F.x[F.off + (long)i * F.si + (long)j * F.sj + k];
+            const long h = k & 1, m = k >> 1;
+            double x = xold[rb + h * hz + m];
+            if (e) x = x + (double)e[eb + h * ehz + m];
             return x;
         };
         // the z neighbours beyond this lane's vectors that no adjacent lane
@@ -845,8 +886,8 @@ __global__ void __launch_bounds__(kNormThreads) k_refine_split(LevelArgs<double>
         const bool need_lo = m0 == 0 ? L.bc_lo[2] == BC_PER : lane == 0;
         const bool need_hi = m0 + W == half ? L.bc_hi[2] == BC_PER : lane == 63;
         double zl = 0.0, zh = 0.0;
-        if (need_lo) zl = m0 == 0 ? cell(hz + half - 1, ehz + half - 1) : cell(hz + m0 - 1, ehz + m0 - 1);
-        if (need_hi) zh = m0 + W == half ? cell(0, 0) : cell(m0 + W, m0 + W);
+        if (need_lo) zl = cell(m0 == 0 ? nz - 1 : 2 * m0 - 1);
+        if (need_hi) zh = cell(m0 + W == half ? 0 : 2 * (m0 + W));
         double ce[W], co[W];
         row_vals(i, j, m0, ce, co);
         // x/y neighbours: [0] E, [1] W, [2] N, [3] S
@@ -905,9 +946,21 @@ __global__ void __launch_bounds__(kNormThreads) k_refine_split(LevelArgs<double>
         }
         const double c2x = L.c2[0][i], c1x = L.c1[0][i];
         const double c2y = DIM == 3 ? L.c2[1][j] : 0.0, c1y = DIM == 3 ? L.c1[1][j] : 0.0;
-        const double2 be = *reinterpret_cast<const double2 *>(b + rb + m0);
-        const double2 bo = *reinterpret_cast<const double2 *>(b + rb + hz + m0);
-        const double bev[W] = {be.x, be.y}, bov[W] = {bo.x, bo.y};
+        double bev[W], bov[W];
+        if (CB) {
+            const double *__restrict__ br = F.b + F.off + (long)i * F.si + (long)j * F.sj + 2 * m0;
+            bev[0] = br[0];
+            bov[0] = br[1];
+            bev[1] = br[2];
+            bov[1] = br[3];
+        } else {
+            const double2 be = *reinterpret_cast<const double2 *>(b + rb + m0);
+            const double2 bo = *reinterpret_cast<const double2 *>(b + rb + hz + m0);
+            bev[0] = be.x;
+            bev[1] = be.y;
+            bov[0] = bo.x;
+            bov[1] = bo.y;
+        }
         double re[W], ro[W];
 #pragma unroll
         for (int p = 0; p < W; ++p) {

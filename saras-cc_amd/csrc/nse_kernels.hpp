@@ -94,6 +94,16 @@ __device__ __forceinline__ double z_wall_mask(const Geom &G, int type, bool hi, 
 
 __device__ __forceinline__ int pick(int ax, int i, int j, int k) { return ax == 0 ? i : (ax == 1 ? j : k); }
 
+// The core index a neighbour load of index q (at most 2 outside [0, n)) reads:
+// the periodic image on a periodic axis, else the edge cell a wall rule
+// starts from.  Always inside [0, n): on the one-row y axis of a planar grid
+// (n = 1, periodic) the image of q = +-2 is row 0 as well -- the second-order
+// edge stencil there never uses it, but the load is issued unconditionally.
+__device__ __forceinline__ int core_index(int per, int q, int n) {
+    const int w = per ? (q < 0 ? q + n : (q >= n ? q - n : q)) : q;
+    return w < 0 ? 0 : (w >= n ? n - 1 : w);
+}
+
 // Core-only field with on-the-fly ghost cells.
 struct CoreField {
     const double *f;
@@ -101,31 +111,35 @@ struct CoreField {
     __device__ __forceinline__ double at(const Geom &G, int i, int j, int k) const {
         return f[(long)i * G.sx + (long)j * G.sy + k];
     }
+    // The cell's own load is issued unconditionally from an index that is
+    // always inside the core (the wrapped one on a periodic axis, else the
+    // edge cell a wall rule reads), and the wall rule is applied to the loaded
+    // value afterwards: no load waits behind a lane-dependent branch, so a
+    // stencil's loads are issued together.
     __device__ __forceinline__ double nb(const Geom &G, int ax, int i, int j, int k, int off) const {
         int c[3] = {i, j, k};
-        int q = c[ax] + off;
+        const int q = c[ax] + off;
         const int n = G.n[ax];
-        if (q >= 0 && q < n) {
-            c[ax] = q;
-            return f[(long)c[0] * G.sx + (long)c[1] * G.sy + c[2]];
-        }
-        if (G.per[ax]) {
-            c[ax] = q < 0 ? q + n : q - n;
-            return f[(long)c[0] * G.sx + (long)c[1] * G.sy + c[2]];
-        }
-        // select between the two walls of this axis without a run-time index
-        // into the rule table (keeps the table in kernel-argument memory)
-        const bool hi = q >= n;
+        const bool lo = q < 0, hi = q >= n;
+        c[ax] = core_index(G.per[ax], q, n);
+        const double v = f[(long)c[0] * G.sx + (long)c[1] * G.sy + c[2]];
+        if (!(lo || hi) || G.per[ax]) return v;
+        return wall(G, ax, hi, q, c, v);
+    }
+    // the ghost across a wall of axis ax (c: the edge cell, inner: its value);
+    // select between the two walls without a run-time index into the rule
+    // table (keeps the table in kernel-argument memory)
+    __device__ __forceinline__ double wall(const Geom &G, int ax, bool hi, int q, const int (&c)[3],
+                                           double inner) const {
+        const int n = G.n[ax];
         if (ax == 2 && q == n + 1 && bc.top2) return bc.top2[(long)c[0] * G.n[1] + c[1]];
         const int type = hi ? bc.type[2 * ax + 1] : bc.type[2 * ax];
         const double val = hi ? bc.val[2 * ax + 1] : bc.val[2 * ax];
         const double kh = hi ? bc.kh[2 * ax + 1] : bc.kh[2 * ax];
-        c[ax] = hi ? n - 1 : 0;
         if (type == 4) {
             const double *gp = hi ? bc.ghost[2 * ax + 1] : bc.ghost[2 * ax];
             return gp[ghost_plane_index(G.n, ax, c[0], c[1], c[2])];
         }
-        const double inner = f[(long)c[0] * G.sx + (long)c[1] * G.sy + c[2]];
         if (type > 1) return z_wall_mask(G, type, hi, c[0], c[1], inner);
         return type == 0 ? 2.0 * val - inner : inner - kh;
     }
@@ -158,25 +172,19 @@ struct SplitField {
     __device__ __forceinline__ double at(const Geom &G, int i, int j, int k) const {
         return f[(long)i * G.sx + (long)j * G.sy + zo(k)];
     }
+    // loads first, wall rule after, as CoreField::nb
     __device__ __forceinline__ double nb(const Geom &G, int ax, int i, int j, int k, int off) const {
         int c[3] = {i, j, k};
-        int q = c[ax] + off;
+        const int q = c[ax] + off;
         const int n = G.n[ax];
-        if (q >= 0 && q < n) {
-            c[ax] = q;
-            return f[(long)c[0] * G.sx + (long)c[1] * G.sy + zo(c[2])];
-        }
-        if (G.per[ax]) {
-            c[ax] = q < 0 ? q + n : q - n;
-            return f[(long)c[0] * G.sx + (long)c[1] * G.sy + zo(c[2])];
-        }
-        const bool hi = q >= n;
+        const bool lo = q < 0, hi = q >= n;
+        c[ax] = core_index(G.per[ax], q, n);
+        const double v = f[(long)c[0] * G.sx + (long)c[1] * G.sy + zo(c[2])];
+        if (!(lo || hi) || G.per[ax]) return v;
         const int type = hi ? bc.type[2 * ax + 1] : bc.type[2 * ax];
         const double val = hi ? bc.val[2 * ax + 1] : bc.val[2 * ax];
         const double kh = hi ? bc.kh[2 * ax + 1] : bc.kh[2 * ax];
-        c[ax] = hi ? n - 1 : 0;
-        const double inner = f[(long)c[0] * G.sx + (long)c[1] * G.sy + zo(c[2])];
-        return type == 0 ? 2.0 * val - inner : inner - kh;
+        return type == 0 ? 2.0 * val - v : v - kh;
     }
 };
 
@@ -228,11 +236,12 @@ __device__ __forceinline__ double d1(const Geom &G, const F &A, int ax, int i, i
     double v;
     if (G.dscheme == 1) {
         v = (fp - fm) * 0.5;
-    } else if (q == 0 || q == G.n[ax] - 1) {
-        v = 0.5 * (fp - fm);
     } else {
+        // loaded before the edge test (which differs between lanes); an edge
+        // cell does not use them
         const double fm2 = A.nb(G, ax, i, j, k, -2), fp2 = A.nb(G, ax, i, j, k, 2);
-        v = (8.0 * (fp - fm) - (fp2 - fm2)) * kRecip12;
+        if (q == 0 || q == G.n[ax] - 1) v = 0.5 * (fp - fm);
+        else v = (8.0 * (fp - fm) - (fp2 - fm2)) * kRecip12;
     }
     v = v * G.ih[ax];
     return G.xi_x[ax][q] * v;
@@ -247,13 +256,15 @@ __device__ __forceinline__ double d2(const Geom &G, const F &A, int ax, int i, i
     if (G.dscheme == 1) {
         t = (fp - fm) * 0.5;
         o = (fm - 2.0 * f) + fp;
-    } else if (q == 0 || q == G.n[ax] - 1) {
-        t = 0.5 * (fp - fm);
-        o = fp - 2.0 * f + fm;
     } else {
         const double fm2 = A.nb(G, ax, i, j, k, -2), fp2 = A.nb(G, ax, i, j, k, 2);
-        t = (8.0 * (fp - fm) - (fp2 - fm2)) * kRecip12;
-        o = ((-30.0 * f + 16.0 * (fp + fm)) - (fp2 + fm2)) * kRecip12;
+        if (q == 0 || q == G.n[ax] - 1) {
+            t = 0.5 * (fp - fm);
+            o = fp - 2.0 * f + fm;
+        } else {
+            t = (8.0 * (fp - fm) - (fp2 - fm2)) * kRecip12;
+            o = ((-30.0 * f + 16.0 * (fp + fm)) - (fp2 + fm2)) * kRecip12;
+        }
     }
     t = t * G.ih[ax];
     o = o * G.ih2[ax];
@@ -285,13 +296,19 @@ __device__ __forceinline__ double mor_pair(const Geom &G, const FU &U, const FC 
     const double cp = Cf.nb(G, a, i, j, k, 1), cm = Cf.nb(G, a, i, j, k, -1);
     const double up = U.nb(G, a, i, j, k, 1), um = U.nb(G, a, i, j, k, -1);
     const double pp = up * cp, pm = um * cm;
+    // the reach-2 values are loaded before the lane-dependent edge test
+    double cp2 = 0.0, cm2 = 0.0, up2 = 0.0, um2 = 0.0;
+    if (G.dscheme != 1) {
+        cp2 = Cf.nb(G, a, i, j, k, 2);
+        cm2 = Cf.nb(G, a, i, j, k, -2);
+        up2 = U.nb(G, a, i, j, k, 2);
+        um2 = U.nb(G, a, i, j, k, -2);
+    }
     double adv, dv;
     if (mor_edge(G, a, q)) {
         adv = u * xm * (cp - cm) * G.i2d[a];
         dv = xm * (pp - pm) * G.i2d[a];
     } else {
-        const double cp2 = Cf.nb(G, a, i, j, k, 2), cm2 = Cf.nb(G, a, i, j, k, -2);
-        const double up2 = U.nb(G, a, i, j, k, 2), um2 = U.nb(G, a, i, j, k, -2);
         const double pp2 = up2 * cp2, pm2 = um2 * cm2;
         adv = u * xm * (-cp2 + 8.0 * cp - 8.0 * cm + cm2) * G.i2d[a] / 6.0;
         dv = xm * (-pp2 + 8.0 * pp - 8.0 * pm + pm2) * G.i2d[a] / 6.0;
@@ -308,19 +325,19 @@ __device__ __forceinline__ double upwind_term(const Geom &G, double u, const FC 
     const int q = pick(a, i, j, k);
     const double xm = G.xi_x[a][q], i2d = G.i2d[a];
     const double fm = Cf.nb(G, a, i, j, k, -1), fp = Cf.nb(G, a, i, j, k, 1);
+    // every load is issued before the lane-dependent edge and Peclet tests
+    const double dh = G.xc[a][q + 1] - G.xc[a][q];
+    const double f = Cf.at(G, i, j, k);
     if (G.dscheme == 1) {
-        const double dh = G.xc[a][q + 1] - G.xc[a][q];
         const double pe = fabs(u) * dh / diff;
         if (pe < G.pe_limit) return u * xm * (fp - fm) * i2d;
-        const double f = Cf.at(G, i, j, k);
         const double A = G.up1[0], B = G.up1[1], Cc = G.up1[2];
         if (u > 0) return u * xm * (-A * fm + B * f - Cc * fp) * i2d;
         return u * xm * (A * fp - B * f + Cc * fm) * i2d;
     }
+    const double fm2 = Cf.nb(G, a, i, j, k, -2), fp2 = Cf.nb(G, a, i, j, k, 2);
     if (q == 0 || q == G.n[a] - 1) return u * xm * (fp - fm) * i2d;
-    const double dh = G.xc[a][q + 1] - G.xc[a][q];
     const double pe = fabs(u) * dh / diff;
-    const double fm2 = Cf.nb(G, a, i, j, k, -2), fp2 = Cf.nb(G, a, i, j, k, 2), f = Cf.at(G, i, j, k);
     if (pe < G.pe_limit) return u * xm * (-fp2 + 8.0 * fp - 8.0 * fm + fm2) * i2d / 6.0;
     const double A = G.up2[0], B = G.up2[1], Cc = G.up2[2], D = G.up2[3];
     if (u > 0) return u * xm * (A * fm2 - B * fm + Cc * f - D * fp) * i2d;

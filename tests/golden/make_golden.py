@@ -2,7 +2,7 @@
 their decks' settings by the FP64 CPU oracle (oracle/, built by
 `make -C oracle`).  Run from the repository root:
 
-    python tests/golden/make_golden.py rbc256      # about 3 minutes on 8 cores
+    python tests/golden/make_golden.py rbc256      # about 100 minutes on 8 cores
     python tests/golden/make_golden.py tgv_les64   # about 15 minutes on 8 cores
 
 rbc256 -> tests/golden/rbc256_steps.json
@@ -13,10 +13,11 @@ rbc256 -> tests/golden/rbc256_steps.json
     dScheme 2, Morinishi non-linear term, dt = 1e-3, V(3,6) pressure solves
     of at most 20 cycles to 1e-6 with "Solve Coarsest" off (with it on the
     reference algorithm's pressure solve diverges on this grid).  Initial
-    state u = 0, P = 1, T = 1 - z + U(-1e-3, 1e-3) (seed 2).  Per step: the
-    tseries quantities (KE, max|div|, NuVol, NuWall; tseries.cc:209-490), the
-    Jacobi counts and the pressure solve's cycles and residual; after the last
-    step, per field: the SHA-256 of the core's bytes, L2 (rms), L-inf, sum.
+    state u = 0, P = 1, T = 1 - z + U(-1e-3, 1e-3) (seed 2), 100 steps as the
+    deck asks.  Per step: the tseries quantities (KE, max|div|, NuVol, NuWall;
+    tseries.cc:209-490), the Jacobi counts and the pressure solve's cycles and
+    residual; after step 3 and after the last step, per field: the SHA-256 of
+    the core's bytes, L2 (rms), L-inf, sum.
 
 tgv_les64 -> tests/golden/tgv_les64_history.json
     Decaying Taylor-Green vortex in a periodic (2 pi)^3 box at 64^3, Re =
@@ -52,7 +53,7 @@ def field_summary(a):
 
 
 # ---- config 2: 256^3 Rayleigh-Benard ----------------------------------------
-RBC = dict(n=256, pad=2, steps=3, ra=1e7, pr=1.0, dt=1e-3, seed=2)
+RBC = dict(n=256, pad=2, steps=100, early=3, ra=1e7, pr=1.0, dt=1e-3, seed=2)
 
 
 def rbc_mg():
@@ -84,6 +85,8 @@ def make_rbc256(threads):
     z = O.field_metrics(g, 2)["x"]
     s.set(**rbc_initial(z, n, pad, c["seed"]))
     hist = []
+    core = (slice(pad, -pad),) * 3
+    early = None
     for step in range(c["steps"]):
         t0 = time.time()
         r = s.step()
@@ -93,10 +96,13 @@ def make_rbc256(threads):
         hist.append(dict(step=step + 1, ke=dg["ke"], divmax=dg["divmax"], nuvol=dg["nuvol"], nuwall=dg["nuwall"],
                          jacobi_iters=list(s.jacobi_iters()), mg_cycles=mg["cycles"], mg_residual=mg["residual"]))
         print(f"rbc256 step {step + 1}: {time.time() - t0:.1f} s, {hist[-1]}", flush=True)
+        if step + 1 == c["early"]:
+            f = s.get()
+            early = {k: field_summary(f[k][core]) for k in ("vx", "vy", "vz", "p", "t")}
     f = s.get()
-    core = (slice(pad, -pad),) * 3
     fields = {k: field_summary(f[k][core]) for k in ("vx", "vy", "vz", "p", "t")}
-    return dict(config=dict(RBC, nu=nu, kappa=kappa, mg=rbc_mg(), threads=threads), history=hist, fields=fields)
+    return dict(config=dict(RBC, nu=nu, kappa=kappa, mg=rbc_mg(), threads=threads), history=hist,
+                fields_early=early, fields=fields)
 
 
 # ---- config 5: Taylor-Green LES at 64^3 -------------------------------------

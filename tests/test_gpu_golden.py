@@ -1,5 +1,6 @@
-"""The two time-stepping configurations at their decks' settings against the
-FP64 CPU oracle's fixtures (tests/golden/make_golden.py writes them).
+"""The configurations at their decks' sizes against the FP64 CPU oracle: the
+two time-stepping ones through fixtures (tests/golden/make_golden.py writes
+them), the 512^3 Poisson solve against the oracle run in the test.
 
 rbc256: 3-D Rayleigh-Benard convection at its full 256^3 size, the deck's
 100 lsRK3 steps of the FP64 path.  Bar: per step the Jacobi counts, the
@@ -34,6 +35,7 @@ LES_HIST_RTOL = 1e-8
 LES_EPS_RTOL = 1e-6
 RBC_MIXED_HIST_RTOL = 1e-4
 RBC_MIXED_FIELD_RTOL = 1e-4
+MIXED_512_RTOL = 1e-12
 
 
 def load(name):
@@ -204,3 +206,44 @@ def test_tgv_les64_history_matches_oracle_fixture():
     for k in ("ke", "sgs_ke", "sgs_dissipation", "nu_sgs", "divmax"):
         assert worst[k] <= LES_HIST_RTOL, (k, worst)
     assert worst["eps"] <= LES_EPS_RTOL, worst
+
+
+def test_config3_512_periodic_fourier_solves_match_oracle():
+    """Config 3 at its deck's size: the 512^3 periodic Poisson problem with
+    the 64-mode Fourier right-hand side (bench.fourier_rhs), V(2,2), solved to
+    a relative L-inf residual of 1e-10.  The FP64 path equals the oracle's
+    FP64 solve bit for bit (cycle count, residual, every value of the returned
+    array); the mixed-precision solve reaches the tolerance by the oracle's
+    own computeError and differs from the oracle's solution by at most
+    MIXED_512_RTOL of its largest value (modulo the mean: the periodic
+    problem's solution is defined up to a constant)."""
+    import oracle as O
+    from bench import fourier_rhs
+
+    n, tol = 512, 1e-10
+    kw = dict(vc_count=100, vc_tol=tol, mg_tol=1e-2, pre=2, post=2)
+    b = fourier_rhs(n)
+    mo = O.Multigrid(O.grid_desc((n, n, n), periodic=(1, 1, 1)), O.mg_params(bc=O.BC_NEUMANN, **kw))
+    xo, so = mo.solve(np.zeros_like(b), b, 1)
+    print(f"oracle 512^3: {so['cycles']} cycles, residual {so['residual']:.3e}", flush=True)
+    g = S.grid_desc((n, n, n), periodic=(1, 1, 1))
+    ms = S.Multigrid(g, S.mg_params(bc=S.BC_NEUMANN, precision=S.PREC_FP64, **kw))
+    xs, ss, status = ms.solve(np.zeros_like(b), b)
+    ms.close()
+    assert status == S.OK and ss["cycles"] == so["cycles"], (ss, so)
+    assert ss["residual"] == so["residual"], (ss, so)
+    assert np.array_equal(xs, xo), float(np.abs(xs - xo).max())
+    del xs
+    mm = S.Multigrid(g, S.mg_params(bc=S.BC_NEUMANN, precision=S.PREC_MIXED, inner_cycles=4, max_refine=40, **kw))
+    xm, sm, status = mm.solve(np.zeros_like(b), b)
+    mm.close()
+    assert status == S.OK and sm["converged"], sm
+    mo.set_level(0, xm, b)
+    mo.impose_bc(0, 0)
+    r = mo.error(0)
+    assert r < 1.5 * tol, (r, sm)
+    c_m, c_o = xm[1:-1, 1:-1, 1:-1], xo[1:-1, 1:-1, 1:-1]
+    d = np.abs((c_m - c_m.mean()) - (c_o - c_o.mean())).max() / np.abs(c_o - c_o.mean()).max()
+    print(json.dumps(dict(fp32_cycles=sm["cycles"], refinements=sm["refinements"], oracle_residual=r,
+                          mixed_vs_oracle_linf_rel=float(d))), flush=True)
+    assert d <= MIXED_512_RTOL, d

@@ -389,6 +389,39 @@ def test_fp32_split_rows_match_z_contiguous(case, variant, monkeypatch):
 
 
 
+@pytest.mark.parametrize("switch", [({}, {"SARAS_RR_LANE": "1"}), ({}, {"SARAS_NO_RED_FOLD": "1"}),
+                                    ({}, {"SARAS_TAIL_LDS_KB": "8"}),
+                                    ({"SARAS_NO_SPLIT": "1"}, {"SARAS_MARCH_WGS": "128"})],
+                         ids=["rr-lane", "no-red-fold", "tail-8kb", "z-contiguous-march-128"])
+@pytest.mark.parametrize("prec", [S.PREC_FP64, S.PREC_MIXED], ids=["fp64", "mixed"])
+def test_kernel_variant_switches_bitexact(switch, prec, monkeypatch):
+    """The handle-creation switches that select another kernel arrangement of
+    the same arithmetic (restriction with all eight children in one lane; the
+    coarse level's first red pass left to the sweep instead of the
+    restriction; a smaller tail LDS budget, so fewer levels run in the tail;
+    another march grid) give whole solves bit-identical to the default."""
+    g = S.grid_desc((64, 64, 64), periodic=(1, 1, 0), gtype=(0, 0, 2), beta=(1, 1, 0.8), pad=1)
+    p = S.mg_params(vc_count=6, vc_tol=1e-14, solve_flag=0, mg_tol=1e-2, pre=2, post=2, bc=S.BC_NEUMANN,
+                    precision=prec, inner_cycles=3, max_refine=2)
+    rng = np.random.default_rng(33)
+    b = np.zeros((66, 66, 66))
+    c = rng.standard_normal((64, 64, 64))
+    b[1:-1, 1:-1, 1:-1] = c - c.mean()
+    base_env, other_env = switch
+    for k, v in base_env.items():
+        monkeypatch.setenv(k, v)
+    base = S.Multigrid(g, p)
+    xs, ss, _ = base.solve(np.zeros_like(b), b)
+    base.close()
+    for k, v in other_env.items():
+        monkeypatch.setenv(k, v)
+    other = S.Multigrid(g, p)
+    xo, so, _ = other.solve(np.zeros_like(b), b)
+    other.close()
+    assert ss["cycles"] == so["cycles"] and ss["residual"] == so["residual"], (ss, so)
+    np.testing.assert_array_equal(xs, xo)
+
+
 @pytest.mark.parametrize("tail", [True, False], ids=["tail-kernel", "per-level"])
 @pytest.mark.parametrize("gt", [(0, 0, 0), (0, 0, 2)], ids=["uniform", "tanh-z"])
 def test_2d_top_dirichlet_solve_bitexact(gt, tail, monkeypatch):

@@ -263,10 +263,11 @@ struct saras_mg {
     std::unique_ptr<Hierarchy<float>> h32;
     double *x64b = nullptr;            // second FP64 iterate buffer (MIXED)
     double *partial = nullptr;         // norm partials
-    double *norm_out = nullptr;        // device [2]
-    double *norm_host = nullptr;       // pinned [2]
-    int *iters_dev = nullptr;          // coarse iteration count
-    int *iters_host = nullptr;         // pinned
+    // The norm's last reduction and the coarse solves write these pinned,
+    // host-coherent words directly: the host reads them after synchronising
+    // the stream, with no copy kernel in between.
+    double *norm_host = nullptr;       // [numerator, denominator] of computeError
+    int *iters_host = nullptr;         // coarse iteration count of the last coarse solve
     double *stage_x = nullptr, *stage_b = nullptr;  // device staging for host pointers
     size_t stage_elems = 0;
     hipStream_t stream = nullptr;
@@ -296,7 +297,7 @@ struct saras_mg {
     ~saras_mg() {
         for (auto &c : graph_cache)
             for (auto &e : c) (void)hipGraphExecDestroy(e.exec);
-        (void)hipFree(x64b); (void)hipFree(partial); (void)hipFree(norm_out); (void)hipFree(iters_dev);
+        (void)hipFree(x64b); (void)hipFree(partial);
         (void)hipHostFree(norm_host); (void)hipHostFree(iters_host);
         (void)hipFree(stage_x); (void)hipFree(stage_b);
         if (ev0) (void)hipEventDestroy(ev0);
@@ -539,7 +540,7 @@ struct Ops {
     void coarse(int l, bool zero_bc, int mode, int sweeps) {
         DevLevel<T> &L = H.lv[l];
         auto a = level_args(m, H, l, inhom(l, zero_bc));
-        saras::CoarseCtl ctl{mode, sweeps, m->max_count, m->p.mg_tol, m->iters_dev, coarse_rel_floor()};
+        saras::CoarseCtl ctl{mode, sweeps, m->max_count, m->p.mg_tol, m->iters_host, coarse_rel_floor()};
         const long cells = (long)L.n[0] * L.n[1] * L.n[2];
         int threads = 64;  // power of two: the kernel's tree reduction relies on it
         while (threads < cells && threads < 1024) threads <<= 1;
@@ -686,7 +687,7 @@ struct Ops {
         A.tol = m->p.mg_tol;
         A.rel_floor = coarse_rel_floor();
         A.zero_bc_first = S != 0;
-        A.iters_out = m->iters_dev;
+        A.iters_out = m->iters_host;
         size_t ncoef = 0;
         for (int l = S; l <= D; ++l)
             for (int q = 0; q < 3; ++q) {
@@ -806,11 +807,11 @@ void launch_norm(int type, saras_mg *m, const saras::LevelArgs<double> &a, const
             hipLaunchKernelGGL((saras::k_refine_split<2, DIM, true>), dim3(nb), dim3(nt), 0, s, a, x, nullptr, 0L, 0L,
                                0L, b, nullptr, nullptr, m->partial, saras::FullRows{});
         if (type == 0)
-            hipLaunchKernelGGL((saras::k_norm_final<0>), dim3(1), dim3(nt), 0, s, m->partial, saras::kNormBlocks, m->norm_out);
+            hipLaunchKernelGGL((saras::k_norm_final<0>), dim3(1), dim3(nt), 0, s, m->partial, saras::kNormBlocks, m->norm_host);
         else if (type == 1)
-            hipLaunchKernelGGL((saras::k_norm_final<1>), dim3(1), dim3(nt), 0, s, m->partial, saras::kNormBlocks, m->norm_out);
+            hipLaunchKernelGGL((saras::k_norm_final<1>), dim3(1), dim3(nt), 0, s, m->partial, saras::kNormBlocks, m->norm_host);
         else
-            hipLaunchKernelGGL((saras::k_norm_final<2>), dim3(1), dim3(nt), 0, s, m->partial, saras::kNormBlocks, m->norm_out);
+            hipLaunchKernelGGL((saras::k_norm_final<2>), dim3(1), dim3(nt), 0, s, m->partial, saras::kNormBlocks, m->norm_host);
         return;
     }
     switch (type) {
@@ -818,19 +819,19 @@ void launch_norm(int type, saras_mg *m, const saras::LevelArgs<double> &a, const
             hipLaunchKernelGGL((saras::k_norm_partial<0, DIM>), dim3(saras::kNormBlocks), dim3(saras::kNormThreads), 0,
                                s, a, x, b, m->partial);
             hipLaunchKernelGGL((saras::k_norm_final<0>), dim3(1), dim3(saras::kNormThreads), 0, s, m->partial,
-                               saras::kNormBlocks, m->norm_out);
+                               saras::kNormBlocks, m->norm_host);
             break;
         case 1:
             hipLaunchKernelGGL((saras::k_norm_partial<1, DIM>), dim3(saras::kNormBlocks), dim3(saras::kNormThreads), 0,
                                s, a, x, b, m->partial);
             hipLaunchKernelGGL((saras::k_norm_final<1>), dim3(1), dim3(saras::kNormThreads), 0, s, m->partial,
-                               saras::kNormBlocks, m->norm_out);
+                               saras::kNormBlocks, m->norm_host);
             break;
         default:
             hipLaunchKernelGGL((saras::k_norm_partial<2, DIM>), dim3(saras::kNormBlocks), dim3(saras::kNormThreads), 0,
                                s, a, x, b, m->partial);
             hipLaunchKernelGGL((saras::k_norm_final<2>), dim3(1), dim3(saras::kNormThreads), 0, s, m->partial,
-                               saras::kNormBlocks, m->norm_out);
+                               saras::kNormBlocks, m->norm_host);
             break;
     }
 }
@@ -841,7 +842,6 @@ saras_status compute_error(saras_mg *m, int type, const double *x, hipStream_t s
     auto a = level_args(m, m->h64, 0, true);
     if (m->G.dim == 3) launch_norm<3>(type, m, a, x, m->h64.lv[0].b, s);
     else launch_norm<2>(type, m, a, x, m->h64.lv[0].b, s);
-    HIP_TRY(hipMemcpyAsync(m->norm_host, m->norm_out, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     const double num = m->norm_host[0], den = m->norm_host[1];
     const auto &L = m->h64.lv[0];
@@ -854,12 +854,17 @@ saras_status compute_error(saras_mg *m, int type, const double *x, hipStream_t s
     return SARAS_OK;
 }
 
-saras_status fetch_coarse_iters(saras_mg *m, hipStream_t s) {
-    HIP_TRY(hipMemcpyAsync(m->iters_host, m->iters_dev, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
+// the last coarse solve's iteration count, once the stream has been
+// synchronised after it
+void note_coarse_iters(saras_mg *m) {
     const int it = *m->iters_host;
     m->coarse_iters_max = std::max(m->coarse_iters_max, it);
     if (it > m->max_count) m->coarse_capped = 1;
+}
+
+saras_status fetch_coarse_iters(saras_mg *m, hipStream_t s) {
+    HIP_TRY(hipStreamSynchronize(s));
+    note_coarse_iters(m);
     return SARAS_OK;
 }
 
@@ -1059,10 +1064,9 @@ saras_status saras_mg_create(const saras_grid_desc *grid, const saras_mg_params 
     st = m->h64.pack_tail(tail_first_level(m.get(), m->h64));
     if (st) return st;
     HIP_TRY(hipMalloc((void **)&m->partial, 2 * saras::kNormBlocks * sizeof(double)));
-    HIP_TRY(hipMalloc((void **)&m->norm_out, 2 * sizeof(double)));
-    HIP_TRY(hipMalloc((void **)&m->iters_dev, sizeof(int)));
-    HIP_TRY(hipHostMalloc((void **)&m->norm_host, 2 * sizeof(double), hipHostMallocDefault));
-    HIP_TRY(hipHostMalloc((void **)&m->iters_host, sizeof(int), hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void **)&m->norm_host, 2 * sizeof(double), hipHostMallocCoherent));
+    HIP_TRY(hipHostMalloc((void **)&m->iters_host, sizeof(int), hipHostMallocCoherent));
+    *m->iters_host = 0;
     HIP_TRY(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
     HIP_TRY(hipEventCreate(&m->ev0));
     HIP_TRY(hipEventCreate(&m->ev1));
@@ -1098,12 +1102,10 @@ static saras_status solve_fp64(saras_mg *m, hipStream_t s, saras_mg_stats *st) {
     for (int c = 0; c < m->p.vc_count; ++c) {
         saras_status ve = run_vcycle(m, ops);
         if (ve) return ve;
-        if (m->p.solve_flag) {
-            saras_status e = fetch_coarse_iters(m, s);
-            if (e) return e;
-        }
         saras_status e = compute_error(m, m->p.res_type, L0.x, s, &res);
         if (e) return e;
+        // compute_error synchronised the stream after this cycle's coarse solve
+        if (m->p.solve_flag) note_coarse_iters(m);
         cycles = c + 1;
         if (res < m->p.vc_tol) {
             conv = true;
@@ -1127,7 +1129,7 @@ void launch_refine_split(saras_mg *m, const saras::LevelArgs<double> &a, const f
     hipLaunchKernelGGL((saras::k_refine_split<TYPE, DIM, false, CX, CB>), dim3(nb), dim3(nt), 0, s, a, L0.x, e, esx,
                        esy, ehz, L0.b, m->x64b, r32, m->partial, fr);
     hipLaunchKernelGGL((saras::k_norm_final<TYPE>), dim3(1), dim3(nt), 0, s, m->partial, saras::kNormBlocks,
-                       m->norm_out);
+                       m->norm_host);
 }
 
 template <int TYPE, int DIM>
@@ -1155,19 +1157,19 @@ void launch_refine(int type, saras_mg *m, const saras::LevelArgs<double> &a, con
             hipLaunchKernelGGL((saras::k_refine<0, DIM>), dim3(saras::kNormBlocks), dim3(saras::kNormThreads), 0, s, a,
                                L0.x, e, esx, esy, L0.b, m->x64b, r32, esx, esy, ehz, m->partial);
             hipLaunchKernelGGL((saras::k_norm_final<0>), dim3(1), dim3(saras::kNormThreads), 0, s, m->partial,
-                               saras::kNormBlocks, m->norm_out);
+                               saras::kNormBlocks, m->norm_host);
             break;
         case 1:
             hipLaunchKernelGGL((saras::k_refine<1, DIM>), dim3(saras::kNormBlocks), dim3(saras::kNormThreads), 0, s, a,
                                L0.x, e, esx, esy, L0.b, m->x64b, r32, esx, esy, ehz, m->partial);
             hipLaunchKernelGGL((saras::k_norm_final<1>), dim3(1), dim3(saras::kNormThreads), 0, s, m->partial,
-                               saras::kNormBlocks, m->norm_out);
+                               saras::kNormBlocks, m->norm_host);
             break;
         default:
             hipLaunchKernelGGL((saras::k_refine<2, DIM>), dim3(saras::kNormBlocks), dim3(saras::kNormThreads), 0, s, a,
                                L0.x, e, esx, esy, L0.b, m->x64b, r32, esx, esy, ehz, m->partial);
             hipLaunchKernelGGL((saras::k_norm_final<2>), dim3(1), dim3(saras::kNormThreads), 0, s, m->partial,
-                               saras::kNormBlocks, m->norm_out);
+                               saras::kNormBlocks, m->norm_host);
             break;
     }
 }
@@ -1191,7 +1193,6 @@ static saras_status refine_step(saras_mg *m, hipStream_t s, bool with_e, const s
     if (m->G.dim == 3) launch_refine<3>(m->p.res_type, m, a, e, F0.sx, F0.sy, F0.hz, F0.b, fr, s);
     else launch_refine<2>(m->p.res_type, m, a, e, F0.sx, F0.sy, F0.hz, F0.b, fr, s);
     std::swap(L0.x, m->x64b);
-    HIP_TRY(hipMemcpyAsync(m->norm_host, m->norm_out, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     const double num = m->norm_host[0], den = m->norm_host[1];
     const int pointCount = L0.n[0] * L0.n[1] * L0.n[2];
@@ -1480,7 +1481,6 @@ saras_status saras_mg_coarse_solve(saras_mg *mg, int level, int zero_bc, int *it
     Ops<double> ops{mg, mg->h64, mg->stream, true, mg->p.sor};
     ops.coarse(level, zero_bc != 0, 0, 0);
     if (ops.err) return ops.err;
-    HIP_TRY(hipMemcpyAsync(mg->iters_host, mg->iters_dev, sizeof(int), hipMemcpyDeviceToHost, mg->stream));
     HIP_TRY(hipStreamSynchronize(mg->stream));
     if (iters) *iters = *mg->iters_host;
     return SARAS_OK;

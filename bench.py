@@ -443,9 +443,9 @@ def les_history(steps, n=384):
 
 # FP64 operations per cell of the sub-grid stress kernel at 384^3, from
 # rocprofv3 --pmc SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 (wave instructions x
-# 64 lanes, an FMA counted twice): 317.46 GFLOP per launch.
+# 64 lanes, an FMA counted twice): 199.2 GFLOP per launch.
 # Source: profiles/les_step_384cubed.md
-SGS_FLOP_PER_CELL = 5606.5
+SGS_FLOP_PER_CELL = 3518.1
 FP64_VECTOR_PEAK_TFLOPS = 78.6
 
 

@@ -118,9 +118,12 @@ saras_status saras_mg_create(const saras_grid_desc *grid, const saras_mg_params 
  * initial guess and on return its core and first ghost layer hold the
  * solution (outLHS.F(stagFull(0)) = lhs(0)(stagFull(0))).  b_full is the RHS
  * (core read).  ptr_kind: SARAS_PTR_HOST or SARAS_PTR_DEVICE.  stream: a
- * hipStream_t or NULL for the handle's own stream.  Returns SARAS_OK when
- * the residual fell below vc_tol, SARAS_NOT_CONVERGED when vc_count cycles
- * ran out (the reference is silent there, poisson.cc:127-185). */
+ * hipStream_t or NULL for the handle's own stream.  The call returns after
+ * the stream has finished the solve; with device pointers the arrays must
+ * stay unchanged by other streams while it runs (the mixed-precision solve
+ * reads b_full and the initial guess in place throughout).  Returns SARAS_OK
+ * when the residual fell below vc_tol, SARAS_NOT_CONVERGED when vc_count
+ * cycles ran out (the reference is silent there, poisson.cc:127-185). */
 saras_status saras_mg_solve(saras_mg *mg, double *x_full, const double *b_full, int ptr_kind,
                             void *stream, saras_mg_stats *stats);
 

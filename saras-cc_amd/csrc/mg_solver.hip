@@ -382,6 +382,10 @@ struct Ops {
     T w;
 
     int pend_level = -1;  // level whose next sweep also applies the prolongation
+    // level 0's iterate is zero and has not been written: set by the caller
+    // instead of clearing it, for a V-cycle whose first operation is a
+    // split-row sweep of level 0 (see zero_start_ok)
+    bool zero0 = false;
     int skip_red = -1;    // level whose next sweep's red pass the restriction already applied
     // a host-side guard that refused a launch: the caller returns this status
     // (with the message in saras_last_error) instead of the process aborting
@@ -445,8 +449,21 @@ struct Ops {
 
     template <int DIM, bool ZP, bool FINE, bool PRO>
     void split_launch(const dim3 &g, const saras::LevelArgs<T> &a, DevLevel<T> &L, int colour, int nb, const T *e,
-                      long esx, long esy, long ehz) {
+                      long esx, long esy, long ehz, bool z = false) {
         const dim3 bl(saras::kSplitThreads);
+        // z: the finest level's first red pass on an iterate known to be zero
+        if constexpr (FINE && !PRO) {
+            if (z) {
+                if (w == T(1)) {
+                    if (ZP) hipLaunchKernelGGL((saras::k_rb_split_zp<T, DIM, FINE, PRO, true, true>), g, bl, 0, s, a, L.x, L.b, colour, w, nb, e, esx, esy, ehz);
+                    else hipLaunchKernelGGL((saras::k_rb_split<T, DIM, FINE, PRO, true, true>), g, bl, 0, s, a, L.x, L.b, colour, w, nb, e, esx, esy, ehz);
+                } else {
+                    if (ZP) hipLaunchKernelGGL((saras::k_rb_split_zp<T, DIM, FINE, PRO, false, true>), g, bl, 0, s, a, L.x, L.b, colour, w, nb, e, esx, esy, ehz);
+                    else hipLaunchKernelGGL((saras::k_rb_split<T, DIM, FINE, PRO, false, true>), g, bl, 0, s, a, L.x, L.b, colour, w, nb, e, esx, esy, ehz);
+                }
+                return;
+            }
+        }
         // w == 1 (every reference deck): the pass that skips the cells' own values
         if (w == T(1)) {
             if (ZP) hipLaunchKernelGGL((saras::k_rb_split_zp<T, DIM, FINE, PRO, true>), g, bl, 0, s, a, L.x, L.b, colour, w, nb, e, esx, esy, ehz);
@@ -459,7 +476,7 @@ struct Ops {
 
     template <int DIM>
     void split_pass(bool fine, bool pro, int grid, const saras::LevelArgs<T> &a, DevLevel<T> &L, int colour, int nb,
-                    const T *e, long esx, long esy, long ehz) {
+                    const T *e, long esx, long esy, long ehz, bool z = false) {
         const dim3 g(grid);
         // rows that wrap in z, or rows split between waves, need the
         // z-prefetching variant (see rb_split_body)
@@ -468,7 +485,7 @@ struct Ops {
         if (zp) {
             if (fine) {
                 if (pro) split_launch<DIM, true, true, true>(g, a, L, colour, nb, e, esx, esy, ehz);
-                else split_launch<DIM, true, true, false>(g, a, L, colour, nb, e, esx, esy, ehz);
+                else split_launch<DIM, true, true, false>(g, a, L, colour, nb, e, esx, esy, ehz, z);
             } else {
                 if (pro) split_launch<DIM, true, false, true>(g, a, L, colour, nb, e, esx, esy, ehz);
                 else split_launch<DIM, true, false, false>(g, a, L, colour, nb, e, esx, esy, ehz);
@@ -477,7 +494,7 @@ struct Ops {
         }
         if (fine) {
             if (pro) split_launch<DIM, false, true, true>(g, a, L, colour, nb, e, esx, esy, ehz);
-            else split_launch<DIM, false, true, false>(g, a, L, colour, nb, e, esx, esy, ehz);
+            else split_launch<DIM, false, true, false>(g, a, L, colour, nb, e, esx, esy, ehz, z);
         } else {
             if (pro) split_launch<DIM, false, false, true>(g, a, L, colour, nb, e, esx, esy, ehz);
             else split_launch<DIM, false, false, false>(g, a, L, colour, nb, e, esx, esy, ehz);
@@ -506,9 +523,14 @@ struct Ops {
             }
             const int first = skip_red == l ? 1 : 0;
             if (first) skip_red = -1;
+            // the finest level's iterate is known to be zero: its first red
+            // pass reads no x and clears the black half itself
+            const bool z = l == 0 && zero0 && first == 0 && !pro;
+            if (l == 0) zero0 = false;
             for (int colour = first; colour < 2; ++colour) {
-                if (H.dim == 3) split_pass<3>(l == 0, pro, grid, a, L, colour, nb, e, esx, esy, ehz);
-                else split_pass<2>(l == 0, pro, grid, a, L, colour, nb, e, esx, esy, ehz);
+                const bool zc = z && colour == 0;
+                if (H.dim == 3) split_pass<3>(l == 0, pro, grid, a, L, colour, nb, e, esx, esy, ehz, zc);
+                else split_pass<2>(l == 0, pro, grid, a, L, colour, nb, e, esx, esy, ehz, zc);
             }
             return;
         }
@@ -1229,6 +1251,10 @@ static saras_status solve_mixed(saras_mg *m, hipStream_t s, saras_mg_stats *st,
     // never accuracy: the FP64 residual alone decides convergence.
     constexpr int kProbeCycles = 3;
     constexpr double kFp32Floor = 1e-6;
+    // level 0 is not the coarsest (whose smoothing is the coarse solve) and
+    // the V-cycle opens with its pre-smoothing sweep, not the tail kernel
+    const bool zero_start = !m->graphs && F0.hz && m->p.pre_smooth > 0 && m->depth > 0 &&
+                            (m->tail_off || ops.tail_start() > 0);
     double rho = 1.0;
     bool measured = false;
     while (!(res < m->p.vc_tol) && refs < m->p.max_refine && cycles < cap) {
@@ -1241,7 +1267,11 @@ static saras_status solve_mixed(saras_mg *m, hipStream_t s, saras_mg_stats *st,
             if (need < 1.0) k = 1;
             else if (need < (double)k) k = (int)need;
         }
-        HIP_TRY(hipMemsetAsync(F0.x, 0, F0.elems * sizeof(float), s));
+        // the correction starts from zero: cleared by the first red pass of
+        // the round's first cycle where that pass runs on split rows (eager
+        // launches; a replayed graph keeps the separate clear)
+        if (zero_start) ops.zero0 = true;
+        else HIP_TRY(hipMemsetAsync(F0.x, 0, F0.elems * sizeof(float), s));
         int ran = 0;
         for (int c = 0; c < k && cycles < cap; ++c) {
             e = run_vcycle(m, ops);

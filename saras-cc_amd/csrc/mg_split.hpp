@@ -130,10 +130,16 @@ __device__ __forceinline__ T coarse_at(const T *__restrict__ e, long base, long 
 // signed zero), so the cells' own values are read only where q is a zero
 // (to give the sign the reference's sum has) or where a wall rule needs them:
 // a pass reads 1.5 words per updated cell pair instead of 2.
-template <typename T, int DIM, bool PRO, bool ZPRE, bool W1>
+// Z: the level's iterate is known to be zero (the FP32 correction at the
+// start of a refinement); the red pass reads no x at all, uses +0 wherever
+// it would have read a cell, and writes +0 to the black half of its vectors,
+// which the iterate would otherwise have been cleared to beforehand: the
+// same values as clearing x and running the ordinary pass.
+template <typename T, int DIM, bool PRO, bool ZPRE, bool W1, bool Z = false>
 __device__ __forceinline__ void rb_split_body(const LevelArgs<T> &L, T *__restrict__ x, const T *__restrict__ b,
                                               int colour, T w, int nblocks, const T *__restrict__ e, long esx,
                                               long esy, long ehz) {
+    static_assert(!(Z && PRO), "a zero iterate has no prolongation to apply");
     using VT = Vec16<T>;
     using V = typename VT::V;
     constexpr int W = VT::W;
@@ -170,9 +176,9 @@ __device__ __forceinline__ void rb_split_body(const LevelArgs<T> &L, T *__restri
         need_c = need_c || (h == 0 && m0 == 0 && L.bc_lo[2] != BC_PER) ||
                  (h == 1 && m0 + W == half && L.bc_hi[2] != BC_PER);
     }
-    V cv{};
-    if (need_c) cv = *reinterpret_cast<const V *>(x + own);
-    const V ov = *reinterpret_cast<const V *>(x + oth);
+    V cv{}, ov{};
+    if (!Z && need_c) cv = *reinterpret_cast<const V *>(x + own);
+    if (!Z) ov = *reinterpret_cast<const V *>(x + oth);
     const V bv = *reinterpret_cast<const V *>(b + own);
     const V c2v = *reinterpret_cast<const V *>(L.c2zs + h * half + m0);
     const V c1v = *reinterpret_cast<const V *>(L.c1zs + h * half + m0);
@@ -207,7 +213,7 @@ __device__ __forceinline__ void rb_split_body(const LevelArgs<T> &L, T *__restri
         }
     }
     }
-    const T zx = (ZPRE && zoff >= 0) ? x[zoff] : T(0);
+    const T zx = (!Z && ZPRE && zoff >= 0) ? x[zoff] : T(0);
     T c[W], o[W];
 #pragma unroll
     for (int q = 0; q < W; ++q) {
@@ -231,7 +237,8 @@ __device__ __forceinline__ void rb_split_body(const LevelArgs<T> &L, T *__restri
     // nbr_i, nbr_j: the neighbour row's indices (wrapped), for its coarse row
     auto side = [&](T(&out)[W], bool inside, long off, int bc, int axis, int nbr_i, int nbr_j) {
         if (inside || bc == BC_PER) {
-            const V nv = *reinterpret_cast<const V *>(x + own + off);
+            V nv{};
+            if (!Z) nv = *reinterpret_cast<const V *>(x + own + off);
 #pragma unroll
             for (int q = 0; q < W; ++q) out[q] = VT::get(nv, q);
             if (PRO && colour == 0) {
@@ -268,7 +275,7 @@ __device__ __forceinline__ void rb_split_body(const LevelArgs<T> &L, T *__restri
     // the z value beyond the vector, with the correction when the red pass of
     // a prolongation sweep reads it (coarse cell K): prefetched (ZPRE) or read here
     auto zread = [&](long off, int K) -> T {
-        const T v = x[off];
+        const T v = Z ? T(0) : x[off];
         return (PRO && colour == 0) ? coarse_at(e, ebase, ehz, K) + v : v;
     };
     auto zfix = [&](long off, int K) -> T {
@@ -331,7 +338,8 @@ __device__ __forceinline__ void rb_split_body(const LevelArgs<T> &L, T *__restri
     }
     if (W1 && zero && !need_c) {
         // a zero quotient takes the sign of (1 - w) x + q: read the cells now
-        const V lv = *reinterpret_cast<const V *>(x + own);
+        V lv{};
+        if (!Z) lv = *reinterpret_cast<const V *>(x + own);
 #pragma unroll
         for (int q = 0; q < W; ++q) c[q] = PRO ? evo[q] + VT::get(lv, q) : VT::get(lv, q);
     }
@@ -339,20 +347,21 @@ __device__ __forceinline__ void rb_split_body(const LevelArgs<T> &L, T *__restri
     for (int q = 0; q < W; ++q)
         if (!W1 || r[q] == T(0)) r[q] = (T(1.0) - w) * c[q] + r[q];
     *reinterpret_cast<V *>(x + own) = VT::make(r);
+    if (Z) *reinterpret_cast<V *>(x + oth) = V{};
 }
 
-template <typename T, int DIM, bool FINE, bool PRO, bool W1>
+template <typename T, int DIM, bool FINE, bool PRO, bool W1, bool Z = false>
 __global__ void __launch_bounds__(kSplitThreads) k_rb_split(LevelArgs<T> L, T *__restrict__ x,
                                                             const T *__restrict__ b, int colour, T w, int nblocks,
                                                             const T *__restrict__ e, long esx, long esy, long ehz) {
-    rb_split_body<T, DIM, PRO, false, W1>(L, x, b, colour, w, nblocks, e, esx, esy, ehz);
+    rb_split_body<T, DIM, PRO, false, W1, Z>(L, x, b, colour, w, nblocks, e, esx, esy, ehz);
 }
 
-template <typename T, int DIM, bool FINE, bool PRO, bool W1>
+template <typename T, int DIM, bool FINE, bool PRO, bool W1, bool Z = false>
 __global__ void __launch_bounds__(kSplitThreads) k_rb_split_zp(LevelArgs<T> L, T *__restrict__ x,
                                                                const T *__restrict__ b, int colour, T w, int nblocks,
                                                                const T *__restrict__ e, long esx, long esy, long ehz) {
-    rb_split_body<T, DIM, PRO, true, W1>(L, x, b, colour, w, nblocks, e, esx, esy, ehz);
+    rb_split_body<T, DIM, PRO, true, W1, Z>(L, x, b, colour, w, nblocks, e, esx, esy, ehz);
 }
 
 // ---------------------------------------------------------------------------

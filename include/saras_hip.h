@@ -146,7 +146,9 @@ saras_status saras_level_metrics(const saras_grid_desc *grid, int level, int axi
 
 /* ---- single operations on one level (the private members of poisson the
  * V-cycle is built from), for kernel-level parity testing.  Level arrays are
- * padded (n_l+2)^3 FP64 arrays in host memory, lower bound -1. ---- */
+ * padded (n_l+2)^3 FP64 arrays in host memory, lower bound -1.  These act on
+ * the FP64 hierarchy; a mixed-precision saras_mg_solve works from the
+ * caller's arrays and leaves its level 0 undefined. ---- */
 saras_status saras_mg_set_level(saras_mg *mg, int level, const double *lhs_pad, const double *rhs_pad);
 saras_status saras_mg_get_level(saras_mg *mg, int level, double *lhs_pad, double *rhs_pad);
 /* multigrid_d3::smooth (poisson_d3.cc:96-246) */

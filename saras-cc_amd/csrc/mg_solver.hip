@@ -903,12 +903,14 @@ void load_core(saras_mg *m, const double *full, const DevLevel<T> &L, T *dst, hi
                        L.n[0], L.n[1], L.n[2], L.sx, L.sy, L.hz, dst);
 }
 
-// core + the [-1, n] ghost shell, phased as the reference's final imposeBC
+// core + the [-1, n] ghost shell, phased as the reference's final imposeBC;
+// core == nullptr: the core is in `full` already, only the shell is written
 void store_full(saras_mg *m, double *full, const double *core, hipStream_t s) {
     const auto &L = m->h64.lv[0];
     const long n = (long)L.n[0] * L.n[1] * L.n[2];
-    hipLaunchKernelGGL(saras::k_store_core, dim3(blocks_for(n)), dim3(kThreads), 0, s, full, m->g.pad, m->g.dim,
-                       L.n[0], L.n[1], L.n[2], L.sx, L.sy, L.hz, core);
+    if (core)
+        hipLaunchKernelGGL(saras::k_store_core, dim3(blocks_for(n)), dim3(kThreads), 0, s, full, m->g.pad, m->g.dim,
+                           L.n[0], L.n[1], L.n[2], L.sx, L.sy, L.hz, core);
     const int ext[3] = {L.n[0] + 2, m->g.dim == 3 ? L.n[1] + 2 : 1, L.n[2] + 2};
     for (int op = 0; op < 2; ++op)
         for (int a = 0; a < 3; ++a) {
@@ -1143,13 +1145,13 @@ static saras_status solve_fp64(saras_mg *m, hipStream_t s, saras_mg_stats *st) {
 
 }  // extern "C"
 
-template <int TYPE, int DIM, bool CX, bool CB>
+template <int TYPE, int DIM, bool CX, bool CB, bool CXO = false>
 void launch_refine_split(saras_mg *m, const saras::LevelArgs<double> &a, const float *e, long esx, long esy, long ehz,
                          float *r32, const saras::FullRows &fr, hipStream_t s) {
     auto &L0 = m->h64.lv[0];
     const unsigned nb = saras::kNormBlocks, nt = saras::kNormThreads;
-    hipLaunchKernelGGL((saras::k_refine_split<TYPE, DIM, false, CX, CB>), dim3(nb), dim3(nt), 0, s, a, L0.x, e, esx,
-                       esy, ehz, L0.b, m->x64b, r32, m->partial, fr);
+    hipLaunchKernelGGL((saras::k_refine_split<TYPE, DIM, false, CX, CB, CXO>), dim3(nb), dim3(nt), 0, s, a, L0.x, e,
+                       esx, esy, ehz, L0.b, m->x64b, r32, m->partial, fr);
     hipLaunchKernelGGL((saras::k_norm_final<TYPE>), dim3(1), dim3(nt), 0, s, m->partial, saras::kNormBlocks,
                        m->norm_host);
 }
@@ -1158,12 +1160,14 @@ template <int TYPE, int DIM>
 void launch_refine_split(saras_mg *m, const saras::LevelArgs<double> &a, const float *e, long esx, long esy, long ehz,
                          float *r32, const saras::FullRows &fr, hipStream_t s) {
     if (fr.x) launch_refine_split<TYPE, DIM, true, true>(m, a, e, esx, esy, ehz, r32, fr, s);
+    else if (fr.xo) launch_refine_split<TYPE, DIM, false, true, true>(m, a, e, esx, esy, ehz, r32, fr, s);
     else if (fr.b) launch_refine_split<TYPE, DIM, false, true>(m, a, e, esx, esy, ehz, r32, fr, s);
     else launch_refine_split<TYPE, DIM, false, false>(m, a, e, esx, esy, ehz, r32, fr, s);
 }
 
-// fr: the caller's arrays when x_old (only with no correction e) and/or b are
-// read from them instead of level 0; empty otherwise
+// fr: the caller's arrays when x_old and/or b are read from them instead of
+// level 0, or x_new written there (fr.xo, with b from the caller and x_old
+// from level 0); empty otherwise
 template <int DIM>
 void launch_refine(int type, saras_mg *m, const saras::LevelArgs<double> &a, const float *e, long esx, long esy,
                    long ehz, float *r32, const saras::FullRows &fr, hipStream_t s) {
@@ -1202,19 +1206,21 @@ static saras_status refine_step(saras_mg *m, hipStream_t s, bool with_e, const s
     auto &F0 = m->h32->lv[0];
     auto a = level_args(m, m->h64, 0, true);
     const float *e = with_e ? F0.x : nullptr;
-    // host-side guards: the caller-layout reads exist on split rows only, and
-    // x_old is read from the caller's array only for the initial residual
-    if ((fr.x || fr.b) && !(L0.hz && F0.hz)) {
+    // host-side guards: the caller-layout reads and writes exist on split rows
+    // only, always with b from the caller, and never read and write the
+    // caller's iterate in one pass
+    if ((fr.x || fr.b || fr.xo) && !(L0.hz && F0.hz && fr.b)) {
         set_error("internal error: caller-layout refinement on a level without split rows");
         return SARAS_HIP_ERROR;
     }
-    if (fr.x && with_e) {
-        set_error("internal error: caller-layout iterate with a correction");
+    if (fr.x && fr.xo) {
+        set_error("internal error: refinement reading and writing the caller's iterate");
         return SARAS_HIP_ERROR;
     }
     if (m->G.dim == 3) launch_refine<3>(m->p.res_type, m, a, e, F0.sx, F0.sy, F0.hz, F0.b, fr, s);
     else launch_refine<2>(m->p.res_type, m, a, e, F0.sx, F0.sy, F0.hz, F0.b, fr, s);
-    std::swap(L0.x, m->x64b);
+    // x_new is in level 0's other buffer, or (fr.xo) in the caller's array
+    if (!fr.xo) std::swap(L0.x, m->x64b);
     HIP_TRY(hipStreamSynchronize(s));
     const double num = m->norm_host[0], den = m->norm_host[1];
     const int pointCount = L0.n[0] * L0.n[1] * L0.n[2];
@@ -1230,14 +1236,19 @@ static saras_status refine_step(saras_mg *m, hipStream_t s, bool with_e, const s
 // walls) on the correction equation A e = r.
 // full: the caller's x (initial guess) and b, read in place by the FP64
 // passes when both levels 0 have split rows (empty: level 0 holds them).
+// With full.xo, the refinement expected to end the solve writes its iterate
+// straight into the caller's array; *x_in_caller says where the solution is.
 static saras_status solve_mixed(saras_mg *m, hipStream_t s, saras_mg_stats *st,
-                                const saras::FullRows &full = saras::FullRows{}) {
+                                const saras::FullRows &full = saras::FullRows{}, bool *x_in_caller = nullptr) {
     Ops<float> ops{m, *m->h32, s, false, (float)m->p.sor};
     auto &F0 = m->h32->lv[0];
     double res = 0.0;
-    saras::FullRows later = full;
+    saras::FullRows first = full, later = full;
+    first.xo = nullptr;
     later.x = nullptr;
-    saras_status e = refine_step(m, s, false, full, &res);
+    later.xo = nullptr;
+    bool in_caller = false;
+    saras_status e = refine_step(m, s, false, first, &res);
     if (e) return e;
     int cycles = 0, refs = 0;
     const int cap = m->p.vc_count > 0 ? m->p.vc_count : (1 << 30);
@@ -1280,8 +1291,22 @@ static saras_status solve_mixed(saras_mg *m, hipStream_t s, saras_mg_stats *st,
             ++ran;
         }
         const double before = res;
-        e = refine_step(m, s, true, later, &res);
+        // where this refinement reads and writes the FP64 iterate: after a
+        // refinement that wrote the caller's array, read it from there; else
+        // write there when this one is expected to end the solve (the
+        // measured rate predicts the tolerance, or a cap ends the loop).  A
+        // wrong guess costs nothing: the next refinement reads it in place.
+        saras::FullRows fr = later;
+        if (in_caller) {
+            fr.x = full.x;
+        } else if (full.xo) {
+            const bool capped = refs + 1 >= m->p.max_refine || cycles >= cap;
+            const bool expect = measured && rho > 0.0 && rho < 1.0 && res * std::pow(rho, ran) < m->p.vc_tol;
+            if (capped || expect) fr.xo = full.xo;
+        }
+        e = refine_step(m, s, true, fr, &res);
         if (e) return e;
+        in_caller = fr.xo != nullptr;
         ++refs;
         if (!std::isfinite(res)) break;
         rho = (ran > 0 && before > 0.0 && res < before) ? std::pow(res / before, 1.0 / ran) : 1.0;
@@ -1295,6 +1320,7 @@ static saras_status solve_mixed(saras_mg *m, hipStream_t s, saras_mg_stats *st,
     st->refinements = refs;
     st->residual = res;
     st->converged = res < m->p.vc_tol;
+    if (x_in_caller) *x_in_caller = in_caller;
     return SARAS_OK;
 }
 
@@ -1367,6 +1393,7 @@ saras_status saras_mg_solve(saras_mg *mg, double *x_full, const double *b_full, 
     auto &L0 = m->h64.lv[0];
     saras_mg_stats st{};
     saras_status e = SARAS_OK;
+    bool in_caller = false;   // the solution's core is in dx already
     if (m->p.precision == SARAS_PREC_FP64) {
         load_core<double>(m, db, L0, L0.b, s);
         load_core<double>(m, dx, L0, L0.x, s);
@@ -1378,17 +1405,18 @@ saras_status saras_mg_solve(saras_mg *mg, double *x_full, const double *b_full, 
         saras::FullRows fr;
         fr.x = dx;
         fr.b = db;
+        fr.xo = dx;
         fr.si = m->g.dim == 3 ? fy * fz : fz;
         fr.sj = m->g.dim == 3 ? fz : 0;
         fr.off = m->g.dim == 3 ? p * fy * fz + p * fz + p : p * fz + p;
-        e = solve_mixed(m, s, &st, fr);
+        e = solve_mixed(m, s, &st, fr, &in_caller);
     } else {
         load_core<double>(m, db, L0, L0.b, s);
         load_core<double>(m, dx, L0, L0.x, s);
         e = solve_mixed(m, s, &st);
     }
     if (e) return e;
-    store_full(m, dx, L0.x, s);
+    store_full(m, dx, in_caller ? nullptr : L0.x, s);
     HIP_TRY(hipEventRecord(m->ev1, s));
     if (ptr_kind == SARAS_PTR_HOST)
         HIP_TRY(hipMemcpyAsync(x_full, m->stage_x, fe * sizeof(double), hipMemcpyDeviceToHost, s));

@@ -807,18 +807,53 @@ __global__ void __launch_bounds__(kSplitThreads) k_prolong_split(int nx, int ny,
 // CX / CB: x_old / b are read from the caller's padded array (FullRows: the
 // core cell (i, j, k) at off + i si + j sj + k) instead of the level's split
 // rows, so the solve needs no separate layout conversion of its inputs; the
-// values, and so the results, are the same.
+// values, and so the results, are the same.  CXO: x_new is written into the
+// caller's array (xo) instead of the split rows -- the refinement expected
+// to be the last, which then needs no layout conversion of its output either
+// (never together with CX: x_old and x_new are then the same array).
 struct FullRows {
     const double *x = nullptr, *b = nullptr;
+    double *xo = nullptr;
     long si = 0, sj = 0, off = 0;
 };
 
-template <int TYPE, int DIM, bool NORM_ONLY = false, bool CX = false, bool CB = false>
+// Four consecutive doubles of a caller's row, whose start need not be
+// 16-byte aligned (an odd pad puts the core at an odd element): two 16-byte
+// accesses when it is, else 8 + 16 + 8 bytes.  The parity is the same along
+// a row, so the lanes of a wave take one branch.
+__device__ __forceinline__ void load4(const double *__restrict__ p, double (&v)[4]) {
+    if ((reinterpret_cast<unsigned long>(p) & 15) == 0) {
+        const double2 a = *reinterpret_cast<const double2 *>(p), c = *reinterpret_cast<const double2 *>(p + 2);
+        v[0] = a.x;
+        v[1] = a.y;
+        v[2] = c.x;
+        v[3] = c.y;
+    } else {
+        const double2 m = *reinterpret_cast<const double2 *>(p + 1);
+        v[0] = p[0];
+        v[1] = m.x;
+        v[2] = m.y;
+        v[3] = p[3];
+    }
+}
+__device__ __forceinline__ void store4(double *__restrict__ p, double v0, double v1, double v2, double v3) {
+    if ((reinterpret_cast<unsigned long>(p) & 15) == 0) {
+        *reinterpret_cast<double2 *>(p) = make_double2(v0, v1);
+        *reinterpret_cast<double2 *>(p + 2) = make_double2(v2, v3);
+    } else {
+        p[0] = v0;
+        *reinterpret_cast<double2 *>(p + 1) = make_double2(v1, v2);
+        p[3] = v3;
+    }
+}
+
+template <int TYPE, int DIM, bool NORM_ONLY = false, bool CX = false, bool CB = false, bool CXO = false>
 __global__ void __launch_bounds__(kNormThreads) k_refine_split(LevelArgs<double> L, const double *__restrict__ xold,
                                                                const float *__restrict__ e, long esx, long esy,
                                                                long ehz, const double *__restrict__ b,
                                                                double *__restrict__ xnew, float *__restrict__ r32,
                                                                double *__restrict__ partial, FullRows F) {
+    static_assert(!(CX && CXO), "x_old and x_new cannot both be the caller's array");
     constexpr int W = 2;
     const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
     const int half = nz >> 1, q = half / W;
@@ -829,11 +864,21 @@ __global__ void __launch_bounds__(kNormThreads) k_refine_split(LevelArgs<double>
     // rows of the x_old + e iterate: even and odd halves at m0
     auto row_vals = [&](int ii, int jj, int m0, double (&ve)[W], double (&vo)[W]) {
         if (CX) {
-            const double *__restrict__ xr = F.x + F.off + (long)ii * F.si + (long)jj * F.sj + 2 * m0;
+            double xr[4];
+            load4(F.x + F.off + (long)ii * F.si + (long)jj * F.sj + 2 * m0, xr);
             ve[0] = xr[0];
             vo[0] = xr[1];
             ve[1] = xr[2];
             vo[1] = xr[3];
+            if (e) {
+                const long eb = (long)ii * esx + (long)jj * esy;
+                const float2 fe = *reinterpret_cast<const float2 *>(e + eb + m0);
+                const float2 fo = *reinterpret_cast<const float2 *>(e + eb + ehz + m0);
+                ve[0] = ve[0] + (double)fe.x;
+                ve[1] = ve[1] + (double)fe.y;
+                vo[0] = vo[0] + (double)fo.x;
+                vo[1] = vo[1] + (double)fo.y;
+            }
             return;
         }
         const long rb = (long)ii * L.sx + (long)jj * L.sy;
@@ -883,9 +928,8 @@ __global__ void __launch_bounds__(kNormThreads) k_refine_split(LevelArgs<double>
         const long eb = (long)i * esx + (long)j * esy;
         // the iterate at cell k of this row
         auto cell = [&](int k) {
-            if (CX) return F.x[F.off + (long)i * F.si + (long)j * F.sj + k];
             const long h = k & 1, m = k >> 1;
-            double x = xold[rb + h * hz + m];
+            double x = CX ? F.x[F.off + (long)i * F.si + (long)j * F.sj + k] : xold[rb + h * hz + m];
             if (e) x = x + (double)e[eb + h * ehz + m];
             return x;
         };
@@ -957,7 +1001,8 @@ __global__ void __launch_bounds__(kNormThreads) k_refine_split(LevelArgs<double>
         const double c2y = DIM == 3 ? L.c2[1][j] : 0.0, c1y = DIM == 3 ? L.c1[1][j] : 0.0;
         double bev[W], bov[W];
         if (CB) {
-            const double *__restrict__ br = F.b + F.off + (long)i * F.si + (long)j * F.sj + 2 * m0;
+            double br[4];
+            load4(F.b + F.off + (long)i * F.si + (long)j * F.sj + 2 * m0, br);
             bev[0] = br[0];
             bov[0] = br[1];
             bev[1] = br[2];
@@ -996,8 +1041,12 @@ __global__ void __launch_bounds__(kNormThreads) k_refine_split(LevelArgs<double>
             ro[p] = residual<double, DIM>(s, c2x, c1x, c2y, c1y, L.c2zs[half + m0 + p], L.c1zs[half + m0 + p], bov[p]);
         }
         if (!NORM_ONLY) {
-            *reinterpret_cast<double2 *>(xnew + rb + m0) = make_double2(ce[0], ce[1]);
-            *reinterpret_cast<double2 *>(xnew + rb + hz + m0) = make_double2(co[0], co[1]);
+            if (CXO) {
+                store4(F.xo + F.off + (long)i * F.si + (long)j * F.sj + 2 * m0, ce[0], co[0], ce[1], co[1]);
+            } else {
+                *reinterpret_cast<double2 *>(xnew + rb + m0) = make_double2(ce[0], ce[1]);
+                *reinterpret_cast<double2 *>(xnew + rb + hz + m0) = make_double2(co[0], co[1]);
+            }
             *reinterpret_cast<float2 *>(r32 + eb + m0) = make_float2((float)re[0], (float)re[1]);
             *reinterpret_cast<float2 *>(r32 + eb + ehz + m0) = make_float2((float)ro[0], (float)ro[1]);
         }

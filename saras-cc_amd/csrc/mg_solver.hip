@@ -1145,13 +1145,13 @@ static saras_status solve_fp64(saras_mg *m, hipStream_t s, saras_mg_stats *st) {
 
 }  // extern "C"
 
-template <int TYPE, int DIM, bool CX, bool CB, bool CXO = false>
+template <int TYPE, int DIM, bool CX, bool CB, bool CXO = false, bool NOXW = false>
 void launch_refine_split(saras_mg *m, const saras::LevelArgs<double> &a, const float *e, long esx, long esy, long ehz,
                          float *r32, const saras::FullRows &fr, hipStream_t s) {
     auto &L0 = m->h64.lv[0];
     const unsigned nb = saras::kNormBlocks, nt = saras::kNormThreads;
-    hipLaunchKernelGGL((saras::k_refine_split<TYPE, DIM, false, CX, CB, CXO>), dim3(nb), dim3(nt), 0, s, a, L0.x, e,
-                       esx, esy, ehz, L0.b, m->x64b, r32, m->partial, fr);
+    hipLaunchKernelGGL((saras::k_refine_split<TYPE, DIM, false, CX, CB, CXO, NOXW>), dim3(nb), dim3(nt), 0, s, a,
+                       L0.x, e, esx, esy, ehz, L0.b, m->x64b, r32, m->partial, fr);
     hipLaunchKernelGGL((saras::k_norm_final<TYPE>), dim3(1), dim3(nt), 0, s, m->partial, saras::kNormBlocks,
                        m->norm_host);
 }
@@ -1159,7 +1159,10 @@ void launch_refine_split(saras_mg *m, const saras::LevelArgs<double> &a, const f
 template <int TYPE, int DIM>
 void launch_refine_split(saras_mg *m, const saras::LevelArgs<double> &a, const float *e, long esx, long esy, long ehz,
                          float *r32, const saras::FullRows &fr, hipStream_t s) {
-    if (fr.x) launch_refine_split<TYPE, DIM, true, true>(m, a, e, esx, esy, ehz, r32, fr, s);
+    // x_old in the caller's array with no correction: the initial residual,
+    // whose iterate stays there (nothing to write but r32)
+    if (fr.x && !e) launch_refine_split<TYPE, DIM, true, true, false, true>(m, a, e, esx, esy, ehz, r32, fr, s);
+    else if (fr.x) launch_refine_split<TYPE, DIM, true, true>(m, a, e, esx, esy, ehz, r32, fr, s);
     else if (fr.xo) launch_refine_split<TYPE, DIM, false, true, true>(m, a, e, esx, esy, ehz, r32, fr, s);
     else if (fr.b) launch_refine_split<TYPE, DIM, false, true>(m, a, e, esx, esy, ehz, r32, fr, s);
     else launch_refine_split<TYPE, DIM, false, false>(m, a, e, esx, esy, ehz, r32, fr, s);
@@ -1219,8 +1222,9 @@ static saras_status refine_step(saras_mg *m, hipStream_t s, bool with_e, const s
     }
     if (m->G.dim == 3) launch_refine<3>(m->p.res_type, m, a, e, F0.sx, F0.sy, F0.hz, F0.b, fr, s);
     else launch_refine<2>(m->p.res_type, m, a, e, F0.sx, F0.sy, F0.hz, F0.b, fr, s);
-    // x_new is in level 0's other buffer, or (fr.xo) in the caller's array
-    if (!fr.xo) std::swap(L0.x, m->x64b);
+    // x_new is in level 0's other buffer, or (fr.xo) in the caller's array, or
+    // (x_old from the caller's array, no correction) still only there
+    if (!fr.xo && !(fr.x && !with_e)) std::swap(L0.x, m->x64b);
     HIP_TRY(hipStreamSynchronize(s));
     const double num = m->norm_host[0], den = m->norm_host[1];
     const int pointCount = L0.n[0] * L0.n[1] * L0.n[2];
@@ -1247,7 +1251,9 @@ static saras_status solve_mixed(saras_mg *m, hipStream_t s, saras_mg_stats *st,
     first.xo = nullptr;
     later.x = nullptr;
     later.xo = nullptr;
-    bool in_caller = false;
+    // the FP64 iterate lives in the caller's array: the initial guess until
+    // the first refinement reads it from there
+    bool in_caller = first.x != nullptr;
     saras_status e = refine_step(m, s, false, first, &res);
     if (e) return e;
     int cycles = 0, refs = 0;

@@ -847,7 +847,10 @@ __device__ __forceinline__ void store4(double *__restrict__ p, double v0, double
     }
 }
 
-template <int TYPE, int DIM, bool NORM_ONLY = false, bool CX = false, bool CB = false, bool CXO = false>
+// NOXW: r32 and the norm only, x_new not written (the initial residual of a
+// solve whose iterate stays in the caller's array until the first refinement).
+template <int TYPE, int DIM, bool NORM_ONLY = false, bool CX = false, bool CB = false, bool CXO = false,
+          bool NOXW = false>
 __global__ void __launch_bounds__(kNormThreads) k_refine_split(LevelArgs<double> L, const double *__restrict__ xold,
                                                                const float *__restrict__ e, long esx, long esy,
                                                                long ehz, const double *__restrict__ b,
@@ -1041,7 +1044,8 @@ __global__ void __launch_bounds__(kNormThreads) k_refine_split(LevelArgs<double>
             ro[p] = residual<double, DIM>(s, c2x, c1x, c2y, c1y, L.c2zs[half + m0 + p], L.c1zs[half + m0 + p], bov[p]);
         }
         if (!NORM_ONLY) {
-            if (CXO) {
+            if (NOXW) {
+            } else if (CXO) {
                 store4(F.xo + F.off + (long)i * F.si + (long)j * F.sj + 2 * m0, ce[0], co[0], ce[1], co[1]);
             } else {
                 *reinterpret_cast<double2 *>(xnew + rb + m0) = make_double2(ce[0], ce[1]);

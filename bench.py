@@ -63,13 +63,13 @@ import saras_amd as S  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 N = 256
-DOMINANT = "k_rb_split_zp<float,3,true,false,true>"
+DOMINANT = "k_rb_split_zp<float,3,true,false,true,false>"
 # HBM bytes per launch of the dominant kernel (one FP32 colour pass of the
 # finest level of the z-periodic 256^3 grid, SOR parameter 1) from rocprofv3
 # --pmc FETCH_SIZE / WRITE_SIZE (KiB per dispatch), calibrated against the
-# copy kernel (FETCH x 2.0, WRITE x 1.0): (2 * 33867.3 + 32768.0) KiB.
+# copy kernel (FETCH x 2.0, WRITE x 1.0): (2 * 33870.9 + 32768.0) KiB.
 # Source: profiles/headline_sweep_hbm_traffic.md
-DOMINANT_TRAFFIC_BYTES = (2 * 33867.3 + 32768.0) * 1024
+DOMINANT_TRAFFIC_BYTES = (2 * 33870.9 + 32768.0) * 1024
 
 
 def bytes_per_vcycle(shapes, s, nu1, nu2):

@@ -382,9 +382,9 @@ struct Ops {
     T w;
 
     int pend_level = -1;  // level whose next sweep also applies the prolongation
-    // level 0's iterate is zero and has not been written: set by the caller
-    // instead of clearing it, for a V-cycle whose first operation is a
-    // split-row sweep of level 0 (see zero_start_ok)
+    // level 0's iterate is zero and has not been written: set by the mixed
+    // solve instead of clearing it, for a V-cycle whose first operation is a
+    // split-row sweep of level 0 (the conditions are its `zero_start`)
     bool zero0 = false;
     int skip_red = -1;    // level whose next sweep's red pass the restriction already applied
     // a host-side guard that refused a launch: the caller returns this status

@@ -884,11 +884,6 @@ void note_coarse_iters(saras_mg *m) {
     if (it > m->max_count) m->coarse_capped = 1;
 }
 
-saras_status fetch_coarse_iters(saras_mg *m, hipStream_t s) {
-    HIP_TRY(hipStreamSynchronize(s));
-    note_coarse_iters(m);
-    return SARAS_OK;
-}
 
 long full_elems(const saras_grid_desc &g) {
     const long p = g.pad;
@@ -1318,10 +1313,9 @@ static saras_status solve_mixed(saras_mg *m, hipStream_t s, saras_mg_stats *st,
         rho = (ran > 0 && before > 0.0 && res < before) ? std::pow(res / before, 1.0 / ran) : 1.0;
         measured = true;
     }
-    if (m->p.solve_flag) {
-        e = fetch_coarse_iters(m, s);
-        if (e) return e;
-    }
+    // the last refinement synchronised the stream after the last cycle's
+    // coarse solve
+    if (m->p.solve_flag && cycles > 0) note_coarse_iters(m);
     st->cycles = cycles;
     st->refinements = refs;
     st->residual = res;

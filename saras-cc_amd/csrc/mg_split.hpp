@@ -221,9 +221,13 @@ __device__ __forceinline__ void rb_split_body(const LevelArgs<T> &L, T *__restri
         o[q] = VT::get(ov, q);
     }
     // own coarse row of the prolongation: cells m0.. of both halves map to K = m0..
+    // The black pass of W1 reads no cell of its own colour (its neighbours are
+    // the red values just written), so it needs the correction only where it
+    // reads its own cells: on wall rows, or for a zero quotient (loaded there).
     const long ebase = (long)(i >> 1) * esx + (long)(j >> 1) * esy;
     T evo[W];
-    if (PRO) {
+    const bool evo_now = PRO && (colour == 0 || !W1 || need_c);
+    if (evo_now) {
         coarse_vec(e, ebase, ehz, m0, evo);
 #pragma unroll
         for (int q = 0; q < W; ++q) {
@@ -340,6 +344,7 @@ __device__ __forceinline__ void rb_split_body(const LevelArgs<T> &L, T *__restri
         // a zero quotient takes the sign of (1 - w) x + q: read the cells now
         V lv{};
         if (!Z) lv = *reinterpret_cast<const V *>(x + own);
+        if (PRO && !evo_now) coarse_vec(e, ebase, ehz, m0, evo);
 #pragma unroll
         for (int q = 0; q < W; ++q) c[q] = PRO ? evo[q] + VT::get(lv, q) : VT::get(lv, q);
     }

@@ -10,11 +10,12 @@ iterative-refinement loop, inputs and outputs resident in HBM.  The plain
 FP64 solve of the same problem is timed beside it.
 
 value: algorithmic HBM bytes of the mixed solve / its device time, in TB/s:
-FP32 cycles x B_V + refinement passes x 32 N + the initial residual 28 N,
+FP32 cycles x B_V + refinement passes x 32 N + the initial residual 20 N,
 with the per-level V-cycle model  B_l = N_l s (3 (nu1 + nu2) + 2 + 1/8 + 2 + 1/8)
 (3 words per smoothing sweep, residual+restriction reads x, b and writes
 b_{l+1}, prolongation reads e_{l+1} and reads/writes x), s = bytes per word;
-a refinement pass reads x64, b64, e32 and writes x64, r32.
+a refinement pass reads x64, b64, e32 and writes x64, r32; the initial
+residual reads x64, b64 and writes r32.
 fp64_equivalent_tbps: the bytes the plain FP64 solve moves for the same
 result (its cycles x (B_V + 16 N for computeError)) / the mixed solve's time.
 
@@ -201,7 +202,7 @@ def solve_bytes(shapes, nu1, nu2, prec, cycles, refinements):
     n0 = float(np.prod(shapes[0]))
     if prec == S.PREC_FP64:
         return cycles * (bytes_per_vcycle(shapes, 8, nu1, nu2) + 16 * n0)
-    return cycles * bytes_per_vcycle(shapes, 4, nu1, nu2) + refinements * 32 * n0 + 28 * n0
+    return cycles * bytes_per_vcycle(shapes, 4, nu1, nu2) + refinements * 32 * n0 + 20 * n0
 
 
 def headline_solve(steps, warmup, nu1, nu2, n=N, tol=1e-10, inner=4):

@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
 #include <mutex>
 #include <string>
 
@@ -30,5 +31,27 @@ std::recursive_mutex &alloc_mutex();
 
 // Text returned by saras_last_error() on the calling thread.
 void set_last_error(const std::string &msg);
+
+// Waits until the pinned host-coherent word *seq holds `want`, the ticket the
+// last reduction kernel launched on s stores after its result (a system-scope
+// release, so the result is visible once the ticket is).  Polling the word
+// returns as soon as the result has landed, without waiting for the
+// stream's completion signal.  The poll stops once the stream reports
+// completion or an error, so a failed launch cannot leave it spinning: the
+// stream is then synchronised, which returns the error.  SARAS_NORM_SYNC=1
+// synchronises the stream instead of polling.
+inline hipError_t wait_ticket(const unsigned *seq, unsigned want, hipStream_t s) {
+    static const bool sync_only = [] {
+        const char *env = std::getenv("SARAS_NORM_SYNC");
+        return env && env[0] == '1';
+    }();
+    for (unsigned spin = 1; !sync_only; ++spin) {
+        if (__atomic_load_n(seq, __ATOMIC_ACQUIRE) == want) return hipSuccess;
+        if ((spin & 63) == 0 && hipStreamQuery(s) != hipErrorNotReady) break;
+    }
+    const hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return e;
+    return __atomic_load_n(seq, __ATOMIC_ACQUIRE) == want ? hipSuccess : hipErrorUnknown;
+}
 
 }  // namespace saras

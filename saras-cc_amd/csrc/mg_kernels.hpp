@@ -382,9 +382,12 @@ __global__ void __launch_bounds__(kNormThreads) k_norm_partial(LevelArgs<double>
     }
 }
 
+// out and seq are pinned host-coherent words: the host polls seq for the
+// ticket, which is stored after the two values with release ordering.
 template <int TYPE>
 __global__ void __launch_bounds__(kNormThreads) k_norm_final(const double *__restrict__ partial, int nblocks,
-                                                             double *__restrict__ out) {
+                                                             double *__restrict__ out, unsigned *seq,
+                                                             unsigned ticket) {
     double ar = 0.0, ab = 0.0;
     for (int q = threadIdx.x; q < nblocks; q += blockDim.x) {
         ar = norm_comb<TYPE>(ar, partial[2 * q]);
@@ -394,6 +397,7 @@ __global__ void __launch_bounds__(kNormThreads) k_norm_final(const double *__res
     if (threadIdx.x == 0) {
         out[0] = ar;
         out[1] = ab;
+        __hip_atomic_store(seq, ticket, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 

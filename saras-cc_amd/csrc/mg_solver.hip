@@ -264,9 +264,12 @@ struct saras_mg {
     double *x64b = nullptr;            // second FP64 iterate buffer (MIXED)
     double *partial = nullptr;         // norm partials
     // The norm's last reduction and the coarse solves write these pinned,
-    // host-coherent words directly: the host reads them after synchronising
-    // the stream, with no copy kernel in between.
+    // host-coherent words directly, with no copy kernel in between: the host
+    // reads the norm once its ticket shows it written, and a coarse count
+    // once a later norm has been read (kernels of a stream run in order).
     double *norm_host = nullptr;       // [numerator, denominator] of computeError
+    unsigned *norm_seq = nullptr;      // ticket stored after norm_host (see wait_ticket)
+    unsigned norm_ticket = 0;          // ticket of the last norm launched
     int *iters_host = nullptr;         // coarse iteration count of the last coarse solve
     double *stage_x = nullptr, *stage_b = nullptr;  // device staging for host pointers
     size_t stage_elems = 0;
@@ -299,6 +302,7 @@ struct saras_mg {
             for (auto &e : c) (void)hipGraphExecDestroy(e.exec);
         (void)hipFree(x64b); (void)hipFree(partial);
         (void)hipHostFree(norm_host); (void)hipHostFree(iters_host);
+        (void)hipHostFree(norm_seq);
         (void)hipFree(stage_x); (void)hipFree(stage_b);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
@@ -829,11 +833,14 @@ void launch_norm(int type, saras_mg *m, const saras::LevelArgs<double> &a, const
             hipLaunchKernelGGL((saras::k_refine_split<2, DIM, true>), dim3(nb), dim3(nt), 0, s, a, x, nullptr, 0L, 0L,
                                0L, b, nullptr, nullptr, m->partial, saras::FullRows{});
         if (type == 0)
-            hipLaunchKernelGGL((saras::k_norm_final<0>), dim3(1), dim3(nt), 0, s, m->partial, saras::kNormBlocks, m->norm_host);
+            hipLaunchKernelGGL((saras::k_norm_final<0>), dim3(1), dim3(nt), 0, s, m->partial, saras::kNormBlocks,
+                               m->norm_host, m->norm_seq, ++m->norm_ticket);
         else if (type == 1)
-            hipLaunchKernelGGL((saras::k_norm_final<1>), dim3(1), dim3(nt), 0, s, m->partial, saras::kNormBlocks, m->norm_host);
+            hipLaunchKernelGGL((saras::k_norm_final<1>), dim3(1), dim3(nt), 0, s, m->partial, saras::kNormBlocks,
+                               m->norm_host, m->norm_seq, ++m->norm_ticket);
         else
-            hipLaunchKernelGGL((saras::k_norm_final<2>), dim3(1), dim3(nt), 0, s, m->partial, saras::kNormBlocks, m->norm_host);
+            hipLaunchKernelGGL((saras::k_norm_final<2>), dim3(1), dim3(nt), 0, s, m->partial, saras::kNormBlocks,
+                               m->norm_host, m->norm_seq, ++m->norm_ticket);
         return;
     }
     switch (type) {
@@ -841,21 +848,26 @@ void launch_norm(int type, saras_mg *m, const saras::LevelArgs<double> &a, const
             hipLaunchKernelGGL((saras::k_norm_partial<0, DIM>), dim3(saras::kNormBlocks), dim3(saras::kNormThreads), 0,
                                s, a, x, b, m->partial);
             hipLaunchKernelGGL((saras::k_norm_final<0>), dim3(1), dim3(saras::kNormThreads), 0, s, m->partial,
-                               saras::kNormBlocks, m->norm_host);
+                               saras::kNormBlocks, m->norm_host, m->norm_seq, ++m->norm_ticket);
             break;
         case 1:
             hipLaunchKernelGGL((saras::k_norm_partial<1, DIM>), dim3(saras::kNormBlocks), dim3(saras::kNormThreads), 0,
                                s, a, x, b, m->partial);
             hipLaunchKernelGGL((saras::k_norm_final<1>), dim3(1), dim3(saras::kNormThreads), 0, s, m->partial,
-                               saras::kNormBlocks, m->norm_host);
+                               saras::kNormBlocks, m->norm_host, m->norm_seq, ++m->norm_ticket);
             break;
         default:
             hipLaunchKernelGGL((saras::k_norm_partial<2, DIM>), dim3(saras::kNormBlocks), dim3(saras::kNormThreads), 0,
                                s, a, x, b, m->partial);
             hipLaunchKernelGGL((saras::k_norm_final<2>), dim3(1), dim3(saras::kNormThreads), 0, s, m->partial,
-                               saras::kNormBlocks, m->norm_host);
+                               saras::kNormBlocks, m->norm_host, m->norm_seq, ++m->norm_ticket);
             break;
     }
+}
+
+// the norm of the last k_norm_final launched on s, once it is on the host
+static hipError_t wait_norm(saras_mg *m, hipStream_t s) {
+    return saras::wait_ticket(m->norm_seq, m->norm_ticket, s);
 }
 
 // computeError (poisson_d3.cc:563-663): ratio formed on the host exactly as
@@ -864,7 +876,7 @@ saras_status compute_error(saras_mg *m, int type, const double *x, hipStream_t s
     auto a = level_args(m, m->h64, 0, true);
     if (m->G.dim == 3) launch_norm<3>(type, m, a, x, m->h64.lv[0].b, s);
     else launch_norm<2>(type, m, a, x, m->h64.lv[0].b, s);
-    HIP_TRY(hipStreamSynchronize(s));
+    HIP_TRY(wait_norm(m, s));
     const double num = m->norm_host[0], den = m->norm_host[1];
     const auto &L = m->h64.lv[0];
     const int pointCount = L.n[0] * L.n[1] * L.n[2];
@@ -876,8 +888,8 @@ saras_status compute_error(saras_mg *m, int type, const double *x, hipStream_t s
     return SARAS_OK;
 }
 
-// the last coarse solve's iteration count, once the stream has been
-// synchronised after it
+// the last coarse solve's iteration count, once a norm launched after it has
+// been read
 void note_coarse_iters(saras_mg *m) {
     const int it = *m->iters_host;
     m->coarse_iters_max = std::max(m->coarse_iters_max, it);
@@ -1084,6 +1096,8 @@ saras_status saras_mg_create(const saras_grid_desc *grid, const saras_mg_params 
     if (st) return st;
     HIP_TRY(hipMalloc((void **)&m->partial, 2 * saras::kNormBlocks * sizeof(double)));
     HIP_TRY(hipHostMalloc((void **)&m->norm_host, 2 * sizeof(double), hipHostMallocCoherent));
+    HIP_TRY(hipHostMalloc((void **)&m->norm_seq, sizeof(unsigned), hipHostMallocCoherent));
+    *m->norm_seq = 0;
     HIP_TRY(hipHostMalloc((void **)&m->iters_host, sizeof(int), hipHostMallocCoherent));
     *m->iters_host = 0;
     HIP_TRY(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
@@ -1123,7 +1137,7 @@ static saras_status solve_fp64(saras_mg *m, hipStream_t s, saras_mg_stats *st) {
         if (ve) return ve;
         saras_status e = compute_error(m, m->p.res_type, L0.x, s, &res);
         if (e) return e;
-        // compute_error synchronised the stream after this cycle's coarse solve
+        // compute_error read a norm launched after this cycle's coarse solve
         if (m->p.solve_flag) note_coarse_iters(m);
         cycles = c + 1;
         if (res < m->p.vc_tol) {
@@ -1148,7 +1162,7 @@ void launch_refine_split(saras_mg *m, const saras::LevelArgs<double> &a, const f
     hipLaunchKernelGGL((saras::k_refine_split<TYPE, DIM, false, CX, CB, CXO, NOXW>), dim3(nb), dim3(nt), 0, s, a,
                        L0.x, e, esx, esy, ehz, L0.b, m->x64b, r32, m->partial, fr);
     hipLaunchKernelGGL((saras::k_norm_final<TYPE>), dim3(1), dim3(nt), 0, s, m->partial, saras::kNormBlocks,
-                       m->norm_host);
+                       m->norm_host, m->norm_seq, ++m->norm_ticket);
 }
 
 template <int TYPE, int DIM>
@@ -1181,19 +1195,19 @@ void launch_refine(int type, saras_mg *m, const saras::LevelArgs<double> &a, con
             hipLaunchKernelGGL((saras::k_refine<0, DIM>), dim3(saras::kNormBlocks), dim3(saras::kNormThreads), 0, s, a,
                                L0.x, e, esx, esy, L0.b, m->x64b, r32, esx, esy, ehz, m->partial);
             hipLaunchKernelGGL((saras::k_norm_final<0>), dim3(1), dim3(saras::kNormThreads), 0, s, m->partial,
-                               saras::kNormBlocks, m->norm_host);
+                               saras::kNormBlocks, m->norm_host, m->norm_seq, ++m->norm_ticket);
             break;
         case 1:
             hipLaunchKernelGGL((saras::k_refine<1, DIM>), dim3(saras::kNormBlocks), dim3(saras::kNormThreads), 0, s, a,
                                L0.x, e, esx, esy, L0.b, m->x64b, r32, esx, esy, ehz, m->partial);
             hipLaunchKernelGGL((saras::k_norm_final<1>), dim3(1), dim3(saras::kNormThreads), 0, s, m->partial,
-                               saras::kNormBlocks, m->norm_host);
+                               saras::kNormBlocks, m->norm_host, m->norm_seq, ++m->norm_ticket);
             break;
         default:
             hipLaunchKernelGGL((saras::k_refine<2, DIM>), dim3(saras::kNormBlocks), dim3(saras::kNormThreads), 0, s, a,
                                L0.x, e, esx, esy, L0.b, m->x64b, r32, esx, esy, ehz, m->partial);
             hipLaunchKernelGGL((saras::k_norm_final<2>), dim3(1), dim3(saras::kNormThreads), 0, s, m->partial,
-                               saras::kNormBlocks, m->norm_host);
+                               saras::kNormBlocks, m->norm_host, m->norm_seq, ++m->norm_ticket);
             break;
     }
 }
@@ -1220,7 +1234,7 @@ static saras_status refine_step(saras_mg *m, hipStream_t s, bool with_e, const s
     // x_new is in level 0's other buffer, or (fr.xo) in the caller's array, or
     // (x_old from the caller's array, no correction) still only there
     if (!fr.xo && !(fr.x && !with_e)) std::swap(L0.x, m->x64b);
-    HIP_TRY(hipStreamSynchronize(s));
+    HIP_TRY(wait_norm(m, s));
     const double num = m->norm_host[0], den = m->norm_host[1];
     const int pointCount = L0.n[0] * L0.n[1] * L0.n[2];
     const int type = m->p.res_type;
@@ -1313,8 +1327,7 @@ static saras_status solve_mixed(saras_mg *m, hipStream_t s, saras_mg_stats *st,
         rho = (ran > 0 && before > 0.0 && res < before) ? std::pow(res / before, 1.0 / ran) : 1.0;
         measured = true;
     }
-    // the last refinement synchronised the stream after the last cycle's
-    // coarse solve
+    // the last refinement's norm came after the last cycle's coarse solve
     if (m->p.solve_flag && cycles > 0) note_coarse_iters(m);
     st->cycles = cycles;
     st->refinements = refs;

@@ -775,7 +775,10 @@ __device__ __forceinline__ void block_max_store(double v, double *part, long blo
 }
 
 // max over n partials -> out[0] (one block of 1024 threads)
-__global__ void __launch_bounds__(1024) k_max_final(const double *__restrict__ part, long n, double *__restrict__ out) {
+// out and seq are pinned host-coherent words: the host polls seq for the
+// ticket, which is stored after the maximum with release ordering.
+__global__ void __launch_bounds__(1024) k_max_final(const double *__restrict__ part, long n, double *__restrict__ out,
+                                                    unsigned *seq, unsigned ticket) {
     __shared__ double red[16];
     double v = 0.0;
     // unrolled so that several loads are in flight (the maximum does not
@@ -795,6 +798,7 @@ __global__ void __launch_bounds__(1024) k_max_final(const double *__restrict__ p
         double m = red[0];
         for (int q = 1; q < (int)(blockDim.x >> 6); ++q) m = (red[q] > m || red[q] != red[q]) ? red[q] : m;
         out[0] = m;
+        __hip_atomic_store(seq, ticket, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 

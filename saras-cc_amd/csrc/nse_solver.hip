@@ -75,6 +75,8 @@ struct saras_nse {
     double *part = nullptr;             // per-block residual maxima of a Jacobi launch
     long nblocks = 0;
     double *res_host = nullptr;         // pinned, host-coherent [1]: written by k_max_final
+    unsigned *res_seq = nullptr;        // its ticket, stored after it (saras::wait_ticket)
+    unsigned res_ticket = 0;
     int max_iter = 0;
     double *diag_part = nullptr, *diag_host = nullptr;
     double *stage_a = nullptr, *stage_b = nullptr;
@@ -110,6 +112,7 @@ struct saras_nse {
         (void)hipFree(metrics);
         (void)hipFree(part);
         (void)hipHostFree(res_host);
+        (void)hipHostFree(res_seq);
         (void)hipFree(diag_part);
         (void)hipHostFree(diag_host);
         (void)hipFree(stage_a);
@@ -261,9 +264,10 @@ saras_status solve_field(saras_nse *s, int f, double beta, int *iters) {
     for (;;) {
         hipLaunchKernelGGL(saras::nse::k_jacobi<true>, s->grid3(), dim3(s->threads), 0, s->s, s->G,
                            CoreField{cur, s->bc[f]}, s->R[f], other, dt, diff, beta, s->part);
-        hipLaunchKernelGGL(saras::nse::k_max_final, dim3(1), dim3(1024), 0, s->s, s->part, s->nblocks, s->res_host);
+        hipLaunchKernelGGL(saras::nse::k_max_final, dim3(1), dim3(1024), 0, s->s, s->part, s->nblocks, s->res_host,
+                           s->res_seq, ++s->res_ticket);
         NSE_TRY(hipGetLastError());
-        NSE_TRY(hipStreamSynchronize(s->s));
+        NSE_TRY(saras::wait_ticket(s->res_seq, s->res_ticket, s->s));
         const double r = s->res_host[0];
         if (r < s->p.cn_tol) break;
         it += 1;
@@ -736,6 +740,8 @@ saras_status saras_nse_create(const saras_grid_desc *grid, const saras_mg_params
     if (hipMalloc((void **)&s->part, sizeof(double) * s->nblocks) != hipSuccess) return fail(SARAS_OOM);
     // written by the maximum's last reduction directly (pinned, host-coherent)
     if (hipHostMalloc((void **)&s->res_host, sizeof(double), hipHostMallocCoherent) != hipSuccess) return fail(SARAS_OOM);
+    if (hipHostMalloc((void **)&s->res_seq, sizeof(unsigned), hipHostMallocCoherent) != hipSuccess) return fail(SARAS_OOM);
+    *s->res_seq = 0;
     if (hipMalloc((void **)&s->diag_part, sizeof(double) * saras::nse::kDiagSums * saras::nse::kDiagBlocks) != hipSuccess)
         return fail(SARAS_OOM);
     if (hipHostMalloc((void **)&s->diag_host, sizeof(double) * saras::nse::kDiagSums * saras::nse::kDiagBlocks) != hipSuccess)
@@ -1055,7 +1061,8 @@ saras_status saras_nse_cfl(saras_nse *s, double courant, double dt_max, double *
     const double ax = s->g.len[0] / s->g.n[0], ay = s->g.len[1] / s->g.n[1], az = s->g.len[2] / s->g.n[2];
     hipLaunchKernelGGL(saras::nse::k_cfl, s->grid3(), dim3(s->threads), 0, s->s, s->G, s->F[0], s->F[1], s->F[2],
                        ax, ay, az, s->part);
-    hipLaunchKernelGGL(saras::nse::k_max_final, dim3(1), dim3(1024), 0, s->s, s->part, s->nblocks, s->res_host);
+    hipLaunchKernelGGL(saras::nse::k_max_final, dim3(1), dim3(1024), 0, s->s, s->part, s->nblocks, s->res_host,
+                       s->res_seq, ++s->res_ticket);
     NSE_TRY(hipGetLastError());
     NSE_TRY(hipStreamSynchronize(s->s));
     double d = courant / s->res_host[0];

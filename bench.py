@@ -48,6 +48,12 @@ dScheme 2, Morinishi, RK3 with Jacobi CN, V(3,6) pressure solves of 20
 cycles): device time per full step, FP64 and with mixed-precision pressure
 solves.  "Solve Coarsest" is off: with it on, the reference algorithm's
 pressure solve diverges on this stretched Neumann grid in the first stage.
+
+les_384_step: the 384^3 decaying Taylor-Green LES (Re 1600, spiral sub-grid
+model, RK3/CN) per step in the deck's precision recipe (FP32 V-cycles in
+the FP64 pressure refinement, the sub-grid structure-function integral in
+FP32, every other stencil FP64) and in FP64 throughout, with the fields'
+difference between the two.
 """
 import argparse
 import json
@@ -324,10 +330,11 @@ def rbc_step(steps, warmup, n=256):
     return out
 
 
-def tgv_les_case(n, noise=0.01, seed=5):
+def tgv_les_case(n, noise=0.01, seed=5, sf32=0):
     """Decaying Taylor-Green vortex in a periodic (2 pi)^3 box, Re = 1600
     (nu = 1/1600, unit velocity and length), seeded uniform noise of 1 % of
-    the peak velocity, spiral sub-grid model, RK3/CN with FP64 stencils.
+    the peak velocity, spiral sub-grid model, RK3/CN with FP64 stencils
+    (sf32: the sub-grid model's structure-function integral in FP32).
     Returns the grid, the parameters and the initial fields."""
     pad = 2
     L = 2 * np.pi
@@ -344,7 +351,7 @@ def tgv_les_case(n, noise=0.01, seed=5):
     zero = np.zeros(sh)
     nu = S.nse_physics(2, re=1600.0)["nu"]
     prm = S.nse_params(dscheme=2, nlscheme=3, has_T=0, nu=nu, kappa=nu, dt=1e-3, cn_tol=1e-6,
-                       les=1, vbc=[([S.FBC_DIRICHLET] * 6, [0.0] * 6)] * 3)
+                       les=1, vbc=[([S.FBC_DIRICHLET] * 6, [0.0] * 6)] * 3, les_sf32=sf32)
     return g, prm, dict(vx=u, vy=v, vz=w, p=zero, t=zero, pp=zero)
 
 
@@ -356,14 +363,16 @@ def tgv_les_mg(prec):
 
 
 def les_step(steps, warmup, n=384):
-    """The Taylor-Green LES with the mixed-precision pressure solve, timed;
-    the same run with FP64 pressure solves beside it.  The model enters once
-    solTime > 5 dt, so at least six warm-up steps run before the timed ones."""
+    """The Taylor-Green LES in the deck's precision recipe, timed: FP32
+    V-cycles inside the FP64 residual and refinement of the pressure solve,
+    the sub-grid structure-function integral in FP32; the same run in FP64
+    throughout beside it.  The model enters once solTime > 5 dt, so at least
+    six warm-up steps run before the timed ones."""
     pad = 2
     L = 2 * np.pi
-    g, prm, f0 = tgv_les_case(n)
     res, fields = {}, {}
-    for name, prec in (("mixed", S.PREC_MIXED), ("fp64", S.PREC_FP64)):
+    for name, prec, sf32 in (("mixed", S.PREC_MIXED, 1), ("fp64", S.PREC_FP64, 0)):
+        g, prm, f0 = tgv_les_case(n, sf32=sf32)
         s = S.Nse(g, tgv_les_mg(prec), prm)
         s.set(**f0)
         for _ in range(max(warmup, 6)):
@@ -373,20 +382,19 @@ def les_step(steps, warmup, n=384):
         S.check(r)
         d = s.diag()
         sg, t = s.sgs()
-        res[name] = dict(ms=ms, st=st, d=d, sg=sg, t=t)
-        if prec == S.PREC_MIXED:
-            res[name]["sgs_ms"] = s.bench_sgs(3)
+        res[name] = dict(ms=ms, st=st, d=d, sg=sg, t=t, sgs_ms=s.bench_sgs(3))
         fields[name] = {k: v[(slice(pad, -pad),) * 3].copy() for k, v in s.get().items() if k in ("vx", "vy", "vz", "p")}
         s.close()
     vol = L ** 3
     m = res["mixed"]
-    return dict(ms_per_step=round(m["ms"] / steps, 3), fp64_pressure_ms_per_step=round(res["fp64"]["ms"] / steps, 3),
+    return dict(ms_per_step=round(m["ms"] / steps, 3), fp64_ms_per_step=round(res["fp64"]["ms"] / steps, 3),
                 steps=steps, cells=[n, n, n], jacobi_iters=m["st"]["jacobi_iters"],
                 pressure_cycles=m["st"]["mg"]["cycles"], pressure_residual=m["st"]["mg"]["residual"], time=m["t"],
-                ke=m["d"]["ke"], ke_fp64_pressure=res["fp64"]["d"]["ke"], divmax=m["d"]["divmax"],
+                ke=m["d"]["ke"], ke_fp64=res["fp64"]["d"]["ke"], divmax=m["d"]["divmax"],
                 sgs_ke=m["sg"]["sgke"] / vol, sgs_dissipation=m["sg"]["disp"] / vol, nu_sgs=m["sg"]["nusg"] / vol,
-                mixed_vs_fp64_pressure=field_diffs(fields["mixed"], fields["fp64"]),
-                steps_compared=max(warmup, 6) + steps, sgs_stress=sgs_rate(m["sgs_ms"], n))
+                sgs_ke_fp64=res["fp64"]["sg"]["sgke"] / vol, mixed_vs_fp64=field_diffs(fields["mixed"], fields["fp64"]),
+                steps_compared=max(warmup, 6) + steps, sgs_stress=sgs_rate(res["fp64"]["sgs_ms"], n),
+                sgs_stress_sf32_ms=round(m["sgs_ms"], 4))
 
 
 def les_history(steps, n=384):

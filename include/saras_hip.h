@@ -231,6 +231,9 @@ typedef struct {
                                 hydro.cc:268-271): added to the x-momentum right-hand side of the core.
                                 A hydrodynamics forcing: must be 0 when has_T is set */
     double plate_radius;     /* "Plate Radius" of SARAS_FBC_HOT_PLATE (ignored otherwise) */
+    int les_sf32;            /* 1: the sub-grid model's structure-function integral (the sfIntegral
+                                sum of spiral.cc:157-206, most of its arithmetic) in single
+                                precision, the rest of the model in double; 0: all double */
 } saras_nse_params;
 
 typedef struct {

@@ -552,10 +552,22 @@ __device__ __forceinline__ double sf_integral(double d) {   // spiral.cc:620-631
     return 12.2946 * (c * c) - 6.0 - 0.573159 / (d * sqrt(d)) * sin(3.14159 * d - 0.785398);
 }
 
+// sf_integral in single precision (the same expression)
+__device__ __forceinline__ float sf_integral_f(float d) {
+    const float d2 = d * d;
+    if (d < 0.873469f) return 7.4022f * d2 - 1.82642f * d2 * d2;
+    const float c = cbrtf(d);
+    return 12.2946f * (c * c) - 6.0f - 0.573159f / (d * sqrtf(d)) * sinf(3.14159f * d - 0.785398f);
+}
+
 // One core cell of spiral::computeSG (spiral.cc:157-206 with sgsStress
 // :472-548, eigenvalueSymm :643-~690, eigenvectorSymm :702-~742) on a fully
 // periodic box: writes the six stresses, returns false where the reference
-// aborts.
+// aborts.  SF32: the structure-function integral Qd (the 13 or 26
+// sfIntegral terms, most of the cell's arithmetic) in single precision; the
+// velocity gradients, eigen-decomposition, F2 and the energy integral stay
+// in double, so the stress carries the single-precision rounding of Qd only.
+template <bool SF32 = false>
 __device__ bool sgs_cell(const Geom &G, const CoreField *V, double nu, int i, int j, int k, double T[6],
                          double &K, double S[6]) {
     double dudx[3][3];
@@ -650,7 +662,18 @@ __device__ bool sgs_cell(const Geom &G, const CoreField *V, double nu, int i, in
     const double dzp = G.xc[2][k + 1] - G.xc[2][k], dzm = G.xc[2][k - 1] - G.xc[2][k];
     auto close = [](double a, double b) { return fabs(a + b) <= 1e-12 * fabs(a); };
     const bool mirror = close(dxp, dxm) && close(dyp, dym) && close(dzp, dzm);
-    auto sf_at = [&](int di, int dj, int dk) {
+    const float fdxm = (float)dxm, fdxp = (float)dxp, fdym = (float)dym, fdyp = (float)dyp;
+    const float fdzm = (float)dzm, fdzp = (float)dzp, fe0 = (float)e[0], fe1 = (float)e[1], fe2 = (float)e[2];
+    const float fdel = (float)del;
+    auto sf_at = [&](int di, int dj, int dk) -> double {
+        if (SF32) {
+            const float dx = di < 0 ? fdxm : (di > 0 ? fdxp : 0.0f);
+            const float dy = dj < 0 ? fdym : (dj > 0 ? fdyp : 0.0f);
+            const float dz = dk < 0 ? fdzm : (dk > 0 ? fdzp : 0.0f);
+            const float dx2 = dx * dx + dy * dy + dz * dz;
+            const float dxe = dx * fe0 + dy * fe1 + dz * fe2;
+            return (double)sf_integral_f(sqrtf(fmaxf(dx2 - dxe * dxe, 0.0f)) / fdel);
+        }
         const double dx = di < 0 ? dxm : (di > 0 ? dxp : 0.0);
         const double dy = dj < 0 ? dym : (dj > 0 ? dyp : 0.0);
         const double dz = dk < 0 ? dzm : (dk > 0 ? dzp : 0.0);
@@ -688,6 +711,7 @@ __device__ bool sgs_cell(const Geom &G, const CoreField *V, double nu, int i, in
 
 // sub-grid stresses of every core cell, per-block partial diagnostics
 // [sum |K| dV, sum dissipation dV, sum nu_t dV], failure flag
+template <bool SF32>
 __global__ void __launch_bounds__(256) k_sgs_stress(Geom G, CoreField V0, CoreField V1, CoreField V2, double nu,
                                                     double *__restrict__ Txx, double *__restrict__ Tyy,
                                                     double *__restrict__ Tzz, double *__restrict__ Txy,
@@ -699,7 +723,7 @@ __global__ void __launch_bounds__(256) k_sgs_stress(Geom G, CoreField V0, CoreFi
     if (k < G.n[2]) {
         const CoreField V[3] = {V0, V1, V2};
         double T[6], K, S[6];
-        if (!sgs_cell(G, V, nu, i, j, k, T, K, S)) {
+        if (!sgs_cell<SF32>(G, V, nu, i, j, k, T, K, S)) {
             atomicOr(fail, 1);
             T[0] = T[1] = T[2] = T[3] = T[4] = T[5] = 0.0;
             K = 0.0;

@@ -161,14 +161,24 @@ namespace {
 
 saras_status plume_flags(saras_nse *s);
 
+// the stresses and per-block diagnostics of every core cell (les_sf32: the
+// structure-function integral in single precision)
+void launch_sgs(saras_nse *s) {
+    if (s->p.les_sf32)
+        hipLaunchKernelGGL(saras::nse::k_sgs_stress<true>, s->grid3(), dim3(s->threads), 0, s->s, s->G, s->field(0),
+                           s->field(1), s->field(2), s->p.nu, s->Tsg[0], s->Tsg[1], s->Tsg[2], s->Tsg[3], s->Tsg[4],
+                           s->Tsg[5], s->sgs_part, reinterpret_cast<int *>(s->sgs_dev + 3));
+    else
+        hipLaunchKernelGGL(saras::nse::k_sgs_stress<false>, s->grid3(), dim3(s->threads), 0, s->s, s->G, s->field(0),
+                           s->field(1), s->field(2), s->p.nu, s->Tsg[0], s->Tsg[1], s->Tsg[2], s->Tsg[3], s->Tsg[4],
+                           s->Tsg[5], s->sgs_part, reinterpret_cast<int *>(s->sgs_dev + 3));
+}
+
 // spiral::computeSG stresses and diagnostics (spiral.cc:129-206); the
 // divergence of the stresses is taken inside the RHS kernels
 saras_status compute_sg(saras_nse *s) {
-    const dim3 gr = s->grid3(), bl(s->threads);
     NSE_TRY(hipMemsetAsync(s->sgs_dev, 0, 4 * sizeof(double), s->s));
-    hipLaunchKernelGGL(saras::nse::k_sgs_stress, gr, bl, 0, s->s, s->G, s->field(0), s->field(1), s->field(2),
-                       s->p.nu, s->Tsg[0], s->Tsg[1], s->Tsg[2], s->Tsg[3], s->Tsg[4], s->Tsg[5], s->sgs_part,
-                       reinterpret_cast<int *>(s->sgs_dev + 3));
+    launch_sgs(s);
     hipLaunchKernelGGL(saras::nse::k_sum3_final, dim3(1), dim3(1024), 0, s->s, s->sgs_part, s->nblocks, s->sgs_dev);
     NSE_TRY(hipGetLastError());
     NSE_TRY(hipMemcpyAsync(s->sgs_host, s->sgs_dev, 4 * sizeof(double), hipMemcpyDeviceToHost, s->s));
@@ -934,10 +944,7 @@ saras_status saras_nse_bench_sgs(saras_nse *s, int count, double *ms) {
     }
     NSE_TRY(hipStreamSynchronize(s->s));
     NSE_TRY(hipEventRecord(s->ev0, s->s));
-    for (int c = 0; c < count; ++c)
-        hipLaunchKernelGGL(saras::nse::k_sgs_stress, s->grid3(), dim3(s->threads), 0, s->s, s->G, s->field(0),
-                           s->field(1), s->field(2), s->p.nu, s->Tsg[0], s->Tsg[1], s->Tsg[2], s->Tsg[3], s->Tsg[4],
-                           s->Tsg[5], s->sgs_part, reinterpret_cast<int *>(s->sgs_dev + 3));
+    for (int c = 0; c < count; ++c) launch_sgs(s);
     NSE_TRY(hipGetLastError());
     NSE_TRY(hipEventRecord(s->ev1, s->s));
     NSE_TRY(hipStreamSynchronize(s->s));

@@ -94,6 +94,7 @@ std::string key_of(const saras_grid_desc &g, const saras_mg_params &m, const sar
     put(k, q.les);
     put(k, q.mean_pgrad);
     put(k, q.plate_radius);
+    put(k, q.les_sf32);
     return k;
 }
 

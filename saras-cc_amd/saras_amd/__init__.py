@@ -58,7 +58,7 @@ class NseParams(C.Structure):
                 ("tbc_type", C.c_int * 6), ("tbc_val", C.c_double * 6),
                 ("pbc_type", C.c_int * 6), ("pbc_val", C.c_double * 6), ("scheme", C.c_int),
                 ("pe_limit", C.c_double), ("central_bias", C.c_double), ("les", C.c_int),
-                ("mean_pgrad", C.c_double), ("plate_radius", C.c_double)]
+                ("mean_pgrad", C.c_double), ("plate_radius", C.c_double), ("les_sf32", C.c_int)]
 
 
 class NseStats(C.Structure):
@@ -402,19 +402,21 @@ def bench_stream_copy(n_doubles, reps):
 
 def nse_params(dscheme=2, nlscheme=3, has_T=1, nu=1e-2, kappa=1e-2, dt=1e-3, cn_tol=1e-6,
                buoy=(0.0, 0.0, -1.0), vbc=None, tbc=None, pbc=None, scheme=0, pe_limit=2.0, central_bias=0.8,
-               les=0, mean_pgrad=0.0, plate_radius=0.25):
+               les=0, mean_pgrad=0.0, plate_radius=0.25, les_sf32=0):
     """Time-step parameters.  vbc: [(types[6], vals[6]) per component];
     tbc / pbc: (types[6], vals[6]).  Defaults: the Rayleigh-Benard wiring of
     hydro.cc:287-485 / scalar.cc:129-201 (no-slip walls, T = 1 at the bottom
     and 0 at the top, Neumann T on side walls, Neumann P).  mean_pgrad: the
     constant x pressure gradient of force type 5 (constantPGrad, force.h:148-153;
-    hydrodynamics runs, has_T = 0)."""
+    hydrodynamics runs, has_T = 0).  les_sf32: the sub-grid model's
+    structure-function integral in single precision."""
     d = NseParams()
     d.dscheme, d.nlscheme, d.has_T = dscheme, nlscheme, int(has_T)
     d.nu, d.kappa, d.dt, d.cn_tol = nu, kappa, dt, cn_tol
     d.scheme = scheme
     d.pe_limit, d.central_bias = pe_limit, central_bias
     d.les = int(les)
+    d.les_sf32 = int(les_sf32)
     d.mean_pgrad = mean_pgrad
     d.plate_radius = plate_radius
     for a in range(3):

@@ -397,20 +397,21 @@ def les_step(steps, warmup, n=384):
                 sgs_stress_sf32_ms=round(m["sgs_ms"], 4))
 
 
-def les_history(steps, n=384):
-    """The Taylor-Green LES over `steps` steps, once with the mixed-precision
-    pressure solve and once with FP64 pressure solves (the FP64 path follows
-    the oracle's history to 1e-8 at 64^3 over 500 steps): per step the run
-    time, KE, max|div| and the sub-grid diagnostics, and the dissipation
-    eps = -dKE/dt by centred differences.  Reports the largest differences of
-    the two histories, a few samples of them, and the final fields'
-    differences."""
+def les_history(steps, n=384, sf32=1):
+    """The Taylor-Green LES over `steps` steps, once in the deck's precision
+    recipe (mixed-precision pressure solves; with sf32, the sub-grid
+    structure-function integral in FP32) and once in FP64 throughout (the
+    FP64 path follows the oracle's history to 1e-8 at 64^3 over 500 steps):
+    per step the run time, KE, max|div| and the sub-grid diagnostics, and the
+    dissipation eps = -dKE/dt by centred differences.  Reports the largest
+    differences of the two histories, a few samples of them, and the final
+    fields' differences."""
     pad = 2
     L = 2 * np.pi
     vol = L ** 3
-    g, prm, f0 = tgv_les_case(n)
     hist, fields, wall = {}, {}, {}
-    for name, prec in (("mixed", S.PREC_MIXED), ("fp64", S.PREC_FP64)):
+    for name, prec, sf in (("mixed", S.PREC_MIXED, sf32), ("fp64", S.PREC_FP64, 0)):
+        g, prm, f0 = tgv_les_case(n, sf32=sf)
         s = S.Nse(g, tgv_les_mg(prec), prm)
         s.set(**f0)
         h = []
@@ -445,9 +446,9 @@ def les_history(steps, n=384):
                     eps_mixed=float(eps_a[i - 1]) if 1 <= i <= steps - 2 else None,
                     sgs_dissipation_fp64=b[i]["sgs_dissipation"], sgs_dissipation_mixed=a[i]["sgs_dissipation"])
                for i in marks]
-    return dict(cells=[n, n, n], steps=steps, wall_s={k: round(v, 1) for k, v in wall.items()},
-                mixed_vs_fp64_pressure_history_max_rel=worst, samples=samples,
-                mixed_vs_fp64_pressure_final_fields=field_diffs(fields["mixed"], fields["fp64"]))
+    return dict(cells=[n, n, n], steps=steps, sgs_sf32=bool(sf32), wall_s={k: round(v, 1) for k, v in wall.items()},
+                mixed_vs_fp64_history_max_rel=worst, samples=samples,
+                mixed_vs_fp64_final_fields=field_diffs(fields["mixed"], fields["fp64"]))
 
 
 # FP64 operations per cell of the sub-grid stress kernel at 384^3, from

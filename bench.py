@@ -76,7 +76,7 @@ DOMINANT = "k_rb_split_zp<float,3,true,false,true,false>"
 # --pmc FETCH_SIZE / WRITE_SIZE (KiB per dispatch), calibrated against the
 # copy kernel (FETCH x 2.0, WRITE x 1.0): (2 * 33870.9 + 32768.0) KiB.
 # Source: profiles/headline_sweep_hbm_traffic.md
-DOMINANT_TRAFFIC_BYTES = (2 * 33870.9 + 32768.0) * 1024
+DOMINANT_TRAFFIC_BYTES = (2 * 33873.2 + 32768.0) * 1024
 
 
 def bytes_per_vcycle(shapes, s, nu1, nu2):

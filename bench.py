@@ -74,7 +74,7 @@ DOMINANT = "k_rb_split_zp<float,3,true,false,true,false>"
 # HBM bytes per launch of the dominant kernel (one FP32 colour pass of the
 # finest level of the z-periodic 256^3 grid, SOR parameter 1) from rocprofv3
 # --pmc FETCH_SIZE / WRITE_SIZE (KiB per dispatch), calibrated against the
-# copy kernel (FETCH x 2.0, WRITE x 1.0): (2 * 33870.9 + 32768.0) KiB.
+# copy kernel (FETCH x 2.0, WRITE x 1.0): (2 * 33873.2 + 32768.0) KiB.
 # Source: profiles/headline_sweep_hbm_traffic.md
 DOMINANT_TRAFFIC_BYTES = (2 * 33873.2 + 32768.0) * 1024
 
